@@ -1,0 +1,167 @@
+/* nppe.h -- C ABI of the MI355X pressure Poisson backend for NEREUS's MAC-grid projection step.
+ *
+ * The reference selects its pressure backend through four calls in src/Methods/PPESolver/PressureSettings.h:
+ *   CreatePressureBackend   (:487-514)   -> nppe_create
+ *   CommitPressureMatrix    (:542-574)   -> nppe_commit_grid  (geometric: cell types + spacing tables; the CSR the
+ *                                           reference assembles is optional, nppe_check_csr compares against it)
+ *   SolvePressureSystem     (:578-624)   -> nppe_solve        (host FP64 vectors in IDP order, as AMGX gets them)
+ *   DestroyPressureBackend  (:516-538)   -> nppe_destroy
+ * The backend enum PressureBackend { CPU = 0, AMGX = 1 } (:262) gains a third value that routes to these.
+ *
+ * Conventions (mirroring the reference exactly):
+ *   - Unknowns are the interior FLUID cells numbered lexicographically in (i, j, k), i = y slowest, k = z fastest
+ *     (PressureSolver3D.cpp:133-139; 2D (i, j), PressureSolver2D.cpp:107-112). Cell (i, j, k) of the box lives at
+ *     index (i*nx + j)*nz + k (MAC.h:254).
+ *   - The operator is the reference's assembled matrix: negative semi-definite, scaled 1/dh^2 on a uniform grid and
+ *     in volume-weighted form (face area / face gap) on a stretched one (PressureSolver3D.cpp:32-63,128-178).
+ *   - Stopping: || b - A x ||_2 < tol * max(||b||_2, reference_norm) on the TRUE FP64 residual
+ *     (PressureSettings.h:410, :576-577). When ||b|| <= tol * reference_norm the guess (or 0) is returned with 0
+ *     iterations (:420-425).
+ *   - The caller owns b, x and guess; the handle owns every device buffer.
+ *   - Errors are negative return codes; nothing aborts across the ABI. One handle is used by one thread at a time;
+ *     distinct handles are independent.
+ */
+#ifndef NPPE_H
+#define NPPE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NPPE_ABI_VERSION 1
+
+/* Return codes. NPPE_NOT_CONVERGED is a warning: the result is filled and x holds the last iterate (the reference
+ * warns once and still returns its iterations, PressureSettings.h:472-479). */
+enum nppe_status
+{
+    NPPE_OK = 0,
+    NPPE_NOT_CONVERGED = 1,
+    NPPE_ERR_ARGUMENT = -1,
+    NPPE_ERR_DEVICE = -2,
+    NPPE_ERR_NO_GRID = -3,
+    NPPE_ERR_SIZE = -4,
+    NPPE_ERR_NONFINITE = -5,
+    NPPE_ERR_BUDGET = -6,
+    NPPE_ERR_MISMATCH = -7,
+    NPPE_ERR_INTERNAL = -8
+};
+
+/* Solve methods. */
+enum nppe_method
+{
+    NPPE_METHOD_MGPCG = 0, /* FP32 geometric-multigrid-preconditioned CG inside FP64 iterative refinement */
+    NPPE_METHOD_CG64 = 1   /* FP64 matrix-free Jacobi-PCG (Eigen's ConjugateGradient<..., DiagonalPreconditioner>) */
+};
+
+/* Cell type codes (Definitions.h:191-196). */
+enum nppe_cell
+{
+    NPPE_FLUID = 0,
+    NPPE_SOLID = 1,
+    NPPE_EMPTY = 2,
+    NPPE_INFLOW = 3,
+    NPPE_PRESSURE = 4,
+    NPPE_OUTFLOW = 5
+};
+
+typedef struct nppe_options
+{
+    int32_t device;          /* HIP device ordinal */
+    int32_t method;          /* enum nppe_method */
+    double tolerance;        /* relative tolerance; eigen_tolerance's meaning (PressureSettings.h:225) */
+    int32_t max_iters;       /* cap on iterations (inner iterations summed over refinement passes for MGPCG) */
+    int32_t compatibility;   /* NEEDS_COMPATIBILITY_CONDITION: a singular system's solution is returned mean-free */
+    int32_t pre_smooth;      /* red-black Gauss-Seidel sweeps before / after the coarse correction (default 2 / 2) */
+    int32_t post_smooth;
+    double inner_reduction;  /* residual reduction each FP32 inner solve aims for (default 1e-4) */
+    int32_t max_refinements; /* FP64 refinement passes (default 8) */
+    int32_t reserved0;
+    uint64_t hbm_budget;     /* bytes of device memory the handle may hold; 0 = no limit */
+} nppe_options;
+
+/* Fills *opt with the defaults: device 0, MGPCG, tolerance 1e-8, 2000 iterations, compatibility on, V(2,2),
+ * inner reduction 1e-4, 8 refinement passes, no budget. */
+void nppe_default_options(nppe_options *opt);
+
+/* The grid the reference hands CommitPressureMatrix through its MAC* (PressureSettings.h:542): box dimensions
+ * including the one-cell ring, cell types, and the per-axis width tables MAC::Width(a, n) for n = 0 .. N-1
+ * (MAC.h:78, ring cells included). FaceGap is recomputed from the widths with MAC::GapTable's expression
+ * (MAC.cpp:165-175). */
+typedef struct nppe_grid
+{
+    int32_t dim;                 /* 2 or 3 */
+    int32_t nx, ny, nz;          /* MAC Nx, Ny, Nz (ring included); nz = 1 in 2D */
+    int32_t non_uniform;         /* MAC::nonUniform */
+    int32_t reserved0;
+    double dh;                   /* MAC::dh */
+    const uint8_t *cell_types;   /* nx*ny*nz bytes, index (i*nx + j)*nz + k */
+    const double *width_x;       /* nx entries */
+    const double *width_y;       /* ny entries */
+    const double *width_z;       /* nz entries (ignored in 2D) */
+} nppe_grid;
+
+typedef struct nppe_result
+{
+    int32_t iterations;          /* MGPCG: FP32 inner iterations over all passes; CG64: CG iterations */
+    int32_t refinements;         /* FP64 refinement passes (MGPCG); 0 for CG64 */
+    int32_t status;              /* NPPE_OK or NPPE_NOT_CONVERGED */
+    int32_t levels;              /* multigrid levels used */
+    double rhs_norm;             /* ||b||_2 */
+    double residual_norm;        /* true ||b - A x||_2 in FP64 */
+    double relative_residual;    /* residual_norm / max(rhs_norm, reference_norm) */
+    double device_ms;            /* device time of the solve */
+    double algorithmic_bytes;    /* sum over the solve's launches of each kernel's compulsory HBM bytes */
+} nppe_result;
+
+typedef struct nppe_solver nppe_solver;
+
+/* CreatePressureBackend (PressureSettings.h:487-514). opt may be NULL for the defaults. */
+int nppe_create(nppe_solver **out, const nppe_options *opt);
+
+/* DestroyPressureBackend (PressureSettings.h:516-538). NULL is accepted. */
+int nppe_destroy(nppe_solver *h);
+
+/* Message of the last error on this handle (empty when none); a NULL handle gives the last creation error. */
+const char *nppe_last_error(const nppe_solver *h);
+
+/* CommitPressureMatrix (PressureSettings.h:542-574) and the assembly it follows (PressureSolver3D.cpp:128-178):
+ * numbers the unknowns, decides singularity (no Dirichlet face), uploads the cell types and tables and builds the
+ * multigrid hierarchy. Returns the number of unknowns (>= 0) or an error. */
+int64_t nppe_commit_grid(nppe_solver *h, const nppe_grid *g);
+
+/* Unknown count and singularity of the committed system (PressureSystem::A.rows(), ::singular,
+ * PressureSettings.h:284-297). */
+int nppe_system_info(const nppe_solver *h, int64_t *n, int32_t *singular);
+
+/* The IDP numbering of the committed system: order[q] = box index of unknown q (n entries). */
+int nppe_unknown_order(const nppe_solver *h, int32_t *order);
+
+/* Self-check for CommitPressureMatrix's triplets: compares the reference's assembled CSR (rowptr n+1, col/val nnz)
+ * entry by entry with the matrix-free operator's coefficients. NPPE_OK when every entry is bit-identical,
+ * NPPE_ERR_MISMATCH otherwise (last_error names the first difference). */
+int nppe_check_csr(nppe_solver *h, int32_t n, int64_t nnz, const int32_t *rowptr, const int32_t *col,
+                   const double *val);
+
+/* SolvePressureSystem (PressureSettings.h:578-624): x (n entries, resized by the caller) = A^-1 b, from guess when
+ * not NULL, to tol * max(||b||, reference_norm). res may be NULL. */
+int nppe_solve(nppe_solver *h, const double *b, double *x, const double *guess, double reference_norm,
+               nppe_result *res);
+
+/* As nppe_solve with b, x and guess in device memory (FP64, IDP order, n entries), on the handle's stream. */
+int nppe_solve_device(nppe_solver *h, const double *d_b, double *d_x, const double *d_guess, double reference_norm,
+                      nppe_result *res);
+
+/* y = A x with the matrix-free FP64 operator (host vectors, IDP order); the SpMV the reference's CG runs
+ * (Eigen SparseDenseProduct over PressureSystem::A). */
+int nppe_apply(nppe_solver *h, const double *x, double *y);
+
+/* Device bytes the handle holds. */
+int nppe_memory_bytes(const nppe_solver *h, uint64_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

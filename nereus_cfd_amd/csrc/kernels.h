@@ -1,0 +1,68 @@
+// Launchers of the CDNA4 kernels (kernels.hip). Every launcher enqueues on the given stream and returns the HIP
+// status of the launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "nppe_internal.h"
+
+namespace nppe
+{
+
+// ---- FP64 fine level ------------------------------------------------------------------------------------------
+// box[order[q]] = src[q] (0 when src is null) for the n unknowns; with sumSlot >= 0 also sums src^2 into
+// st->sums[sumSlot].
+hipError_t launch_scatter64(hipStream_t s, int64_t n, const int32_t *order, const double *src, double *box,
+                            SolveState *st, int sumSlot, double *partials);
+// dst[q] = box[order[q]] - (meanSlot >= 0 ? st->sums[meanSlot] / n : 0).
+hipError_t launch_gather64(hipStream_t s, int64_t n, const int32_t *order, const double *box, double *dst,
+                           const SolveState *st, int meanSlot);
+// st->sums[slot] = sum of box[order[q]].
+hipError_t launch_sum64(hipStream_t s, int64_t n, const int32_t *order, const double *box, SolveState *st, int slot,
+                        double *partials);
+// Outer refinement step: xn = x + e (e may be null) is written to xout (a different buffer), then r = b - A xn in
+// FP64 with the reference's exact coefficients, stored as FP32 in r32 and/or FP64 in r64; st->sums[1] = ||r||^2.
+hipError_t launch_update_residual64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty,
+                                    const double *x, const float *e, double *xout, const double *b, float *r32,
+                                    double *r64, SolveState *st, double *partials);
+// y = A x in FP64 (the reference's SpMV, row sums in ascending column order).
+hipError_t launch_apply64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const double *x,
+                          double *y);
+// FP64 Jacobi-PCG kernels (Eigen's ConjugateGradient loop, ConjugateGradient.h:57-88).
+hipError_t launch_cg64_init(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const double *r,
+                            double *p, SolveState *st, double *partials);
+hipError_t launch_cg64_spmv(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const double *p,
+                            double *q, SolveState *st, double *partials);
+hipError_t launch_cg64_update(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, double *x,
+                              double *r, const double *p, const double *q, SolveState *st, double *partials);
+hipError_t launch_cg64_direction(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty,
+                                 const double *r, double *p, SolveState *st);
+
+// ---- FP32 multigrid ------------------------------------------------------------------------------------------
+hipError_t launch_smooth(hipStream_t s, const Box &B, const Tables32 &T, const uint8_t *ty, float *x, const float *b,
+                         int color, int zero, const SolveState *st);
+hipError_t launch_residual_restrict(hipStream_t s, const Box &Bf, const Tables32 &Tf, const uint8_t *tyf,
+                                    const float *x, const float *b, const Box &Bc, float *bc, const SolveState *st);
+hipError_t launch_prolong(hipStream_t s, const Box &Bf, const uint8_t *tyf, float *x, const Box &Bc,
+                          const uint8_t *tyc, const float *xc, const SolveState *st);
+// Coarsest level: x[order[i]] = sum_j M[i][j] b[order[j]] in FP64 (M: the dense pseudo-inverse, n x n).
+hipError_t launch_coarse_solve(hipStream_t s, int n, const int32_t *order, const double *M, const float *b, float *x,
+                               const SolveState *st);
+
+// ---- FP32 CG around the V-cycle ------------------------------------------------------------------------------
+// <r, z> -> beta (0 on the first direction), rz.
+hipError_t launch_cg_rz(hipStream_t s, const Box &B, const uint8_t *ty, const float *r, const float *z, SolveState *st,
+                        double *partials);
+// p = z + beta p (p = z on the first direction).
+hipError_t launch_cg_direction(hipStream_t s, const Box &B, const uint8_t *ty, const float *z, float *p,
+                               SolveState *st);
+// q = A p, <p, q> -> alpha.
+hipError_t launch_cg_spmv(hipStream_t s, const Box &B, const Tables32 &T, const uint8_t *ty, const float *p, float *q,
+                          SolveState *st, double *partials);
+// e += alpha p (e = alpha p on the first iteration), r -= alpha q, <r, r> -> convergence flag.
+hipError_t launch_cg_update(hipStream_t s, const Box &B, const uint8_t *ty, float *e, float *r, const float *p,
+                            const float *q, SolveState *st, double *partials);
+// Inner-solve set-up from the outer residual norm st->sums[1]: thr2 = red^2 * sums[1], counters cleared.
+hipError_t launch_inner_setup(hipStream_t s, SolveState *st, double reduction, int maxIters);
+
+}  // namespace nppe

@@ -1,0 +1,68 @@
+// Internal types shared by the host driver (solver.cpp, hierarchy.cpp) and the kernels (kernels.hip).
+//
+// Box layout on the device: every level keeps the full box of its grid, ring included, in (slow, mid, fast) order
+// with the fast axis padded to a pitch. In 3D slow/mid/fast are the reference's i (y), j (x), k (z)
+// (MAC.h:254: index (i*Nx + j)*Nz + k); in 2D there is one slow plane and mid/fast are i (y), j (x).
+#pragma once
+
+#include <cstdint>
+
+namespace nppe
+{
+
+constexpr int kBlockFast = 64;  // tile extent along the fast axis: one wavefront
+constexpr int kBlockMid = 4;    // tile extent along the mid axis: four wavefronts per block
+constexpr int kBlockThreads = kBlockFast * kBlockMid;
+constexpr int kMaxDenseCoarse = 1024;  // largest coarsest level solved by the dense pseudo-inverse
+
+// Geometry of one level's box.
+struct Box
+{
+    int ns, nm, nf;     // extents incl. ring along slow, mid, fast (2D: ns = 1)
+    int pitch;          // padded fast extent
+    int threeD;         // 1 in 3D
+    int tilesF, tilesM; // tiles of the interior along fast and mid
+    int planes;         // slow planes of the interior (1 in 2D)
+    int64_t sM, sS;     // element strides of mid and slow
+    int64_t cells;      // allocated elements (ns * nm * pitch)
+    __host__ __device__ int64_t at(int s, int m, int f) const { return s * sS + m * sM + f; }
+    __host__ __device__ int blocks() const { return tilesF * tilesM * planes; }
+};
+
+// FP32 coefficient tables of one level, in A-units: for the face between cells n and n+1 along an axis,
+// g[n] = sigma / FaceGap(n); for a Dirichlet ring neighbour the coefficient is area * h[n] with
+// h[n] = 2 sigma / Width(n) (c * FaceGap / (Width/2), PressureSettings.h:307-310). Widths w[] form the areas.
+// In 2D the slow tables hold a single 1.0 so that the area products reduce to one width.
+struct Tables32
+{
+    const float *wS, *wM, *wF;
+    const float *gS, *gM, *gF;
+    const float *hS, *hM, *hF;
+};
+
+// FP64 tables of the fine level, exactly as the reference forms them: widths and FaceGap(n) = (w[n]+w[n+1])/2.
+struct Tables64
+{
+    const double *wS, *wM, *wF;
+    const double *gS, *gM, *gF;
+    double scale;   // MatrixScale3D: 1/(dh*dh) uniform, 1 stretched
+    int unitCoef;   // 3D uniform: every face coefficient is exactly 1.0 (PressureSolver3D.cpp:49-53)
+};
+
+// Device-resident scalars of one solve. Written by the last block of each reduction.
+struct SolveState
+{
+    double rz;        // <r, z> of the current search direction
+    double alpha;
+    double beta;
+    double rr;        // <r, r> after the last update
+    double thr2;      // inner stop: rr < thr2
+    double sums[4];   // FP64 reductions: [0] ||b||^2, [1] ||r||^2, [2] sum x, [3] spare
+    int done;         // inner loop finished (converged or capped)
+    int iters;        // inner iterations taken
+    int maxIters;     // inner cap
+    int first;        // next direction update is the first of the solve (beta = 0)
+    unsigned counters[8];
+};
+
+}  // namespace nppe

@@ -1,0 +1,1096 @@
+// The C ABI (include/nppe.h) and the host side of the solve: device buffers, the multigrid hierarchy, the FP64
+// refinement loop around the FP32 MGPCG inner solve, and the FP64 Jacobi-PCG baseline.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/nppe.h"
+#include "../../include/nppe_profile.h"
+#include "hierarchy.h"
+#include "kernels.h"
+
+using namespace nppe;
+
+namespace
+{
+
+std::mutex g_create_mutex;
+std::string g_create_error;
+
+constexpr int kInnerChunk = 4;      // inner iterations enqueued between host checks
+constexpr int kMaxLevels = 24;
+constexpr int64_t kCoarsestTarget = 512;
+
+// Algorithmic (compulsory HBM) bytes per launch, per cell of the box the kernel sweeps: s = value bytes, f = 1.
+struct Traffic
+{
+    double bytes = 0.0;
+    void add(double b) { bytes += b; }
+};
+
+struct DevLevel
+{
+    HostLevel host;
+    Box box{};
+    uint8_t *types = nullptr;
+    float *x = nullptr, *b = nullptr;
+    float *tab = nullptr;
+    Tables32 t32{};
+    double cellsBox = 0.0;   // ns * nm * nf (the box, unpitched)
+};
+
+}  // namespace
+
+struct nppe_solver
+{
+    nppe_options opt{};
+    std::string err;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool committed = false;
+    int dim = 3, Nx = 0, Ny = 0, Nz = 0;
+    int64_t n = 0;
+    bool singular = false;
+    bool nonUniform = false;
+    double dh = 0.0;
+
+    std::vector<int32_t> orderHost;   // unpitched box index of each unknown
+    int32_t *dOrder = nullptr;        // pitched box index of each unknown
+    std::vector<DevLevel> levels;
+    double *dTab64 = nullptr;
+    Tables64 t64{};
+    double *dX64[2] = {nullptr, nullptr};
+    double *dB64 = nullptr;
+    float *dE = nullptr, *dR = nullptr, *dP = nullptr, *dQ = nullptr, *dZ = nullptr;
+    double *dR64 = nullptr, *dP64 = nullptr, *dQ64 = nullptr;
+    double *dCompB = nullptr, *dCompX = nullptr, *dCompG = nullptr;
+    double *dCoarseM = nullptr;
+    int32_t *dCoarseOrder = nullptr;
+    int nCoarse = 0;
+    bool coarseDense = false;
+    SolveState *dState = nullptr;
+    SolveState *hState = nullptr;   // pinned
+    double *dPartials = nullptr;
+    int64_t partialsCap = 0;
+    uint64_t bytesHeld = 0;
+    std::vector<void *> allocations;
+
+    // profiling (nppe_profile.h)
+    bool profiling = false;
+    std::vector<hipEvent_t> evPool;
+    struct Rec
+    {
+        int kind, iter;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    std::vector<Rec> recs;
+    size_t evNext = 0;
+    int curIter = -1;
+    double profMs[NPPE_KERNEL_KINDS] = {};
+    double profBytes[NPPE_KERNEL_KINDS] = {};
+    int64_t profCount[NPPE_KERNEL_KINDS] = {};
+};
+
+namespace
+{
+
+int fail(nppe_solver *h, int code, const std::string &msg)
+{
+    if (h) h->err = msg;
+    return code;
+}
+
+int hip_fail(nppe_solver *h, hipError_t e, const char *what)
+{
+    return fail(h, NPPE_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(h, call, what)                                   \
+    do                                                           \
+    {                                                            \
+        hipError_t e_ = (call);                                  \
+        if (e_ != hipSuccess) return hip_fail((h), e_, (what));  \
+    } while (0)
+
+template <class T>
+hipError_t dalloc(nppe_solver *h, T **p, size_t count)
+{
+    const size_t bytes = std::max<size_t>(1, count) * sizeof(T);
+    if (h->opt.hbm_budget && h->bytesHeld + bytes > h->opt.hbm_budget) return hipErrorOutOfMemory;
+    hipError_t e = hipMalloc((void **)p, bytes);
+    if (e != hipSuccess) return e;
+    h->allocations.push_back((void *)*p);
+    h->bytesHeld += bytes;
+    return hipMemsetAsync(*p, 0, bytes, h->stream);
+}
+
+void free_grid(nppe_solver *h)
+{
+    for (void *p : h->allocations) (void)hipFree(p);
+    h->allocations.clear();
+    h->bytesHeld = 0;
+    h->levels.clear();
+    h->dOrder = nullptr;
+    h->dTab64 = nullptr;
+    h->dX64[0] = h->dX64[1] = nullptr;
+    h->dB64 = nullptr;
+    h->dE = h->dR = h->dP = h->dQ = h->dZ = nullptr;
+    h->dR64 = h->dP64 = h->dQ64 = nullptr;
+    h->dCompB = h->dCompX = h->dCompG = nullptr;
+    h->dCoarseM = nullptr;
+    h->dCoarseOrder = nullptr;
+    h->dPartials = nullptr;
+    h->committed = false;
+}
+
+// ---- profiling -----------------------------------------------------------------------------------------------
+struct Scope
+{
+    nppe_solver *h;
+    int kind;
+    double bytes;
+    hipEvent_t b = nullptr;
+    Scope(nppe_solver *h_, int kind_, double bytes_) : h(h_), kind(kind_), bytes(bytes_)
+    {
+        if (!h->profiling) return;
+        if (h->evNext + 2 > h->evPool.size())
+            for (int q = 0; q < 256; q++)
+            {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) return;
+                h->evPool.push_back(e);
+            }
+        hipEvent_t a = h->evPool[h->evNext++];
+        b = h->evPool[h->evNext++];
+        (void)hipEventRecord(a, h->stream);
+        h->recs.push_back({kind, h->curIter, a, b, bytes});
+    }
+    ~Scope()
+    {
+        if (h->profiling && b) (void)hipEventRecord(b, h->stream);
+    }
+};
+
+// After a synchronisation: fold the recorded launches of inner iterations < itersDone (or of no iteration) into
+// the per-kind totals; launches of iterations that were skipped on the device are dropped.
+void profile_collect(nppe_solver *h, int itersDone)
+{
+    if (!h->profiling) return;
+    for (const auto &r : h->recs)
+    {
+        if (r.iter >= itersDone) continue;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) continue;
+        h->profMs[r.kind] += ms;
+        h->profBytes[r.kind] += r.bytes;
+        h->profCount[r.kind] += 1;
+    }
+    h->recs.clear();
+    h->evNext = 0;
+}
+
+// ---- multigrid ------------------------------------------------------------------------------------------------
+double lvl_cells(const DevLevel &L) { return L.cellsBox; }
+
+hipError_t vcycle(nppe_solver *h, Traffic &tr, float *x0, const float *b0)
+{
+    const int nl = (int)h->levels.size();
+    const int nu1 = std::max(1, h->opt.pre_smooth), nu2 = std::max(0, h->opt.post_smooth);
+    const double s = 4.0, f = 1.0;
+    hipError_t e = hipSuccess;
+    for (int l = 0; l < nl - 1; l++)
+    {
+        DevLevel &L = h->levels[l];
+        float *x = l == 0 ? x0 : L.x;
+        const float *b = l == 0 ? b0 : L.b;
+        const double N = lvl_cells(L);
+        for (int sw = 0; sw < nu1; sw++)
+        {
+            {
+                Scope sc(h, NPPE_K_SMOOTH, sw == 0 ? (s + s / 2 + f) * N / 2 : (3 * s + f) * N / 2);
+                if ((e = launch_smooth(h->stream, L.box, L.t32, L.types, x, b, 0, sw == 0, h->dState)) != hipSuccess)
+                    return e;
+            }
+            {
+                Scope sc(h, NPPE_K_SMOOTH, sw == 0 ? (s + s / 2 + s / 2 + f) * N / 2 : (3 * s + f) * N / 2);
+                if ((e = launch_smooth(h->stream, L.box, L.t32, L.types, x, b, 1, 0, h->dState)) != hipSuccess)
+                    return e;
+            }
+            tr.add(sw == 0 ? (2 * s + f) * N : (3 * s + f) * N);
+        }
+        DevLevel &C = h->levels[l + 1];
+        const double rb = (2 * s + f) * N + s * lvl_cells(C);
+        {
+            Scope sc(h, NPPE_K_RESTRICT, rb);
+            if ((e = launch_residual_restrict(h->stream, L.box, L.t32, L.types, x, b, C.box, C.b, h->dState)) !=
+                hipSuccess)
+                return e;
+        }
+        tr.add(rb);
+    }
+    {
+        DevLevel &L = h->levels[nl - 1];
+        float *x = nl == 1 ? x0 : L.x;
+        const float *b = nl == 1 ? b0 : L.b;
+        if (h->coarseDense)
+        {
+            const double cb = 8.0 * h->nCoarse * (double)h->nCoarse + 8.0 * h->nCoarse * 2;
+            Scope sc(h, NPPE_K_COARSE, cb);
+            if ((e = launch_coarse_solve(h->stream, h->nCoarse, h->dCoarseOrder, h->dCoarseM, b, x, h->dState)) !=
+                hipSuccess)
+                return e;
+            tr.add(cb);
+        }
+        else
+        {
+            // Coarsest level too large for the dense solve: symmetric Gauss-Seidel sweeps.
+            const double N = lvl_cells(L);
+            for (int sw = 0; sw < 16; sw++)
+            {
+                Scope sc(h, NPPE_K_SMOOTH, (3 * s + f) * N);
+                if ((e = launch_smooth(h->stream, L.box, L.t32, L.types, x, b, sw & 1, sw == 0, h->dState)) !=
+                    hipSuccess)
+                    return e;
+                tr.add((3 * s + f) * N / 2);
+            }
+        }
+    }
+    for (int l = nl - 2; l >= 0; l--)
+    {
+        DevLevel &L = h->levels[l];
+        DevLevel &C = h->levels[l + 1];
+        float *x = l == 0 ? x0 : L.x;
+        const float *b = l == 0 ? b0 : L.b;
+        const double N = lvl_cells(L);
+        const double pb = s * lvl_cells(C) + 2 * s * N + f * N;
+        {
+            Scope sc(h, NPPE_K_PROLONG, pb);
+            if ((e = launch_prolong(h->stream, L.box, L.types, x, C.box, C.types, C.x, h->dState)) != hipSuccess)
+                return e;
+        }
+        tr.add(pb);
+        for (int sw = 0; sw < nu2; sw++)
+        {
+            {
+                Scope sc(h, NPPE_K_SMOOTH, (3 * s + f) * N / 2);
+                if ((e = launch_smooth(h->stream, L.box, L.t32, L.types, x, b, 1, 0, h->dState)) != hipSuccess)
+                    return e;
+            }
+            {
+                Scope sc(h, NPPE_K_SMOOTH, (3 * s + f) * N / 2);
+                if ((e = launch_smooth(h->stream, L.box, L.t32, L.types, x, b, 0, 0, h->dState)) != hipSuccess)
+                    return e;
+            }
+            tr.add((3 * s + f) * N);
+        }
+    }
+    return hipSuccess;
+}
+
+// One inner MGPCG iteration: z = V(r); beta; p; q = A p, alpha; e, r update.
+hipError_t inner_iteration(nppe_solver *h, Traffic &tr)
+{
+    const DevLevel &L0 = h->levels[0];
+    const double N = L0.cellsBox, s = 4.0, f = 1.0;
+    hipError_t e;
+    if ((e = vcycle(h, tr, h->dZ, h->dR)) != hipSuccess) return e;
+    {
+        Scope sc(h, NPPE_K_CG_DOT, 2 * s * N + f * N);
+        if ((e = launch_cg_rz(h->stream, L0.box, L0.types, h->dR, h->dZ, h->dState, h->dPartials)) != hipSuccess)
+            return e;
+    }
+    {
+        Scope sc(h, NPPE_K_CG_DIRECTION, 3 * s * N + f * N);
+        if ((e = launch_cg_direction(h->stream, L0.box, L0.types, h->dZ, h->dP, h->dState)) != hipSuccess) return e;
+    }
+    {
+        Scope sc(h, NPPE_K_CG_SPMV, (2 * s + f) * N);
+        if ((e = launch_cg_spmv(h->stream, L0.box, L0.t32, L0.types, h->dP, h->dQ, h->dState, h->dPartials)) !=
+            hipSuccess)
+            return e;
+    }
+    {
+        Scope sc(h, NPPE_K_CG_UPDATE, 6 * s * N + f * N);
+        if ((e = launch_cg_update(h->stream, L0.box, L0.types, h->dE, h->dR, h->dP, h->dQ, h->dState,
+                                  h->dPartials)) != hipSuccess)
+            return e;
+    }
+    tr.add((2 * s + f) * N + (3 * s + f) * N + (2 * s + f) * N + (6 * s + f) * N);
+    return hipSuccess;
+}
+
+int sync_state(nppe_solver *h)
+{
+    HIP_TRY(h, hipMemcpyAsync(h->hState, h->dState, sizeof(SolveState), hipMemcpyDeviceToHost, h->stream),
+            "state copy");
+    HIP_TRY(h, hipStreamSynchronize(h->stream), "stream synchronize");
+    return NPPE_OK;
+}
+
+// ---- set-up ---------------------------------------------------------------------------------------------------
+
+int build(nppe_solver *h, const nppe_grid *g)
+{
+    free_grid(h);
+    const int dim = g->dim;
+    const int nz = dim == 2 ? 1 : g->nz;
+    h->dim = dim;
+    h->Nx = g->nx;
+    h->Ny = g->ny;
+    h->Nz = nz;
+    h->nonUniform = g->non_uniform != 0;
+    h->dh = g->dh;
+    HostLevel fine = fine_level(dim, g->nx, g->ny, nz, g->cell_types, g->width_x, g->width_y,
+                                dim == 3 ? g->width_z : nullptr);
+
+    // Unknown numbering and singularity (PressureSolver3D.cpp:133-172): interior FLUID cells in index order;
+    // singular when no FLUID cell has a Dirichlet face (EMPTY/PRESSURE, or a FLUID ring cell).
+    h->orderHost.clear();
+    bool dirichlet = false;
+    {
+        const int ns = fine.ns, nm = fine.nm, nf = fine.nf;
+        auto at = [&](int s, int m, int f) { return ((int64_t)s * nm + m) * nf + f; };
+        const int s0 = fine.threeD ? 1 : 0, s1 = fine.threeD ? ns - 2 : 0;
+        for (int s = s0; s <= s1; s++)
+            for (int m = 1; m <= nm - 2; m++)
+                for (int f = 1; f <= nf - 2; f++)
+                {
+                    const int64_t c = at(s, m, f);
+                    if (fine.types[c] != 0) continue;
+                    h->orderHost.push_back((int32_t)c);
+                    const int ds[6] = {1, -1, 0, 0, 0, 0}, dm[6] = {0, 0, 1, -1, 0, 0}, df[6] = {0, 0, 0, 0, 1, -1};
+                    for (int q = fine.threeD ? 0 : 2; q < 6; q++)
+                    {
+                        const int S = s + ds[q], M = m + dm[q], F = f + df[q];
+                        const uint8_t t = fine.types[at(S, M, F)];
+                        const bool ring = (fine.threeD && (S == 0 || S == ns - 1)) || M == 0 || M == nm - 1 ||
+                                          F == 0 || F == nf - 1;
+                        if (t == 2 || t == 4 || (t == 0 && ring)) dirichlet = true;
+                    }
+                }
+    }
+    h->n = (int64_t)h->orderHost.size();
+    h->singular = !dirichlet;
+
+    // Scale factors: the reference's MatrixScale (1/dh^2 uniform, 1 stretched); sigma turns the volume-form
+    // coefficients area/gap into A-units (3D uniform: c = 1 * scale while area/gap = dh).
+    const bool unitCoef = dim == 3 && !h->nonUniform;
+    const double scale = h->nonUniform ? 1.0 : 1.0 / (g->dh * g->dh);
+    const double sigma = unitCoef ? scale / g->dh : scale;
+
+    // Levels: coarsen until the coarsest has at most kCoarsestTarget unknowns or stops shrinking.
+    std::vector<HostLevel> hl;
+    hl.push_back(std::move(fine));
+    while ((int)hl.size() < kMaxLevels && hl.back().unknowns > kCoarsestTarget)
+    {
+        HostLevel c = coarsen(hl.back());
+        const HostLevel &p = hl.back();
+        if (c.ns == p.ns && c.nm == p.nm && c.nf == p.nf) break;
+        if (c.unknowns == 0) break;
+        hl.push_back(std::move(c));
+    }
+    h->levels.resize(hl.size());
+    for (size_t l = 0; l < hl.size(); l++)
+    {
+        DevLevel &L = h->levels[l];
+        L.host = std::move(hl[l]);
+        const HostLevel &H = L.host;
+        L.box = make_box(H.ns, H.nm, H.nf, H.threeD);
+        L.cellsBox = (double)H.ns * H.nm * H.nf;
+        HIP_TRY(h, dalloc(h, &L.types, (size_t)L.box.cells), "allocate types");
+        {
+            std::vector<uint8_t> pitched((size_t)L.box.cells, 1);
+            for (int s = 0; s < H.ns; s++)
+                for (int m = 0; m < H.nm; m++)
+                    std::memcpy(&pitched[(size_t)L.box.at(s, m, 0)], &H.types[((size_t)s * H.nm + m) * H.nf], H.nf);
+            HIP_TRY(h, hipMemcpyAsync(L.types, pitched.data(), pitched.size(), hipMemcpyHostToDevice, h->stream),
+                    "upload types");
+            HIP_TRY(h, hipStreamSynchronize(h->stream), "upload types");
+        }
+        int off[9];
+        std::vector<float> tab = tables32(H, sigma, off);
+        HIP_TRY(h, dalloc(h, &L.tab, tab.size()), "allocate tables");
+        HIP_TRY(h, hipMemcpy(L.tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice), "upload tables");
+        L.t32 = Tables32{L.tab + off[0], L.tab + off[1], L.tab + off[2], L.tab + off[3], L.tab + off[4],
+                         L.tab + off[5], L.tab + off[6], L.tab + off[7], L.tab + off[8]};
+        if (l > 0)
+        {
+            HIP_TRY(h, dalloc(h, &L.x, (size_t)L.box.cells), "allocate level x");
+            HIP_TRY(h, dalloc(h, &L.b, (size_t)L.box.cells), "allocate level b");
+        }
+    }
+    // Coarsest: dense pseudo-inverse when small enough.
+    {
+        const DevLevel &C = h->levels.back();
+        h->coarseDense = false;
+        h->nCoarse = 0;
+        if (C.host.unknowns <= kMaxDenseCoarse)
+        {
+            std::vector<int64_t> ord;
+            std::vector<double> M;
+            std::string e2;
+            if (!coarse_pseudo_inverse(C.host, sigma, ord, M, e2)) return fail(h, NPPE_ERR_INTERNAL, e2);
+            h->nCoarse = (int)ord.size();
+            std::vector<int32_t> pitched(ord.size());
+            for (size_t q = 0; q < ord.size(); q++)
+            {
+                const int64_t c = ord[q];
+                const int f = (int)(c % C.host.nf), m = (int)((c / C.host.nf) % C.host.nm),
+                          s = (int)(c / ((int64_t)C.host.nf * C.host.nm));
+                pitched[q] = (int32_t)C.box.at(s, m, f);
+            }
+            HIP_TRY(h, dalloc(h, &h->dCoarseM, M.size()), "allocate coarse matrix");
+            HIP_TRY(h, dalloc(h, &h->dCoarseOrder, pitched.size()), "allocate coarse order");
+            HIP_TRY(h, hipMemcpy(h->dCoarseM, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice),
+                    "upload coarse matrix");
+            HIP_TRY(h, hipMemcpy(h->dCoarseOrder, pitched.data(), pitched.size() * sizeof(int32_t),
+                                 hipMemcpyHostToDevice),
+                    "upload coarse order");
+            h->coarseDense = true;
+        }
+    }
+    // Fine FP64 data.
+    const DevLevel &F = h->levels[0];
+    {
+        int off[6];
+        std::vector<double> tab = tables64(F.host, off);
+        HIP_TRY(h, dalloc(h, &h->dTab64, tab.size()), "allocate FP64 tables");
+        HIP_TRY(h, hipMemcpy(h->dTab64, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice),
+                "upload FP64 tables");
+        h->t64 = Tables64{h->dTab64 + off[0], h->dTab64 + off[1], h->dTab64 + off[2], h->dTab64 + off[3],
+                          h->dTab64 + off[4], h->dTab64 + off[5], scale, unitCoef ? 1 : 0};
+    }
+    {
+        std::vector<int32_t> pitched(h->orderHost.size());
+        for (size_t q = 0; q < pitched.size(); q++)
+        {
+            const int64_t c = h->orderHost[q];
+            const int f = (int)(c % F.host.nf), m = (int)((c / F.host.nf) % F.host.nm),
+                      s = (int)(c / ((int64_t)F.host.nf * F.host.nm));
+            pitched[q] = (int32_t)F.box.at(s, m, f);
+        }
+        HIP_TRY(h, dalloc(h, &h->dOrder, pitched.size()), "allocate order");
+        HIP_TRY(h, hipMemcpy(h->dOrder, pitched.data(), pitched.size() * sizeof(int32_t), hipMemcpyHostToDevice),
+                "upload order");
+    }
+    const size_t cells = (size_t)F.box.cells;
+    HIP_TRY(h, dalloc(h, &h->dX64[0], cells), "allocate x64");
+    HIP_TRY(h, dalloc(h, &h->dX64[1], cells), "allocate x64");
+    HIP_TRY(h, dalloc(h, &h->dB64, cells), "allocate b64");
+    if (h->opt.method == NPPE_METHOD_MGPCG)
+    {
+        HIP_TRY(h, dalloc(h, &h->dE, cells), "allocate e");
+        HIP_TRY(h, dalloc(h, &h->dR, cells), "allocate r");
+        HIP_TRY(h, dalloc(h, &h->dP, cells), "allocate p");
+        HIP_TRY(h, dalloc(h, &h->dQ, cells), "allocate q");
+        HIP_TRY(h, dalloc(h, &h->dZ, cells), "allocate z");
+    }
+    else
+    {
+        HIP_TRY(h, dalloc(h, &h->dR64, cells), "allocate r64");
+        HIP_TRY(h, dalloc(h, &h->dP64, cells), "allocate p64");
+        HIP_TRY(h, dalloc(h, &h->dQ64, cells), "allocate q64");
+    }
+    const size_t nn = (size_t)std::max<int64_t>(1, h->n);
+    HIP_TRY(h, dalloc(h, &h->dCompB, nn), "allocate staging b");
+    HIP_TRY(h, dalloc(h, &h->dCompX, nn), "allocate staging x");
+    HIP_TRY(h, dalloc(h, &h->dCompG, nn), "allocate staging guess");
+    int64_t maxBlocks = (h->n + kBlockThreads - 1) / kBlockThreads;
+    for (const DevLevel &L : h->levels) maxBlocks = std::max<int64_t>(maxBlocks, L.box.blocks());
+    h->partialsCap = maxBlocks * 4;
+    HIP_TRY(h, dalloc(h, &h->dPartials, (size_t)h->partialsCap), "allocate partials");
+    HIP_TRY(h, dalloc(h, &h->dState, 1), "allocate state");
+    HIP_TRY(h, hipStreamSynchronize(h->stream), "commit");
+    h->committed = true;
+    return NPPE_OK;
+}
+
+int validate_grid(nppe_solver *h, const nppe_grid *g)
+{
+    if (!g || (g->dim != 2 && g->dim != 3)) return fail(h, NPPE_ERR_ARGUMENT, "grid: dim must be 2 or 3");
+    const int nz = g->dim == 2 ? 1 : g->nz;
+    if (g->nx < 3 || g->ny < 3 || (g->dim == 3 && nz < 3))
+        return fail(h, NPPE_ERR_ARGUMENT, "grid: every axis needs at least one interior cell plus the ring");
+    if ((int64_t)g->nx * g->ny * nz > (int64_t)1 << 31)
+        return fail(h, NPPE_ERR_SIZE, "grid: more than 2^31 cells");
+    if (!g->cell_types || !g->width_x || !g->width_y || (g->dim == 3 && !g->width_z))
+        return fail(h, NPPE_ERR_ARGUMENT, "grid: null cell types or width table");
+    if (!(g->dh > 0.0) || !std::isfinite(g->dh)) return fail(h, NPPE_ERR_ARGUMENT, "grid: dh must be positive");
+    const double *ws[3] = {g->width_x, g->width_y, g->width_z};
+    const int ns[3] = {g->nx, g->ny, nz};
+    for (int a = 0; a < g->dim; a++)
+        for (int q = 0; q < ns[a]; q++)
+            if (!(ws[a][q] > 0.0) || !std::isfinite(ws[a][q]))
+                return fail(h, NPPE_ERR_ARGUMENT, "grid: widths must be positive and finite");
+    const size_t cells = (size_t)g->nx * g->ny * nz;
+    for (size_t c = 0; c < cells; c++)
+        if (g->cell_types[c] > 5) return fail(h, NPPE_ERR_ARGUMENT, "grid: cell type out of range (0..5)");
+    return NPPE_OK;
+}
+
+// ---- solves ---------------------------------------------------------------------------------------------------
+
+int solve_mgpcg(nppe_solver *h, const double *dB, double *dX, const double *dGuess, double refNorm, nppe_result *res)
+{
+    const DevLevel &L0 = h->levels[0];
+    const double N = L0.cellsBox;
+    const double n = (double)h->n;
+    Traffic tr;
+    hipEvent_t t0, t1;
+    HIP_TRY(h, hipEventCreate(&t0), "event");
+    HIP_TRY(h, hipEventCreate(&t1), "event");
+    HIP_TRY(h, hipEventRecord(t0, h->stream), "event");
+    h->curIter = -1;
+    int cur = 0;
+    {
+        Scope sc(h, NPPE_K_SCATTER, 20.0 * n);
+        HIP_TRY(h, launch_scatter64(h->stream, h->n, h->dOrder, dB, h->dB64, h->dState, 0, h->dPartials), "scatter b");
+    }
+    {
+        Scope sc(h, NPPE_K_SCATTER, 20.0 * n);
+        HIP_TRY(h, launch_scatter64(h->stream, h->n, h->dOrder, dGuess, h->dX64[cur], h->dState, -1, h->dPartials),
+                "scatter guess");
+    }
+    tr.add(40.0 * n);
+    {
+        Scope sc(h, NPPE_K_RESIDUAL64, 21.0 * N);
+        HIP_TRY(h, launch_update_residual64(h->stream, L0.box, h->t64, L0.types, h->dX64[cur], nullptr, nullptr,
+                                            h->dB64, h->dR, nullptr, h->dState, h->dPartials),
+                "residual");
+    }
+    tr.add(21.0 * N);
+    int rc = sync_state(h);
+    if (rc) return rc;
+    profile_collect(h, 0);
+    const double bnorm = std::sqrt(h->hState->sums[0]);
+    double rnorm = std::sqrt(h->hState->sums[1]);
+    if (!std::isfinite(bnorm) || !std::isfinite(rnorm))
+        return fail(h, NPPE_ERR_NONFINITE, "non-finite right-hand side or guess");
+    const double tol = h->opt.tolerance;
+    const double denom = std::max(bnorm, refNorm);
+    const double target = tol * denom;
+    int passes = 0, inner = 0, status = NPPE_OK;
+    bool skip = false;
+    if (bnorm == 0.0)
+    {
+        // Eigen sets x = 0 for a zero right-hand side (ConjugateGradient.h:46-52)
+        skip = true;
+        HIP_TRY(h, launch_scatter64(h->stream, h->n, h->dOrder, nullptr, h->dX64[cur], h->dState, -1, h->dPartials),
+                "zero x");
+        rnorm = 0.0;
+    }
+    if (refNorm > 0.0 && bnorm <= tol * refNorm)
+        skip = true;   // the guess (or 0) is the answer (PressureSettings.h:420-425)
+    while (!skip && !(rnorm < target))
+    {
+        if (passes >= h->opt.max_refinements || inner >= h->opt.max_iters)
+        {
+            status = NPPE_NOT_CONVERGED;
+            break;
+        }
+        // Aim each inner solve at the larger of the configured reduction and what the target still needs
+        // (with a margin for the FP32 round-off that the FP64 residual sees).
+        const double need = 0.25 * target / rnorm;
+        const double red = std::min(0.5, std::max(h->opt.inner_reduction, need));
+        HIP_TRY(h, launch_inner_setup(h->stream, h->dState, red, h->opt.max_iters - inner), "inner setup");
+        int launched = 0;
+        double perIter = 0.0;
+        for (;;)
+        {
+            for (int q = 0; q < kInnerChunk; q++)
+            {
+                h->curIter = launched++;
+                Traffic it;
+                HIP_TRY(h, inner_iteration(h, it), "inner iteration");
+                perIter = it.bytes;
+            }
+            rc = sync_state(h);
+            if (rc) return rc;
+            if (h->hState->done) break;
+        }
+        const int done = h->hState->iters;
+        profile_collect(h, done);
+        tr.add(perIter * done);   // launched iterations past convergence exit at once and move nothing
+        inner += done;
+        h->curIter = -1;
+        {
+            Scope sc(h, NPPE_K_RESIDUAL64, 33.0 * N);
+            HIP_TRY(h, launch_update_residual64(h->stream, L0.box, h->t64, L0.types, h->dX64[cur], h->dE,
+                                                h->dX64[1 - cur], h->dB64, h->dR, nullptr, h->dState, h->dPartials),
+                    "refinement residual");
+        }
+        tr.add(33.0 * N);
+        cur = 1 - cur;
+        rc = sync_state(h);
+        if (rc) return rc;
+        profile_collect(h, 0);
+        rnorm = std::sqrt(h->hState->sums[1]);
+        if (!std::isfinite(rnorm)) return fail(h, NPPE_ERR_NONFINITE, "non-finite residual during refinement");
+        passes++;
+    }
+    // Output: the pure-Neumann null mode (mean) projected out in FP64 when the system is singular and the
+    // compatibility condition is on.
+    const bool project = h->singular && h->opt.compatibility && h->n > 0;
+    if (project)
+    {
+        Scope sc(h, NPPE_K_MEAN, 12.0 * n);
+        HIP_TRY(h, launch_sum64(h->stream, h->n, h->dOrder, h->dX64[cur], h->dState, 2, h->dPartials), "mean");
+        tr.add(12.0 * n);
+    }
+    {
+        Scope sc(h, NPPE_K_GATHER, 20.0 * n);
+        HIP_TRY(h, launch_gather64(h->stream, h->n, h->dOrder, h->dX64[cur], dX, h->dState, project ? 2 : -1),
+                "gather x");
+    }
+    tr.add(20.0 * n);
+    HIP_TRY(h, hipEventRecord(t1, h->stream), "event");
+    rc = sync_state(h);
+    if (rc) return rc;
+    profile_collect(h, 0);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (res)
+    {
+        // the launched-but-skipped inner iterations did no work: count bytes of the iterations that ran
+        res->iterations = inner;
+        res->refinements = passes;
+        res->status = status;
+        res->levels = (int)h->levels.size();
+        res->rhs_norm = bnorm;
+        res->residual_norm = rnorm;
+        res->relative_residual = denom > 0.0 ? rnorm / denom : rnorm;
+        res->device_ms = ms;
+        res->algorithmic_bytes = tr.bytes;
+    }
+    return status;
+}
+
+int solve_cg64(nppe_solver *h, const double *dB, double *dX, const double *dGuess, double refNorm, nppe_result *res)
+{
+    const DevLevel &L0 = h->levels[0];
+    const double N = L0.cellsBox;
+    const double n = (double)h->n;
+    Traffic tr;
+    hipEvent_t t0, t1;
+    HIP_TRY(h, hipEventCreate(&t0), "event");
+    HIP_TRY(h, hipEventCreate(&t1), "event");
+    HIP_TRY(h, hipEventRecord(t0, h->stream), "event");
+    h->curIter = -1;
+    HIP_TRY(h, launch_scatter64(h->stream, h->n, h->dOrder, dB, h->dB64, h->dState, 0, h->dPartials), "scatter b");
+    HIP_TRY(h, launch_scatter64(h->stream, h->n, h->dOrder, dGuess, h->dX64[0], h->dState, -1, h->dPartials),
+            "scatter guess");
+    tr.add(40.0 * n);
+    HIP_TRY(h, launch_update_residual64(h->stream, L0.box, h->t64, L0.types, h->dX64[0], nullptr, nullptr, h->dB64,
+                                        nullptr, h->dR64, h->dState, h->dPartials),
+            "residual");
+    tr.add(24.0 * N + N);
+    int rc = sync_state(h);
+    if (rc) return rc;
+    profile_collect(h, 0);
+    const double bn2 = h->hState->sums[0];
+    double rr = h->hState->sums[1];
+    if (!std::isfinite(bn2) || !std::isfinite(rr)) return fail(h, NPPE_ERR_NONFINITE, "non-finite input");
+    const double bnorm = std::sqrt(bn2);
+    // SolveEigenPressure's floor (PressureSettings.h:415-427), then Eigen's threshold (ConjugateGradient.h:53-54).
+    double tol = h->opt.tolerance;
+    bool skip = false;
+    int status = NPPE_OK, iters = 0;
+    if (refNorm > 0.0)
+    {
+        if (bnorm <= tol * refNorm) skip = true;
+        else tol = std::min(0.5, tol * refNorm / bnorm);
+    }
+    if (bn2 == 0.0)
+    {
+        skip = true;
+        HIP_TRY(h, launch_scatter64(h->stream, h->n, h->dOrder, nullptr, h->dX64[0], h->dState, -1, h->dPartials),
+                "zero x");
+    }
+    const double thr = std::max(tol * tol * bn2, 2.2250738585072014e-308);
+    if (!skip && !(rr < thr))
+    {
+        SolveState init{};
+        HIP_TRY(h, hipMemcpyAsync(h->hState, h->dState, sizeof(SolveState), hipMemcpyDeviceToHost, h->stream),
+                "state");
+        HIP_TRY(h, hipStreamSynchronize(h->stream), "state");
+        init = *h->hState;
+        init.thr2 = thr;
+        init.maxIters = h->opt.max_iters;
+        init.done = h->opt.max_iters <= 0 ? 2 : 0;
+        init.iters = 0;
+        *h->hState = init;
+        HIP_TRY(h, hipMemcpyAsync(h->dState, h->hState, sizeof(SolveState), hipMemcpyHostToDevice, h->stream),
+                "state");
+        {
+            Scope sc(h, NPPE_K_CG64_INIT, 16.0 * N + N);
+            HIP_TRY(h, launch_cg64_init(h->stream, L0.box, h->t64, L0.types, h->dR64, h->dP64, h->dState,
+                                        h->dPartials),
+                    "cg64 init");
+        }
+        tr.add(17.0 * N);
+        int launched = 0;
+        for (;;)
+        {
+            for (int q = 0; q < 8; q++)
+            {
+                h->curIter = launched++;
+                {
+                    Scope sc(h, NPPE_K_CG64_SPMV, 17.0 * N);
+                    HIP_TRY(h, launch_cg64_spmv(h->stream, L0.box, h->t64, L0.types, h->dP64, h->dQ64, h->dState,
+                                                h->dPartials),
+                            "cg64 spmv");
+                }
+                {
+                    Scope sc(h, NPPE_K_CG64_UPDATE, 49.0 * N);
+                    HIP_TRY(h, launch_cg64_update(h->stream, L0.box, h->t64, L0.types, h->dX64[0], h->dR64, h->dP64,
+                                                  h->dQ64, h->dState, h->dPartials),
+                            "cg64 update");
+                }
+                {
+                    Scope sc(h, NPPE_K_CG64_DIRECTION, 25.0 * N);
+                    HIP_TRY(h, launch_cg64_direction(h->stream, L0.box, h->t64, L0.types, h->dR64, h->dP64,
+                                                     h->dState),
+                            "cg64 direction");
+                }
+            }
+            rc = sync_state(h);
+            if (rc) return rc;
+            if (h->hState->done) break;
+        }
+        iters = h->hState->iters;
+        // passes that ran: iters + 1 when the last one converged (Eigen does not count it), spmv+update only
+        const int ran = iters + (h->hState->done == 1 ? 1 : 0);
+        profile_collect(h, ran);
+        tr.add((17.0 + 49.0 + 25.0) * N * ran);
+        if (h->hState->done == 2) status = NPPE_NOT_CONVERGED;
+    }
+    else if (skip && refNorm > 0.0 && bnorm <= h->opt.tolerance * refNorm)
+    {
+        // x already holds the guess (or 0)
+    }
+    // True residual of the result.
+    h->curIter = -1;
+    HIP_TRY(h, launch_update_residual64(h->stream, L0.box, h->t64, L0.types, h->dX64[0], nullptr, nullptr, h->dB64,
+                                        nullptr, h->dR64, h->dState, h->dPartials),
+            "final residual");
+    tr.add(25.0 * N);
+    const bool project = h->singular && h->opt.compatibility && h->n > 0;
+    if (project)
+    {
+        HIP_TRY(h, launch_sum64(h->stream, h->n, h->dOrder, h->dX64[0], h->dState, 2, h->dPartials), "mean");
+        tr.add(12.0 * n);
+    }
+    HIP_TRY(h, launch_gather64(h->stream, h->n, h->dOrder, h->dX64[0], dX, h->dState, project ? 2 : -1), "gather");
+    tr.add(20.0 * n);
+    HIP_TRY(h, hipEventRecord(t1, h->stream), "event");
+    rc = sync_state(h);
+    if (rc) return rc;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    const double rnorm = std::sqrt(h->hState->sums[1]);
+    if (res)
+    {
+        res->iterations = iters;
+        res->refinements = 0;
+        res->status = status;
+        res->levels = 1;
+        res->rhs_norm = bnorm;
+        res->residual_norm = rnorm;
+        const double denom = std::max(bnorm, refNorm);
+        res->relative_residual = denom > 0.0 ? rnorm / denom : rnorm;
+        res->device_ms = ms;
+        res->algorithmic_bytes = tr.bytes;
+    }
+    return status;
+}
+
+int solve_device(nppe_solver *h, const double *dB, double *dX, const double *dGuess, double refNorm, nppe_result *res)
+{
+    if (!h) return NPPE_ERR_ARGUMENT;
+    if (!h->committed) return fail(h, NPPE_ERR_NO_GRID, "no grid committed");
+    if (h->n > 0 && (!dB || !dX)) return fail(h, NPPE_ERR_ARGUMENT, "null b or x");
+    HIP_TRY(h, hipSetDevice(h->device), "set device");
+    if (h->n == 0)
+    {
+        if (res) *res = nppe_result{};
+        return NPPE_OK;
+    }
+    h->err.clear();
+    return h->opt.method == NPPE_METHOD_CG64 ? solve_cg64(h, dB, dX, dGuess, refNorm, res)
+                                             : solve_mgpcg(h, dB, dX, dGuess, refNorm, res);
+}
+
+}  // namespace
+
+// ---- C ABI --------------------------------------------------------------------------------------------------------
+
+extern "C" {
+
+void nppe_default_options(nppe_options *o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->device = 0;
+    o->method = NPPE_METHOD_MGPCG;
+    o->tolerance = 1e-8;
+    o->max_iters = 2000;
+    o->compatibility = 1;
+    o->pre_smooth = 2;
+    o->post_smooth = 2;
+    o->inner_reduction = 1e-4;
+    o->max_refinements = 8;
+    o->hbm_budget = 0;
+}
+
+int nppe_create(nppe_solver **out, const nppe_options *opt)
+{
+    std::lock_guard<std::mutex> lock(g_create_mutex);
+    g_create_error.clear();
+    if (!out)
+    {
+        g_create_error = "null output handle";
+        return NPPE_ERR_ARGUMENT;
+    }
+    *out = nullptr;
+    nppe_options o;
+    nppe_default_options(&o);
+    if (opt) o = *opt;
+    if (!(o.tolerance > 0.0) || o.max_iters < 0 || (o.method != NPPE_METHOD_MGPCG && o.method != NPPE_METHOD_CG64) ||
+        o.pre_smooth < 1 || o.post_smooth < 0 || !(o.inner_reduction > 0.0 && o.inner_reduction < 1.0) ||
+        o.max_refinements < 1)
+    {
+        g_create_error = "invalid options";
+        return NPPE_ERR_ARGUMENT;
+    }
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0)
+    {
+        g_create_error = std::string("no HIP device: ") + hipGetErrorString(e);
+        return NPPE_ERR_DEVICE;
+    }
+    if (o.device < 0 || o.device >= count)
+    {
+        g_create_error = "device ordinal out of range";
+        return NPPE_ERR_ARGUMENT;
+    }
+    nppe_solver *h = new (std::nothrow) nppe_solver();
+    if (!h)
+    {
+        g_create_error = "out of host memory";
+        return NPPE_ERR_INTERNAL;
+    }
+    h->opt = o;
+    h->device = o.device;
+    if ((e = hipSetDevice(h->device)) != hipSuccess || (e = hipStreamCreate(&h->stream)) != hipSuccess ||
+        (e = hipHostMalloc((void **)&h->hState, sizeof(SolveState), hipHostMallocDefault)) != hipSuccess)
+    {
+        g_create_error = std::string("device set-up failed: ") + hipGetErrorString(e);
+        if (h->stream) (void)hipStreamDestroy(h->stream);
+        delete h;
+        return NPPE_ERR_DEVICE;
+    }
+    std::memset(h->hState, 0, sizeof(SolveState));
+    *out = h;
+    return NPPE_OK;
+}
+
+int nppe_destroy(nppe_solver *h)
+{
+    if (!h) return NPPE_OK;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    free_grid(h);
+    for (hipEvent_t e : h->evPool) (void)hipEventDestroy(e);
+    if (h->hState) (void)hipHostFree(h->hState);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return NPPE_OK;
+}
+
+const char *nppe_last_error(const nppe_solver *h)
+{
+    if (!h) return g_create_error.c_str();
+    return h->err.c_str();
+}
+
+int64_t nppe_commit_grid(nppe_solver *h, const nppe_grid *g)
+{
+    if (!h) return NPPE_ERR_ARGUMENT;
+    h->err.clear();
+    int rc = validate_grid(h, g);
+    if (rc) return rc;
+    HIP_TRY(h, hipSetDevice(h->device), "set device");
+    rc = build(h, g);
+    if (rc)
+    {
+        free_grid(h);
+        return rc;
+    }
+    return h->n;
+}
+
+int nppe_system_info(const nppe_solver *h, int64_t *n, int32_t *singular)
+{
+    if (!h) return NPPE_ERR_ARGUMENT;
+    if (!h->committed) return NPPE_ERR_NO_GRID;
+    if (n) *n = h->n;
+    if (singular) *singular = h->singular ? 1 : 0;
+    return NPPE_OK;
+}
+
+int nppe_unknown_order(const nppe_solver *h, int32_t *order)
+{
+    if (!h || !order) return NPPE_ERR_ARGUMENT;
+    if (!h->committed) return NPPE_ERR_NO_GRID;
+    std::memcpy(order, h->orderHost.data(), h->orderHost.size() * sizeof(int32_t));
+    return NPPE_OK;
+}
+
+int nppe_solve_device(nppe_solver *h, const double *d_b, double *d_x, const double *d_guess, double reference_norm,
+                      nppe_result *res)
+{
+    return solve_device(h, d_b, d_x, d_guess, reference_norm, res);
+}
+
+int nppe_solve(nppe_solver *h, const double *b, double *x, const double *guess, double reference_norm,
+               nppe_result *res)
+{
+    if (!h) return NPPE_ERR_ARGUMENT;
+    if (!h->committed) return fail(h, NPPE_ERR_NO_GRID, "no grid committed");
+    if (h->n > 0 && (!b || !x)) return fail(h, NPPE_ERR_ARGUMENT, "null b or x");
+    HIP_TRY(h, hipSetDevice(h->device), "set device");
+    const size_t bytes = (size_t)h->n * sizeof(double);
+    if (h->n > 0)
+    {
+        HIP_TRY(h, hipMemcpyAsync(h->dCompB, b, bytes, hipMemcpyHostToDevice, h->stream), "upload b");
+        if (guess) HIP_TRY(h, hipMemcpyAsync(h->dCompG, guess, bytes, hipMemcpyHostToDevice, h->stream), "upload guess");
+    }
+    int rc = solve_device(h, h->dCompB, h->dCompX, guess ? h->dCompG : nullptr, reference_norm, res);
+    if (rc < 0) return rc;
+    if (h->n > 0)
+    {
+        HIP_TRY(h, hipMemcpyAsync(x, h->dCompX, bytes, hipMemcpyDeviceToHost, h->stream), "download x");
+        HIP_TRY(h, hipStreamSynchronize(h->stream), "download x");
+    }
+    return rc;
+}
+
+int nppe_apply(nppe_solver *h, const double *x, double *y)
+{
+    if (!h || (!x && h->n) || (!y && h->n)) return NPPE_ERR_ARGUMENT;
+    if (!h->committed) return fail(h, NPPE_ERR_NO_GRID, "no grid committed");
+    if (h->n == 0) return NPPE_OK;
+    HIP_TRY(h, hipSetDevice(h->device), "set device");
+    const DevLevel &L0 = h->levels[0];
+    const size_t bytes = (size_t)h->n * sizeof(double);
+    HIP_TRY(h, hipMemcpyAsync(h->dCompB, x, bytes, hipMemcpyHostToDevice, h->stream), "upload x");
+    HIP_TRY(h, launch_scatter64(h->stream, h->n, h->dOrder, h->dCompB, h->dX64[0], h->dState, -1, h->dPartials),
+            "scatter");
+    HIP_TRY(h, launch_apply64(h->stream, L0.box, h->t64, L0.types, h->dX64[0], h->dB64), "apply");
+    HIP_TRY(h, launch_gather64(h->stream, h->n, h->dOrder, h->dB64, h->dCompX, h->dState, -1), "gather");
+    HIP_TRY(h, hipMemcpyAsync(y, h->dCompX, bytes, hipMemcpyDeviceToHost, h->stream), "download y");
+    HIP_TRY(h, hipStreamSynchronize(h->stream), "apply");
+    return NPPE_OK;
+}
+
+int nppe_check_csr(nppe_solver *h, int32_t n, int64_t nnz, const int32_t *rowptr, const int32_t *col,
+                   const double *val)
+{
+    if (!h || !rowptr || (nnz > 0 && (!col || !val))) return NPPE_ERR_ARGUMENT;
+    if (!h->committed) return fail(h, NPPE_ERR_NO_GRID, "no grid committed");
+    if ((int64_t)n != h->n) return fail(h, NPPE_ERR_MISMATCH, "row count differs from the committed unknowns");
+    if (rowptr[0] != 0 || (int64_t)rowptr[n] != nnz) return fail(h, NPPE_ERR_ARGUMENT, "malformed rowptr");
+    // Column j of the matrix-free operator is A e_j: probe with unit vectors grouped so that no two probes share
+    // a row (cells 3 apart along every axis in each of the 27 residue classes).
+    std::vector<int> cls(n);
+    const HostLevel &F = h->levels[0].host;
+    for (int32_t q = 0; q < n; q++)
+    {
+        const int64_t c = h->orderHost[q];
+        const int f = (int)(c % F.nf), m = (int)((c / F.nf) % F.nm), s = (int)(c / ((int64_t)F.nf * F.nm));
+        cls[q] = (s % 3) * 9 + (m % 3) * 3 + (f % 3);
+    }
+    std::vector<int32_t> inv(F.ns * (size_t)F.nm * F.nf, -1);
+    for (int32_t q = 0; q < n; q++) inv[h->orderHost[q]] = q;
+    std::vector<double> x(n), y(n);
+    for (int k = 0; k < 27; k++)
+    {
+        bool any = false;
+        for (int32_t q = 0; q < n; q++)
+        {
+            x[q] = cls[q] == k ? 1.0 : 0.0;
+            any = any || cls[q] == k;
+        }
+        if (!any) continue;
+        int rc = nppe_apply(h, x.data(), y.data());
+        if (rc) return rc;
+        // every row r: the entries in columns of class k
+        for (int32_t r = 0; r < n; r++)
+        {
+            double expect = 0.0;
+            int hits = 0;
+            for (int32_t e = rowptr[r]; e < rowptr[r + 1]; e++)
+                if (col[e] >= 0 && col[e] < n && cls[col[e]] == k)
+                {
+                    expect = val[e];
+                    hits++;
+                }
+            if (hits > 1) return fail(h, NPPE_ERR_MISMATCH, "CSR row has two columns in one probe class");
+            if (y[r] != expect)
+            {
+                char buf[160];
+                std::snprintf(buf, sizeof(buf), "row %d: operator %.17g, CSR %.17g (probe class %d)", r, y[r], expect,
+                              k);
+                return fail(h, NPPE_ERR_MISMATCH, buf);
+            }
+        }
+    }
+    return NPPE_OK;
+}
+
+int nppe_memory_bytes(const nppe_solver *h, uint64_t *bytes)
+{
+    if (!h || !bytes) return NPPE_ERR_ARGUMENT;
+    *bytes = h->bytesHeld;
+    return NPPE_OK;
+}
+
+int nppe_profile_enable(nppe_solver *h, int on)
+{
+    if (!h) return NPPE_ERR_ARGUMENT;
+    h->profiling = on != 0;
+    h->recs.clear();
+    h->evNext = 0;
+    for (int k = 0; k < NPPE_KERNEL_KINDS; k++)
+    {
+        h->profMs[k] = 0.0;
+        h->profBytes[k] = 0.0;
+        h->profCount[k] = 0;
+    }
+    return NPPE_OK;
+}
+
+int nppe_profile_read(const nppe_solver *h, int kind, int64_t *launches, double *total_ms, double *bytes)
+{
+    if (!h || kind < 0 || kind >= NPPE_KERNEL_KINDS) return NPPE_ERR_ARGUMENT;
+    if (launches) *launches = h->profCount[kind];
+    if (total_ms) *total_ms = h->profMs[kind];
+    if (bytes) *bytes = h->profBytes[kind];
+    return NPPE_OK;
+}
+
+}  // extern "C"

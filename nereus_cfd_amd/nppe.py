@@ -1,0 +1,287 @@
+"""ctypes binding of libnppe.so, the C ABI declared in include/nppe.h, include/nppe_profile.h and
+include/nppe_synth.h.
+
+The library is the product; this module only marshals numpy arrays (host) or device pointers (e.g. from torch) to
+it. Loading fails loudly when libnppe.so is missing: there is no fallback solver.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnppe.so")
+
+OK, NOT_CONVERGED = 0, 1
+ERR_ARGUMENT, ERR_DEVICE, ERR_NO_GRID, ERR_SIZE, ERR_NONFINITE, ERR_BUDGET, ERR_MISMATCH, ERR_INTERNAL = (
+    -1, -2, -3, -4, -5, -6, -7, -8)
+METHOD_MGPCG, METHOD_CG64 = 0, 1
+FLUID, SOLID, EMPTY, INFLOW, PRESSURE, OUTFLOW = 0, 1, 2, 3, 4, 5
+
+KERNEL_KINDS = [
+    "smooth", "restrict", "prolong", "coarse", "cg_dot", "cg_direction", "cg_spmv", "cg_update", "residual64",
+    "scatter", "gather", "mean", "cg64_init", "cg64_spmv", "cg64_update", "cg64_direction",
+]
+
+# Every symbol the headers in include/ declare.
+EXPORTS = [
+    "nppe_default_options", "nppe_create", "nppe_destroy", "nppe_last_error", "nppe_commit_grid",
+    "nppe_system_info", "nppe_unknown_order", "nppe_check_csr", "nppe_solve", "nppe_solve_device", "nppe_apply",
+    "nppe_memory_bytes", "nppe_profile_enable", "nppe_profile_read", "nppe_synth_cosine_rhs",
+]
+
+
+class NppeError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"nppe error {code}: {msg}")
+        self.code = code
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("device", C.c_int32), ("method", C.c_int32), ("tolerance", C.c_double), ("max_iters", C.c_int32),
+        ("compatibility", C.c_int32), ("pre_smooth", C.c_int32), ("post_smooth", C.c_int32),
+        ("inner_reduction", C.c_double), ("max_refinements", C.c_int32), ("reserved0", C.c_int32),
+        ("hbm_budget", C.c_uint64),
+    ]
+
+
+class GridDesc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("non_uniform", C.c_int32),
+        ("reserved0", C.c_int32), ("dh", C.c_double), ("cell_types", C.c_void_p), ("width_x", C.c_void_p),
+        ("width_y", C.c_void_p), ("width_z", C.c_void_p),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32), ("refinements", C.c_int32), ("status", C.c_int32), ("levels", C.c_int32),
+        ("rhs_norm", C.c_double), ("residual_norm", C.c_double), ("relative_residual", C.c_double),
+        ("device_ms", C.c_double), ("algorithmic_bytes", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Loads libnppe.so; raises when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} is missing: build it with `python -m nereus_cfd_amd.build` "
+                                "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    P, I32, I64, D = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+    L.nppe_default_options.argtypes = [P]
+    L.nppe_default_options.restype = None
+    L.nppe_create.argtypes = [P, P]
+    L.nppe_create.restype = C.c_int
+    L.nppe_destroy.argtypes = [P]
+    L.nppe_destroy.restype = C.c_int
+    L.nppe_last_error.argtypes = [P]
+    L.nppe_last_error.restype = C.c_char_p
+    L.nppe_commit_grid.argtypes = [P, P]
+    L.nppe_commit_grid.restype = I64
+    L.nppe_system_info.argtypes = [P, P, P]
+    L.nppe_system_info.restype = C.c_int
+    L.nppe_unknown_order.argtypes = [P, P]
+    L.nppe_unknown_order.restype = C.c_int
+    L.nppe_check_csr.argtypes = [P, I32, I64, P, P, P]
+    L.nppe_check_csr.restype = C.c_int
+    L.nppe_solve.argtypes = [P, P, P, P, D, P]
+    L.nppe_solve.restype = C.c_int
+    L.nppe_solve_device.argtypes = [P, P, P, P, D, P]
+    L.nppe_solve_device.restype = C.c_int
+    L.nppe_apply.argtypes = [P, P, P]
+    L.nppe_apply.restype = C.c_int
+    L.nppe_memory_bytes.argtypes = [P, P]
+    L.nppe_memory_bytes.restype = C.c_int
+    L.nppe_profile_enable.argtypes = [P, C.c_int]
+    L.nppe_profile_enable.restype = C.c_int
+    L.nppe_profile_read.argtypes = [P, C.c_int, P, P, P]
+    L.nppe_profile_read.restype = C.c_int
+    L.nppe_synth_cosine_rhs.argtypes = [P, C.c_uint64, I32, D, I32, P]
+    L.nppe_synth_cosine_rhs.restype = I64
+    _lib = L
+    return L
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def default_options(**kw) -> Options:
+    o = Options()
+    lib().nppe_default_options(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+@dataclass
+class Grid:
+    """A MAC grid as CommitPressureMatrix receives it: cell types (ny, nx, nz) in MAC order and the width tables
+    (ring cells included)."""
+    dim: int
+    types: np.ndarray
+    wx: np.ndarray
+    wy: np.ndarray
+    wz: np.ndarray
+    dh: float
+    non_uniform: bool = False
+
+    def desc(self) -> GridDesc:
+        ny, nx, nz = self.types.shape
+        self._keep = (np.ascontiguousarray(self.types, np.uint8), np.ascontiguousarray(self.wx, np.float64),
+                      np.ascontiguousarray(self.wy, np.float64), np.ascontiguousarray(self.wz, np.float64))
+        t, wx, wy, wz = self._keep
+        return GridDesc(self.dim, nx, ny, nz, int(self.non_uniform), 0, float(self.dh), t.ctypes.data, wx.ctypes.data,
+                        wy.ctypes.data, wz.ctypes.data)
+
+    @property
+    def unknowns(self) -> int:
+        t = self.types
+        inner = t[1:-1, 1:-1, 1:-1] if self.dim == 3 else t[1:-1, 1:-1, :]
+        return int((inner == FLUID).sum())
+
+
+def uniform_grid(n, dim=3, length=1.0, walls=SOLID) -> Grid:
+    """n interior cells per axis (int or tuple (nx, ny[, nz])), spacing length/n_x, every ring cell `walls`."""
+    if isinstance(n, int):
+        n = (n,) * dim
+    nx, ny = n[0], n[1]
+    nz = n[2] if dim == 3 else 1
+    dh = length / nx
+    shape = (ny + 2, nx + 2, nz + 2) if dim == 3 else (ny + 2, nx + 2, 1)
+    t = np.zeros(shape, np.uint8)
+    t[0] = walls
+    t[-1] = walls
+    t[:, 0] = walls
+    t[:, -1] = walls
+    if dim == 3:
+        t[:, :, 0] = walls
+        t[:, :, -1] = walls
+    return Grid(dim, t, np.full(nx + 2, dh), np.full(ny + 2, dh), np.full(nz + 2 if dim == 3 else 1, dh), dh, False)
+
+
+def synth_rhs(grid: Grid, seed: int = 20261203, modes: int = 8, noise: float = 0.01, zero_mean: bool = True):
+    d = grid.desc()
+    b = np.zeros(max(1, grid.unknowns), np.float64)
+    n = lib().nppe_synth_cosine_rhs(C.byref(d), seed, modes, noise, int(zero_mean), _ptr(b))
+    if n < 0:
+        raise NppeError(int(n), "synth_cosine_rhs")
+    return b[:n].copy()
+
+
+class Solver:
+    """One backend handle (nppe_solver): create, commit a grid, solve."""
+
+    def __init__(self, **opts):
+        self.opts = default_options(**opts)
+        h = C.c_void_p()
+        rc = lib().nppe_create(C.byref(h), C.byref(self.opts))
+        if rc != OK:
+            raise NppeError(rc, lib().nppe_last_error(None).decode())
+        self.h = h
+        self.n = 0
+        self.singular = False
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().nppe_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc: int, what: str):
+        if rc < 0:
+            raise NppeError(rc, f"{what}: {lib().nppe_last_error(self.h).decode()}")
+        return rc
+
+    def commit(self, grid: Grid) -> int:
+        d = grid.desc()
+        n = lib().nppe_commit_grid(self.h, C.byref(d))
+        self._check(int(n) if n < 0 else 0, "commit_grid")
+        n64, sing = C.c_int64(), C.c_int32()
+        lib().nppe_system_info(self.h, C.byref(n64), C.byref(sing))
+        self.n, self.singular = int(n64.value), bool(sing.value)
+        self.grid = grid
+        return self.n
+
+    def order(self) -> np.ndarray:
+        o = np.zeros(max(1, self.n), np.int32)
+        self._check(lib().nppe_unknown_order(self.h, _ptr(o)), "unknown_order")
+        return o[: self.n]
+
+    def solve(self, b, guess=None, reference_norm=0.0):
+        b = np.ascontiguousarray(b, np.float64)
+        if b.shape != (self.n,):
+            raise ValueError(f"b has shape {b.shape}, the system has {self.n} unknowns")
+        g = None if guess is None else np.ascontiguousarray(guess, np.float64)
+        x = np.zeros(max(1, self.n), np.float64)
+        res = Result()
+        rc = self._check(lib().nppe_solve(self.h, _ptr(b), _ptr(x), _ptr(g), float(reference_norm), C.byref(res)),
+                         "solve")
+        return x[: self.n], res, rc
+
+    def solve_device(self, b_ptr: int, x_ptr: int, guess_ptr: int = 0, reference_norm=0.0):
+        res = Result()
+        rc = self._check(lib().nppe_solve_device(self.h, C.c_void_p(b_ptr), C.c_void_p(x_ptr),
+                                                 C.c_void_p(guess_ptr) if guess_ptr else None,
+                                                 float(reference_norm), C.byref(res)), "solve_device")
+        return res, rc
+
+    def apply(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(max(1, self.n), np.float64)
+        self._check(lib().nppe_apply(self.h, _ptr(x), _ptr(y)), "apply")
+        return y[: self.n]
+
+    def check_csr(self, rowptr, col, val) -> int:
+        rowptr = np.ascontiguousarray(rowptr, np.int32)
+        col = np.ascontiguousarray(col, np.int32)
+        val = np.ascontiguousarray(val, np.float64)
+        rc = lib().nppe_check_csr(self.h, self.n, len(col), _ptr(rowptr), _ptr(col), _ptr(val))
+        return rc
+
+    def last_error(self) -> str:
+        return lib().nppe_last_error(self.h).decode()
+
+    def memory_bytes(self) -> int:
+        v = C.c_uint64()
+        self._check(lib().nppe_memory_bytes(self.h, C.byref(v)), "memory_bytes")
+        return int(v.value)
+
+    def profile(self, on: bool = True):
+        self._check(lib().nppe_profile_enable(self.h, int(on)), "profile_enable")
+
+    def profile_read(self):
+        out = {}
+        for k, name in enumerate(KERNEL_KINDS):
+            cnt, ms, by = C.c_int64(), C.c_double(), C.c_double()
+            lib().nppe_profile_read(self.h, k, C.byref(cnt), C.byref(ms), C.byref(by))
+            if cnt.value:
+                out[name] = {"launches": int(cnt.value), "ms": ms.value, "bytes": by.value}
+        return out

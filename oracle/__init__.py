@@ -1,0 +1,199 @@
+"""CPU oracle for the pressure Poisson path (test infrastructure only).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg import this package, and only as
+the checker. The product library (``nereus_cfd_amd``) never imports it.
+
+* ``liboracle.so`` (``oracle.c``): a plain-C restatement of the reference's spacing tables (MAC.cpp:19-175),
+  matrix assembly (PressureSolver3D.cpp:128-178, PressureSolver2D.cpp:100-152) and Eigen 3.3.7 Jacobi-PCG
+  (ConjugateGradient.h:28-91), wrapped here with numpy.
+* ``_ref/ref_harness``: the reference's own sources compiled in place (``make -C oracle ref``); ``refcase``
+  drives it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+FLUID, SOLID, EMPTY, INFLOW, PRESSURE, OUTFLOW = 0, 1, 2, 3, 4, 5
+REFINE_HARD, REFINE_RAMP, REFINE_SYMMETRIC, REFINE_RAMP_BACK = 0, 1, 2, 3
+
+
+class _Ref(C.Structure):
+    _fields_ = [("start", C.c_double), ("end", C.c_double), ("factor", C.c_int32), ("mode", C.c_int32)]
+
+
+class _Grid(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32), ("Nx", C.c_int32), ("Ny", C.c_int32), ("Nz", C.c_int32),
+        ("nonUniform", C.c_int32), ("dh", C.c_double),
+        ("types", C.c_void_p), ("wx", C.c_void_p), ("wy", C.c_void_p), ("wz", C.c_void_p),
+    ]
+
+
+_lib = None
+
+
+def build():
+    """Compiles liboracle.so (gcc)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.oracle_build_axis.argtypes = [C.c_double, C.c_double, C.c_int, C.c_double, P, C.c_int, C.c_int, P, C.c_int]
+        L.oracle_build_axis.restype = C.c_int
+        L.oracle_gap_table.argtypes = [P, C.c_int, P]
+        L.oracle_count_unknowns.argtypes = [P]
+        L.oracle_count_unknowns.restype = C.c_int64
+        L.oracle_assemble.argtypes = [P, P, P, P, P, P]
+        L.oracle_assemble.restype = C.c_int64
+        L.oracle_spmv.argtypes = [C.c_int32, P, P, P, P, P]
+        L.oracle_cg.argtypes = [C.c_int32, P, P, P, P, P, C.c_double, C.c_int, C.c_int, P]
+        L.oracle_cg.restype = C.c_int
+        L.oracle_solve_pressure.argtypes = [C.c_int32, P, P, P, P, P, P, C.c_double, C.c_int, C.c_double]
+        L.oracle_solve_pressure.restype = C.c_int
+        L.oracle_subtract_mean.argtypes = [P, C.c_int64]
+        L.oracle_relres.argtypes = [C.c_int32, P, P, P, P, P]
+        L.oracle_relres.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+@dataclass
+class Refinement:
+    start: float
+    end: float
+    factor: int
+    mode: int = REFINE_HARD
+
+
+def build_axis(a0, af, n_base, dh, refs=(), non_uniform=False):
+    """Widths of one axis including the ring (MAC::BuildAxis + MAC::AddRing)."""
+    arr = (_Ref * max(1, len(refs)))(*[_Ref(r.start, r.end, r.factor, r.mode) for r in refs])
+    cap = 4 * n_base * max([1] + [int(r.factor) for r in refs]) + 16
+    out = np.zeros(cap, np.float64)
+    n = lib().oracle_build_axis(a0, af, n_base, dh, C.cast(arr, C.c_void_p), len(refs), int(non_uniform),
+                                _p(out), cap)
+    if n < 0:
+        raise ValueError("axis too long")
+    return out[:n].copy()
+
+
+def gap_table(w):
+    w = np.ascontiguousarray(w, np.float64)
+    g = np.zeros(len(w) + 1, np.float64)
+    lib().oracle_gap_table(_p(w), len(w), _p(g))
+    return g
+
+
+@dataclass
+class Grid:
+    """A MAC grid as the pressure path sees it: dims incl. the ring, cell types and width tables."""
+    dim: int
+    types: np.ndarray          # uint8, shape (Ny, Nx, Nz) (Nz = 1 in 2D)
+    wx: np.ndarray
+    wy: np.ndarray
+    wz: np.ndarray
+    dh: float
+    non_uniform: bool = False
+
+    @property
+    def shape(self):
+        return self.types.shape
+
+    def _c(self):
+        Ny, Nx, Nz = self.types.shape
+        self._keep = [np.ascontiguousarray(self.types, np.uint8), np.ascontiguousarray(self.wx, np.float64),
+                      np.ascontiguousarray(self.wy, np.float64), np.ascontiguousarray(self.wz, np.float64)]
+        t, wx, wy, wz = self._keep
+        return _Grid(self.dim, Nx, Ny, Nz, int(self.non_uniform), self.dh, t.ctypes.data, wx.ctypes.data,
+                     wy.ctypes.data, wz.ctypes.data)
+
+
+@dataclass
+class System:
+    n: int
+    order: np.ndarray
+    rowptr: np.ndarray
+    col: np.ndarray
+    val: np.ndarray
+    singular: bool
+    extra: dict = field(default_factory=dict)
+
+    def matvec(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.n, np.float64)
+        lib().oracle_spmv(self.n, _p(self.rowptr), _p(self.col), _p(self.val), _p(x), _p(y))
+        return y
+
+    def relres(self, b, x):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.ascontiguousarray(x, np.float64)
+        return lib().oracle_relres(self.n, _p(self.rowptr), _p(self.col), _p(self.val), _p(b), _p(x))
+
+    def cg(self, b, tol, max_iters=100000, x0=None, precond="diagonal"):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(self.n, np.float64) if x0 is None else np.array(x0, np.float64)
+        err = np.zeros(1, np.float64)
+        it = lib().oracle_cg(self.n, _p(self.rowptr), _p(self.col), _p(self.val), _p(b), _p(x), tol, max_iters,
+                             0 if precond == "diagonal" else 1, _p(err))
+        return x, it, float(err[0])
+
+    def solve_pressure(self, b, tol, max_iters=100000, guess=None, reference_norm=0.0):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(self.n, np.float64)
+        g = None if guess is None else np.ascontiguousarray(guess, np.float64)
+        it = lib().oracle_solve_pressure(self.n, _p(self.rowptr), _p(self.col), _p(self.val), _p(b), _p(x), _p(g),
+                                         tol, max_iters, reference_norm)
+        return x, it
+
+    def dense(self):
+        A = np.zeros((self.n, self.n))
+        for r in range(self.n):
+            for q in range(self.rowptr[r], self.rowptr[r + 1]):
+                A[r, self.col[q]] = self.val[q]
+        return A
+
+
+def assemble(grid: Grid) -> System:
+    g = grid._c()
+    L = lib()
+    n = int(L.oracle_count_unknowns(C.byref(g)))
+    nq = 7 if grid.dim == 3 else 5
+    order = np.zeros(max(n, 1), np.int32)
+    rowptr = np.zeros(n + 1, np.int32)
+    col = np.zeros(max(1, nq * n), np.int32)
+    val = np.zeros(max(1, nq * n), np.float64)
+    sing = np.zeros(1, np.int32)
+    nnz = int(L.oracle_assemble(C.byref(g), _p(order), _p(rowptr), _p(col), _p(val), _p(sing)))
+    return System(n, order[:n].copy(), rowptr, col[:nnz].copy(), val[:nnz].copy(), bool(sing[0]))
+
+
+def subtract_mean(v):
+    v = np.ascontiguousarray(v, np.float64).copy()
+    lib().oracle_subtract_mean(_p(v), len(v))
+    return v
+
+
+def compatibility(b):
+    """The RHS compatibility step: the mean removed twice (PressureSolver3D.cpp:300-307)."""
+    b = np.array(b, np.float64)
+    if len(b) == 0:
+        return b
+    return subtract_mean(subtract_mean(b))

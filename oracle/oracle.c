@@ -1,0 +1,430 @@
+/* CPU oracle: plain-C restatement of the reference's pressure path. TEST INFRASTRUCTURE ONLY -- only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load this, as the checker; the product library never
+ * does. Every function names the reference lines it follows; see oracle.h for how each one is pinned. */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---- spacing tables: MAC.cpp:19-175 ---------------------------------------------------------------------------- */
+
+/* MAC::FactorAt (MAC.cpp:19-41): the largest local refinement factor over the intervals containing x. */
+static double factor_at(const oracle_refinement *refs, int nrefs, double x)
+{
+    double f = 1.0;
+    for (int q = 0; q < nrefs; q++)
+    {
+        const oracle_refinement *r = &refs[q];
+        if (r->factor <= 1) continue;
+        if (x < r->start || x > r->end) continue;
+        double local = (double)r->factor;
+        const double span = r->end - r->start;
+        if (span > 0.0 && r->mode != ORACLE_REFINE_HARD)
+        {
+            const double t = (x - r->start) / span;
+            const double sN = (r->mode == ORACLE_REFINE_RAMP)        ? t
+                              : (r->mode == ORACLE_REFINE_RAMP_BACK) ? (1.0 - t)
+                                                                     : (1.0 - fabs(2.0 * t - 1.0));
+            local = 1.0 + (r->factor - 1.0) * sN;
+        }
+        if (local > f) f = local;
+    }
+    return f;
+}
+
+typedef struct
+{
+    double *v;
+    int n, cap, overflow;
+} dvec;
+
+static void dpush(dvec *d, double x)
+{
+    if (d->n >= d->cap) { d->overflow = 1; return; }
+    d->v[d->n++] = x;
+}
+
+static int cmp_double(const void *a, const void *b)
+{
+    const double x = *(const double *)a, y = *(const double *)b;
+    return (x > y) - (x < y);
+}
+
+/* MAC::BuildAxis (MAC.cpp:43-146) followed by MAC::AddRing (:150-161); only the widths are produced (face and
+ * centre positions are not read by the pressure path). */
+int oracle_build_axis(double a0, double af, int nBase, double dh, const oracle_refinement *refs, int nrefs,
+                      int nonUniform, double *widths, int cap)
+{
+    (void)af;
+    int anyActive = 0, anyGraded = 0;
+    for (int q = 0; q < nrefs; q++)
+        if (refs[q].factor > 1)
+        {
+            anyActive = 1;
+            if (refs[q].mode != ORACLE_REFINE_HARD) anyGraded = 1;
+        }
+    if (cap < 3) return -1;
+    dvec size = {widths + 1, 0, cap - 2, 0};
+
+    if (!nonUniform || !anyActive || nBase <= 0)
+    {
+        for (int j = 0; j < nBase; j++) dpush(&size, dh);
+    }
+    else
+    {
+        const double aEnd = a0 + nBase * dh;
+        if (!anyGraded)
+        {
+            for (int j = 0; j < nBase; j++)
+            {
+                const double c = a0 + j * dh + dh / 2.0;
+                int f = (int)(factor_at(refs, nrefs, c) + 0.5);
+                if (f < 1) f = 1;
+                for (int t = 0; t < f; t++) dpush(&size, dh / (double)f);
+            }
+        }
+        else
+        {
+            double *anchors = (double *)malloc(sizeof(double) * (size_t)(2 + 2 * nrefs));
+            int na = 0;
+            anchors[na++] = a0;
+            anchors[na++] = aEnd;
+            for (int q = 0; q < nrefs; q++)
+                if (refs[q].factor > 1 && refs[q].mode == ORACLE_REFINE_HARD)
+                {
+                    const double e2[2] = {refs[q].start, refs[q].end};
+                    for (int s = 0; s < 2; s++)
+                        if (e2[s] > a0 + 1e-12 && e2[s] < aEnd - 1e-12) anchors[na++] = e2[s];
+                }
+            qsort(anchors, (size_t)na, sizeof(double), cmp_double);
+            int ne = 0;
+            for (int s = 0; s < na; s++)
+                if (ne == 0 || anchors[s] - anchors[ne - 1] > 1e-12) anchors[ne++] = anchors[s];
+            const int guard = 4000000;
+            for (int s = 0; s + 1 < ne; s++)
+            {
+                const double p = anchors[s], q = anchors[s + 1], L = q - p;
+                int graded = 0;
+                for (int m = 0; m < nrefs; m++)
+                    if (refs[m].factor > 1 && refs[m].mode != ORACLE_REFINE_HARD && refs[m].start < q - 1e-12 &&
+                        refs[m].end > p + 1e-12)
+                        graded = 1;
+                const double fm = factor_at(refs, nrefs, 0.5 * (p + q));
+                if (!graded)
+                {
+                    int n = (int)lround(L * fm / dh);
+                    if (n < 1) n = 1;
+                    for (int t = 0; t < n; t++) dpush(&size, L / n);
+                    continue;
+                }
+                const int first = size.n;
+                double x = p;
+                while (x < q - 1e-12 && size.n < guard && !size.overflow)
+                {
+                    double fa = factor_at(refs, nrefs, x);
+                    if (fa < 1.0) fa = 1.0;
+                    const double h = dh / fa;
+                    if (h <= 0.0 || !isfinite(h)) break;
+                    dpush(&size, h);
+                    x += h;
+                }
+                if (size.n == first) dpush(&size, L);
+                double sum = 0.0;
+                for (int t = first; t < size.n; t++) sum += size.v[t];
+                for (int t = first; t < size.n; t++) size.v[t] *= L / sum;
+            }
+            free(anchors);
+        }
+    }
+    if (size.overflow || size.n == 0) return size.n == 0 && !size.overflow ? 0 : -1;
+    /* AddRing: one cell outside each end mirroring the adjacent width. */
+    widths[0] = size.v[0];
+    widths[size.n + 1] = size.v[size.n - 1];
+    return size.n + 2;
+}
+
+/* MAC::GapTable (MAC.cpp:165-175). */
+void oracle_gap_table(const double *w, int n, double *gap)
+{
+    for (int m = -1; m <= n - 1; m++)
+    {
+        const double a = w[m < 0 ? 0 : (m >= n ? n - 1 : m)];
+        const double b = w[m + 1 >= n ? n - 1 : m + 1];
+        gap[m + 1] = 0.5 * (a + b);
+    }
+}
+
+/* ---- assembly: PressureSolver3D.cpp:32-87,128-178; PressureSolver2D.cpp:31-60,100-152 ------------------------- */
+
+static int is_dirichlet(int t) { return t == ORACLE_EMPTY || t == ORACLE_PRESSURE; }
+
+static double face_gap(const double *w, int N, int m)
+{
+    /* FaceGap(a, m) = gap[m + 1] of GapTable over the N widths. */
+    const double a = w[m < 0 ? 0 : (m >= N ? N - 1 : m)];
+    const double b = w[m + 1 >= N ? N - 1 : m + 1];
+    return 0.5 * (a + b);
+}
+
+/* DirichletRingFactor (PressureSettings.h:307-310). */
+static double ring_factor_axis(const double *w, int N, int n, int s)
+{
+    return face_gap(w, N, s < 0 ? n - 1 : n) / (0.5 * w[n]);
+}
+
+int64_t oracle_count_unknowns(const oracle_grid *g)
+{
+    int64_t n = 0;
+    const int Nz = g->dim == 2 ? 1 : g->Nz;
+    const int k0 = g->dim == 2 ? 0 : 1, k1 = g->dim == 2 ? 1 : Nz - 1;
+    for (int i = 1; i < g->Ny - 1; i++)
+        for (int j = 1; j < g->Nx - 1; j++)
+            for (int k = k0; k < k1; k++)
+                if (g->types[((int64_t)i * g->Nx + j) * Nz + k] == ORACLE_FLUID) n++;
+    return n;
+}
+
+typedef struct
+{
+    int32_t col;
+    double val;
+} entry;
+
+static void sort_row(entry *e, int m)
+{
+    for (int a = 1; a < m; a++)
+    {
+        entry t = e[a];
+        int b = a - 1;
+        while (b >= 0 && e[b].col > t.col) { e[b + 1] = e[b]; b--; }
+        e[b + 1] = t;
+    }
+}
+
+int64_t oracle_assemble(const oracle_grid *g, int32_t *order, int32_t *rowptr, int32_t *col, double *val,
+                        int32_t *singular)
+{
+    const int dim = g->dim, Nx = g->Nx, Ny = g->Ny, Nz = dim == 2 ? 1 : g->Nz;
+    const int64_t cells = (int64_t)Nx * Ny * Nz;
+    int32_t *idp = (int32_t *)malloc(sizeof(int32_t) * (size_t)cells);
+    for (int64_t c = 0; c < cells; c++) idp[c] = -1;
+    const int k0 = dim == 2 ? 0 : 1, k1 = dim == 2 ? 1 : Nz - 1;
+    int32_t n = 0;
+    /* IDP numbering, (i, j, k) lexicographic (PressureSolver3D.cpp:133-139; 2D :107-112). */
+    for (int i = 1; i < Ny - 1; i++)
+        for (int j = 1; j < Nx - 1; j++)
+            for (int k = k0; k < k1; k++)
+            {
+                const int64_t c = ((int64_t)i * Nx + j) * Nz + k;
+                if (g->types[c] == ORACLE_FLUID) { order[n] = (int32_t)c; idp[c] = n++; }
+            }
+    const double scale = g->nonUniform ? 1.0 : 1.0 / (g->dh * g->dh);
+    const int nq = dim == 2 ? 4 : 6;
+    int dirichlet = 0;
+    int64_t nnz = 0;
+    rowptr[0] = 0;
+    for (int32_t row = 0; row < n; row++)
+    {
+        const int64_t c = order[row];
+        const int k = (int)(c % Nz), j = (int)((c / Nz) % Nx), i = (int)(c / ((int64_t)Nz * Nx));
+        int ni[6], nj[6], nk[6];
+        double cq[6];
+        if (dim == 3)
+        {
+            /* neighbour order (i+1, i-1, j+1, j-1, k+1, k-1): PressureSolver3D.cpp:66-68 */
+            const int DI[6] = {1, -1, 0, 0, 0, 0}, DJ[6] = {0, 0, 1, -1, 0, 0}, DK[6] = {0, 0, 0, 0, 1, -1};
+            for (int q = 0; q < 6; q++) { ni[q] = i + DI[q]; nj[q] = j + DJ[q]; nk[q] = k + DK[q]; }
+            if (!g->nonUniform)
+                for (int q = 0; q < 6; q++) cq[q] = 1.0;
+            else
+            {
+                /* FaceCoefs3D (:47-63) with MAC::FaceArea (MAC.h:91-93). */
+                const double ax = g->wy[i] * g->wz[k];
+                const double ay = g->wx[j] * g->wz[k];
+                const double az = g->wx[j] * g->wy[i];
+                cq[0] = ay / face_gap(g->wy, Ny, i);
+                cq[1] = ay / face_gap(g->wy, Ny, i - 1);
+                cq[2] = ax / face_gap(g->wx, Nx, j);
+                cq[3] = ax / face_gap(g->wx, Nx, j - 1);
+                cq[4] = az / face_gap(g->wz, Nz, k);
+                cq[5] = az / face_gap(g->wz, Nz, k - 1);
+            }
+        }
+        else
+        {
+            /* N, S, E, W (PressureSolver2D.cpp:122-128) */
+            const int I2[4] = {i + 1, i - 1, i, i}, J2[4] = {j, j, j + 1, j - 1};
+            for (int q = 0; q < 4; q++) { ni[q] = I2[q]; nj[q] = J2[q]; nk[q] = 0; }
+            cq[0] = g->wx[j] / face_gap(g->wy, Ny, i);
+            cq[1] = g->wx[j] / face_gap(g->wy, Ny, i - 1);
+            cq[2] = g->wy[i] / face_gap(g->wx, Nx, j);
+            cq[3] = g->wy[i] / face_gap(g->wx, Nx, j - 1);
+        }
+        entry e[7];
+        int m = 0;
+        double diag = 0.0;
+        for (int q = 0; q < nq; q++)
+        {
+            const int64_t nc = ((int64_t)ni[q] * Nx + nj[q]) * Nz + nk[q];
+            if (g->types[nc] == ORACLE_FLUID)
+            {
+                diag -= cq[q];
+                if (idp[nc] >= 0) { e[m].col = idp[nc]; e[m].val = scale * cq[q]; m++; }
+                else dirichlet = 1;
+            }
+        }
+        for (int q = 0; q < nq; q++)
+        {
+            const int64_t nc = ((int64_t)ni[q] * Nx + nj[q]) * Nz + nk[q];
+            if (!is_dirichlet(g->types[nc])) continue;
+            /* RingFactor3D (:80-87) / RingFactor2D (:54-60) */
+            double rf = 1.0;
+            const int onRing = ni[q] == 0 || ni[q] == Ny - 1 || nj[q] == 0 || nj[q] == Nx - 1 ||
+                               (dim == 3 && (nk[q] == 0 || nk[q] == Nz - 1));
+            if (onRing)
+            {
+                const int s = (q % 2 == 0) ? 1 : -1;
+                rf = q < 2 ? ring_factor_axis(g->wy, Ny, i, s)
+                           : (q < 4 ? ring_factor_axis(g->wx, Nx, j, s) : ring_factor_axis(g->wz, Nz, k, s));
+            }
+            diag -= cq[q] * rf;
+            dirichlet = 1;
+        }
+        if (diag != 0.0) { e[m].col = row; e[m].val = scale * diag; m++; }
+        /* setFromTriplets + makeCompressed: columns ascending within the row (PressureSettings.h:557-559). */
+        sort_row(e, m);
+        for (int a = 0; a < m; a++) { col[nnz] = e[a].col; val[nnz] = e[a].val; nnz++; }
+        rowptr[row + 1] = (int32_t)nnz;
+    }
+    *singular = dirichlet ? 0 : 1;
+    free(idp);
+    return nnz;
+}
+
+/* ---- Krylov: Eigen 3.3.7 ConjugateGradient.h:28-91, BasicPreconditioners.h:36-91 ------------------------------- */
+
+void oracle_spmv(int32_t n, const int32_t *rowptr, const int32_t *col, const double *val, const double *x, double *y)
+{
+    for (int32_t r = 0; r < n; r++)
+    {
+        double t = 0.0;
+        for (int32_t q = rowptr[r]; q < rowptr[r + 1]; q++) t += val[q] * x[col[q]];
+        y[r] = t;
+    }
+}
+
+static double dot(int32_t n, const double *a, const double *b)
+{
+    double s = 0.0;
+    for (int32_t q = 0; q < n; q++) s += a[q] * b[q];
+    return s;
+}
+
+int oracle_cg(int32_t n, const int32_t *rowptr, const int32_t *col, const double *val, const double *b, double *x,
+              double tol, int maxIters, int precond, double *tol_error)
+{
+    double *r = (double *)malloc(sizeof(double) * (size_t)(n ? n : 1));
+    double *p = (double *)malloc(sizeof(double) * (size_t)(n ? n : 1));
+    double *z = (double *)malloc(sizeof(double) * (size_t)(n ? n : 1));
+    double *t = (double *)malloc(sizeof(double) * (size_t)(n ? n : 1));
+    double *inv = (double *)malloc(sizeof(double) * (size_t)(n ? n : 1));
+    /* DiagonalPreconditioner::factorize: 1/diag, or 1 where the row has no (or a zero) diagonal. */
+    for (int32_t rr = 0; rr < n; rr++)
+    {
+        inv[rr] = 1.0;
+        if (precond != 0) continue;
+        for (int32_t q = rowptr[rr]; q < rowptr[rr + 1]; q++)
+            if (col[q] == rr && val[q] != 0.0) inv[rr] = 1.0 / val[q];
+    }
+    int iters = 0;
+    oracle_spmv(n, rowptr, col, val, x, t);
+    for (int32_t q = 0; q < n; q++) r[q] = b[q] - t[q];
+    const double rhsNorm2 = dot(n, b, b);
+    if (rhsNorm2 == 0.0)
+    {
+        for (int32_t q = 0; q < n; q++) x[q] = 0.0;
+        *tol_error = 0.0;
+        goto done;
+    }
+    {
+        const double considerAsZero = 2.2250738585072014e-308; /* numeric_limits<double>::min() */
+        double threshold = tol * tol * rhsNorm2;
+        if (threshold < considerAsZero) threshold = considerAsZero;
+        double rn2 = dot(n, r, r);
+        if (rn2 < threshold)
+        {
+            *tol_error = sqrt(rn2 / rhsNorm2);
+            goto done;
+        }
+        for (int32_t q = 0; q < n; q++) p[q] = inv[q] * r[q];
+        double absNew = dot(n, r, p);
+        int i = 0;
+        while (i < maxIters)
+        {
+            oracle_spmv(n, rowptr, col, val, p, t);
+            const double alpha = absNew / dot(n, p, t);
+            for (int32_t q = 0; q < n; q++) x[q] += alpha * p[q];
+            for (int32_t q = 0; q < n; q++) r[q] -= alpha * t[q];
+            rn2 = dot(n, r, r);
+            if (rn2 < threshold) break;
+            for (int32_t q = 0; q < n; q++) z[q] = inv[q] * r[q];
+            const double absOld = absNew;
+            absNew = dot(n, r, z);
+            const double beta = absNew / absOld;
+            for (int32_t q = 0; q < n; q++) p[q] = z[q] + beta * p[q];
+            i++;
+        }
+        *tol_error = sqrt(rn2 / rhsNorm2);
+        iters = i;
+    }
+done:
+    free(r); free(p); free(z); free(t); free(inv);
+    return iters;
+}
+
+/* SolveEigenPressure's floor handling (PressureSettings.h:415-427) for eigen_solver = CG, DIAGONAL. */
+int oracle_solve_pressure(int32_t n, const int32_t *rowptr, const int32_t *col, const double *val, const double *b,
+                          double *x, const double *guess, double tol, int maxIters, double referenceNorm)
+{
+    double s = tol;
+    if (referenceNorm > 0.0)
+    {
+        const double bn = sqrt(dot(n, b, b));
+        if (bn <= tol * referenceNorm)
+        {
+            for (int32_t q = 0; q < n; q++) x[q] = guess ? guess[q] : 0.0;
+            return 0;
+        }
+        s = tol * referenceNorm / bn;
+        if (s > 0.5) s = 0.5;
+    }
+    for (int32_t q = 0; q < n; q++) x[q] = guess ? guess[q] : 0.0;
+    double err = 0.0;
+    return oracle_cg(n, rowptr, col, val, b, x, s, maxIters, 0, &err);
+}
+
+/* SubtractMean (PressureSettings.h:314-321). */
+void oracle_subtract_mean(double *v, int64_t n)
+{
+    if (v == NULL || n <= 0) return;
+    double m = 0.0;
+    for (int64_t i = 0; i < n; i++) m += v[i];
+    m /= (double)n;
+    for (int64_t i = 0; i < n; i++) v[i] -= m;
+}
+
+double oracle_relres(int32_t n, const int32_t *rowptr, const int32_t *col, const double *val, const double *b,
+                     const double *x)
+{
+    double rr = 0.0, bb = 0.0;
+    for (int32_t r = 0; r < n; r++)
+    {
+        double t = 0.0;
+        for (int32_t q = rowptr[r]; q < rowptr[r + 1]; q++) t += val[q] * x[col[q]];
+        const double d = b[r] - t;
+        rr += d * d;
+        bb += b[r] * b[r];
+    }
+    return bb > 0.0 ? sqrt(rr / bb) : sqrt(rr);
+}

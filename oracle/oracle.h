@@ -1,0 +1,82 @@
+/* CPU oracle for the NEREUS pressure Poisson path: a plain-C restatement of the reference's MAC spacing tables,
+ * matrix assembly and FP64 Krylov solve, used ONLY by the tests and by bench.py's CPU-baseline leg as the checker.
+ * Nothing in the product library links or calls it.
+ *
+ * Parity status of each function (how it is pinned):
+ *   oracle_build_axis     - bit-exact against the compiled reference's MAC tables (golden fixtures), uniform,
+ *                           hard and graded refinement; widths sum to the domain.
+ *   oracle_gap_table      - bit-exact against the reference's FaceGap tables (golden fixtures).
+ *   oracle_assemble       - bit-exact CSR (pattern and values) against the compiled reference's assembly for
+ *                           2D/3D, uniform/stretched, solids, EMPTY/PRESSURE/FLUID ring cells (golden fixtures);
+ *                           plus symmetry, A*1 = 0 on Neumann rows and the linear-field closed form.
+ *   oracle_cg             - iteration counts and solutions against the reference's Eigen CG (golden fixtures);
+ *                           exact solve on tiny systems against a dense factorisation.
+ *   oracle_subtract_mean  - closed form (mean of the result is 0 to round-off).
+ */
+#ifndef NPPE_ORACLE_H
+#define NPPE_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Cell types (reference src/headers/Core/Definitions.h:191-196). */
+enum { ORACLE_FLUID = 0, ORACLE_SOLID = 1, ORACLE_EMPTY = 2, ORACLE_INFLOW = 3, ORACLE_PRESSURE = 4, ORACLE_OUTFLOW = 5 };
+/* Refinement modes (Definitions.h:238-244). */
+enum { ORACLE_REFINE_HARD = 0, ORACLE_REFINE_RAMP = 1, ORACLE_REFINE_SYMMETRIC = 2, ORACLE_REFINE_RAMP_BACK = 3 };
+
+typedef struct
+{
+    double start, end;
+    int32_t factor, mode;
+} oracle_refinement;
+
+/* Widths of one axis including the two ring cells (MAC::BuildAxis + MAC::AddRing). Returns the count, or -1 when
+ * `cap` is too small. */
+int oracle_build_axis(double a0, double af, int nBase, double dh, const oracle_refinement *refs, int nrefs,
+                      int nonUniform, double *widths, int cap);
+
+/* FaceGap(a, m) for m = -1 .. n-1, stored at gap[m + 1] (MAC::GapTable). */
+void oracle_gap_table(const double *w, int n, double *gap);
+
+typedef struct
+{
+    int32_t dim;        /* 2 or 3 */
+    int32_t Nx, Ny, Nz; /* cells including the ring; Nz = 1 in 2D */
+    int32_t nonUniform;
+    double dh;
+    const uint8_t *types; /* index (i*Nx + j)*Nz + k */
+    const double *wx, *wy, *wz; /* widths incl. ring: Nx, Ny, Nz entries (wz unused in 2D) */
+} oracle_grid;
+
+/* Unknowns: interior FLUID cells. */
+int64_t oracle_count_unknowns(const oracle_grid *g);
+
+/* Assembles the reference's CSR matrix. order[n] gets each unknown's box index; rowptr[n+1], col/val up to
+ * 7n (5n in 2D) entries. Returns nnz; *singular set when no Dirichlet face exists. */
+int64_t oracle_assemble(const oracle_grid *g, int32_t *order, int32_t *rowptr, int32_t *col, double *val,
+                        int32_t *singular);
+
+void oracle_spmv(int32_t n, const int32_t *rowptr, const int32_t *col, const double *val, const double *x, double *y);
+
+/* Jacobi (precond = 0) or unpreconditioned (precond = 1) CG with Eigen 3.3.7's stopping rule; x holds the guess on
+ * entry. Returns iterations; *tol_error gets sqrt(|r|^2/|b|^2) of the recursive residual. */
+int oracle_cg(int32_t n, const int32_t *rowptr, const int32_t *col, const double *val, const double *b, double *x,
+              double tol, int maxIters, int precond, double *tol_error);
+
+/* SolveEigenPressure's absolute-floor mapping (PressureSettings.h:411-427) around oracle_cg. */
+int oracle_solve_pressure(int32_t n, const int32_t *rowptr, const int32_t *col, const double *val, const double *b,
+                          double *x, const double *guess, double tol, int maxIters, double referenceNorm);
+
+void oracle_subtract_mean(double *v, int64_t n);
+
+/* |b - A x|_2 / |b|_2 in FP64. */
+double oracle_relres(int32_t n, const int32_t *rowptr, const int32_t *col, const double *val, const double *b,
+                     const double *x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,230 @@
+// ref_harness: drives the reference's own pressure path, compiled from its sources (oracle/Makefile, target `ref`),
+// to write golden vectors and to time the reference's CPU solvers. Test infrastructure only; never linked into the
+// product library.
+//
+//   ref_harness <case.bin> <out.bin>
+//
+// The case names a grid the way a NEREUS config does (dimension, GRID_SIZE, domain, NON_UNIFORM_GRID and refinement
+// intervals), the cell types, and a list of solves. The harness builds the MAC grid with the reference's
+// MAC::InitializeGrid (MAC.cpp:275-328), sets the cell types, assembles with
+// PressureSolver{2D,3D}::InitializePressureSolver (PressureSolver3D.cpp:89-102, :128-178) and solves with the
+// reference's SolvePressureSystem (PressureSettings.h:578-624) for each requested eigen_solver / preconditioner /
+// tolerance. It writes the spacing tables, the unknown numbering, the CSR matrix, and each solution with its
+// iteration count and wall times.
+
+#include <omp.h>
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "Methods/PPESolver/PressureSettings.h"
+#include "headers/Core/MAC.h"
+#include "headers/Core/Definitions.h"
+#include "headers/Core/Scene.h"
+#include "headers/Core/SimulationManager.h"
+// The solvers keep their system in private statics; the harness reads them.
+#define private public
+#include "Methods/PPESolver/PressureSolver2D.h"
+#include "Methods/PPESolver/PressureSolver3D.h"
+#undef private
+#include "TestMethods.h"
+
+namespace
+{
+struct Reader
+{
+    FILE *f;
+    template <class T> T get()
+    {
+        T v;
+        if (std::fread(&v, sizeof(T), 1, f) != 1) { std::fprintf(stderr, "ref_harness: short case file\n"); std::exit(2); }
+        return v;
+    }
+    template <class T> void get(T *p, size_t n)
+    {
+        if (n && std::fread(p, sizeof(T), n, f) != n) { std::fprintf(stderr, "ref_harness: short case file\n"); std::exit(2); }
+    }
+};
+
+struct Writer
+{
+    FILE *f;
+    template <class T> void put(T v) { std::fwrite(&v, sizeof(T), 1, f); }
+    template <class T> void put(const T *p, size_t n) { if (n) std::fwrite(p, sizeof(T), n, f); }
+};
+
+struct SolveSpec
+{
+    int solver, precond, maxIters, repeats;
+    double tol, refNorm;
+};
+
+MAC GRID;
+
+void WriteAxis(Writer &w, const MAC &g, MAC::Axis a, int N)
+{
+    for (int n = 0; n < N; n++) w.put<double>(g.Width(a, n));
+    for (int n = -1; n < N; n++) w.put<double>(g.FaceGap(a, n));
+}
+} // namespace
+
+int main(int argc, char **argv)
+{
+    if (argc != 3) { std::fprintf(stderr, "usage: ref_harness <case.bin> <out.bin>\n"); return 2; }
+    FILE *fi = std::fopen(argv[1], "rb");
+    if (!fi) { std::fprintf(stderr, "ref_harness: cannot open %s\n", argv[1]); return 2; }
+    Reader r{fi};
+    char magic[8];
+    r.get(magic, 8);
+    if (std::memcmp(magic, "NPPECASE", 8) != 0) { std::fprintf(stderr, "ref_harness: bad magic\n"); return 2; }
+
+    const int dim = r.get<int32_t>();
+    const int gridSize = r.get<int32_t>();
+    double dom[6];
+    r.get(dom, 6);
+    const int nonUniform = r.get<int32_t>();
+    int nrefs[3];
+    r.get(nrefs, 3);
+    std::vector<GridRefinement> refs[3];
+    for (int a = 0; a < 3; a++)
+        for (int q = 0; q < nrefs[a]; q++)
+        {
+            GridRefinement g;
+            g.active = true;
+            g.start = r.get<double>();
+            g.end = r.get<double>();
+            g.factor = r.get<int32_t>();
+            g.mode = r.get<int32_t>();
+            refs[a].push_back(g);
+        }
+
+    DIMENSION = dim;
+    SelectMethod("ADI");
+    GPU_ACCELERATION = 0;
+    SIMULATION.domain = Domain{dom[0], dom[1], dom[2], dom[3], dom[4], dom[5]};
+    SIMULATION.GRID_SIZE = gridSize;
+    SIMULATION.RHO = 1.0;
+    SIMULATION.dt = 1e-3;
+    SIMULATION.time = 0.0;
+    SIMULATION.grid.nonUniform = nonUniform != 0;
+    SIMULATION.grid.refineX = refs[0];
+    SIMULATION.grid.refineY = refs[1];
+    SIMULATION.grid.refineZ = refs[2];
+    PRESSURE_SETTINGS.warmStart = false;
+
+    GRID = MAC();
+    GRID.InitializeGrid(SIMULATION.domain, dim == 2);
+    const int Nx = GRID.Nx, Ny = GRID.Ny, Nz = dim == 2 ? 1 : GRID.Nz;
+    const size_t cells = (size_t)Nx * Ny * Nz;
+
+    FILE *fo = std::fopen(argv[2], "wb");
+    if (!fo) { std::fprintf(stderr, "ref_harness: cannot write %s\n", argv[2]); return 2; }
+    Writer w{fo};
+    w.put("NPPEOUT1", 8);
+    w.put<int32_t>(Nx);
+    w.put<int32_t>(Ny);
+    w.put<int32_t>(Nz);
+    w.put<int32_t>(GRID.nonUniform ? 1 : 0);
+    w.put<double>(GRID.dh);
+    WriteAxis(w, GRID, MAC::X, Nx);
+    WriteAxis(w, GRID, MAC::Y, Ny);
+    if (dim == 3) WriteAxis(w, GRID, MAC::Z, Nz);
+
+    const int hasTypes = r.get<int32_t>();
+    if (!hasTypes) { std::fclose(fo); return 0; }
+    const int32_t tNx = r.get<int32_t>(), tNy = r.get<int32_t>(), tNz = r.get<int32_t>();
+    if (tNx != Nx || tNy != Ny || tNz != Nz)
+    {
+        std::fprintf(stderr, "ref_harness: types are %dx%dx%d, the grid is %dx%dx%d\n", tNx, tNy, tNz, Nx, Ny, Nz);
+        return 3;
+    }
+    std::vector<uint8_t> types(cells);
+    r.get(types.data(), cells);
+    for (int i = 0; i < Ny; i++)
+        for (int j = 0; j < Nx; j++)
+            for (int k = 0; k < Nz; k++)
+            {
+                const int t = types[((size_t)i * Nx + j) * Nz + k];
+                if (dim == 2) GRID.SetSolid(i, j, t);
+                else GRID.SetSolid(i, j, k, t);
+            }
+    PRESSURE_SETTINGS.needsCompatibilityCondition = r.get<int32_t>() != 0;
+
+    int n = 0, nnz = 0, singular = 0;
+    const RowMatrix *A = nullptr;
+    PressureSystem *sys = nullptr;
+    if (dim == 2)
+    {
+        PressureSolver2D::InitializePressureSolver(&GRID, PRESSURE_CHORIN);
+        sys = &PressureSolver2D::SYSTEM;
+    }
+    else
+    {
+        PressureSolver3D::InitializePressureSolver(&GRID, PRESSURE_CHORIN);
+        sys = &PressureSolver3D::SYSTEM;
+    }
+    A = &sys->A;
+    n = (int)A->rows();
+    nnz = (int)A->nonZeros();
+    singular = sys->singular ? 1 : 0;
+    w.put<int32_t>(n);
+    w.put<int32_t>(nnz);
+    w.put<int32_t>(singular);
+    // The unknowns in matrix order, as box indices (i*Nx + j)*Nz + k.
+    std::vector<int32_t> order(n, -1);
+    for (int i = 0; i < Ny; i++)
+        for (int j = 0; j < Nx; j++)
+            for (int k = 0; k < Nz; k++)
+            {
+                const int id = dim == 2 ? PressureSolver2D::GetIDP(i, j) : PressureSolver3D::GetIDP(i, j, k);
+                if (id >= 0) order[id] = (int32_t)(((size_t)i * Nx + j) * Nz + k);
+            }
+    w.put(order.data(), order.size());
+    w.put(A->outerIndexPtr(), (size_t)n + 1);
+    w.put(A->innerIndexPtr(), (size_t)nnz);
+    w.put(A->valuePtr(), (size_t)nnz);
+
+    const int nsolves = r.get<int32_t>();
+    std::vector<SolveSpec> specs(nsolves);
+    for (SolveSpec &s : specs)
+    {
+        s.solver = r.get<int32_t>();
+        s.precond = r.get<int32_t>();
+        s.tol = r.get<double>();
+        s.maxIters = r.get<int32_t>();
+        s.refNorm = r.get<double>();
+        s.repeats = std::max(1, (int)r.get<int32_t>());
+    }
+    Eigen::VectorXd b(n);
+    if (nsolves > 0) r.get(b.data(), (size_t)n);
+    std::fclose(fi);
+
+    w.put<int32_t>(nsolves);
+    for (const SolveSpec &s : specs)
+    {
+        sys->eigen.solver = s.solver;
+        sys->eigen.preconditioner = s.precond;
+        sys->eigen.tolerance = s.tol;
+        sys->eigen.maxIters = s.maxIters;
+        Eigen::VectorXd x;
+        int iters = 0;
+        std::vector<double> times;
+        for (int rep = 0; rep < s.repeats; rep++)
+        {
+            const double t0 = omp_get_wtime();
+            iters = SolvePressureSystem(*sys, b, x, nullptr, s.refNorm);
+            times.push_back(omp_get_wtime() - t0);
+        }
+        w.put<int32_t>(iters);
+        w.put<int32_t>((int32_t)times.size());
+        w.put(times.data(), times.size());
+        w.put(x.data(), (size_t)n);
+    }
+    w.put<int32_t>(omp_get_max_threads());
+    std::fclose(fo);
+    return 0;
+}

@@ -1,0 +1,131 @@
+"""Drives ``_ref/ref_harness`` (the reference's own pressure path, compiled from its sources by ``make -C oracle
+ref``). Test infrastructure only: used to write golden fixtures and for bench.py's CPU-baseline leg.
+
+Solver codes follow the reference's PRESSURE block keys (PressureSettings.h:44-45): eigen_solver BICGSTAB=0, CG=1,
+AMG=2 (AMGCL CG + smoothed aggregation, SPAI0); eigen_preconditioner DIAGONAL=0, IDENTITY=1.
+"""
+from __future__ import annotations
+
+import os
+import struct
+import subprocess
+import tempfile
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import HERE, Refinement
+
+HARNESS = os.path.join(HERE, "_ref", "ref_harness")
+BICGSTAB, CG, AMG = 0, 1, 2
+DIAGONAL, IDENTITY = 0, 1
+
+
+def available() -> bool:
+    return os.path.exists(HARNESS)
+
+
+@dataclass
+class SolveSpec:
+    solver: int = CG
+    precond: int = DIAGONAL
+    tol: float = 1e-12
+    max_iters: int = 2000
+    ref_norm: float = 0.0
+    repeats: int = 1
+
+
+@dataclass
+class RefCase:
+    dim: int
+    grid_size: int
+    domain: Sequence[float] = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
+    non_uniform: bool = False
+    refine: Sequence[Sequence[Refinement]] = ((), (), ())
+    types: Optional[np.ndarray] = None        # uint8 (Ny, Nx, Nz)
+    compat: bool = False
+    solves: List[SolveSpec] = field(default_factory=list)
+    rhs: Optional[np.ndarray] = None
+
+    def encode(self) -> bytes:
+        out = [b"NPPECASE", struct.pack("<ii", self.dim, self.grid_size), struct.pack("<6d", *self.domain),
+               struct.pack("<i", int(self.non_uniform)),
+               struct.pack("<3i", *[len(r) for r in self.refine])]
+        for axis in self.refine:
+            for r in axis:
+                out.append(struct.pack("<ddii", r.start, r.end, r.factor, r.mode))
+        if self.types is None:
+            out.append(struct.pack("<i", 0))
+            return b"".join(out)
+        Ny, Nx, Nz = self.types.shape
+        out.append(struct.pack("<iiii", 1, Nx, Ny, Nz))
+        out.append(np.ascontiguousarray(self.types, np.uint8).tobytes())
+        out.append(struct.pack("<i", int(self.compat)))
+        out.append(struct.pack("<i", len(self.solves)))
+        for s in self.solves:
+            out.append(struct.pack("<iidid", s.solver, s.precond, s.tol, s.max_iters, s.ref_norm))
+            out.append(struct.pack("<i", s.repeats))
+        if self.solves:
+            out.append(np.ascontiguousarray(self.rhs, np.float64).tobytes())
+        return b"".join(out)
+
+
+class _Buf:
+    def __init__(self, data: bytes):
+        self.d, self.o = data, 0
+
+    def take(self, fmt):
+        v = struct.unpack_from(fmt, self.d, self.o)
+        self.o += struct.calcsize(fmt)
+        return v
+
+    def arr(self, dtype, n):
+        a = np.frombuffer(self.d, dtype, n, self.o).copy()
+        self.o += a.nbytes
+        return a
+
+
+def run(case: RefCase, timeout: float = 3600.0, threads: Optional[int] = None) -> dict:
+    """Runs the harness on one case and returns its tables, system and solves."""
+    if not available():
+        raise FileNotFoundError(HARNESS + " is not built (make -C oracle ref)")
+    env = dict(os.environ)
+    if threads is not None:
+        env["OMP_NUM_THREADS"] = str(threads)
+    with tempfile.TemporaryDirectory() as td:
+        cin, cout = os.path.join(td, "case.bin"), os.path.join(td, "out.bin")
+        with open(cin, "wb") as f:
+            f.write(case.encode())
+        subprocess.run([HARNESS, cin, cout], check=True, cwd=td, env=env, timeout=timeout,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
+        with open(cout, "rb") as f:
+            data = f.read()
+    b = _Buf(data)
+    assert b.take("8s")[0] == b"NPPEOUT1"
+    Nx, Ny, Nz, nu = b.take("<iiii")
+    dh = b.take("<d")[0]
+    res = {"Nx": Nx, "Ny": Ny, "Nz": Nz, "non_uniform": bool(nu), "dh": dh}
+    axes = [("x", Nx), ("y", Ny)] + ([("z", Nz)] if case.dim == 3 else [])
+    for name, N in axes:
+        res["w" + name] = b.arr(np.float64, N)
+        res["gap" + name] = b.arr(np.float64, N + 1)
+    if case.dim == 2:
+        res["wz"] = np.array([dh])
+    if case.types is None:
+        return res
+    n, nnz, singular = b.take("<iii")
+    res.update(n=n, nnz=nnz, singular=bool(singular))
+    res["order"] = b.arr(np.int32, n)
+    res["rowptr"] = b.arr(np.int32, n + 1)
+    res["col"] = b.arr(np.int32, nnz)
+    res["val"] = b.arr(np.float64, nnz)
+    nsolves = b.take("<i")[0]
+    res["solves"] = []
+    for _ in range(nsolves):
+        iters, nt = b.take("<ii")
+        times = b.arr(np.float64, nt)
+        x = b.arr(np.float64, n)
+        res["solves"].append({"iters": iters, "times": times, "x": x})
+    res["threads"] = b.take("<i")[0]
+    return res
