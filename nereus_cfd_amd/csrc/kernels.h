@@ -64,5 +64,7 @@ hipError_t launch_cg_update(hipStream_t s, const Box &B, const uint8_t *ty, floa
                             const float *q, SolveState *st, double *partials);
 // Inner-solve set-up from the outer residual norm st->sums[1]: thr2 = red^2 * sums[1], counters cleared.
 hipError_t launch_inner_setup(hipStream_t s, SolveState *st, double reduction, int maxIters);
+// FP64 CG set-up: thr2, iteration cap, counters cleared.
+hipError_t launch_state_init(hipStream_t s, SolveState *st, double thr2, int maxIters);
 
 }  // namespace nppe

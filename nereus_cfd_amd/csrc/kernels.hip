@@ -14,6 +14,8 @@
 //   SOLID / INFLOW / OUTFLOW    -> nothing (Neumann)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.h"
 
 namespace nppe
@@ -32,21 +34,14 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-__device__ __forceinline__ double load_relaxed(const double *p)
-{
-    unsigned long long u =
-        __hip_atomic_load(reinterpret_cast<const unsigned long long *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return __longlong_as_double((long long)u);
-}
 
-// Block sum of NV values; the block's partials go to partials[a * nblk + blk]. Returns true in the last block to
-// finish, with the totals (summed over blocks in index order) in tot, identical in every thread of that block.
+// Block sum of NV per-thread values, written by thread 0 to partials[a * nblk + blk] (no atomics: the sums over
+// blocks are taken in index order by k_finalize, so every reduction is deterministic).
 template <int NV>
-__device__ bool reduce_last(double (&v)[NV], double *partials, unsigned *counter, int nblk, int blk, double (&tot)[NV])
+__device__ __forceinline__ void block_sum(double (&v)[NV], double *partials, int nblk, int blk)
 {
     constexpr int kWaves = kBlockThreads / 64;
     __shared__ double sh[NV][kWaves];
-    __shared__ int amLast;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
 #pragma unroll
     for (int a = 0; a < NV; a++)
@@ -65,37 +60,12 @@ __device__ bool reduce_last(double (&v)[NV], double *partials, unsigned *counter
             for (int q = 0; q < kWaves; q++) x += sh[a][q];
             partials[(size_t)a * nblk + blk] = x;
         }
-        __threadfence();
-        const unsigned prev = atomicAdd(counter, 1u);
-        amLast = (prev == (unsigned)nblk - 1u);
     }
-    __syncthreads();
-    if (!amLast) return false;
-    __threadfence();
-#pragma unroll
-    for (int a = 0; a < NV; a++)
-    {
-        double x = 0.0;
-        for (int q = tid; q < nblk; q += kBlockThreads) x += load_relaxed(&partials[(size_t)a * nblk + q]);
-        x = wave_sum(x);
-        __syncthreads();
-        if (lane == 0) sh[a][w] = x;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int a = 0; a < NV; a++)
-    {
-        double x = 0.0;
-#pragma unroll
-        for (int q = 0; q < kWaves; q++) x += sh[a][q];
-        tot[a] = x;
-    }
-    if (tid == 0) *counter = 0u;
-    return true;
 }
 
-// Logical tile of this block after the XCD remap: consecutive hardware workgroup ids go round-robin to the eight
-// XCDs, so XCD x receives ids x, x+8, ...; those are given the x-th contiguous share of the logical tiles.
+// Logical block id after the XCD remap: consecutive hardware workgroup ids go round-robin to the eight XCDs, so
+// XCD x receives ids x, x+8, ...; those are given the x-th contiguous share of the logical blocks, and each
+// logical block a contiguous run of tiles -- one XCD sweeps a contiguous run of planes.
 __device__ __forceinline__ int logical_block()
 {
     const int b = blockIdx.x, nb = gridDim.x;
@@ -103,11 +73,23 @@ __device__ __forceinline__ int logical_block()
     return xcd < extra ? xcd * (per + 1) + r : extra * (per + 1) + (xcd - extra) * per + r;
 }
 
-// Interior cell of this thread; false outside the interior.
-__device__ __forceinline__ bool tile_cell(const Box &B, int lb, int &s, int &m, int &f)
+struct TileRange
 {
-    const int tf = lb % B.tilesF;
-    const int t2 = lb / B.tilesF;
+    int begin, end, lb;
+};
+
+__device__ __forceinline__ TileRange my_tiles(int ntiles)
+{
+    const int lb = logical_block();
+    const int G = gridDim.x;
+    return TileRange{(int)((int64_t)lb * ntiles / G), (int)((int64_t)(lb + 1) * ntiles / G), lb};
+}
+
+// Interior cell of this thread in tile t; false outside the interior.
+__device__ __forceinline__ bool tile_cell(const Box &B, int t, int &s, int &m, int &f)
+{
+    const int tf = t % B.tilesF;
+    const int t2 = t / B.tilesF;
     const int tm = t2 % B.tilesM;
     const int tp = t2 / B.tilesM;
     f = 1 + tf * kBlockFast + (threadIdx.x & (kBlockFast - 1));
@@ -278,62 +260,139 @@ __device__ __forceinline__ double row64(const Box &B, const Op64 &o, int64_t idx
     return t;
 }
 
+// ---- scalar finalisation -----------------------------------------------------------------------------------
+
+// What k_finalize does with the block partials of the kernel before it.
+enum FinOp
+{
+    FIN_SUM0 = 0,     // sums[0] = t0
+    FIN_SUM1,         // sums[1] = t0
+    FIN_SUM2,         // sums[2] = t0
+    FIN_RZ,           // beta = t0 / rz (0 on the first direction); rz = t0
+    FIN_PQ,           // alpha = rz / t0; first = 0
+    FIN_RR,           // rr = t0; iters++; done when rr < thr2 (1) or at the cap (2)
+    FIN_CG64_INIT,    // rz = t0; iters = 0
+    FIN_CG64_PQ,      // alpha = rz / t0
+    FIN_CG64_UPDATE   // rr = t0; converged -> done = 1 (not counted, as Eigen); else beta = t1 / rz, rz = t1, iters++
+};
+
+__device__ __forceinline__ bool fin_in_loop(int op) { return op >= FIN_RZ; }
+
+// One block: sums the partials of each value over the nblk blocks in index order, then applies op.
+__global__ __launch_bounds__(kBlockThreads) void k_finalize(SolveState *st, const double *__restrict__ partials,
+                                                            int nblk, int nv, int op)
+{
+    if (fin_in_loop(op) && st->done) return;
+    constexpr int kWaves = kBlockThreads / 64;
+    __shared__ double sh[2][kWaves];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (int a = 0; a < nv; a++)
+    {
+        double x = 0.0;
+        for (int q = tid; q < nblk; q += kBlockThreads) x += partials[(size_t)a * nblk + q];
+        x = wave_sum(x);
+        if (lane == 0) sh[a][w] = x;
+    }
+    __syncthreads();
+    if (tid != 0) return;
+    double t[2] = {0.0, 0.0};
+    for (int a = 0; a < nv; a++)
+        for (int q = 0; q < kWaves; q++) t[a] += sh[a][q];
+    switch (op)
+    {
+    case FIN_SUM0: st->sums[0] = t[0]; break;
+    case FIN_SUM1: st->sums[1] = t[0]; break;
+    case FIN_SUM2: st->sums[2] = t[0]; break;
+    case FIN_RZ:
+        st->beta = (st->first || st->rz == 0.0) ? 0.0 : t[0] / st->rz;
+        st->rz = t[0];
+        break;
+    case FIN_PQ:
+        st->alpha = t[0] != 0.0 ? st->rz / t[0] : 0.0;
+        st->first = 0;
+        break;
+    case FIN_RR:
+        st->rr = t[0];
+        st->iters += 1;
+        if (t[0] < st->thr2) st->done = 1;
+        else if (st->iters >= st->maxIters) st->done = 2;
+        break;
+    case FIN_CG64_INIT:
+        st->rz = t[0];
+        st->iters = 0;
+        break;
+    case FIN_CG64_PQ: st->alpha = t[0] != 0.0 ? st->rz / t[0] : 0.0; break;
+    case FIN_CG64_UPDATE:
+        st->rr = t[0];
+        if (t[0] < st->thr2)
+            st->done = 1;   // Eigen breaks before counting this pass (ConjugateGradient.h:78-79)
+        else
+        {
+            st->beta = st->rz != 0.0 ? t[1] / st->rz : 0.0;
+            st->rz = t[1];
+            st->iters += 1;
+            if (st->iters >= st->maxIters) st->done = 2;
+        }
+        break;
+    default: break;
+    }
+}
+
 // ---- FP64 kernels --------------------------------------------------------------------------------------------
 
 __global__ __launch_bounds__(kBlockThreads) void k_scatter64(int64_t n, const int32_t *__restrict__ order,
                                                              const double *__restrict__ src, double *__restrict__ box,
-                                                             SolveState *st, int sumSlot, double *partials)
+                                                             double *partials, int withSum)
 {
-    const int64_t q = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x;
-    double v = 0.0;
-    if (q < n)
+    double v2 = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x; q < n; q += (int64_t)gridDim.x * kBlockThreads)
     {
-        v = src ? src[q] : 0.0;
+        const double v = src ? src[q] : 0.0;
         box[order[q]] = v;
+        v2 += v * v;
     }
-    if (sumSlot < 0) return;
-    double a[1] = {v * v}, tot[1];
-    if (reduce_last<1>(a, partials, &st->counters[0], gridDim.x, blockIdx.x, tot) && threadIdx.x == 0)
-        st->sums[sumSlot] = tot[0];
+    if (!withSum) return;
+    double a[1] = {v2};
+    block_sum<1>(a, partials, gridDim.x, blockIdx.x);
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_gather64(int64_t n, const int32_t *__restrict__ order,
                                                             const double *__restrict__ box, double *__restrict__ dst,
                                                             const SolveState *st, int meanSlot)
 {
-    const int64_t q = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x;
-    if (q >= n) return;
     const double mean = meanSlot >= 0 ? st->sums[meanSlot] / (double)n : 0.0;
-    dst[q] = box[order[q]] - mean;
+    for (int64_t q = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x; q < n; q += (int64_t)gridDim.x * kBlockThreads)
+        dst[q] = box[order[q]] - mean;
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_sum64(int64_t n, const int32_t *__restrict__ order,
-                                                         const double *__restrict__ box, SolveState *st, int slot,
-                                                         double *partials)
+                                                         const double *__restrict__ box, double *partials)
 {
-    const int64_t q = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x;
-    double a[1] = {q < n ? box[order[q]] : 0.0}, tot[1];
-    if (reduce_last<1>(a, partials, &st->counters[1], gridDim.x, blockIdx.x, tot) && threadIdx.x == 0)
-        st->sums[slot] = tot[0];
+    double acc = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x; q < n; q += (int64_t)gridDim.x * kBlockThreads)
+        acc += box[order[q]];
+    double a[1] = {acc};
+    block_sum<1>(a, partials, gridDim.x, blockIdx.x);
 }
 
 // One refinement step: xn = x + e (e may be null), written to xout, and r = b - A xn in FP64 with the reference's
 // exact coefficients (neighbour values xn formed on the fly, so x and xout must differ), stored as FP32 in r32
-// and/or FP64 in r64; st->sums[1] = ||r||^2.
+// and/or FP64 in r64; partial sums of r^2.
 __global__ __launch_bounds__(kBlockThreads) void k_update_residual64(Box B, Tables64 T, const uint8_t *__restrict__ ty,
                                                                      const double *__restrict__ x,
                                                                      const float *__restrict__ e,
                                                                      double *__restrict__ xout,
                                                                      const double *__restrict__ b,
                                                                      float *__restrict__ r32, double *__restrict__ r64,
-                                                                     SolveState *st, double *partials)
+                                                                     double *partials)
 {
 #pragma clang fp contract(off)
-    const int lb = logical_block();
-    int s, m, f;
+    const TileRange tr = my_tiles(B.blocks());
     double rr = 0.0;
-    if (tile_cell(B, lb, s, m, f))
+    for (int t = tr.begin; t < tr.end; t++)
     {
+        int s, m, f;
+        if (!tile_cell(B, t, s, m, f)) continue;
         const int64_t idx = B.at(s, m, f);
         double r = 0.0;
         if (ty[idx] == kFluid)
@@ -353,26 +412,29 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64(Box B, Tabl
         }
         if (r32) r32[idx] = (float)r;
         if (r64) r64[idx] = r;
-        rr = r * r;
+        rr += r * r;
     }
-    double a[1] = {rr}, tot[1];
-    if (reduce_last<1>(a, partials, &st->counters[2], gridDim.x, lb, tot) && threadIdx.x == 0) st->sums[1] = tot[0];
+    double a[1] = {rr};
+    block_sum<1>(a, partials, gridDim.x, tr.lb);
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_apply64(Box B, Tables64 T, const uint8_t *__restrict__ ty,
                                                            const double *__restrict__ x, double *__restrict__ y)
 {
-    const int lb = logical_block();
-    int s, m, f;
-    if (!tile_cell(B, lb, s, m, f)) return;
-    const int64_t idx = B.at(s, m, f);
-    double v = 0.0;
-    if (ty[idx] == kFluid)
+    const TileRange tr = my_tiles(B.blocks());
+    for (int t = tr.begin; t < tr.end; t++)
     {
-        const Op64 o = op64(B, T, ty, s, m, f, idx);
-        v = row64(B, o, idx, [&](int64_t d) { return x[idx + d]; });
+        int s, m, f;
+        if (!tile_cell(B, t, s, m, f)) continue;
+        const int64_t idx = B.at(s, m, f);
+        double v = 0.0;
+        if (ty[idx] == kFluid)
+        {
+            const Op64 o = op64(B, T, ty, s, m, f, idx);
+            v = row64(B, o, idx, [&](int64_t d) { return x[idx + d]; });
+        }
+        y[idx] = v;
     }
-    y[idx] = v;
 }
 
 // Jacobi: Eigen's DiagonalPreconditioner takes 1/value of the diagonal entry, or 1 where the row has none.
@@ -380,94 +442,78 @@ __device__ __forceinline__ double inv_diag64(const Op64 &o) { return o.diag != 0
 
 __global__ __launch_bounds__(kBlockThreads) void k_cg64_init(Box B, Tables64 T, const uint8_t *__restrict__ ty,
                                                              const double *__restrict__ r, double *__restrict__ p,
-                                                             SolveState *st, double *partials)
+                                                             double *partials)
 {
-    const int lb = logical_block();
-    int s, m, f;
+    const TileRange tr = my_tiles(B.blocks());
     double rz = 0.0;
-    if (tile_cell(B, lb, s, m, f))
+    for (int t = tr.begin; t < tr.end; t++)
     {
+        int s, m, f;
+        if (!tile_cell(B, t, s, m, f)) continue;
         const int64_t idx = B.at(s, m, f);
         double pv = 0.0;
         if (ty[idx] == kFluid)
         {
             const Op64 o = op64(B, T, ty, s, m, f, idx);
             pv = inv_diag64(o) * r[idx];
-            rz = r[idx] * pv;
+            rz += r[idx] * pv;
         }
         p[idx] = pv;
     }
-    double a[1] = {rz}, tot[1];
-    if (reduce_last<1>(a, partials, &st->counters[3], gridDim.x, lb, tot) && threadIdx.x == 0)
-    {
-        st->rz = tot[0];
-        st->iters = 0;
-    }
+    double a[1] = {rz};
+    block_sum<1>(a, partials, gridDim.x, tr.lb);
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_cg64_spmv(Box B, Tables64 T, const uint8_t *__restrict__ ty,
                                                              const double *__restrict__ p, double *__restrict__ q,
-                                                             SolveState *st, double *partials)
+                                                             const SolveState *st, double *partials)
 {
     if (st->done) return;
-    const int lb = logical_block();
-    int s, m, f;
+    const TileRange tr = my_tiles(B.blocks());
     double pq = 0.0;
-    if (tile_cell(B, lb, s, m, f))
+    for (int t = tr.begin; t < tr.end; t++)
     {
+        int s, m, f;
+        if (!tile_cell(B, t, s, m, f)) continue;
         const int64_t idx = B.at(s, m, f);
         double v = 0.0;
         if (ty[idx] == kFluid)
         {
             const Op64 o = op64(B, T, ty, s, m, f, idx);
             v = row64(B, o, idx, [&](int64_t d) { return p[idx + d]; });
-            pq = p[idx] * v;
+            pq += p[idx] * v;
         }
         q[idx] = v;
     }
-    double a[1] = {pq}, tot[1];
-    if (reduce_last<1>(a, partials, &st->counters[4], gridDim.x, lb, tot) && threadIdx.x == 0)
-        st->alpha = tot[0] != 0.0 ? st->rz / tot[0] : 0.0;
+    double a[1] = {pq};
+    block_sum<1>(a, partials, gridDim.x, tr.lb);
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_cg64_update(Box B, Tables64 T, const uint8_t *__restrict__ ty,
                                                                double *__restrict__ x, double *__restrict__ r,
                                                                const double *__restrict__ p,
-                                                               const double *__restrict__ q, SolveState *st,
+                                                               const double *__restrict__ q, const SolveState *st,
                                                                double *partials)
 {
     if (st->done) return;
     const double alpha = st->alpha;
-    const int lb = logical_block();
-    int s, m, f;
+    const TileRange tr = my_tiles(B.blocks());
     double rr = 0.0, rz = 0.0;
-    if (tile_cell(B, lb, s, m, f))
+    for (int t = tr.begin; t < tr.end; t++)
     {
+        int s, m, f;
+        if (!tile_cell(B, t, s, m, f)) continue;
         const int64_t idx = B.at(s, m, f);
-        if (ty[idx] == kFluid)
-        {
-            x[idx] += alpha * p[idx];
-            const double rv = r[idx] - alpha * q[idx];
-            r[idx] = rv;
-            const Op64 o = op64(B, T, ty, s, m, f, idx);
-            rr = rv * rv;
-            rz = rv * (inv_diag64(o) * rv);
-        }
+        if (ty[idx] != kFluid) continue;
+        x[idx] += alpha * p[idx];
+        const double rv = r[idx] - alpha * q[idx];
+        r[idx] = rv;
+        const Op64 o = op64(B, T, ty, s, m, f, idx);
+        rr += rv * rv;
+        rz += rv * (inv_diag64(o) * rv);
     }
-    double a[2] = {rr, rz}, tot[2];
-    if (reduce_last<2>(a, partials, &st->counters[5], gridDim.x, lb, tot) && threadIdx.x == 0)
-    {
-        st->rr = tot[0];
-        if (tot[0] < st->thr2)
-            st->done = 1;   // Eigen breaks before counting this pass (ConjugateGradient.h:78-79)
-        else
-        {
-            st->beta = st->rz != 0.0 ? tot[1] / st->rz : 0.0;
-            st->rz = tot[1];
-            st->iters += 1;
-            if (st->iters >= st->maxIters) st->done = 2;
-        }
-    }
+    double a[2] = {rr, rz};
+    block_sum<2>(a, partials, gridDim.x, tr.lb);
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_cg64_direction(Box B, Tables64 T, const uint8_t *__restrict__ ty,
@@ -476,13 +522,16 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg64_direction(Box B, Tables6
 {
     if (st->done) return;
     const double beta = st->beta;
-    const int lb = logical_block();
-    int s, m, f;
-    if (!tile_cell(B, lb, s, m, f)) return;
-    const int64_t idx = B.at(s, m, f);
-    if (ty[idx] != kFluid) return;
-    const Op64 o = op64(B, T, ty, s, m, f, idx);
-    p[idx] = inv_diag64(o) * r[idx] + beta * p[idx];
+    const TileRange tr = my_tiles(B.blocks());
+    for (int t = tr.begin; t < tr.end; t++)
+    {
+        int s, m, f;
+        if (!tile_cell(B, t, s, m, f)) continue;
+        const int64_t idx = B.at(s, m, f);
+        if (ty[idx] != kFluid) continue;
+        const Op64 o = op64(B, T, ty, s, m, f, idx);
+        p[idx] = inv_diag64(o) * r[idx] + beta * p[idx];
+    }
 }
 
 // ---- FP32 multigrid kernels ----------------------------------------------------------------------------------
@@ -494,15 +543,18 @@ __global__ __launch_bounds__(kBlockThreads) void k_smooth(Box B, Tables32 T, con
                                                           int color, int zero, const SolveState *st)
 {
     if (st->done) return;
-    const int lb = logical_block();
-    int s, m, f;
-    if (!tile_cell(B, lb, s, m, f)) return;
-    if (((s + m + f) & 1) != color) return;
-    const int64_t idx = B.at(s, m, f);
-    if (ty[idx] != kFluid) return;
-    const Op32 o = op32(B, T, ty, s, m, f, idx);
-    const float acc = zero ? 0.f : offsum32(B, o, x, idx);
-    x[idx] = o.diag != 0.f ? (b[idx] - acc) / o.diag : 0.f;
+    const TileRange tr = my_tiles(B.blocks());
+    for (int t = tr.begin; t < tr.end; t++)
+    {
+        int s, m, f;
+        if (!tile_cell(B, t, s, m, f)) continue;
+        if (((s + m + f) & 1) != color) continue;
+        const int64_t idx = B.at(s, m, f);
+        if (ty[idx] != kFluid) continue;
+        const Op32 o = op32(B, T, ty, s, m, f, idx);
+        const float acc = zero ? 0.f : offsum32(B, o, x, idx);
+        x[idx] = o.diag != 0.f ? (b[idx] - acc) / o.diag : 0.f;
+    }
 }
 
 // Coarse cell (S, M, F) gathers the residual b - A x of its fine children (fine indices 2C-1, 2C along each
@@ -514,22 +566,25 @@ __global__ __launch_bounds__(kBlockThreads) void k_residual_restrict(Box Bf, Tab
                                                                      float *__restrict__ bc, const SolveState *st)
 {
     if (st->done) return;
-    const int lb = logical_block();
-    int S, M, F;
-    if (!tile_cell(Bc, lb, S, M, F)) return;
-    float sum = 0.f;
-    const int s0 = Bf.threeD ? 2 * S - 1 : 0, s1 = Bf.threeD ? min(2 * S, Bf.ns - 2) : 0;
-    const int m1 = min(2 * M, Bf.nm - 2), f1 = min(2 * F, Bf.nf - 2);
-    for (int s = s0; s <= s1; s++)
-        for (int m = 2 * M - 1; m <= m1; m++)
-            for (int f = 2 * F - 1; f <= f1; f++)
-            {
-                const int64_t idx = Bf.at(s, m, f);
-                if (tyf[idx] != kFluid) continue;
-                const Op32 o = op32(Bf, Tf, tyf, s, m, f, idx);
-                sum += b[idx] - (offsum32(Bf, o, x, idx) + o.diag * x[idx]);
-            }
-    bc[Bc.at(S, M, F)] = sum;
+    const TileRange tr = my_tiles(Bc.blocks());
+    for (int t = tr.begin; t < tr.end; t++)
+    {
+        int S, M, F;
+        if (!tile_cell(Bc, t, S, M, F)) continue;
+        float sum = 0.f;
+        const int s0 = Bf.threeD ? 2 * S - 1 : 0, s1 = Bf.threeD ? min(2 * S, Bf.ns - 2) : 0;
+        const int m1 = min(2 * M, Bf.nm - 2), f1 = min(2 * F, Bf.nf - 2);
+        for (int s = s0; s <= s1; s++)
+            for (int m = 2 * M - 1; m <= m1; m++)
+                for (int f = 2 * F - 1; f <= f1; f++)
+                {
+                    const int64_t idx = Bf.at(s, m, f);
+                    if (tyf[idx] != kFluid) continue;
+                    const Op32 o = op32(Bf, Tf, tyf, s, m, f, idx);
+                    sum += b[idx] - (offsum32(Bf, o, x, idx) + o.diag * x[idx]);
+                }
+        bc[Bc.at(S, M, F)] = sum;
+    }
 }
 
 // x += P xc: every fine FLUID cell takes its parent's correction when the parent is a coarse unknown.
@@ -539,14 +594,17 @@ __global__ __launch_bounds__(kBlockThreads) void k_prolong(Box Bf, const uint8_t
                                                            const float *__restrict__ xc, const SolveState *st)
 {
     if (st->done) return;
-    const int lb = logical_block();
-    int s, m, f;
-    if (!tile_cell(Bf, lb, s, m, f)) return;
-    const int64_t idx = Bf.at(s, m, f);
-    if (tyf[idx] != kFluid) return;
-    const int64_t pidx = Bc.at(Bf.threeD ? (s + 1) >> 1 : 0, (m + 1) >> 1, (f + 1) >> 1);
-    const float v = xc[pidx];
-    x[idx] += tyc[pidx] == kFluid ? v : 0.f;
+    const TileRange tr = my_tiles(Bf.blocks());
+    for (int t = tr.begin; t < tr.end; t++)
+    {
+        int s, m, f;
+        if (!tile_cell(Bf, t, s, m, f)) continue;
+        const int64_t idx = Bf.at(s, m, f);
+        if (tyf[idx] != kFluid) continue;
+        const int64_t pidx = Bc.at(Bf.threeD ? (s + 1) >> 1 : 0, (m + 1) >> 1, (f + 1) >> 1);
+        const float v = xc[pidx];
+        x[idx] += tyc[pidx] == kFluid ? v : 0.f;
+    }
 }
 
 // Coarsest level: dense FP64 pseudo-inverse times the gathered right-hand side. One wavefront per row.
@@ -573,36 +631,34 @@ __global__ __launch_bounds__(kBlockThreads) void k_coarse_solve(int n, const int
 
 __global__ __launch_bounds__(kBlockThreads) void k_cg_rz(Box B, const uint8_t *__restrict__ ty,
                                                          const float *__restrict__ r, const float *__restrict__ z,
-                                                         SolveState *st, double *partials)
+                                                         const SolveState *st, double *partials)
 {
     if (st->done) return;
-    const int lb = logical_block();
-    int s, m, f;
+    const TileRange tr = my_tiles(B.blocks());
     double v = 0.0;
-    if (tile_cell(B, lb, s, m, f))
+    for (int t = tr.begin; t < tr.end; t++)
     {
+        int s, m, f;
+        if (!tile_cell(B, t, s, m, f)) continue;
         const int64_t idx = B.at(s, m, f);
-        if (ty[idx] == kFluid) v = (double)r[idx] * (double)z[idx];
+        if (ty[idx] == kFluid) v += (double)r[idx] * (double)z[idx];
     }
-    double a[1] = {v}, tot[1];
-    if (reduce_last<1>(a, partials, &st->counters[6], gridDim.x, lb, tot) && threadIdx.x == 0)
-    {
-        st->beta = (st->first || st->rz == 0.0) ? 0.0 : tot[0] / st->rz;
-        st->rz = tot[0];
-    }
+    double a[1] = {v};
+    block_sum<1>(a, partials, gridDim.x, tr.lb);
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_cg_direction(Box B, const uint8_t *__restrict__ ty,
                                                                 const float *__restrict__ z, float *__restrict__ p,
-                                                                SolveState *st)
+                                                                const SolveState *st)
 {
     if (st->done) return;
     const int first = st->first;
     const float beta = (float)st->beta;
-    const int lb = logical_block();
-    int s, m, f;
-    if (tile_cell(B, lb, s, m, f))
+    const TileRange tr = my_tiles(B.blocks());
+    for (int t = tr.begin; t < tr.end; t++)
     {
+        int s, m, f;
+        if (!tile_cell(B, t, s, m, f)) continue;
         const int64_t idx = B.at(s, m, f);
         if (ty[idx] == kFluid) p[idx] = first ? z[idx] : z[idx] + beta * p[idx];
     }
@@ -610,63 +666,51 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_direction(Box B, const uin
 
 __global__ __launch_bounds__(kBlockThreads) void k_cg_spmv(Box B, Tables32 T, const uint8_t *__restrict__ ty,
                                                            const float *__restrict__ p, float *__restrict__ q,
-                                                           SolveState *st, double *partials)
+                                                           const SolveState *st, double *partials)
 {
     if (st->done) return;
-    const int lb = logical_block();
-    int s, m, f;
+    const TileRange tr = my_tiles(B.blocks());
     double pq = 0.0;
-    if (tile_cell(B, lb, s, m, f))
+    for (int t = tr.begin; t < tr.end; t++)
     {
+        int s, m, f;
+        if (!tile_cell(B, t, s, m, f)) continue;
         const int64_t idx = B.at(s, m, f);
-        if (ty[idx] == kFluid)
-        {
-            const Op32 o = op32(B, T, ty, s, m, f, idx);
-            const float pc = p[idx];
-            const float v = offsum32(B, o, p, idx) + o.diag * pc;
-            q[idx] = v;
-            pq = (double)pc * (double)v;
-        }
+        if (ty[idx] != kFluid) continue;
+        const Op32 o = op32(B, T, ty, s, m, f, idx);
+        const float pc = p[idx];
+        const float v = offsum32(B, o, p, idx) + o.diag * pc;
+        q[idx] = v;
+        pq += (double)pc * (double)v;
     }
-    double a[1] = {pq}, tot[1];
-    if (reduce_last<1>(a, partials, &st->counters[7], gridDim.x, lb, tot) && threadIdx.x == 0)
-    {
-        st->alpha = tot[0] != 0.0 ? st->rz / tot[0] : 0.0;
-        st->first = 0;
-    }
+    double a[1] = {pq};
+    block_sum<1>(a, partials, gridDim.x, tr.lb);
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_cg_update(Box B, const uint8_t *__restrict__ ty,
                                                              float *__restrict__ e, float *__restrict__ r,
                                                              const float *__restrict__ p, const float *__restrict__ q,
-                                                             SolveState *st, double *partials)
+                                                             const SolveState *st, double *partials)
 {
     if (st->done) return;
     const int firstIter = st->iters == 0;
     const float alpha = (float)st->alpha;
-    const int lb = logical_block();
-    int s, m, f;
+    const TileRange tr = my_tiles(B.blocks());
     double rr = 0.0;
-    if (tile_cell(B, lb, s, m, f))
+    for (int t = tr.begin; t < tr.end; t++)
     {
+        int s, m, f;
+        if (!tile_cell(B, t, s, m, f)) continue;
         const int64_t idx = B.at(s, m, f);
-        if (ty[idx] == kFluid)
-        {
-            const float pv = p[idx];
-            e[idx] = firstIter ? alpha * pv : e[idx] + alpha * pv;
-            const float rv = r[idx] - alpha * q[idx];
-            r[idx] = rv;
-            rr = (double)rv * (double)rv;
-        }
+        if (ty[idx] != kFluid) continue;
+        const float pv = p[idx];
+        e[idx] = firstIter ? alpha * pv : e[idx] + alpha * pv;
+        const float rv = r[idx] - alpha * q[idx];
+        r[idx] = rv;
+        rr += (double)rv * (double)rv;
     }
-    double a[1] = {rr}, tot[1];
-    if (reduce_last<1>(a, partials, &st->counters[0], gridDim.x, lb, tot) && threadIdx.x == 0)
-    {
-        st->rr = tot[0];
-        st->iters += 1;
-        if (tot[0] < st->thr2) st->done = 1;
-        else if (st->iters >= st->maxIters) st->done = 2;
-    }
+    double a[1] = {rr};
+    block_sum<1>(a, partials, gridDim.x, tr.lb);
 }
 
 __global__ void k_inner_setup(SolveState *st, double reduction, int maxIters)
@@ -680,8 +724,30 @@ __global__ void k_inner_setup(SolveState *st, double reduction, int maxIters)
     st->done = (r2 == 0.0 || maxIters <= 0) ? 1 : 0;
 }
 
-inline dim3 grid_of(const Box &B) { return dim3((unsigned)B.blocks()); }
-inline dim3 grid_1d(int64_t n) { return dim3((unsigned)((n + kBlockThreads - 1) / kBlockThreads)); }
+__global__ void k_state_init(SolveState *st, double thr2, int maxIters)
+{
+    st->thr2 = thr2;
+    st->iters = 0;
+    st->first = 1;
+    st->maxIters = maxIters;
+    st->done = maxIters <= 0 ? 2 : 0;
+}
+
+inline int grid_of(const Box &B) { return std::min(B.blocks(), kMaxGrid); }
+inline int grid_1d(int64_t n) { return (int)std::min<int64_t>((n + kBlockThreads - 1) / kBlockThreads, kMaxGrid); }
+
+hipError_t finalize(hipStream_t s, SolveState *st, const double *partials, int nblk, int nv, FinOp op)
+{
+    k_finalize<<<1, kBlockThreads, 0, s>>>(st, partials, nblk, nv, (int)op);
+    return hipGetLastError();
+}
+
+#define LAUNCH_OK()                                    \
+    do                                                 \
+    {                                                  \
+        hipError_t e_ = hipGetLastError();             \
+        if (e_ != hipSuccess) return e_;               \
+    } while (0)
 
 }  // namespace
 
@@ -689,8 +755,11 @@ hipError_t launch_scatter64(hipStream_t s, int64_t n, const int32_t *order, cons
                             SolveState *st, int sumSlot, double *partials)
 {
     if (n <= 0) return hipSuccess;
-    k_scatter64<<<grid_1d(n), kBlockThreads, 0, s>>>(n, order, src, box, st, sumSlot, partials);
-    return hipGetLastError();
+    const int G = grid_1d(n);
+    k_scatter64<<<G, kBlockThreads, 0, s>>>(n, order, src, box, partials, sumSlot >= 0);
+    LAUNCH_OK();
+    if (sumSlot < 0) return hipSuccess;
+    return finalize(s, st, partials, G, 1, sumSlot == 0 ? FIN_SUM0 : (sumSlot == 1 ? FIN_SUM1 : FIN_SUM2));
 }
 
 hipError_t launch_gather64(hipStream_t s, int64_t n, const int32_t *order, const double *box, double *dst,
@@ -705,16 +774,20 @@ hipError_t launch_sum64(hipStream_t s, int64_t n, const int32_t *order, const do
                         double *partials)
 {
     if (n <= 0) return hipSuccess;
-    k_sum64<<<grid_1d(n), kBlockThreads, 0, s>>>(n, order, box, st, slot, partials);
-    return hipGetLastError();
+    const int G = grid_1d(n);
+    k_sum64<<<G, kBlockThreads, 0, s>>>(n, order, box, partials);
+    LAUNCH_OK();
+    return finalize(s, st, partials, G, 1, slot == 0 ? FIN_SUM0 : (slot == 1 ? FIN_SUM1 : FIN_SUM2));
 }
 
 hipError_t launch_update_residual64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty,
                                     const double *x, const float *e, double *xout, const double *b, float *r32,
                                     double *r64, SolveState *st, double *partials)
 {
-    k_update_residual64<<<grid_of(B), kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, st, partials);
-    return hipGetLastError();
+    const int G = grid_of(B);
+    k_update_residual64<<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, partials);
+    LAUNCH_OK();
+    return finalize(s, st, partials, G, 1, FIN_SUM1);
 }
 
 hipError_t launch_apply64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const double *x,
@@ -727,22 +800,28 @@ hipError_t launch_apply64(hipStream_t s, const Box &B, const Tables64 &T, const 
 hipError_t launch_cg64_init(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const double *r,
                             double *p, SolveState *st, double *partials)
 {
-    k_cg64_init<<<grid_of(B), kBlockThreads, 0, s>>>(B, T, ty, r, p, st, partials);
-    return hipGetLastError();
+    const int G = grid_of(B);
+    k_cg64_init<<<G, kBlockThreads, 0, s>>>(B, T, ty, r, p, partials);
+    LAUNCH_OK();
+    return finalize(s, st, partials, G, 1, FIN_CG64_INIT);
 }
 
 hipError_t launch_cg64_spmv(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const double *p,
                             double *q, SolveState *st, double *partials)
 {
-    k_cg64_spmv<<<grid_of(B), kBlockThreads, 0, s>>>(B, T, ty, p, q, st, partials);
-    return hipGetLastError();
+    const int G = grid_of(B);
+    k_cg64_spmv<<<G, kBlockThreads, 0, s>>>(B, T, ty, p, q, st, partials);
+    LAUNCH_OK();
+    return finalize(s, st, partials, G, 1, FIN_CG64_PQ);
 }
 
 hipError_t launch_cg64_update(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, double *x,
                               double *r, const double *p, const double *q, SolveState *st, double *partials)
 {
-    k_cg64_update<<<grid_of(B), kBlockThreads, 0, s>>>(B, T, ty, x, r, p, q, st, partials);
-    return hipGetLastError();
+    const int G = grid_of(B);
+    k_cg64_update<<<G, kBlockThreads, 0, s>>>(B, T, ty, x, r, p, q, st, partials);
+    LAUNCH_OK();
+    return finalize(s, st, partials, G, 2, FIN_CG64_UPDATE);
 }
 
 hipError_t launch_cg64_direction(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty,
@@ -779,16 +858,17 @@ hipError_t launch_coarse_solve(hipStream_t s, int n, const int32_t *order, const
     if (n <= 0) return hipSuccess;
     if (n > kMaxDenseCoarse) return hipErrorInvalidValue;
     const int rowsPerBlock = kBlockThreads / 64;
-    k_coarse_solve<<<dim3((unsigned)((n + rowsPerBlock - 1) / rowsPerBlock)), kBlockThreads, 0, s>>>(n, order, M, b,
-                                                                                                     x, st);
+    k_coarse_solve<<<(n + rowsPerBlock - 1) / rowsPerBlock, kBlockThreads, 0, s>>>(n, order, M, b, x, st);
     return hipGetLastError();
 }
 
 hipError_t launch_cg_rz(hipStream_t s, const Box &B, const uint8_t *ty, const float *r, const float *z, SolveState *st,
                         double *partials)
 {
-    k_cg_rz<<<grid_of(B), kBlockThreads, 0, s>>>(B, ty, r, z, st, partials);
-    return hipGetLastError();
+    const int G = grid_of(B);
+    k_cg_rz<<<G, kBlockThreads, 0, s>>>(B, ty, r, z, st, partials);
+    LAUNCH_OK();
+    return finalize(s, st, partials, G, 1, FIN_RZ);
 }
 
 hipError_t launch_cg_direction(hipStream_t s, const Box &B, const uint8_t *ty, const float *z, float *p,
@@ -801,20 +881,30 @@ hipError_t launch_cg_direction(hipStream_t s, const Box &B, const uint8_t *ty, c
 hipError_t launch_cg_spmv(hipStream_t s, const Box &B, const Tables32 &T, const uint8_t *ty, const float *p, float *q,
                           SolveState *st, double *partials)
 {
-    k_cg_spmv<<<grid_of(B), kBlockThreads, 0, s>>>(B, T, ty, p, q, st, partials);
-    return hipGetLastError();
+    const int G = grid_of(B);
+    k_cg_spmv<<<G, kBlockThreads, 0, s>>>(B, T, ty, p, q, st, partials);
+    LAUNCH_OK();
+    return finalize(s, st, partials, G, 1, FIN_PQ);
 }
 
 hipError_t launch_cg_update(hipStream_t s, const Box &B, const uint8_t *ty, float *e, float *r, const float *p,
                             const float *q, SolveState *st, double *partials)
 {
-    k_cg_update<<<grid_of(B), kBlockThreads, 0, s>>>(B, ty, e, r, p, q, st, partials);
-    return hipGetLastError();
+    const int G = grid_of(B);
+    k_cg_update<<<G, kBlockThreads, 0, s>>>(B, ty, e, r, p, q, st, partials);
+    LAUNCH_OK();
+    return finalize(s, st, partials, G, 1, FIN_RR);
 }
 
 hipError_t launch_inner_setup(hipStream_t s, SolveState *st, double reduction, int maxIters)
 {
     k_inner_setup<<<1, 1, 0, s>>>(st, reduction, maxIters);
+    return hipGetLastError();
+}
+
+hipError_t launch_state_init(hipStream_t s, SolveState *st, double thr2, int maxIters)
+{
+    k_state_init<<<1, 1, 0, s>>>(st, thr2, maxIters);
     return hipGetLastError();
 }
 

@@ -14,6 +14,7 @@ constexpr int kBlockFast = 64;  // tile extent along the fast axis: one wavefron
 constexpr int kBlockMid = 4;    // tile extent along the mid axis: four wavefronts per block
 constexpr int kBlockThreads = kBlockFast * kBlockMid;
 constexpr int kMaxDenseCoarse = 1024;  // largest coarsest level solved by the dense pseudo-inverse
+constexpr int kMaxGrid = 2048;         // blocks per stencil launch: 8 per CU on 256 CUs; each loops over its tiles
 
 // Geometry of one level's box.
 struct Box
@@ -62,7 +63,6 @@ struct SolveState
     int iters;        // inner iterations taken
     int maxIters;     // inner cap
     int first;        // next direction update is the first of the solve (beta = 0)
-    unsigned counters[8];
 };
 
 }  // namespace nppe

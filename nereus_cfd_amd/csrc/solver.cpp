@@ -504,9 +504,7 @@ int build(nppe_solver *h, const nppe_grid *g)
     HIP_TRY(h, dalloc(h, &h->dCompB, nn), "allocate staging b");
     HIP_TRY(h, dalloc(h, &h->dCompX, nn), "allocate staging x");
     HIP_TRY(h, dalloc(h, &h->dCompG, nn), "allocate staging guess");
-    int64_t maxBlocks = (h->n + kBlockThreads - 1) / kBlockThreads;
-    for (const DevLevel &L : h->levels) maxBlocks = std::max<int64_t>(maxBlocks, L.box.blocks());
-    h->partialsCap = maxBlocks * 4;
+    h->partialsCap = (int64_t)kMaxGrid * 4;   // every reduction launch has at most kMaxGrid blocks, <= 2 values
     HIP_TRY(h, dalloc(h, &h->dPartials, (size_t)h->partialsCap), "allocate partials");
     HIP_TRY(h, dalloc(h, &h->dState, 1), "allocate state");
     HIP_TRY(h, hipStreamSynchronize(h->stream), "commit");
@@ -720,18 +718,7 @@ int solve_cg64(nppe_solver *h, const double *dB, double *dX, const double *dGues
     const double thr = std::max(tol * tol * bn2, 2.2250738585072014e-308);
     if (!skip && !(rr < thr))
     {
-        SolveState init{};
-        HIP_TRY(h, hipMemcpyAsync(h->hState, h->dState, sizeof(SolveState), hipMemcpyDeviceToHost, h->stream),
-                "state");
-        HIP_TRY(h, hipStreamSynchronize(h->stream), "state");
-        init = *h->hState;
-        init.thr2 = thr;
-        init.maxIters = h->opt.max_iters;
-        init.done = h->opt.max_iters <= 0 ? 2 : 0;
-        init.iters = 0;
-        *h->hState = init;
-        HIP_TRY(h, hipMemcpyAsync(h->dState, h->hState, sizeof(SolveState), hipMemcpyHostToDevice, h->stream),
-                "state");
+        HIP_TRY(h, launch_state_init(h->stream, h->dState, thr, h->opt.max_iters), "cg64 state");
         {
             Scope sc(h, NPPE_K_CG64_INIT, 16.0 * N + N);
             HIP_TRY(h, launch_cg64_init(h->stream, L0.box, h->t64, L0.types, h->dR64, h->dP64, h->dState,

@@ -1,0 +1,135 @@
+"""Writes the golden fixtures in this folder from the reference's own pressure path, compiled from its sources
+(``make -C oracle ref``; needs /root/reference, so it runs in the build container only).
+
+Each fixture is a .npz (no pickles) holding the case description (dimension, GRID_SIZE, domain, refinement
+intervals, cell types, right-hand side) and what the reference produced for it: the spacing tables
+(MAC::BuildAxis/AddRing/GapTable), the unknown numbering, the CSR matrix (PressureSolver{2D,3D}::AssembleSystem,
+CommitPressureMatrix), the singular flag, and solutions with iteration counts from SolvePressureSystem with
+eigen_solver=CG/DIAGONAL and eigen_solver=AMG (AMGCL).
+
+    python tests/golden/make_fixtures.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle as O  # noqa: E402
+from oracle import refcase as R  # noqa: E402
+
+FLUID, SOLID, EMPTY, INFLOW, PRESSURE, OUTFLOW = 0, 1, 2, 3, 4, 5
+
+
+def ring_mask(shape, dim):
+    r = np.zeros(shape, bool)
+    r[0] = r[-1] = True
+    r[:, 0] = r[:, -1] = True
+    if dim == 3:
+        r[:, :, 0] = r[:, :, -1] = True
+    return r
+
+
+def layout(kind, shape, dim, rng):
+    t = np.zeros(shape, np.uint8)
+    ring = ring_mask(shape, dim)
+    t[ring] = SOLID
+    if kind == "neumann":
+        pass
+    elif kind == "outlet":
+        t[:, -1][ring[:, -1]] = EMPTY
+    elif kind == "pressure":
+        t[ring] = PRESSURE
+    elif kind == "mixed":
+        codes = np.array([SOLID, EMPTY, PRESSURE, INFLOW, OUTFLOW, FLUID], np.uint8)
+        t[ring] = codes[rng.integers(0, len(codes), int(ring.sum()))]
+        r = rng.random(shape)
+        inner = ~ring
+        t[inner & (r < 0.1)] = SOLID
+        t[inner & (r >= 0.1) & (r < 0.15)] = EMPTY
+        t[inner & (r >= 0.15) & (r < 0.17)] = PRESSURE
+    elif kind == "freesurface":
+        inner = ~ring
+        t[inner] = EMPTY
+        Ny, Nx = shape[0], shape[1]
+        if dim == 3:
+            t[1:max(2, Ny // 2), 1:max(2, (2 * Nx) // 3), 1:-1] = FLUID
+        else:
+            t[1:max(2, Ny // 2), 1:max(2, (2 * Nx) // 3), 0] = FLUID
+        idx = np.argwhere(inner)
+        for c in idx[rng.integers(0, len(idx), 6)]:
+            t[tuple(c)] = FLUID
+    else:
+        raise ValueError(kind)
+    return t
+
+
+H, RA, SY, RB = O.REFINE_HARD, O.REFINE_RAMP, O.REFINE_SYMMETRIC, O.REFINE_RAMP_BACK
+REF = O.Refinement
+
+CASES = [
+    # name, dim, GRID_SIZE, domain, non_uniform, refinements (x, y, z), layout, compat
+    ("u3_neumann_8", 3, 8, (0, 1, 0, 1, 0, 1), False, ((), (), ()), "neumann", True),
+    ("u3_outlet_9", 3, 9, (0, 1, 0, 1, 0, 1), False, ((), (), ()), "outlet", False),
+    ("u3_pressure_7", 3, 7, (0, 1, 0, 1, 0, 1), False, ((), (), ()), "pressure", False),
+    ("u3_mixed_10", 3, 10, (0, 1, 0, 1, 0, 1), False, ((), (), ()), "mixed", False),
+    ("u3_freesurface_10", 3, 10, (0, 1, 0, 1, 0, 1), False, ((), (), ()), "freesurface", False),
+    ("u3_box_6x8x10", 3, 6, (0, 1, 0, 8 / 6, 0, 10 / 6), False, ((), (), ()), "outlet", False),
+    ("s3_hard_mixed", 3, 8, (0, 1, 0, 1, 0, 1), True,
+     ((REF(0.25, 0.5, 2, H),), (REF(0.5, 1.0, 3, H),), ()), "mixed", False),
+    ("s3_graded_outlet", 3, 10, (0, 1, 0, 1, 0, 1), True,
+     ((REF(0.3, 0.7, 3, SY),), (REF(0.0, 0.4, 2, RA), REF(0.6, 1.0, 2, RB)), (REF(0.2, 0.5, 2, H), REF(0.5, 0.8, 3, RA))),
+     "outlet", False),
+    ("s3_equal_neumann", 3, 8, (0, 1, 0, 1, 0, 1), True, ((), (), ()), "neumann", True),
+    ("u2_neumann_16", 2, 16, (0, 1, 0, 1, 0, 1), False, ((), (), ()), "neumann", True),
+    ("u2_pressure_12", 2, 12, (0, 1, 0, 1, 0, 1), False, ((), (), ()), "pressure", False),
+    ("u2_mixed_20", 2, 20, (0, 1, 0, 1, 0, 1), False, ((), (), ()), "mixed", False),
+    ("s2_ramp_outlet", 2, 16, (0, 1, 0, 1, 0, 1), True, ((REF(0.1, 0.6, 3, RA),), (REF(0.0, 0.5, 2, SY),), ()),
+     "outlet", False),
+    ("s2_units_hard", 2, 20, (0, 1, 0, 1, 0, 1), True, ((REF(0.4, 0.6, 2, H),), (), ()), "freesurface", False),
+]
+
+
+def main():
+    if not R.available():
+        raise SystemExit("oracle/_ref/ref_harness is not built: make -C oracle ref")
+    for q, (name, dim, gs, dom, nu, refs, kind, compat) in enumerate(CASES):
+        probe = R.run(R.RefCase(dim=dim, grid_size=gs, domain=dom, non_uniform=nu, refine=refs))
+        shape = (probe["Ny"], probe["Nx"], probe["Nz"])
+        rng = np.random.default_rng(1000 + q)
+        types = layout(kind, shape, dim, rng)
+        inner = types[1:-1, 1:-1, 1:-1] if dim == 3 else types[1:-1, 1:-1, :]
+        n = int((inner == FLUID).sum())
+        b = rng.standard_normal(n)
+        if compat:
+            b = O.compatibility(b)
+        solves = [R.SolveSpec(R.CG, R.DIAGONAL, 1e-12, 5000), R.SolveSpec(R.AMG, R.DIAGONAL, 1e-10, 500)]
+        case = R.RefCase(dim=dim, grid_size=gs, domain=dom, non_uniform=nu, refine=refs, types=types, compat=compat,
+                         solves=solves, rhs=b)
+        out = R.run(case, threads=1)
+        flat_refs = []
+        for a, axis in enumerate(refs):
+            for r in axis:
+                flat_refs.append([a, r.start, r.end, r.factor, r.mode])
+        np.savez_compressed(
+            os.path.join(HERE, name + ".npz"),
+            dim=dim, grid_size=gs, domain=np.array(dom, float), non_uniform=int(nu),
+            refinements=np.array(flat_refs, float).reshape(-1, 5), compat=int(compat), types=types, rhs=b,
+            dh=out["dh"], wx=out["wx"], wy=out["wy"], wz=out["wz"], gapx=out["gapx"], gapy=out["gapy"],
+            gapz=out.get("gapz", np.zeros(0)), ref_non_uniform=int(out["non_uniform"]),
+            n=out["n"], singular=int(out["singular"]), order=out["order"], rowptr=out["rowptr"], col=out["col"],
+            val=out["val"],
+            cg_x=out["solves"][0]["x"], cg_iters=out["solves"][0]["iters"], cg_tol=1e-12,
+            amg_x=out["solves"][1]["x"], amg_iters=out["solves"][1]["iters"], amg_tol=1e-10,
+        )
+        print(f"{name}: dims {shape} n={out['n']} nnz={out['nnz']} singular={out['singular']} "
+              f"cg_iters={out['solves'][0]['iters']} amg_iters={out['solves'][1]['iters']}")
+
+
+if __name__ == "__main__":
+    main()
