@@ -152,6 +152,41 @@ HostLevel coarsen(const HostLevel &f)
     return c;
 }
 
+void stencil_codes(const HostLevel &L, std::vector<uint8_t> &code8, std::vector<uint16_t> &code16)
+{
+    const int ns = L.ns, nm = L.nm, nf = L.nf;
+    const size_t cells = (size_t)ns * nm * nf;
+    code8.assign(cells, 0);
+    code16.assign(cells, 0);
+    auto at = [&](int s, int m, int f) { return ((size_t)s * nm + m) * nf + f; };
+    const int s0 = L.threeD ? 1 : 0, s1 = L.threeD ? ns - 2 : 0;
+    const int ds[6] = {1, -1, 0, 0, 0, 0}, dm[6] = {0, 0, 1, -1, 0, 0}, df[6] = {0, 0, 0, 0, 1, -1};
+    for (int s = s0; s <= s1; s++)
+        for (int m = 1; m <= nm - 2; m++)
+            for (int f = 1; f <= nf - 2; f++)
+            {
+                const size_t c = at(s, m, f);
+                if (L.types[c] != kFluid) continue;
+                uint8_t c8 = 1u << 6;
+                uint16_t c16 = 1u << 12;
+                for (int q = L.threeD ? 0 : 2; q < 6; q++)
+                {
+                    const int S = s + ds[q], M = m + dm[q], F = f + df[q];
+                    const bool ring = (L.threeD && (S == 0 || S == ns - 1)) || M == 0 || M == nm - 1 || F == 0 ||
+                                      F == nf - 1;
+                    const uint8_t t = L.types[at(S, M, F)];
+                    unsigned st = 0;
+                    if (t == kFluid) st = ring ? 2u : 1u;
+                    else if (is_dirichlet(t)) st = ring ? 3u : 2u;
+                    if (st == 1u) c8 |= (uint8_t)(1u << q);
+                    if (st >= 2u) c8 |= (uint8_t)(1u << 7);
+                    c16 |= (uint16_t)(st << (2 * q));
+                }
+                code8[c] = c8;
+                code16[c] = c16;
+            }
+}
+
 std::vector<float> tables32(const HostLevel &L, double sigma, int off[9])
 {
     std::vector<float> out;

@@ -32,6 +32,13 @@ HostLevel fine_level(int dim, int nx, int ny, int nz, const uint8_t *types, cons
 // child is EMPTY/PRESSURE, else FLUID if any child is FLUID, else SOLID; its width is the sum of its children's.
 HostLevel coarsen(const HostLevel &f);
 
+// Per-cell stencil codes of a level (unpitched box order). code8: bits 0-5 = neighbour q (slow+, slow-, mid+,
+// mid-, fast+, fast-) is a coupled unknown; bit 6 = the cell is an unknown; bit 7 = the cell has a Dirichlet
+// neighbour (read code16 for it). code16: two bits per neighbour q at bit 2q -- 0 Neumann, 1 coupled unknown,
+// 2 Dirichlet with the face coefficient (an interior EMPTY/PRESSURE cell, or a FLUID ring cell), 3 Dirichlet ring
+// cell (EMPTY/PRESSURE on the ring: coefficient times FaceGap/(Width/2)); bit 12 = the cell is an unknown.
+void stencil_codes(const HostLevel &L, std::vector<uint8_t> &code8, std::vector<uint16_t> &code16);
+
 // FP32 tables (widths, sigma/gap, 2 sigma/width) of a level, packed as wS wM wF gS gM gF hS hM hF; offsets[9].
 std::vector<float> tables32(const HostLevel &L, double sigma, int offsets[9]);
 

@@ -39,29 +39,30 @@ hipError_t launch_cg64_direction(hipStream_t s, const Box &B, const Tables64 &T,
                                  const double *r, double *p, SolveState *st);
 
 // ---- FP32 multigrid ------------------------------------------------------------------------------------------
-hipError_t launch_smooth(hipStream_t s, const Box &B, const Tables32 &T, const uint8_t *ty, float *x, const float *b,
-                         int color, int zero, const SolveState *st);
-hipError_t launch_residual_restrict(hipStream_t s, const Box &Bf, const Tables32 &Tf, const uint8_t *tyf,
-                                    const float *x, const float *b, const Box &Bc, float *bc, const SolveState *st);
-hipError_t launch_prolong(hipStream_t s, const Box &Bf, const uint8_t *tyf, float *x, const Box &Bc,
-                          const uint8_t *tyc, const float *xc, const SolveState *st);
+// One full red-black Gauss-Seidel sweep x_out = S(x_in) (c1: colour updated first; zero: x_in taken as 0), in one
+// pass: tiles of 64 x tileRows (8 or 16) cells marching along the slow axis in chunks of planesPerChunk planes.
+hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float *xout, const float *b, int c1,
+                       int zero, int tileRows, int planesPerChunk, const SolveState *st);
+hipError_t launch_residual_restrict(hipStream_t s, const Level32 &Lf, const float *x, const float *b,
+                                    const Level32 &Lc, float *bc, const SolveState *st);
+hipError_t launch_prolong(hipStream_t s, const Level32 &Lf, float *x, const Level32 &Lc, const float *xc,
+                          const SolveState *st);
 // Coarsest level: x[order[i]] = sum_j M[i][j] b[order[j]] in FP64 (M: the dense pseudo-inverse, n x n).
 hipError_t launch_coarse_solve(hipStream_t s, int n, const int32_t *order, const double *M, const float *b, float *x,
                                const SolveState *st);
 
 // ---- FP32 CG around the V-cycle ------------------------------------------------------------------------------
 // <r, z> -> beta (0 on the first direction), rz.
-hipError_t launch_cg_rz(hipStream_t s, const Box &B, const uint8_t *ty, const float *r, const float *z, SolveState *st,
+hipError_t launch_cg_rz(hipStream_t s, const Level32 &L, const float *r, const float *z, SolveState *st,
                         double *partials);
 // p = z + beta p (p = z on the first direction).
-hipError_t launch_cg_direction(hipStream_t s, const Box &B, const uint8_t *ty, const float *z, float *p,
-                               SolveState *st);
+hipError_t launch_cg_direction(hipStream_t s, const Level32 &L, const float *z, float *p, SolveState *st);
 // q = A p, <p, q> -> alpha.
-hipError_t launch_cg_spmv(hipStream_t s, const Box &B, const Tables32 &T, const uint8_t *ty, const float *p, float *q,
-                          SolveState *st, double *partials);
+hipError_t launch_cg_spmv(hipStream_t s, const Level32 &L, const float *p, float *q, SolveState *st,
+                          double *partials);
 // e += alpha p (e = alpha p on the first iteration), r -= alpha q, <r, r> -> convergence flag.
-hipError_t launch_cg_update(hipStream_t s, const Box &B, const uint8_t *ty, float *e, float *r, const float *p,
-                            const float *q, SolveState *st, double *partials);
+hipError_t launch_cg_update(hipStream_t s, const Level32 &L, float *e, float *r, const float *p, const float *q,
+                            SolveState *st, double *partials);
 // Inner-solve set-up from the outer residual norm st->sums[1]: thr2 = red^2 * sums[1], counters cleared.
 hipError_t launch_inner_setup(hipStream_t s, SolveState *st, double reduction, int maxIters);
 // FP64 CG set-up: thr2, iteration cap, counters cleared.

@@ -100,7 +100,9 @@ __device__ __forceinline__ bool tile_cell(const Box &B, int t, int &s, int &m, i
 
 // ---- FP32 operator -------------------------------------------------------------------------------------------
 // Coefficients of the cell at (s, m, f) in direction order slow+, slow-, mid+, mid-, fast+, fast-: c[q] for a
-// coupled neighbour (coupled bit q set), and the diagonal (negative, or 0 for an isolated cell).
+// coupled neighbour (coupled bit q set), and the diagonal (negative, or 0 for an isolated cell). Built from the
+// stencil code: the common cell (no Dirichlet neighbour) needs only its code8 byte; a cell beside a Dirichlet
+// cell reads its code16 word.
 struct Op32
 {
     float c[6];
@@ -108,42 +110,52 @@ struct Op32
     unsigned coupled;
 };
 
-__device__ __forceinline__ Op32 op32(const Box &B, const Tables32 &T, const uint8_t *ty, int s, int m, int f,
-                                     int64_t idx)
+__device__ __forceinline__ Op32 op32(const Level32 &L, int s, int m, int f, int64_t idx, unsigned code)
 {
     Op32 o;
-    const float wS = T.wS[s], wM = T.wM[m], wF = T.wF[f];
-    const float aS = wM * wF, aM = wS * wF, aF = wS * wM;
+    o.coupled = code & 63u;
     float ring[6];
-    o.c[0] = aS * T.gS[s];
-    o.c[1] = aS * T.gS[s - (B.threeD ? 1 : 0)];
-    o.c[2] = aM * T.gM[m];
-    o.c[3] = aM * T.gM[m - 1];
-    o.c[4] = aF * T.gF[f];
-    o.c[5] = aF * T.gF[f - 1];
-    ring[0] = ring[1] = aS * T.hS[s];
-    ring[2] = ring[3] = aM * T.hM[m];
-    ring[4] = ring[5] = aF * T.hF[f];
-    const int64_t off[6] = {B.sS, -B.sS, B.sM, -B.sM, 1, -1};
-    const bool onRing[6] = {s + 1 == B.ns - 1, s - 1 == 0, m + 1 == B.nm - 1, m - 1 == 0, f + 1 == B.nf - 1,
-                            f - 1 == 0};
-    float diag = 0.f;
-    unsigned cp = 0;
-#pragma unroll
-    for (int q = 0; q < 6; q++)
+    if (L.uniform)
     {
-        if (q < 2 && !B.threeD) continue;
-        const uint8_t t = ty[idx + off[q]];
-        if (t == kFluid)
+#pragma unroll
+        for (int q = 0; q < 6; q++)
         {
-            diag -= o.c[q];
-            if (!onRing[q]) cp |= 1u << q;
+            o.c[q] = L.cu;
+            ring[q] = L.cr;
         }
-        else if (is_dirichlet(t))
-            diag -= onRing[q] ? ring[q] : o.c[q];
+    }
+    else
+    {
+        const Tables32 &T = L.T;
+        const float wS = T.wS[s], wM = T.wM[m], wF = T.wF[f];
+        const float aS = wM * wF, aM = wS * wF, aF = wS * wM;
+        o.c[0] = aS * T.gS[s];
+        o.c[1] = aS * T.gS[s - (L.B.threeD ? 1 : 0)];
+        o.c[2] = aM * T.gM[m];
+        o.c[3] = aM * T.gM[m - 1];
+        o.c[4] = aF * T.gF[f];
+        o.c[5] = aF * T.gF[f - 1];
+        ring[0] = ring[1] = aS * T.hS[s];
+        ring[2] = ring[3] = aM * T.hM[m];
+        ring[4] = ring[5] = aF * T.hF[f];
+    }
+    float diag = 0.f;
+    if (!(code & 128u))
+    {
+#pragma unroll
+        for (int q = 0; q < 6; q++) diag -= ((o.coupled >> q) & 1u) ? o.c[q] : 0.f;
+    }
+    else
+    {
+        const unsigned c16 = L.c16[idx];
+#pragma unroll
+        for (int q = 0; q < 6; q++)
+        {
+            const unsigned st = (c16 >> (2 * q)) & 3u;
+            diag -= st == 3u ? ring[q] : (st != 0u ? o.c[q] : 0.f);
+        }
     }
     o.diag = diag;
-    o.coupled = cp;
     return o;
 }
 
@@ -159,6 +171,37 @@ __device__ __forceinline__ float offsum32(const Box &B, const Op32 &o, const flo
         acc += ((o.coupled >> q) & 1u) ? o.c[q] * v : 0.f;
     }
     return acc;
+}
+
+// 1/n for the n coupled neighbours of an interior cell of a uniform level (no Dirichlet neighbour): there the
+// diagonal is -cu n, so a Gauss-Seidel update is (sum of coupled x - b / cu) / n and A x = cu (sum - n x).
+__constant__ float kInvN[8] = {0.f, 1.f, 0.5f, 1.f / 3.f, 0.25f, 0.2f, 1.f / 6.f, 0.f};
+
+__device__ __forceinline__ bool fast_cell(const Level32 &L, unsigned code) { return L.uniform && !(code & 128u); }
+
+// Sum of the coupled neighbours' x (uniform fast path).
+__device__ __forceinline__ float nbsum32(const Box &B, unsigned code, const float *x, int64_t idx)
+{
+    float acc = 0.f;
+    if (B.threeD)
+    {
+        acc += (code & 1u) ? x[idx + B.sS] : 0.f;
+        acc += (code & 2u) ? x[idx - B.sS] : 0.f;
+    }
+    acc += (code & 4u) ? x[idx + B.sM] : 0.f;
+    acc += (code & 8u) ? x[idx - B.sM] : 0.f;
+    acc += (code & 16u) ? x[idx + 1] : 0.f;
+    acc += (code & 32u) ? x[idx - 1] : 0.f;
+    return acc;
+}
+
+// (A x)_idx in FP32.
+__device__ __forceinline__ float apply32(const Level32 &L, int s, int m, int f, int64_t idx, unsigned code,
+                                         const float *x)
+{
+    if (fast_cell(L, code)) return L.cu * (nbsum32(L.B, code, x, idx) - (float)__popc(code & 63u) * x[idx]);
+    const Op32 o = op32(L, s, m, f, idx, code);
+    return offsum32(L.B, o, x, idx) + o.diag * x[idx];
 }
 
 // ---- FP64 operator with the reference's exact arithmetic ----------------------------------------------------
@@ -258,6 +301,36 @@ __device__ __forceinline__ double row64(const Box &B, const Op64 &o, int64_t idx
     if (o.coupled & 1u) t += o.v[0] * xat(B.sS);
     (void)idx;
     return t;
+}
+
+// (A x)_idx in FP64 with the reference's exact arithmetic. A unit-coefficient (3D uniform) cell with no Dirichlet
+// neighbour has off-diagonal scale and diagonal -n * scale, summed in the same column order, so it skips the
+// coefficient work; every other cell goes through op64.
+template <class X>
+__device__ __forceinline__ double apply64(const Box &B, const Tables64 &T, const uint8_t *ty, int s, int m, int f,
+                                          int64_t idx, X xat)
+{
+#pragma clang fp contract(off)
+    if (T.unitCoef)
+    {
+        const unsigned code = T.c8[idx];
+        if (!(code & 128u))
+        {
+            const double v = T.scale * 1.0;
+            const double dv = T.scale * (-(double)__popc(code & 63u));
+            double t = 0.0;
+            if (code & 2u) t += v * xat(-B.sS);
+            if (code & 8u) t += v * xat(-B.sM);
+            if (code & 32u) t += v * xat(-1);
+            if (code & 63u) t += dv * xat(0);
+            if (code & 16u) t += v * xat(1);
+            if (code & 4u) t += v * xat(B.sM);
+            if (code & 1u) t += v * xat(B.sS);
+            return t;
+        }
+    }
+    const Op64 o = op64(B, T, ty, s, m, f, idx);
+    return row64(B, o, idx, xat);
 }
 
 // ---- scalar finalisation -----------------------------------------------------------------------------------
@@ -397,16 +470,15 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64(Box B, Tabl
         double r = 0.0;
         if (ty[idx] == kFluid)
         {
-            const Op64 o = op64(B, T, ty, s, m, f, idx);
             double ax;
             if (e)
             {
-                ax = row64(B, o, idx, [&](int64_t d) { return x[idx + d] + (double)e[idx + d]; });
+                ax = apply64(B, T, ty, s, m, f, idx, [&](int64_t d) { return x[idx + d] + (double)e[idx + d]; });
                 xout[idx] = x[idx] + (double)e[idx];
             }
             else
             {
-                ax = row64(B, o, idx, [&](int64_t d) { return x[idx + d]; });
+                ax = apply64(B, T, ty, s, m, f, idx, [&](int64_t d) { return x[idx + d]; });
             }
             r = b[idx] - ax;
         }
@@ -428,17 +500,29 @@ __global__ __launch_bounds__(kBlockThreads) void k_apply64(Box B, Tables64 T, co
         if (!tile_cell(B, t, s, m, f)) continue;
         const int64_t idx = B.at(s, m, f);
         double v = 0.0;
-        if (ty[idx] == kFluid)
-        {
-            const Op64 o = op64(B, T, ty, s, m, f, idx);
-            v = row64(B, o, idx, [&](int64_t d) { return x[idx + d]; });
-        }
+        if (ty[idx] == kFluid) v = apply64(B, T, ty, s, m, f, idx, [&](int64_t d) { return x[idx + d]; });
         y[idx] = v;
     }
 }
 
 // Jacobi: Eigen's DiagonalPreconditioner takes 1/value of the diagonal entry, or 1 where the row has none.
 __device__ __forceinline__ double inv_diag64(const Op64 &o) { return o.diag != 0.0 ? 1.0 / o.dval : 1.0; }
+
+__device__ __forceinline__ double inv_diag_at(const Box &B, const Tables64 &T, const uint8_t *ty, int s, int m, int f,
+                                              int64_t idx)
+{
+#pragma clang fp contract(off)
+    if (T.unitCoef)
+    {
+        const unsigned code = T.c8[idx];
+        if (!(code & 128u))
+        {
+            const unsigned n = __popc(code & 63u);
+            return n ? 1.0 / (T.scale * (-(double)n)) : 1.0;
+        }
+    }
+    return inv_diag64(op64(B, T, ty, s, m, f, idx));
+}
 
 __global__ __launch_bounds__(kBlockThreads) void k_cg64_init(Box B, Tables64 T, const uint8_t *__restrict__ ty,
                                                              const double *__restrict__ r, double *__restrict__ p,
@@ -454,8 +538,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg64_init(Box B, Tables64 T, 
         double pv = 0.0;
         if (ty[idx] == kFluid)
         {
-            const Op64 o = op64(B, T, ty, s, m, f, idx);
-            pv = inv_diag64(o) * r[idx];
+            pv = inv_diag_at(B, T, ty, s, m, f, idx) * r[idx];
             rz += r[idx] * pv;
         }
         p[idx] = pv;
@@ -479,8 +562,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg64_spmv(Box B, Tables64 T, 
         double v = 0.0;
         if (ty[idx] == kFluid)
         {
-            const Op64 o = op64(B, T, ty, s, m, f, idx);
-            v = row64(B, o, idx, [&](int64_t d) { return p[idx + d]; });
+            v = apply64(B, T, ty, s, m, f, idx, [&](int64_t d) { return p[idx + d]; });
             pq += p[idx] * v;
         }
         q[idx] = v;
@@ -508,9 +590,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg64_update(Box B, Tables64 T
         x[idx] += alpha * p[idx];
         const double rv = r[idx] - alpha * q[idx];
         r[idx] = rv;
-        const Op64 o = op64(B, T, ty, s, m, f, idx);
         rr += rv * rv;
-        rz += rv * (inv_diag64(o) * rv);
+        rz += rv * (inv_diag_at(B, T, ty, s, m, f, idx) * rv);
     }
     double a[2] = {rr, rz};
     block_sum<2>(a, partials, gridDim.x, tr.lb);
@@ -529,43 +610,292 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg64_direction(Box B, Tables6
         if (!tile_cell(B, t, s, m, f)) continue;
         const int64_t idx = B.at(s, m, f);
         if (ty[idx] != kFluid) continue;
-        const Op64 o = op64(B, T, ty, s, m, f, idx);
-        p[idx] = inv_diag64(o) * r[idx] + beta * p[idx];
+        p[idx] = inv_diag_at(B, T, ty, s, m, f, idx) * r[idx] + beta * p[idx];
     }
 }
 
 // ---- FP32 multigrid kernels ----------------------------------------------------------------------------------
 
-// One colour of a red-black Gauss-Seidel sweep: x = (b - sum_coupled c x_nb) / diag on cells with
-// (s + m + f) % 2 == color. zero: the sweep starts from x = 0 (the neighbours are not read).
-__global__ __launch_bounds__(kBlockThreads) void k_smooth(Box B, Tables32 T, const uint8_t *__restrict__ ty,
-                                                          float *__restrict__ x, const float *__restrict__ b,
-                                                          int color, int zero, const SolveState *st)
+// One full red-black Gauss-Seidel sweep in one pass over memory (x_in -> x_out, never in place).
+//
+// A block owns a TF x TM tile of the mid/fast plane and marches along the slow axis over a chunk of planes. The
+// old x of four consecutive planes sits in LDS with a two-cell halo. At plane p the first colour is updated on the
+// tile plus a one-cell halo (the halo values are recomputed from the same old data the neighbouring block uses,
+// so every block sees identical numbers), then the second colour on the tile of plane p-1, whose six neighbours
+// are all first-colour values already final, and plane p-1 leaves for x_out. Colour of cell (s, m, f) is
+// (s + m + f) & 1; c1 is the colour updated first (0 = red first for the pre-smoother, 1 for the adjoint
+// post-smoother). zero: x_in is taken as 0 (the first sweep of a V-cycle level).
+constexpr int kTF = 64;
+constexpr int kLW = kTF + 4;   // LDS row length: the tile plus a two-cell halo each side
+
+template <int TM>
+struct Rbgs
+{
+    static constexpr int kLoad = ((TM + 4) * kLW + kBlockThreads - 1) / kBlockThreads;   // x loads per thread
+    static constexpr int kC1 = ((TM + 2) * 33 + kBlockThreads - 1) / kBlockThreads;      // first-colour cells
+    static constexpr int kC2 = (TM * 32 + kBlockThreads - 1) / kBlockThreads;            // second-colour cells
+    static constexpr int kStore = (TM * kTF + kBlockThreads - 1) / kBlockThreads;
+};
+
+// Per-thread staging of one plane's global data, loaded a step before it is used.
+template <int TM>
+struct RbgsRegs
+{
+    float x[Rbgs<TM>::kLoad];
+    float b1[Rbgs<TM>::kC1];
+    unsigned c1[Rbgs<TM>::kC1];
+    float b2[Rbgs<TM>::kC2];
+    unsigned c2[Rbgs<TM>::kC2];
+};
+
+struct RbgsCtx
+{
+    Level32 L;
+    int m0, f0;
+};
+
+// Cell k of colour col among rows [r0, r0 + rows) and columns [cA, cA + 2*perRow) (local halo-2 coordinates) of
+// plane pl: returns false when the slot is empty or outside the interior.
+__device__ __forceinline__ bool rbgs_cell(const RbgsCtx &C, int pl, int col, int j, int r0, int rows, int cA,
+                                          int perRow, int &lr, int &lc, int64_t &idx)
+{
+    if (j >= rows * perRow) return false;
+    lr = r0 + j / perRow;
+    const int k = j - (j / perRow) * perRow;
+    const int m = C.m0 - 2 + lr;
+    const int fA = C.f0 - 2 + cA;
+    const int par = (col - pl - m - fA) & 1;
+    lc = cA + 2 * k + par;
+    const int f = C.f0 - 2 + lc;
+    const Box &B = C.L.B;
+    if (lc > cA + 2 * perRow - 1 || m < 1 || m > B.nm - 2 || f < 1 || f > B.nf - 2) return false;
+    idx = B.at(pl, m, f);
+    return true;
+}
+
+template <int TM>
+__device__ __forceinline__ void rbgs_fetch_x(const RbgsCtx &C, const float *xin, int zero, int pl, float (&v)[Rbgs<TM>::kLoad])
+{
+    const Box &B = C.L.B;
+    const bool planeOk = !zero && pl >= 0 && pl < B.ns;
+#pragma unroll
+    for (int k = 0; k < Rbgs<TM>::kLoad; k++)
+    {
+        const int i = threadIdx.x + k * kBlockThreads;
+        const int lr = i / kLW, lc = i - lr * kLW;
+        const int m = C.m0 - 2 + lr, f = C.f0 - 2 + lc;
+        v[k] = (planeOk && lr < TM + 4 && m >= 0 && m < B.nm && f >= 0 && f < B.nf) ? xin[B.at(pl, m, f)] : 0.f;
+    }
+}
+
+template <int TM>
+__device__ __forceinline__ void rbgs_put_x(float (*X)[kLW], const float (&v)[Rbgs<TM>::kLoad])
+{
+#pragma unroll
+    for (int k = 0; k < Rbgs<TM>::kLoad; k++)
+    {
+        const int i = threadIdx.x + k * kBlockThreads;
+        const int lr = i / kLW, lc = i - lr * kLW;
+        if (lr < TM + 4) X[lr][lc] = v[k];
+    }
+}
+
+// code and b of the first-colour cells of plane pl (tile + 1-cell halo) and second-colour cells of plane pl2.
+template <int TM>
+__device__ __forceinline__ void rbgs_fetch_cells(const RbgsCtx &C, const float *__restrict__ b, int pl, int c1,
+                                                 int pl2, int c2, RbgsRegs<TM> &R)
+{
+    const Box &B = C.L.B;
+    const bool ok1 = B.threeD ? (pl >= 1 && pl <= B.ns - 2) : pl == 0;
+    const bool ok2 = B.threeD ? (pl2 >= 1 && pl2 <= B.ns - 2) : pl2 == 0;
+#pragma unroll
+    for (int k = 0; k < Rbgs<TM>::kC1; k++)
+    {
+        int lr, lc;
+        int64_t idx;
+        R.c1[k] = 0u;
+        R.b1[k] = 0.f;
+        if (ok1 && rbgs_cell(C, pl, c1, threadIdx.x + k * kBlockThreads, 1, TM + 2, 1, 33, lr, lc, idx))
+        {
+            R.c1[k] = C.L.c8[idx];
+            R.b1[k] = b[idx];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < Rbgs<TM>::kC2; k++)
+    {
+        int lr, lc;
+        int64_t idx;
+        R.c2[k] = 0u;
+        R.b2[k] = 0.f;
+        if (ok2 && rbgs_cell(C, pl2, c2, threadIdx.x + k * kBlockThreads, 2, TM, 2, 32, lr, lc, idx))
+        {
+            R.c2[k] = C.L.c8[idx];
+            R.b2[k] = b[idx];
+        }
+    }
+}
+
+// Gauss-Seidel update of one cell from LDS neighbours: Xc the cell's plane, Xp / Xm the planes above / below.
+__device__ __forceinline__ float rbgs_update(const RbgsCtx &C, int pl, int lr, int lc, int64_t idx, unsigned code,
+                                             float bv, float (*Xc)[kLW], float (*Xp)[kLW], float (*Xm)[kLW])
+{
+    if (fast_cell(C.L, code))
+    {
+        float acc = 0.f;
+        if (C.L.B.threeD)
+        {
+            acc += (code & 1u) ? Xp[lr][lc] : 0.f;
+            acc += (code & 2u) ? Xm[lr][lc] : 0.f;
+        }
+        acc += (code & 4u) ? Xc[lr + 1][lc] : 0.f;
+        acc += (code & 8u) ? Xc[lr - 1][lc] : 0.f;
+        acc += (code & 16u) ? Xc[lr][lc + 1] : 0.f;
+        acc += (code & 32u) ? Xc[lr][lc - 1] : 0.f;
+        return (acc - bv * C.L.icu) * kInvN[__popc(code & 63u)];
+    }
+    const int m = C.m0 - 2 + lr, f = C.f0 - 2 + lc;
+    const Op32 o = op32(C.L, pl, m, f, idx, code);
+    float acc = 0.f;
+    if (C.L.B.threeD)
+    {
+        acc += (o.coupled & 1u) ? o.c[0] * Xp[lr][lc] : 0.f;
+        acc += (o.coupled & 2u) ? o.c[1] * Xm[lr][lc] : 0.f;
+    }
+    acc += (o.coupled & 4u) ? o.c[2] * Xc[lr + 1][lc] : 0.f;
+    acc += (o.coupled & 8u) ? o.c[3] * Xc[lr - 1][lc] : 0.f;
+    acc += (o.coupled & 16u) ? o.c[4] * Xc[lr][lc + 1] : 0.f;
+    acc += (o.coupled & 32u) ? o.c[5] * Xc[lr][lc - 1] : 0.f;
+    return o.diag != 0.f ? (bv - acc) / o.diag : 0.f;
+}
+
+template <int TM>
+__device__ __forceinline__ void rbgs_colour1(const RbgsCtx &C, float (*X)[TM + 4][kLW], int pl, int c1,
+                                             const RbgsRegs<TM> &R)
+{
+    float(*Xc)[kLW] = X[pl & 3];
+    float(*Xp)[kLW] = X[(pl + 1) & 3];
+    float(*Xm)[kLW] = X[(pl + 3) & 3];
+#pragma unroll
+    for (int k = 0; k < Rbgs<TM>::kC1; k++)
+    {
+        if (!(R.c1[k] & 64u)) continue;
+        int lr, lc;
+        int64_t idx;
+        rbgs_cell(C, pl, c1, threadIdx.x + k * kBlockThreads, 1, TM + 2, 1, 33, lr, lc, idx);
+        Xc[lr][lc] = rbgs_update(C, pl, lr, lc, idx, R.c1[k], R.b1[k], Xc, Xp, Xm);
+    }
+}
+
+template <int TM>
+__device__ __forceinline__ void rbgs_colour2(const RbgsCtx &C, float (*X)[TM + 4][kLW], int pl, int c2,
+                                             const RbgsRegs<TM> &R)
+{
+    float(*Xc)[kLW] = X[pl & 3];
+    float(*Xp)[kLW] = X[(pl + 1) & 3];
+    float(*Xm)[kLW] = X[(pl + 3) & 3];
+#pragma unroll
+    for (int k = 0; k < Rbgs<TM>::kC2; k++)
+    {
+        if (!(R.c2[k] & 64u)) continue;
+        int lr, lc;
+        int64_t idx;
+        rbgs_cell(C, pl, c2, threadIdx.x + k * kBlockThreads, 2, TM, 2, 32, lr, lc, idx);
+        Xc[lr][lc] = rbgs_update(C, pl, lr, lc, idx, R.c2[k], R.b2[k], Xc, Xp, Xm);
+    }
+}
+
+template <int TM>
+__device__ __forceinline__ void rbgs_store(const RbgsCtx &C, float (*X)[TM + 4][kLW], float *xout, int pl)
+{
+    const Box &B = C.L.B;
+    float(*Xc)[kLW] = X[pl & 3];
+#pragma unroll
+    for (int k = 0; k < Rbgs<TM>::kStore; k++)
+    {
+        const int i = threadIdx.x + k * kBlockThreads;
+        const int r = i / kTF, c = i - r * kTF;
+        const int m = C.m0 + r, f = C.f0 + c;
+        if (r < TM && m <= B.nm - 2 && f <= B.nf - 2) xout[B.at(pl, m, f)] = Xc[r + 2][c + 2];
+    }
+}
+
+// One full red-black Gauss-Seidel sweep in one pass over memory (x_in -> x_out, never in place).
+//
+// A block owns a 64 x TM tile of the mid/fast plane and marches along the slow axis over a chunk of planes. The old
+// x of four consecutive planes sits in LDS with a two-cell halo. At plane p the first colour is updated on the tile
+// plus a one-cell halo (halo values are recomputed from the same old data the neighbouring block uses, so every
+// block sees identical numbers), then the second colour on the tile of plane p-1, whose six neighbours are all
+// first-colour values already final, and plane p-1 leaves for x_out. Global loads for the next plane are issued
+// before the current plane is computed. Colour of cell (s, m, f) is (s + m + f) & 1; c1 is updated first (0: red
+// first, the pre-smoother; 1: its adjoint, the post-smoother). zero: x_in is taken as 0.
+template <int TM>
+__global__ __launch_bounds__(kBlockThreads) void k_rbgs(Level32 L, const float *__restrict__ xin,
+                                                        float *__restrict__ xout, const float *__restrict__ b, int c1,
+                                                        int zero, int planesPerChunk, int tilesF, int tilesM,
+                                                        const SolveState *st)
 {
     if (st->done) return;
-    const TileRange tr = my_tiles(B.blocks());
-    for (int t = tr.begin; t < tr.end; t++)
+    __shared__ float X[4][TM + 4][kLW];
+    const int lb = logical_block();
+    const int tf = lb % tilesF, t2 = lb / tilesF, tm = t2 % tilesM, chunk = t2 / tilesM;
+    const RbgsCtx C{L, 1 + tm * TM, 1 + tf * kTF};
+    const int c2 = c1 ^ 1;
+    RbgsRegs<TM> R;
+    if (!L.B.threeD)
     {
-        int s, m, f;
-        if (!tile_cell(B, t, s, m, f)) continue;
-        if (((s + m + f) & 1) != color) continue;
-        const int64_t idx = B.at(s, m, f);
-        if (ty[idx] != kFluid) continue;
-        const Op32 o = op32(B, T, ty, s, m, f, idx);
-        const float acc = zero ? 0.f : offsum32(B, o, x, idx);
-        x[idx] = o.diag != 0.f ? (b[idx] - acc) / o.diag : 0.f;
+        float v[Rbgs<TM>::kLoad];
+        rbgs_fetch_x<TM>(C, xin, zero, 0, v);
+        rbgs_fetch_cells<TM>(C, b, 0, c1, 0, c2, R);
+        rbgs_put_x<TM>(X[0], v);
+        __syncthreads();
+        rbgs_colour1<TM>(C, X, 0, c1, R);
+        __syncthreads();
+        rbgs_colour2<TM>(C, X, 0, c2, R);
+        __syncthreads();
+        rbgs_store<TM>(C, X, xout, 0);
+        return;
+    }
+    const int s0 = 1 + chunk * planesPerChunk;
+    const int s1 = min(s0 + planesPerChunk - 1, L.B.ns - 2);
+    {
+        float v[Rbgs<TM>::kLoad];
+        rbgs_fetch_x<TM>(C, xin, zero, s0 - 2, v);
+        rbgs_put_x<TM>(X[(s0 - 2) & 3], v);
+        rbgs_fetch_x<TM>(C, xin, zero, s0 - 1, v);
+        rbgs_put_x<TM>(X[(s0 - 1) & 3], v);
+        rbgs_fetch_x<TM>(C, xin, zero, s0, R.x);
+        rbgs_fetch_cells<TM>(C, b, s0 - 1, c1, s0 - 2, c2, R);
+    }
+    for (int p = s0 - 1; p <= s1 + 1; p++)
+    {
+        // plane p+1 (fetched last step) into LDS; issue the global loads of the next step
+        rbgs_put_x<TM>(X[(p + 1) & 3], R.x);
+        RbgsRegs<TM> N;
+        rbgs_fetch_x<TM>(C, xin, zero, p + 2, N.x);
+        rbgs_fetch_cells<TM>(C, b, p + 1, c1, p, c2, N);
+        __syncthreads();
+        if (p >= 1 && p <= L.B.ns - 2) rbgs_colour1<TM>(C, X, p, c1, R);
+        __syncthreads();
+        const int q = p - 1;
+        if (q >= s0)
+        {
+            rbgs_colour2<TM>(C, X, q, c2, R);
+            __syncthreads();
+            rbgs_store<TM>(C, X, xout, q);
+        }
+        R = N;
     }
 }
 
 // Coarse cell (S, M, F) gathers the residual b - A x of its fine children (fine indices 2C-1, 2C along each
 // coarsened axis, clipped to the fine interior): bc = sum (R = P^T for the injection prolongation).
-__global__ __launch_bounds__(kBlockThreads) void k_residual_restrict(Box Bf, Tables32 Tf,
-                                                                     const uint8_t *__restrict__ tyf,
-                                                                     const float *__restrict__ x,
-                                                                     const float *__restrict__ b, Box Bc,
+__global__ __launch_bounds__(kBlockThreads) void k_residual_restrict(Level32 Lf, const float *__restrict__ x,
+                                                                     const float *__restrict__ b, Level32 Lc,
                                                                      float *__restrict__ bc, const SolveState *st)
 {
     if (st->done) return;
+    const Box &Bf = Lf.B, &Bc = Lc.B;
     const TileRange tr = my_tiles(Bc.blocks());
     for (int t = tr.begin; t < tr.end; t++)
     {
@@ -579,31 +909,30 @@ __global__ __launch_bounds__(kBlockThreads) void k_residual_restrict(Box Bf, Tab
                 for (int f = 2 * F - 1; f <= f1; f++)
                 {
                     const int64_t idx = Bf.at(s, m, f);
-                    if (tyf[idx] != kFluid) continue;
-                    const Op32 o = op32(Bf, Tf, tyf, s, m, f, idx);
-                    sum += b[idx] - (offsum32(Bf, o, x, idx) + o.diag * x[idx]);
+                    const unsigned code = Lf.c8[idx];
+                    if (!(code & 64u)) continue;
+                    sum += b[idx] - apply32(Lf, s, m, f, idx, code, x);
                 }
         bc[Bc.at(S, M, F)] = sum;
     }
 }
 
-// x += P xc: every fine FLUID cell takes its parent's correction when the parent is a coarse unknown.
-__global__ __launch_bounds__(kBlockThreads) void k_prolong(Box Bf, const uint8_t *__restrict__ tyf,
-                                                           float *__restrict__ x, Box Bc,
-                                                           const uint8_t *__restrict__ tyc,
+// x += P xc: every fine unknown takes its parent's correction when the parent is a coarse unknown.
+__global__ __launch_bounds__(kBlockThreads) void k_prolong(Level32 Lf, float *__restrict__ x, Level32 Lc,
                                                            const float *__restrict__ xc, const SolveState *st)
 {
     if (st->done) return;
+    const Box &Bf = Lf.B, &Bc = Lc.B;
     const TileRange tr = my_tiles(Bf.blocks());
     for (int t = tr.begin; t < tr.end; t++)
     {
         int s, m, f;
         if (!tile_cell(Bf, t, s, m, f)) continue;
         const int64_t idx = Bf.at(s, m, f);
-        if (tyf[idx] != kFluid) continue;
+        if (!(Lf.c8[idx] & 64u)) continue;
         const int64_t pidx = Bc.at(Bf.threeD ? (s + 1) >> 1 : 0, (m + 1) >> 1, (f + 1) >> 1);
         const float v = xc[pidx];
-        x[idx] += tyc[pidx] == kFluid ? v : 0.f;
+        x[idx] += (Lc.c8[pidx] & 64u) ? v : 0.f;
     }
 }
 
@@ -629,11 +958,12 @@ __global__ __launch_bounds__(kBlockThreads) void k_coarse_solve(int n, const int
 
 // ---- FP32 CG kernels -----------------------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kBlockThreads) void k_cg_rz(Box B, const uint8_t *__restrict__ ty,
-                                                         const float *__restrict__ r, const float *__restrict__ z,
-                                                         const SolveState *st, double *partials)
+__global__ __launch_bounds__(kBlockThreads) void k_cg_rz(Level32 L, const float *__restrict__ r,
+                                                         const float *__restrict__ z, const SolveState *st,
+                                                         double *partials)
 {
     if (st->done) return;
+    const Box &B = L.B;
     const TileRange tr = my_tiles(B.blocks());
     double v = 0.0;
     for (int t = tr.begin; t < tr.end; t++)
@@ -641,17 +971,17 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_rz(Box B, const uint8_t *_
         int s, m, f;
         if (!tile_cell(B, t, s, m, f)) continue;
         const int64_t idx = B.at(s, m, f);
-        if (ty[idx] == kFluid) v += (double)r[idx] * (double)z[idx];
+        if (L.c8[idx] & 64u) v += (double)r[idx] * (double)z[idx];
     }
     double a[1] = {v};
     block_sum<1>(a, partials, gridDim.x, tr.lb);
 }
 
-__global__ __launch_bounds__(kBlockThreads) void k_cg_direction(Box B, const uint8_t *__restrict__ ty,
-                                                                const float *__restrict__ z, float *__restrict__ p,
-                                                                const SolveState *st)
+__global__ __launch_bounds__(kBlockThreads) void k_cg_direction(Level32 L, const float *__restrict__ z,
+                                                                float *__restrict__ p, const SolveState *st)
 {
     if (st->done) return;
+    const Box &B = L.B;
     const int first = st->first;
     const float beta = (float)st->beta;
     const TileRange tr = my_tiles(B.blocks());
@@ -660,15 +990,16 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_direction(Box B, const uin
         int s, m, f;
         if (!tile_cell(B, t, s, m, f)) continue;
         const int64_t idx = B.at(s, m, f);
-        if (ty[idx] == kFluid) p[idx] = first ? z[idx] : z[idx] + beta * p[idx];
+        if (L.c8[idx] & 64u) p[idx] = first ? z[idx] : z[idx] + beta * p[idx];
     }
 }
 
-__global__ __launch_bounds__(kBlockThreads) void k_cg_spmv(Box B, Tables32 T, const uint8_t *__restrict__ ty,
-                                                           const float *__restrict__ p, float *__restrict__ q,
-                                                           const SolveState *st, double *partials)
+__global__ __launch_bounds__(kBlockThreads) void k_cg_spmv(Level32 L, const float *__restrict__ p,
+                                                           float *__restrict__ q, const SolveState *st,
+                                                           double *partials)
 {
     if (st->done) return;
+    const Box &B = L.B;
     const TileRange tr = my_tiles(B.blocks());
     double pq = 0.0;
     for (int t = tr.begin; t < tr.end; t++)
@@ -676,10 +1007,10 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_spmv(Box B, Tables32 T, co
         int s, m, f;
         if (!tile_cell(B, t, s, m, f)) continue;
         const int64_t idx = B.at(s, m, f);
-        if (ty[idx] != kFluid) continue;
-        const Op32 o = op32(B, T, ty, s, m, f, idx);
+        const unsigned code = L.c8[idx];
+        if (!(code & 64u)) continue;
         const float pc = p[idx];
-        const float v = offsum32(B, o, p, idx) + o.diag * pc;
+        const float v = apply32(L, s, m, f, idx, code, p);
         q[idx] = v;
         pq += (double)pc * (double)v;
     }
@@ -687,12 +1018,12 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_spmv(Box B, Tables32 T, co
     block_sum<1>(a, partials, gridDim.x, tr.lb);
 }
 
-__global__ __launch_bounds__(kBlockThreads) void k_cg_update(Box B, const uint8_t *__restrict__ ty,
-                                                             float *__restrict__ e, float *__restrict__ r,
+__global__ __launch_bounds__(kBlockThreads) void k_cg_update(Level32 L, float *__restrict__ e, float *__restrict__ r,
                                                              const float *__restrict__ p, const float *__restrict__ q,
                                                              const SolveState *st, double *partials)
 {
     if (st->done) return;
+    const Box &B = L.B;
     const int firstIter = st->iters == 0;
     const float alpha = (float)st->alpha;
     const TileRange tr = my_tiles(B.blocks());
@@ -702,7 +1033,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_update(Box B, const uint8_
         int s, m, f;
         if (!tile_cell(B, t, s, m, f)) continue;
         const int64_t idx = B.at(s, m, f);
-        if (ty[idx] != kFluid) continue;
+        if (!(L.c8[idx] & 64u)) continue;
         const float pv = p[idx];
         e[idx] = firstIter ? alpha * pv : e[idx] + alpha * pv;
         const float rv = r[idx] - alpha * q[idx];
@@ -831,24 +1162,34 @@ hipError_t launch_cg64_direction(hipStream_t s, const Box &B, const Tables64 &T,
     return hipGetLastError();
 }
 
-hipError_t launch_smooth(hipStream_t s, const Box &B, const Tables32 &T, const uint8_t *ty, float *x, const float *b,
-                         int color, int zero, const SolveState *st)
+hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float *xout, const float *b, int c1,
+                       int zero, int tileRows, int planesPerChunk, const SolveState *st)
 {
-    k_smooth<<<grid_of(B), kBlockThreads, 0, s>>>(B, T, ty, x, b, color, zero, st);
+    const Box &B = L.B;
+    const int tilesF = std::max(1, (B.nf - 2 + kTF - 1) / kTF);
+    const int TM = tileRows == 8 ? 8 : 16;
+    const int tilesM = std::max(1, (B.nm - 2 + TM - 1) / TM);
+    const int P = std::max(1, planesPerChunk);
+    const int chunks = B.threeD ? std::max(1, (B.ns - 2 + P - 1) / P) : 1;
+    const int grid = tilesF * tilesM * chunks;
+    if (TM == 8)
+        k_rbgs<8><<<grid, kBlockThreads, 0, s>>>(L, xin, xout, b, c1, zero, P, tilesF, tilesM, st);
+    else
+        k_rbgs<16><<<grid, kBlockThreads, 0, s>>>(L, xin, xout, b, c1, zero, P, tilesF, tilesM, st);
     return hipGetLastError();
 }
 
-hipError_t launch_residual_restrict(hipStream_t s, const Box &Bf, const Tables32 &Tf, const uint8_t *tyf,
-                                    const float *x, const float *b, const Box &Bc, float *bc, const SolveState *st)
+hipError_t launch_residual_restrict(hipStream_t s, const Level32 &Lf, const float *x, const float *b,
+                                    const Level32 &Lc, float *bc, const SolveState *st)
 {
-    k_residual_restrict<<<grid_of(Bc), kBlockThreads, 0, s>>>(Bf, Tf, tyf, x, b, Bc, bc, st);
+    k_residual_restrict<<<grid_of(Lc.B), kBlockThreads, 0, s>>>(Lf, x, b, Lc, bc, st);
     return hipGetLastError();
 }
 
-hipError_t launch_prolong(hipStream_t s, const Box &Bf, const uint8_t *tyf, float *x, const Box &Bc,
-                          const uint8_t *tyc, const float *xc, const SolveState *st)
+hipError_t launch_prolong(hipStream_t s, const Level32 &Lf, float *x, const Level32 &Lc, const float *xc,
+                          const SolveState *st)
 {
-    k_prolong<<<grid_of(Bf), kBlockThreads, 0, s>>>(Bf, tyf, x, Bc, tyc, xc, st);
+    k_prolong<<<grid_of(Lf.B), kBlockThreads, 0, s>>>(Lf, x, Lc, xc, st);
     return hipGetLastError();
 }
 
@@ -862,36 +1203,35 @@ hipError_t launch_coarse_solve(hipStream_t s, int n, const int32_t *order, const
     return hipGetLastError();
 }
 
-hipError_t launch_cg_rz(hipStream_t s, const Box &B, const uint8_t *ty, const float *r, const float *z, SolveState *st,
+hipError_t launch_cg_rz(hipStream_t s, const Level32 &L, const float *r, const float *z, SolveState *st,
                         double *partials)
 {
-    const int G = grid_of(B);
-    k_cg_rz<<<G, kBlockThreads, 0, s>>>(B, ty, r, z, st, partials);
+    const int G = grid_of(L.B);
+    k_cg_rz<<<G, kBlockThreads, 0, s>>>(L, r, z, st, partials);
     LAUNCH_OK();
     return finalize(s, st, partials, G, 1, FIN_RZ);
 }
 
-hipError_t launch_cg_direction(hipStream_t s, const Box &B, const uint8_t *ty, const float *z, float *p,
-                               SolveState *st)
+hipError_t launch_cg_direction(hipStream_t s, const Level32 &L, const float *z, float *p, SolveState *st)
 {
-    k_cg_direction<<<grid_of(B), kBlockThreads, 0, s>>>(B, ty, z, p, st);
+    k_cg_direction<<<grid_of(L.B), kBlockThreads, 0, s>>>(L, z, p, st);
     return hipGetLastError();
 }
 
-hipError_t launch_cg_spmv(hipStream_t s, const Box &B, const Tables32 &T, const uint8_t *ty, const float *p, float *q,
-                          SolveState *st, double *partials)
+hipError_t launch_cg_spmv(hipStream_t s, const Level32 &L, const float *p, float *q, SolveState *st,
+                          double *partials)
 {
-    const int G = grid_of(B);
-    k_cg_spmv<<<G, kBlockThreads, 0, s>>>(B, T, ty, p, q, st, partials);
+    const int G = grid_of(L.B);
+    k_cg_spmv<<<G, kBlockThreads, 0, s>>>(L, p, q, st, partials);
     LAUNCH_OK();
     return finalize(s, st, partials, G, 1, FIN_PQ);
 }
 
-hipError_t launch_cg_update(hipStream_t s, const Box &B, const uint8_t *ty, float *e, float *r, const float *p,
-                            const float *q, SolveState *st, double *partials)
+hipError_t launch_cg_update(hipStream_t s, const Level32 &L, float *e, float *r, const float *p, const float *q,
+                            SolveState *st, double *partials)
 {
-    const int G = grid_of(B);
-    k_cg_update<<<G, kBlockThreads, 0, s>>>(B, ty, e, r, p, q, st, partials);
+    const int G = grid_of(L.B);
+    k_cg_update<<<G, kBlockThreads, 0, s>>>(L, e, r, p, q, st, partials);
     LAUNCH_OK();
     return finalize(s, st, partials, G, 1, FIN_RR);
 }

@@ -41,6 +41,19 @@ struct Tables32
     const float *hS, *hM, *hF;
 };
 
+// One FP32 multigrid level as the kernels see it: box, tables, stencil codes (see stencil_codes in
+// hierarchy.h) and, when every width of the level is equal, the constant face coefficient cu and ring-Dirichlet
+// coefficient cr = 2 cu that replace the tables.
+struct Level32
+{
+    Box B;
+    Tables32 T;
+    const uint8_t *c8;
+    const uint16_t *c16;
+    float cu, cr, icu;   // icu = 1 / cu
+    int uniform;
+};
+
 // FP64 tables of the fine level, exactly as the reference forms them: widths and FaceGap(n) = (w[n]+w[n+1])/2.
 struct Tables64
 {
@@ -48,6 +61,7 @@ struct Tables64
     const double *gS, *gM, *gF;
     double scale;   // MatrixScale3D: 1/(dh*dh) uniform, 1 stretched
     int unitCoef;   // 3D uniform: every face coefficient is exactly 1.0 (PressureSolver3D.cpp:49-53)
+    const uint8_t *c8;   // the fine level's stencil codes (a cell without a Dirichlet neighbour takes a fast path)
 };
 
 // Device-resident scalars of one solve. Written by the last block of each reduction.

@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -41,9 +42,15 @@ struct DevLevel
     HostLevel host;
     Box box{};
     uint8_t *types = nullptr;
-    float *x = nullptr, *b = nullptr;
+    uint8_t *c8 = nullptr;
+    uint16_t *c16 = nullptr;
+    float *x[2] = {nullptr, nullptr};   // ping-pong pair of the smoother (level 0 uses the solver's z pair)
+    float *b = nullptr;
     float *tab = nullptr;
     Tables32 t32{};
+    Level32 v{};
+    int tileRows = 8;        // smoother tile: 64 x tileRows cells, planesPerChunk planes per block
+    int planesPerChunk = 4;
     double cellsBox = 0.0;   // ns * nm * nf (the box, unpitched)
 };
 
@@ -69,7 +76,7 @@ struct nppe_solver
     Tables64 t64{};
     double *dX64[2] = {nullptr, nullptr};
     double *dB64 = nullptr;
-    float *dE = nullptr, *dR = nullptr, *dP = nullptr, *dQ = nullptr, *dZ = nullptr;
+    float *dE = nullptr, *dR = nullptr, *dP = nullptr, *dQ = nullptr, *dZ = nullptr, *dZ2 = nullptr;
     double *dR64 = nullptr, *dP64 = nullptr, *dQ64 = nullptr;
     double *dCompB = nullptr, *dCompX = nullptr, *dCompG = nullptr;
     double *dCoarseM = nullptr;
@@ -143,7 +150,7 @@ void free_grid(nppe_solver *h)
     h->dTab64 = nullptr;
     h->dX64[0] = h->dX64[1] = nullptr;
     h->dB64 = nullptr;
-    h->dE = h->dR = h->dP = h->dQ = h->dZ = nullptr;
+    h->dE = h->dR = h->dP = h->dQ = h->dZ = h->dZ2 = nullptr;
     h->dR64 = h->dP64 = h->dQ64 = nullptr;
     h->dCompB = h->dCompX = h->dCompG = nullptr;
     h->dCoarseM = nullptr;
@@ -201,66 +208,70 @@ void profile_collect(nppe_solver *h, int itersDone)
 // ---- multigrid ------------------------------------------------------------------------------------------------
 double lvl_cells(const DevLevel &L) { return L.cellsBox; }
 
-hipError_t vcycle(nppe_solver *h, Traffic &tr, float *x0, const float *b0)
+// One V-cycle z = M^-1 r0 with zero initial guess on every level. Level l keeps its iterate in x[l][cur[l]]
+// (sweeps ping-pong between the pair). Returns, in *zsel, which of the two level-0 buffers holds z.
+hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel)
 {
     const int nl = (int)h->levels.size();
     const int nu1 = std::max(1, h->opt.pre_smooth), nu2 = std::max(0, h->opt.post_smooth);
     const double s = 4.0, f = 1.0;
     hipError_t e = hipSuccess;
+    int cur[kMaxLevels] = {};
+    auto xbuf = [&](int l, int k) -> float * { return l == 0 ? (k ? h->dZ2 : h->dZ) : h->levels[l].x[k]; };
+    auto bbuf = [&](int l) -> const float * { return l == 0 ? b0 : h->levels[l].b; };
     for (int l = 0; l < nl - 1; l++)
     {
         DevLevel &L = h->levels[l];
-        float *x = l == 0 ? x0 : L.x;
-        const float *b = l == 0 ? b0 : L.b;
-        const double N = lvl_cells(L);
+        const double N = L.cellsBox;
         for (int sw = 0; sw < nu1; sw++)
         {
-            {
-                Scope sc(h, NPPE_K_SMOOTH, sw == 0 ? (s + s / 2 + f) * N / 2 : (3 * s + f) * N / 2);
-                if ((e = launch_smooth(h->stream, L.box, L.t32, L.types, x, b, 0, sw == 0, h->dState)) != hipSuccess)
-                    return e;
-            }
-            {
-                Scope sc(h, NPPE_K_SMOOTH, sw == 0 ? (s + s / 2 + s / 2 + f) * N / 2 : (3 * s + f) * N / 2);
-                if ((e = launch_smooth(h->stream, L.box, L.t32, L.types, x, b, 1, 0, h->dState)) != hipSuccess)
-                    return e;
-            }
-            tr.add(sw == 0 ? (2 * s + f) * N : (3 * s + f) * N);
+            const bool zero = sw == 0;
+            const double by = zero ? (2 * s + f) * N : (3 * s + f) * N;
+            Scope sc(h, NPPE_K_SMOOTH, by);
+            float *xin = xbuf(l, cur[l]);
+            float *xout = zero ? xin : xbuf(l, cur[l] ^ 1);
+            if ((e = launch_rbgs(h->stream, L.v, zero ? nullptr : xin, xout, bbuf(l), 0, zero ? 1 : 0, L.tileRows,
+                                 L.planesPerChunk, h->dState)) != hipSuccess)
+                return e;
+            if (!zero) cur[l] ^= 1;
+            tr.add(by);
         }
         DevLevel &C = h->levels[l + 1];
-        const double rb = (2 * s + f) * N + s * lvl_cells(C);
+        const double rb = (2 * s + f) * N + s * C.cellsBox;
         {
             Scope sc(h, NPPE_K_RESTRICT, rb);
-            if ((e = launch_residual_restrict(h->stream, L.box, L.t32, L.types, x, b, C.box, C.b, h->dState)) !=
+            if ((e = launch_residual_restrict(h->stream, L.v, xbuf(l, cur[l]), bbuf(l), C.v, C.b, h->dState)) !=
                 hipSuccess)
                 return e;
         }
         tr.add(rb);
     }
     {
-        DevLevel &L = h->levels[nl - 1];
-        float *x = nl == 1 ? x0 : L.x;
-        const float *b = nl == 1 ? b0 : L.b;
+        const int l = nl - 1;
+        DevLevel &L = h->levels[l];
         if (h->coarseDense)
         {
             const double cb = 8.0 * h->nCoarse * (double)h->nCoarse + 8.0 * h->nCoarse * 2;
             Scope sc(h, NPPE_K_COARSE, cb);
-            if ((e = launch_coarse_solve(h->stream, h->nCoarse, h->dCoarseOrder, h->dCoarseM, b, x, h->dState)) !=
-                hipSuccess)
+            if ((e = launch_coarse_solve(h->stream, h->nCoarse, h->dCoarseOrder, h->dCoarseM, bbuf(l), xbuf(l, 0),
+                                         h->dState)) != hipSuccess)
                 return e;
             tr.add(cb);
         }
         else
         {
             // Coarsest level too large for the dense solve: symmetric Gauss-Seidel sweeps.
-            const double N = lvl_cells(L);
+            const double N = L.cellsBox;
             for (int sw = 0; sw < 16; sw++)
             {
                 Scope sc(h, NPPE_K_SMOOTH, (3 * s + f) * N);
-                if ((e = launch_smooth(h->stream, L.box, L.t32, L.types, x, b, sw & 1, sw == 0, h->dState)) !=
-                    hipSuccess)
+                float *xin = xbuf(l, cur[l]);
+                float *xout = sw == 0 ? xin : xbuf(l, cur[l] ^ 1);
+                if ((e = launch_rbgs(h->stream, L.v, sw == 0 ? nullptr : xin, xout, bbuf(l), sw & 1, sw == 0, L.tileRows,
+                                     L.planesPerChunk, h->dState)) != hipSuccess)
                     return e;
-                tr.add((3 * s + f) * N / 2);
+                if (sw) cur[l] ^= 1;
+                tr.add((3 * s + f) * N);
             }
         }
     }
@@ -268,31 +279,27 @@ hipError_t vcycle(nppe_solver *h, Traffic &tr, float *x0, const float *b0)
     {
         DevLevel &L = h->levels[l];
         DevLevel &C = h->levels[l + 1];
-        float *x = l == 0 ? x0 : L.x;
-        const float *b = l == 0 ? b0 : L.b;
-        const double N = lvl_cells(L);
-        const double pb = s * lvl_cells(C) + 2 * s * N + f * N;
+        const double N = L.cellsBox;
+        const double pb = s * C.cellsBox + 2 * s * N + f * N;
         {
             Scope sc(h, NPPE_K_PROLONG, pb);
-            if ((e = launch_prolong(h->stream, L.box, L.types, x, C.box, C.types, C.x, h->dState)) != hipSuccess)
+            if ((e = launch_prolong(h->stream, L.v, xbuf(l, cur[l]), C.v, xbuf(l + 1, cur[l + 1]), h->dState)) !=
+                hipSuccess)
                 return e;
         }
         tr.add(pb);
         for (int sw = 0; sw < nu2; sw++)
         {
-            {
-                Scope sc(h, NPPE_K_SMOOTH, (3 * s + f) * N / 2);
-                if ((e = launch_smooth(h->stream, L.box, L.t32, L.types, x, b, 1, 0, h->dState)) != hipSuccess)
-                    return e;
-            }
-            {
-                Scope sc(h, NPPE_K_SMOOTH, (3 * s + f) * N / 2);
-                if ((e = launch_smooth(h->stream, L.box, L.t32, L.types, x, b, 0, 0, h->dState)) != hipSuccess)
-                    return e;
-            }
-            tr.add((3 * s + f) * N);
+            const double by = (3 * s + f) * N;
+            Scope sc(h, NPPE_K_SMOOTH, by);
+            if ((e = launch_rbgs(h->stream, L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l), 1, 0, L.tileRows,
+                                 L.planesPerChunk, h->dState)) != hipSuccess)
+                return e;
+            cur[l] ^= 1;
+            tr.add(by);
         }
     }
+    *zsel = cur[0];
     return hipSuccess;
 }
 
@@ -302,26 +309,25 @@ hipError_t inner_iteration(nppe_solver *h, Traffic &tr)
     const DevLevel &L0 = h->levels[0];
     const double N = L0.cellsBox, s = 4.0, f = 1.0;
     hipError_t e;
-    if ((e = vcycle(h, tr, h->dZ, h->dR)) != hipSuccess) return e;
+    int zsel = 0;
+    if ((e = vcycle(h, tr, h->dR, &zsel)) != hipSuccess) return e;
+    const float *z = zsel ? h->dZ2 : h->dZ;
     {
         Scope sc(h, NPPE_K_CG_DOT, 2 * s * N + f * N);
-        if ((e = launch_cg_rz(h->stream, L0.box, L0.types, h->dR, h->dZ, h->dState, h->dPartials)) != hipSuccess)
-            return e;
+        if ((e = launch_cg_rz(h->stream, L0.v, h->dR, z, h->dState, h->dPartials)) != hipSuccess) return e;
     }
     {
         Scope sc(h, NPPE_K_CG_DIRECTION, 3 * s * N + f * N);
-        if ((e = launch_cg_direction(h->stream, L0.box, L0.types, h->dZ, h->dP, h->dState)) != hipSuccess) return e;
+        if ((e = launch_cg_direction(h->stream, L0.v, z, h->dP, h->dState)) != hipSuccess) return e;
     }
     {
         Scope sc(h, NPPE_K_CG_SPMV, (2 * s + f) * N);
-        if ((e = launch_cg_spmv(h->stream, L0.box, L0.t32, L0.types, h->dP, h->dQ, h->dState, h->dPartials)) !=
-            hipSuccess)
-            return e;
+        if ((e = launch_cg_spmv(h->stream, L0.v, h->dP, h->dQ, h->dState, h->dPartials)) != hipSuccess) return e;
     }
     {
         Scope sc(h, NPPE_K_CG_UPDATE, 6 * s * N + f * N);
-        if ((e = launch_cg_update(h->stream, L0.box, L0.types, h->dE, h->dR, h->dP, h->dQ, h->dState,
-                                  h->dPartials)) != hipSuccess)
+        if ((e = launch_cg_update(h->stream, L0.v, h->dE, h->dR, h->dP, h->dQ, h->dState, h->dPartials)) !=
+            hipSuccess)
             return e;
     }
     tr.add((2 * s + f) * N + (3 * s + f) * N + (2 * s + f) * N + (6 * s + f) * N);
@@ -416,15 +422,50 @@ int build(nppe_solver *h, const nppe_grid *g)
                     "upload types");
             HIP_TRY(h, hipStreamSynchronize(h->stream), "upload types");
         }
+        {
+            std::vector<uint8_t> c8h;
+            std::vector<uint16_t> c16h;
+            stencil_codes(H, c8h, c16h);
+            std::vector<uint8_t> p8((size_t)L.box.cells, 0);
+            std::vector<uint16_t> p16((size_t)L.box.cells, 0);
+            for (int s = 0; s < H.ns; s++)
+                for (int m = 0; m < H.nm; m++)
+                {
+                    const size_t src = ((size_t)s * H.nm + m) * H.nf, dst = (size_t)L.box.at(s, m, 0);
+                    std::memcpy(&p8[dst], &c8h[src], H.nf);
+                    std::memcpy(&p16[dst], &c16h[src], H.nf * sizeof(uint16_t));
+                }
+            HIP_TRY(h, dalloc(h, &L.c8, p8.size()), "allocate codes");
+            HIP_TRY(h, dalloc(h, &L.c16, p16.size()), "allocate codes");
+            HIP_TRY(h, hipMemcpy(L.c8, p8.data(), p8.size(), hipMemcpyHostToDevice), "upload codes");
+            HIP_TRY(h, hipMemcpy(L.c16, p16.data(), p16.size() * sizeof(uint16_t), hipMemcpyHostToDevice),
+                    "upload codes");
+        }
         int off[9];
         std::vector<float> tab = tables32(H, sigma, off);
         HIP_TRY(h, dalloc(h, &L.tab, tab.size()), "allocate tables");
         HIP_TRY(h, hipMemcpy(L.tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice), "upload tables");
         L.t32 = Tables32{L.tab + off[0], L.tab + off[1], L.tab + off[2], L.tab + off[3], L.tab + off[4],
                          L.tab + off[5], L.tab + off[6], L.tab + off[7], L.tab + off[8]};
+        L.v.B = L.box;
+        L.v.T = L.t32;
+        L.v.c8 = L.c8;
+        L.v.c16 = L.c16;
+        L.v.uniform = H.uniform ? 1 : 0;
+        if (const char *e = std::getenv("NPPE_RBGS_TM")) L.tileRows = std::atoi(e) == 8 ? 8 : 16;
+        if (const char *e = std::getenv("NPPE_RBGS_P")) L.planesPerChunk = std::max(1, std::atoi(e));
+        {
+            // constant coefficient of a uniform level: sigma * area / gap = sigma * w (3D) or sigma (2D)
+            const double w = H.w[1][0];
+            const double cu = H.threeD ? sigma * w : sigma;
+            L.v.cu = (float)cu;
+            L.v.cr = (float)(2.0 * cu);
+            L.v.icu = (float)(1.0 / cu);
+        }
         if (l > 0)
         {
-            HIP_TRY(h, dalloc(h, &L.x, (size_t)L.box.cells), "allocate level x");
+            HIP_TRY(h, dalloc(h, &L.x[0], (size_t)L.box.cells), "allocate level x");
+            HIP_TRY(h, dalloc(h, &L.x[1], (size_t)L.box.cells), "allocate level x");
             HIP_TRY(h, dalloc(h, &L.b, (size_t)L.box.cells), "allocate level b");
         }
     }
@@ -467,7 +508,7 @@ int build(nppe_solver *h, const nppe_grid *g)
         HIP_TRY(h, hipMemcpy(h->dTab64, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice),
                 "upload FP64 tables");
         h->t64 = Tables64{h->dTab64 + off[0], h->dTab64 + off[1], h->dTab64 + off[2], h->dTab64 + off[3],
-                          h->dTab64 + off[4], h->dTab64 + off[5], scale, unitCoef ? 1 : 0};
+                          h->dTab64 + off[4], h->dTab64 + off[5], scale, unitCoef ? 1 : 0, F.c8};
     }
     {
         std::vector<int32_t> pitched(h->orderHost.size());
@@ -493,6 +534,7 @@ int build(nppe_solver *h, const nppe_grid *g)
         HIP_TRY(h, dalloc(h, &h->dP, cells), "allocate p");
         HIP_TRY(h, dalloc(h, &h->dQ, cells), "allocate q");
         HIP_TRY(h, dalloc(h, &h->dZ, cells), "allocate z");
+        HIP_TRY(h, dalloc(h, &h->dZ2, cells), "allocate z");
     }
     else
     {
