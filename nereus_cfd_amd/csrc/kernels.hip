@@ -888,6 +888,199 @@ __global__ __launch_bounds__(kBlockThreads) void k_rbgs(Level32 L, const float *
     }
 }
 
+// ---- red-black Gauss-Seidel sweep, 2.5D march (3D boxes) -----------------------------------------------------
+//
+// Each lane owns a pair of neighbouring cells (f, f+1) of one row and marches with it along the slow axis, so on
+// every plane it holds exactly one cell of each colour and no lane idles. Colour of (s, m, f) is (s + m + f) & 1 and
+// c1 is the colour updated first. At plane step s the lane updates its first-colour cell of plane s from old values
+// (x_in: slow neighbours from its registers, in-row neighbours from the neighbouring lanes by shuffle, the rows
+// above and below from x_in through L1), publishes it in LDS, then updates its second-colour cell of plane s-1,
+// whose six neighbours are first-colour values already final: slow ones from its own registers, in-row ones by
+// shuffle, mid ones from LDS. Two extra waves march the rows just outside the tile and only publish first-colour
+// values; lanes 0 and 63 also update the first-colour cells just left / right of the tile when the second-colour
+// cell at the tile edge needs them. Every redundant value is computed from the same old data its owner uses, so
+// blocks agree bit for bit and the sweep equals the two-pass red-black sweep. One barrier per plane.
+constexpr int kPW = 128;   // tile width: 64 lanes x 2 cells
+
+// Gauss-Seidel value of one cell from its b and neighbours (slow+, slow-, mid+, mid-, fast+, fast-). UNI: the
+// level is uniform, so even a cell beside a Dirichlet cell needs no tables (its code16 gives the ring factors).
+template <bool UNI>
+__device__ __forceinline__ float gs_update(const Level32 &L, int s, int m, int f, int64_t idx, unsigned code, float bv,
+                                           const float (&nb)[6])
+{
+    if (UNI)
+    {
+        float acc = 0.f;
+#pragma unroll
+        for (int q = 0; q < 6; q++) acc += ((code >> q) & 1u) ? nb[q] : 0.f;
+        if (!(code & 128u)) return (acc - bv * L.icu) * kInvN[__popc(code & 63u)];
+        const unsigned c16 = L.c16[idx];
+        float d = 0.f;   // -diag / cu
+#pragma unroll
+        for (int q = 0; q < 6; q++)
+        {
+            const unsigned st = (c16 >> (2 * q)) & 3u;
+            d += st == 3u ? 2.f : (st != 0u ? 1.f : 0.f);
+        }
+        return d != 0.f ? (acc - bv * L.icu) / d : 0.f;
+    }
+    const Op32 o = op32(L, s, m, f, idx, code);
+    float acc = 0.f;
+#pragma unroll
+    for (int q = 0; q < 6; q++) acc += ((o.coupled >> q) & 1u) ? o.c[q] * nb[q] : 0.f;
+    return o.diag != 0.f ? (bv - acc) / o.diag : 0.f;
+}
+
+template <int TM, bool UNI>
+__global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_march(Level32 L, const float *__restrict__ xin,
+                                                              float *__restrict__ xout, const float *__restrict__ b,
+                                                              int c1, int zero, int P, int tilesF, int tilesM,
+                                                              const SolveState *st)
+{
+    if (st->done) return;
+    const Box &B = L.B;
+    __shared__ float red[2][TM + 2][kPW + 2];   // first-colour values of rows m0-1 .. m0+TM, columns f0-1 .. f0+128
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;   // wave 0: row m0-1, wave TM+1: row m0+TM
+    const int lb = logical_block();
+    const int tf = lb % tilesF, t2 = lb / tilesF, tm = t2 % tilesM, chunk = t2 / tilesM;
+    const int f0 = 1 + tf * kPW, m0 = 1 + tm * TM;
+    const int m = m0 - 1 + wave;
+    const bool haloRow = wave == 0 || wave == TM + 1;
+    const bool rowOk = m >= 1 && m <= B.nm - 2;
+    const int fa = f0 + 2 * lane;                     // the pair (fa, fa + 1)
+    const bool okA = rowOk && fa <= B.nf - 2, okB = rowOk && fa + 1 <= B.nf - 2;
+    const int s0 = 1 + chunk * P, s1 = min(s0 + P - 1, B.ns - 2);
+    const bool leftLane = lane == 0 && !haloRow, rightLane = lane == 63 && !haloRow;
+    const int fe = leftLane ? f0 - 1 : f0 + kPW;      // edge column of an edge lane
+    const bool okE = (leftLane || rightLane) && rowOk && fe >= 1 && fe <= B.nf - 2;
+    const int64_t rowBase = B.at(0, rowOk ? m : 1, 0);
+    const bool rd = !zero;
+
+    auto X = [&](int sp, int f) -> float {   // old value, 0 outside the box or for the zero start
+        if (!rd || sp < 0 || sp >= B.ns) return 0.f;
+        return xin[rowBase + (int64_t)sp * B.sS + f];
+    };
+    auto inPlane = [&](int sp) { return sp >= 1 && sp <= B.ns - 2; };
+    // Old values of the pair and the edge column at planes sp-1, sp, sp+1 and (prefetched) sp+2.
+    float xa[4], xb[4], xe[4];
+    {
+        const int sp = s0 - 1;
+        xa[1] = okA ? X(sp - 1, fa) : 0.f;
+        xb[1] = okB ? X(sp - 1, fa + 1) : 0.f;
+        xe[1] = okE ? X(sp - 1, fe) : 0.f;
+        xa[2] = okA ? X(sp, fa) : 0.f;
+        xb[2] = okB ? X(sp, fa + 1) : 0.f;
+        xe[2] = okE ? X(sp, fe) : 0.f;
+        xa[3] = okA ? X(sp + 1, fa) : 0.f;
+        xb[3] = okB ? X(sp + 1, fa + 1) : 0.f;
+        xe[3] = okE ? X(sp + 1, fe) : 0.f;
+    }
+    // b and codes of the pair at the current plane (prefetched a step ahead) and at the previous one.
+    float bA = 0.f, bB = 0.f, bPrevA = 0.f, bPrevB = 0.f;
+    unsigned cA = 0u, cB = 0u, cPrevA = 0u, cPrevB = 0u;
+    if (inPlane(s0 - 1))
+    {
+        const int64_t i = rowBase + (int64_t)(s0 - 1) * B.sS + fa;
+        if (okA) { cA = L.c8[i]; bA = b[i]; }
+        if (okB) { cB = L.c8[i + 1]; bB = b[i + 1]; }
+    }
+    // New first-colour values of the lane's column at planes sp-2, sp-1, sp.
+    float R[3] = {0.f, 0.f, 0.f};
+    for (int sp = s0 - 1; sp <= s1 + 1; sp++)
+    {
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+        {
+            xa[k] = xa[k + 1];
+            xb[k] = xb[k + 1];
+            xe[k] = xe[k + 1];
+        }
+        // prefetch for the next step: old x of plane sp+2, b and codes of plane sp+1
+        xa[3] = okA ? X(sp + 2, fa) : 0.f;
+        xb[3] = okB ? X(sp + 2, fa + 1) : 0.f;
+        xe[3] = okE ? X(sp + 2, fe) : 0.f;
+        float nbA = 0.f, nbB = 0.f;
+        unsigned ncA = 0u, ncB = 0u;
+        if (inPlane(sp + 1))
+        {
+            const int64_t i = rowBase + (int64_t)(sp + 1) * B.sS + fa;
+            if (okA) { ncA = L.c8[i]; nbA = b[i]; }
+            if (okB) { ncB = L.c8[i + 1]; nbB = b[i + 1]; }
+        }
+        R[0] = R[1];
+        R[1] = R[2];
+        const bool planeIn = inPlane(sp);
+        // first colour at plane sp: element e1 of the pair; in-row neighbours are old values
+        const int e1 = (c1 - sp - m - fa) & 1;
+        float leftNb = __shfl_up(xb[1], 1, 64);
+        float rightNb = __shfl_down(xa[1], 1, 64);
+        if (lane == 0) leftNb = (rowOk && fa - 1 >= 0) ? (okE && leftLane ? xe[1] : X(sp, fa - 1)) : 0.f;
+        if (lane == 63) rightNb = (rowOk && fa + 2 < B.nf) ? (okE && rightLane ? xe[1] : X(sp, fa + 2)) : 0.f;
+        const int64_t idxA = rowBase + (int64_t)sp * B.sS + fa;
+        float rnew = 0.f;
+        {
+            const unsigned code = e1 ? cB : cA;
+            if (planeIn && (code & 64u))
+            {
+                const int64_t idx = idxA + e1;
+                const float nb[6] = {e1 ? xb[2] : xa[2], e1 ? xb[0] : xa[0], rd ? xin[idx + B.sM] : 0.f,
+                                     rd ? xin[idx - B.sM] : 0.f, e1 ? rightNb : xb[1], e1 ? xa[1] : leftNb};
+                rnew = gs_update<UNI>(L, sp, m, fa + e1, idx, code, e1 ? bB : bA, nb);
+            }
+        }
+        // the edge first-colour cell next to the tile-edge second-colour cell, same plane
+        float redge = 0.f;
+        if (okE && planeIn && ((c1 - sp - m - fe) & 1) == 0)
+        {
+            const int64_t idx = rowBase + (int64_t)sp * B.sS + fe;
+            const unsigned code = L.c8[idx];
+            if (code & 64u)
+            {
+                const float nb[6] = {xe[2], xe[0], rd ? xin[idx + B.sM] : 0.f, rd ? xin[idx - B.sM] : 0.f,
+                                     leftLane ? xa[1] : X(sp, fe + 1), leftLane ? X(sp, fe - 1) : xb[1]};
+                redge = gs_update<UNI>(L, sp, m, fe, idx, code, b[idx], nb);
+            }
+        }
+        R[2] = rnew;
+        // Publish plane sp's first-colour values (column 1 + 2 lane + e1; edges at 0 and kPW + 1). One barrier per
+        // plane: it orders this step's writes of red[sp & 1] after the previous step's reads of the same buffer
+        // (plane sp - 2), and the previous step's writes of red[q & 1] before this step's reads.
+        __syncthreads();
+        red[sp & 1][wave][1 + 2 * lane + e1] = rnew;
+        if (leftLane) red[sp & 1][wave][0] = redge;
+        if (rightLane) red[sp & 1][wave][kPW + 1] = redge;
+        // second colour at plane q = sp - 1 on tile rows: the colour flips with the plane, so at q the first-colour
+        // element is e1 ^ 1 and the second-colour element is e1
+        const int q = sp - 1;
+        if (!haloRow && q >= s0 && q <= s1)
+        {
+            const int eq = e1;
+            const unsigned code = eq ? cPrevB : cPrevA;
+            float out = eq ? xb[0] : xa[0];   // a non-unknown keeps its (zero) value
+            const float rq = R[1];            // this pair's first-colour value at plane q (element eq ^ 1)
+            if (code & 64u)
+            {
+                const int c = 1 + 2 * lane + eq;
+                const float left = eq ? rq : red[q & 1][wave][c - 1];
+                const float right = eq ? red[q & 1][wave][c + 1] : rq;
+                const float nb[6] = {R[2], R[0], red[q & 1][wave + 1][c], red[q & 1][wave - 1][c], right, left};
+                out = gs_update<UNI>(L, q, m, fa + eq, rowBase + (int64_t)q * B.sS + fa + eq, code,
+                                     eq ? bPrevB : bPrevA, nb);
+            }
+            if (okA) xout[rowBase + (int64_t)q * B.sS + fa] = eq ? rq : out;
+            if (okB) xout[rowBase + (int64_t)q * B.sS + fa + 1] = eq ? out : rq;
+        }
+        bPrevA = bA;
+        bPrevB = bB;
+        cPrevA = cA;
+        cPrevB = cB;
+        bA = nbA;
+        bB = nbB;
+        cA = ncA;
+        cB = ncB;
+    }
+}
+
 // Coarse cell (S, M, F) gathers the residual b - A x of its fine children (fine indices 2C-1, 2C along each
 // coarsened axis, clipped to the fine interior): bc = sum (R = P^T for the injection prolongation).
 __global__ __launch_bounds__(kBlockThreads) void k_residual_restrict(Level32 Lf, const float *__restrict__ x,
@@ -1044,6 +1237,256 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_update(Level32 L, float *_
     block_sum<1>(a, partials, gridDim.x, tr.lb);
 }
 
+// ---- 2.5D marching stencil kernels (3D boxes) ------------------------------------------------------------------
+// A block owns a 64 (fast) x 4 (mid) column of the interior and walks a chunk of slow planes, keeping the slow-axis
+// neighbours of its cells in registers: every plane of the field is read from memory once, and the in-plane
+// neighbours come from lines the block's own waves have just brought into L1.
+struct March
+{
+    int f, m, s0, s1, lb;
+    bool in;
+};
+
+__device__ __forceinline__ March march_setup(const Box &B, int P)
+{
+    March M;
+    M.lb = logical_block();
+    const int tf = M.lb % B.tilesF, t2 = M.lb / B.tilesF, tm = t2 % B.tilesM, chunk = t2 / B.tilesM;
+    M.f = 1 + tf * kBlockFast + (threadIdx.x & (kBlockFast - 1));
+    M.m = 1 + tm * kBlockMid + (threadIdx.x / kBlockFast);
+    M.s0 = 1 + chunk * P;
+    M.s1 = min(M.s0 + P - 1, B.ns - 2);
+    M.in = M.f <= B.nf - 2 && M.m <= B.nm - 2;
+    return M;
+}
+
+// (A x) of one cell in FP32 from its value and its six neighbours (slow+, slow-, mid+, mid-, fast+, fast-).
+__device__ __forceinline__ float apply32v(const Level32 &L, int s, int m, int f, int64_t idx, unsigned code, float xc,
+                                          const float (&nb)[6])
+{
+    float acc = 0.f;
+    if (fast_cell(L, code))
+    {
+#pragma unroll
+        for (int q = 0; q < 6; q++) acc += ((code >> q) & 1u) ? nb[q] : 0.f;
+        return L.cu * (acc - (float)__popc(code & 63u) * xc);
+    }
+    const Op32 o = op32(L, s, m, f, idx, code);
+#pragma unroll
+    for (int q = 0; q < 6; q++) acc += ((o.coupled >> q) & 1u) ? o.c[q] * nb[q] : 0.f;
+    return acc + o.diag * xc;
+}
+
+// (A x) of one cell in FP64 with the reference's exact arithmetic, from its value and neighbours (order as above):
+// the same sums in the same order as apply64 / row64.
+__device__ __forceinline__ double apply64v(const Box &B, const Tables64 &T, const uint8_t *ty, int s, int m, int f,
+                                           int64_t idx, double xc, const double (&nb)[6])
+{
+#pragma clang fp contract(off)
+    if (T.unitCoef)
+    {
+        const unsigned code = T.c8[idx];
+        if (!(code & 128u))
+        {
+            const double v = T.scale * 1.0;
+            const double dv = T.scale * (-(double)__popc(code & 63u));
+            double t = 0.0;
+            if (code & 2u) t += v * nb[1];
+            if (code & 8u) t += v * nb[3];
+            if (code & 32u) t += v * nb[5];
+            if (code & 63u) t += dv * xc;
+            if (code & 16u) t += v * nb[4];
+            if (code & 4u) t += v * nb[2];
+            if (code & 1u) t += v * nb[0];
+            return t;
+        }
+    }
+    const Op64 o = op64(B, T, ty, s, m, f, idx);
+    double t = 0.0;
+    if (o.coupled & 2u) t += o.v[1] * nb[1];
+    if (o.coupled & 8u) t += o.v[3] * nb[3];
+    if (o.coupled & 32u) t += o.v[5] * nb[5];
+    if (o.diag != 0.0) t += o.dval * xc;
+    if (o.coupled & 16u) t += o.v[4] * nb[4];
+    if (o.coupled & 4u) t += o.v[2] * nb[2];
+    if (o.coupled & 1u) t += o.v[0] * nb[0];
+    return t;
+}
+
+__global__ __launch_bounds__(kBlockThreads) void k_cg_spmv_m(Level32 L, const float *__restrict__ p,
+                                                             float *__restrict__ q, const SolveState *st,
+                                                             double *partials, int P)
+{
+    if (st->done) return;
+    const Box &B = L.B;
+    const March M = march_setup(B, P);
+    double pq = 0.0;
+    if (M.in)
+    {
+        int64_t idx = B.at(M.s0, M.m, M.f);
+        float pm = p[idx - B.sS], pc = p[idx];
+        for (int s = M.s0; s <= M.s1; s++, idx += B.sS)
+        {
+            const float pp = p[idx + B.sS];
+            const unsigned code = L.c8[idx];
+            if (code & 64u)
+            {
+                const float nb[6] = {pp, pm, p[idx + B.sM], p[idx - B.sM], p[idx + 1], p[idx - 1]};
+                const float v = apply32v(L, s, M.m, M.f, idx, code, pc, nb);
+                q[idx] = v;
+                pq += (double)pc * (double)v;
+            }
+            pm = pc;
+            pc = pp;
+        }
+    }
+    double a[1] = {pq};
+    block_sum<1>(a, partials, gridDim.x, M.lb);
+}
+
+// Residual of every fine cell, summed over each 2 x 2 x 2 family into the coarse right-hand side: fast pairs are
+// neighbouring lanes (shuffle), mid pairs neighbouring rows of the block (LDS), slow pairs consecutive planes of
+// the march (P even, chunks start on an odd plane). A family cut by the end of an odd interior keeps its 1 or 2
+// children (the same clipping as the coarse widths).
+__global__ __launch_bounds__(kBlockThreads) void k_residual_restrict_m(Level32 Lf, const float *__restrict__ x,
+                                                                       const float *__restrict__ b, Level32 Lc,
+                                                                       float *__restrict__ bc, const SolveState *st,
+                                                                       int P)
+{
+    if (st->done) return;
+    const Box &B = Lf.B;
+    const March M = march_setup(B, P);
+    __shared__ float sh[kBlockMid][kBlockFast];
+    const int tx = threadIdx.x & (kBlockFast - 1), ty = threadIdx.x / kBlockFast;
+    int64_t idx = B.at(M.s0, M.in ? M.m : 1, M.in ? M.f : 1);
+    float xm = 0.f, xc = 0.f;
+    if (M.in)
+    {
+        xm = x[idx - B.sS];
+        xc = x[idx];
+    }
+    float acc = 0.f;
+    for (int s = M.s0; s <= M.s1; s++, idx += B.sS)
+    {
+        float r = 0.f, xp = 0.f;
+        if (M.in)
+        {
+            xp = x[idx + B.sS];
+            const unsigned code = Lf.c8[idx];
+            if (code & 64u)
+            {
+                const float nb[6] = {xp, xm, x[idx + B.sM], x[idx - B.sM], x[idx + 1], x[idx - 1]};
+                r = b[idx] - apply32v(Lf, s, M.m, M.f, idx, code, xc, nb);
+            }
+        }
+        r += __shfl_xor(r, 1, 64);
+        sh[ty][tx] = r;
+        __syncthreads();
+        acc += r + sh[ty ^ 1][tx];
+        __syncthreads();
+        if (!(s & 1) || s == M.s1)
+        {
+            if (M.in && !(tx & 1) && !(ty & 1))
+                bc[Lc.B.at((s + 1) >> 1, (M.m + 1) >> 1, (M.f + 1) >> 1)] = acc;
+            acc = 0.f;
+        }
+        xm = xc;
+        xc = xp;
+    }
+}
+
+__global__ __launch_bounds__(kBlockThreads) void k_update_residual64_m(Box B, Tables64 T,
+                                                                       const uint8_t *__restrict__ ty,
+                                                                       const double *__restrict__ x,
+                                                                       const float *__restrict__ e,
+                                                                       double *__restrict__ xout,
+                                                                       const double *__restrict__ b,
+                                                                       float *__restrict__ r32,
+                                                                       double *__restrict__ r64, double *partials,
+                                                                       int P)
+{
+#pragma clang fp contract(off)
+    const March M = march_setup(B, P);
+    double rr = 0.0;
+    if (M.in)
+    {
+        auto X = [&](int64_t i) { return e ? x[i] + (double)e[i] : x[i]; };
+        int64_t idx = B.at(M.s0, M.m, M.f);
+        double xm = X(idx - B.sS), xc = X(idx);
+        for (int s = M.s0; s <= M.s1; s++, idx += B.sS)
+        {
+            const double xp = X(idx + B.sS);
+            double r = 0.0;
+            if (T.c8[idx] & 64u)
+            {
+                const double nb[6] = {xp, xm, X(idx + B.sM), X(idx - B.sM), X(idx + 1), X(idx - 1)};
+                r = b[idx] - apply64v(B, T, ty, s, M.m, M.f, idx, xc, nb);
+                if (e) xout[idx] = xc;
+            }
+            if (r32) r32[idx] = (float)r;
+            if (r64) r64[idx] = r;
+            rr += r * r;
+            xm = xc;
+            xc = xp;
+        }
+    }
+    double a[1] = {rr};
+    block_sum<1>(a, partials, gridDim.x, M.lb);
+}
+
+__global__ __launch_bounds__(kBlockThreads) void k_apply64_m(Box B, Tables64 T, const uint8_t *__restrict__ ty,
+                                                             const double *__restrict__ x, double *__restrict__ y,
+                                                             int P)
+{
+    const March M = march_setup(B, P);
+    if (!M.in) return;
+    int64_t idx = B.at(M.s0, M.m, M.f);
+    double xm = x[idx - B.sS], xc = x[idx];
+    for (int s = M.s0; s <= M.s1; s++, idx += B.sS)
+    {
+        const double xp = x[idx + B.sS];
+        double v = 0.0;
+        if (T.c8[idx] & 64u)
+        {
+            const double nb[6] = {xp, xm, x[idx + B.sM], x[idx - B.sM], x[idx + 1], x[idx - 1]};
+            v = apply64v(B, T, ty, s, M.m, M.f, idx, xc, nb);
+        }
+        y[idx] = v;
+        xm = xc;
+        xc = xp;
+    }
+}
+
+__global__ __launch_bounds__(kBlockThreads) void k_cg64_spmv_m(Box B, Tables64 T, const uint8_t *__restrict__ ty,
+                                                               const double *__restrict__ p, double *__restrict__ q,
+                                                               const SolveState *st, double *partials, int P)
+{
+    if (st->done) return;
+    const March M = march_setup(B, P);
+    double pq = 0.0;
+    if (M.in)
+    {
+        int64_t idx = B.at(M.s0, M.m, M.f);
+        double pm = p[idx - B.sS], pc = p[idx];
+        for (int s = M.s0; s <= M.s1; s++, idx += B.sS)
+        {
+            const double pp = p[idx + B.sS];
+            double v = 0.0;
+            if (T.c8[idx] & 64u)
+            {
+                const double nb[6] = {pp, pm, p[idx + B.sM], p[idx - B.sM], p[idx + 1], p[idx - 1]};
+                v = apply64v(B, T, ty, s, M.m, M.f, idx, pc, nb);
+                pq += pc * v;
+            }
+            q[idx] = v;
+            pm = pc;
+            pc = pp;
+        }
+    }
+    double a[1] = {pq};
+    block_sum<1>(a, partials, gridDim.x, M.lb);
+}
+
 __global__ void k_inner_setup(SolveState *st, double reduction, int maxIters)
 {
     const double r2 = st->sums[1];
@@ -1065,6 +1508,18 @@ __global__ void k_state_init(SolveState *st, double thr2, int maxIters)
 }
 
 inline int grid_of(const Box &B) { return std::min(B.blocks(), kMaxGrid); }
+
+// Planes per marching chunk: enough chunks for about 4096 blocks, at least 2 planes, even (the restriction pairs
+// planes), at most 64.
+inline int march_planes(const Box &B)
+{
+    const int cols = B.tilesF * B.tilesM;
+    const int chunks = std::max(1, (4096 + cols - 1) / cols);
+    int P = (B.ns - 2 + chunks - 1) / chunks;
+    P = std::max(2, std::min(64, P));
+    return (P + 1) & ~1;
+}
+inline int march_grid(const Box &B, int P) { return B.tilesF * B.tilesM * ((B.ns - 2 + P - 1) / P); }
 inline int grid_1d(int64_t n) { return (int)std::min<int64_t>((n + kBlockThreads - 1) / kBlockThreads, kMaxGrid); }
 
 hipError_t finalize(hipStream_t s, SolveState *st, const double *partials, int nblk, int nv, FinOp op)
@@ -1115,6 +1570,13 @@ hipError_t launch_update_residual64(hipStream_t s, const Box &B, const Tables64 
                                     const double *x, const float *e, double *xout, const double *b, float *r32,
                                     double *r64, SolveState *st, double *partials)
 {
+    if (B.threeD)
+    {
+        const int P = march_planes(B), G = march_grid(B, P);
+        k_update_residual64_m<<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, partials, P);
+        LAUNCH_OK();
+        return finalize(s, st, partials, G, 1, FIN_SUM1);
+    }
     const int G = grid_of(B);
     k_update_residual64<<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, partials);
     LAUNCH_OK();
@@ -1124,6 +1586,12 @@ hipError_t launch_update_residual64(hipStream_t s, const Box &B, const Tables64 
 hipError_t launch_apply64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const double *x,
                           double *y)
 {
+    if (B.threeD)
+    {
+        const int P = march_planes(B);
+        k_apply64_m<<<march_grid(B, P), kBlockThreads, 0, s>>>(B, T, ty, x, y, P);
+        return hipGetLastError();
+    }
     k_apply64<<<grid_of(B), kBlockThreads, 0, s>>>(B, T, ty, x, y);
     return hipGetLastError();
 }
@@ -1140,6 +1608,13 @@ hipError_t launch_cg64_init(hipStream_t s, const Box &B, const Tables64 &T, cons
 hipError_t launch_cg64_spmv(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const double *p,
                             double *q, SolveState *st, double *partials)
 {
+    if (B.threeD)
+    {
+        const int P = march_planes(B), G = march_grid(B, P);
+        k_cg64_spmv_m<<<G, kBlockThreads, 0, s>>>(B, T, ty, p, q, st, partials, P);
+        LAUNCH_OK();
+        return finalize(s, st, partials, G, 1, FIN_CG64_PQ);
+    }
     const int G = grid_of(B);
     k_cg64_spmv<<<G, kBlockThreads, 0, s>>>(B, T, ty, p, q, st, partials);
     LAUNCH_OK();
@@ -1166,6 +1641,27 @@ hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float 
                        int zero, int tileRows, int planesPerChunk, const SolveState *st)
 {
     const Box &B = L.B;
+    if (B.threeD)
+    {
+        const int TM = tileRows <= 4 ? 4 : (tileRows <= 6 ? 6 : 8);
+        const int tilesF = std::max(1, (B.nf - 2 + kPW - 1) / kPW);
+        const int tilesM = std::max(1, (B.nm - 2 + TM - 1) / TM);
+        const int P = std::max(1, planesPerChunk);
+        const int chunks = std::max(1, (B.ns - 2 + P - 1) / P);
+        const int grid = tilesF * tilesM * chunks;
+#define NPPE_RBGS_MARCH(TMv)                                                                                       \
+    (L.uniform ? (k_rbgs_march<TMv, true><<<grid, 64 * (TMv + 2), 0, s>>>(L, xin, xout, b, c1, zero, P, tilesF,    \
+                                                                          tilesM, st),                             \
+                  0)                                                                                               \
+               : (k_rbgs_march<TMv, false><<<grid, 64 * (TMv + 2), 0, s>>>(L, xin, xout, b, c1, zero, P, tilesF,   \
+                                                                           tilesM, st),                            \
+                  0))
+        if (TM == 4) NPPE_RBGS_MARCH(4);
+        else if (TM == 6) NPPE_RBGS_MARCH(6);
+        else NPPE_RBGS_MARCH(8);
+#undef NPPE_RBGS_MARCH
+        return hipGetLastError();
+    }
     const int tilesF = std::max(1, (B.nf - 2 + kTF - 1) / kTF);
     const int TM = tileRows == 8 ? 8 : 16;
     const int tilesM = std::max(1, (B.nm - 2 + TM - 1) / TM);
@@ -1182,6 +1678,12 @@ hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float 
 hipError_t launch_residual_restrict(hipStream_t s, const Level32 &Lf, const float *x, const float *b,
                                     const Level32 &Lc, float *bc, const SolveState *st)
 {
+    if (Lf.B.threeD)
+    {
+        const int P = march_planes(Lf.B);
+        k_residual_restrict_m<<<march_grid(Lf.B, P), kBlockThreads, 0, s>>>(Lf, x, b, Lc, bc, st, P);
+        return hipGetLastError();
+    }
     k_residual_restrict<<<grid_of(Lc.B), kBlockThreads, 0, s>>>(Lf, x, b, Lc, bc, st);
     return hipGetLastError();
 }
@@ -1221,6 +1723,13 @@ hipError_t launch_cg_direction(hipStream_t s, const Level32 &L, const float *z, 
 hipError_t launch_cg_spmv(hipStream_t s, const Level32 &L, const float *p, float *q, SolveState *st,
                           double *partials)
 {
+    if (L.B.threeD)
+    {
+        const int P = march_planes(L.B), G = march_grid(L.B, P);
+        k_cg_spmv_m<<<G, kBlockThreads, 0, s>>>(L, p, q, st, partials, P);
+        LAUNCH_OK();
+        return finalize(s, st, partials, G, 1, FIN_PQ);
+    }
     const int G = grid_of(L.B);
     k_cg_spmv<<<G, kBlockThreads, 0, s>>>(L, p, q, st, partials);
     LAUNCH_OK();

@@ -49,8 +49,8 @@ struct DevLevel
     float *tab = nullptr;
     Tables32 t32{};
     Level32 v{};
-    int tileRows = 8;        // smoother tile: 64 x tileRows cells, planesPerChunk planes per block
-    int planesPerChunk = 4;
+    int tileRows = 6;        // smoother tile: tileRows rows per block (3D: 128 cells wide), planesPerChunk planes
+    int planesPerChunk = 8;
     double cellsBox = 0.0;   // ns * nm * nf (the box, unpitched)
 };
 
@@ -452,7 +452,7 @@ int build(nppe_solver *h, const nppe_grid *g)
         L.v.c8 = L.c8;
         L.v.c16 = L.c16;
         L.v.uniform = H.uniform ? 1 : 0;
-        if (const char *e = std::getenv("NPPE_RBGS_TM")) L.tileRows = std::atoi(e) == 8 ? 8 : 16;
+        if (const char *e = std::getenv("NPPE_RBGS_TM")) L.tileRows = std::max(1, std::atoi(e));
         if (const char *e = std::getenv("NPPE_RBGS_P")) L.planesPerChunk = std::max(1, std::atoi(e));
         {
             // constant coefficient of a uniform level: sigma * area / gap = sigma * w (3D) or sigma (2D)
@@ -546,7 +546,14 @@ int build(nppe_solver *h, const nppe_grid *g)
     HIP_TRY(h, dalloc(h, &h->dCompB, nn), "allocate staging b");
     HIP_TRY(h, dalloc(h, &h->dCompX, nn), "allocate staging x");
     HIP_TRY(h, dalloc(h, &h->dCompG, nn), "allocate staging guess");
-    h->partialsCap = (int64_t)kMaxGrid * 4;   // every reduction launch has at most kMaxGrid blocks, <= 2 values
+    // Every reduction launch has at most max(kMaxGrid, columns x chunks) blocks (marching kernels use chunks of
+    // at least two planes) and at most two values per block.
+    {
+        int64_t most = kMaxGrid;
+        for (const DevLevel &L : h->levels)
+            most = std::max<int64_t>(most, (int64_t)L.box.tilesF * L.box.tilesM * ((L.box.ns - 2 + 1) / 2 + 1));
+        h->partialsCap = most * 2;
+    }
     HIP_TRY(h, dalloc(h, &h->dPartials, (size_t)h->partialsCap), "allocate partials");
     HIP_TRY(h, dalloc(h, &h->dState, 1), "allocate state");
     HIP_TRY(h, hipStreamSynchronize(h->stream), "commit");
