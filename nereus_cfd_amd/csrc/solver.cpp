@@ -88,6 +88,7 @@ struct nppe_solver
     double *dPartials = nullptr;
     int64_t partialsCap = 0;
     uint64_t bytesHeld = 0;
+    bool budgetHit = false;
     std::vector<void *> allocations;
 
     // profiling (nppe_profile.h)
@@ -118,6 +119,8 @@ int fail(nppe_solver *h, int code, const std::string &msg)
 
 int hip_fail(nppe_solver *h, hipError_t e, const char *what)
 {
+    if (h && h->budgetHit)
+        return fail(h, NPPE_ERR_BUDGET, std::string(what) + ": the HBM budget of the handle is exhausted");
     return fail(h, NPPE_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
@@ -132,7 +135,11 @@ template <class T>
 hipError_t dalloc(nppe_solver *h, T **p, size_t count)
 {
     const size_t bytes = std::max<size_t>(1, count) * sizeof(T);
-    if (h->opt.hbm_budget && h->bytesHeld + bytes > h->opt.hbm_budget) return hipErrorOutOfMemory;
+    if (h->opt.hbm_budget && h->bytesHeld + bytes > h->opt.hbm_budget)
+    {
+        h->budgetHit = true;
+        return hipErrorOutOfMemory;
+    }
     hipError_t e = hipMalloc((void **)p, bytes);
     if (e != hipSuccess) return e;
     h->allocations.push_back((void *)*p);
@@ -157,6 +164,7 @@ void free_grid(nppe_solver *h)
     h->dCoarseOrder = nullptr;
     h->dPartials = nullptr;
     h->committed = false;
+    h->budgetHit = false;
 }
 
 // ---- profiling -----------------------------------------------------------------------------------------------

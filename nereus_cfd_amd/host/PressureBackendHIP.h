@@ -1,0 +1,196 @@
+// PressureBackendHIP.h -- C++17 host side of the MI355X pressure backend, shaped like the reference's backend layer
+// (src/Methods/PPESolver/PressureSettings.h:483-624) so that NEREUS can route PressureBackend = HIP through it:
+//
+//   CreatePressureBackend   (PressureSettings.h:487-514) -> CreatePressureBackendHIP
+//   DestroyPressureBackend  (PressureSettings.h:516-538) -> DestroyPressureBackendHIP
+//   CommitPressureMatrix    (PressureSettings.h:542-574) -> CommitPressureMatrixHIP   (reads the MAC grid, not triplets)
+//   SolvePressureSystem     (PressureSettings.h:578-624) -> SolvePressureSystemHIP    (returns iterations)
+//   ReadEigenSettings       (PressureSettings.h:212-255) -> ReadHipSettings           (the hip_* keys)
+//
+// Behaviour mirrored from the reference: the solve returns its iteration count and a non-converged solve warns once
+// on stderr and still returns (PressureSettings.h:472-479); vectors are FP64 in IDP order; a zero-size system
+// returns 0 iterations. Errors of the C ABI (negative codes) throw std::runtime_error, the C++ analogue of
+// AMGX_SAFE_CALL's abort. Header-only; link with libnppe.so.
+#ifndef NEREUS_PRESSURE_BACKEND_HIP_H
+#define NEREUS_PRESSURE_BACKEND_HIP_H
+
+#include <cctype>
+#include <cstdint>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/nppe.h"
+
+namespace nereus_hip
+{
+
+// The backend's PRESSURE-block keys. They carry a hip_ prefix: an AMGX build passes every key that is not eigen_*
+// verbatim to AMGX (PressureSettings.h:158-173), so AmgxConfigString must skip hip_* keys as it skips eigen_*.
+struct HipSettings
+{
+    int method = NPPE_METHOD_MGPCG;   // hip_solver = MGPCG | CG
+    double tolerance = 1e-8;          // hip_tolerance
+    int maxIters = 2000;              // hip_max_iters
+    int preSmooth = 2;                // hip_pre_smooth
+    int postSmooth = 2;               // hip_post_smooth
+    double innerReduction = 1e-4;     // hip_inner_reduction
+    int device = 0;                   // hip_device
+};
+
+inline std::string Trim(const std::string &s)
+{
+    size_t a = 0, b = s.size();
+    while (a < b && std::isspace((unsigned char)s[a])) a++;
+    while (b > a && std::isspace((unsigned char)s[b - 1])) b--;
+    return s.substr(a, b - a);
+}
+
+// Splits the block on newlines and commas, drops blanks and # comments, reads the hip_* keys (values are
+// case-insensitive); unknown hip_* keys and bad values warn and keep the default, as ReadEigenSettings does.
+inline HipSettings ReadHipSettings(const std::string &text)
+{
+    HipSettings s;
+    std::string token;
+    for (size_t n = 0; n <= text.size(); n++)
+    {
+        const char c = n < text.size() ? text[n] : '\n';
+        if (c != ',' && c != '\n')
+        {
+            token.push_back(c);
+            continue;
+        }
+        const size_t hash = token.find('#');
+        if (hash != std::string::npos) token.erase(hash);
+        const std::string t = Trim(token);
+        token.clear();
+        const size_t eq = t.find('=');
+        if (t.empty() || eq == std::string::npos || t.rfind("hip_", 0) != 0) continue;
+        const std::string key = Trim(t.substr(0, eq));
+        std::string val = Trim(t.substr(eq + 1));
+        for (char &ch : val) ch = (char)std::toupper((unsigned char)ch);
+        try
+        {
+            if (key == "hip_solver")
+            {
+                if (val == "MGPCG") s.method = NPPE_METHOD_MGPCG;
+                else if (val == "CG") s.method = NPPE_METHOD_CG64;
+                else std::cerr << "Warning: unknown hip_solver '" << val << "', using MGPCG." << std::endl;
+            }
+            else if (key == "hip_tolerance") s.tolerance = std::stod(val);
+            else if (key == "hip_max_iters") s.maxIters = std::stoi(val);
+            else if (key == "hip_pre_smooth") s.preSmooth = std::stoi(val);
+            else if (key == "hip_post_smooth") s.postSmooth = std::stoi(val);
+            else if (key == "hip_inner_reduction") s.innerReduction = std::stod(val);
+            else if (key == "hip_device") s.device = std::stoi(val);
+            else std::cerr << "Warning: unknown pressure key '" << key << "'." << std::endl;
+        }
+        catch (...)
+        {
+            std::cerr << "Warning: could not read " << key << " = " << val << "." << std::endl;
+        }
+    }
+    if (!(s.tolerance > 0.0)) s.tolerance = 1e-8;
+    if (s.maxIters <= 0) s.maxIters = 2000;
+    if (s.preSmooth < 1) s.preSmooth = 2;
+    if (s.postSmooth < 0) s.postSmooth = 2;
+    if (!(s.innerReduction > 0.0 && s.innerReduction < 1.0)) s.innerReduction = 1e-4;
+    return s;
+}
+
+// The handle PressureSystem would hold next to its AMGXSolver* (PressureSettings.h:295): raw new/delete, as the
+// reference stores its AMGX handle.
+struct HipPressureSolver
+{
+    nppe_solver *h = nullptr;
+    int64_t n = 0;
+    bool singular = false;
+    bool warned = false;
+    nppe_result last{};
+};
+
+inline void Check(int rc, const nppe_solver *h, const char *what)
+{
+    if (rc < 0) throw std::runtime_error(std::string(what) + ": " + nppe_last_error(h));
+}
+
+inline HipPressureSolver *CreatePressureBackendHIP(const HipSettings &s, bool needsCompatibility)
+{
+    nppe_options o;
+    nppe_default_options(&o);
+    o.device = s.device;
+    o.method = s.method;
+    o.tolerance = s.tolerance;
+    o.max_iters = s.maxIters;
+    o.pre_smooth = s.preSmooth;
+    o.post_smooth = s.postSmooth;
+    o.inner_reduction = s.innerReduction;
+    o.compatibility = needsCompatibility ? 1 : 0;
+    HipPressureSolver *p = new HipPressureSolver();
+    const int rc = nppe_create(&p->h, &o);
+    if (rc != NPPE_OK)
+    {
+        delete p;
+        Check(rc, nullptr, "CreatePressureBackendHIP");
+    }
+    return p;
+}
+
+inline void DestroyPressureBackendHIP(HipPressureSolver *&p)
+{
+    if (!p) return;
+    nppe_destroy(p->h);
+    delete p;
+    p = nullptr;
+}
+
+// The MAC grid as CommitPressureMatrix sees it through its MAC* argument: MAC::Nx/Ny/Nz (ring included),
+// MAC::CellTypes() (MAC.h:167), MAC::Width(a, n) for every n (MAC.h:78), nonUniform and dh. Returns the number of
+// unknowns, which must equal the row count the reference assembled (PressureSolver3D.cpp:133-139).
+inline int64_t CommitPressureMatrixHIP(HipPressureSolver *p, int dim, int Nx, int Ny, int Nz, bool nonUniform,
+                                       double dh, const std::vector<unsigned char> &cellTypes,
+                                       const std::vector<double> &widthX, const std::vector<double> &widthY,
+                                       const std::vector<double> &widthZ)
+{
+    nppe_grid g{};
+    g.dim = dim;
+    g.nx = Nx;
+    g.ny = Ny;
+    g.nz = dim == 2 ? 1 : Nz;
+    g.non_uniform = nonUniform ? 1 : 0;
+    g.dh = dh;
+    g.cell_types = cellTypes.data();
+    g.width_x = widthX.data();
+    g.width_y = widthY.data();
+    g.width_z = dim == 3 ? widthZ.data() : nullptr;
+    const int64_t n = nppe_commit_grid(p->h, &g);
+    Check((int)(n < 0 ? n : 0), p->h, "CommitPressureMatrixHIP");
+    int32_t sing = 0;
+    nppe_system_info(p->h, &p->n, &sing);
+    p->singular = sing != 0;
+    return n;
+}
+
+// x = A^-1 b to tol * max(||b||, referenceNorm), from guess when not null; x is resized to n. Returns iterations.
+inline int SolvePressureSystemHIP(HipPressureSolver *p, const std::vector<double> &b, std::vector<double> &x,
+                                  const std::vector<double> *guess, double referenceNorm)
+{
+    x.resize((size_t)p->n);
+    if (p->n == 0) return 0;
+    if ((int64_t)b.size() != p->n) throw std::runtime_error("SolvePressureSystemHIP: b has the wrong size");
+    const int rc = nppe_solve(p->h, b.data(), x.data(), guess ? guess->data() : nullptr, referenceNorm, &p->last);
+    Check(rc, p->h, "SolvePressureSystemHIP");
+    if (rc == NPPE_NOT_CONVERGED && !p->warned)
+    {
+        p->warned = true;
+        std::cerr << "Warning: the HIP pressure solve did not converge (relative residual "
+                  << p->last.relative_residual << " after " << p->last.iterations
+                  << " iterations). Reported once." << std::endl;
+    }
+    return p->last.iterations;
+}
+
+}  // namespace nereus_hip
+
+#endif
