@@ -76,14 +76,14 @@ typedef struct nppe_options
     int32_t compatibility;   /* NEEDS_COMPATIBILITY_CONDITION: a singular system's solution is returned mean-free */
     int32_t pre_smooth;      /* red-black Gauss-Seidel sweeps before / after the coarse correction (default 2 / 2) */
     int32_t post_smooth;
-    double inner_reduction;  /* residual reduction each FP32 inner solve aims for (default 1e-4) */
+    double inner_reduction;  /* residual reduction each FP32 inner solve aims for at most (default 3e-5) */
     int32_t max_refinements; /* FP64 refinement passes (default 8) */
     int32_t reserved0;
     uint64_t hbm_budget;     /* bytes of device memory the handle may hold; 0 = no limit */
 } nppe_options;
 
 /* Fills *opt with the defaults: device 0, MGPCG, tolerance 1e-8, 2000 iterations, compatibility on, V(2,2),
- * inner reduction 1e-4, 8 refinement passes, no budget. */
+ * inner reduction 3e-5, 8 refinement passes, no budget. */
 void nppe_default_options(nppe_options *opt);
 
 /* The grid the reference hands CommitPressureMatrix through its MAC* (PressureSettings.h:542): box dimensions

@@ -1081,6 +1081,234 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_march(Level32 L, const f
     }
 }
 
+// ---- red-black Gauss-Seidel sweep, whole-row waves (3D uniform levels) -----------------------------------------
+//
+// As k_rbgs_march, but a wave spans the whole interior row: lane l owns the pairs (1 + 2p, 2 + 2p) for
+// p = l, l + 64, ... (K pairs), so no tile edge exists along the fast axis and no edge values are recomputed. The
+// block is TM rows plus one halo row above and below (halo waves publish first-colour values only). Plane bases are
+// scalar and per-lane offsets 32-bit; loads outside the interior are clamped into the box and their values masked
+// by the stencil codes, so the loop has no bounds branches. Uniform levels only (constant coefficients).
+template <int TM, int K, bool ZERO>
+__global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const float *__restrict__ xin,
+                                                            float *__restrict__ xout, const float *__restrict__ b,
+                                                            int c1, int P, int tilesM, const SolveState *st)
+{
+    if (st->done) return;
+    const Box &B = L.B;
+    constexpr int kCols = 128 * K + 2;
+    __shared__ float red[2][TM + 2][kCols];   // first-colour values by column f, rows m0-1 .. m0+TM
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lb = logical_block();
+    const int tm = lb % tilesM, chunk = lb / tilesM;
+    const int m0 = 1 + tm * TM;
+    const int m = m0 - 1 + wave;
+    const bool haloRow = wave == 0 || wave == TM + 1;
+    const bool rowIn = m >= 1 && m <= B.nm - 2;
+    const int mc = min(max(m, 0), B.nm - 1);
+    const int s0 = 1 + chunk * P, s1 = min(s0 + P - 1, B.ns - 2);
+    const uint32_t sM = (uint32_t)B.sM;
+    // per pair: first cell fa = 1 + 2p, clamped load column fc (fc + 1 stays in the box)
+    uint32_t off[K];
+    bool okA[K], okB[K];
+    int fa[K];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+    {
+        fa[k] = 1 + 2 * (lane + 64 * k);
+        okA[k] = rowIn && fa[k] <= B.nf - 2;
+        okB[k] = rowIn && fa[k] + 1 <= B.nf - 2;
+        const int fc = min(fa[k], B.nf - 2);
+        off[k] = (uint32_t)mc * sM + (uint32_t)fc;
+    }
+    for (int i = threadIdx.x; i < 2 * (TM + 2) * kCols; i += 64 * (TM + 2)) (&red[0][0][0])[i] = 0.f;
+    auto plane = [&](const float *base, int pl) { return base + (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
+    auto planeC = [&](int pl) { return L.c8 + (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
+    float xa[4][K], xb[4][K];
+    float bA[K], bB[K], bPA[K], bPB[K];
+    unsigned cA[K], cB[K], cPA[K], cPB[K];
+    float R[3][K];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+    {
+        bPA[k] = bPB[k] = 0.f;
+        cPA[k] = cPB[k] = 0u;
+        R[0][k] = R[1][k] = R[2][k] = 0.f;
+#pragma unroll
+        for (int j = 0; j < 4; j++) xa[j][k] = xb[j][k] = 0.f;
+    }
+    if (!ZERO)
+    {
+#pragma unroll
+        for (int j = 1; j < 4; j++)
+        {
+            const float *px = plane(xin, s0 - 3 + j);
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                xa[j][k] = px[off[k]];
+                xb[j][k] = px[off[k] + 1];
+            }
+        }
+    }
+    {
+        const int pl = s0 - 1;
+        const bool in = pl >= 1 && pl <= B.ns - 2;
+        const float *pb = plane(b, pl);
+        const uint8_t *pc = planeC(pl);
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            bA[k] = pb[off[k]];
+            bB[k] = pb[off[k] + 1];
+            cA[k] = (in && okA[k]) ? pc[off[k]] : 0u;
+            cB[k] = (in && okB[k]) ? pc[off[k] + 1] : 0u;
+        }
+    }
+    __syncthreads();
+    const float icu = L.icu;
+    for (int sp = s0 - 1; sp <= s1 + 1; sp++)
+    {
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+            {
+                xa[j][k] = xa[j + 1][k];
+                xb[j][k] = xb[j + 1][k];
+            }
+            R[0][k] = R[1][k];
+            R[1][k] = R[2][k];
+        }
+        // prefetch: old x of plane sp+2; b and codes of plane sp+1
+        float nbA[K], nbB[K];
+        unsigned ncA[K], ncB[K];
+        {
+            const float *px = plane(xin, sp + 2);
+            const float *pb = plane(b, sp + 1);
+            const uint8_t *pc = planeC(sp + 1);
+            const bool in = sp + 1 >= 1 && sp + 1 <= B.ns - 2;
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                xa[3][k] = ZERO ? 0.f : px[off[k]];
+                xb[3][k] = ZERO ? 0.f : px[off[k] + 1];
+                nbA[k] = pb[off[k]];
+                nbB[k] = pb[off[k] + 1];
+                ncA[k] = (in && okA[k]) ? pc[off[k]] : 0u;
+                ncB[k] = (in && okB[k]) ? pc[off[k] + 1] : 0u;
+            }
+        }
+        const int e1 = (c1 - sp - m - 1) & 1;   // first-colour element of every pair on this row and plane
+        const float *pxs = plane(xin, sp);
+        float rnew[K];
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            // in-row old neighbours: left of fa is the previous pair's b element, right of fa+1 the next pair's a
+            float left = __shfl_up(xb[1][k], 1, 64);
+            float right = __shfl_down(xa[1][k], 1, 64);
+            if (K > 1)
+            {
+                const float wrapL = k > 0 ? __shfl(xb[1][k > 0 ? k - 1 : 0], 63, 64) : 0.f;
+                const float wrapR = k + 1 < K ? __shfl(xa[1][k + 1 < K ? k + 1 : k], 0, 64) : 0.f;
+                if (lane == 0) left = wrapL;
+                if (lane == 63) right = wrapR;
+            }
+            const unsigned code = e1 ? cB[k] : cA[k];
+            float v = 0.f;
+            if (code & 64u)
+            {
+                const uint32_t o = off[k] + (uint32_t)e1;
+                float acc = 0.f;
+                acc += (code & 1u) ? (e1 ? xb[2][k] : xa[2][k]) : 0.f;
+                acc += (code & 2u) ? (e1 ? xb[0][k] : xa[0][k]) : 0.f;
+                if (!ZERO)
+                {
+                    acc += (code & 4u) ? pxs[o + sM] : 0.f;
+                    acc += (code & 8u) ? pxs[o - sM] : 0.f;
+                }
+                acc += (code & 16u) ? (e1 ? right : xb[1][k]) : 0.f;
+                acc += (code & 32u) ? (e1 ? xa[1][k] : left) : 0.f;
+                const float bv = e1 ? bB[k] : bA[k];
+                if (!(code & 128u))
+                    v = (acc - bv * icu) * kInvN[__popc(code & 63u)];
+                else
+                {
+                    const unsigned c16 = L.c16[(size_t)min(max(sp, 0), B.ns - 1) * (size_t)B.sS + o];
+                    float d = 0.f;
+#pragma unroll
+                    for (int q = 0; q < 6; q++)
+                    {
+                        const unsigned stt = (c16 >> (2 * q)) & 3u;
+                        d += stt == 3u ? 2.f : (stt != 0u ? 1.f : 0.f);
+                    }
+                    v = d != 0.f ? (acc - bv * icu) / d : 0.f;
+                }
+            }
+            rnew[k] = v;
+            R[2][k] = v;
+        }
+        __syncthreads();   // previous readers of red[sp & 1] (plane sp-2) are done; red[q & 1] is complete
+#pragma unroll
+        for (int k = 0; k < K; k++) red[sp & 1][wave][fa[k] + e1] = rnew[k];
+        __syncthreads();
+        const int q = sp - 1;
+        if (!haloRow && q >= s0 && q <= s1)
+        {
+            float *po = xout + (size_t)q * (size_t)B.sS;
+            const float(*rq)[kCols] = red[q & 1];
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                const int eq = e1;   // second-colour element at plane q
+                const unsigned code = eq ? cPB[k] : cPA[k];
+                float out = 0.f;
+                if (code & 64u)
+                {
+                    const int f = fa[k] + eq;
+                    float acc = 0.f;
+                    acc += (code & 1u) ? R[2][k] : 0.f;
+                    acc += (code & 2u) ? R[0][k] : 0.f;
+                    acc += (code & 4u) ? rq[wave + 1][f] : 0.f;
+                    acc += (code & 8u) ? rq[wave - 1][f] : 0.f;
+                    acc += (code & 16u) ? (eq ? rq[wave][f + 1] : R[1][k]) : 0.f;
+                    acc += (code & 32u) ? (eq ? R[1][k] : rq[wave][f - 1]) : 0.f;
+                    const float bv = eq ? bPB[k] : bPA[k];
+                    if (!(code & 128u))
+                        out = (acc - bv * icu) * kInvN[__popc(code & 63u)];
+                    else
+                    {
+                        const unsigned c16 = L.c16[(size_t)q * (size_t)B.sS + off[k] + (uint32_t)eq];
+                        float d = 0.f;
+#pragma unroll
+                        for (int qq = 0; qq < 6; qq++)
+                        {
+                            const unsigned stt = (c16 >> (2 * qq)) & 3u;
+                            d += stt == 3u ? 2.f : (stt != 0u ? 1.f : 0.f);
+                        }
+                        out = d != 0.f ? (acc - bv * icu) / d : 0.f;
+                    }
+                }
+                if (okA[k]) po[off[k]] = eq ? R[1][k] : out;
+                if (okB[k]) po[off[k] + 1] = eq ? out : R[1][k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            bPA[k] = bA[k];
+            bPB[k] = bB[k];
+            cPA[k] = cA[k];
+            cPB[k] = cB[k];
+            bA[k] = nbA[k];
+            bB[k] = nbB[k];
+            cA[k] = ncA[k];
+            cB[k] = ncB[k];
+        }
+    }
+}
+
 // Coarse cell (S, M, F) gathers the residual b - A x of its fine children (fine indices 2C-1, 2C along each
 // coarsened axis, clipped to the fine interior): bc = sum (R = P^T for the injection prolongation).
 __global__ __launch_bounds__(kBlockThreads) void k_residual_restrict(Level32 Lf, const float *__restrict__ x,
@@ -1641,6 +1869,24 @@ hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float 
                        int zero, int tileRows, int planesPerChunk, const SolveState *st)
 {
     const Box &B = L.B;
+    if (B.threeD && L.uniform && B.nf - 2 <= 512)
+    {
+        const int K = B.nf - 2 <= 128 ? 1 : (B.nf - 2 <= 256 ? 2 : 4);
+        const int TM = 6;
+        const int tilesM = std::max(1, (B.nm - 2 + TM - 1) / TM);
+        const int P = std::max(1, planesPerChunk);
+        const int chunks = std::max(1, (B.ns - 2 + P - 1) / P);
+        const int grid = tilesM * chunks;
+        const int threads = 64 * (TM + 2);
+#define NPPE_ROW(Kv)                                                                                                 \
+    (zero ? (k_rbgs_row<6, Kv, true><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st), 0)                  \
+          : (k_rbgs_row<6, Kv, false><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st), 0))
+        if (K == 1) NPPE_ROW(1);
+        else if (K == 2) NPPE_ROW(2);
+        else NPPE_ROW(4);
+#undef NPPE_ROW
+        return hipGetLastError();
+    }
     if (B.threeD)
     {
         const int TM = tileRows <= 4 ? 4 : (tileRows <= 6 ? 6 : 8);

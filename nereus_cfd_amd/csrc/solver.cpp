@@ -89,6 +89,7 @@ struct nppe_solver
     int64_t partialsCap = 0;
     uint64_t bytesHeld = 0;
     bool budgetHit = false;
+    int coarseNu = 0;   // sweeps on levels >= 2 (0: the same as the fine levels)
     std::vector<void *> allocations;
 
     // profiling (nppe_profile.h)
@@ -221,16 +222,17 @@ double lvl_cells(const DevLevel &L) { return L.cellsBox; }
 hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel)
 {
     const int nl = (int)h->levels.size();
-    const int nu1 = std::max(1, h->opt.pre_smooth), nu2 = std::max(0, h->opt.post_smooth);
     const double s = 4.0, f = 1.0;
     hipError_t e = hipSuccess;
     int cur[kMaxLevels] = {};
+    const int coarseNu = h->coarseNu;
     auto xbuf = [&](int l, int k) -> float * { return l == 0 ? (k ? h->dZ2 : h->dZ) : h->levels[l].x[k]; };
     auto bbuf = [&](int l) -> const float * { return l == 0 ? b0 : h->levels[l].b; };
     for (int l = 0; l < nl - 1; l++)
     {
         DevLevel &L = h->levels[l];
         const double N = L.cellsBox;
+        const int nu1 = (coarseNu > 0 && l >= 2) ? coarseNu : std::max(1, h->opt.pre_smooth);
         for (int sw = 0; sw < nu1; sw++)
         {
             const bool zero = sw == 0;
@@ -296,6 +298,7 @@ hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel)
                 return e;
         }
         tr.add(pb);
+        const int nu2 = (coarseNu > 0 && l >= 2) ? coarseNu : std::max(0, h->opt.post_smooth);
         for (int sw = 0; sw < nu2; sw++)
         {
             const double by = (3 * s + f) * N;
@@ -412,6 +415,7 @@ int build(nppe_solver *h, const nppe_grid *g)
         if (c.unknowns == 0) break;
         hl.push_back(std::move(c));
     }
+    h->coarseNu = std::getenv("NPPE_COARSE_NU") ? std::atoi(std::getenv("NPPE_COARSE_NU")) : 0;
     h->levels.resize(hl.size());
     for (size_t l = 0; l < hl.size(); l++)
     {
@@ -460,6 +464,12 @@ int build(nppe_solver *h, const nppe_grid *g)
         L.v.c8 = L.c8;
         L.v.c16 = L.c16;
         L.v.uniform = H.uniform ? 1 : 0;
+        {
+            // planes per marching chunk: about 1024 blocks of 6-row tiles, between 1 and 4 planes
+            const int rows = std::max(1, (L.box.nm - 2 + 5) / 6);
+            const int planes = L.box.threeD ? L.box.ns - 2 : 1;
+            L.planesPerChunk = std::max(1, std::min(4, (int)((int64_t)planes * rows / 1024)));
+        }
         if (const char *e = std::getenv("NPPE_RBGS_TM")) L.tileRows = std::max(1, std::atoi(e));
         if (const char *e = std::getenv("NPPE_RBGS_P")) L.planesPerChunk = std::max(1, std::atoi(e));
         {
@@ -894,7 +904,7 @@ void nppe_default_options(nppe_options *o)
     o->compatibility = 1;
     o->pre_smooth = 2;
     o->post_smooth = 2;
-    o->inner_reduction = 1e-4;
+    o->inner_reduction = 3e-5;
     o->max_refinements = 8;
     o->hbm_budget = 0;
 }

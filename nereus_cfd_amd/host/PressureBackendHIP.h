@@ -35,7 +35,7 @@ struct HipSettings
     int maxIters = 2000;              // hip_max_iters
     int preSmooth = 2;                // hip_pre_smooth
     int postSmooth = 2;               // hip_post_smooth
-    double innerReduction = 1e-4;     // hip_inner_reduction
+    double innerReduction = 3e-5;     // hip_inner_reduction
     int device = 0;                   // hip_device
 };
 
@@ -95,7 +95,7 @@ inline HipSettings ReadHipSettings(const std::string &text)
     if (s.maxIters <= 0) s.maxIters = 2000;
     if (s.preSmooth < 1) s.preSmooth = 2;
     if (s.postSmooth < 0) s.postSmooth = 2;
-    if (!(s.innerReduction > 0.0 && s.innerReduction < 1.0)) s.innerReduction = 1e-4;
+    if (!(s.innerReduction > 0.0 && s.innerReduction < 1.0)) s.innerReduction = 3e-5;
     return s;
 }
 
