@@ -161,6 +161,42 @@ int nppe_apply(nppe_solver *h, const double *x, double *y);
 /* Device bytes the handle holds. */
 int nppe_memory_bytes(const nppe_solver *h, uint64_t *bytes);
 
+/* Changes the relative tolerance of later solves (a per-call eigen_tolerance; PressureSettings.h:369-371 sets the
+ * AMGCL tolerance per call the same way). */
+int nppe_set_tolerance(nppe_solver *h, double tolerance);
+
+/* Upper estimate of the device bytes nppe_commit_grid will allocate for g, for admission against an HBM budget
+ * before any allocation. 0 for an invalid grid. */
+uint64_t nppe_estimate_bytes(const nppe_grid *g, int32_t method);
+
+/* ---- many systems at once (no counterpart in the reference, which solves one system per process) ------------- */
+
+/* Solves count independent committed systems concurrently: each handle runs on its own stream, driven by
+ * `workers` host threads (0: one per system, at most 64). b[i], x[i] are host (device == 0) or device
+ * (device == 1) FP64 vectors in IDP order of hs[i]; results go to res[i] (may be NULL). Returns NPPE_OK when every
+ * solve returned NPPE_OK, the first negative code otherwise, or NPPE_NOT_CONVERGED. Each result is bit-identical to
+ * the same solve run alone. */
+int nppe_batch_solve(nppe_solver *const *hs, const double *const *b, double *const *x, int32_t count,
+                     int32_t workers, int32_t device, nppe_result *res);
+
+/* A solve server: requests arrive at any time, are admitted against an HBM budget into `slots` concurrent solver
+ * slots as slots and memory free up, and run concurrently on the device (continuous batching). Grids are
+ * registered once under a key and stay resident (committed) between requests while the budget allows; idle
+ * resident grids are evicted least recently used. */
+typedef struct nppe_server nppe_server;
+int nppe_server_create(nppe_server **out, const nppe_options *opt, int32_t slots, uint64_t hbm_budget);
+int nppe_server_destroy(nppe_server *s);
+/* Registers (copies) a grid under key. */
+int nppe_server_add_grid(nppe_server *s, uint64_t key, const nppe_grid *g);
+/* Queues a solve of the grid registered under key with right-hand side b (copied; IDP order) to the given relative
+ * tolerance; the solution is written to x (host, caller-owned, must stay valid until the request is waited on).
+ * Returns a ticket (>= 0) or an error. */
+int64_t nppe_server_submit(nppe_server *s, uint64_t key, const double *b, double tolerance, double *x);
+/* Blocks until the request is done; its result, and its latency from submit to completion in milliseconds. */
+int nppe_server_wait(nppe_server *s, int64_t ticket, nppe_result *res, double *latency_ms);
+/* Device bytes the server holds now, and the most it has held. */
+int nppe_server_memory(const nppe_server *s, uint64_t *now, uint64_t *peak);
+
 #ifdef __cplusplus
 }
 #endif

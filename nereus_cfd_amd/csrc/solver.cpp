@@ -90,6 +90,9 @@ struct nppe_solver
     uint64_t bytesHeld = 0;
     bool budgetHit = false;
     int coarseNu = 0;   // sweeps on levels >= 2 (0: the same as the fine levels)
+    bool useGraphs = true;                 // replay the inner iterations from a captured hipGraph
+    hipGraphExec_t innerGraph = nullptr;   // kInnerChunk inner iterations, captured at the first solve
+    double graphBytesPerIter = 0.0;
     std::vector<void *> allocations;
 
     // profiling (nppe_profile.h)
@@ -150,6 +153,11 @@ hipError_t dalloc(nppe_solver *h, T **p, size_t count)
 
 void free_grid(nppe_solver *h)
 {
+    if (h->innerGraph)
+    {
+        (void)hipGraphExecDestroy(h->innerGraph);
+        h->innerGraph = nullptr;
+    }
     for (void *p : h->allocations) (void)hipFree(p);
     h->allocations.clear();
     h->bytesHeld = 0;
@@ -416,6 +424,7 @@ int build(nppe_solver *h, const nppe_grid *g)
         hl.push_back(std::move(c));
     }
     h->coarseNu = std::getenv("NPPE_COARSE_NU") ? std::atoi(std::getenv("NPPE_COARSE_NU")) : 0;
+    h->useGraphs = !(std::getenv("NPPE_GRAPHS") && std::atoi(std::getenv("NPPE_GRAPHS")) == 0);
     h->levels.resize(hl.size());
     for (size_t l = 0; l < hl.size(); l++)
     {
@@ -669,15 +678,40 @@ int solve_mgpcg(nppe_solver *h, const double *dB, double *dX, const double *dGue
         HIP_TRY(h, launch_inner_setup(h->stream, h->dState, red, h->opt.max_iters - inner), "inner setup");
         int launched = 0;
         double perIter = 0.0;
+        const bool graphs = h->useGraphs && !h->profiling;
+        if (graphs && !h->innerGraph)
+        {
+            // Capture kInnerChunk iterations once: the launches, their arguments and the z ping-pong parity are the
+            // same every time, and every kernel reads the device-side done flag, so one graph serves every chunk.
+            hipGraph_t g = nullptr;
+            Traffic it;
+            HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+            hipError_t ce = hipSuccess;
+            for (int q = 0; q < kInnerChunk && ce == hipSuccess; q++) ce = inner_iteration(h, it);
+            hipError_t ee = hipStreamEndCapture(h->stream, &g);
+            if (ce != hipSuccess) return hip_fail(h, ce, "graph capture");
+            if (ee != hipSuccess) return hip_fail(h, ee, "graph capture");
+            hipError_t ie = hipGraphInstantiate(&h->innerGraph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) return hip_fail(h, ie, "graph instantiate");
+            h->graphBytesPerIter = it.bytes / kInnerChunk;
+        }
         for (;;)
         {
-            for (int q = 0; q < kInnerChunk; q++)
+            if (graphs)
             {
-                h->curIter = launched++;
-                Traffic it;
-                HIP_TRY(h, inner_iteration(h, it), "inner iteration");
-                perIter = it.bytes;
+                HIP_TRY(h, hipGraphLaunch(h->innerGraph, h->stream), "graph launch");
+                launched += kInnerChunk;
+                perIter = h->graphBytesPerIter;
             }
+            else
+                for (int q = 0; q < kInnerChunk; q++)
+                {
+                    h->curIter = launched++;
+                    Traffic it;
+                    HIP_TRY(h, inner_iteration(h, it), "inner iteration");
+                    perIter = it.bytes;
+                }
             rc = sync_state(h);
             if (rc) return rc;
             if (h->hState->done) break;
@@ -1114,6 +1148,41 @@ int nppe_check_csr(nppe_solver *h, int32_t n, int64_t nnz, const int32_t *rowptr
         }
     }
     return NPPE_OK;
+}
+
+int nppe_set_tolerance(nppe_solver *h, double tolerance)
+{
+    if (!h || !(tolerance > 0.0)) return NPPE_ERR_ARGUMENT;
+    h->opt.tolerance = tolerance;
+    return NPPE_OK;
+}
+
+uint64_t nppe_estimate_bytes(const nppe_grid *g, int32_t method)
+{
+    if (!g || (g->dim != 2 && g->dim != 3) || g->nx < 3 || g->ny < 3) return 0;
+    const bool t3 = g->dim == 3;
+    int ns = t3 ? g->ny : 1, nm = t3 ? g->nx : g->ny, nf = t3 ? g->nz : g->nx;
+    double total = 0.0;
+    for (int l = 0; l < kMaxLevels; l++)
+    {
+        const double cells = (double)ns * nm * ((nf + 15) / 16 * 16);
+        // types, code8, code16, tables; FP32 iterate pair and right-hand side on coarse levels
+        total += cells * (1 + 1 + 2) + (l > 0 ? cells * 12.0 : 0.0) + 4096.0;
+        if (l == 0)
+        {
+            total += cells * 24.0;                                      // x64 pair, b64
+            total += cells * (method == NPPE_METHOD_CG64 ? 24.0 : 24.0);   // FP32 CG vectors / FP64 r, p, q
+            total += (double)(ns - (t3 ? 2 : 0)) * (nm - 2) * (nf - 2) * (24.0 + 4.0);   // staging + order
+        }
+        const int si = ns - 2, mi = nm - 2, fi = nf - 2;
+        const double interior = (t3 ? (double)si : 1.0) * mi * fi;
+        if (interior <= 512.0) break;
+        if (t3) ns = (si + 1) / 2 + 2;
+        nm = (mi + 1) / 2 + 2;
+        nf = (fi + 1) / 2 + 2;
+    }
+    total += (double)kMaxDenseCoarse * kMaxDenseCoarse * 8.0 / 4.0 + 1.0e6;   // coarse matrix, partials, state
+    return (uint64_t)(total * 1.05);
 }
 
 int nppe_memory_bytes(const nppe_solver *h, uint64_t *bytes)

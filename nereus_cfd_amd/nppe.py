@@ -32,6 +32,8 @@ EXPORTS = [
     "nppe_default_options", "nppe_create", "nppe_destroy", "nppe_last_error", "nppe_commit_grid",
     "nppe_system_info", "nppe_unknown_order", "nppe_check_csr", "nppe_solve", "nppe_solve_device", "nppe_apply",
     "nppe_memory_bytes", "nppe_profile_enable", "nppe_profile_read", "nppe_synth_cosine_rhs",
+    "nppe_set_tolerance", "nppe_estimate_bytes", "nppe_batch_solve", "nppe_server_create", "nppe_server_destroy",
+    "nppe_server_add_grid", "nppe_server_submit", "nppe_server_wait", "nppe_server_memory",
 ]
 
 
@@ -112,6 +114,24 @@ def lib():
     L.nppe_profile_read.restype = C.c_int
     L.nppe_synth_cosine_rhs.argtypes = [P, C.c_uint64, I32, D, I32, P]
     L.nppe_synth_cosine_rhs.restype = I64
+    L.nppe_set_tolerance.argtypes = [P, D]
+    L.nppe_set_tolerance.restype = C.c_int
+    L.nppe_estimate_bytes.argtypes = [P, I32]
+    L.nppe_estimate_bytes.restype = C.c_uint64
+    L.nppe_batch_solve.argtypes = [P, P, P, I32, I32, I32, P]
+    L.nppe_batch_solve.restype = C.c_int
+    L.nppe_server_create.argtypes = [P, P, I32, C.c_uint64]
+    L.nppe_server_create.restype = C.c_int
+    L.nppe_server_destroy.argtypes = [P]
+    L.nppe_server_destroy.restype = C.c_int
+    L.nppe_server_add_grid.argtypes = [P, C.c_uint64, P]
+    L.nppe_server_add_grid.restype = C.c_int
+    L.nppe_server_submit.argtypes = [P, C.c_uint64, P, D, P]
+    L.nppe_server_submit.restype = I64
+    L.nppe_server_wait.argtypes = [P, I64, P, P]
+    L.nppe_server_wait.restype = C.c_int
+    L.nppe_server_memory.argtypes = [P, P, P]
+    L.nppe_server_memory.restype = C.c_int
     _lib = L
     return L
 
@@ -274,6 +294,9 @@ class Solver:
         self._check(lib().nppe_memory_bytes(self.h, C.byref(v)), "memory_bytes")
         return int(v.value)
 
+    def set_tolerance(self, tol: float):
+        self._check(lib().nppe_set_tolerance(self.h, float(tol)), "set_tolerance")
+
     def profile(self, on: bool = True):
         self._check(lib().nppe_profile_enable(self.h, int(on)), "profile_enable")
 
@@ -285,3 +308,91 @@ class Solver:
             if cnt.value:
                 out[name] = {"launches": int(cnt.value), "ms": ms.value, "bytes": by.value}
         return out
+
+
+def estimate_bytes(grid: Grid, method: int = METHOD_MGPCG) -> int:
+    d = grid.desc()
+    return int(lib().nppe_estimate_bytes(C.byref(d), method))
+
+
+def batch_solve(solvers, bs, workers=0):
+    """Solves every (solver, b) pair concurrently (host vectors); returns [(x, Result)], rc."""
+    n = len(solvers)
+    hs = (C.c_void_p * n)(*[s.h for s in solvers])
+    bs = [np.ascontiguousarray(b, np.float64) for b in bs]
+    xs = [np.zeros(max(1, s.n), np.float64) for s in solvers]
+    bp = (C.c_void_p * n)(*[b.ctypes.data for b in bs])
+    xp = (C.c_void_p * n)(*[x.ctypes.data for x in xs])
+    res = (Result * n)()
+    rc = lib().nppe_batch_solve(C.cast(hs, C.c_void_p), C.cast(bp, C.c_void_p), C.cast(xp, C.c_void_p), n, workers,
+                                0, C.cast(res, C.c_void_p))
+    if rc < 0:
+        raise NppeError(rc, "batch_solve")
+    return [(xs[i][: solvers[i].n], res[i]) for i in range(n)], rc
+
+
+def batch_solve_device(solvers, b_ptrs, x_ptrs, workers=0):
+    """As batch_solve with device pointers (e.g. torch data_ptr()); returns [Result], rc."""
+    n = len(solvers)
+    hs = (C.c_void_p * n)(*[s.h for s in solvers])
+    bp = (C.c_void_p * n)(*b_ptrs)
+    xp = (C.c_void_p * n)(*x_ptrs)
+    res = (Result * n)()
+    rc = lib().nppe_batch_solve(C.cast(hs, C.c_void_p), C.cast(bp, C.c_void_p), C.cast(xp, C.c_void_p), n, workers,
+                                1, C.cast(res, C.c_void_p))
+    if rc < 0:
+        raise NppeError(rc, "batch_solve_device")
+    return list(res), rc
+
+
+class Server:
+    """nppe_server: register grids, submit solve requests at any time, wait for them (continuous batching)."""
+
+    def __init__(self, slots=4, hbm_budget=0, **opts):
+        self.opts = default_options(**opts)
+        h = C.c_void_p()
+        rc = lib().nppe_server_create(C.byref(h), C.byref(self.opts), slots, hbm_budget)
+        if rc != OK:
+            raise NppeError(rc, "server_create")
+        self.h = h
+        self._pending = {}
+
+    def add_grid(self, key: int, grid: Grid):
+        d = grid.desc()
+        rc = lib().nppe_server_add_grid(self.h, key, C.byref(d))
+        if rc != OK:
+            raise NppeError(rc, "server_add_grid")
+
+    def submit(self, key: int, b, tolerance: float) -> int:
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(max(1, len(b)), np.float64)
+        t = lib().nppe_server_submit(self.h, key, b.ctypes.data if len(b) else None, float(tolerance), x.ctypes.data)
+        if t < 0:
+            raise NppeError(int(t), "server_submit")
+        self._pending[int(t)] = (x, len(b))
+        return int(t)
+
+    def wait(self, ticket: int):
+        res = Result()
+        lat = C.c_double()
+        rc = lib().nppe_server_wait(self.h, ticket, C.byref(res), C.byref(lat))
+        x, n = self._pending.pop(ticket)
+        if rc < 0:
+            raise NppeError(rc, "server_wait")
+        return x[:n], res, float(lat.value), rc
+
+    def memory(self):
+        now, peak = C.c_uint64(), C.c_uint64()
+        lib().nppe_server_memory(self.h, C.byref(now), C.byref(peak))
+        return int(now.value), int(peak.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().nppe_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -1,0 +1,148 @@
+"""Seeded synthetic pressure systems for the benchmark configurations (no counterpart in the reference, whose systems
+come from a running simulation). Every generator returns nppe Grids in the reference's MAC conventions: cell types
+indexed (i, j, k) = (y, x, z) with the one-cell ring, and width tables with the ring cells mirroring their
+neighbours (MAC.cpp:150-161).
+
+* stretched_channel: 3D channel 256 x 128 x 128 cells (x, y, z), a cylinder of radius 0.05 (the channel height is 1)
+  across z, voxelised from its signed distance at the cell centres as SOLID cells; widths graded geometrically
+  towards the cylinder with a neighbour ratio of at most 1.05; inflow (Neumann) at x-min, a p = 0 outlet (EMPTY ring)
+  at x-max, SOLID walls elsewhere; the operator is the stretched (volume-weighted) one.
+* flip_free_surface: a 3D box of uniform cells with SOLID walls, a liquid dam-break block filling 30-50 % of the
+  volume plus 0-200 liquid spheres of radius 2-6 cells; every other interior cell is EMPTY (p = 0, no fractional
+  ghost fluid, as PressureSolver3D.cpp:162-167).
+* serving_mix: request descriptions for the serving stream (2D 256^2-2048^2 or 3D 64^3-256^3; Neumann, Dirichlet
+  outlet or free-surface; tolerance 1e-6 .. 1e-10).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .nppe import Grid, FLUID, SOLID, EMPTY, INFLOW
+
+
+def graded_widths(n: int, length: float, centre: float, ratio: float = 1.05) -> np.ndarray:
+    """n interior widths summing to length, smallest at `centre` and growing geometrically (at most `ratio` between
+    neighbours) away from it, capped where the cap keeps the sum; ring widths mirror the outer cells."""
+    idx = np.arange(n)
+    # cell nearest to the centre under a uniform guess
+    ic = int(np.clip(round(centre / length * n - 0.5), 0, n - 1))
+    growth = ratio ** np.abs(idx - ic)
+
+    def total(w0, cap):
+        return np.minimum(w0 * growth, cap).sum()
+
+    h = length / n
+    # choose the cap as 1.6 uniform cells and solve for the smallest width by bisection
+    cap = 1.6 * h
+    lo, hi = 1e-6 * h, cap
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if total(mid, cap) < length:
+            lo = mid
+        else:
+            hi = mid
+    w = np.minimum(0.5 * (lo + hi) * growth, cap)
+    w *= length / w.sum()
+    return np.concatenate([[w[0]], w, [w[-1]]])
+
+
+def _ring(shape, dim):
+    r = np.zeros(shape, bool)
+    r[0] = r[-1] = True
+    r[:, 0] = r[:, -1] = True
+    if dim == 3:
+        r[:, :, 0] = r[:, :, -1] = True
+    return r
+
+
+def stretched_channel(nx=256, ny=128, nz=128, radius=0.05, ratio=1.05) -> Grid:
+    Lx, Ly, Lz = nx / ny, 1.0, nz / ny
+    cx, cy = 0.25 * Lx, 0.5 * Ly
+    wx = graded_widths(nx, Lx, cx, ratio)
+    wy = graded_widths(ny, Ly, cy, ratio)
+    wz = np.full(nz + 2, Lz / nz)
+    # cell centres
+    xc = np.cumsum(wx) - 0.5 * wx - wx[0]
+    yc = np.cumsum(wy) - 0.5 * wy - wy[0]
+    t = np.zeros((ny + 2, nx + 2, nz + 2), np.uint8)
+    ring = _ring(t.shape, 3)
+    t[ring] = SOLID
+    t[1:-1, 0, 1:-1] = INFLOW
+    t[1:-1, -1, 1:-1] = EMPTY
+    sdf = np.sqrt((xc[None, :] - cx) ** 2 + (yc[:, None] - cy) ** 2) - radius    # (ny+2, nx+2)
+    solid = (sdf < 0.0)
+    solid[0, :] = solid[-1, :] = False
+    solid[:, 0] = solid[:, -1] = False
+    t[:, :, 1:-1][solid] = SOLID
+    return Grid(3, t, wx, wy, wz, Ly / ny, True)
+
+
+def flip_free_surface(shape, rng: np.random.Generator) -> Grid:
+    nx, ny, nz = shape
+    t = np.full((ny + 2, nx + 2, nz + 2), EMPTY, np.uint8)
+    ring = _ring(t.shape, 3)
+    t[ring] = SOLID
+    fill = rng.uniform(0.3, 0.5)
+    # dam-break block from the corner: full depth in z, a fraction of x and y
+    fx = rng.uniform(0.5, 0.9)
+    fy = min(1.0, fill / fx)
+    bx, by = max(1, int(fx * nx)), max(1, int(fy * ny))
+    t[1:by + 1, 1:bx + 1, 1:nz + 1] = FLUID
+    nspheres = int(rng.integers(0, 201))
+    if nspheres:
+        zz, yy, xx = None, None, None
+        for _ in range(nspheres):
+            r = rng.uniform(2.0, 6.0)
+            c = rng.uniform([1, 1, 1], [nx, ny, nz])
+            x0, x1 = int(max(1, c[0] - r)), int(min(nx, c[0] + r)) + 1
+            y0, y1 = int(max(1, c[1] - r)), int(min(ny, c[1] + r)) + 1
+            z0, z1 = int(max(1, c[2] - r)), int(min(nz, c[2] + r)) + 1
+            if x0 >= x1 or y0 >= y1 or z0 >= z1:
+                continue
+            Y, X, Z = np.meshgrid(np.arange(y0, y1), np.arange(x0, x1), np.arange(z0, z1), indexing="ij")
+            inside = (X + 0.5 - 1 - c[0]) ** 2 + (Y + 0.5 - 1 - c[1]) ** 2 + (Z + 0.5 - 1 - c[2]) ** 2 <= r * r
+            sub = t[y0:y1, x0:x1, z0:z1]
+            sub[inside] = FLUID
+    dh = 1.0 / nx
+    return Grid(3, t, np.full(nx + 2, dh), np.full(ny + 2, dh), np.full(nz + 2, dh), dh, False)
+
+
+def batch_shapes(count: int, rng: np.random.Generator, lo=64, hi=160):
+    return [tuple(int(v) for v in rng.integers(lo, hi + 1, size=3)) for _ in range(count)]
+
+
+def random_rhs(grid: Grid, rng: np.random.Generator, zero_mean: bool) -> np.ndarray:
+    b = rng.standard_normal(grid.unknowns)
+    if zero_mean and len(b):
+        b = b - b.mean()
+        b = b - b.mean()
+    return b
+
+
+def serving_mix(count: int, rng: np.random.Generator, sizes2d=(256, 512, 1024, 2048), sizes3d=(64, 96, 128, 192, 256)):
+    """Request descriptions: (dim, n, bc, tolerance, seed). bc in {'neumann', 'outlet', 'freesurface'}."""
+    out = []
+    for q in range(count):
+        dim = 2 if rng.random() < 0.5 else 3
+        n = int(rng.choice(sizes2d if dim == 2 else sizes3d))
+        bc = str(rng.choice(["neumann", "outlet", "freesurface"]))
+        tol = float(10.0 ** rng.uniform(-10.0, -6.0))
+        out.append((dim, n, bc, tol, int(rng.integers(0, 2 ** 31))))
+    return out
+
+
+def serving_grid(dim: int, n: int, bc: str, seed: int) -> Grid:
+    from .nppe import uniform_grid
+    if bc == "freesurface" and dim == 3:
+        return flip_free_surface((n, n, n), np.random.default_rng(seed))
+    g = uniform_grid(n, dim=dim)
+    if bc == "outlet":
+        if dim == 3:
+            g.types[1:-1, -1, 1:-1] = EMPTY
+        else:
+            g.types[1:-1, -1, 0] = EMPTY
+    elif bc == "freesurface":   # 2D: liquid below a random level, air above
+        rng = np.random.default_rng(seed)
+        level = int(rng.uniform(0.3, 0.7) * n)
+        g.types[level + 1:-1, 1:-1, 0] = EMPTY
+    return g
