@@ -9,6 +9,7 @@ plain-CG comparison), and on the host the reference's own AMGCL and Eigen CG (or
 reference sources) on a bounded sample.
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--size 256] [--no-cpu] [--no-cg64]
+    python bench.py --config channel|batched|serving      (the other configurations, one JSON line each)
 """
 from __future__ import annotations
 
@@ -159,6 +160,150 @@ def cpu_leg(args):
     }
 
 
+def channel_leg(args):
+    """Stretched 256 x 128 x 128 channel with a voxelised cylinder and a p = 0 outlet, to 1e-10."""
+    import torch
+    import nereus_cfd_amd as N
+    import nereus_cfd_amd.workloads as W
+    g = W.stretched_channel()
+    rng = np.random.default_rng(SEED)
+    b = W.random_rhs(g, rng, zero_mean=False)
+    tb = torch.from_numpy(b).cuda()
+    tx = torch.zeros_like(tb)
+    torch.cuda.synchronize()
+    with N.Solver(tolerance=1e-10, max_iters=4000) as s:
+        s.commit(g)
+        for _ in range(args.warmup):
+            s.solve_device(tb.data_ptr(), tx.data_ptr())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        lat = []
+        for _ in range(args.steps):
+            ts = time.perf_counter()
+            res, rc = s.solve_device(tb.data_ptr(), tx.data_ptr())
+            lat.append(time.perf_counter() - ts)
+            if rc != 0 or not res.relative_residual < 1e-10:
+                raise SystemExit(f"channel solve did not converge: {res.relative_residual}")
+        el = time.perf_counter() - t0
+    return {"metric": "stretched-channel MGPCG solves/s to 1e-10 rel res", "value": args.steps / el,
+            "unit": "solves/s", "ms_per_step": 1e3 * el / args.steps, "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+            "p99_ms": 1e3 * float(np.percentile(lat, 99)), "iterations": int(res.iterations),
+            "refinements": int(res.refinements), "relative_residual": float(res.relative_residual),
+            "gbps": res.algorithmic_bytes / (el / args.steps) / 1e9,
+            "config": {"workload": "3D stretched channel 256x128x128 cells, neighbour ratio <= 1.05 towards a "
+                                   "cylinder r=0.05 (SOLID voxels), inflow x-min, p=0 outlet x-max, SOLID walls; "
+                                   f"N(0,1) RHS seed {SEED}; MGPCG + FP64 refinement to 1e-10",
+                       "unknowns": int(len(b))}}
+
+
+def batched_leg(args):
+    """32 FLIP-like free-surface systems of sizes U{64..160}^3, solved concurrently to 1e-8."""
+    import torch
+    import nereus_cfd_amd as N
+    import nereus_cfd_amd.workloads as W
+    rng = np.random.default_rng(SEED)
+    grids = [W.flip_free_surface(shape, rng) for shape in W.batch_shapes(32, rng)]
+    bs = [W.random_rhs(g, rng, zero_mean=False) for g in grids]
+    solvers = [N.Solver(tolerance=1e-8) for _ in grids]
+    t0 = time.perf_counter()
+    for h, g in zip(solvers, grids):
+        h.commit(g)
+    setup = time.perf_counter() - t0
+    tbs = [torch.from_numpy(b).cuda() for b in bs]
+    txs = [torch.zeros_like(t) for t in tbs]
+    torch.cuda.synchronize()
+    bp, xp = [t.data_ptr() for t in tbs], [t.data_ptr() for t in txs]
+    for _ in range(args.warmup):
+        N.batch_solve_device(solvers, bp, xp)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res, rc = N.batch_solve_device(solvers, bp, xp)
+        if rc != 0:
+            raise SystemExit(f"batched solve failed: {rc}")
+    el = time.perf_counter() - t0
+    # the same systems one after another on one stream, for comparison
+    t1 = time.perf_counter()
+    for h, b, x in zip(solvers, bp, xp):
+        h.solve_device(b, x)
+    seq = time.perf_counter() - t1
+    by = sum(r.algorithmic_bytes for r in res)
+    for h in solvers:
+        h.close()
+    cells = sum(int(np.prod(g.types.shape)) for g in grids)
+    return {"metric": "batched free-surface MGPCG solves/s (32 systems, 64^3-160^3, to 1e-8)",
+            "value": 32 * args.steps / el, "unit": "solves/s", "ms_per_step": 1e3 * el / args.steps,
+            "gbps": by / (el / args.steps) / 1e9, "sequential_ms_per_batch": 1e3 * seq,
+            "max_iterations": int(max(r.iterations for r in res)), "worst_relative_residual":
+                float(max(r.relative_residual for r in res)), "setup_s": setup,
+            "config": {"workload": f"32 FLIP-like 3D grids, sizes U{{64..160}} per axis, liquid block 30-50% + "
+                                   f"0-200 spheres r 2-6, EMPTY air (p=0), SOLID walls, N(0,1) RHS seed {SEED}",
+                       "cells": cells}}
+
+
+def serving_leg(args):
+    """Poisson arrivals of solve requests (2D 256^2-2048^2 / 3D 64^3-256^3, Neumann / outlet / free surface,
+    tolerance 1e-6..1e-10) through the solve server; the same stream through a server running the FP64 CG."""
+    import threading
+    import nereus_cfd_amd as N
+    import nereus_cfd_amd.workloads as W
+    rng = np.random.default_rng(SEED)
+    reqs = W.serving_mix(args.requests, rng)
+    rate = args.rate
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(reqs)))
+    keys = {}
+    grids = {}
+    for (dim, n, bc, tol, seed) in reqs:
+        k = (dim, n, bc, seed % 4)
+        if k not in keys:
+            keys[k] = len(keys)
+            grids[keys[k]] = W.serving_grid(dim, n, bc, seed % 4)
+    rhs = []
+    for (dim, n, bc, tol, seed) in reqs:
+        g = grids[keys[(dim, n, bc, seed % 4)]]
+        rhs.append(W.random_rhs(g, np.random.default_rng(seed), zero_mean=(bc == "neumann")))
+
+    def run(method):
+        out = []
+        with N.Server(slots=args.slots, hbm_budget=int(args.budget_gb * 2 ** 30), method=method,
+                      max_iters=100000) as srv:
+            for k, g in grids.items():
+                srv.add_grid(k, g)
+            tickets = [None] * len(reqs)
+            t0 = time.perf_counter()
+
+            def submitter():
+                for q, (dim, n, bc, tol, seed) in enumerate(reqs):
+                    delay = arrivals[q] - (time.perf_counter() - t0)
+                    if delay > 0:
+                        time.sleep(delay)
+                    tickets[q] = srv.submit(keys[(dim, n, bc, seed % 4)], rhs[q], tol)
+
+            th = threading.Thread(target=submitter)
+            th.start()
+            for q in range(len(reqs)):
+                while tickets[q] is None:
+                    time.sleep(0.0005)
+                x, res, lat, rc = srv.wait(tickets[q])
+                out.append((lat, res.relative_residual <= reqs[q][3] * 1.0000001, rc))
+            th.join()
+            el = time.perf_counter() - t0
+            peak = srv.memory()[1]
+        lats = np.array([o[0] for o in out])
+        return {"solves_per_s": len(reqs) / el, "p50_ms": float(np.percentile(lats, 50)),
+                "p99_ms": float(np.percentile(lats, 99)), "all_converged": all(o[1] and o[2] == 0 for o in out),
+                "peak_hbm_gb": peak / 2 ** 30, "wall_s": el}
+
+    mg = run(N.METHOD_MGPCG)
+    cg = run(N.METHOD_CG64) if not args.no_cg64 else None
+    return {"metric": "serving solves/s and p50/p99 latency at a fixed Poisson arrival rate",
+            "value": mg["solves_per_s"], "unit": "solves/s", "p50_ms": mg["p50_ms"], "p99_ms": mg["p99_ms"],
+            "all_converged": mg["all_converged"], "peak_hbm_gb": mg["peak_hbm_gb"], "fp64_cg": cg,
+            "config": {"workload": f"{len(reqs)} requests, Poisson arrivals at {rate}/s; 2D 256^2-2048^2 or 3D "
+                                   f"64^3-256^3; Neumann / p=0 outlet / free surface; tolerance 10^U(-10,-6); "
+                                   f"{len(grids)} distinct resident grids; {args.slots} slots, HBM budget "
+                                   f"{args.budget_gb} GiB", "seed": SEED}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,11 +312,23 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg64", action="store_true")
+    ap.add_argument("--config", default="headline", choices=["headline", "channel", "batched", "serving"])
+    ap.add_argument("--requests", type=int, default=32)
+    ap.add_argument("--rate", type=float, default=40.0)
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--budget-gb", type=float, default=64.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
               file=sys.stderr)
         sys.exit(2)
+    if args.config != "headline":
+        leg = {"channel": channel_leg, "batched": batched_leg, "serving": serving_leg}[args.config]
+        out = leg(args)
+        out.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+                    "dtype": "fp32+fp64", "data": "synthetic", "vs_baseline": None})
+        print(json.dumps(out))
+        return
     res = gpu_leg(args)
     cpu = None if args.no_cpu else cpu_leg(args)
     line = {
