@@ -40,6 +40,21 @@ def workload(n):
     return g, b
 
 
+def measured_traffic():
+    """HBM bytes per launch of the finest-level sweep at 256^3, from the FETCH_SIZE and WRITE_SIZE counter runs
+    summarised (with their commands and the copy-kernel calibration) in profiles/k_rbgs_row_256cube_hbm_traffic.md;
+    None when that summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_rbgs_row_256cube_hbm_traffic.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch ="):
+                    return int(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def gpu_leg(args):
     import torch
     import nereus_cfd_amd as N
@@ -61,7 +76,9 @@ def gpu_leg(args):
         res, rc = s.solve_device(tb.data_ptr(), tx.data_ptr())
         log(f"[bench] warmup {w}: iters={res.iterations} passes={res.refinements} relres={res.relative_residual:.3e} "
             f"device_ms={res.device_ms:.3f}")
-    s.profile(True)
+    # HIP events bracket only the finest-level red-black sweeps (the dominant kernel) in the timed region; every
+    # other launch runs without events around it.
+    s.profile(True, kinds=["smooth0"])
     lat = []
     bytes_total = 0.0
     iters = []
@@ -77,9 +94,14 @@ def gpu_leg(args):
             raise SystemExit(f"solve {k} did not converge: rc={rc} relres={res.relative_residual}")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    prof = s.profile_read()
+    dom = ("smooth0", s.profile_read()["smooth0"])
     s.profile(False)
     log(f"[bench] timed {args.steps} solves in {elapsed:.4f}s")
+    # Per-kind breakdown of one further (untimed) solve with events around every launch.
+    s.profile(True)
+    s.solve_device(tb.data_ptr(), tx.data_ptr())
+    prof = s.profile_read()
+    s.profile(False)
 
     # Accuracy of the last result: true FP64 residual (reported by the solve) and the pressure error against the
     # FP64 matrix-free CG solution of the same system (relative L2 after mean removal).
@@ -100,11 +122,14 @@ def gpu_leg(args):
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 4),
                         "gbps": round(v["bytes"] / max(v["ms"], 1e-12) / 1e6, 1)} for k, v in prof.items()},
     }
-    dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
     achieved = dom[1]["bytes"] / dom[1]["ms"] / 1e6
-    out["roofline"] = {"bound": "hbm", "kernel": dom[0], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                       "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                       "avg_launch_us": round(1e3 * dom[1]["ms"] / dom[1]["launches"], 2)}
+    traffic = measured_traffic() if n == 256 else None
+    out["roofline"] = {"bound": "hbm", "kernel": "k_rbgs_row (finest-level red-black Gauss-Seidel sweep)",
+                       "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                       "launches": dom[1]["launches"],
+                       "avg_launch_us": round(1e3 * dom[1]["ms"] / dom[1]["launches"], 2),
+                       "bytes_per_launch": round(dom[1]["bytes"] / dom[1]["launches"])}
     s.close()
 
     if not args.no_cg64:
@@ -194,6 +219,30 @@ def channel_leg(args):
                                    "cylinder r=0.05 (SOLID voxels), inflow x-min, p=0 outlet x-max, SOLID walls; "
                                    f"N(0,1) RHS seed {SEED}; MGPCG + FP64 refinement to 1e-10",
                        "unknowns": int(len(b))}}
+
+
+def copy_leg(args):
+    """A plain device copy of 1 GiB of FP32 (y = 1.0 * x, one read and one write of every byte, 4x the 256 MB
+    Infinity Cache): the measured copy bandwidth of this GPU, and the known-size kernel that scales the HBM
+    counters (FETCH_SIZE, WRITE_SIZE) of a profiled run to bytes."""
+    import torch
+    n = 1 << 28
+    x = torch.rand(n, device="cuda:0", dtype=torch.float32)
+    y = torch.empty_like(x)
+    for _ in range(max(1, args.warmup)):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(args.steps):
+        torch.mul(x, 1.0, out=y)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / args.steps
+    moved = 2 * 4 * n
+    return {"metric": "device copy GB/s", "value": round(moved / ms / 1e6, 1), "unit": "GB/s", "ms_per_step": ms,
+            "scaling": "weak", "config": {"workload": "y = 1.0 * x over 2^28 FP32 (1 GiB read + 1 GiB write)",
+                                          "bytes_read": 4 * n, "bytes_written": 4 * n}}
 
 
 def batched_leg(args):
@@ -312,7 +361,7 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg64", action="store_true")
-    ap.add_argument("--config", default="headline", choices=["headline", "channel", "batched", "serving"])
+    ap.add_argument("--config", default="headline", choices=["headline", "channel", "batched", "serving", "copy"])
     ap.add_argument("--requests", type=int, default=32)
     ap.add_argument("--rate", type=float, default=40.0)
     ap.add_argument("--slots", type=int, default=4)
@@ -323,7 +372,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.config != "headline":
-        leg = {"channel": channel_leg, "batched": batched_leg, "serving": serving_leg}[args.config]
+        leg = {"channel": channel_leg, "batched": batched_leg, "serving": serving_leg, "copy": copy_leg}[args.config]
         out = leg(args)
         out.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
                     "dtype": "fp32+fp64", "data": "synthetic", "vs_baseline": None})

@@ -15,7 +15,7 @@ extern "C" {
 
 enum nppe_kernel_kind
 {
-    NPPE_K_SMOOTH = 0,       /* red-black Gauss-Seidel colour sweep, FP32, any level */
+    NPPE_K_SMOOTH = 0,       /* red-black Gauss-Seidel sweep (both colours), FP32, levels >= 1 */
     NPPE_K_RESTRICT = 1,     /* residual + restriction, FP32 */
     NPPE_K_PROLONG = 2,      /* prolongation + correction, FP32 */
     NPPE_K_COARSE = 3,       /* coarsest-level dense FP64 solve */
@@ -31,13 +31,17 @@ enum nppe_kernel_kind
     NPPE_K_CG64_SPMV = 13,   /* FP64 q = A p with <p, q> */
     NPPE_K_CG64_UPDATE = 14, /* FP64 x, r update with <r, r>, <r, M^-1 r> */
     NPPE_K_CG64_DIRECTION = 15,
-    NPPE_KERNEL_KINDS = 16
+    NPPE_K_SMOOTH0 = 16,     /* red-black Gauss-Seidel sweep (both colours), FP32, the finest level */
+    NPPE_KERNEL_KINDS = 17
 };
 
 struct nppe_solver;
 
 /* Turns event timing on or off and clears the totals. */
 int nppe_profile_enable(struct nppe_solver *h, int on);
+/* Restricts the timing to the kinds whose bit (1 << kind) is set in mask (all kinds after nppe_profile_enable);
+ * launches of the other kinds run without events around them. */
+int nppe_profile_select(struct nppe_solver *h, uint32_t mask);
 /* Totals since the last nppe_profile_enable: launches that did work, their summed duration and algorithmic bytes. */
 int nppe_profile_read(const struct nppe_solver *h, int kind, int64_t *launches, double *total_ms, double *bytes);
 

@@ -26,7 +26,8 @@ namespace
 std::mutex g_create_mutex;
 std::string g_create_error;
 
-constexpr int kInnerChunk = 4;      // inner iterations enqueued between host checks
+constexpr int kInnerChunk = 4;      // inner iterations enqueued before the first host check of a solve
+constexpr int kMaxChunk = 64;       // most inner iterations enqueued between two host checks
 constexpr int kMaxLevels = 24;
 constexpr int64_t kCoarsestTarget = 512;
 
@@ -91,12 +92,14 @@ struct nppe_solver
     bool budgetHit = false;
     int coarseNu = 0;   // sweeps on levels >= 2 (0: the same as the fine levels)
     bool useGraphs = true;                 // replay the inner iterations from a captured hipGraph
-    hipGraphExec_t innerGraph = nullptr;   // kInnerChunk inner iterations, captured at the first solve
+    hipGraphExec_t innerGraph = nullptr;   // one inner iteration, captured at the first solve
     double graphBytesPerIter = 0.0;
+    double rate = 0.0;   // observed per-iteration reduction of the inner residual norm (0: none yet)
     std::vector<void *> allocations;
 
     // profiling (nppe_profile.h)
     bool profiling = false;
+    uint32_t profMask = ~0u;
     std::vector<hipEvent_t> evPool;
     struct Rec
     {
@@ -158,6 +161,7 @@ void free_grid(nppe_solver *h)
         (void)hipGraphExecDestroy(h->innerGraph);
         h->innerGraph = nullptr;
     }
+    h->rate = 0.0;
     for (void *p : h->allocations) (void)hipFree(p);
     h->allocations.clear();
     h->bytesHeld = 0;
@@ -185,7 +189,7 @@ struct Scope
     hipEvent_t b = nullptr;
     Scope(nppe_solver *h_, int kind_, double bytes_) : h(h_), kind(kind_), bytes(bytes_)
     {
-        if (!h->profiling) return;
+        if (!h->profiling || !((h->profMask >> kind) & 1u)) return;
         if (h->evNext + 2 > h->evPool.size())
             for (int q = 0; q < 256; q++)
             {
@@ -245,7 +249,7 @@ hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel)
         {
             const bool zero = sw == 0;
             const double by = zero ? (2 * s + f) * N : (3 * s + f) * N;
-            Scope sc(h, NPPE_K_SMOOTH, by);
+            Scope sc(h, l == 0 ? NPPE_K_SMOOTH0 : NPPE_K_SMOOTH, by);
             float *xin = xbuf(l, cur[l]);
             float *xout = zero ? xin : xbuf(l, cur[l] ^ 1);
             if ((e = launch_rbgs(h->stream, L.v, zero ? nullptr : xin, xout, bbuf(l), 0, zero ? 1 : 0, L.tileRows,
@@ -310,7 +314,7 @@ hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel)
         for (int sw = 0; sw < nu2; sw++)
         {
             const double by = (3 * s + f) * N;
-            Scope sc(h, NPPE_K_SMOOTH, by);
+            Scope sc(h, l == 0 ? NPPE_K_SMOOTH0 : NPPE_K_SMOOTH, by);
             if ((e = launch_rbgs(h->stream, L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l), 1, 0, L.tileRows,
                                  L.planesPerChunk, h->dState)) != hipSuccess)
                 return e;
@@ -681,31 +685,41 @@ int solve_mgpcg(nppe_solver *h, const double *dB, double *dX, const double *dGue
         const bool graphs = h->useGraphs && !h->profiling;
         if (graphs && !h->innerGraph)
         {
-            // Capture kInnerChunk iterations once: the launches, their arguments and the z ping-pong parity are the
-            // same every time, and every kernel reads the device-side done flag, so one graph serves every chunk.
+            // Capture one iteration once: its launches, their arguments and the z ping-pong parity are the same
+            // every time, and every kernel reads the device-side done flag, so one graph serves every iteration.
             hipGraph_t g = nullptr;
             Traffic it;
             HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "graph capture");
-            hipError_t ce = hipSuccess;
-            for (int q = 0; q < kInnerChunk && ce == hipSuccess; q++) ce = inner_iteration(h, it);
-            hipError_t ee = hipStreamEndCapture(h->stream, &g);
+            const hipError_t ce = inner_iteration(h, it);
+            const hipError_t ee = hipStreamEndCapture(h->stream, &g);
             if (ce != hipSuccess) return hip_fail(h, ce, "graph capture");
             if (ee != hipSuccess) return hip_fail(h, ee, "graph capture");
-            hipError_t ie = hipGraphInstantiate(&h->innerGraph, g, nullptr, nullptr, 0);
+            const hipError_t ie = hipGraphInstantiate(&h->innerGraph, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             if (ie != hipSuccess) return hip_fail(h, ie, "graph instantiate");
-            h->graphBytesPerIter = it.bytes / kInnerChunk;
+            h->graphBytesPerIter = it.bytes;
         }
+        // Iterations to enqueue before the next host check: the count the observed per-iteration reduction
+        // predicts for the reduction still needed (the device stops at convergence, so an overshoot costs only
+        // launches that exit at once; an undershoot costs one more check).
+        const double rr0 = rnorm * rnorm;
+        auto predict = [&](double needRed) {
+            if (!(h->rate > 0.0 && h->rate < 1.0) || !(needRed < 1.0)) return kInnerChunk;
+            const double k = std::ceil(std::log(needRed) / std::log(h->rate));
+            return (int)std::max(1.0, std::min((double)kMaxChunk, k));
+        };
+        int chunk = predict(red);
         for (;;)
         {
             if (graphs)
-            {
-                HIP_TRY(h, hipGraphLaunch(h->innerGraph, h->stream), "graph launch");
-                launched += kInnerChunk;
-                perIter = h->graphBytesPerIter;
-            }
+                for (int q = 0; q < chunk; q++)
+                {
+                    HIP_TRY(h, hipGraphLaunch(h->innerGraph, h->stream), "graph launch");
+                    launched++;
+                    perIter = h->graphBytesPerIter;
+                }
             else
-                for (int q = 0; q < kInnerChunk; q++)
+                for (int q = 0; q < chunk; q++)
                 {
                     h->curIter = launched++;
                     Traffic it;
@@ -714,7 +728,11 @@ int solve_mgpcg(nppe_solver *h, const double *dB, double *dX, const double *dGue
                 }
             rc = sync_state(h);
             if (rc) return rc;
-            if (h->hState->done) break;
+            const SolveState &st = *h->hState;
+            if (st.iters > 0 && st.rr > 0.0 && st.rr < rr0)
+                h->rate = std::pow(st.rr / rr0, 0.5 / st.iters);
+            if (st.done) break;
+            chunk = st.rr > 0.0 && st.thr2 > 0.0 ? predict(std::sqrt(st.thr2 / st.rr)) : kInnerChunk;
         }
         const int done = h->hState->iters;
         profile_collect(h, done);
@@ -1196,6 +1214,7 @@ int nppe_profile_enable(nppe_solver *h, int on)
 {
     if (!h) return NPPE_ERR_ARGUMENT;
     h->profiling = on != 0;
+    h->profMask = ~0u;
     h->recs.clear();
     h->evNext = 0;
     for (int k = 0; k < NPPE_KERNEL_KINDS; k++)
@@ -1204,6 +1223,13 @@ int nppe_profile_enable(nppe_solver *h, int on)
         h->profBytes[k] = 0.0;
         h->profCount[k] = 0;
     }
+    return NPPE_OK;
+}
+
+int nppe_profile_select(nppe_solver *h, uint32_t mask)
+{
+    if (!h) return NPPE_ERR_ARGUMENT;
+    h->profMask = mask;
     return NPPE_OK;
 }
 

@@ -24,14 +24,14 @@ FLUID, SOLID, EMPTY, INFLOW, PRESSURE, OUTFLOW = 0, 1, 2, 3, 4, 5
 
 KERNEL_KINDS = [
     "smooth", "restrict", "prolong", "coarse", "cg_dot", "cg_direction", "cg_spmv", "cg_update", "residual64",
-    "scatter", "gather", "mean", "cg64_init", "cg64_spmv", "cg64_update", "cg64_direction",
+    "scatter", "gather", "mean", "cg64_init", "cg64_spmv", "cg64_update", "cg64_direction", "smooth0",
 ]
 
 # Every symbol the headers in include/ declare.
 EXPORTS = [
     "nppe_default_options", "nppe_create", "nppe_destroy", "nppe_last_error", "nppe_commit_grid",
     "nppe_system_info", "nppe_unknown_order", "nppe_check_csr", "nppe_solve", "nppe_solve_device", "nppe_apply",
-    "nppe_memory_bytes", "nppe_profile_enable", "nppe_profile_read", "nppe_synth_cosine_rhs",
+    "nppe_memory_bytes", "nppe_profile_enable", "nppe_profile_select", "nppe_profile_read", "nppe_synth_cosine_rhs",
     "nppe_set_tolerance", "nppe_estimate_bytes", "nppe_batch_solve", "nppe_server_create", "nppe_server_destroy",
     "nppe_server_add_grid", "nppe_server_submit", "nppe_server_wait", "nppe_server_memory",
 ]
@@ -110,6 +110,8 @@ def lib():
     L.nppe_memory_bytes.restype = C.c_int
     L.nppe_profile_enable.argtypes = [P, C.c_int]
     L.nppe_profile_enable.restype = C.c_int
+    L.nppe_profile_select.argtypes = [P, C.c_uint32]
+    L.nppe_profile_select.restype = C.c_int
     L.nppe_profile_read.argtypes = [P, C.c_int, P, P, P]
     L.nppe_profile_read.restype = C.c_int
     L.nppe_synth_cosine_rhs.argtypes = [P, C.c_uint64, I32, D, I32, P]
@@ -297,8 +299,14 @@ class Solver:
     def set_tolerance(self, tol: float):
         self._check(lib().nppe_set_tolerance(self.h, float(tol)), "set_tolerance")
 
-    def profile(self, on: bool = True):
+    def profile(self, on: bool = True, kinds=None):
+        """Event timing per kernel kind; `kinds` (names from KERNEL_KINDS) limits it to those kinds."""
         self._check(lib().nppe_profile_enable(self.h, int(on)), "profile_enable")
+        if on and kinds is not None:
+            mask = 0
+            for k in kinds:
+                mask |= 1 << KERNEL_KINDS.index(k)
+            self._check(lib().nppe_profile_select(self.h, mask), "profile_select")
 
     def profile_read(self):
         out = {}
