@@ -13,6 +13,7 @@
 //   EMPTY / PRESSURE neighbour  -> diag -= c * rf, rf = FaceGap / (Width/2) on the domain ring, 1 inside
 //   SOLID / INFLOW / OUTFLOW    -> nothing (Neumann)
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include <algorithm>
 
@@ -175,7 +176,18 @@ __device__ __forceinline__ float offsum32(const Box &B, const Op32 &o, const flo
 
 // 1/n for the n coupled neighbours of an interior cell of a uniform level (no Dirichlet neighbour): there the
 // diagonal is -cu n, so a Gauss-Seidel update is (sum of coupled x - b / cu) / n and A x = cu (sum - n x).
-__constant__ float kInvN[8] = {0.f, 1.f, 0.5f, 1.f / 3.f, 0.25f, 0.2f, 1.f / 6.f, 0.f};
+// Selected in registers rather than read from a table: a table in memory is a dependent load in every update.
+__device__ __forceinline__ float inv_count(unsigned code)
+{
+    const unsigned n = __popc(code & 63u);
+    float r = n == 1u ? 1.f : 0.f;
+    r = n == 2u ? 0.5f : r;
+    r = n == 3u ? 1.f / 3.f : r;
+    r = n == 4u ? 0.25f : r;
+    r = n == 5u ? 0.2f : r;
+    r = n == 6u ? 1.f / 6.f : r;
+    return r;
+}
 
 __device__ __forceinline__ bool fast_cell(const Level32 &L, unsigned code) { return L.uniform && !(code & 128u); }
 
@@ -752,7 +764,7 @@ __device__ __forceinline__ float rbgs_update(const RbgsCtx &C, int pl, int lr, i
         acc += (code & 8u) ? Xc[lr - 1][lc] : 0.f;
         acc += (code & 16u) ? Xc[lr][lc + 1] : 0.f;
         acc += (code & 32u) ? Xc[lr][lc - 1] : 0.f;
-        return (acc - bv * C.L.icu) * kInvN[__popc(code & 63u)];
+        return (acc - bv * C.L.icu) * inv_count(code);
     }
     const int m = C.m0 - 2 + lr, f = C.f0 - 2 + lc;
     const Op32 o = op32(C.L, pl, m, f, idx, code);
@@ -913,7 +925,7 @@ __device__ __forceinline__ float gs_update(const Level32 &L, int s, int m, int f
         float acc = 0.f;
 #pragma unroll
         for (int q = 0; q < 6; q++) acc += ((code >> q) & 1u) ? nb[q] : 0.f;
-        if (!(code & 128u)) return (acc - bv * L.icu) * kInvN[__popc(code & 63u)];
+        if (!(code & 128u)) return (acc - bv * L.icu) * inv_count(code);
         const unsigned c16 = L.c16[idx];
         float d = 0.f;   // -diag / cu
 #pragma unroll
@@ -1088,6 +1100,11 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_march(Level32 L, const f
 // block is TM rows plus one halo row above and below (halo waves publish first-colour values only). Plane bases are
 // scalar and per-lane offsets 32-bit; loads outside the interior are clamped into the box and their values masked
 // by the stencil codes, so the loop has no bounds branches. Uniform levels only (constant coefficients).
+//
+// Every global load of a plane is issued one (b, codes, the halo waves' outer rows) or two (x) planes ahead of its
+// use: the old x of the mid-axis neighbour rows comes from the neighbouring waves through LDS (each wave publishes
+// its own row), and the halo waves, whose outer neighbour row belongs to no wave of the block, prefetch that row.
+// So no load is waited for in the plane that issues it.
 template <int TM, int K, bool ZERO>
 __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const float *__restrict__ xin,
                                                             float *__restrict__ xout, const float *__restrict__ b,
@@ -1096,7 +1113,8 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
     if (st->done) return;
     const Box &B = L.B;
     constexpr int kCols = 128 * K + 2;
-    __shared__ float red[2][TM + 2][kCols];   // first-colour values by column f, rows m0-1 .. m0+TM
+    __shared__ float red[2][TM + 2][kCols];      // first-colour values by column f, rows m0-1 .. m0+TM
+    __shared__ float old[TM + 2][2][64 * K];     // old x of plane sp by row, element (a = 0, b = 1) and pair
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lb = logical_block();
     const int tm = lb % tilesM, chunk = lb / tilesM;
@@ -1105,10 +1123,14 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
     const bool haloRow = wave == 0 || wave == TM + 1;
     const bool rowIn = m >= 1 && m <= B.nm - 2;
     const int mc = min(max(m, 0), B.nm - 1);
+    // the row beyond a halo row (outside the block), clamped into the box
+    const int mo = min(max(wave == 0 ? m - 1 : m + 1, 0), B.nm - 1);
     const int s0 = 1 + chunk * P, s1 = min(s0 + P - 1, B.ns - 2);
     const uint32_t sM = (uint32_t)B.sM;
-    // per pair: first cell fa = 1 + 2p, clamped load column fc (fc + 1 stays in the box)
-    uint32_t off[K];
+    // per pair: first cell fa = 1 + 2p; load column fc = fa clamped to the last odd column whose pair stays in the
+    // box (odd, so the pair load stays aligned)
+    const int fcMax = ((B.nf - 2) & 1) ? B.nf - 2 : B.nf - 3;
+    uint32_t off[K], offO[K];
     bool okA[K], okB[K];
     int fa[K];
 #pragma unroll
@@ -1117,124 +1139,144 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
         fa[k] = 1 + 2 * (lane + 64 * k);
         okA[k] = rowIn && fa[k] <= B.nf - 2;
         okB[k] = rowIn && fa[k] + 1 <= B.nf - 2;
-        const int fc = min(fa[k], B.nf - 2);
+        const int fc = min(fa[k], fcMax);
         off[k] = (uint32_t)mc * sM + (uint32_t)fc;
+        offO[k] = (uint32_t)mo * sM + (uint32_t)fc;
     }
     for (int i = threadIdx.x; i < 2 * (TM + 2) * kCols; i += 64 * (TM + 2)) (&red[0][0][0])[i] = 0.f;
     auto plane = [&](const float *base, int pl) { return base + (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
     auto planeC = [&](int pl) { return L.c8 + (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
-    float xa[4][K], xb[4][K];
-    float bA[K], bB[K], bPA[K], bPB[K];
-    unsigned cA[K], cB[K], cPA[K], cPB[K];
-    float R[3][K];
+    // one aligned load per pair (the arrays are allocated so that odd columns start pair-aligned)
+    auto pair = [](const float *row, uint32_t o) { return *reinterpret_cast<const float2 *>(row + o); };
+    auto pairC = [](const uint8_t *row, uint32_t o) { return (unsigned)*reinterpret_cast<const uint16_t *>(row + o); };
+    // Per-plane values live in slots indexed by plane mod 4 (relative to the first plane s0-1), so the march
+    // renames registers instead of moving them: a move of a register that a load in flight will write waits for
+    // every load. x is loaded two planes ahead, b, codes and the halo waves' outer row one plane ahead.
+    float2 X[4][K];    // old x of the pair
+    float2 O[2][K];    // halo waves: old x of the outer row's pair
+    float2 Bv[4][K];   // b of the pair
+    unsigned C[4][K];  // codes of the pair (low byte: element a), masked where used
+    float R[4][K];     // first-colour value of this lane's pair
 #pragma unroll
-    for (int k = 0; k < K; k++)
-    {
-        bPA[k] = bPB[k] = 0.f;
-        cPA[k] = cPB[k] = 0u;
-        R[0][k] = R[1][k] = R[2][k] = 0.f;
-#pragma unroll
-        for (int j = 0; j < 4; j++) xa[j][k] = xb[j][k] = 0.f;
-    }
-    if (!ZERO)
-    {
-#pragma unroll
-        for (int j = 1; j < 4; j++)
-        {
-            const float *px = plane(xin, s0 - 3 + j);
-#pragma unroll
-            for (int k = 0; k < K; k++)
-            {
-                xa[j][k] = px[off[k]];
-                xb[j][k] = px[off[k] + 1];
-            }
-        }
-    }
-    {
-        const int pl = s0 - 1;
-        const bool in = pl >= 1 && pl <= B.ns - 2;
-        const float *pb = plane(b, pl);
-        const uint8_t *pc = planeC(pl);
+    for (int j = 0; j < 4; j++)
 #pragma unroll
         for (int k = 0; k < K; k++)
         {
-            bA[k] = pb[off[k]];
-            bB[k] = pb[off[k] + 1];
-            cA[k] = (in && okA[k]) ? pc[off[k]] : 0u;
-            cB[k] = (in && okB[k]) ? pc[off[k] + 1] : 0u;
+            X[j][k] = Bv[j][k] = make_float2(0.f, 0.f);
+            O[j & 1][k] = make_float2(0.f, 0.f);
+            R[j][k] = 0.f;
+            C[j][k] = 0u;
+        }
+    const int sp0 = s0 - 1;
+    // slots: plane sp0 + d -> d & 3 (d & 1 for O)
+    if (!ZERO)
+    {
+#pragma unroll
+        for (int d = -1; d <= 1; d++)
+        {
+            const float *px = plane(xin, sp0 + d);
+#pragma unroll
+            for (int k = 0; k < K; k++) X[(d + 4) & 3][k] = pair(px, off[k]);
+        }
+        if (haloRow)
+        {
+            const float *px = plane(xin, sp0);
+#pragma unroll
+            for (int k = 0; k < K; k++) O[0][k] = pair(px, offO[k]);
+        }
+    }
+    {
+        const float *pb = plane(b, sp0);
+        const uint8_t *pc = planeC(sp0);
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            Bv[0][k] = pair(pb, off[k]);
+            C[0][k] = pairC(pc, off[k]);
         }
     }
     __syncthreads();
     const float icu = L.icu;
-    for (int sp = s0 - 1; sp <= s1 + 1; sp++)
-    {
-#pragma unroll
-        for (int k = 0; k < K; k++)
-        {
-#pragma unroll
-            for (int j = 0; j < 3; j++)
-            {
-                xa[j][k] = xa[j + 1][k];
-                xb[j][k] = xb[j + 1][k];
-            }
-            R[0][k] = R[1][k];
-            R[1][k] = R[2][k];
-        }
-        // prefetch: old x of plane sp+2; b and codes of plane sp+1
-        float nbA[K], nbB[K];
-        unsigned ncA[K], ncB[K];
+    const int wUp = min(wave + 1, TM + 1), wDn = max(wave - 1, 0);
+
+    // One plane step at sp = sp0 + 4g + U: first colour of plane sp, second colour (and store) of plane sp - 1.
+    auto step = [&](int sp, auto phase) {
+        constexpr int U = decltype(phase)::value;
+        constexpr int Sm2 = (U + 2) & 3, Sm1 = (U + 3) & 3, S0 = U & 3, S1 = (U + 1) & 3, S2 = (U + 2) & 3;
+        constexpr int O0 = U & 1, O1 = (U + 1) & 1;
+        // prefetch
         {
             const float *px = plane(xin, sp + 2);
             const float *pb = plane(b, sp + 1);
             const uint8_t *pc = planeC(sp + 1);
-            const bool in = sp + 1 >= 1 && sp + 1 <= B.ns - 2;
 #pragma unroll
             for (int k = 0; k < K; k++)
             {
-                xa[3][k] = ZERO ? 0.f : px[off[k]];
-                xb[3][k] = ZERO ? 0.f : px[off[k] + 1];
-                nbA[k] = pb[off[k]];
-                nbB[k] = pb[off[k] + 1];
-                ncA[k] = (in && okA[k]) ? pc[off[k]] : 0u;
-                ncB[k] = (in && okB[k]) ? pc[off[k] + 1] : 0u;
+                if (!ZERO) X[S2][k] = pair(px, off[k]);
+                Bv[S1][k] = pair(pb, off[k]);
+                C[S1][k] = pairC(pc, off[k]);
+            }
+            if (!ZERO && haloRow)
+            {
+                const float *po = plane(xin, sp + 1);
+#pragma unroll
+                for (int k = 0; k < K; k++) O[O1][k] = pair(po, offO[k]);
             }
         }
+        // publish this row's old x of plane sp for the neighbouring rows
+        if (!ZERO)
+        {
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                old[wave][0][lane + 64 * k] = X[S0][k].x;
+                old[wave][1][lane + 64 * k] = X[S0][k].y;
+            }
+        }
+        // (1) old[] is complete; readers of red[sp & 1] (the second colour of plane sp-2) are done
+        __syncthreads();
         const int e1 = (c1 - sp - m - 1) & 1;   // first-colour element of every pair on this row and plane
-        const float *pxs = plane(xin, sp);
+        const bool spIn = sp >= 1 && sp <= B.ns - 2;
         float rnew[K];
 #pragma unroll
         for (int k = 0; k < K; k++)
         {
+            const float xa = X[S0][k].x, xb = X[S0][k].y;
             // in-row old neighbours: left of fa is the previous pair's b element, right of fa+1 the next pair's a
-            float left = __shfl_up(xb[1][k], 1, 64);
-            float right = __shfl_down(xa[1][k], 1, 64);
+            float left = __shfl_up(xb, 1, 64);
+            float right = __shfl_down(xa, 1, 64);
             if (K > 1)
             {
-                const float wrapL = k > 0 ? __shfl(xb[1][k > 0 ? k - 1 : 0], 63, 64) : 0.f;
-                const float wrapR = k + 1 < K ? __shfl(xa[1][k + 1 < K ? k + 1 : k], 0, 64) : 0.f;
+                const float wrapL = k > 0 ? __shfl(X[S0][k > 0 ? k - 1 : 0].y, 63, 64) : 0.f;
+                const float wrapR = k + 1 < K ? __shfl(X[S0][k + 1 < K ? k + 1 : k].x, 0, 64) : 0.f;
                 if (lane == 0) left = wrapL;
                 if (lane == 63) right = wrapR;
             }
-            const unsigned code = e1 ? cB[k] : cA[k];
+            float up = 0.f, dn = 0.f;   // old x of rows m+1, m-1 at plane sp, element fa + e1
+            if (!ZERO)
+            {
+                const float lu = old[wUp][e1][lane + 64 * k], ld = old[wDn][e1][lane + 64 * k];
+                const float outer = e1 ? O[O0][k].y : O[O0][k].x;
+                up = wave == TM + 1 ? outer : lu;
+                dn = wave == 0 ? outer : ld;
+            }
+            const unsigned code = (spIn && (e1 ? okB[k] : okA[k])) ? ((C[S0][k] >> (8 * e1)) & 255u) : 0u;
             float v = 0.f;
             if (code & 64u)
             {
-                const uint32_t o = off[k] + (uint32_t)e1;
                 float acc = 0.f;
-                acc += (code & 1u) ? (e1 ? xb[2][k] : xa[2][k]) : 0.f;
-                acc += (code & 2u) ? (e1 ? xb[0][k] : xa[0][k]) : 0.f;
-                if (!ZERO)
-                {
-                    acc += (code & 4u) ? pxs[o + sM] : 0.f;
-                    acc += (code & 8u) ? pxs[o - sM] : 0.f;
-                }
-                acc += (code & 16u) ? (e1 ? right : xb[1][k]) : 0.f;
-                acc += (code & 32u) ? (e1 ? xa[1][k] : left) : 0.f;
-                const float bv = e1 ? bB[k] : bA[k];
+                acc += (code & 1u) ? (e1 ? X[S1][k].y : X[S1][k].x) : 0.f;
+                acc += (code & 2u) ? (e1 ? X[Sm1][k].y : X[Sm1][k].x) : 0.f;
+                acc += (code & 4u) ? up : 0.f;
+                acc += (code & 8u) ? dn : 0.f;
+                acc += (code & 16u) ? (e1 ? right : xb) : 0.f;
+                acc += (code & 32u) ? (e1 ? xa : left) : 0.f;
+                const float bv = e1 ? Bv[S0][k].y : Bv[S0][k].x;
                 if (!(code & 128u))
-                    v = (acc - bv * icu) * kInvN[__popc(code & 63u)];
+                    v = (acc - bv * icu) * inv_count(code);
                 else
                 {
+                    const uint32_t o = off[k] + (uint32_t)e1;
                     const unsigned c16 = L.c16[(size_t)min(max(sp, 0), B.ns - 1) * (size_t)B.sS + o];
                     float d = 0.f;
 #pragma unroll
@@ -1247,11 +1289,11 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
                 }
             }
             rnew[k] = v;
-            R[2][k] = v;
+            R[S0][k] = v;
         }
-        __syncthreads();   // previous readers of red[sp & 1] (plane sp-2) are done; red[q & 1] is complete
 #pragma unroll
         for (int k = 0; k < K; k++) red[sp & 1][wave][fa[k] + e1] = rnew[k];
+        // (2) red[sp & 1] is complete; readers of old[] are done
         __syncthreads();
         const int q = sp - 1;
         if (!haloRow && q >= s0 && q <= s1)
@@ -1262,21 +1304,21 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
             for (int k = 0; k < K; k++)
             {
                 const int eq = e1;   // second-colour element at plane q
-                const unsigned code = eq ? cPB[k] : cPA[k];
+                const unsigned code = (eq ? okB[k] : okA[k]) ? ((C[Sm1][k] >> (8 * eq)) & 255u) : 0u;
                 float out = 0.f;
                 if (code & 64u)
                 {
                     const int f = fa[k] + eq;
                     float acc = 0.f;
-                    acc += (code & 1u) ? R[2][k] : 0.f;
-                    acc += (code & 2u) ? R[0][k] : 0.f;
+                    acc += (code & 1u) ? R[S0][k] : 0.f;
+                    acc += (code & 2u) ? R[Sm2][k] : 0.f;
                     acc += (code & 4u) ? rq[wave + 1][f] : 0.f;
                     acc += (code & 8u) ? rq[wave - 1][f] : 0.f;
-                    acc += (code & 16u) ? (eq ? rq[wave][f + 1] : R[1][k]) : 0.f;
-                    acc += (code & 32u) ? (eq ? R[1][k] : rq[wave][f - 1]) : 0.f;
-                    const float bv = eq ? bPB[k] : bPA[k];
+                    acc += (code & 16u) ? (eq ? rq[wave][f + 1] : R[Sm1][k]) : 0.f;
+                    acc += (code & 32u) ? (eq ? R[Sm1][k] : rq[wave][f - 1]) : 0.f;
+                    const float bv = eq ? Bv[Sm1][k].y : Bv[Sm1][k].x;
                     if (!(code & 128u))
-                        out = (acc - bv * icu) * kInvN[__popc(code & 63u)];
+                        out = (acc - bv * icu) * inv_count(code);
                     else
                     {
                         const unsigned c16 = L.c16[(size_t)q * (size_t)B.sS + off[k] + (uint32_t)eq];
@@ -1290,22 +1332,23 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
                         out = d != 0.f ? (acc - bv * icu) / d : 0.f;
                     }
                 }
-                if (okA[k]) po[off[k]] = eq ? R[1][k] : out;
-                if (okB[k]) po[off[k] + 1] = eq ? out : R[1][k];
+                const float va = eq ? R[Sm1][k] : out, vb = eq ? out : R[Sm1][k];
+                if (okB[k])
+                    *reinterpret_cast<float2 *>(po + off[k]) = make_float2(va, vb);
+                else if (okA[k])
+                    po[off[k]] = va;
             }
         }
-#pragma unroll
-        for (int k = 0; k < K; k++)
-        {
-            bPA[k] = bA[k];
-            bPB[k] = bB[k];
-            cPA[k] = cA[k];
-            cPB[k] = cB[k];
-            bA[k] = nbA[k];
-            bB[k] = nbB[k];
-            cA[k] = ncA[k];
-            cB[k] = ncB[k];
-        }
+    };
+    for (int sp = sp0; sp <= s1 + 1; sp += 4)
+    {
+        step(sp, std::integral_constant<int, 0>());
+        if (sp + 1 > s1 + 1) break;
+        step(sp + 1, std::integral_constant<int, 1>());
+        if (sp + 2 > s1 + 1) break;
+        step(sp + 2, std::integral_constant<int, 2>());
+        if (sp + 3 > s1 + 1) break;
+        step(sp + 3, std::integral_constant<int, 3>());
     }
 }
 
@@ -1865,11 +1908,20 @@ hipError_t launch_cg64_direction(hipStream_t s, const Box &B, const Tables64 &T,
     return hipGetLastError();
 }
 
+// The whole-row sweep reads and writes cell pairs (2p + 1, 2p + 2) with single aligned accesses: the arrays must
+// have element 1 of every row pair-aligned (odd float offsets 8-byte aligned, odd code offsets 2-byte aligned).
+inline bool rbgs_pairs_aligned(const Level32 &L, const float *xin, const float *xout, const float *b)
+{
+    auto oddAligned = [](const void *p, uintptr_t a) { return p == nullptr || (((uintptr_t)p + (a >> 1)) % a) == 0; };
+    return oddAligned(xin, 8) && oddAligned(xout, 8) && oddAligned(b, 8) && oddAligned(L.c8, 2) &&
+           (L.B.sM % 16) == 0 && (L.B.sS % 16) == 0;
+}
+
 hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float *xout, const float *b, int c1,
                        int zero, int tileRows, int planesPerChunk, const SolveState *st)
 {
     const Box &B = L.B;
-    if (B.threeD && L.uniform && B.nf - 2 <= 512)
+    if (B.threeD && L.uniform && B.nf - 2 <= 512 && rbgs_pairs_aligned(L, xin, xout, b))
     {
         const int K = B.nf - 2 <= 128 ? 1 : (B.nf - 2 <= 256 ? 2 : 4);
         const int TM = 6;

@@ -154,6 +154,18 @@ hipError_t dalloc(nppe_solver *h, T **p, size_t count)
     return hipMemsetAsync(*p, 0, bytes, h->stream);
 }
 
+// A box array (pitch and plane strides multiples of 16 elements) whose element 1 sits on a 16-element boundary:
+// the cell pair (2p + 1, 2p + 2) of every row then starts at an address aligned to the pair's size, so the
+// whole-row sweep reads a pair with one load (rbgs_pairs_aligned checks this before choosing that kernel).
+template <class T>
+hipError_t dalloc_pairs(nppe_solver *h, T **p, size_t count)
+{
+    T *raw = nullptr;
+    const hipError_t e = dalloc(h, &raw, count + 16);
+    *p = raw ? raw + 1 : nullptr;
+    return e;
+}
+
 void free_grid(nppe_solver *h)
 {
     if (h->innerGraph)
@@ -460,7 +472,7 @@ int build(nppe_solver *h, const nppe_grid *g)
                     std::memcpy(&p8[dst], &c8h[src], H.nf);
                     std::memcpy(&p16[dst], &c16h[src], H.nf * sizeof(uint16_t));
                 }
-            HIP_TRY(h, dalloc(h, &L.c8, p8.size()), "allocate codes");
+            HIP_TRY(h, dalloc_pairs(h, &L.c8, p8.size()), "allocate codes");
             HIP_TRY(h, dalloc(h, &L.c16, p16.size()), "allocate codes");
             HIP_TRY(h, hipMemcpy(L.c8, p8.data(), p8.size(), hipMemcpyHostToDevice), "upload codes");
             HIP_TRY(h, hipMemcpy(L.c16, p16.data(), p16.size() * sizeof(uint16_t), hipMemcpyHostToDevice),
@@ -495,9 +507,9 @@ int build(nppe_solver *h, const nppe_grid *g)
         }
         if (l > 0)
         {
-            HIP_TRY(h, dalloc(h, &L.x[0], (size_t)L.box.cells), "allocate level x");
-            HIP_TRY(h, dalloc(h, &L.x[1], (size_t)L.box.cells), "allocate level x");
-            HIP_TRY(h, dalloc(h, &L.b, (size_t)L.box.cells), "allocate level b");
+            HIP_TRY(h, dalloc_pairs(h, &L.x[0], (size_t)L.box.cells), "allocate level x");
+            HIP_TRY(h, dalloc_pairs(h, &L.x[1], (size_t)L.box.cells), "allocate level x");
+            HIP_TRY(h, dalloc_pairs(h, &L.b, (size_t)L.box.cells), "allocate level b");
         }
     }
     // Coarsest: dense pseudo-inverse when small enough.
@@ -560,12 +572,12 @@ int build(nppe_solver *h, const nppe_grid *g)
     HIP_TRY(h, dalloc(h, &h->dB64, cells), "allocate b64");
     if (h->opt.method == NPPE_METHOD_MGPCG)
     {
-        HIP_TRY(h, dalloc(h, &h->dE, cells), "allocate e");
-        HIP_TRY(h, dalloc(h, &h->dR, cells), "allocate r");
-        HIP_TRY(h, dalloc(h, &h->dP, cells), "allocate p");
-        HIP_TRY(h, dalloc(h, &h->dQ, cells), "allocate q");
-        HIP_TRY(h, dalloc(h, &h->dZ, cells), "allocate z");
-        HIP_TRY(h, dalloc(h, &h->dZ2, cells), "allocate z");
+        HIP_TRY(h, dalloc_pairs(h, &h->dE, cells), "allocate e");
+        HIP_TRY(h, dalloc_pairs(h, &h->dR, cells), "allocate r");
+        HIP_TRY(h, dalloc_pairs(h, &h->dP, cells), "allocate p");
+        HIP_TRY(h, dalloc_pairs(h, &h->dQ, cells), "allocate q");
+        HIP_TRY(h, dalloc_pairs(h, &h->dZ, cells), "allocate z");
+        HIP_TRY(h, dalloc_pairs(h, &h->dZ2, cells), "allocate z");
     }
     else
     {
