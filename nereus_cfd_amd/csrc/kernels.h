@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "nppe_internal.h"
 
 namespace nppe
@@ -43,6 +45,13 @@ hipError_t launch_cg64_direction(hipStream_t s, const Box &B, const Tables64 &T,
 // pass: tiles of 64 x tileRows (8 or 16) cells marching along the slow axis in chunks of planesPerChunk planes.
 hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float *xout, const float *b, int c1,
                        int zero, int tileRows, int planesPerChunk, const SolveState *st);
+// Which sweep kernel launch_rbgs runs for these operands, and the tile heights it accepts for it (every tile shape
+// gives bit-identical results: a colour's updates are independent of each other).
+enum { RBGS_ROW = 0, RBGS_MARCH = 1, RBGS_2D = 2 };
+int rbgs_family(const Level32 &L, const float *xin, const float *xout, const float *b);
+void rbgs_tile_candidates(int family, std::vector<int> &tileRows);
+// Blocks of one sweep launch of that family and tile shape.
+int64_t rbgs_blocks(const Box &B, int family, int tileRows, int planesPerChunk);
 hipError_t launch_residual_restrict(hipStream_t s, const Level32 &Lf, const float *x, const float *b,
                                     const Level32 &Lc, float *bc, const SolveState *st);
 hipError_t launch_prolong(hipStream_t s, const Level32 &Lf, float *x, const Level32 &Lc, const float *xc,

@@ -1910,32 +1910,69 @@ hipError_t launch_cg64_direction(hipStream_t s, const Box &B, const Tables64 &T,
 
 // The whole-row sweep reads and writes cell pairs (2p + 1, 2p + 2) with single aligned accesses: the arrays must
 // have element 1 of every row pair-aligned (odd float offsets 8-byte aligned, odd code offsets 2-byte aligned).
-inline bool rbgs_pairs_aligned(const Level32 &L, const float *xin, const float *xout, const float *b)
+bool rbgs_pairs_aligned(const Level32 &L, const float *xin, const float *xout, const float *b)
 {
     auto oddAligned = [](const void *p, uintptr_t a) { return p == nullptr || (((uintptr_t)p + (a >> 1)) % a) == 0; };
     return oddAligned(xin, 8) && oddAligned(xout, 8) && oddAligned(b, 8) && oddAligned(L.c8, 2) &&
            (L.B.sM % 16) == 0 && (L.B.sS % 16) == 0;
 }
 
+int rbgs_family(const Level32 &L, const float *xin, const float *xout, const float *b)
+{
+    const Box &B = L.B;
+    if (B.threeD && L.uniform && B.nf - 2 <= 512 && rbgs_pairs_aligned(L, xin, xout, b)) return RBGS_ROW;
+    return B.threeD ? RBGS_MARCH : RBGS_2D;
+}
+
+int64_t rbgs_blocks(const Box &B, int family, int tileRows, int planesPerChunk)
+{
+    const int P = std::max(1, planesPerChunk);
+    const int64_t chunks = B.threeD ? std::max(1, (B.ns - 2 + P - 1) / P) : 1;
+    if (family == RBGS_ROW)
+    {
+        const int TM = (tileRows >= 10 && B.nf - 2 <= 256) ? 14 : 6;
+        return (int64_t)std::max(1, (B.nm - 2 + TM - 1) / TM) * chunks;
+    }
+    if (family == RBGS_MARCH)
+    {
+        const int TM = tileRows <= 4 ? 4 : (tileRows <= 6 ? 6 : 8);
+        return (int64_t)std::max(1, (B.nf - 2 + kPW - 1) / kPW) * std::max(1, (B.nm - 2 + TM - 1) / TM) * chunks;
+    }
+    const int TM = tileRows == 8 ? 8 : 16;
+    return (int64_t)std::max(1, (B.nf - 2 + kTF - 1) / kTF) * std::max(1, (B.nm - 2 + TM - 1) / TM);
+}
+
+void rbgs_tile_candidates(int family, std::vector<int> &tileRows)
+{
+    if (family == RBGS_ROW) tileRows = {6, 14};
+    else if (family == RBGS_MARCH) tileRows = {4, 6, 8};
+    else tileRows = {8, 16};
+}
+
 hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float *xout, const float *b, int c1,
                        int zero, int tileRows, int planesPerChunk, const SolveState *st)
 {
     const Box &B = L.B;
-    if (B.threeD && L.uniform && B.nf - 2 <= 512 && rbgs_pairs_aligned(L, xin, xout, b))
+    if (rbgs_family(L, xin, xout, b) == RBGS_ROW)
     {
         const int K = B.nf - 2 <= 128 ? 1 : (B.nf - 2 <= 256 ? 2 : 4);
-        const int TM = 6;
+        const int TM = (tileRows >= 10 && K <= 2) ? 14 : 6;
         const int tilesM = std::max(1, (B.nm - 2 + TM - 1) / TM);
         const int P = std::max(1, planesPerChunk);
         const int chunks = std::max(1, (B.ns - 2 + P - 1) / P);
         const int grid = tilesM * chunks;
         const int threads = 64 * (TM + 2);
-#define NPPE_ROW(Kv)                                                                                                 \
-    (zero ? (k_rbgs_row<6, Kv, true><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st), 0)                  \
-          : (k_rbgs_row<6, Kv, false><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st), 0))
-        if (K == 1) NPPE_ROW(1);
-        else if (K == 2) NPPE_ROW(2);
-        else NPPE_ROW(4);
+#define NPPE_ROW(TMv, Kv)                                                                                            \
+    (zero ? (k_rbgs_row<TMv, Kv, true><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st), 0)                \
+          : (k_rbgs_row<TMv, Kv, false><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st), 0))
+        if (TM == 14)
+        {
+            if (K == 1) NPPE_ROW(14, 1);
+            else NPPE_ROW(14, 2);
+        }
+        else if (K == 1) NPPE_ROW(6, 1);
+        else if (K == 2) NPPE_ROW(6, 2);
+        else NPPE_ROW(6, 4);
 #undef NPPE_ROW
         return hipGetLastError();
     }

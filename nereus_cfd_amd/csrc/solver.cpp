@@ -8,9 +8,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <new>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/nppe.h"
@@ -379,6 +381,121 @@ int sync_state(nppe_solver *h)
 
 // ---- set-up ---------------------------------------------------------------------------------------------------
 
+// ---- smoother tiles ---------------------------------------------------------------------------------------------
+// Each level's sweep runs in tiles of tileRows rows marching planesPerChunk planes. Every shape gives bit-identical
+// results (the updates of one colour are independent), so the shape is chosen for speed alone: levels of at least
+// kTuneCells interior cells time every candidate shape on the device at commit (an autotuner; choices are cached
+// for the process per device and level shape), smaller levels take the shape a model picks. NPPE_RBGS_TM /
+// NPPE_RBGS_P force a shape and NPPE_AUTOTUNE=0 turns the timing off.
+constexpr int64_t kTuneCells = 32 * 32 * 32;
+
+struct TileKey
+{
+    int dev, ns, nm, nf, uniform, family;
+    bool operator<(const TileKey &o) const
+    {
+        return std::tie(dev, ns, nm, nf, uniform, family) < std::tie(o.dev, o.ns, o.nm, o.nf, o.uniform, o.family);
+    }
+};
+std::mutex gTileMu;
+std::map<TileKey, std::pair<int, int>> gTiles;
+
+// Model: rounds of resident blocks (two per CU) times the plane steps of a block (P + 2).
+int model_planes(const Box &B, int family, int tm, int cus)
+{
+    if (!B.threeD) return 1;
+    const int64_t cap = 2LL * std::max(1, cus);
+    int best = 1;
+    double bestCost = 1e300;
+    for (int P = 1; P <= std::min(64, B.ns - 2); P++)
+    {
+        const int64_t blocks = rbgs_blocks(B, family, tm, P);
+        const double cost = (double)((blocks + cap - 1) / cap) * (P + 2);
+        if (cost < bestCost) { bestCost = cost; best = P; }
+    }
+    return best;
+}
+
+hipError_t choose_tiles(nppe_solver *h)
+{
+    int cus = 256;
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, h->opt.device) == hipSuccess) cus = prop.multiProcessorCount;
+    }
+    const char *envTM = std::getenv("NPPE_RBGS_TM"), *envP = std::getenv("NPPE_RBGS_P");
+    const char *envTune = std::getenv("NPPE_AUTOTUNE");
+    const bool tune = !(envTune && std::atoi(envTune) == 0) && !envTM && !envP;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t err = hipSuccess;
+    for (size_t l = 0; l < h->levels.size(); l++)
+    {
+        DevLevel &L = h->levels[l];
+        float *xin = l == 0 ? h->dZ : L.x[0];
+        float *xout = l == 0 ? h->dZ2 : L.x[1];
+        const float *b = l == 0 ? h->dR : L.b;
+        if (!xin || !xout || !b) continue;   // not an FP32 multigrid handle
+        const int family = rbgs_family(L.v, xin, xout, b);
+        std::vector<int> tms;
+        rbgs_tile_candidates(family, tms);
+        L.tileRows = family == RBGS_2D ? 16 : 6;
+        L.planesPerChunk = model_planes(L.box, family, L.tileRows, cus);
+        if (envTM) L.tileRows = std::max(1, std::atoi(envTM));
+        if (envP) L.planesPerChunk = std::max(1, std::atoi(envP));
+        const int64_t interior = (int64_t)(L.box.threeD ? L.box.ns - 2 : 1) * (L.box.nm - 2) * (L.box.nf - 2);
+        if (!tune || interior < kTuneCells) continue;
+        const TileKey key{h->opt.device, L.box.ns, L.box.nm, L.box.nf, L.v.uniform, family};
+        {
+            std::lock_guard<std::mutex> lock(gTileMu);
+            auto it = gTiles.find(key);
+            if (it != gTiles.end())
+            {
+                L.tileRows = it->second.first;
+                L.planesPerChunk = it->second.second;
+                continue;
+            }
+        }
+        if (!e0 && ((err = hipEventCreate(&e0)) != hipSuccess || (err = hipEventCreate(&e1)) != hipSuccess)) break;
+        static const int kPlanes[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32};
+        float bestMs = 1e30f;
+        int bestTM = L.tileRows, bestP = L.planesPerChunk;
+        for (int tm : tms)
+            for (int P : kPlanes)
+            {
+                if (L.box.threeD ? P > std::max(1, L.box.ns - 2) : P != 1) continue;
+                constexpr int kReps = 4;
+                for (int r = 0; r <= kReps && err == hipSuccess; r++)
+                {
+                    if (r == 1) err = hipEventRecord(e0, h->stream);
+                    if (err == hipSuccess) err = launch_rbgs(h->stream, L.v, xin, xout, b, 1, 0, tm, P, h->dState);
+                }
+                if (err == hipSuccess) err = hipEventRecord(e1, h->stream);
+                if (err == hipSuccess) err = hipEventSynchronize(e1);
+                float ms = 0.f;
+                if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+                if (err != hipSuccess) break;
+                if (ms < bestMs) { bestMs = ms; bestTM = tm; bestP = P; }
+            }
+        if (err != hipSuccess) break;
+        L.tileRows = bestTM;
+        L.planesPerChunk = bestP;
+        std::lock_guard<std::mutex> lock(gTileMu);
+        gTiles[key] = {bestTM, bestP};
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    // the timing sweeps wrote the level buffers: clear them again
+    for (size_t l = 0; l < h->levels.size() && err == hipSuccess; l++)
+    {
+        DevLevel &L = h->levels[l];
+        float *bufs[2] = {l == 0 ? h->dZ : L.x[0], l == 0 ? h->dZ2 : L.x[1]};
+        for (float *p : bufs)
+            if (p && err == hipSuccess) err = hipMemsetAsync(p, 0, (size_t)L.box.cells * sizeof(float), h->stream);
+    }
+    if (err == hipSuccess) err = hipStreamSynchronize(h->stream);
+    return err;
+}
+
 int build(nppe_solver *h, const nppe_grid *g)
 {
     free_grid(h);
@@ -490,14 +607,6 @@ int build(nppe_solver *h, const nppe_grid *g)
         L.v.c16 = L.c16;
         L.v.uniform = H.uniform ? 1 : 0;
         {
-            // planes per marching chunk: about 1024 blocks of 6-row tiles, between 1 and 4 planes
-            const int rows = std::max(1, (L.box.nm - 2 + 5) / 6);
-            const int planes = L.box.threeD ? L.box.ns - 2 : 1;
-            L.planesPerChunk = std::max(1, std::min(4, (int)((int64_t)planes * rows / 1024)));
-        }
-        if (const char *e = std::getenv("NPPE_RBGS_TM")) L.tileRows = std::max(1, std::atoi(e));
-        if (const char *e = std::getenv("NPPE_RBGS_P")) L.planesPerChunk = std::max(1, std::atoi(e));
-        {
             // constant coefficient of a uniform level: sigma * area / gap = sigma * w (3D) or sigma (2D)
             const double w = H.w[1][0];
             const double cu = H.threeD ? sigma * w : sigma;
@@ -600,6 +709,7 @@ int build(nppe_solver *h, const nppe_grid *g)
     HIP_TRY(h, dalloc(h, &h->dPartials, (size_t)h->partialsCap), "allocate partials");
     HIP_TRY(h, dalloc(h, &h->dState, 1), "allocate state");
     HIP_TRY(h, hipStreamSynchronize(h->stream), "commit");
+    HIP_TRY(h, choose_tiles(h), "smoother tiles");
     h->committed = true;
     return NPPE_OK;
 }

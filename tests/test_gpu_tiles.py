@@ -1,0 +1,52 @@
+"""Smoother tile shapes: the autotuned shape and every forced shape (rows per tile, planes per chunk) give
+bit-identical solutions, because the updates of one red-black colour do not depend on each other."""
+import os
+
+import numpy as np
+import pytest
+
+import nereus_cfd_amd as N
+from tests.cases import make
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [("6", "1"), ("6", "5"), ("14", "3"), ("14", "16"), ("4", "2"), ("8", "7"), ("16", "1")]
+
+
+def _solve(g, b, env):
+    saved = {k: os.environ.get(k) for k in ("NPPE_RBGS_TM", "NPPE_RBGS_P", "NPPE_AUTOTUNE")}
+    try:
+        for k in saved:
+            os.environ.pop(k, None)
+        os.environ.update(env)
+        with N.Solver(tolerance=1e-9) as h:
+            h.commit(g)
+            x, res, rc = h.solve(b)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    assert rc == 0
+    return x, res
+
+
+@pytest.mark.parametrize("dim,n,kind,non_uniform", [
+    (3, 40, "mixed", False),        # whole-row sweep (uniform), Dirichlet codes, solids
+    (3, 36, "outlet", True),        # marching sweep (stretched)
+    (3, 72, "neumann", False),      # large enough for the timed autotuning
+    (2, 96, "mixed", False),        # 2D sweep
+])
+def test_tile_shapes_bit_identical(dim, n, kind, non_uniform):
+    g, _ = make(dim, n, kind, non_uniform, seed=3)
+    b = np.random.default_rng(5).standard_normal(g.unknowns)
+    if kind == "neumann":
+        b -= b.mean()
+    ref, rres = _solve(g, b, {})
+    for tm, p in SHAPES:
+        x, res = _solve(g, b, {"NPPE_RBGS_TM": tm, "NPPE_RBGS_P": p})
+        assert res.iterations == rres.iterations
+        assert np.array_equal(x, ref), (tm, p)
+    x, _ = _solve(g, b, {"NPPE_AUTOTUNE": "0"})
+    assert np.array_equal(x, ref)
