@@ -43,8 +43,14 @@ hipError_t launch_cg64_direction(hipStream_t s, const Box &B, const Tables64 &T,
 // ---- FP32 multigrid ------------------------------------------------------------------------------------------
 // One full red-black Gauss-Seidel sweep x_out = S(x_in) (c1: colour updated first; zero: x_in taken as 0), in one
 // pass: tiles of 64 x tileRows (8 or 16) cells marching along the slow axis in chunks of planesPerChunk planes.
+// With rzPartials non-null the sweep is the CG's last level-0 sweep and also reduces <b, x_out> into st, as
+// launch_cg_rz(b, x_out) does. Where the sweep kernel computes the block sums itself, *rzBlocks is set to their
+// count and launch_rz_finalize(st, rzPartials, *rzBlocks) completes the reduction; otherwise *rzBlocks is 0 and
+// the reduction is complete.
 hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float *xout, const float *b, int c1,
-                       int zero, int tileRows, int planesPerChunk, const SolveState *st);
+                       int zero, int tileRows, int planesPerChunk, SolveState *st, double *rzPartials = nullptr,
+                       int *rzBlocks = nullptr);
+hipError_t launch_rz_finalize(hipStream_t s, SolveState *st, const double *partials, int nblk);
 // Which sweep kernel launch_rbgs runs for these operands, and the tile heights it accepts for it (every tile shape
 // gives bit-identical results: a colour's updates are independent of each other).
 enum { RBGS_ROW = 0, RBGS_MARCH = 1, RBGS_2D = 2 };
@@ -69,9 +75,16 @@ hipError_t launch_cg_direction(hipStream_t s, const Level32 &L, const float *z, 
 // q = A p, <p, q> -> alpha.
 hipError_t launch_cg_spmv(hipStream_t s, const Level32 &L, const float *p, float *q, SolveState *st,
                           double *partials);
-// e += alpha p (e = alpha p on the first iteration), r -= alpha q, <r, r> -> convergence flag.
-hipError_t launch_cg_update(hipStream_t s, const Level32 &L, float *e, float *r, const float *p, const float *q,
-                            SolveState *st, double *partials);
+// Whether launch_cg_dirspmv serves this level (3D boxes); otherwise launch_cg_direction + launch_cg_spmv.
+bool cg_fused_direction(const Level32 &L);
+// p = z + beta p and q = A p with <p, q> -> alpha in one pass; p is read from one of (pA, pB) and written to the
+// other (pA written on even inner iterations).
+hipError_t launch_cg_dirspmv(hipStream_t s, const Level32 &L, const float *z, float *pA, float *pB, float *q,
+                             SolveState *st, double *partials);
+// e += alpha p (e = alpha p on the first iteration), r -= alpha q, <r, r> -> convergence flag; p is pA on even
+// inner iterations and pB on odd ones (pA == pB for an in-place direction).
+hipError_t launch_cg_update(hipStream_t s, const Level32 &L, float *e, float *r, const float *pA, const float *pB,
+                            const float *q, SolveState *st, double *partials);
 // Inner-solve set-up from the outer residual norm st->sums[1]: thr2 = red^2 * sums[1], counters cleared.
 hipError_t launch_inner_setup(hipStream_t s, SolveState *st, double reduction, int maxIters);
 // FP64 CG set-up: thr2, iteration cap, counters cleared.

@@ -1105,10 +1105,13 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_march(Level32 L, const f
 // use: the old x of the mid-axis neighbour rows comes from the neighbouring waves through LDS (each wave publishes
 // its own row), and the halo waves, whose outer neighbour row belongs to no wave of the block, prefetch that row.
 // So no load is waited for in the plane that issues it.
+// With dot non-null the sweep also writes its block's part of <b, x_out> over the unknowns to dot[block] (the
+// CG's <r, z> when the sweep is the last one of the V-cycle on the finest level).
 template <int TM, int K, bool ZERO>
 __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const float *__restrict__ xin,
                                                             float *__restrict__ xout, const float *__restrict__ b,
-                                                            int c1, int P, int tilesM, const SolveState *st)
+                                                            int c1, int P, int tilesM, const SolveState *st,
+                                                            double *__restrict__ dot)
 {
     if (st->done) return;
     const Box &B = L.B;
@@ -1198,6 +1201,7 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
     __syncthreads();
     const float icu = L.icu;
     const int wUp = min(wave + 1, TM + 1), wDn = max(wave - 1, 0);
+    double bx = 0.0;   // <b, x_out> of this lane's stored cells
 
     // One plane step at sp = sp0 + 4g + U: first colour of plane sp, second colour (and store) of plane sp - 1.
     auto step = [&](int sp, auto phase) {
@@ -1333,6 +1337,12 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
                     }
                 }
                 const float va = eq ? R[Sm1][k] : out, vb = eq ? out : R[Sm1][k];
+                if (dot)
+                {
+                    const unsigned ca = okA[k] ? (C[Sm1][k] & 255u) : 0u, cb = okB[k] ? ((C[Sm1][k] >> 8) & 255u) : 0u;
+                    bx += (ca & 64u) ? (double)Bv[Sm1][k].x * (double)va : 0.0;
+                    bx += (cb & 64u) ? (double)Bv[Sm1][k].y * (double)vb : 0.0;
+                }
                 if (okB[k])
                     *reinterpret_cast<float2 *>(po + off[k]) = make_float2(va, vb);
                 else if (okA[k])
@@ -1349,6 +1359,19 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
         step(sp + 2, std::integral_constant<int, 2>());
         if (sp + 3 > s1 + 1) break;
         step(sp + 3, std::integral_constant<int, 3>());
+    }
+    if (dot)
+    {
+        __shared__ double wsum[TM + 2];
+        const double v = wave_sum(bx);
+        if (lane == 0) wsum[wave] = v;
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            double t = 0.0;
+            for (int w = 0; w < TM + 2; w++) t += wsum[w];
+            dot[lb] = t;
+        }
     }
 }
 
@@ -1482,13 +1505,17 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_spmv(Level32 L, const floa
     block_sum<1>(a, partials, gridDim.x, tr.lb);
 }
 
+// p is pA on even inner iterations and pB on odd ones (the fused direction kernel ping-pongs; pass pA == pB when
+// the direction is updated in place).
 __global__ __launch_bounds__(kBlockThreads) void k_cg_update(Level32 L, float *__restrict__ e, float *__restrict__ r,
-                                                             const float *__restrict__ p, const float *__restrict__ q,
+                                                             const float *__restrict__ pA,
+                                                             const float *__restrict__ pB, const float *__restrict__ q,
                                                              const SolveState *st, double *partials)
 {
     if (st->done) return;
     const Box &B = L.B;
     const int firstIter = st->iters == 0;
+    const float *__restrict__ p = (st->iters & 1) ? pB : pA;
     const float alpha = (float)st->alpha;
     const TileRange tr = my_tiles(B.blocks());
     double rr = 0.0;
@@ -1551,12 +1578,11 @@ __device__ __forceinline__ float apply32v(const Level32 &L, int s, int m, int f,
 // (A x) of one cell in FP64 with the reference's exact arithmetic, from its value and neighbours (order as above):
 // the same sums in the same order as apply64 / row64.
 __device__ __forceinline__ double apply64v(const Box &B, const Tables64 &T, const uint8_t *ty, int s, int m, int f,
-                                           int64_t idx, double xc, const double (&nb)[6])
+                                           int64_t idx, double xc, const double (&nb)[6], unsigned code)
 {
 #pragma clang fp contract(off)
     if (T.unitCoef)
     {
-        const unsigned code = T.c8[idx];
         if (!(code & 128u))
         {
             const double v = T.scale * 1.0;
@@ -1610,6 +1636,71 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_spmv_m(Level32 L, const fl
             pm = pc;
             pc = pp;
         }
+    }
+    double a[1] = {pq};
+    block_sum<1>(a, partials, gridDim.x, M.lb);
+}
+
+// The CG direction and its product in one pass (3D boxes): p_new = z + beta p_old (z on the first direction),
+// q = A p_new, and <p_new, q>. p ping-pongs between pA (even inner iterations) and pB (odd), so every block reads
+// only old directions. A plane's p_new is formed once per cell (and per halo cell) into LDS; the slow neighbours
+// stay in registers; every load is issued a plane ahead (the cell's own values two planes ahead).
+__global__ __launch_bounds__(kBlockThreads) void k_cg_dirspmv_t(Level32 L, const float *__restrict__ z,
+                                                                float *__restrict__ pA, float *__restrict__ pB,
+                                                                float *__restrict__ q, const SolveState *st,
+                                                                double *partials, int P)
+{
+    if (st->done) return;
+    const Box &B = L.B;
+    const int first = st->first;
+    const float beta = (float)st->beta;
+    const bool odd = st->iters & 1;
+    float *__restrict__ pn = odd ? pB : pA;
+    const float *__restrict__ po = odd ? pA : pB;
+    __shared__ float Ps[kBlockMid + 2][kBlockFast + 2];
+    const March M = march_setup(B, P);
+    const int tx = threadIdx.x & (kBlockFast - 1), tyy = threadIdx.x / kBlockFast;
+    const int f0 = M.f - tx, m0 = M.m - tyy;
+    const int fc = min(M.f, B.nf - 1), mcl = min(M.m, B.nm - 1);
+    const int64_t own = (int64_t)mcl * B.sM + fc;
+    int hx = -1, hy = -1;
+    const int t = threadIdx.x;
+    if (t < 64) { hy = 0; hx = t + 1; }
+    else if (t < 128) { hy = kBlockMid + 1; hx = t - 64 + 1; }
+    else if (t < 128 + kBlockMid) { hy = t - 128 + 1; hx = 0; }
+    else if (t < 128 + 2 * kBlockMid) { hy = t - 128 - kBlockMid + 1; hx = kBlockFast + 1; }
+    const bool halo = hx >= 0;
+    const int64_t hoff = halo ? (int64_t)min(max(m0 - 1 + hy, 0), B.nm - 1) * B.sM + min(max(f0 - 1 + hx, 0), B.nf - 1)
+                              : own;
+    auto PV = [&](int64_t i) { return first ? z[i] : z[i] + beta * po[i]; };
+    auto plane = [&](int sl) { return (int64_t)min(max(sl, 0), B.ns - 1) * B.sS; };
+    double pq = 0.0;
+    float Pm = PV(plane(M.s0 - 1) + own), Pc = PV(plane(M.s0) + own), Pp = PV(plane(M.s0 + 1) + own);
+    float H = halo ? PV(plane(M.s0) + hoff) : 0.f;
+    unsigned cc = L.c8[plane(M.s0) + own];
+    for (int sl = M.s0; sl <= M.s1; sl++)
+    {
+        const float Pn = PV(plane(sl + 2) + own);
+        const float Hn = halo ? PV(plane(sl + 1) + hoff) : 0.f;
+        const unsigned cn = L.c8[plane(sl + 1) + own];
+        Ps[tyy + 1][tx + 1] = Pc;
+        if (halo) Ps[hy][hx] = H;
+        __syncthreads();
+        const int64_t idx = plane(sl) + own;
+        if (M.in && (cc & 64u))
+        {
+            const float nb[6] = {Pp, Pm, Ps[tyy + 2][tx + 1], Ps[tyy][tx + 1], Ps[tyy + 1][tx + 2], Ps[tyy + 1][tx]};
+            const float v = apply32v(L, sl, M.m, M.f, idx, cc, Pc, nb);
+            pn[idx] = Pc;
+            q[idx] = v;
+            pq += (double)Pc * (double)v;
+        }
+        __syncthreads();
+        Pm = Pc;
+        Pc = Pp;
+        Pp = Pn;
+        H = Hn;
+        cc = cn;
     }
     double a[1] = {pq};
     block_sum<1>(a, partials, gridDim.x, M.lb);
@@ -1691,7 +1782,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64_m(Box B, Ta
             if (T.c8[idx] & 64u)
             {
                 const double nb[6] = {xp, xm, X(idx + B.sM), X(idx - B.sM), X(idx + 1), X(idx - 1)};
-                r = b[idx] - apply64v(B, T, ty, s, M.m, M.f, idx, xc, nb);
+                r = b[idx] - apply64v(B, T, ty, s, M.m, M.f, idx, xc, nb, T.c8[idx]);
                 if (e) xout[idx] = xc;
             }
             if (r32) r32[idx] = (float)r;
@@ -1700,6 +1791,81 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64_m(Box B, Ta
             xm = xc;
             xc = xp;
         }
+    }
+    double a[1] = {rr};
+    block_sum<1>(a, partials, gridDim.x, M.lb);
+}
+
+// As k_update_residual64_m, with the plane staged in LDS: each thread forms X = x + e of its cell (and, for 136 of
+// the 256 threads, of one halo cell) once per plane, the in-plane neighbours come from LDS, and the loads of a
+// plane are issued a plane ahead (the cell's own X two planes ahead). Tile positions outside the box load clamped
+// in-box values that no interior cell couples to.
+__global__ __launch_bounds__(kBlockThreads) void k_update_residual64_t(Box B, Tables64 T,
+                                                                       const uint8_t *__restrict__ ty,
+                                                                       const double *__restrict__ x,
+                                                                       const float *__restrict__ e,
+                                                                       double *__restrict__ xout,
+                                                                       const double *__restrict__ b,
+                                                                       float *__restrict__ r32,
+                                                                       double *__restrict__ r64, double *partials,
+                                                                       int P)
+{
+#pragma clang fp contract(off)
+    __shared__ double Xs[kBlockMid + 2][kBlockFast + 2];
+    const March M = march_setup(B, P);
+    const int tx = threadIdx.x & (kBlockFast - 1), tyy = threadIdx.x / kBlockFast;
+    const int f0 = M.f - tx, m0 = M.m - tyy;
+    // own cell (clamped into the box) and its halo assignment
+    const int fc = min(M.f, B.nf - 1), mcl = min(M.m, B.nm - 1);
+    const int64_t own = (int64_t)mcl * B.sM + fc;
+    int hx = -1, hy = -1;
+    const int t = threadIdx.x;
+    if (t < 64) { hy = 0; hx = t + 1; }
+    else if (t < 128) { hy = kBlockMid + 1; hx = t - 64 + 1; }
+    else if (t < 128 + kBlockMid) { hy = t - 128 + 1; hx = 0; }
+    else if (t < 128 + 2 * kBlockMid) { hy = t - 128 - kBlockMid + 1; hx = kBlockFast + 1; }
+    const bool halo = hx >= 0;
+    const int64_t hoff = halo ? (int64_t)min(max(m0 - 1 + hy, 0), B.nm - 1) * B.sM + min(max(f0 - 1 + hx, 0), B.nf - 1)
+                              : own;
+    auto X = [&](int64_t i) { return e ? x[i] + (double)e[i] : x[i]; };
+    auto plane = [&](int s) { return (int64_t)min(max(s, 0), B.ns - 1) * B.sS; };
+    double rr = 0.0;
+    // registers: own X at s-1, s, s+1; halo X at s; b and code at s
+    double Xm = X(plane(M.s0 - 1) + own), Xc = X(plane(M.s0) + own), Xp = X(plane(M.s0 + 1) + own);
+    double H = halo ? X(plane(M.s0) + hoff) : 0.0;
+    double bc = b[plane(M.s0) + own];
+    unsigned cc = T.c8[plane(M.s0) + own];
+    for (int s = M.s0; s <= M.s1; s++)
+    {
+        // prefetch: own X two planes ahead, halo, b and code one plane ahead
+        const double Xn = X(plane(s + 2) + own);
+        const double Hn = halo ? X(plane(s + 1) + hoff) : 0.0;
+        const double bn = b[plane(s + 1) + own];
+        const unsigned cn = T.c8[plane(s + 1) + own];
+        Xs[tyy + 1][tx + 1] = Xc;
+        if (halo) Xs[hy][hx] = H;
+        __syncthreads();
+        const int64_t idx = plane(s) + own;
+        double r = 0.0;
+        if (M.in && (cc & 64u))
+        {
+            const double nb[6] = {Xp, Xm, Xs[tyy + 2][tx + 1], Xs[tyy][tx + 1], Xs[tyy + 1][tx + 2], Xs[tyy + 1][tx]};
+            r = bc - apply64v(B, T, ty, s, M.m, M.f, idx, Xc, nb, cc);
+            if (e) xout[idx] = Xc;
+        }
+        if (M.in)
+        {
+            if (r32) r32[idx] = (float)r;
+            if (r64) r64[idx] = r;
+        }
+        rr += r * r;
+        __syncthreads();
+        Xm = Xc;
+        Xc = Xp;
+        Xp = Xn;
+        H = Hn;
+        bc = bn;
+        cc = cn;
     }
     double a[1] = {rr};
     block_sum<1>(a, partials, gridDim.x, M.lb);
@@ -1720,7 +1886,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_apply64_m(Box B, Tables64 T, 
         if (T.c8[idx] & 64u)
         {
             const double nb[6] = {xp, xm, x[idx + B.sM], x[idx - B.sM], x[idx + 1], x[idx - 1]};
-            v = apply64v(B, T, ty, s, M.m, M.f, idx, xc, nb);
+            v = apply64v(B, T, ty, s, M.m, M.f, idx, xc, nb, T.c8[idx]);
         }
         y[idx] = v;
         xm = xc;
@@ -1746,7 +1912,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg64_spmv_m(Box B, Tables64 T
             if (T.c8[idx] & 64u)
             {
                 const double nb[6] = {pp, pm, p[idx + B.sM], p[idx - B.sM], p[idx + 1], p[idx - 1]};
-                v = apply64v(B, T, ty, s, M.m, M.f, idx, pc, nb);
+                v = apply64v(B, T, ty, s, M.m, M.f, idx, pc, nb, T.c8[idx]);
                 pq += pc * v;
             }
             q[idx] = v;
@@ -1844,7 +2010,7 @@ hipError_t launch_update_residual64(hipStream_t s, const Box &B, const Tables64 
     if (B.threeD)
     {
         const int P = march_planes(B), G = march_grid(B, P);
-        k_update_residual64_m<<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, partials, P);
+        k_update_residual64_t<<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, partials, P);
         LAUNCH_OK();
         return finalize(s, st, partials, G, 1, FIN_SUM1);
     }
@@ -1950,8 +2116,16 @@ void rbgs_tile_candidates(int family, std::vector<int> &tileRows)
 }
 
 hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float *xout, const float *b, int c1,
-                       int zero, int tileRows, int planesPerChunk, const SolveState *st)
+                       int zero, int tileRows, int planesPerChunk, SolveState *st, double *rzPartials,
+                       int *rzBlocks)
 {
+    if (rzBlocks) *rzBlocks = 0;
+    if (rzPartials && rbgs_family(L, xin, xout, b) != RBGS_ROW)
+    {
+        hipError_t e = launch_rbgs(s, L, xin, xout, b, c1, zero, tileRows, planesPerChunk, st, nullptr, nullptr);
+        if (e != hipSuccess) return e;
+        return launch_cg_rz(s, L, b, xout, st, rzPartials);
+    }
     const Box &B = L.B;
     if (rbgs_family(L, xin, xout, b) == RBGS_ROW)
     {
@@ -1963,8 +2137,8 @@ hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float 
         const int grid = tilesM * chunks;
         const int threads = 64 * (TM + 2);
 #define NPPE_ROW(TMv, Kv)                                                                                            \
-    (zero ? (k_rbgs_row<TMv, Kv, true><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st), 0)                \
-          : (k_rbgs_row<TMv, Kv, false><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st), 0))
+    (zero ? (k_rbgs_row<TMv, Kv, true><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st, rzPartials), 0)    \
+          : (k_rbgs_row<TMv, Kv, false><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st, rzPartials), 0))
         if (TM == 14)
         {
             if (K == 1) NPPE_ROW(14, 1);
@@ -1974,6 +2148,7 @@ hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float 
         else if (K == 2) NPPE_ROW(6, 2);
         else NPPE_ROW(6, 4);
 #undef NPPE_ROW
+        if (rzPartials && rzBlocks) *rzBlocks = grid;
         return hipGetLastError();
     }
     if (B.threeD)
@@ -2040,6 +2215,11 @@ hipError_t launch_coarse_solve(hipStream_t s, int n, const int32_t *order, const
     return hipGetLastError();
 }
 
+hipError_t launch_rz_finalize(hipStream_t s, SolveState *st, const double *partials, int nblk)
+{
+    return finalize(s, st, partials, nblk, 1, FIN_RZ);
+}
+
 hipError_t launch_cg_rz(hipStream_t s, const Level32 &L, const float *r, const float *z, SolveState *st,
                         double *partials)
 {
@@ -2071,11 +2251,22 @@ hipError_t launch_cg_spmv(hipStream_t s, const Level32 &L, const float *p, float
     return finalize(s, st, partials, G, 1, FIN_PQ);
 }
 
-hipError_t launch_cg_update(hipStream_t s, const Level32 &L, float *e, float *r, const float *p, const float *q,
-                            SolveState *st, double *partials)
+bool cg_fused_direction(const Level32 &L) { return L.B.threeD; }
+
+hipError_t launch_cg_dirspmv(hipStream_t s, const Level32 &L, const float *z, float *pA, float *pB, float *q,
+                             SolveState *st, double *partials)
+{
+    const int P = march_planes(L.B), G = march_grid(L.B, P);
+    k_cg_dirspmv_t<<<G, kBlockThreads, 0, s>>>(L, z, pA, pB, q, st, partials, P);
+    LAUNCH_OK();
+    return finalize(s, st, partials, G, 1, FIN_PQ);
+}
+
+hipError_t launch_cg_update(hipStream_t s, const Level32 &L, float *e, float *r, const float *pA, const float *pB,
+                            const float *q, SolveState *st, double *partials)
 {
     const int G = grid_of(L.B);
-    k_cg_update<<<G, kBlockThreads, 0, s>>>(L, e, r, p, q, st, partials);
+    k_cg_update<<<G, kBlockThreads, 0, s>>>(L, e, r, pA, pB, q, st, partials);
     LAUNCH_OK();
     return finalize(s, st, partials, G, 1, FIN_RR);
 }

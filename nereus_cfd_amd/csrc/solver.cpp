@@ -79,7 +79,7 @@ struct nppe_solver
     Tables64 t64{};
     double *dX64[2] = {nullptr, nullptr};
     double *dB64 = nullptr;
-    float *dE = nullptr, *dR = nullptr, *dP = nullptr, *dQ = nullptr, *dZ = nullptr, *dZ2 = nullptr;
+    float *dE = nullptr, *dR = nullptr, *dP = nullptr, *dP2 = nullptr, *dQ = nullptr, *dZ = nullptr, *dZ2 = nullptr;
     double *dR64 = nullptr, *dP64 = nullptr, *dQ64 = nullptr;
     double *dCompB = nullptr, *dCompX = nullptr, *dCompG = nullptr;
     double *dCoarseM = nullptr;
@@ -184,7 +184,7 @@ void free_grid(nppe_solver *h)
     h->dTab64 = nullptr;
     h->dX64[0] = h->dX64[1] = nullptr;
     h->dB64 = nullptr;
-    h->dE = h->dR = h->dP = h->dQ = h->dZ = h->dZ2 = nullptr;
+    h->dE = h->dR = h->dP = h->dP2 = h->dQ = h->dZ = h->dZ2 = nullptr;
     h->dR64 = h->dP64 = h->dQ64 = nullptr;
     h->dCompB = h->dCompX = h->dCompG = nullptr;
     h->dCoarseM = nullptr;
@@ -244,9 +244,11 @@ void profile_collect(nppe_solver *h, int itersDone)
 double lvl_cells(const DevLevel &L) { return L.cellsBox; }
 
 // One V-cycle z = M^-1 r0 with zero initial guess on every level. Level l keeps its iterate in x[l][cur[l]]
-// (sweeps ping-pong between the pair). Returns, in *zsel, which of the two level-0 buffers holds z.
-hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel)
+// (sweeps ping-pong between the pair). Returns, in *zsel, which of the two level-0 buffers holds z; with
+// rzPartials non-null the last level-0 sweep also reduces <r0, z> into the state (*rzDone tells whether it did).
+hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel, double *rzPartials, bool *rzDone)
 {
+    *rzDone = false;
     const int nl = (int)h->levels.size();
     const double s = 4.0, f = 1.0;
     hipError_t e = hipSuccess;
@@ -328,10 +330,23 @@ hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel)
         for (int sw = 0; sw < nu2; sw++)
         {
             const double by = (3 * s + f) * N;
-            Scope sc(h, l == 0 ? NPPE_K_SMOOTH0 : NPPE_K_SMOOTH, by);
-            if ((e = launch_rbgs(h->stream, L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l), 1, 0, L.tileRows,
-                                 L.planesPerChunk, h->dState)) != hipSuccess)
-                return e;
+            const bool dot = l == 0 && sw == nu2 - 1 && rzPartials;
+            int rzBlocks = 0;
+            {
+                Scope sc(h, l == 0 ? NPPE_K_SMOOTH0 : NPPE_K_SMOOTH, by);
+                if ((e = launch_rbgs(h->stream, L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l), 1, 0,
+                                     L.tileRows, L.planesPerChunk, h->dState, dot ? rzPartials : nullptr,
+                                     &rzBlocks)) != hipSuccess)
+                    return e;
+            }
+            if (dot)
+            {
+                *rzDone = true;
+                Scope sc(h, NPPE_K_CG_DOT, 0.0);
+                if (rzBlocks > 0 && (e = launch_rz_finalize(h->stream, h->dState, rzPartials, rzBlocks)) != hipSuccess)
+                    return e;
+                if (rzBlocks == 0) tr.add(2 * s * N + f * N);   // the separate <r, z> pass
+            }
             cur[l] ^= 1;
             tr.add(by);
         }
@@ -347,27 +362,40 @@ hipError_t inner_iteration(nppe_solver *h, Traffic &tr)
     const double N = L0.cellsBox, s = 4.0, f = 1.0;
     hipError_t e;
     int zsel = 0;
-    if ((e = vcycle(h, tr, h->dR, &zsel)) != hipSuccess) return e;
+    bool rzDone = false;
+    if ((e = vcycle(h, tr, h->dR, &zsel, h->dPartials, &rzDone)) != hipSuccess) return e;
     const float *z = zsel ? h->dZ2 : h->dZ;
+    if (!rzDone)
     {
         Scope sc(h, NPPE_K_CG_DOT, 2 * s * N + f * N);
         if ((e = launch_cg_rz(h->stream, L0.v, h->dR, z, h->dState, h->dPartials)) != hipSuccess) return e;
     }
+    const bool fused = cg_fused_direction(L0.v);
+    float *pB = fused ? h->dP2 : h->dP;
+    if (fused)
     {
-        Scope sc(h, NPPE_K_CG_DIRECTION, 3 * s * N + f * N);
-        if ((e = launch_cg_direction(h->stream, L0.v, z, h->dP, h->dState)) != hipSuccess) return e;
+        // z, p_old, codes in; p_new, q out
+        Scope sc(h, NPPE_K_CG_SPMV, (4 * s + f) * N);
+        if ((e = launch_cg_dirspmv(h->stream, L0.v, z, h->dP, pB, h->dQ, h->dState, h->dPartials)) != hipSuccess)
+            return e;
     }
+    else
     {
+        {
+            Scope sc(h, NPPE_K_CG_DIRECTION, 3 * s * N + f * N);
+            if ((e = launch_cg_direction(h->stream, L0.v, z, h->dP, h->dState)) != hipSuccess) return e;
+        }
         Scope sc(h, NPPE_K_CG_SPMV, (2 * s + f) * N);
         if ((e = launch_cg_spmv(h->stream, L0.v, h->dP, h->dQ, h->dState, h->dPartials)) != hipSuccess) return e;
     }
     {
         Scope sc(h, NPPE_K_CG_UPDATE, 6 * s * N + f * N);
-        if ((e = launch_cg_update(h->stream, L0.v, h->dE, h->dR, h->dP, h->dQ, h->dState, h->dPartials)) !=
+        if ((e = launch_cg_update(h->stream, L0.v, h->dE, h->dR, h->dP, pB, h->dQ, h->dState, h->dPartials)) !=
             hipSuccess)
             return e;
     }
-    tr.add((2 * s + f) * N + (3 * s + f) * N + (2 * s + f) * N + (6 * s + f) * N);
+    tr.add((rzDone ? 0.0 : (2 * s + f) * N) + (fused ? (4 * s + f) * N : (3 * s + f) * N + (2 * s + f) * N) +
+           (6 * s + f) * N);
     return hipSuccess;
 }
 
@@ -685,6 +713,7 @@ int build(nppe_solver *h, const nppe_grid *g)
         HIP_TRY(h, dalloc_pairs(h, &h->dR, cells), "allocate r");
         HIP_TRY(h, dalloc_pairs(h, &h->dP, cells), "allocate p");
         HIP_TRY(h, dalloc_pairs(h, &h->dQ, cells), "allocate q");
+        HIP_TRY(h, dalloc_pairs(h, &h->dP2, cells), "allocate p");
         HIP_TRY(h, dalloc_pairs(h, &h->dZ, cells), "allocate z");
         HIP_TRY(h, dalloc_pairs(h, &h->dZ2, cells), "allocate z");
     }
@@ -1311,7 +1340,7 @@ uint64_t nppe_estimate_bytes(const nppe_grid *g, int32_t method)
         if (l == 0)
         {
             total += cells * 24.0;                                      // x64 pair, b64
-            total += cells * (method == NPPE_METHOD_CG64 ? 24.0 : 24.0);   // FP32 CG vectors / FP64 r, p, q
+            total += cells * (method == NPPE_METHOD_CG64 ? 24.0 : 28.0);   // FP64 r, p, q / FP32 e, r, p pair, q, z pair
             total += (double)(ns - (t3 ? 2 : 0)) * (nm - 2) * (nf - 2) * (24.0 + 4.0);   // staging + order
         }
         const int si = ns - 2, mi = nm - 2, fi = nf - 2;
