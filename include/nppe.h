@@ -194,6 +194,8 @@ int nppe_server_add_grid(nppe_server *s, uint64_t key, const nppe_grid *g);
 int64_t nppe_server_submit(nppe_server *s, uint64_t key, const double *b, double tolerance, double *x);
 /* Blocks until the request is done; its result, and its latency from submit to completion in milliseconds. */
 int nppe_server_wait(nppe_server *s, int64_t ticket, nppe_result *res, double *latency_ms);
+/* The error message of the request last waited on ("" when it succeeded); valid until the next wait. */
+const char *nppe_server_last_error(const nppe_server *s);
 /* Device bytes the server holds now, and the most it has held. */
 int nppe_server_memory(const nppe_server *s, uint64_t *now, uint64_t *peak);
 

@@ -43,14 +43,10 @@ hipError_t launch_cg64_direction(hipStream_t s, const Box &B, const Tables64 &T,
 // ---- FP32 multigrid ------------------------------------------------------------------------------------------
 // One full red-black Gauss-Seidel sweep x_out = S(x_in) (c1: colour updated first; zero: x_in taken as 0), in one
 // pass: tiles of 64 x tileRows (8 or 16) cells marching along the slow axis in chunks of planesPerChunk planes.
-// With rzPartials non-null the sweep is the CG's last level-0 sweep and also reduces <b, x_out> into st, as
-// launch_cg_rz(b, x_out) does. Where the sweep kernel computes the block sums itself, *rzBlocks is set to their
-// count and launch_rz_finalize(st, rzPartials, *rzBlocks) completes the reduction; otherwise *rzBlocks is 0 and
-// the reduction is complete.
+// With rzPartials non-null the sweep is the CG's last level-0 sweep and also reduces <b, x_out> into st (as
+// launch_cg_rz(b, x_out) does; inside the sweep kernel where it can).
 hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float *xout, const float *b, int c1,
-                       int zero, int tileRows, int planesPerChunk, SolveState *st, double *rzPartials = nullptr,
-                       int *rzBlocks = nullptr);
-hipError_t launch_rz_finalize(hipStream_t s, SolveState *st, const double *partials, int nblk);
+                       int zero, int tileRows, int planesPerChunk, SolveState *st, double *rzPartials = nullptr);
 // Which sweep kernel launch_rbgs runs for these operands, and the tile heights it accepts for it (every tile shape
 // gives bit-identical results: a colour's updates are independent of each other).
 enum { RBGS_ROW = 0, RBGS_MARCH = 1, RBGS_2D = 2 };

@@ -36,14 +36,32 @@ __device__ __forceinline__ double wave_sum(double v)
 }
 
 
-// Block sum of NV per-thread values, written by thread 0 to partials[a * nblk + blk] (no atomics: the sums over
-// blocks are taken in index order by k_finalize, so every reduction is deterministic).
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double *partials, int nblk, int blk)
+// A reduction's destination: the block partials, the state its finalisation updates, the FinOp to apply, and
+// whether the kernel finalises itself (fused) or a one-block k_finalize launch follows it.
+struct Red
 {
-    constexpr int kWaves = kBlockThreads / 64;
-    __shared__ double sh[NV][kWaves];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    double *partials;
+    SolveState *st;
+    int op;
+    int fused;
+};
+
+__device__ void fin_apply(SolveState *st, double *partials, int nblk, int nv, int op);
+
+// Block sum of NV per-thread values, written by thread 0 to partials[a * nblk + blk]. Fused, the last block of
+// the grid to finish (a ticket counter in the state) then sums the partials of all blocks in index order and
+// applies the FinOp (fin_apply); every thread of every block of the grid must call it. There the partials are
+// agent-scope atomic stores (written through to the device coherence point), each completed before its block
+// takes a ticket, and the last block reads them with agent-scope atomic loads; a device-scope fence instead would
+// write back the whole L2 of the XCD from every block. The completion wait lengthens every block, so large grids
+// leave the finalisation to a k_finalize launch. Values are never combined atomically: every reduction is
+// deterministic.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], const Red &red, int nblk, int blk)
+{
+    __shared__ double sh[NV][16];
+    __shared__ int lastBlock;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = blockDim.x >> 6;
 #pragma unroll
     for (int a = 0; a < NV; a++)
     {
@@ -51,17 +69,34 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *partials, int
         if (lane == 0) sh[a][w] = x;
     }
     __syncthreads();
+    if (!red.fused)
+    {
+        if (tid == 0)
+#pragma unroll
+            for (int a = 0; a < NV; a++)
+            {
+                double x = 0.0;
+                for (int q = 0; q < nw; q++) x += sh[a][q];
+                red.partials[(size_t)a * nblk + blk] = x;
+            }
+        return;
+    }
     if (tid == 0)
     {
 #pragma unroll
         for (int a = 0; a < NV; a++)
         {
             double x = 0.0;
-#pragma unroll
-            for (int q = 0; q < kWaves; q++) x += sh[a][q];
-            partials[(size_t)a * nblk + blk] = x;
+            for (int q = 0; q < nw; q++) x += sh[a][q];
+            __hip_atomic_store(&red.partials[(size_t)a * nblk + blk], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+        __builtin_amdgcn_s_waitcnt(0);   // the partial stores have completed
+        lastBlock = __hip_atomic_fetch_add(&red.st->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                    (unsigned)(nblk - 1);
     }
+    __syncthreads();
+    if (!lastBlock) return;
+    fin_apply(red.st, red.partials, nblk, NV, red.op);
 }
 
 // Logical block id after the XCD remap: consecutive hardware workgroup ids go round-robin to the eight XCDs, so
@@ -347,7 +382,7 @@ __device__ __forceinline__ double apply64(const Box &B, const Tables64 &T, const
 
 // ---- scalar finalisation -----------------------------------------------------------------------------------
 
-// What k_finalize does with the block partials of the kernel before it.
+// What the last block of a reducing kernel does with the block partials (fin_apply).
 enum FinOp
 {
     FIN_SUM0 = 0,     // sums[0] = t0
@@ -363,26 +398,27 @@ enum FinOp
 
 __device__ __forceinline__ bool fin_in_loop(int op) { return op >= FIN_RZ; }
 
-// One block: sums the partials of each value over the nblk blocks in index order, then applies op.
-__global__ __launch_bounds__(kBlockThreads) void k_finalize(SolveState *st, const double *__restrict__ partials,
-                                                            int nblk, int nv, int op)
+// Run by every thread of the last block of a reducing kernel: sums the partials of each value over the nblk
+// blocks in index order, applies op, and resets the ticket for the next reduction.
+__device__ void fin_apply(SolveState *st, double *partials, int nblk, int nv, int op)
 {
-    if (fin_in_loop(op) && st->done) return;
-    constexpr int kWaves = kBlockThreads / 64;
-    __shared__ double sh[2][kWaves];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    __shared__ double sh[2][16];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = blockDim.x >> 6;
     for (int a = 0; a < nv; a++)
     {
         double x = 0.0;
-        for (int q = tid; q < nblk; q += kBlockThreads) x += partials[(size_t)a * nblk + q];
+        for (int q = tid; q < nblk; q += blockDim.x)
+            x += __hip_atomic_load(&partials[(size_t)a * nblk + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         x = wave_sum(x);
         if (lane == 0) sh[a][w] = x;
     }
     __syncthreads();
     if (tid != 0) return;
+    __hip_atomic_store(&st->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (fin_in_loop(op) && st->done) return;
     double t[2] = {0.0, 0.0};
     for (int a = 0; a < nv; a++)
-        for (int q = 0; q < kWaves; q++) t[a] += sh[a][q];
+        for (int q = 0; q < nw; q++) t[a] += sh[a][q];
     switch (op)
     {
     case FIN_SUM0: st->sums[0] = t[0]; break;
@@ -423,11 +459,16 @@ __global__ __launch_bounds__(kBlockThreads) void k_finalize(SolveState *st, cons
     }
 }
 
+__global__ __launch_bounds__(kBlockThreads) void k_finalize(SolveState *st, double *partials, int nblk, int nv, int op)
+{
+    fin_apply(st, partials, nblk, nv, op);
+}
+
 // ---- FP64 kernels --------------------------------------------------------------------------------------------
 
 __global__ __launch_bounds__(kBlockThreads) void k_scatter64(int64_t n, const int32_t *__restrict__ order,
                                                              const double *__restrict__ src, double *__restrict__ box,
-                                                             double *partials, int withSum)
+                                                             Red red, int withSum)
 {
     double v2 = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x; q < n; q += (int64_t)gridDim.x * kBlockThreads)
@@ -438,7 +479,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_scatter64(int64_t n, const in
     }
     if (!withSum) return;
     double a[1] = {v2};
-    block_sum<1>(a, partials, gridDim.x, blockIdx.x);
+    block_sum<1>(a, red, gridDim.x, blockIdx.x);
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_gather64(int64_t n, const int32_t *__restrict__ order,
@@ -451,13 +492,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_gather64(int64_t n, const int
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_sum64(int64_t n, const int32_t *__restrict__ order,
-                                                         const double *__restrict__ box, double *partials)
+                                                         const double *__restrict__ box, Red red)
 {
     double acc = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x; q < n; q += (int64_t)gridDim.x * kBlockThreads)
         acc += box[order[q]];
     double a[1] = {acc};
-    block_sum<1>(a, partials, gridDim.x, blockIdx.x);
+    block_sum<1>(a, red, gridDim.x, blockIdx.x);
 }
 
 // One refinement step: xn = x + e (e may be null), written to xout, and r = b - A xn in FP64 with the reference's
@@ -469,7 +510,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64(Box B, Tabl
                                                                      double *__restrict__ xout,
                                                                      const double *__restrict__ b,
                                                                      float *__restrict__ r32, double *__restrict__ r64,
-                                                                     double *partials)
+                                                                     Red red)
 {
 #pragma clang fp contract(off)
     const TileRange tr = my_tiles(B.blocks());
@@ -499,7 +540,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64(Box B, Tabl
         rr += r * r;
     }
     double a[1] = {rr};
-    block_sum<1>(a, partials, gridDim.x, tr.lb);
+    block_sum<1>(a, red, gridDim.x, tr.lb);
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_apply64(Box B, Tables64 T, const uint8_t *__restrict__ ty,
@@ -538,7 +579,7 @@ __device__ __forceinline__ double inv_diag_at(const Box &B, const Tables64 &T, c
 
 __global__ __launch_bounds__(kBlockThreads) void k_cg64_init(Box B, Tables64 T, const uint8_t *__restrict__ ty,
                                                              const double *__restrict__ r, double *__restrict__ p,
-                                                             double *partials)
+                                                             Red red)
 {
     const TileRange tr = my_tiles(B.blocks());
     double rz = 0.0;
@@ -556,12 +597,12 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg64_init(Box B, Tables64 T, 
         p[idx] = pv;
     }
     double a[1] = {rz};
-    block_sum<1>(a, partials, gridDim.x, tr.lb);
+    block_sum<1>(a, red, gridDim.x, tr.lb);
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_cg64_spmv(Box B, Tables64 T, const uint8_t *__restrict__ ty,
                                                              const double *__restrict__ p, double *__restrict__ q,
-                                                             const SolveState *st, double *partials)
+                                                             const SolveState *st, Red red)
 {
     if (st->done) return;
     const TileRange tr = my_tiles(B.blocks());
@@ -580,14 +621,14 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg64_spmv(Box B, Tables64 T, 
         q[idx] = v;
     }
     double a[1] = {pq};
-    block_sum<1>(a, partials, gridDim.x, tr.lb);
+    block_sum<1>(a, red, gridDim.x, tr.lb);
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_cg64_update(Box B, Tables64 T, const uint8_t *__restrict__ ty,
                                                                double *__restrict__ x, double *__restrict__ r,
                                                                const double *__restrict__ p,
                                                                const double *__restrict__ q, const SolveState *st,
-                                                               double *partials)
+                                                               Red red)
 {
     if (st->done) return;
     const double alpha = st->alpha;
@@ -606,7 +647,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg64_update(Box B, Tables64 T
         rz += rv * (inv_diag_at(B, T, ty, s, m, f, idx) * rv);
     }
     double a[2] = {rr, rz};
-    block_sum<2>(a, partials, gridDim.x, tr.lb);
+    block_sum<2>(a, red, gridDim.x, tr.lb);
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_cg64_direction(Box B, Tables64 T, const uint8_t *__restrict__ ty,
@@ -1105,13 +1146,13 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_march(Level32 L, const f
 // use: the old x of the mid-axis neighbour rows comes from the neighbouring waves through LDS (each wave publishes
 // its own row), and the halo waves, whose outer neighbour row belongs to no wave of the block, prefetch that row.
 // So no load is waited for in the plane that issues it.
-// With dot non-null the sweep also writes its block's part of <b, x_out> over the unknowns to dot[block] (the
-// CG's <r, z> when the sweep is the last one of the V-cycle on the finest level).
+// With dot.partials non-null the sweep also reduces <b, x_out> over the unknowns into dot (the CG's <r, z> when the
+// sweep is the last one of the V-cycle on the finest level).
 template <int TM, int K, bool ZERO>
 __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const float *__restrict__ xin,
                                                             float *__restrict__ xout, const float *__restrict__ b,
                                                             int c1, int P, int tilesM, const SolveState *st,
-                                                            double *__restrict__ dot)
+                                                            Red dot)
 {
     if (st->done) return;
     const Box &B = L.B;
@@ -1337,7 +1378,7 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
                     }
                 }
                 const float va = eq ? R[Sm1][k] : out, vb = eq ? out : R[Sm1][k];
-                if (dot)
+                if (dot.partials)
                 {
                     const unsigned ca = okA[k] ? (C[Sm1][k] & 255u) : 0u, cb = okB[k] ? ((C[Sm1][k] >> 8) & 255u) : 0u;
                     bx += (ca & 64u) ? (double)Bv[Sm1][k].x * (double)va : 0.0;
@@ -1360,18 +1401,10 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
         if (sp + 3 > s1 + 1) break;
         step(sp + 3, std::integral_constant<int, 3>());
     }
-    if (dot)
+    if (dot.partials)
     {
-        __shared__ double wsum[TM + 2];
-        const double v = wave_sum(bx);
-        if (lane == 0) wsum[wave] = v;
-        __syncthreads();
-        if (threadIdx.x == 0)
-        {
-            double t = 0.0;
-            for (int w = 0; w < TM + 2; w++) t += wsum[w];
-            dot[lb] = t;
-        }
+        double a[1] = {bx};
+        block_sum<1>(a, dot, gridDim.x, lb);
     }
 }
 
@@ -1447,7 +1480,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_coarse_solve(int n, const int
 
 __global__ __launch_bounds__(kBlockThreads) void k_cg_rz(Level32 L, const float *__restrict__ r,
                                                          const float *__restrict__ z, const SolveState *st,
-                                                         double *partials)
+                                                         Red red)
 {
     if (st->done) return;
     const Box &B = L.B;
@@ -1461,7 +1494,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_rz(Level32 L, const float 
         if (L.c8[idx] & 64u) v += (double)r[idx] * (double)z[idx];
     }
     double a[1] = {v};
-    block_sum<1>(a, partials, gridDim.x, tr.lb);
+    block_sum<1>(a, red, gridDim.x, tr.lb);
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_cg_direction(Level32 L, const float *__restrict__ z,
@@ -1483,7 +1516,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_direction(Level32 L, const
 
 __global__ __launch_bounds__(kBlockThreads) void k_cg_spmv(Level32 L, const float *__restrict__ p,
                                                            float *__restrict__ q, const SolveState *st,
-                                                           double *partials)
+                                                           Red red)
 {
     if (st->done) return;
     const Box &B = L.B;
@@ -1502,7 +1535,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_spmv(Level32 L, const floa
         pq += (double)pc * (double)v;
     }
     double a[1] = {pq};
-    block_sum<1>(a, partials, gridDim.x, tr.lb);
+    block_sum<1>(a, red, gridDim.x, tr.lb);
 }
 
 // p is pA on even inner iterations and pB on odd ones (the fused direction kernel ping-pongs; pass pA == pB when
@@ -1510,7 +1543,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_spmv(Level32 L, const floa
 __global__ __launch_bounds__(kBlockThreads) void k_cg_update(Level32 L, float *__restrict__ e, float *__restrict__ r,
                                                              const float *__restrict__ pA,
                                                              const float *__restrict__ pB, const float *__restrict__ q,
-                                                             const SolveState *st, double *partials)
+                                                             const SolveState *st, Red red)
 {
     if (st->done) return;
     const Box &B = L.B;
@@ -1532,7 +1565,66 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_update(Level32 L, float *_
         rr += (double)rv * (double)rv;
     }
     double a[1] = {rr};
-    block_sum<1>(a, partials, gridDim.x, tr.lb);
+    block_sum<1>(a, red, gridDim.x, tr.lb);
+}
+
+// k_cg_update over the whole box as aligned quads, without the codes: outside the unknowns p, q and r are zero
+// (never written, or written as zero), so there e and r stay zero and add nothing to <r, r>. Quad j holds elements
+// 4j-1 .. 4j+2 of the pair-aligned arrays (element -1 is the allocation's leading pad).
+__global__ __launch_bounds__(kBlockThreads) void k_cg_update_v(int64_t nq, float4 *__restrict__ e,
+                                                               float4 *__restrict__ r,
+                                                               const float4 *__restrict__ pA,
+                                                               const float4 *__restrict__ pB,
+                                                               const float4 *__restrict__ q, const SolveState *st,
+                                                               Red red)
+{
+    if (st->done) return;
+    const bool firstIter = st->iters == 0;
+    const float alpha = (float)st->alpha;
+    const float4 *__restrict__ p = (st->iters & 1) ? pB : pA;
+    double rr = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x * 2;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x * 2 + threadIdx.x; j < nq; j += stride)
+    {
+        const int64_t j2 = j + blockDim.x;
+        const bool two = j2 < nq;
+        const float4 pv = p[j], qv = q[j], rv0 = r[j];
+        const float4 pv2 = two ? p[j2] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 qv2 = two ? q[j2] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 rv2 = two ? r[j2] : make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 ev = make_float4(0.f, 0.f, 0.f, 0.f), ev2 = ev;
+        if (!firstIter)
+        {
+            ev = e[j];
+            if (two) ev2 = e[j2];
+        }
+        auto upd = [&](float4 &ee, float4 &rr4, const float4 &pp, const float4 &qq) {
+            ee.x = firstIter ? alpha * pp.x : ee.x + alpha * pp.x;
+            ee.y = firstIter ? alpha * pp.y : ee.y + alpha * pp.y;
+            ee.z = firstIter ? alpha * pp.z : ee.z + alpha * pp.z;
+            ee.w = firstIter ? alpha * pp.w : ee.w + alpha * pp.w;
+            rr4.x = rr4.x - alpha * qq.x;
+            rr4.y = rr4.y - alpha * qq.y;
+            rr4.z = rr4.z - alpha * qq.z;
+            rr4.w = rr4.w - alpha * qq.w;
+        };
+        float4 r1 = rv0, r2 = rv2;
+        upd(ev, r1, pv, qv);
+        e[j] = ev;
+        r[j] = r1;
+        rr += (double)r1.x * (double)r1.x + (double)r1.y * (double)r1.y + (double)r1.z * (double)r1.z +
+              (double)r1.w * (double)r1.w;
+        if (two)
+        {
+            upd(ev2, r2, pv2, qv2);
+            e[j2] = ev2;
+            r[j2] = r2;
+            rr += (double)r2.x * (double)r2.x + (double)r2.y * (double)r2.y + (double)r2.z * (double)r2.z +
+                  (double)r2.w * (double)r2.w;
+        }
+    }
+    double a[1] = {rr};
+    block_sum<1>(a, red, gridDim.x, blockIdx.x);
 }
 
 // ---- 2.5D marching stencil kernels (3D boxes) ------------------------------------------------------------------
@@ -1612,7 +1704,7 @@ __device__ __forceinline__ double apply64v(const Box &B, const Tables64 &T, cons
 
 __global__ __launch_bounds__(kBlockThreads) void k_cg_spmv_m(Level32 L, const float *__restrict__ p,
                                                              float *__restrict__ q, const SolveState *st,
-                                                             double *partials, int P)
+                                                             Red red, int P)
 {
     if (st->done) return;
     const Box &B = L.B;
@@ -1638,7 +1730,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_spmv_m(Level32 L, const fl
         }
     }
     double a[1] = {pq};
-    block_sum<1>(a, partials, gridDim.x, M.lb);
+    block_sum<1>(a, red, gridDim.x, M.lb);
 }
 
 // The CG direction and its product in one pass (3D boxes): p_new = z + beta p_old (z on the first direction),
@@ -1648,7 +1740,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_spmv_m(Level32 L, const fl
 __global__ __launch_bounds__(kBlockThreads) void k_cg_dirspmv_t(Level32 L, const float *__restrict__ z,
                                                                 float *__restrict__ pA, float *__restrict__ pB,
                                                                 float *__restrict__ q, const SolveState *st,
-                                                                double *partials, int P)
+                                                                Red red, int P)
 {
     if (st->done) return;
     const Box &B = L.B;
@@ -1703,7 +1795,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_dirspmv_t(Level32 L, const
         cc = cn;
     }
     double a[1] = {pq};
-    block_sum<1>(a, partials, gridDim.x, M.lb);
+    block_sum<1>(a, red, gridDim.x, M.lb);
 }
 
 // Residual of every fine cell, summed over each 2 x 2 x 2 family into the coarse right-hand side: fast pairs are
@@ -1764,7 +1856,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64_m(Box B, Ta
                                                                        double *__restrict__ xout,
                                                                        const double *__restrict__ b,
                                                                        float *__restrict__ r32,
-                                                                       double *__restrict__ r64, double *partials,
+                                                                       double *__restrict__ r64, Red red,
                                                                        int P)
 {
 #pragma clang fp contract(off)
@@ -1793,7 +1885,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64_m(Box B, Ta
         }
     }
     double a[1] = {rr};
-    block_sum<1>(a, partials, gridDim.x, M.lb);
+    block_sum<1>(a, red, gridDim.x, M.lb);
 }
 
 // As k_update_residual64_m, with the plane staged in LDS: each thread forms X = x + e of its cell (and, for 136 of
@@ -1807,7 +1899,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64_t(Box B, Ta
                                                                        double *__restrict__ xout,
                                                                        const double *__restrict__ b,
                                                                        float *__restrict__ r32,
-                                                                       double *__restrict__ r64, double *partials,
+                                                                       double *__restrict__ r64, Red red,
                                                                        int P)
 {
 #pragma clang fp contract(off)
@@ -1868,7 +1960,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64_t(Box B, Ta
         cc = cn;
     }
     double a[1] = {rr};
-    block_sum<1>(a, partials, gridDim.x, M.lb);
+    block_sum<1>(a, red, gridDim.x, M.lb);
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_apply64_m(Box B, Tables64 T, const uint8_t *__restrict__ ty,
@@ -1896,7 +1988,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_apply64_m(Box B, Tables64 T, 
 
 __global__ __launch_bounds__(kBlockThreads) void k_cg64_spmv_m(Box B, Tables64 T, const uint8_t *__restrict__ ty,
                                                                const double *__restrict__ p, double *__restrict__ q,
-                                                               const SolveState *st, double *partials, int P)
+                                                               const SolveState *st, Red red, int P)
 {
     if (st->done) return;
     const March M = march_setup(B, P);
@@ -1921,7 +2013,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg64_spmv_m(Box B, Tables64 T
         }
     }
     double a[1] = {pq};
-    block_sum<1>(a, partials, gridDim.x, M.lb);
+    block_sum<1>(a, red, gridDim.x, M.lb);
 }
 
 __global__ void k_inner_setup(SolveState *st, double reduction, int maxIters)
@@ -1959,9 +2051,21 @@ inline int march_planes(const Box &B)
 inline int march_grid(const Box &B, int P) { return B.tilesF * B.tilesM * ((B.ns - 2 + P - 1) / P); }
 inline int grid_1d(int64_t n) { return (int)std::min<int64_t>((n + kBlockThreads - 1) / kBlockThreads, kMaxGrid); }
 
-hipError_t finalize(hipStream_t s, SolveState *st, const double *partials, int nblk, int nv, FinOp op)
+// Grids of at most kFuseBlocks blocks finalise their reduction in the kernel (no extra launch); larger grids are
+// followed by a k_finalize launch (the in-kernel ticket wait would lengthen every block).
+constexpr int kFuseBlocks = 512;
+
+Red red_of(double *partials, SolveState *st, FinOp op, int blocks)
 {
-    k_finalize<<<1, kBlockThreads, 0, s>>>(st, partials, nblk, nv, (int)op);
+    return Red{partials, st, (int)op, blocks <= kFuseBlocks ? 1 : 0};
+}
+
+// After a launch that reduced into red over nblk blocks: the separate finalisation when the kernel did not fuse it.
+hipError_t finish(hipStream_t s, const Red &red, int nblk, int nv)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || red.fused) return e;
+    k_finalize<<<1, kBlockThreads, 0, s>>>(red.st, red.partials, nblk, nv, red.op);
     return hipGetLastError();
 }
 
@@ -1979,10 +2083,11 @@ hipError_t launch_scatter64(hipStream_t s, int64_t n, const int32_t *order, cons
 {
     if (n <= 0) return hipSuccess;
     const int G = grid_1d(n);
-    k_scatter64<<<G, kBlockThreads, 0, s>>>(n, order, src, box, partials, sumSlot >= 0);
-    LAUNCH_OK();
-    if (sumSlot < 0) return hipSuccess;
-    return finalize(s, st, partials, G, 1, sumSlot == 0 ? FIN_SUM0 : (sumSlot == 1 ? FIN_SUM1 : FIN_SUM2));
+    const FinOp op = sumSlot == 0 ? FIN_SUM0 : (sumSlot == 1 ? FIN_SUM1 : FIN_SUM2);
+    const Red red = red_of(partials, st, op, G);
+    k_scatter64<<<G, kBlockThreads, 0, s>>>(n, order, src, box, red, sumSlot >= 0);
+    if (sumSlot < 0) return hipGetLastError();
+    return finish(s, red, G, 1);
 }
 
 hipError_t launch_gather64(hipStream_t s, int64_t n, const int32_t *order, const double *box, double *dst,
@@ -1998,9 +2103,10 @@ hipError_t launch_sum64(hipStream_t s, int64_t n, const int32_t *order, const do
 {
     if (n <= 0) return hipSuccess;
     const int G = grid_1d(n);
-    k_sum64<<<G, kBlockThreads, 0, s>>>(n, order, box, partials);
-    LAUNCH_OK();
-    return finalize(s, st, partials, G, 1, slot == 0 ? FIN_SUM0 : (slot == 1 ? FIN_SUM1 : FIN_SUM2));
+    const FinOp op = slot == 0 ? FIN_SUM0 : (slot == 1 ? FIN_SUM1 : FIN_SUM2);
+    const Red red = red_of(partials, st, op, G);
+    k_sum64<<<G, kBlockThreads, 0, s>>>(n, order, box, red);
+    return finish(s, red, G, 1);
 }
 
 hipError_t launch_update_residual64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty,
@@ -2010,14 +2116,14 @@ hipError_t launch_update_residual64(hipStream_t s, const Box &B, const Tables64 
     if (B.threeD)
     {
         const int P = march_planes(B), G = march_grid(B, P);
-        k_update_residual64_t<<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, partials, P);
-        LAUNCH_OK();
-        return finalize(s, st, partials, G, 1, FIN_SUM1);
+        const Red red = red_of(partials, st, FIN_SUM1, G);
+        k_update_residual64_t<<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, red, P);
+        return finish(s, red, G, 1);
     }
     const int G = grid_of(B);
-    k_update_residual64<<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, partials);
-    LAUNCH_OK();
-    return finalize(s, st, partials, G, 1, FIN_SUM1);
+    const Red red = red_of(partials, st, FIN_SUM1, G);
+    k_update_residual64<<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, red);
+    return finish(s, red, G, 1);
 }
 
 hipError_t launch_apply64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const double *x,
@@ -2037,9 +2143,9 @@ hipError_t launch_cg64_init(hipStream_t s, const Box &B, const Tables64 &T, cons
                             double *p, SolveState *st, double *partials)
 {
     const int G = grid_of(B);
-    k_cg64_init<<<G, kBlockThreads, 0, s>>>(B, T, ty, r, p, partials);
-    LAUNCH_OK();
-    return finalize(s, st, partials, G, 1, FIN_CG64_INIT);
+    const Red red = red_of(partials, st, FIN_CG64_INIT, G);
+    k_cg64_init<<<G, kBlockThreads, 0, s>>>(B, T, ty, r, p, red);
+    return finish(s, red, G, 1);
 }
 
 hipError_t launch_cg64_spmv(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const double *p,
@@ -2048,23 +2154,23 @@ hipError_t launch_cg64_spmv(hipStream_t s, const Box &B, const Tables64 &T, cons
     if (B.threeD)
     {
         const int P = march_planes(B), G = march_grid(B, P);
-        k_cg64_spmv_m<<<G, kBlockThreads, 0, s>>>(B, T, ty, p, q, st, partials, P);
-        LAUNCH_OK();
-        return finalize(s, st, partials, G, 1, FIN_CG64_PQ);
+        const Red red = red_of(partials, st, FIN_CG64_PQ, G);
+        k_cg64_spmv_m<<<G, kBlockThreads, 0, s>>>(B, T, ty, p, q, st, red, P);
+        return finish(s, red, G, 1);
     }
     const int G = grid_of(B);
-    k_cg64_spmv<<<G, kBlockThreads, 0, s>>>(B, T, ty, p, q, st, partials);
-    LAUNCH_OK();
-    return finalize(s, st, partials, G, 1, FIN_CG64_PQ);
+    const Red red = red_of(partials, st, FIN_CG64_PQ, G);
+    k_cg64_spmv<<<G, kBlockThreads, 0, s>>>(B, T, ty, p, q, st, red);
+    return finish(s, red, G, 1);
 }
 
 hipError_t launch_cg64_update(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, double *x,
                               double *r, const double *p, const double *q, SolveState *st, double *partials)
 {
     const int G = grid_of(B);
-    k_cg64_update<<<G, kBlockThreads, 0, s>>>(B, T, ty, x, r, p, q, st, partials);
-    LAUNCH_OK();
-    return finalize(s, st, partials, G, 2, FIN_CG64_UPDATE);
+    const Red red = red_of(partials, st, FIN_CG64_UPDATE, G);
+    k_cg64_update<<<G, kBlockThreads, 0, s>>>(B, T, ty, x, r, p, q, st, red);
+    return finish(s, red, G, 2);
 }
 
 hipError_t launch_cg64_direction(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty,
@@ -2116,13 +2222,11 @@ void rbgs_tile_candidates(int family, std::vector<int> &tileRows)
 }
 
 hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float *xout, const float *b, int c1,
-                       int zero, int tileRows, int planesPerChunk, SolveState *st, double *rzPartials,
-                       int *rzBlocks)
+                       int zero, int tileRows, int planesPerChunk, SolveState *st, double *rzPartials)
 {
-    if (rzBlocks) *rzBlocks = 0;
     if (rzPartials && rbgs_family(L, xin, xout, b) != RBGS_ROW)
     {
-        hipError_t e = launch_rbgs(s, L, xin, xout, b, c1, zero, tileRows, planesPerChunk, st, nullptr, nullptr);
+        hipError_t e = launch_rbgs(s, L, xin, xout, b, c1, zero, tileRows, planesPerChunk, st, nullptr);
         if (e != hipSuccess) return e;
         return launch_cg_rz(s, L, b, xout, st, rzPartials);
     }
@@ -2136,9 +2240,10 @@ hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float 
         const int chunks = std::max(1, (B.ns - 2 + P - 1) / P);
         const int grid = tilesM * chunks;
         const int threads = 64 * (TM + 2);
+        const Red dot = red_of(rzPartials, st, FIN_RZ, grid);
 #define NPPE_ROW(TMv, Kv)                                                                                            \
-    (zero ? (k_rbgs_row<TMv, Kv, true><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st, rzPartials), 0)    \
-          : (k_rbgs_row<TMv, Kv, false><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st, rzPartials), 0))
+    (zero ? (k_rbgs_row<TMv, Kv, true><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st, dot), 0)           \
+          : (k_rbgs_row<TMv, Kv, false><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st, dot), 0))
         if (TM == 14)
         {
             if (K == 1) NPPE_ROW(14, 1);
@@ -2148,8 +2253,7 @@ hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float 
         else if (K == 2) NPPE_ROW(6, 2);
         else NPPE_ROW(6, 4);
 #undef NPPE_ROW
-        if (rzPartials && rzBlocks) *rzBlocks = grid;
-        return hipGetLastError();
+        return rzPartials ? finish(s, dot, grid, 1) : hipGetLastError();
     }
     if (B.threeD)
     {
@@ -2215,18 +2319,13 @@ hipError_t launch_coarse_solve(hipStream_t s, int n, const int32_t *order, const
     return hipGetLastError();
 }
 
-hipError_t launch_rz_finalize(hipStream_t s, SolveState *st, const double *partials, int nblk)
-{
-    return finalize(s, st, partials, nblk, 1, FIN_RZ);
-}
-
 hipError_t launch_cg_rz(hipStream_t s, const Level32 &L, const float *r, const float *z, SolveState *st,
                         double *partials)
 {
     const int G = grid_of(L.B);
-    k_cg_rz<<<G, kBlockThreads, 0, s>>>(L, r, z, st, partials);
-    LAUNCH_OK();
-    return finalize(s, st, partials, G, 1, FIN_RZ);
+    const Red red = red_of(partials, st, FIN_RZ, G);
+    k_cg_rz<<<G, kBlockThreads, 0, s>>>(L, r, z, st, red);
+    return finish(s, red, G, 1);
 }
 
 hipError_t launch_cg_direction(hipStream_t s, const Level32 &L, const float *z, float *p, SolveState *st)
@@ -2241,14 +2340,14 @@ hipError_t launch_cg_spmv(hipStream_t s, const Level32 &L, const float *p, float
     if (L.B.threeD)
     {
         const int P = march_planes(L.B), G = march_grid(L.B, P);
-        k_cg_spmv_m<<<G, kBlockThreads, 0, s>>>(L, p, q, st, partials, P);
-        LAUNCH_OK();
-        return finalize(s, st, partials, G, 1, FIN_PQ);
+        const Red red = red_of(partials, st, FIN_PQ, G);
+        k_cg_spmv_m<<<G, kBlockThreads, 0, s>>>(L, p, q, st, red, P);
+        return finish(s, red, G, 1);
     }
     const int G = grid_of(L.B);
-    k_cg_spmv<<<G, kBlockThreads, 0, s>>>(L, p, q, st, partials);
-    LAUNCH_OK();
-    return finalize(s, st, partials, G, 1, FIN_PQ);
+    const Red red = red_of(partials, st, FIN_PQ, G);
+    k_cg_spmv<<<G, kBlockThreads, 0, s>>>(L, p, q, st, red);
+    return finish(s, red, G, 1);
 }
 
 bool cg_fused_direction(const Level32 &L) { return L.B.threeD; }
@@ -2257,18 +2356,32 @@ hipError_t launch_cg_dirspmv(hipStream_t s, const Level32 &L, const float *z, fl
                              SolveState *st, double *partials)
 {
     const int P = march_planes(L.B), G = march_grid(L.B, P);
-    k_cg_dirspmv_t<<<G, kBlockThreads, 0, s>>>(L, z, pA, pB, q, st, partials, P);
-    LAUNCH_OK();
-    return finalize(s, st, partials, G, 1, FIN_PQ);
+    const Red red = red_of(partials, st, FIN_PQ, G);
+    k_cg_dirspmv_t<<<G, kBlockThreads, 0, s>>>(L, z, pA, pB, q, st, red, P);
+    return finish(s, red, G, 1);
 }
 
 hipError_t launch_cg_update(hipStream_t s, const Level32 &L, float *e, float *r, const float *pA, const float *pB,
                             const float *q, SolveState *st, double *partials)
 {
+    auto quad = [](const float *p) { return ((uintptr_t)(p - 1) & 15u) == 0; };
+    if (quad(e) && quad(r) && quad(pA) && quad(pB) && quad(q))
+    {
+        // quads covering elements -1 .. cells-1 (the allocation extends at least 15 elements past the box)
+        const int64_t nq = (L.B.cells + 1 + 3) / 4;
+        const int G = (int)std::min<int64_t>((nq + 2 * kBlockThreads - 1) / (2 * kBlockThreads), kMaxGrid);
+        const Red red = red_of(partials, st, FIN_RR, G);
+        k_cg_update_v<<<G, kBlockThreads, 0, s>>>(nq, reinterpret_cast<float4 *>(e - 1),
+                                                   reinterpret_cast<float4 *>(r - 1),
+                                                   reinterpret_cast<const float4 *>(pA - 1),
+                                                   reinterpret_cast<const float4 *>(pB - 1),
+                                                   reinterpret_cast<const float4 *>(q - 1), st, red);
+        return finish(s, red, G, 1);
+    }
     const int G = grid_of(L.B);
-    k_cg_update<<<G, kBlockThreads, 0, s>>>(L, e, r, pA, pB, q, st, partials);
-    LAUNCH_OK();
-    return finalize(s, st, partials, G, 1, FIN_RR);
+    const Red red = red_of(partials, st, FIN_RR, G);
+    k_cg_update<<<G, kBlockThreads, 0, s>>>(L, e, r, pA, pB, q, st, red);
+    return finish(s, red, G, 1);
 }
 
 hipError_t launch_inner_setup(hipStream_t s, SolveState *st, double reduction, int maxIters)

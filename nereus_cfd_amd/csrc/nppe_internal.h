@@ -77,6 +77,7 @@ struct SolveState
     int iters;        // inner iterations taken
     int maxIters;     // inner cap
     int first;        // next direction update is the first of the solve (beta = 0)
+    unsigned ticket;  // blocks of the running reduction that have written their partial (0 between kernels)
 };
 
 }  // namespace nppe

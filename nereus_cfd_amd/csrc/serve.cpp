@@ -50,6 +50,7 @@ struct Request
     double submitted = 0.0, finished = 0.0;
     bool done = false;
     int rc = 0;
+    std::string err;
     nppe_result res{};
 };
 
@@ -78,6 +79,7 @@ struct nppe_server
     int64_t nextId = 0;
     uint64_t bytesNow = 0, bytesPeak = 0;
     bool stop = false;
+    std::string lastError;
     std::vector<std::thread> workers;
 
     // Under mu: the first queued request that can start now, reserving its grid; evicts idle residents (least
@@ -171,7 +173,10 @@ struct nppe_server
                 nppe_set_tolerance(h, r->tol);
                 rc = nppe_solve(h, r->b.data(), r->x, nullptr, 0.0, &r->res);
             }
+            std::string err;
+            if (rc != NPPE_OK && rc != NPPE_NOT_CONVERGED) err = nppe_last_error(h);
             std::lock_guard<std::mutex> lock(mu);
+            r->err = err;
             if (!rs->committed)
             {
                 // commit failed: drop the reservation
@@ -332,9 +337,15 @@ int nppe_server_wait(nppe_server *s, int64_t ticket, nppe_result *res, double *l
     if (res) *res = r->res;
     if (latency_ms) *latency_ms = r->finished - r->submitted;
     const int rc = r->rc;
+    s->lastError = r->err;
     s->requests.erase(it);
     delete r;
     return rc;
+}
+
+const char *nppe_server_last_error(const nppe_server *s)
+{
+    return s ? s->lastError.c_str() : "null server";
 }
 
 int nppe_server_memory(const nppe_server *s, uint64_t *now, uint64_t *peak)

@@ -140,6 +140,10 @@ int hip_fail(nppe_solver *h, hipError_t e, const char *what)
         if (e_ != hipSuccess) return hip_fail((h), e_, (what));  \
     } while (0)
 
+// Serialises device allocation and release against stream capture: a capture on one handle's stream must not see
+// another thread's allocator synchronise the device.
+std::mutex gCaptureMu;
+
 template <class T>
 hipError_t dalloc(nppe_solver *h, T **p, size_t count)
 {
@@ -149,11 +153,23 @@ hipError_t dalloc(nppe_solver *h, T **p, size_t count)
         h->budgetHit = true;
         return hipErrorOutOfMemory;
     }
-    hipError_t e = hipMalloc((void **)p, bytes);
+    hipError_t e;
+    {
+        std::lock_guard<std::mutex> lock(gCaptureMu);
+        e = hipMalloc((void **)p, bytes);
+    }
     if (e != hipSuccess) return e;
     h->allocations.push_back((void *)*p);
     h->bytesHeld += bytes;
     return hipMemsetAsync(*p, 0, bytes, h->stream);
+}
+
+// Host-to-device copy ordered on the handle's stream (never on the null stream, which would synchronise with
+// other handles' streams), complete on return.
+hipError_t upload(nppe_solver *h, void *dst, const void *src, size_t bytes)
+{
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream);
+    return e == hipSuccess ? hipStreamSynchronize(h->stream) : e;
 }
 
 // A box array (pitch and plane strides multiples of 16 elements) whose element 1 sits on a 16-element boundary:
@@ -176,7 +192,10 @@ void free_grid(nppe_solver *h)
         h->innerGraph = nullptr;
     }
     h->rate = 0.0;
-    for (void *p : h->allocations) (void)hipFree(p);
+    {
+        std::lock_guard<std::mutex> lock(gCaptureMu);
+        for (void *p : h->allocations) (void)hipFree(p);
+    }
     h->allocations.clear();
     h->bytesHeld = 0;
     h->levels.clear();
@@ -331,21 +350,18 @@ hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel, doubl
         {
             const double by = (3 * s + f) * N;
             const bool dot = l == 0 && sw == nu2 - 1 && rzPartials;
-            int rzBlocks = 0;
             {
                 Scope sc(h, l == 0 ? NPPE_K_SMOOTH0 : NPPE_K_SMOOTH, by);
                 if ((e = launch_rbgs(h->stream, L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l), 1, 0,
-                                     L.tileRows, L.planesPerChunk, h->dState, dot ? rzPartials : nullptr,
-                                     &rzBlocks)) != hipSuccess)
+                                     L.tileRows, L.planesPerChunk, h->dState, dot ? rzPartials : nullptr)) !=
+                    hipSuccess)
                     return e;
             }
             if (dot)
             {
                 *rzDone = true;
-                Scope sc(h, NPPE_K_CG_DOT, 0.0);
-                if (rzBlocks > 0 && (e = launch_rz_finalize(h->stream, h->dState, rzPartials, rzBlocks)) != hipSuccess)
-                    return e;
-                if (rzBlocks == 0) tr.add(2 * s * N + f * N);   // the separate <r, z> pass
+                if (rbgs_family(L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l)) != RBGS_ROW)
+                    tr.add(2 * s * N + f * N);   // the separate <r, z> pass
             }
             cur[l] ^= 1;
             tr.add(by);
@@ -619,14 +635,14 @@ int build(nppe_solver *h, const nppe_grid *g)
                 }
             HIP_TRY(h, dalloc_pairs(h, &L.c8, p8.size()), "allocate codes");
             HIP_TRY(h, dalloc(h, &L.c16, p16.size()), "allocate codes");
-            HIP_TRY(h, hipMemcpy(L.c8, p8.data(), p8.size(), hipMemcpyHostToDevice), "upload codes");
-            HIP_TRY(h, hipMemcpy(L.c16, p16.data(), p16.size() * sizeof(uint16_t), hipMemcpyHostToDevice),
+            HIP_TRY(h, upload(h, L.c8, p8.data(), p8.size()), "upload codes");
+            HIP_TRY(h, upload(h, L.c16, p16.data(), p16.size() * sizeof(uint16_t)),
                     "upload codes");
         }
         int off[9];
         std::vector<float> tab = tables32(H, sigma, off);
         HIP_TRY(h, dalloc(h, &L.tab, tab.size()), "allocate tables");
-        HIP_TRY(h, hipMemcpy(L.tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice), "upload tables");
+        HIP_TRY(h, upload(h, L.tab, tab.data(), tab.size() * sizeof(float)), "upload tables");
         L.t32 = Tables32{L.tab + off[0], L.tab + off[1], L.tab + off[2], L.tab + off[3], L.tab + off[4],
                          L.tab + off[5], L.tab + off[6], L.tab + off[7], L.tab + off[8]};
         L.v.B = L.box;
@@ -671,10 +687,9 @@ int build(nppe_solver *h, const nppe_grid *g)
             }
             HIP_TRY(h, dalloc(h, &h->dCoarseM, M.size()), "allocate coarse matrix");
             HIP_TRY(h, dalloc(h, &h->dCoarseOrder, pitched.size()), "allocate coarse order");
-            HIP_TRY(h, hipMemcpy(h->dCoarseM, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice),
+            HIP_TRY(h, upload(h, h->dCoarseM, M.data(), M.size() * sizeof(double)),
                     "upload coarse matrix");
-            HIP_TRY(h, hipMemcpy(h->dCoarseOrder, pitched.data(), pitched.size() * sizeof(int32_t),
-                                 hipMemcpyHostToDevice),
+            HIP_TRY(h, upload(h, h->dCoarseOrder, pitched.data(), pitched.size() * sizeof(int32_t)),
                     "upload coarse order");
             h->coarseDense = true;
         }
@@ -685,7 +700,7 @@ int build(nppe_solver *h, const nppe_grid *g)
         int off[6];
         std::vector<double> tab = tables64(F.host, off);
         HIP_TRY(h, dalloc(h, &h->dTab64, tab.size()), "allocate FP64 tables");
-        HIP_TRY(h, hipMemcpy(h->dTab64, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice),
+        HIP_TRY(h, upload(h, h->dTab64, tab.data(), tab.size() * sizeof(double)),
                 "upload FP64 tables");
         h->t64 = Tables64{h->dTab64 + off[0], h->dTab64 + off[1], h->dTab64 + off[2], h->dTab64 + off[3],
                           h->dTab64 + off[4], h->dTab64 + off[5], scale, unitCoef ? 1 : 0, F.c8};
@@ -700,7 +715,7 @@ int build(nppe_solver *h, const nppe_grid *g)
             pitched[q] = (int32_t)F.box.at(s, m, f);
         }
         HIP_TRY(h, dalloc(h, &h->dOrder, pitched.size()), "allocate order");
-        HIP_TRY(h, hipMemcpy(h->dOrder, pitched.data(), pitched.size() * sizeof(int32_t), hipMemcpyHostToDevice),
+        HIP_TRY(h, upload(h, h->dOrder, pitched.data(), pitched.size() * sizeof(int32_t)),
                 "upload order");
     }
     const size_t cells = (size_t)F.box.cells;
@@ -840,6 +855,7 @@ int solve_mgpcg(nppe_solver *h, const double *dB, double *dX, const double *dGue
             // every time, and every kernel reads the device-side done flag, so one graph serves every iteration.
             hipGraph_t g = nullptr;
             Traffic it;
+            std::lock_guard<std::mutex> lock(gCaptureMu);
             HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "graph capture");
             const hipError_t ce = inner_iteration(h, it);
             const hipError_t ee = hipStreamEndCapture(h->stream, &g);
@@ -1152,7 +1168,10 @@ int nppe_create(nppe_solver **out, const nppe_options *opt)
     }
     h->opt = o;
     h->device = o.device;
-    if ((e = hipSetDevice(h->device)) != hipSuccess || (e = hipStreamCreate(&h->stream)) != hipSuccess ||
+    // non-blocking: no implicit synchronisation with the null stream (default priority)
+    std::unique_lock<std::mutex> devLock(gCaptureMu);
+    if ((e = hipSetDevice(h->device)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipHostMalloc((void **)&h->hState, sizeof(SolveState), hipHostMallocDefault)) != hipSuccess)
     {
         g_create_error = std::string("device set-up failed: ") + hipGetErrorString(e);
@@ -1172,8 +1191,11 @@ int nppe_destroy(nppe_solver *h)
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     free_grid(h);
     for (hipEvent_t e : h->evPool) (void)hipEventDestroy(e);
-    if (h->hState) (void)hipHostFree(h->hState);
-    if (h->stream) (void)hipStreamDestroy(h->stream);
+    {
+        std::lock_guard<std::mutex> lock(gCaptureMu);
+        if (h->hState) (void)hipHostFree(h->hState);
+        if (h->stream) (void)hipStreamDestroy(h->stream);
+    }
     delete h;
     return NPPE_OK;
 }

@@ -33,7 +33,8 @@ EXPORTS = [
     "nppe_system_info", "nppe_unknown_order", "nppe_check_csr", "nppe_solve", "nppe_solve_device", "nppe_apply",
     "nppe_memory_bytes", "nppe_profile_enable", "nppe_profile_select", "nppe_profile_read", "nppe_synth_cosine_rhs",
     "nppe_set_tolerance", "nppe_estimate_bytes", "nppe_batch_solve", "nppe_server_create", "nppe_server_destroy",
-    "nppe_server_add_grid", "nppe_server_submit", "nppe_server_wait", "nppe_server_memory",
+    "nppe_server_add_grid", "nppe_server_submit", "nppe_server_wait", "nppe_server_last_error",
+    "nppe_server_memory",
 ]
 
 
@@ -132,6 +133,8 @@ def lib():
     L.nppe_server_submit.restype = I64
     L.nppe_server_wait.argtypes = [P, I64, P, P]
     L.nppe_server_wait.restype = C.c_int
+    L.nppe_server_last_error.argtypes = [P]
+    L.nppe_server_last_error.restype = C.c_char_p
     L.nppe_server_memory.argtypes = [P, P, P]
     L.nppe_server_memory.restype = C.c_int
     _lib = L
@@ -386,7 +389,7 @@ class Server:
         rc = lib().nppe_server_wait(self.h, ticket, C.byref(res), C.byref(lat))
         x, n = self._pending.pop(ticket)
         if rc < 0:
-            raise NppeError(rc, "server_wait")
+            raise NppeError(rc, "server_wait: " + lib().nppe_server_last_error(self.h).decode())
         return x[:n], res, float(lat.value), rc
 
     def memory(self):
