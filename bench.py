@@ -395,7 +395,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"3D pure-Neumann 7-point Poisson {args.size}^3 cells, h=1/{args.size}, all faces "
                                f"SOLID; RHS 8 cosine modes + 0.01 N(0,1), mt19937_64 seed {SEED}, zero mean; "
-                               f"MGPCG FP32 V(2,2) RBGS + FP64 refinement to 1e-8 true relative residual",
+                               f"MGPCG: FP32 V(2,2) red-black Gauss-Seidel multigrid preconditioner in an FP64 CG "
+                               f"recursion, to 1e-8 true FP64 relative residual",
                    "cells": args.size ** 3, "tolerance": 1e-8},
         "roofline": res.pop("roofline"),
         "cpu_baseline": cpu,

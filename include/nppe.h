@@ -56,6 +56,18 @@ enum nppe_method
     NPPE_METHOD_CG64 = 1   /* FP64 matrix-free Jacobi-PCG (Eigen's ConjugateGradient<..., DiagonalPreconditioner>) */
 };
 
+/* nppe_options::pre_smooth / post_smooth: chosen from the grid at commit. */
+#define NPPE_SMOOTH_AUTO (-1)
+
+/* Where MGPCG runs its Krylov recursion (the V-cycle preconditioner is FP32 either way). */
+enum nppe_krylov
+{
+    NPPE_KRYLOV_REFINE = 0, /* FP32 CG passes inside FP64 iterative refinement (true FP64 residual between passes) */
+    NPPE_KRYLOV_FP64 = 1    /* one CG with x, r, alpha, beta and A p in FP64 (directions stored in FP32) around the
+                               FP32 V-cycle; the true FP64 residual is checked at its end (3D grids; 2D grids use
+                               NPPE_KRYLOV_REFINE) */
+};
+
 /* Cell type codes (Definitions.h:191-196). */
 enum nppe_cell
 {
@@ -74,16 +86,18 @@ typedef struct nppe_options
     double tolerance;        /* relative tolerance; eigen_tolerance's meaning (PressureSettings.h:225) */
     int32_t max_iters;       /* cap on iterations (inner iterations summed over refinement passes for MGPCG) */
     int32_t compatibility;   /* NEEDS_COMPATIBILITY_CONDITION: a singular system's solution is returned mean-free */
-    int32_t pre_smooth;      /* red-black Gauss-Seidel sweeps before / after the coarse correction (default 2 / 2) */
+    int32_t pre_smooth;      /* red-black Gauss-Seidel sweeps before / after the coarse correction, or
+                                NPPE_SMOOTH_AUTO (the default): 2 / 2 on uniform grids, 4 / 4 on stretched ones */
     int32_t post_smooth;
     double inner_reduction;  /* residual reduction each FP32 inner solve aims for at most (default 3e-5) */
-    int32_t max_refinements; /* FP64 refinement passes (default 8) */
-    int32_t reserved0;
+    int32_t max_refinements; /* FP64 refinement passes (default 8); with NPPE_KRYLOV_FP64, restarts after a true
+                                residual that misses the target */
+    int32_t krylov;          /* enum nppe_krylov; default NPPE_KRYLOV_FP64 */
     uint64_t hbm_budget;     /* bytes of device memory the handle may hold; 0 = no limit */
 } nppe_options;
 
-/* Fills *opt with the defaults: device 0, MGPCG, tolerance 1e-8, 2000 iterations, compatibility on, V(2,2),
- * inner reduction 3e-5, 8 refinement passes, no budget. */
+/* Fills *opt with the defaults: device 0, MGPCG with the FP64 Krylov recursion, tolerance 1e-8, 2000 iterations,
+ * compatibility on, automatic sweep counts, inner reduction 3e-5, 8 refinement passes, no budget. */
 void nppe_default_options(nppe_options *opt);
 
 /* The grid the reference hands CommitPressureMatrix through its MAC* (PressureSettings.h:542): box dimensions
@@ -168,6 +182,8 @@ int nppe_set_tolerance(nppe_solver *h, double tolerance);
 /* Upper estimate of the device bytes nppe_commit_grid will allocate for g, for admission against an HBM budget
  * before any allocation. 0 for an invalid grid. */
 uint64_t nppe_estimate_bytes(const nppe_grid *g, int32_t method);
+/* The same for a handle created with *opt (null: the defaults). */
+uint64_t nppe_estimate_bytes_for(const nppe_grid *g, const nppe_options *opt);
 
 /* ---- many systems at once (no counterpart in the reference, which solves one system per process) ------------- */
 

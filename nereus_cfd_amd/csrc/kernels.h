@@ -44,9 +44,11 @@ hipError_t launch_cg64_direction(hipStream_t s, const Box &B, const Tables64 &T,
 // One full red-black Gauss-Seidel sweep x_out = S(x_in) (c1: colour updated first; zero: x_in taken as 0), in one
 // pass: tiles of 64 x tileRows (8 or 16) cells marching along the slow axis in chunks of planesPerChunk planes.
 // With rzPartials non-null the sweep is the CG's last level-0 sweep and also reduces <b, x_out> into st (as
-// launch_cg_rz(b, x_out) does; inside the sweep kernel where it can).
+// launch_cg_rz(b, x_out) does). The whole-row sweep (rbgs_family RBGS_ROW) writes one partial per interior row and
+// plane, which launch_rz_rows_finalize then sums; the other sweeps complete the reduction themselves.
 hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float *xout, const float *b, int c1,
                        int zero, int tileRows, int planesPerChunk, SolveState *st, double *rzPartials = nullptr);
+hipError_t launch_rz_rows_finalize(hipStream_t s, const Box &B, SolveState *st, double *partials);
 // Which sweep kernel launch_rbgs runs for these operands, and the tile heights it accepts for it (every tile shape
 // gives bit-identical results: a colour's updates are independent of each other).
 enum { RBGS_ROW = 0, RBGS_MARCH = 1, RBGS_2D = 2 };
@@ -77,6 +79,14 @@ bool cg_fused_direction(const Level32 &L);
 // other (pA written on even inner iterations).
 hipError_t launch_cg_dirspmv(hipStream_t s, const Level32 &L, const float *z, float *pA, float *pB, float *q,
                              SolveState *st, double *partials);
+// FP64 Krylov around the FP32 preconditioner (3D boxes). Part 1: p = z + beta p formed in FP64, stored in FP32
+// (pA written on even inner iterations, pB on odd), <p, A p> -> alpha, A p in the reference's FP64 arithmetic.
+hipError_t launch_cg_dir64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const float *z,
+                           float *pA, float *pB, SolveState *st, double *partials);
+// Part 2: q = A p from the same FP32 p, x += alpha p, r -= alpha q (x, r FP64), r32 = (float) r, <r, r> ->
+// convergence flag.
+hipError_t launch_cg_update64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const float *pA,
+                              const float *pB, double *x, double *r, float *r32, SolveState *st, double *partials);
 // e += alpha p (e = alpha p on the first iteration), r -= alpha q, <r, r> -> convergence flag; p is pA on even
 // inner iterations and pB on odd ones (pA == pB for an in-place direction).
 hipError_t launch_cg_update(hipStream_t s, const Level32 &L, float *e, float *r, const float *pA, const float *pB,

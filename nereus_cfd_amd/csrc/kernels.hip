@@ -46,7 +46,7 @@ struct Red
     int fused;
 };
 
-__device__ void fin_apply(SolveState *st, double *partials, int nblk, int nv, int op);
+__device__ void fin_apply(SolveState *st, double *partials, int nblk, int nv, int op, bool coherent);
 
 // Block sum of NV per-thread values, written by thread 0 to partials[a * nblk + blk]. Fused, the last block of
 // the grid to finish (a ticket counter in the state) then sums the partials of all blocks in index order and
@@ -96,7 +96,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], const Red &red, int n
     }
     __syncthreads();
     if (!lastBlock) return;
-    fin_apply(red.st, red.partials, nblk, NV, red.op);
+    fin_apply(red.st, red.partials, nblk, NV, red.op, true);
 }
 
 // Logical block id after the XCD remap: consecutive hardware workgroup ids go round-robin to the eight XCDs, so
@@ -398,17 +398,31 @@ enum FinOp
 
 __device__ __forceinline__ bool fin_in_loop(int op) { return op >= FIN_RZ; }
 
-// Run by every thread of the last block of a reducing kernel: sums the partials of each value over the nblk
-// blocks in index order, applies op, and resets the ticket for the next reduction.
-__device__ void fin_apply(SolveState *st, double *partials, int nblk, int nv, int op)
+// Run by every thread of one block (the last block of a fused reducing kernel, or k_finalize): sums the nblk
+// partials of each value in a fixed order (thread t takes entries t, t + 4 bd, ... into four interleaved sums),
+// applies op, and resets the ticket for the next reduction. coherent: read with agent-scope atomic loads (the
+// partials were written by the same kernel's other blocks).
+__device__ void fin_apply(SolveState *st, double *partials, int nblk, int nv, int op, bool coherent)
 {
     __shared__ double sh[2][16];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = blockDim.x >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = blockDim.x >> 6, bd = blockDim.x;
+    auto ld = [&](const double *p) {
+        return coherent ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *p;
+    };
     for (int a = 0; a < nv; a++)
     {
-        double x = 0.0;
-        for (int q = tid; q < nblk; q += blockDim.x)
-            x += __hip_atomic_load(&partials[(size_t)a * nblk + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const double *pa = partials + (size_t)a * nblk;
+        double x0 = 0.0, x1 = 0.0, x2 = 0.0, x3 = 0.0;
+        int q = tid;
+        for (; q + 3 * bd < nblk; q += 4 * bd)
+        {
+            x0 += ld(pa + q);
+            x1 += ld(pa + q + bd);
+            x2 += ld(pa + q + 2 * bd);
+            x3 += ld(pa + q + 3 * bd);
+        }
+        for (; q < nblk; q += bd) x0 += ld(pa + q);
+        double x = (x0 + x1) + (x2 + x3);
         x = wave_sum(x);
         if (lane == 0) sh[a][w] = x;
     }
@@ -459,10 +473,13 @@ __device__ void fin_apply(SolveState *st, double *partials, int nblk, int nv, in
     }
 }
 
-__global__ __launch_bounds__(kBlockThreads) void k_finalize(SolveState *st, double *partials, int nblk, int nv, int op)
+__global__ __launch_bounds__(1024) void k_finalize(SolveState *st, double *partials, int nblk, int nv, int op)
 {
-    fin_apply(st, partials, nblk, nv, op);
+    fin_apply(st, partials, nblk, nv, op, false);
 }
+
+// One block for a separate finalisation: 256 threads, or 1024 for many partials.
+inline int fin_threads(int nblk) { return nblk > 8192 ? 1024 : kBlockThreads; }
 
 // ---- FP64 kernels --------------------------------------------------------------------------------------------
 
@@ -1146,13 +1163,14 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_march(Level32 L, const f
 // use: the old x of the mid-axis neighbour rows comes from the neighbouring waves through LDS (each wave publishes
 // its own row), and the halo waves, whose outer neighbour row belongs to no wave of the block, prefetch that row.
 // So no load is waited for in the plane that issues it.
-// With dot.partials non-null the sweep also reduces <b, x_out> over the unknowns into dot (the CG's <r, z> when the
-// sweep is the last one of the V-cycle on the finest level).
+// With dot non-null the sweep also writes <b, x_out> over the unknowns of each interior row and plane to
+// dot[(plane - 1) * (nm - 2) + row - 1] (the CG's <r, z> when the sweep is the last one of the V-cycle on the
+// finest level): per row, not per block, so the sum over them does not depend on the tile shape.
 template <int TM, int K, bool ZERO>
 __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const float *__restrict__ xin,
                                                             float *__restrict__ xout, const float *__restrict__ b,
                                                             int c1, int P, int tilesM, const SolveState *st,
-                                                            Red dot)
+                                                            double *__restrict__ dot)
 {
     if (st->done) return;
     const Box &B = L.B;
@@ -1242,7 +1260,6 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
     __syncthreads();
     const float icu = L.icu;
     const int wUp = min(wave + 1, TM + 1), wDn = max(wave - 1, 0);
-    double bx = 0.0;   // <b, x_out> of this lane's stored cells
 
     // One plane step at sp = sp0 + 4g + U: first colour of plane sp, second colour (and store) of plane sp - 1.
     auto step = [&](int sp, auto phase) {
@@ -1345,6 +1362,7 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
         {
             float *po = xout + (size_t)q * (size_t)B.sS;
             const float(*rq)[kCols] = red[q & 1];
+            double bx = 0.0;   // <b, x_out> of this lane's cells of the row
 #pragma unroll
             for (int k = 0; k < K; k++)
             {
@@ -1378,7 +1396,7 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
                     }
                 }
                 const float va = eq ? R[Sm1][k] : out, vb = eq ? out : R[Sm1][k];
-                if (dot.partials)
+                if (dot)
                 {
                     const unsigned ca = okA[k] ? (C[Sm1][k] & 255u) : 0u, cb = okB[k] ? ((C[Sm1][k] >> 8) & 255u) : 0u;
                     bx += (ca & 64u) ? (double)Bv[Sm1][k].x * (double)va : 0.0;
@@ -1388,6 +1406,11 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
                     *reinterpret_cast<float2 *>(po + off[k]) = make_float2(va, vb);
                 else if (okA[k])
                     po[off[k]] = va;
+            }
+            if (dot)
+            {
+                const double rowDot = wave_sum(bx);
+                if (lane == 0 && rowIn) dot[(size_t)(q - 1) * (size_t)(B.nm - 2) + (size_t)(m - 1)] = rowDot;
             }
         }
     };
@@ -1400,11 +1423,6 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
         step(sp + 2, std::integral_constant<int, 2>());
         if (sp + 3 > s1 + 1) break;
         step(sp + 3, std::integral_constant<int, 3>());
-    }
-    if (dot.partials)
-    {
-        double a[1] = {bx};
-        block_sum<1>(a, dot, gridDim.x, lb);
     }
 }
 
@@ -1798,6 +1816,140 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_dirspmv_t(Level32 L, const
     block_sum<1>(a, red, gridDim.x, M.lb);
 }
 
+// The halo assignment of the 64 x 4 march tiles: threads 0-63 the row above the tile, 64-127 the row below,
+// 128-131 the column left of it, 132-135 the column right of it (LDS position hy, hx; hx = -1: none).
+__device__ __forceinline__ void tile_halo(int t, int &hy, int &hx)
+{
+    hx = -1;
+    hy = -1;
+    if (t < 64) { hy = 0; hx = t + 1; }
+    else if (t < 128) { hy = kBlockMid + 1; hx = t - 64 + 1; }
+    else if (t < 128 + kBlockMid) { hy = t - 128 + 1; hx = 0; }
+    else if (t < 128 + 2 * kBlockMid) { hy = t - 128 - kBlockMid + 1; hx = kBlockFast + 1; }
+}
+
+// FP64 Krylov around the FP32 preconditioner, part 1 (3D boxes): p = z + beta p_old formed in FP64 and stored in
+// FP32, and <p, A p> with A p computed in the reference's FP64 arithmetic from the stored FP32 p (part 2 applies
+// A to the same FP32 values, so x and r stay consistent). p ping-pongs between pA (even inner iterations) and pB
+// (odd); plane p staged in LDS, slow neighbours in registers, loads a plane ahead.
+__global__ __launch_bounds__(kBlockThreads) void k_cg_dir64_t(Box B, Tables64 T, const uint8_t *__restrict__ ty,
+                                                              const float *__restrict__ z, float *__restrict__ pA,
+                                                              float *__restrict__ pB, const SolveState *st, Red red,
+                                                              int P)
+{
+#pragma clang fp contract(off)
+    if (st->done) return;
+    const int first = st->first;
+    const double beta = st->beta;
+    const bool odd = st->iters & 1;
+    float *__restrict__ pn = odd ? pB : pA;
+    const float *__restrict__ po = odd ? pA : pB;
+    __shared__ double Ps[kBlockMid + 2][kBlockFast + 2];
+    const March M = march_setup(B, P);
+    const int tx = threadIdx.x & (kBlockFast - 1), tyy = threadIdx.x / kBlockFast;
+    const int f0 = M.f - tx, m0 = M.m - tyy;
+    const int fc = min(M.f, B.nf - 1), mcl = min(M.m, B.nm - 1);
+    const int64_t own = (int64_t)mcl * B.sM + fc;
+    int hy, hx;
+    tile_halo(threadIdx.x, hy, hx);
+    const bool halo = hx >= 0;
+    const int64_t hoff = halo ? (int64_t)min(max(m0 - 1 + hy, 0), B.nm - 1) * B.sM + min(max(f0 - 1 + hx, 0), B.nf - 1)
+                              : own;
+    auto PV = [&](int64_t i) { return (double)(float)(first ? (double)z[i] : (double)z[i] + beta * (double)po[i]); };
+    auto plane = [&](int sl) { return (int64_t)min(max(sl, 0), B.ns - 1) * B.sS; };
+    double pq = 0.0;
+    double Pm = PV(plane(M.s0 - 1) + own), Pc = PV(plane(M.s0) + own), Pp = PV(plane(M.s0 + 1) + own);
+    double H = halo ? PV(plane(M.s0) + hoff) : 0.0;
+    unsigned cc = T.c8[plane(M.s0) + own];
+    for (int sl = M.s0; sl <= M.s1; sl++)
+    {
+        const double Pn = PV(plane(sl + 2) + own);
+        const double Hn = halo ? PV(plane(sl + 1) + hoff) : 0.0;
+        const unsigned cn = T.c8[plane(sl + 1) + own];
+        Ps[tyy + 1][tx + 1] = Pc;
+        if (halo) Ps[hy][hx] = H;
+        __syncthreads();
+        const int64_t idx = plane(sl) + own;
+        if (M.in && (cc & 64u))
+        {
+            const double nb[6] = {Pp, Pm, Ps[tyy + 2][tx + 1], Ps[tyy][tx + 1], Ps[tyy + 1][tx + 2], Ps[tyy + 1][tx]};
+            const double v = apply64v(B, T, ty, sl, M.m, M.f, idx, Pc, nb, cc);
+            pn[idx] = (float)Pc;
+            pq += Pc * v;
+        }
+        __syncthreads();
+        Pm = Pc;
+        Pc = Pp;
+        Pp = Pn;
+        H = Hn;
+        cc = cn;
+    }
+    double a[1] = {pq};
+    block_sum<1>(a, red, gridDim.x, M.lb);
+}
+
+// Part 2: q = A p (FP64, from the FP32 p of part 1), x += alpha p, r -= alpha q, r32 = (float) r, and <r, r>.
+__global__ __launch_bounds__(kBlockThreads) void k_cg_update64_t(Box B, Tables64 T, const uint8_t *__restrict__ ty,
+                                                                 const float *__restrict__ pA,
+                                                                 const float *__restrict__ pB,
+                                                                 double *__restrict__ x, double *__restrict__ r,
+                                                                 float *__restrict__ r32, const SolveState *st,
+                                                                 Red red, int P)
+{
+#pragma clang fp contract(off)
+    if (st->done) return;
+    const double alpha = st->alpha;
+    const float *__restrict__ p = (st->iters & 1) ? pB : pA;
+    __shared__ double Ps[kBlockMid + 2][kBlockFast + 2];
+    const March M = march_setup(B, P);
+    const int tx = threadIdx.x & (kBlockFast - 1), tyy = threadIdx.x / kBlockFast;
+    const int f0 = M.f - tx, m0 = M.m - tyy;
+    const int fc = min(M.f, B.nf - 1), mcl = min(M.m, B.nm - 1);
+    const int64_t own = (int64_t)mcl * B.sM + fc;
+    int hy, hx;
+    tile_halo(threadIdx.x, hy, hx);
+    const bool halo = hx >= 0;
+    const int64_t hoff = halo ? (int64_t)min(max(m0 - 1 + hy, 0), B.nm - 1) * B.sM + min(max(f0 - 1 + hx, 0), B.nf - 1)
+                              : own;
+    auto plane = [&](int sl) { return (int64_t)min(max(sl, 0), B.ns - 1) * B.sS; };
+    double rr = 0.0;
+    double Pm = p[plane(M.s0 - 1) + own], Pc = p[plane(M.s0) + own], Pp = p[plane(M.s0 + 1) + own];
+    double H = halo ? (double)p[plane(M.s0) + hoff] : 0.0;
+    unsigned cc = T.c8[plane(M.s0) + own];
+    double xc = x[plane(M.s0) + own], rc = r[plane(M.s0) + own];
+    for (int sl = M.s0; sl <= M.s1; sl++)
+    {
+        const double Pn = p[plane(sl + 2) + own];
+        const double Hn = halo ? (double)p[plane(sl + 1) + hoff] : 0.0;
+        const unsigned cn = T.c8[plane(sl + 1) + own];
+        const double xn = x[plane(sl + 1) + own], rn = r[plane(sl + 1) + own];
+        Ps[tyy + 1][tx + 1] = Pc;
+        if (halo) Ps[hy][hx] = H;
+        __syncthreads();
+        const int64_t idx = plane(sl) + own;
+        if (M.in && (cc & 64u))
+        {
+            const double nb[6] = {Pp, Pm, Ps[tyy + 2][tx + 1], Ps[tyy][tx + 1], Ps[tyy + 1][tx + 2], Ps[tyy + 1][tx]};
+            const double q = apply64v(B, T, ty, sl, M.m, M.f, idx, Pc, nb, cc);
+            const double xv = xc + alpha * Pc, rv = rc - alpha * q;
+            x[idx] = xv;
+            r[idx] = rv;
+            r32[idx] = (float)rv;
+            rr += rv * rv;
+        }
+        __syncthreads();
+        Pm = Pc;
+        Pc = Pp;
+        Pp = Pn;
+        H = Hn;
+        cc = cn;
+        xc = xn;
+        rc = rn;
+    }
+    double a[1] = {rr};
+    block_sum<1>(a, red, gridDim.x, M.lb);
+}
+
 // Residual of every fine cell, summed over each 2 x 2 x 2 family into the coarse right-hand side: fast pairs are
 // neighbouring lanes (shuffle), mid pairs neighbouring rows of the block (LDS), slow pairs consecutive planes of
 // the march (P even, chunks start on an odd plane). A family cut by the end of an odd interior keeps its 1 or 2
@@ -2065,7 +2217,7 @@ hipError_t finish(hipStream_t s, const Red &red, int nblk, int nv)
 {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || red.fused) return e;
-    k_finalize<<<1, kBlockThreads, 0, s>>>(red.st, red.partials, nblk, nv, red.op);
+    k_finalize<<<1, fin_threads(nblk), 0, s>>>(red.st, red.partials, nblk, nv, red.op);
     return hipGetLastError();
 }
 
@@ -2240,10 +2392,9 @@ hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float 
         const int chunks = std::max(1, (B.ns - 2 + P - 1) / P);
         const int grid = tilesM * chunks;
         const int threads = 64 * (TM + 2);
-        const Red dot = red_of(rzPartials, st, FIN_RZ, grid);
 #define NPPE_ROW(TMv, Kv)                                                                                            \
-    (zero ? (k_rbgs_row<TMv, Kv, true><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st, dot), 0)           \
-          : (k_rbgs_row<TMv, Kv, false><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st, dot), 0))
+    (zero ? (k_rbgs_row<TMv, Kv, true><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st, rzPartials), 0)    \
+          : (k_rbgs_row<TMv, Kv, false><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st, rzPartials), 0))
         if (TM == 14)
         {
             if (K == 1) NPPE_ROW(14, 1);
@@ -2253,7 +2404,7 @@ hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float 
         else if (K == 2) NPPE_ROW(6, 2);
         else NPPE_ROW(6, 4);
 #undef NPPE_ROW
-        return rzPartials ? finish(s, dot, grid, 1) : hipGetLastError();
+        return hipGetLastError();
     }
     if (B.threeD)
     {
@@ -2319,6 +2470,13 @@ hipError_t launch_coarse_solve(hipStream_t s, int n, const int32_t *order, const
     return hipGetLastError();
 }
 
+hipError_t launch_rz_rows_finalize(hipStream_t s, const Box &B, SolveState *st, double *partials)
+{
+    const int nrows = (B.ns - 2) * (B.nm - 2);
+    k_finalize<<<1, fin_threads(nrows), 0, s>>>(st, partials, nrows, 1, (int)FIN_RZ);
+    return hipGetLastError();
+}
+
 hipError_t launch_cg_rz(hipStream_t s, const Level32 &L, const float *r, const float *z, SolveState *st,
                         double *partials)
 {
@@ -2358,6 +2516,24 @@ hipError_t launch_cg_dirspmv(hipStream_t s, const Level32 &L, const float *z, fl
     const int P = march_planes(L.B), G = march_grid(L.B, P);
     const Red red = red_of(partials, st, FIN_PQ, G);
     k_cg_dirspmv_t<<<G, kBlockThreads, 0, s>>>(L, z, pA, pB, q, st, red, P);
+    return finish(s, red, G, 1);
+}
+
+hipError_t launch_cg_dir64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const float *z,
+                           float *pA, float *pB, SolveState *st, double *partials)
+{
+    const int P = march_planes(B), G = march_grid(B, P);
+    const Red red = red_of(partials, st, FIN_PQ, G);
+    k_cg_dir64_t<<<G, kBlockThreads, 0, s>>>(B, T, ty, z, pA, pB, st, red, P);
+    return finish(s, red, G, 1);
+}
+
+hipError_t launch_cg_update64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const float *pA,
+                              const float *pB, double *x, double *r, float *r32, SolveState *st, double *partials)
+{
+    const int P = march_planes(B), G = march_grid(B, P);
+    const Red red = red_of(partials, st, FIN_RR, G);
+    k_cg_update64_t<<<G, kBlockThreads, 0, s>>>(B, T, ty, pA, pB, x, r, r32, st, red, P);
     return finish(s, red, G, 1);
 }
 

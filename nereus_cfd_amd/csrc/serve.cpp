@@ -279,7 +279,7 @@ int nppe_server_add_grid(nppe_server *s, uint64_t key, const nppe_grid *g)
     sg.wx.assign(g->width_x, g->width_x + g->nx);
     sg.wy.assign(g->width_y, g->width_y + g->ny);
     if (g->dim == 3) sg.wz.assign(g->width_z, g->width_z + nz);
-    sg.estimate = nppe_estimate_bytes(g, s->opt.method);
+    sg.estimate = nppe_estimate_bytes_for(g, &s->opt);
     if (s->budget && sg.estimate > s->budget) return NPPE_ERR_BUDGET;
     std::lock_guard<std::mutex> lock(s->mu);
     if (s->resident.count(key)) return NPPE_ERR_ARGUMENT;   // a resident grid cannot be replaced

@@ -80,7 +80,9 @@ struct nppe_solver
     double *dX64[2] = {nullptr, nullptr};
     double *dB64 = nullptr;
     float *dE = nullptr, *dR = nullptr, *dP = nullptr, *dP2 = nullptr, *dQ = nullptr, *dZ = nullptr, *dZ2 = nullptr;
-    double *dR64 = nullptr, *dP64 = nullptr, *dQ64 = nullptr;
+    double *dR64 = nullptr, *dP64 = nullptr, *dQ64 = nullptr, *dP64b = nullptr;
+    bool krylov64 = false;   // MGPCG with the FP64 Krylov recursion (nppe_options::krylov)
+    int nuPre = 2, nuPost = 2;   // sweeps before / after the coarse correction (nppe_options, auto resolved)
     double *dCompB = nullptr, *dCompX = nullptr, *dCompG = nullptr;
     double *dCoarseM = nullptr;
     int32_t *dCoarseOrder = nullptr;
@@ -204,7 +206,8 @@ void free_grid(nppe_solver *h)
     h->dX64[0] = h->dX64[1] = nullptr;
     h->dB64 = nullptr;
     h->dE = h->dR = h->dP = h->dP2 = h->dQ = h->dZ = h->dZ2 = nullptr;
-    h->dR64 = h->dP64 = h->dQ64 = nullptr;
+    h->dR64 = h->dP64 = h->dQ64 = h->dP64b = nullptr;
+    h->krylov64 = false;
     h->dCompB = h->dCompX = h->dCompG = nullptr;
     h->dCoarseM = nullptr;
     h->dCoarseOrder = nullptr;
@@ -279,7 +282,7 @@ hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel, doubl
     {
         DevLevel &L = h->levels[l];
         const double N = L.cellsBox;
-        const int nu1 = (coarseNu > 0 && l >= 2) ? coarseNu : std::max(1, h->opt.pre_smooth);
+        const int nu1 = (coarseNu > 0 && l >= 2) ? coarseNu : h->nuPre;
         for (int sw = 0; sw < nu1; sw++)
         {
             const bool zero = sw == 0;
@@ -345,7 +348,7 @@ hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel, doubl
                 return e;
         }
         tr.add(pb);
-        const int nu2 = (coarseNu > 0 && l >= 2) ? coarseNu : std::max(0, h->opt.post_smooth);
+        const int nu2 = (coarseNu > 0 && l >= 2) ? coarseNu : h->nuPost;
         for (int sw = 0; sw < nu2; sw++)
         {
             const double by = (3 * s + f) * N;
@@ -360,7 +363,13 @@ hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel, doubl
             if (dot)
             {
                 *rzDone = true;
-                if (rbgs_family(L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l)) != RBGS_ROW)
+                if (rbgs_family(L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l)) == RBGS_ROW)
+                {
+                    Scope sc(h, NPPE_K_CG_DOT, 0.0);
+                    if ((e = launch_rz_rows_finalize(h->stream, L.box, h->dState, rzPartials)) != hipSuccess)
+                        return e;
+                }
+                else
                     tr.add(2 * s * N + f * N);   // the separate <r, z> pass
             }
             cur[l] ^= 1;
@@ -385,6 +394,25 @@ hipError_t inner_iteration(nppe_solver *h, Traffic &tr)
     {
         Scope sc(h, NPPE_K_CG_DOT, 2 * s * N + f * N);
         if ((e = launch_cg_rz(h->stream, L0.v, h->dR, z, h->dState, h->dPartials)) != hipSuccess) return e;
+    }
+    if (h->krylov64)
+    {
+        // FP64 Krylov: p = z + beta p (stored FP32) with <p, A p>; then A p again from the same p, x and r in FP64,
+        // r32 refreshed for the V-cycle
+        {
+            Scope sc(h, NPPE_K_CG_SPMV, 13.0 * N);   // z, p_old, codes in; p out
+            if ((e = launch_cg_dir64(h->stream, L0.box, h->t64, L0.types, z, h->dP, h->dP2, h->dState,
+                                     h->dPartials)) != hipSuccess)
+                return e;
+        }
+        {
+            Scope sc(h, NPPE_K_CG_UPDATE, 41.0 * N);   // p, codes, x, r in; x, r, r32 out
+            if ((e = launch_cg_update64(h->stream, L0.box, h->t64, L0.types, h->dP, h->dP2, h->dX64[0], h->dR64,
+                                        h->dR, h->dState, h->dPartials)) != hipSuccess)
+                return e;
+        }
+        tr.add((rzDone ? 0.0 : (2 * s + f) * N) + 13.0 * N + 41.0 * N);
+        return hipSuccess;
     }
     const bool fused = cg_fused_direction(L0.v);
     float *pB = fused ? h->dP2 : h->dP;
@@ -729,6 +757,17 @@ int build(nppe_solver *h, const nppe_grid *g)
         HIP_TRY(h, dalloc_pairs(h, &h->dP, cells), "allocate p");
         HIP_TRY(h, dalloc_pairs(h, &h->dQ, cells), "allocate q");
         HIP_TRY(h, dalloc_pairs(h, &h->dP2, cells), "allocate p");
+        // FP64 Krylov (3D): x stays in dX64[0], r in FP64; the directions (FP32) in dP / dP2
+        h->krylov64 = h->opt.krylov == NPPE_KRYLOV_FP64 && F.box.threeD;
+        // automatic sweep counts: V(2,2) on uniform grids; V(4,4) on stretched ones, whose anisotropic cells
+        // point smoothing reduces slowly
+        const int autoNu = F.host.uniform ? 2 : 4;
+        h->nuPre = h->opt.pre_smooth == NPPE_SMOOTH_AUTO ? autoNu : h->opt.pre_smooth;
+        h->nuPost = h->opt.post_smooth == NPPE_SMOOTH_AUTO ? autoNu : h->opt.post_smooth;
+        if (h->krylov64)
+        {
+            HIP_TRY(h, dalloc(h, &h->dR64, cells), "allocate r64");
+        }
         HIP_TRY(h, dalloc_pairs(h, &h->dZ, cells), "allocate z");
         HIP_TRY(h, dalloc_pairs(h, &h->dZ2, cells), "allocate z");
     }
@@ -748,6 +787,8 @@ int build(nppe_solver *h, const nppe_grid *g)
         int64_t most = kMaxGrid;
         for (const DevLevel &L : h->levels)
             most = std::max<int64_t>(most, (int64_t)L.box.tilesF * L.box.tilesM * ((L.box.ns - 2 + 1) / 2 + 1));
+        // the fused <r, z> of the level-0 sweep keeps one partial per interior row and plane
+        most = std::max<int64_t>(most, (int64_t)(h->levels[0].box.ns - 2) * (h->levels[0].box.nm - 2));
         h->partialsCap = most * 2;
     }
     HIP_TRY(h, dalloc(h, &h->dPartials, (size_t)h->partialsCap), "allocate partials");
@@ -783,6 +824,70 @@ int validate_grid(nppe_solver *h, const nppe_grid *g)
 
 // ---- solves ---------------------------------------------------------------------------------------------------
 
+// Enqueues inner iterations (replayed from the captured one-iteration graph unless profiling) until the device
+// state says done, checking on the host between chunks whose length the observed per-iteration reduction
+// predicts for the reduction still needed (needRed at the start; rr0 = the starting <r, r>). The device stops at
+// convergence, so an overshoot costs only launches that exit at once; an undershoot costs one more check.
+// *ran = the iterations that did work.
+int run_inner(nppe_solver *h, double needRed, double rr0, Traffic &tr, int *ran)
+{
+    int launched = 0;
+    double perIter = 0.0;
+    const bool graphs = h->useGraphs && !h->profiling;
+    if (graphs && !h->innerGraph)
+    {
+        // Capture one iteration once: its launches, their arguments and the z ping-pong parity are the same
+        // every time, and every kernel reads the device-side done flag, so one graph serves every iteration.
+        hipGraph_t g = nullptr;
+        Traffic it;
+        std::lock_guard<std::mutex> lock(gCaptureMu);
+        HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+        const hipError_t ce = inner_iteration(h, it);
+        const hipError_t ee = hipStreamEndCapture(h->stream, &g);
+        if (ce != hipSuccess) return hip_fail(h, ce, "graph capture");
+        if (ee != hipSuccess) return hip_fail(h, ee, "graph capture");
+        const hipError_t ie = hipGraphInstantiate(&h->innerGraph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) return hip_fail(h, ie, "graph instantiate");
+        h->graphBytesPerIter = it.bytes;
+    }
+    auto predict = [&](double need) {
+        if (!(h->rate > 0.0 && h->rate < 1.0) || !(need < 1.0)) return kInnerChunk;
+        const double k = std::ceil(std::log(need) / std::log(h->rate));
+        return (int)std::max(1.0, std::min((double)kMaxChunk, k));
+    };
+    int chunk = predict(needRed);
+    for (;;)
+    {
+        if (graphs)
+            for (int q = 0; q < chunk; q++)
+            {
+                HIP_TRY(h, hipGraphLaunch(h->innerGraph, h->stream), "graph launch");
+                launched++;
+                perIter = h->graphBytesPerIter;
+            }
+        else
+            for (int q = 0; q < chunk; q++)
+            {
+                h->curIter = launched++;
+                Traffic it;
+                HIP_TRY(h, inner_iteration(h, it), "inner iteration");
+                perIter = it.bytes;
+            }
+        const int rc = sync_state(h);
+        if (rc) return rc;
+        const SolveState &st = *h->hState;
+        if (st.iters > 0 && st.rr > 0.0 && st.rr < rr0) h->rate = std::pow(st.rr / rr0, 0.5 / st.iters);
+        if (st.done) break;
+        chunk = st.rr > 0.0 && st.thr2 > 0.0 ? predict(std::sqrt(st.thr2 / st.rr)) : kInnerChunk;
+    }
+    *ran = h->hState->iters;
+    profile_collect(h, *ran);
+    tr.add(perIter * *ran);   // launched iterations past convergence exit at once and move nothing
+    h->curIter = -1;
+    return NPPE_OK;
+}
+
 int solve_mgpcg(nppe_solver *h, const double *dB, double *dX, const double *dGuess, double refNorm, nppe_result *res)
 {
     const DevLevel &L0 = h->levels[0];
@@ -808,10 +913,11 @@ int solve_mgpcg(nppe_solver *h, const double *dB, double *dX, const double *dGue
     {
         Scope sc(h, NPPE_K_RESIDUAL64, 21.0 * N);
         HIP_TRY(h, launch_update_residual64(h->stream, L0.box, h->t64, L0.types, h->dX64[cur], nullptr, nullptr,
-                                            h->dB64, h->dR, nullptr, h->dState, h->dPartials),
+                                            h->dB64, h->dR, h->krylov64 ? h->dR64 : nullptr, h->dState,
+                                            h->dPartials),
                 "residual");
     }
-    tr.add(21.0 * N);
+    tr.add((h->krylov64 ? 29.0 : 21.0) * N);
     int rc = sync_state(h);
     if (rc) return rc;
     profile_collect(h, 0);
@@ -841,79 +947,43 @@ int solve_mgpcg(nppe_solver *h, const double *dB, double *dX, const double *dGue
             status = NPPE_NOT_CONVERGED;
             break;
         }
-        // Aim each inner solve at the larger of the configured reduction and what the target still needs
-        // (with a margin for the FP32 round-off that the FP64 residual sees).
-        const double need = 0.25 * target / rnorm;
-        const double red = std::min(0.5, std::max(h->opt.inner_reduction, need));
-        HIP_TRY(h, launch_inner_setup(h->stream, h->dState, red, h->opt.max_iters - inner), "inner setup");
-        int launched = 0;
-        double perIter = 0.0;
-        const bool graphs = h->useGraphs && !h->profiling;
-        if (graphs && !h->innerGraph)
+        if (h->krylov64)
         {
-            // Capture one iteration once: its launches, their arguments and the z ping-pong parity are the same
-            // every time, and every kernel reads the device-side done flag, so one graph serves every iteration.
-            hipGraph_t g = nullptr;
-            Traffic it;
-            std::lock_guard<std::mutex> lock(gCaptureMu);
-            HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "graph capture");
-            const hipError_t ce = inner_iteration(h, it);
-            const hipError_t ee = hipStreamEndCapture(h->stream, &g);
-            if (ce != hipSuccess) return hip_fail(h, ce, "graph capture");
-            if (ee != hipSuccess) return hip_fail(h, ee, "graph capture");
-            const hipError_t ie = hipGraphInstantiate(&h->innerGraph, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (ie != hipSuccess) return hip_fail(h, ie, "graph instantiate");
-            h->graphBytesPerIter = it.bytes;
+            // One FP64 CG from the current x, aimed below the target on its recursive FP64 residual; then the
+            // true residual decides (a miss restarts from the new x, counted as another pass).
+            const double aim = 0.8 * target;
+            HIP_TRY(h, launch_state_init(h->stream, h->dState, aim * aim, h->opt.max_iters - inner), "krylov setup");
+            int done = 0;
+            if ((rc = run_inner(h, std::min(0.5, aim / rnorm), rnorm * rnorm, tr, &done))) return rc;
+            inner += done;
+            {
+                Scope sc(h, NPPE_K_RESIDUAL64, 29.0 * N);
+                HIP_TRY(h, launch_update_residual64(h->stream, L0.box, h->t64, L0.types, h->dX64[cur], nullptr,
+                                                    nullptr, h->dB64, h->dR, h->dR64, h->dState, h->dPartials),
+                        "true residual");
+            }
+            tr.add(29.0 * N);
         }
-        // Iterations to enqueue before the next host check: the count the observed per-iteration reduction
-        // predicts for the reduction still needed (the device stops at convergence, so an overshoot costs only
-        // launches that exit at once; an undershoot costs one more check).
-        const double rr0 = rnorm * rnorm;
-        auto predict = [&](double needRed) {
-            if (!(h->rate > 0.0 && h->rate < 1.0) || !(needRed < 1.0)) return kInnerChunk;
-            const double k = std::ceil(std::log(needRed) / std::log(h->rate));
-            return (int)std::max(1.0, std::min((double)kMaxChunk, k));
-        };
-        int chunk = predict(red);
-        for (;;)
+        else
         {
-            if (graphs)
-                for (int q = 0; q < chunk; q++)
-                {
-                    HIP_TRY(h, hipGraphLaunch(h->innerGraph, h->stream), "graph launch");
-                    launched++;
-                    perIter = h->graphBytesPerIter;
-                }
-            else
-                for (int q = 0; q < chunk; q++)
-                {
-                    h->curIter = launched++;
-                    Traffic it;
-                    HIP_TRY(h, inner_iteration(h, it), "inner iteration");
-                    perIter = it.bytes;
-                }
-            rc = sync_state(h);
-            if (rc) return rc;
-            const SolveState &st = *h->hState;
-            if (st.iters > 0 && st.rr > 0.0 && st.rr < rr0)
-                h->rate = std::pow(st.rr / rr0, 0.5 / st.iters);
-            if (st.done) break;
-            chunk = st.rr > 0.0 && st.thr2 > 0.0 ? predict(std::sqrt(st.thr2 / st.rr)) : kInnerChunk;
+            // Aim each inner solve at the larger of the configured reduction and what the target still needs
+            // (with a margin for the FP32 round-off that the FP64 residual sees).
+            const double need = 0.25 * target / rnorm;
+            const double red = std::min(0.5, std::max(h->opt.inner_reduction, need));
+            HIP_TRY(h, launch_inner_setup(h->stream, h->dState, red, h->opt.max_iters - inner), "inner setup");
+            int done = 0;
+            if ((rc = run_inner(h, red, rnorm * rnorm, tr, &done))) return rc;
+            inner += done;
+            {
+                Scope sc(h, NPPE_K_RESIDUAL64, 33.0 * N);
+                HIP_TRY(h, launch_update_residual64(h->stream, L0.box, h->t64, L0.types, h->dX64[cur], h->dE,
+                                                    h->dX64[1 - cur], h->dB64, h->dR, nullptr, h->dState,
+                                                    h->dPartials),
+                        "refinement residual");
+            }
+            tr.add(33.0 * N);
+            cur = 1 - cur;
         }
-        const int done = h->hState->iters;
-        profile_collect(h, done);
-        tr.add(perIter * done);   // launched iterations past convergence exit at once and move nothing
-        inner += done;
-        h->curIter = -1;
-        {
-            Scope sc(h, NPPE_K_RESIDUAL64, 33.0 * N);
-            HIP_TRY(h, launch_update_residual64(h->stream, L0.box, h->t64, L0.types, h->dX64[cur], h->dE,
-                                                h->dX64[1 - cur], h->dB64, h->dR, nullptr, h->dState, h->dPartials),
-                    "refinement residual");
-        }
-        tr.add(33.0 * N);
-        cur = 1 - cur;
         rc = sync_state(h);
         if (rc) return rc;
         profile_collect(h, 0);
@@ -1121,10 +1191,11 @@ void nppe_default_options(nppe_options *o)
     o->tolerance = 1e-8;
     o->max_iters = 2000;
     o->compatibility = 1;
-    o->pre_smooth = 2;
-    o->post_smooth = 2;
+    o->pre_smooth = NPPE_SMOOTH_AUTO;
+    o->post_smooth = NPPE_SMOOTH_AUTO;
     o->inner_reduction = 3e-5;
     o->max_refinements = 8;
+    o->krylov = NPPE_KRYLOV_FP64;
     o->hbm_budget = 0;
 }
 
@@ -1142,8 +1213,9 @@ int nppe_create(nppe_solver **out, const nppe_options *opt)
     nppe_default_options(&o);
     if (opt) o = *opt;
     if (!(o.tolerance > 0.0) || o.max_iters < 0 || (o.method != NPPE_METHOD_MGPCG && o.method != NPPE_METHOD_CG64) ||
-        o.pre_smooth < 1 || o.post_smooth < 0 || !(o.inner_reduction > 0.0 && o.inner_reduction < 1.0) ||
-        o.max_refinements < 1)
+        (o.pre_smooth < 1 && o.pre_smooth != NPPE_SMOOTH_AUTO) || (o.post_smooth < 0 && o.post_smooth != NPPE_SMOOTH_AUTO) ||
+        !(o.inner_reduction > 0.0 && o.inner_reduction < 1.0) ||
+        o.max_refinements < 1 || (o.krylov != NPPE_KRYLOV_REFINE && o.krylov != NPPE_KRYLOV_FP64))
     {
         g_create_error = "invalid options";
         return NPPE_ERR_ARGUMENT;
@@ -1348,8 +1420,13 @@ int nppe_set_tolerance(nppe_solver *h, double tolerance)
     return NPPE_OK;
 }
 
-uint64_t nppe_estimate_bytes(const nppe_grid *g, int32_t method)
+uint64_t nppe_estimate_bytes_for(const nppe_grid *g, const nppe_options *opt)
 {
+    nppe_options o;
+    nppe_default_options(&o);
+    if (opt) o = *opt;
+    const int32_t method = o.method;
+    const bool k64 = method == NPPE_METHOD_MGPCG && o.krylov == NPPE_KRYLOV_FP64 && g && g->dim == 3;
     if (!g || (g->dim != 2 && g->dim != 3) || g->nx < 3 || g->ny < 3) return 0;
     const bool t3 = g->dim == 3;
     int ns = t3 ? g->ny : 1, nm = t3 ? g->nx : g->ny, nf = t3 ? g->nz : g->nx;
@@ -1363,6 +1440,7 @@ uint64_t nppe_estimate_bytes(const nppe_grid *g, int32_t method)
         {
             total += cells * 24.0;                                      // x64 pair, b64
             total += cells * (method == NPPE_METHOD_CG64 ? 24.0 : 28.0);   // FP64 r, p, q / FP32 e, r, p pair, q, z pair
+            if (k64) total += cells * 8.0;                                  // FP64 Krylov r
             total += (double)(ns - (t3 ? 2 : 0)) * (nm - 2) * (nf - 2) * (24.0 + 4.0);   // staging + order
         }
         const int si = ns - 2, mi = nm - 2, fi = nf - 2;
@@ -1374,6 +1452,14 @@ uint64_t nppe_estimate_bytes(const nppe_grid *g, int32_t method)
     }
     total += (double)kMaxDenseCoarse * kMaxDenseCoarse * 8.0 / 4.0 + 1.0e6;   // coarse matrix, partials, state
     return (uint64_t)(total * 1.05);
+}
+
+uint64_t nppe_estimate_bytes(const nppe_grid *g, int32_t method)
+{
+    nppe_options o;
+    nppe_default_options(&o);
+    o.method = method;
+    return nppe_estimate_bytes_for(g, &o);
 }
 
 int nppe_memory_bytes(const nppe_solver *h, uint64_t *bytes)

@@ -33,9 +33,10 @@ struct HipSettings
     int method = NPPE_METHOD_MGPCG;   // hip_solver = MGPCG | CG
     double tolerance = 1e-8;          // hip_tolerance
     int maxIters = 2000;              // hip_max_iters
-    int preSmooth = 2;                // hip_pre_smooth
-    int postSmooth = 2;               // hip_post_smooth
+    int preSmooth = NPPE_SMOOTH_AUTO;   // hip_pre_smooth (AUTO or a count)
+    int postSmooth = NPPE_SMOOTH_AUTO;  // hip_post_smooth
     double innerReduction = 3e-5;     // hip_inner_reduction
+    int krylov = NPPE_KRYLOV_FP64;    // hip_krylov = FP64 | REFINE
     int device = 0;                   // hip_device
 };
 
@@ -80,8 +81,14 @@ inline HipSettings ReadHipSettings(const std::string &text)
             }
             else if (key == "hip_tolerance") s.tolerance = std::stod(val);
             else if (key == "hip_max_iters") s.maxIters = std::stoi(val);
-            else if (key == "hip_pre_smooth") s.preSmooth = std::stoi(val);
-            else if (key == "hip_post_smooth") s.postSmooth = std::stoi(val);
+            else if (key == "hip_pre_smooth") s.preSmooth = val == "AUTO" ? NPPE_SMOOTH_AUTO : std::stoi(val);
+            else if (key == "hip_post_smooth") s.postSmooth = val == "AUTO" ? NPPE_SMOOTH_AUTO : std::stoi(val);
+            else if (key == "hip_krylov")
+            {
+                if (val == "FP64") s.krylov = NPPE_KRYLOV_FP64;
+                else if (val == "REFINE") s.krylov = NPPE_KRYLOV_REFINE;
+                else std::cerr << "Warning: unknown hip_krylov '" << val << "', using FP64." << std::endl;
+            }
             else if (key == "hip_inner_reduction") s.innerReduction = std::stod(val);
             else if (key == "hip_device") s.device = std::stoi(val);
             else std::cerr << "Warning: unknown pressure key '" << key << "'." << std::endl;
@@ -93,8 +100,8 @@ inline HipSettings ReadHipSettings(const std::string &text)
     }
     if (!(s.tolerance > 0.0)) s.tolerance = 1e-8;
     if (s.maxIters <= 0) s.maxIters = 2000;
-    if (s.preSmooth < 1) s.preSmooth = 2;
-    if (s.postSmooth < 0) s.postSmooth = 2;
+    if (s.preSmooth < 1 && s.preSmooth != NPPE_SMOOTH_AUTO) s.preSmooth = NPPE_SMOOTH_AUTO;
+    if (s.postSmooth < 0 && s.postSmooth != NPPE_SMOOTH_AUTO) s.postSmooth = NPPE_SMOOTH_AUTO;
     if (!(s.innerReduction > 0.0 && s.innerReduction < 1.0)) s.innerReduction = 3e-5;
     return s;
 }
@@ -126,6 +133,7 @@ inline HipPressureSolver *CreatePressureBackendHIP(const HipSettings &s, bool ne
     o.pre_smooth = s.preSmooth;
     o.post_smooth = s.postSmooth;
     o.inner_reduction = s.innerReduction;
+    o.krylov = s.krylov;
     o.compatibility = needsCompatibility ? 1 : 0;
     HipPressureSolver *p = new HipPressureSolver();
     const int rc = nppe_create(&p->h, &o);

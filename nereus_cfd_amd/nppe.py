@@ -20,6 +20,7 @@ OK, NOT_CONVERGED = 0, 1
 ERR_ARGUMENT, ERR_DEVICE, ERR_NO_GRID, ERR_SIZE, ERR_NONFINITE, ERR_BUDGET, ERR_MISMATCH, ERR_INTERNAL = (
     -1, -2, -3, -4, -5, -6, -7, -8)
 METHOD_MGPCG, METHOD_CG64 = 0, 1
+KRYLOV_REFINE, KRYLOV_FP64 = 0, 1
 FLUID, SOLID, EMPTY, INFLOW, PRESSURE, OUTFLOW = 0, 1, 2, 3, 4, 5
 
 KERNEL_KINDS = [
@@ -32,7 +33,7 @@ EXPORTS = [
     "nppe_default_options", "nppe_create", "nppe_destroy", "nppe_last_error", "nppe_commit_grid",
     "nppe_system_info", "nppe_unknown_order", "nppe_check_csr", "nppe_solve", "nppe_solve_device", "nppe_apply",
     "nppe_memory_bytes", "nppe_profile_enable", "nppe_profile_select", "nppe_profile_read", "nppe_synth_cosine_rhs",
-    "nppe_set_tolerance", "nppe_estimate_bytes", "nppe_batch_solve", "nppe_server_create", "nppe_server_destroy",
+    "nppe_set_tolerance", "nppe_estimate_bytes", "nppe_estimate_bytes_for", "nppe_batch_solve", "nppe_server_create", "nppe_server_destroy",
     "nppe_server_add_grid", "nppe_server_submit", "nppe_server_wait", "nppe_server_last_error",
     "nppe_server_memory",
 ]
@@ -48,7 +49,7 @@ class Options(C.Structure):
     _fields_ = [
         ("device", C.c_int32), ("method", C.c_int32), ("tolerance", C.c_double), ("max_iters", C.c_int32),
         ("compatibility", C.c_int32), ("pre_smooth", C.c_int32), ("post_smooth", C.c_int32),
-        ("inner_reduction", C.c_double), ("max_refinements", C.c_int32), ("reserved0", C.c_int32),
+        ("inner_reduction", C.c_double), ("max_refinements", C.c_int32), ("krylov", C.c_int32),
         ("hbm_budget", C.c_uint64),
     ]
 
@@ -121,6 +122,8 @@ def lib():
     L.nppe_set_tolerance.restype = C.c_int
     L.nppe_estimate_bytes.argtypes = [P, I32]
     L.nppe_estimate_bytes.restype = C.c_uint64
+    L.nppe_estimate_bytes_for.argtypes = [P, P]
+    L.nppe_estimate_bytes_for.restype = C.c_uint64
     L.nppe_batch_solve.argtypes = [P, P, P, I32, I32, I32, P]
     L.nppe_batch_solve.restype = C.c_int
     L.nppe_server_create.argtypes = [P, P, I32, C.c_uint64]

@@ -33,7 +33,8 @@ def test_library_exports_every_declared_symbol():
 def test_default_options():
     o = N.default_options()
     assert o.method == N.METHOD_MGPCG and o.tolerance == 1e-8 and o.max_iters == 2000
-    assert o.compatibility == 1 and o.pre_smooth == 2 and o.post_smooth == 2 and o.max_refinements == 8
+    assert o.compatibility == 1 and o.pre_smooth == -1 and o.post_smooth == -1 and o.max_refinements == 8
+    assert o.krylov == N.KRYLOV_FP64
 
 
 def test_create_reports_errors():

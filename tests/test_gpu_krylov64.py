@@ -1,0 +1,87 @@
+"""MGPCG with the FP64 Krylov recursion around the FP32 V-cycle (nppe_options::krylov = NPPE_KRYLOV_FP64, the
+default): the true FP64 residual meets the tolerance, the solution matches the oracle's FP64 CG and the
+refinement scheme's (NPPE_KRYLOV_REFINE), repeated solves are bit-identical, and graph replay equals direct
+launches."""
+import numpy as np
+import pytest
+
+import oracle as O
+import nereus_cfd_amd as N
+import nereus_cfd_amd.workloads as W
+from tests.cases import make
+
+pytestmark = pytest.mark.gpu
+
+REFS = ((O.Refinement(0.3, 0.6, 2, O.REFINE_HARD),), (O.Refinement(0.2, 0.8, 3, O.REFINE_SYMMETRIC),),
+        (O.Refinement(0.1, 0.5, 2, O.REFINE_RAMP),))
+
+
+@pytest.mark.parametrize("n,kind,stretched,tol", [
+    (24, "neumann", False, 1e-8),
+    (20, "outlet", False, 1e-10),
+    (22, "mixed", False, 1e-9),
+    (18, "mixed", True, 1e-9),
+    (20, "freesurface", False, 1e-8),
+])
+def test_krylov64_matches_oracle(n, kind, stretched, tol):
+    g, og = make(3, n, kind, non_uniform=stretched, refs=REFS if stretched else ((), (), ()), seed=4)
+    s = O.assemble(og)
+    b = np.random.default_rng(6).standard_normal(s.n)
+    if s.singular:
+        b -= b.mean()
+        b -= b.mean()
+    xr, _, _ = s.cg(b, 1e-13, 100000)
+    with N.Solver(tolerance=tol, krylov=N.KRYLOV_FP64) as h:
+        assert h.commit(g) == s.n
+        x, res, rc = h.solve(b)
+    assert rc == 0
+    assert s.relres(b, x) < tol * 1.01 and res.relative_residual < tol
+    if s.singular:
+        x = x - x.mean()
+        xr = xr - xr.mean()
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 2e3 * tol
+    with N.Solver(tolerance=tol, krylov=N.KRYLOV_REFINE) as h2:
+        h2.commit(g)
+        x2, _, _ = h2.solve(b)
+    if s.singular:
+        x2 = x2 - x2.mean()
+    assert np.linalg.norm(x - x2) / np.linalg.norm(xr) < 2e3 * tol
+
+
+def test_krylov64_channel_to_1e10():
+    g = W.stretched_channel(48, 24, 16)
+    s = O.assemble(O.Grid(g.dim, g.types, g.wx, g.wy, g.wz, g.dh, g.non_uniform))
+    b = np.random.default_rng(11).standard_normal(s.n)
+    with N.Solver(tolerance=1e-10, krylov=N.KRYLOV_FP64) as h:
+        h.commit(g)
+        x, res, rc = h.solve(b)
+    with N.Solver(tolerance=1e-10, krylov=N.KRYLOV_REFINE) as h2:
+        h2.commit(g)
+        _, res2, _ = h2.solve(b)
+    assert rc == 0 and s.relres(b, x) < 1e-10 * 1.01
+    assert res.refinements == 1 and res.iterations <= res2.iterations
+
+
+def test_krylov64_deterministic_and_graph_equals_direct():
+    g = N.uniform_grid(40)
+    b = N.synth_rhs(g)
+    with N.Solver(krylov=N.KRYLOV_FP64) as h:
+        h.commit(g)
+        x1, r1, _ = h.solve(b)          # graph replay
+        x2, r2, _ = h.solve(b)
+        h.profile(True)                 # direct launches
+        x3, r3, _ = h.solve(b)
+    assert r1.iterations == r2.iterations == r3.iterations
+    assert np.array_equal(x1, x2) and np.array_equal(x1, x3)
+
+
+def test_krylov64_2d_falls_back_to_refinement():
+    g = N.uniform_grid(64, dim=2)
+    b = N.synth_rhs(g)
+    with N.Solver(krylov=N.KRYLOV_FP64) as h:
+        h.commit(g)
+        x, res, rc = h.solve(b)
+    with N.Solver(krylov=N.KRYLOV_REFINE) as h2:
+        h2.commit(g)
+        x2, res2, _ = h2.solve(b)
+    assert rc == 0 and np.array_equal(x, x2)
