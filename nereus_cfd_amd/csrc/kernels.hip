@@ -1720,6 +1720,42 @@ __device__ __forceinline__ double apply64v(const Box &B, const Tables64 &T, cons
     return t;
 }
 
+// (A x) of one cell of a unit-coefficient (3D uniform) grid in FP64 from its value and neighbours (order as
+// apply64v), without the coefficient tables or cell types: every off-diagonal is scale, and the diagonal is
+// -scale times the count of coupled and ring-FLUID neighbours plus interior Dirichlet neighbours plus twice the
+// ring Dirichlet ones (the ring factor FaceGap / (Width / 2) is exactly 2 on uniform spacing). Those are small
+// integers, so the entries and the column-ordered sum are op64 / row64's bit for bit.
+__device__ __forceinline__ double apply64u(const Tables64 &T, int64_t idx, double xc, const double (&nb)[6],
+                                           unsigned code)
+{
+#pragma clang fp contract(off)
+    unsigned coupled = code & 63u, n;
+    if (!(code & 128u))
+        n = __popc(coupled);
+    else
+    {
+        const unsigned c16 = T.c16[idx];
+        n = 0;
+#pragma unroll
+        for (int q = 0; q < 6; q++)
+        {
+            const unsigned st = (c16 >> (2 * q)) & 3u;
+            n += st == 3u ? 2u : (st != 0u ? 1u : 0u);
+        }
+    }
+    const double v = T.scale * 1.0;
+    const double dv = T.scale * (-(double)n);
+    double t = 0.0;
+    if (coupled & 2u) t += v * nb[1];
+    if (coupled & 8u) t += v * nb[3];
+    if (coupled & 32u) t += v * nb[5];
+    if (n) t += dv * xc;
+    if (coupled & 16u) t += v * nb[4];
+    if (coupled & 4u) t += v * nb[2];
+    if (coupled & 1u) t += v * nb[0];
+    return t;
+}
+
 __global__ __launch_bounds__(kBlockThreads) void k_cg_spmv_m(Level32 L, const float *__restrict__ p,
                                                              float *__restrict__ q, const SolveState *st,
                                                              Red red, int P)
@@ -1832,6 +1868,7 @@ __device__ __forceinline__ void tile_halo(int t, int &hy, int &hx)
 // FP32, and <p, A p> with A p computed in the reference's FP64 arithmetic from the stored FP32 p (part 2 applies
 // A to the same FP32 values, so x and r stay consistent). p ping-pongs between pA (even inner iterations) and pB
 // (odd); plane p staged in LDS, slow neighbours in registers, loads a plane ahead.
+template <bool UNI>
 __global__ __launch_bounds__(kBlockThreads) void k_cg_dir64_t(Box B, Tables64 T, const uint8_t *__restrict__ ty,
                                                               const float *__restrict__ z, float *__restrict__ pA,
                                                               float *__restrict__ pB, const SolveState *st, Red red,
@@ -1873,7 +1910,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_dir64_t(Box B, Tables64 T,
         if (M.in && (cc & 64u))
         {
             const double nb[6] = {Pp, Pm, Ps[tyy + 2][tx + 1], Ps[tyy][tx + 1], Ps[tyy + 1][tx + 2], Ps[tyy + 1][tx]};
-            const double v = apply64v(B, T, ty, sl, M.m, M.f, idx, Pc, nb, cc);
+            const double v = UNI ? apply64u(T, idx, Pc, nb, cc) : apply64v(B, T, ty, sl, M.m, M.f, idx, Pc, nb, cc);
             pn[idx] = (float)Pc;
             pq += Pc * v;
         }
@@ -1889,6 +1926,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_dir64_t(Box B, Tables64 T,
 }
 
 // Part 2: q = A p (FP64, from the FP32 p of part 1), x += alpha p, r -= alpha q, r32 = (float) r, and <r, r>.
+template <bool UNI>
 __global__ __launch_bounds__(kBlockThreads) void k_cg_update64_t(Box B, Tables64 T, const uint8_t *__restrict__ ty,
                                                                  const float *__restrict__ pA,
                                                                  const float *__restrict__ pB,
@@ -1930,7 +1968,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_update64_t(Box B, Tables64
         if (M.in && (cc & 64u))
         {
             const double nb[6] = {Pp, Pm, Ps[tyy + 2][tx + 1], Ps[tyy][tx + 1], Ps[tyy + 1][tx + 2], Ps[tyy + 1][tx]};
-            const double q = apply64v(B, T, ty, sl, M.m, M.f, idx, Pc, nb, cc);
+            const double q = UNI ? apply64u(T, idx, Pc, nb, cc) : apply64v(B, T, ty, sl, M.m, M.f, idx, Pc, nb, cc);
             const double xv = xc + alpha * Pc, rv = rc - alpha * q;
             x[idx] = xv;
             r[idx] = rv;
@@ -2044,6 +2082,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64_m(Box B, Ta
 // the 256 threads, of one halo cell) once per plane, the in-plane neighbours come from LDS, and the loads of a
 // plane are issued a plane ahead (the cell's own X two planes ahead). Tile positions outside the box load clamped
 // in-box values that no interior cell couples to.
+template <bool UNI>
 __global__ __launch_bounds__(kBlockThreads) void k_update_residual64_t(Box B, Tables64 T,
                                                                        const uint8_t *__restrict__ ty,
                                                                        const double *__restrict__ x,
@@ -2094,7 +2133,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64_t(Box B, Ta
         if (M.in && (cc & 64u))
         {
             const double nb[6] = {Xp, Xm, Xs[tyy + 2][tx + 1], Xs[tyy][tx + 1], Xs[tyy + 1][tx + 2], Xs[tyy + 1][tx]};
-            r = bc - apply64v(B, T, ty, s, M.m, M.f, idx, Xc, nb, cc);
+            r = bc - (UNI ? apply64u(T, idx, Xc, nb, cc) : apply64v(B, T, ty, s, M.m, M.f, idx, Xc, nb, cc));
             if (e) xout[idx] = Xc;
         }
         if (M.in)
@@ -2115,6 +2154,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64_t(Box B, Ta
     block_sum<1>(a, red, gridDim.x, M.lb);
 }
 
+// UNI: the unit-coefficient row (apply64u), the one the FP64 Krylov and residual kernels use on uniform grids.
+template <bool UNI>
 __global__ __launch_bounds__(kBlockThreads) void k_apply64_m(Box B, Tables64 T, const uint8_t *__restrict__ ty,
                                                              const double *__restrict__ x, double *__restrict__ y,
                                                              int P)
@@ -2130,7 +2171,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_apply64_m(Box B, Tables64 T, 
         if (T.c8[idx] & 64u)
         {
             const double nb[6] = {xp, xm, x[idx + B.sM], x[idx - B.sM], x[idx + 1], x[idx - 1]};
-            v = apply64v(B, T, ty, s, M.m, M.f, idx, xc, nb, T.c8[idx]);
+            v = UNI ? apply64u(T, idx, xc, nb, T.c8[idx]) : apply64v(B, T, ty, s, M.m, M.f, idx, xc, nb, T.c8[idx]);
         }
         y[idx] = v;
         xm = xc;
@@ -2269,7 +2310,10 @@ hipError_t launch_update_residual64(hipStream_t s, const Box &B, const Tables64 
     {
         const int P = march_planes(B), G = march_grid(B, P);
         const Red red = red_of(partials, st, FIN_SUM1, G);
-        k_update_residual64_t<<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, red, P);
+        if (T.unitCoef)
+            k_update_residual64_t<true><<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, red, P);
+        else
+            k_update_residual64_t<false><<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, red, P);
         return finish(s, red, G, 1);
     }
     const int G = grid_of(B);
@@ -2284,7 +2328,10 @@ hipError_t launch_apply64(hipStream_t s, const Box &B, const Tables64 &T, const 
     if (B.threeD)
     {
         const int P = march_planes(B);
-        k_apply64_m<<<march_grid(B, P), kBlockThreads, 0, s>>>(B, T, ty, x, y, P);
+        if (T.unitCoef)
+            k_apply64_m<true><<<march_grid(B, P), kBlockThreads, 0, s>>>(B, T, ty, x, y, P);
+        else
+            k_apply64_m<false><<<march_grid(B, P), kBlockThreads, 0, s>>>(B, T, ty, x, y, P);
         return hipGetLastError();
     }
     k_apply64<<<grid_of(B), kBlockThreads, 0, s>>>(B, T, ty, x, y);
@@ -2524,7 +2571,10 @@ hipError_t launch_cg_dir64(hipStream_t s, const Box &B, const Tables64 &T, const
 {
     const int P = march_planes(B), G = march_grid(B, P);
     const Red red = red_of(partials, st, FIN_PQ, G);
-    k_cg_dir64_t<<<G, kBlockThreads, 0, s>>>(B, T, ty, z, pA, pB, st, red, P);
+    if (T.unitCoef)
+        k_cg_dir64_t<true><<<G, kBlockThreads, 0, s>>>(B, T, ty, z, pA, pB, st, red, P);
+    else
+        k_cg_dir64_t<false><<<G, kBlockThreads, 0, s>>>(B, T, ty, z, pA, pB, st, red, P);
     return finish(s, red, G, 1);
 }
 
@@ -2533,7 +2583,10 @@ hipError_t launch_cg_update64(hipStream_t s, const Box &B, const Tables64 &T, co
 {
     const int P = march_planes(B), G = march_grid(B, P);
     const Red red = red_of(partials, st, FIN_RR, G);
-    k_cg_update64_t<<<G, kBlockThreads, 0, s>>>(B, T, ty, pA, pB, x, r, r32, st, red, P);
+    if (T.unitCoef)
+        k_cg_update64_t<true><<<G, kBlockThreads, 0, s>>>(B, T, ty, pA, pB, x, r, r32, st, red, P);
+    else
+        k_cg_update64_t<false><<<G, kBlockThreads, 0, s>>>(B, T, ty, pA, pB, x, r, r32, st, red, P);
     return finish(s, red, G, 1);
 }
 

@@ -62,6 +62,7 @@ struct Tables64
     double scale;   // MatrixScale3D: 1/(dh*dh) uniform, 1 stretched
     int unitCoef;   // 3D uniform: every face coefficient is exactly 1.0 (PressureSolver3D.cpp:49-53)
     const uint8_t *c8;   // the fine level's stencil codes (a cell without a Dirichlet neighbour takes a fast path)
+    const uint16_t *c16; // and their 2-bit-per-neighbour form (unit-coefficient rows beside a Dirichlet cell)
 };
 
 // Device-resident scalars of one solve. Written by the last block of each reduction.

@@ -731,7 +731,7 @@ int build(nppe_solver *h, const nppe_grid *g)
         HIP_TRY(h, upload(h, h->dTab64, tab.data(), tab.size() * sizeof(double)),
                 "upload FP64 tables");
         h->t64 = Tables64{h->dTab64 + off[0], h->dTab64 + off[1], h->dTab64 + off[2], h->dTab64 + off[3],
-                          h->dTab64 + off[4], h->dTab64 + off[5], scale, unitCoef ? 1 : 0, F.c8};
+                          h->dTab64 + off[4], h->dTab64 + off[5], scale, unitCoef ? 1 : 0, F.c8, F.c16};
     }
     {
         std::vector<int32_t> pitched(h->orderHost.size());
