@@ -46,8 +46,11 @@ hipError_t launch_cg64_direction(hipStream_t s, const Box &B, const Tables64 &T,
 // With rzPartials non-null the sweep is the CG's last level-0 sweep and also reduces <b, x_out> into st (as
 // launch_cg_rz(b, x_out) does). The whole-row sweep (rbgs_family RBGS_ROW) writes one partial per interior row and
 // plane, which launch_rz_rows_finalize then sums; the other sweeps complete the reduction themselves.
+// With prolongFrom non-null the sweep first takes x_in += P x_c (x_c = prolongXc on level prolongFrom): folded
+// into the whole-row sweep's loads (x_in itself is not written), or as a separate prolongation pass before it.
 hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float *xout, const float *b, int c1,
-                       int zero, int tileRows, int planesPerChunk, SolveState *st, double *rzPartials = nullptr);
+                       int zero, int tileRows, int planesPerChunk, SolveState *st, double *rzPartials = nullptr,
+                       const Level32 *prolongFrom = nullptr, const float *prolongXc = nullptr);
 hipError_t launch_rz_rows_finalize(hipStream_t s, const Box &B, SolveState *st, double *partials);
 // Which sweep kernel launch_rbgs runs for these operands, and the tile heights it accepts for it (every tile shape
 // gives bit-identical results: a colour's updates are independent of each other).

@@ -1151,6 +1151,15 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_march(Level32 L, const f
     }
 }
 
+// The coarse correction a whole-row sweep folds in (x_in + P x_c where x is used): the coarse level's iterate and
+// box strides.
+struct Prolong
+{
+    const float *xc;
+    int64_t sS;
+    int sM, ns, nf;
+};
+
 // ---- red-black Gauss-Seidel sweep, whole-row waves (3D uniform levels) -----------------------------------------
 //
 // As k_rbgs_march, but a wave spans the whole interior row: lane l owns the pairs (1 + 2p, 2 + 2p) for
@@ -1166,11 +1175,11 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_march(Level32 L, const f
 // With dot non-null the sweep also writes <b, x_out> over the unknowns of each interior row and plane to
 // dot[(plane - 1) * (nm - 2) + row - 1] (the CG's <r, z> when the sweep is the last one of the V-cycle on the
 // finest level): per row, not per block, so the sum over them does not depend on the tile shape.
-template <int TM, int K, bool ZERO>
+template <int TM, int K, bool ZERO, bool PRO>
 __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const float *__restrict__ xin,
                                                             float *__restrict__ xout, const float *__restrict__ b,
                                                             int c1, int P, int tilesM, const SolveState *st,
-                                                            double *__restrict__ dot)
+                                                            double *__restrict__ dot, Prolong pro)
 {
     if (st->done) return;
     const Box &B = L.B;
@@ -1205,6 +1214,19 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
         off[k] = (uint32_t)mc * sM + (uint32_t)fc;
         offO[k] = (uint32_t)mo * sM + (uint32_t)fc;
     }
+    // PRO: the coarse parent of the pair (2p+1, 2p+2) is column p+1 of row (m+1)/2, plane (s+1)/2
+    uint32_t offC[K], offCO[K];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+    {
+        const int Fc = min(lane + 64 * k + 1, pro.nf - 1);
+        offC[k] = PRO ? (uint32_t)((mc + 1) >> 1) * (uint32_t)pro.sM + (uint32_t)Fc : 0u;
+        offCO[k] = PRO ? (uint32_t)((mo + 1) >> 1) * (uint32_t)pro.sM + (uint32_t)Fc : 0u;
+    }
+    auto planeXC = [&](int pl) {
+        const int sc = (min(max(pl, 0), B.ns - 1) + 1) >> 1;
+        return pro.xc + (size_t)min(sc, pro.ns - 1) * (size_t)pro.sS;
+    };
     for (int i = threadIdx.x; i < 2 * (TM + 2) * kCols; i += 64 * (TM + 2)) (&red[0][0][0])[i] = 0.f;
     auto plane = [&](const float *base, int pl) { return base + (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
     auto planeC = [&](int pl) { return L.c8 + (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
@@ -1219,6 +1241,8 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
     float2 Bv[4][K];   // b of the pair
     unsigned C[4][K];  // codes of the pair (low byte: element a), masked where used
     float R[4][K];     // first-colour value of this lane's pair
+    float XC[4][K];    // PRO: the coarse correction of the pair (added to X where X is used)
+    float OC[2][K];    // PRO: and of the outer row's pair
 #pragma unroll
     for (int j = 0; j < 4; j++)
 #pragma unroll
@@ -1228,6 +1252,8 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
             O[j & 1][k] = make_float2(0.f, 0.f);
             R[j][k] = 0.f;
             C[j][k] = 0u;
+            XC[j][k] = 0.f;
+            OC[j & 1][k] = 0.f;
         }
     const int sp0 = s0 - 1;
     // slots: plane sp0 + d -> d & 3 (d & 1 for O)
@@ -1239,12 +1265,24 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
             const float *px = plane(xin, sp0 + d);
 #pragma unroll
             for (int k = 0; k < K; k++) X[(d + 4) & 3][k] = pair(px, off[k]);
+            if (PRO)
+            {
+                const float *pc = planeXC(sp0 + d);
+#pragma unroll
+                for (int k = 0; k < K; k++) XC[(d + 4) & 3][k] = pc[offC[k]];
+            }
         }
         if (haloRow)
         {
             const float *px = plane(xin, sp0);
 #pragma unroll
             for (int k = 0; k < K; k++) O[0][k] = pair(px, offO[k]);
+            if (PRO)
+            {
+                const float *pc = planeXC(sp0);
+#pragma unroll
+                for (int k = 0; k < K; k++) OC[0][k] = pc[offCO[k]];
+            }
         }
     }
     {
@@ -1284,15 +1322,30 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
 #pragma unroll
                 for (int k = 0; k < K; k++) O[O1][k] = pair(po, offO[k]);
             }
+            if (PRO)
+            {
+                const float *pc2 = planeXC(sp + 2);
+#pragma unroll
+                for (int k = 0; k < K; k++) XC[S2][k] = pc2[offC[k]];
+                if (haloRow)
+                {
+                    const float *pc1 = planeXC(sp + 1);
+#pragma unroll
+                    for (int k = 0; k < K; k++) OC[O1][k] = pc1[offCO[k]];
+                }
+            }
         }
+        // the old value of element e of pair k in slot j where it is used (PRO: with the coarse correction)
+#define NPPE_XV(j, k, e) (PRO ? X[j][k].e + XC[j][k] : X[j][k].e)
+#define NPPE_OV(j, k, e) (PRO ? O[j][k].e + OC[j][k] : O[j][k].e)
         // publish this row's old x of plane sp for the neighbouring rows
         if (!ZERO)
         {
 #pragma unroll
             for (int k = 0; k < K; k++)
             {
-                old[wave][0][lane + 64 * k] = X[S0][k].x;
-                old[wave][1][lane + 64 * k] = X[S0][k].y;
+                old[wave][0][lane + 64 * k] = NPPE_XV(S0, k, x);
+                old[wave][1][lane + 64 * k] = NPPE_XV(S0, k, y);
             }
         }
         // (1) old[] is complete; readers of red[sp & 1] (the second colour of plane sp-2) are done
@@ -1303,14 +1356,14 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
 #pragma unroll
         for (int k = 0; k < K; k++)
         {
-            const float xa = X[S0][k].x, xb = X[S0][k].y;
+            const float xa = NPPE_XV(S0, k, x), xb = NPPE_XV(S0, k, y);
             // in-row old neighbours: left of fa is the previous pair's b element, right of fa+1 the next pair's a
             float left = __shfl_up(xb, 1, 64);
             float right = __shfl_down(xa, 1, 64);
             if (K > 1)
             {
-                const float wrapL = k > 0 ? __shfl(X[S0][k > 0 ? k - 1 : 0].y, 63, 64) : 0.f;
-                const float wrapR = k + 1 < K ? __shfl(X[S0][k + 1 < K ? k + 1 : k].x, 0, 64) : 0.f;
+                const float wrapL = k > 0 ? __shfl(NPPE_XV(S0, k > 0 ? k - 1 : 0, y), 63, 64) : 0.f;
+                const float wrapR = k + 1 < K ? __shfl(NPPE_XV(S0, k + 1 < K ? k + 1 : k, x), 0, 64) : 0.f;
                 if (lane == 0) left = wrapL;
                 if (lane == 63) right = wrapR;
             }
@@ -1318,7 +1371,7 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
             if (!ZERO)
             {
                 const float lu = old[wUp][e1][lane + 64 * k], ld = old[wDn][e1][lane + 64 * k];
-                const float outer = e1 ? O[O0][k].y : O[O0][k].x;
+                const float outer = e1 ? NPPE_OV(O0, k, y) : NPPE_OV(O0, k, x);
                 up = wave == TM + 1 ? outer : lu;
                 dn = wave == 0 ? outer : ld;
             }
@@ -1327,8 +1380,8 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
             if (code & 64u)
             {
                 float acc = 0.f;
-                acc += (code & 1u) ? (e1 ? X[S1][k].y : X[S1][k].x) : 0.f;
-                acc += (code & 2u) ? (e1 ? X[Sm1][k].y : X[Sm1][k].x) : 0.f;
+                acc += (code & 1u) ? (e1 ? NPPE_XV(S1, k, y) : NPPE_XV(S1, k, x)) : 0.f;
+                acc += (code & 2u) ? (e1 ? NPPE_XV(Sm1, k, y) : NPPE_XV(Sm1, k, x)) : 0.f;
                 acc += (code & 4u) ? up : 0.f;
                 acc += (code & 8u) ? dn : 0.f;
                 acc += (code & 16u) ? (e1 ? right : xb) : 0.f;
@@ -1414,6 +1467,8 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
             }
         }
     };
+#undef NPPE_XV
+#undef NPPE_OV
     for (int sp = sp0; sp <= s1 + 1; sp += 4)
     {
         step(sp, std::integral_constant<int, 0>());
@@ -2421,8 +2476,16 @@ void rbgs_tile_candidates(int family, std::vector<int> &tileRows)
 }
 
 hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float *xout, const float *b, int c1,
-                       int zero, int tileRows, int planesPerChunk, SolveState *st, double *rzPartials)
+                       int zero, int tileRows, int planesPerChunk, SolveState *st, double *rzPartials,
+                       const Level32 *prolongFrom, const float *prolongXc)
 {
+    if (prolongFrom && (zero || rbgs_family(L, xin, xout, b) != RBGS_ROW))
+    {
+        // the correction as its own pass, then the plain sweep
+        hipError_t e = launch_prolong(s, L, const_cast<float *>(xin), *prolongFrom, prolongXc, st);
+        if (e != hipSuccess) return e;
+        return launch_rbgs(s, L, xin, xout, b, c1, zero, tileRows, planesPerChunk, st, rzPartials);
+    }
     if (rzPartials && rbgs_family(L, xin, xout, b) != RBGS_ROW)
     {
         hipError_t e = launch_rbgs(s, L, xin, xout, b, c1, zero, tileRows, planesPerChunk, st, nullptr);
@@ -2433,15 +2496,27 @@ hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float 
     if (rbgs_family(L, xin, xout, b) == RBGS_ROW)
     {
         const int K = B.nf - 2 <= 128 ? 1 : (B.nf - 2 <= 256 ? 2 : 4);
-        const int TM = (tileRows >= 10 && K <= 2) ? 14 : 6;
+        // 14-row tiles for K <= 2 when asked, but not for the prolongation-folding K = 2 sweep (1024-thread blocks
+        // cap its registers at 128, which it exceeds)
+        const int TM = (tileRows >= 10 && (K == 1 || (K == 2 && !prolongFrom))) ? 14 : 6;
         const int tilesM = std::max(1, (B.nm - 2 + TM - 1) / TM);
         const int P = std::max(1, planesPerChunk);
         const int chunks = std::max(1, (B.ns - 2 + P - 1) / P);
         const int grid = tilesM * chunks;
         const int threads = 64 * (TM + 2);
+        Prolong pro{nullptr, 0, 0, 1, 1};
+        if (prolongFrom) pro = Prolong{prolongXc, prolongFrom->B.sS, (int)prolongFrom->B.sM, prolongFrom->B.ns,
+                                       prolongFrom->B.nf};
 #define NPPE_ROW(TMv, Kv)                                                                                            \
-    (zero ? (k_rbgs_row<TMv, Kv, true><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st, rzPartials), 0)    \
-          : (k_rbgs_row<TMv, Kv, false><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st, rzPartials), 0))
+    (zero ? (k_rbgs_row<TMv, Kv, true, false><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st,             \
+                                                                       rzPartials, pro),                              \
+             0)                                                                                                      \
+          : (prolongFrom ? (k_rbgs_row<TMv, Kv, false, true><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, \
+                                                                                     st, rzPartials, pro),            \
+                            0)                                                                                       \
+                         : (k_rbgs_row<TMv, Kv, false, false><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, \
+                                                                                       st, rzPartials, pro),          \
+                            0)))
         if (TM == 14)
         {
             if (K == 1) NPPE_ROW(14, 1);

@@ -341,22 +341,31 @@ hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel, doubl
         DevLevel &C = h->levels[l + 1];
         const double N = L.cellsBox;
         const double pb = s * C.cellsBox + 2 * s * N + f * N;
-        {
-            Scope sc(h, NPPE_K_PROLONG, pb);
-            if ((e = launch_prolong(h->stream, L.v, xbuf(l, cur[l]), C.v, xbuf(l + 1, cur[l + 1]), h->dState)) !=
-                hipSuccess)
-                return e;
-        }
-        tr.add(pb);
         const int nu2 = (coarseNu > 0 && l >= 2) ? coarseNu : h->nuPost;
+        // The whole-row sweep folds the prolongation into its first post-smoothing sweep (it reads x + P x_c
+        // where it reads x); other sweeps get the correction as a pass of its own.
+        const bool foldProlong =
+            nu2 >= 1 && rbgs_family(L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l)) == RBGS_ROW;
+        if (!foldProlong)
+        {
+            {
+                Scope sc(h, NPPE_K_PROLONG, pb);
+                if ((e = launch_prolong(h->stream, L.v, xbuf(l, cur[l]), C.v, xbuf(l + 1, cur[l + 1]),
+                                        h->dState)) != hipSuccess)
+                    return e;
+            }
+            tr.add(pb);
+        }
         for (int sw = 0; sw < nu2; sw++)
         {
-            const double by = (3 * s + f) * N;
+            const bool fold = foldProlong && sw == 0;
+            const double by = (3 * s + f) * N + (fold ? s * C.cellsBox : 0.0);
             const bool dot = l == 0 && sw == nu2 - 1 && rzPartials;
             {
                 Scope sc(h, l == 0 ? NPPE_K_SMOOTH0 : NPPE_K_SMOOTH, by);
                 if ((e = launch_rbgs(h->stream, L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l), 1, 0,
-                                     L.tileRows, L.planesPerChunk, h->dState, dot ? rzPartials : nullptr)) !=
+                                     L.tileRows, L.planesPerChunk, h->dState, dot ? rzPartials : nullptr,
+                                     fold ? &C.v : nullptr, fold ? xbuf(l + 1, cur[l + 1]) : nullptr)) !=
                     hipSuccess)
                     return e;
             }
