@@ -67,6 +67,8 @@ def lib():
         L.oracle_subtract_mean.argtypes = [P, C.c_int64]
         L.oracle_relres.argtypes = [C.c_int32, P, P, P, P, P]
         L.oracle_relres.restype = C.c_double
+        L.oracle_rhs.argtypes = [P, P, C.c_int64, P, P, P, P, P, C.c_double, C.c_double, P]
+        L.oracle_project.argtypes = [P, P, P, P, P, C.c_double, C.c_double]
         _lib = L
     return _lib
 
@@ -197,3 +199,33 @@ def compatibility(b):
     if len(b) == 0:
         return b
     return subtract_mean(subtract_mean(b))
+
+
+def field_sizes(grid: Grid):
+    """Lengths of the MAC arrays u, v, w, p (MAC.h:235-238; 2D :369-371): w is empty in 2D."""
+    Ny, Nx, Nz = grid.types.shape
+    if grid.dim == 2:
+        return Ny * (Nx + 1), (Ny + 1) * Nx, 0, Ny * Nx
+    return Ny * (Nx + 1) * Nz, (Ny + 1) * Nx * Nz, Ny * Nx * (Nz + 1), Ny * Nx * Nz
+
+
+def rhs(grid: Grid, system: System, u, v, w, p, rho, dt, p_old=None):
+    """b = rho/dt * div(u) * V - scale * DirichletSum in IDP order (oracle_rhs)."""
+    g = grid._c()
+    arrs = [np.ascontiguousarray(a, np.float64) if a is not None else None for a in (u, v, w, p, p_old)]
+    if arrs[2] is None:
+        arrs[2] = np.zeros(1)
+    b = np.zeros(max(1, system.n), np.float64)
+    lib().oracle_rhs(C.byref(g), _p(system.order), system.n, *[_p(a) for a in arrs], rho, dt, _p(b))
+    return b[:system.n]
+
+
+def project(grid: Grid, u, v, w, p, rho, dt):
+    """(u, v, w) after u -= dt/rho * grad p (oracle_project); returns copies."""
+    g = grid._c()
+    u = np.array(u, np.float64)
+    v = np.array(v, np.float64)
+    w = np.array(w if w is not None else np.zeros(1), np.float64)
+    p = np.ascontiguousarray(p, np.float64)
+    lib().oracle_project(C.byref(g), _p(u), _p(v), _p(w), _p(p), rho, dt)
+    return u, v, (w if grid.dim == 3 else None)

@@ -428,3 +428,177 @@ double oracle_relres(int32_t n, const int32_t *rowptr, const int32_t *col, const
     }
     return bb > 0.0 ? sqrt(rr / bb) : sqrt(rr);
 }
+
+/* ---- right-hand side and projection: PressureSolver3D.cpp:248-260, :288-298, :399-438;
+ *      PressureSolver2D.cpp:218-235, :262-271, :370-396; MAC.cpp:349-367, :443-455; MAC.h:149-155, :363-367 ---- */
+
+static int64_t at3(const oracle_grid *g, int i, int j, int k) { return ((int64_t)i * g->Nx + j) * g->Nz + k; }
+
+/* The known pressure at a Dirichlet neighbour: p_b, or p_b(t^{n+1}) - p_b(t^n) when pOld is given. */
+static double known_p(const double *p, const double *pOld, int64_t c) { return pOld ? p[c] - pOld[c] : p[c]; }
+
+void oracle_rhs(const oracle_grid *g, const int32_t *order, int64_t n, const double *u, const double *v,
+                const double *w, const double *p, const double *pOld, double rho, double dt, double *b)
+{
+    const int Nx = g->Nx, Ny = g->Ny, Nz = g->dim == 2 ? 1 : g->Nz;
+    const double scale = g->nonUniform ? 1.0 : 1.0 / (g->dh * g->dh);
+    const double rhoDt = rho / dt;
+    for (int64_t q = 0; q < n; q++)
+    {
+        const int64_t c = order[q];
+        const int k = (int)(c % Nz), j = (int)((c / Nz) % Nx), i = (int)(c / ((int64_t)Nz * Nx));
+        double div, vol, ds = 0.0;
+        if (g->dim == 3)
+        {
+            const double u0 = u[((int64_t)i * (Nx + 1) + j) * Nz + k], u1 = u[((int64_t)i * (Nx + 1) + j + 1) * Nz + k];
+            const double v0 = v[((int64_t)i * Nx + j) * Nz + k], v1 = v[((int64_t)(i + 1) * Nx + j) * Nz + k];
+            const double w0 = w[((int64_t)i * Nx + j) * (Nz + 1) + k], w1 = w[((int64_t)i * Nx + j) * (Nz + 1) + k + 1];
+            if (g->nonUniform)
+            {
+                /* MAC::GetDivergencyAt, stretched: net face flux over the cell volume (MAC.cpp:353-360) */
+                const double dx = g->wx[j], dy = g->wy[i], dz = g->wz[k];
+                const double fu = (u1 - u0) * dy * dz, fv = (v1 - v0) * dx * dz, fw = (w1 - w0) * dx * dy;
+                div = (fu + fv + fw) / (dx * dy * dz);
+                vol = g->wx[j] * g->wy[i] * g->wz[k];   /* MAC cellVolume (MAC.cpp:243) */
+            }
+            else
+            {
+                const double du = (u1 - u0) / g->dh, dv = (v1 - v0) / g->dh, dw = (w1 - w0) / g->dh;
+                div = du + dv + dw;
+                vol = 1.0;
+            }
+            /* DirichletSum (PressureSolver3D.cpp:248-260): PRESSURE neighbours only, FaceCoefs3D x RingFactor3D */
+            double cq[6];
+            if (!g->nonUniform)
+                for (int a = 0; a < 6; a++) cq[a] = 1.0;
+            else
+            {
+                const double ax = g->wy[i] * g->wz[k], ay = g->wx[j] * g->wz[k], az = g->wx[j] * g->wy[i];
+                cq[0] = ay / face_gap(g->wy, Ny, i);
+                cq[1] = ay / face_gap(g->wy, Ny, i - 1);
+                cq[2] = ax / face_gap(g->wx, Nx, j);
+                cq[3] = ax / face_gap(g->wx, Nx, j - 1);
+                cq[4] = az / face_gap(g->wz, Nz, k);
+                cq[5] = az / face_gap(g->wz, Nz, k - 1);
+            }
+            const int DI[6] = {1, -1, 0, 0, 0, 0}, DJ[6] = {0, 0, 1, -1, 0, 0}, DK[6] = {0, 0, 0, 0, 1, -1};
+            for (int a = 0; a < 6; a++)
+            {
+                const int ni = i + DI[a], nj = j + DJ[a], nk = k + DK[a];
+                const int64_t nc = at3(g, ni, nj, nk);
+                if (g->types[nc] != ORACLE_PRESSURE) continue;
+                double rf = 1.0;
+                if (ni == 0 || ni == Ny - 1 || nj == 0 || nj == Nx - 1 || nk == 0 || nk == Nz - 1)
+                {
+                    const int s = (a % 2 == 0) ? 1 : -1;
+                    rf = a < 2 ? ring_factor_axis(g->wy, Ny, i, s)
+                               : (a < 4 ? ring_factor_axis(g->wx, Nx, j, s) : ring_factor_axis(g->wz, Nz, k, s));
+                }
+                ds += cq[a] * rf * known_p(p, pOld, nc);
+            }
+        }
+        else
+        {
+            const double u0 = u[(int64_t)i * (Nx + 1) + j], u1 = u[(int64_t)i * (Nx + 1) + j + 1];
+            const double v0 = v[(int64_t)i * Nx + j], v1 = v[(int64_t)(i + 1) * Nx + j];
+            if (g->nonUniform)
+            {
+                const double dx = g->wx[j], dy = g->wy[i];
+                const double fu = (u1 - u0) * dy, fv = (v1 - v0) * dx;
+                div = (fu + fv) / (dx * dy);
+                vol = g->wx[j] * g->wy[i];   /* MAC.cpp:235 */
+            }
+            else
+            {
+                const double du = (u1 - u0) / g->dh, dv = (v1 - v0) / g->dh;
+                div = du + dv;
+                vol = 1.0;
+            }
+            /* DirichletSum 2D (PressureSolver2D.cpp:223-235): N, S, E, W */
+            const int NI[4] = {i + 1, i - 1, i, i}, NJ[4] = {j, j, j + 1, j - 1};
+            for (int a = 0; a < 4; a++)
+            {
+                const int64_t nc = (int64_t)NI[a] * Nx + NJ[a];
+                if (g->types[nc] != ORACLE_PRESSURE) continue;
+                const double cf = a < 2 ? g->wx[j] / face_gap(g->wy, Ny, a == 0 ? i : i - 1)
+                                        : g->wy[i] / face_gap(g->wx, Nx, a == 2 ? j : j - 1);
+                double rf = 1.0;
+                if (NI[a] == 0 || NI[a] == Ny - 1 || NJ[a] == 0 || NJ[a] == Nx - 1)
+                    rf = a < 2 ? ring_factor_axis(g->wy, Ny, i, a == 0 ? 1 : -1)
+                               : ring_factor_axis(g->wx, Nx, j, a == 2 ? 1 : -1);
+                ds += cf * rf * known_p(p, pOld, nc);
+            }
+        }
+        b[q] = rhoDt * div * vol - scale * ds;
+    }
+}
+
+/* ProjectionGradient3D / 2D: the face gradient over FaceGap, times DirichletRingFactor where the face's outer cell
+ * is a Dirichlet ring cell. n = the face's index along the axis, N = the axis length, tRing0 / tRingN the types of
+ * the ring cells at 0 and N-1 on the face's line. */
+static double proj_grad(double pc, double pm, const double *wa, int N, int n, int tRing0, int tRingN)
+{
+    const double grad = (pc - pm) / face_gap(wa, N, n - 1);
+    if (n == 1 && is_dirichlet(tRing0)) return grad * ring_factor_axis(wa, N, 1, -1);
+    if (n == N - 1 && is_dirichlet(tRingN)) return grad * ring_factor_axis(wa, N, N - 2, +1);
+    return grad;
+}
+
+static int wall(int a, int b) { return a == ORACLE_SOLID || b == ORACLE_SOLID; }
+
+void oracle_project(const oracle_grid *g, double *u, double *v, double *w, const double *p, double rho, double dt)
+{
+    const int Nx = g->Nx, Ny = g->Ny, Nz = g->dim == 2 ? 1 : g->Nz;
+    const double f = dt / rho;
+    const uint8_t *t = g->types;
+    if (g->dim == 3)
+    {
+        for (int i = 1; i < Ny - 1; i++)
+            for (int j = 1; j < Nx; j++)
+                for (int k = 1; k < Nz - 1; k++)
+                    if (!wall(t[at3(g, i, j, k)], t[at3(g, i, j - 1, k)]))
+                    {
+                        const double gr = proj_grad(p[at3(g, i, j, k)], p[at3(g, i, j - 1, k)], g->wx, Nx, j,
+                                                    t[at3(g, i, 0, k)], t[at3(g, i, Nx - 1, k)]);
+                        double *uf = &u[((int64_t)i * (Nx + 1) + j) * Nz + k];
+                        *uf = *uf - f * gr;
+                    }
+        for (int i = 1; i < Ny; i++)
+            for (int j = 1; j < Nx - 1; j++)
+                for (int k = 1; k < Nz - 1; k++)
+                    if (!wall(t[at3(g, i, j, k)], t[at3(g, i - 1, j, k)]))
+                    {
+                        const double gr = proj_grad(p[at3(g, i, j, k)], p[at3(g, i - 1, j, k)], g->wy, Ny, i,
+                                                    t[at3(g, 0, j, k)], t[at3(g, Ny - 1, j, k)]);
+                        double *vf = &v[((int64_t)i * Nx + j) * Nz + k];
+                        *vf = *vf - f * gr;
+                    }
+        for (int i = 1; i < Ny - 1; i++)
+            for (int j = 1; j < Nx - 1; j++)
+                for (int k = 1; k < Nz; k++)
+                    if (!wall(t[at3(g, i, j, k)], t[at3(g, i, j, k - 1)]))
+                    {
+                        const double gr = proj_grad(p[at3(g, i, j, k)], p[at3(g, i, j, k - 1)], g->wz, Nz, k,
+                                                    t[at3(g, i, j, 0)], t[at3(g, i, j, Nz - 1)]);
+                        double *wf = &w[((int64_t)i * Nx + j) * (Nz + 1) + k];
+                        *wf = *wf - f * gr;
+                    }
+        return;
+    }
+    for (int i = 1; i < Ny - 1; i++)
+        for (int j = 1; j < Nx; j++)
+            if (!wall(t[(int64_t)i * Nx + j], t[(int64_t)i * Nx + j - 1]))
+            {
+                const double gr = proj_grad(p[(int64_t)i * Nx + j], p[(int64_t)i * Nx + j - 1], g->wx, Nx, j,
+                                            t[(int64_t)i * Nx], t[(int64_t)i * Nx + Nx - 1]);
+                u[(int64_t)i * (Nx + 1) + j] = u[(int64_t)i * (Nx + 1) + j] - f * gr;
+            }
+    for (int i = 1; i < Ny; i++)
+        for (int j = 1; j < Nx - 1; j++)
+            if (!wall(t[(int64_t)i * Nx + j], t[(int64_t)(i - 1) * Nx + j]))
+            {
+                const double gr = proj_grad(p[(int64_t)i * Nx + j], p[(int64_t)(i - 1) * Nx + j], g->wy, Ny, i, t[j],
+                                            t[(int64_t)(Ny - 1) * Nx + j]);
+                v[(int64_t)i * Nx + j] = v[(int64_t)i * Nx + j] - f * gr;
+            }
+}

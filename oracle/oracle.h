@@ -12,6 +12,11 @@
  *   oracle_cg             - iteration counts and solutions against the reference's Eigen CG (golden fixtures);
  *                           exact solve on tiny systems against a dense factorisation.
  *   oracle_subtract_mean  - closed form (mean of the result is 0 to round-off).
+ *   oracle_rhs            - bit-exact against the compiled reference's GetDivergencyAt / DirichletSum expression
+ *                           (golden fixtures: 2D/3D, uniform/stretched, PRESSURE ring cells, incremental);
+ *                           plus the closed form of a linear velocity field and of a PRESSURE ring.
+ *   oracle_project        - bit-exact against the compiled reference's ProjectPressure (golden fixtures); plus the
+ *                           discrete divergence of the projected field vanishing after a solve.
  */
 #ifndef NPPE_ORACLE_H
 #define NPPE_ORACLE_H
@@ -71,6 +76,18 @@ int oracle_solve_pressure(int32_t n, const int32_t *rowptr, const int32_t *col, 
                           double *x, const double *guess, double tol, int maxIters, double referenceNorm);
 
 void oracle_subtract_mean(double *v, int64_t n);
+
+/* The pressure equation's right-hand side in IDP order (order[n] from oracle_assemble): b = rho/dt * div(u) * V -
+ * scale * DirichletSum, the expression of PressureSolver3D.cpp:288-298 / PressureSolver2D.cpp:262-271 with
+ * MAC::GetDivergencyAt, the cell volume and DirichletSum (PRESSURE neighbours; p_b - p_b^n when pOld is given).
+ * u, v, w, p, pOld are MAC arrays: u[(i*(Nx+1) + j)*Nz + k], v[(i*Nx + j)*Nz + k] (i <= Ny-1+1),
+ * w[(i*Nx + j)*(Nz+1) + k], p[(i*Nx + j)*Nz + k]; in 2D w is unused and Nz = 1. */
+void oracle_rhs(const oracle_grid *g, const int32_t *order, int64_t n, const double *u, const double *v,
+                const double *w, const double *p, const double *pOld, double rho, double dt, double *b);
+
+/* The projection u -= dt/rho * grad p on every face that is not a wall, with DirichletRingFactor on faces whose
+ * outer cell is a Dirichlet ring cell (PressureSolver3D.cpp:399-438; PressureSolver2D.cpp:370-396), in place. */
+void oracle_project(const oracle_grid *g, double *u, double *v, double *w, const double *p, double rho, double dt);
 
 /* |b - A x|_2 / |b|_2 in FP64. */
 double oracle_relres(int32_t n, const int32_t *rowptr, const int32_t *col, const double *val, const double *b,

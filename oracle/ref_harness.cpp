@@ -10,7 +10,10 @@
 // PressureSolver{2D,3D}::InitializePressureSolver (PressureSolver3D.cpp:89-102, :128-178) and solves with the
 // reference's SolvePressureSystem (PressureSettings.h:578-624) for each requested eigen_solver / preconditioner /
 // tolerance. It writes the spacing tables, the unknown numbering, the CSR matrix, and each solution with its
-// iteration count and wall times.
+// iteration count and wall times. Given velocity and pressure fields it also writes the pressure equation's
+// right-hand side built from them with the reference's MAC::GetDivergencyAt and DirichletSum
+// (PressureSolver3D.cpp:288-298 / PressureSolver2D.cpp:262-271), IncrementReferenceNorm when a previous
+// pressure is given, and the fields after the reference's ProjectPressure.
 
 #include <omp.h>
 #include <algorithm>
@@ -201,6 +204,32 @@ int main(int argc, char **argv)
     }
     Eigen::VectorXd b(n);
     if (nsolves > 0) r.get(b.data(), (size_t)n);
+    // optional fields section
+    const int hasFields = r.get<int32_t>();
+    double rho = 1.0, dt = 1e-3;
+    int hasOld = 0;
+    const size_t nu = (size_t)Ny * (Nx + 1) * Nz, nv = (size_t)(Ny + 1) * Nx * Nz,
+                 nw = dim == 3 ? (size_t)Ny * Nx * (Nz + 1) : 0;
+    std::vector<double> fu, fv, fw, fp, fpOld;
+    if (hasFields)
+    {
+        rho = r.get<double>();
+        dt = r.get<double>();
+        hasOld = r.get<int32_t>();
+        fu.resize(nu);
+        fv.resize(nv);
+        fw.resize(nw);
+        fp.resize(cells);
+        r.get(fu.data(), nu);
+        r.get(fv.data(), nv);
+        if (dim == 3) r.get(fw.data(), nw);
+        r.get(fp.data(), cells);
+        if (hasOld)
+        {
+            fpOld.resize(cells);
+            r.get(fpOld.data(), cells);
+        }
+    }
     std::fclose(fi);
 
     w.put<int32_t>(nsolves);
@@ -225,6 +254,78 @@ int main(int argc, char **argv)
         w.put(x.data(), (size_t)n);
     }
     w.put<int32_t>(omp_get_max_threads());
+
+    if (hasFields)
+    {
+        SIMULATION.RHO = rho;
+        SIMULATION.dt = dt;
+        // load the MAC fields through the public setters (MAC.h:235-246, :369-377)
+        MAC OLD = GRID;
+        for (int i = 0; i < Ny; i++)
+            for (int j = 0; j <= Nx; j++)
+                for (int k = 0; k < Nz; k++)
+                    if (dim == 2) GRID.SetU(i, j, fu[(size_t)i * (Nx + 1) + j]);
+                    else GRID.SetU(i, j, k, fu[((size_t)i * (Nx + 1) + j) * Nz + k]);
+        for (int i = 0; i <= Ny; i++)
+            for (int j = 0; j < Nx; j++)
+                for (int k = 0; k < Nz; k++)
+                    if (dim == 2) GRID.SetV(i, j, fv[(size_t)i * Nx + j]);
+                    else GRID.SetV(i, j, k, fv[((size_t)i * Nx + j) * Nz + k]);
+        if (dim == 3)
+            for (int i = 0; i < Ny; i++)
+                for (int j = 0; j < Nx; j++)
+                    for (int k = 0; k <= Nz; k++) GRID.SetW(i, j, k, fw[((size_t)i * Nx + j) * (Nz + 1) + k]);
+        for (int i = 0; i < Ny; i++)
+            for (int j = 0; j < Nx; j++)
+                for (int k = 0; k < Nz; k++)
+                {
+                    const size_t c = ((size_t)i * Nx + j) * Nz + k;
+                    if (dim == 2)
+                    {
+                        GRID.SetP(i, j, fp[c]);
+                        if (hasOld) OLD.SetP(i, j, fpOld[c]);
+                    }
+                    else
+                    {
+                        GRID.SetP(i, j, k, fp[c]);
+                        if (hasOld) OLD.SetP(i, j, k, fpOld[c]);
+                    }
+                }
+        // the right-hand side: the expression of SolveScheme with the reference's own GetDivergencyAt and
+        // DirichletSum; MatrixScale and the cell volume are the solvers' one-line helpers
+        const double scale = GRID.nonUniform ? 1.0 : 1.0 / (GRID.dh * GRID.dh);
+        const double rhoDt = SIMULATION.RHO / SIMULATION.dt;
+        Eigen::VectorXd rb(n);
+        for (int q = 0; q < n; q++)
+        {
+            const int c = order[q];
+            const int k = c % Nz, j = (c / Nz) % Nx, i = c / (Nz * Nx);
+            if (dim == 2)
+                rb(q) = rhoDt * GRID.GetDivergencyAt(i, j) * (GRID.nonUniform ? GRID.CellVolume(i, j) : 1.0) -
+                        scale * PressureSolver2D::DirichletSum(&GRID, hasOld ? &OLD : nullptr, i, j);
+            else
+                rb(q) = rhoDt * GRID.GetDivergencyAt(i, j, k) * (GRID.nonUniform ? GRID.CellVolume(i, j, k) : 1.0) -
+                        scale * PressureSolver3D::DirichletSum(&GRID, hasOld ? &OLD : nullptr, i, j, k);
+        }
+        w.put(rb.data(), (size_t)n);
+        double ref = 0.0;
+        if (hasOld)
+            ref = dim == 2 ? PressureSolver2D::IncrementReferenceNorm(&OLD, rb)
+                           : PressureSolver3D::IncrementReferenceNorm(&OLD, rb);
+        w.put<double>(ref);
+        if (dim == 2) PressureSolver2D::ProjectPressure(&GRID);
+        else PressureSolver3D::ProjectPressure(&GRID);
+        for (int i = 0; i < Ny; i++)
+            for (int j = 0; j <= Nx; j++)
+                for (int k = 0; k < Nz; k++) w.put<double>(dim == 2 ? GRID.GetU(i, j) : GRID.GetU(i, j, k));
+        for (int i = 0; i <= Ny; i++)
+            for (int j = 0; j < Nx; j++)
+                for (int k = 0; k < Nz; k++) w.put<double>(dim == 2 ? GRID.GetV(i, j) : GRID.GetV(i, j, k));
+        if (dim == 3)
+            for (int i = 0; i < Ny; i++)
+                for (int j = 0; j < Nx; j++)
+                    for (int k = 0; k <= Nz; k++) w.put<double>(GRID.GetW(i, j, k));
+    }
     std::fclose(fo);
     return 0;
 }

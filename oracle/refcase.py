@@ -47,6 +47,10 @@ class RefCase:
     compat: bool = False
     solves: List[SolveSpec] = field(default_factory=list)
     rhs: Optional[np.ndarray] = None
+    # velocity and pressure fields in MAC layout: {"rho", "dt", "u", "v", "w", "p", "p_old" (optional)}; the
+    # harness then returns the right-hand side built from them, the increment reference norm (with p_old) and the
+    # projected fields
+    fields: Optional[dict] = None
 
     def encode(self) -> bytes:
         out = [b"NPPECASE", struct.pack("<ii", self.dim, self.grid_size), struct.pack("<6d", *self.domain),
@@ -68,6 +72,14 @@ class RefCase:
             out.append(struct.pack("<i", s.repeats))
         if self.solves:
             out.append(np.ascontiguousarray(self.rhs, np.float64).tobytes())
+        f = self.fields
+        out.append(struct.pack("<i", 1 if f else 0))
+        if f:
+            out.append(struct.pack("<ddi", f["rho"], f["dt"], 1 if f.get("p_old") is not None else 0))
+            for key in ("u", "v") + (("w",) if self.dim == 3 else ()) + ("p",):
+                out.append(np.ascontiguousarray(f[key], np.float64).tobytes())
+            if f.get("p_old") is not None:
+                out.append(np.ascontiguousarray(f["p_old"], np.float64).tobytes())
         return b"".join(out)
 
 
@@ -128,4 +140,12 @@ def run(case: RefCase, timeout: float = 3600.0, threads: Optional[int] = None) -
         x = b.arr(np.float64, n)
         res["solves"].append({"iters": iters, "times": times, "x": x})
     res["threads"] = b.take("<i")[0]
+    if case.fields:
+        Nzz = Nz if case.dim == 3 else 1
+        res["rhs_built"] = b.arr(np.float64, n)
+        res["ref_norm"] = b.take("<d")[0]
+        res["u"] = b.arr(np.float64, Ny * (Nx + 1) * Nzz)
+        res["v"] = b.arr(np.float64, (Ny + 1) * Nx * Nzz)
+        if case.dim == 3:
+            res["w"] = b.arr(np.float64, Ny * Nx * (Nz + 1))
     return res

@@ -7,6 +7,11 @@ intervals, cell types, right-hand side) and what the reference produced for it: 
 CommitPressureMatrix), the singular flag, and solutions with iteration counts from SolvePressureSystem with
 eigen_solver=CG/DIAGONAL and eigen_solver=AMG (AMGCL).
 
+The ``fields_*.npz`` fixtures add seeded velocity and pressure fields (MAC layout) for some of those cases and
+what the reference computes from them: the right-hand side with MAC::GetDivergencyAt and DirichletSum
+(PressureSolver{2D,3D}::SolveScheme's expression), IncrementReferenceNorm with a previous pressure, and the
+fields after ProjectPressure.
+
     python tests/golden/make_fixtures.py
 """
 from __future__ import annotations
@@ -131,5 +136,35 @@ def main():
               f"cg_iters={out['solves'][0]['iters']} amg_iters={out['solves'][1]['iters']}")
 
 
+FIELD_CASES = ["u3_mixed_10", "u3_pressure_7", "s3_hard_mixed", "s3_graded_outlet", "u2_mixed_20",
+               "u2_pressure_12", "s2_ramp_outlet"]
+
+
+def main_fields():
+    for q, name in enumerate(FIELD_CASES):
+        (_, dim, gs, dom, nu, refs, kind, compat) = next(c for c in CASES if c[0] == name)
+        base = np.load(os.path.join(HERE, name + ".npz"))
+        types = base["types"]
+        grid = O.Grid(dim, types, base["wx"], base["wy"], base["wz"] if dim == 3 else np.array([float(base["dh"])]),
+                      float(base["dh"]), bool(nu))
+        nu_, nv_, nw_, np_ = O.field_sizes(grid)
+        rng = np.random.default_rng(2000 + q)
+        f = {"rho": float(rng.uniform(0.5, 2.0)), "dt": float(rng.uniform(1e-4, 1e-2)),
+             "u": rng.standard_normal(nu_), "v": rng.standard_normal(nv_),
+             "w": rng.standard_normal(nw_) if dim == 3 else None, "p": rng.standard_normal(np_),
+             "p_old": rng.standard_normal(np_) if q % 2 == 0 else None}
+        out = R.run(R.RefCase(dim=dim, grid_size=gs, domain=dom, non_uniform=nu, refine=refs, types=types,
+                              compat=False, fields=f), threads=1)
+        np.savez_compressed(
+            os.path.join(HERE, "fields_" + name + ".npz"),
+            case=name, rho=f["rho"], dt=f["dt"], u=f["u"], v=f["v"], w=f["w"] if dim == 3 else np.zeros(0),
+            p=f["p"], p_old=f["p_old"] if f["p_old"] is not None else np.zeros(0),
+            rhs=out["rhs_built"], ref_norm=out["ref_norm"], u_proj=out["u"], v_proj=out["v"],
+            w_proj=out["w"] if dim == 3 else np.zeros(0))
+        print(f"fields_{name}: n={out['n']} incremental={f['p_old'] is not None} ref_norm={out['ref_norm']:.6g}")
+
+
 if __name__ == "__main__":
-    main()
+    if "--fields" not in sys.argv:
+        main()
+    main_fields()
