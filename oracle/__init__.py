@@ -69,6 +69,9 @@ def lib():
         L.oracle_relres.restype = C.c_double
         L.oracle_rhs.argtypes = [P, P, C.c_int64, P, P, P, P, P, C.c_double, C.c_double, P]
         L.oracle_project.argtypes = [P, P, P, P, P, C.c_double, C.c_double]
+        L.oracle_enforce_pressure_bc.argtypes = [P, P, P]
+        L.oracle_scatter_pressure.argtypes = [P, P, C.c_int64, P, P, P, C.c_int, P]
+        L.oracle_accumulate_increment.argtypes = [P, P, C.c_int64, P, P, C.c_int, P]
         _lib = L
     return _lib
 
@@ -229,3 +232,38 @@ def project(grid: Grid, u, v, w, p, rho, dt):
     p = np.ascontiguousarray(p, np.float64)
     lib().oracle_project(C.byref(g), _p(u), _p(v), _p(w), _p(p), rho, dt)
     return u, v, (w if grid.dim == 3 else None)
+
+
+def _opt64(a):
+    return None if a is None else np.ascontiguousarray(a, np.float64)
+
+
+def enforce_pressure_bc(grid: Grid, p, prescribed=None):
+    """p (a copy) after MAC::EnforcePressureBC (oracle_enforce_pressure_bc); prescribed: box-sized values for the
+    PRESSURE ring cells, or None to keep theirs."""
+    g = grid._c()
+    p = np.array(p, np.float64)
+    pr = _opt64(prescribed)
+    lib().oracle_enforce_pressure_bc(C.byref(g), _p(p), _p(pr))
+    return p
+
+
+def scatter_pressure(grid: Grid, system: System, x, p, p_old=None, level=False, prescribed=None):
+    """p (a copy) after SolveScheme's tail with x as the solve's answer (oracle_scatter_pressure)."""
+    g = grid._c()
+    p = np.array(p, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    po, pr = _opt64(p_old), _opt64(prescribed)
+    lib().oracle_scatter_pressure(C.byref(g), _p(system.order), system.n, _p(x), _p(p), _p(po), int(bool(level)),
+                                  _p(pr))
+    return p
+
+
+def accumulate_increment(grid: Grid, system: System, p_old, p, level=False, prescribed=None):
+    """p (a copy) after AccumulatePressureIncrement (oracle_accumulate_increment)."""
+    g = grid._c()
+    p = np.array(p, np.float64)
+    po, pr = np.ascontiguousarray(p_old, np.float64), _opt64(prescribed)
+    lib().oracle_accumulate_increment(C.byref(g), _p(system.order), system.n, _p(po), _p(p), int(bool(level)),
+                                      _p(pr))
+    return p

@@ -602,3 +602,89 @@ void oracle_project(const oracle_grid *g, double *u, double *v, double *w, const
                 v[(int64_t)i * Nx + j] = v[(int64_t)i * Nx + j] - f * gr;
             }
 }
+
+/* ---- pressure boundary conditions, scatter and increment: MAC.cpp:866-871, :1064-1096; MAC.h:482-486;
+ *      PressureSolver3D.cpp:181-186, :316-336, :374-393; PressureSolver2D.cpp:156-160, :287-304, :340-357 ------- */
+
+static void ring_entry(const oracle_grid *g, double *p, const double *prescribed, int64_t dst, int64_t src)
+{
+    const int t = g->types[dst];
+    if (t == ORACLE_EMPTY) p[dst] = 0.0;
+    else if (t == ORACLE_PRESSURE) { if (prescribed) p[dst] = prescribed[dst]; }
+    else p[dst] = p[src];
+}
+
+void oracle_enforce_pressure_bc(const oracle_grid *g, double *p, const double *prescribed)
+{
+    const int Nx = g->Nx, Ny = g->Ny, Nz = g->dim == 2 ? 1 : g->Nz;
+    if (g->dim == 2)
+    {
+        for (int j = 0; j < Nx; j++)
+        {
+            ring_entry(g, p, prescribed, j, (int64_t)Nx + j);
+            ring_entry(g, p, prescribed, (int64_t)(Ny - 1) * Nx + j, (int64_t)(Ny - 2) * Nx + j);
+        }
+        for (int i = 0; i < Ny; i++)
+        {
+            ring_entry(g, p, prescribed, (int64_t)i * Nx, (int64_t)i * Nx + 1);
+            ring_entry(g, p, prescribed, (int64_t)i * Nx + Nx - 1, (int64_t)i * Nx + Nx - 2);
+        }
+        return;
+    }
+    for (int j = 0; j < Nx; j++)
+        for (int k = 0; k < Nz; k++)
+        {
+            ring_entry(g, p, prescribed, at3(g, 0, j, k), at3(g, 1, j, k));
+            ring_entry(g, p, prescribed, at3(g, Ny - 1, j, k), at3(g, Ny - 2, j, k));
+        }
+    for (int i = 0; i < Ny; i++)
+        for (int k = 0; k < Nz; k++)
+        {
+            ring_entry(g, p, prescribed, at3(g, i, 0, k), at3(g, i, 1, k));
+            ring_entry(g, p, prescribed, at3(g, i, Nx - 1, k), at3(g, i, Nx - 2, k));
+        }
+    for (int i = 0; i < Ny; i++)
+        for (int j = 0; j < Nx; j++)
+        {
+            ring_entry(g, p, prescribed, at3(g, i, j, 0), at3(g, i, j, 1));
+            ring_entry(g, p, prescribed, at3(g, i, j, Nz - 1), at3(g, i, j, Nz - 2));
+        }
+}
+
+/* LevelPressure without a scene (PressureSolver3D.cpp:181-186): v -= mean(v). */
+static void remove_mean(double *v, int64_t n)
+{
+    if (n <= 0) return;
+    double s = 0.0;
+    for (int64_t q = 0; q < n; q++) s += v[q];
+    const double mean = s / (double)n;
+    for (int64_t q = 0; q < n; q++) v[q] -= mean;
+}
+
+void oracle_scatter_pressure(const oracle_grid *g, const int32_t *order, int64_t n, const double *x, double *p,
+                             const double *pOld, int level, const double *prescribed)
+{
+    double *xl = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    for (int64_t q = 0; q < n; q++) xl[q] = x[q];
+    if (!pOld && level) remove_mean(xl, n);
+    for (int64_t q = 0; q < n; q++) p[order[q]] = xl[q];
+    free(xl);
+    oracle_enforce_pressure_bc(g, p, prescribed);
+    if (pOld)
+    {
+        const int64_t cells = (int64_t)g->Nx * g->Ny * (g->dim == 2 ? 1 : g->Nz);
+        for (int64_t c = 0; c < cells; c++)
+            if (g->types[c] == ORACLE_PRESSURE) p[c] = p[c] - pOld[c];
+    }
+}
+
+void oracle_accumulate_increment(const oracle_grid *g, const int32_t *order, int64_t n, const double *pOld,
+                                 double *p, int level, const double *prescribed)
+{
+    double *v = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    for (int64_t q = 0; q < n; q++) v[q] = pOld[order[q]] + p[order[q]];
+    if (level) remove_mean(v, n);
+    for (int64_t q = 0; q < n; q++) p[order[q]] = v[q];
+    free(v);
+    oracle_enforce_pressure_bc(g, p, prescribed);
+}

@@ -17,6 +17,12 @@
  *                           plus the closed form of a linear velocity field and of a PRESSURE ring.
  *   oracle_project        - bit-exact against the compiled reference's ProjectPressure (golden fixtures); plus the
  *                           discrete divergence of the projected field vanishing after a solve.
+ *   oracle_enforce_pressure_bc - bit-exact against the compiled reference's MAC::EnforcePressureBC with a known
+ *                           boundary pressure (golden fixtures: corner and edge cells of the sequential ring list).
+ *   oracle_scatter_pressure - bit-exact against the compiled reference's SolveScheme tail (scatter, ring BC and
+ *                           incremental ring fix) where no mean is removed; within 1e-14 of it where LevelPressure
+ *                           removes the mean (Eigen's vectorised mean sums in another order).
+ *   oracle_accumulate_increment - the same bars against the compiled reference's AccumulatePressureIncrement.
  */
 #ifndef NPPE_ORACLE_H
 #define NPPE_ORACLE_H
@@ -88,6 +94,24 @@ void oracle_rhs(const oracle_grid *g, const int32_t *order, int64_t n, const dou
 /* The projection u -= dt/rho * grad p on every face that is not a wall, with DirichletRingFactor on faces whose
  * outer cell is a Dirichlet ring cell (PressureSolver3D.cpp:399-438; PressureSolver2D.cpp:370-396), in place. */
 void oracle_project(const oracle_grid *g, double *u, double *v, double *w, const double *p, double rho, double dt);
+
+/* MAC::EnforcePressureBC (MAC.cpp:866-871) in the order of MAC::BuildRingList (MAC.cpp:1064-1096; RecordRing,
+ * MAC.h:482-486): every ring cell on the y faces, then the x faces, then (3D) the z faces; an EMPTY cell gets 0, a
+ * PRESSURE cell prescribed[c] (left as it is when prescribed is NULL), any other the inward neighbour's current
+ * value. p is the MAC pressure array. */
+void oracle_enforce_pressure_bc(const oracle_grid *g, double *p, const double *prescribed);
+
+/* SolveScheme's tail after the solve (PressureSolver3D.cpp:316-336; PressureSolver2D.cpp:287-304) with x (IDP
+ * order) as the solve's answer: when pOld is NULL (Chorin) and `level` is set (NeedsCompatibility), x's mean is
+ * removed (LevelPressure without a scene); x is scattered into p, the ring BC applied, and with pOld (incremental)
+ * every PRESSURE cell then has pOld subtracted. */
+void oracle_scatter_pressure(const oracle_grid *g, const int32_t *order, int64_t n, const double *x, double *p,
+                             const double *pOld, int level, const double *prescribed);
+
+/* AccumulatePressureIncrement (PressureSolver3D.cpp:374-393; PressureSolver2D.cpp:340-357): p = pOld + phi at the
+ * unknowns (phi: p on entry), the mean removed when `level` is set, then the ring BC. */
+void oracle_accumulate_increment(const oracle_grid *g, const int32_t *order, int64_t n, const double *pOld,
+                                 double *p, int level, const double *prescribed);
 
 /* |b - A x|_2 / |b|_2 in FP64. */
 double oracle_relres(int32_t n, const int32_t *rowptr, const int32_t *col, const double *val, const double *b,

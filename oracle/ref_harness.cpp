@@ -13,7 +13,10 @@
 // iteration count and wall times. Given velocity and pressure fields it also writes the pressure equation's
 // right-hand side built from them with the reference's MAC::GetDivergencyAt and DirichletSum
 // (PressureSolver3D.cpp:288-298 / PressureSolver2D.cpp:262-271), IncrementReferenceNorm when a previous
-// pressure is given, and the fields after the reference's ProjectPressure.
+// pressure is given, and the fields after the reference's ProjectPressure. It then runs MAC::EnforcePressureBC
+// with a known boundary pressure, the tail of SolveScheme after the solve (PressureSolver3D.cpp:316-336,
+// PressureSolver2D.cpp:287-304) on a given vector standing for the solution, and, with a previous pressure,
+// AccumulatePressureIncrement (PressureSolver3D.cpp:374-393 / PressureSolver2D.cpp:340-357), writing p after each.
 
 #include <omp.h>
 #include <algorithm>
@@ -67,6 +70,10 @@ struct SolveSpec
 };
 
 MAC GRID;
+
+// A boundary pressure with a value on every ring cell, for MAC::PrescribedPressure (MAC.cpp:603-611).
+double BoundaryP3(double x, double y, double z, double t) { return 0.25 + x - 2.0 * y + 0.5 * z + 3.0 * t; }
+double BoundaryP2(double x, double y, double t) { return 0.25 + x - 2.0 * y + 3.0 * t; }
 
 void WriteAxis(Writer &w, const MAC &g, MAC::Axis a, int N)
 {
@@ -210,7 +217,8 @@ int main(int argc, char **argv)
     int hasOld = 0;
     const size_t nu = (size_t)Ny * (Nx + 1) * Nz, nv = (size_t)(Ny + 1) * Nx * Nz,
                  nw = dim == 3 ? (size_t)Ny * Nx * (Nz + 1) : 0;
-    std::vector<double> fu, fv, fw, fp, fpOld;
+    std::vector<double> fu, fv, fw, fp, fpOld, xTail;
+    double tBC = 0.0;
     if (hasFields)
     {
         rho = r.get<double>();
@@ -229,6 +237,9 @@ int main(int argc, char **argv)
             fpOld.resize(cells);
             r.get(fpOld.data(), cells);
         }
+        tBC = r.get<double>();
+        xTail.resize((size_t)n);
+        r.get(xTail.data(), (size_t)n);
     }
     std::fclose(fi);
 
@@ -325,6 +336,49 @@ int main(int argc, char **argv)
             for (int i = 0; i < Ny; i++)
                 for (int j = 0; j < Nx; j++)
                     for (int k = 0; k <= Nz; k++) w.put<double>(GRID.GetW(i, j, k));
+
+        auto putP = [&](MAC &g) {
+            for (int i = 0; i < Ny; i++)
+                for (int j = 0; j < Nx; j++)
+                    for (int k = 0; k < Nz; k++) w.put<double>(dim == 2 ? g.GetP(i, j) : g.GetP(i, j, k));
+        };
+        SIMULATION.time = tBC;
+        SIMULATION.PressureBoundaryFunction = BoundaryP3;
+        SIMULATION.PressureBoundaryFunction2D = BoundaryP2;
+        GRID.EnforcePressureBC(tBC);
+        putP(GRID);
+        // SolveScheme's tail with xTail as the solve's answer: level (Chorin), scatter, ring BC, ring fix
+        // (incremental)
+        Eigen::VectorXd x = Eigen::Map<Eigen::VectorXd>(xTail.data(), n);
+        if (!hasOld)
+        {
+            if (dim == 2) PressureSolver2D::LevelPressure(&GRID, x);
+            else PressureSolver3D::LevelPressure(&GRID, x);
+        }
+        for (int q = 0; q < n; q++)
+        {
+            const int c = order[q];
+            const int k = c % Nz, j = (c / Nz) % Nx, i = c / (Nz * Nx);
+            if (dim == 2) GRID.SetP(i, j, x(q));
+            else GRID.SetP(i, j, k, x(q));
+        }
+        GRID.EnforcePressureBC(tBC);
+        if (hasOld)
+            for (int i = 0; i < Ny; i++)
+                for (int j = 0; j < Nx; j++)
+                    for (int k = 0; k < Nz; k++)
+                        if (dim == 2 ? GRID.GetSolid(i, j) == PRESSURE_CELL : GRID.GetSolid(i, j, k) == PRESSURE_CELL)
+                        {
+                            if (dim == 2) GRID.SetP(i, j, GRID.GetP(i, j) - OLD.GetP(i, j));
+                            else GRID.SetP(i, j, k, GRID.GetP(i, j, k) - OLD.GetP(i, j, k));
+                        }
+        putP(GRID);
+        if (hasOld)
+        {
+            if (dim == 2) PressureSolver2D::AccumulatePressureIncrement(&OLD, &GRID);
+            else PressureSolver3D::AccumulatePressureIncrement(&OLD, &GRID);
+            putP(GRID);
+        }
     }
     std::fclose(fo);
     return 0;

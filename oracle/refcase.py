@@ -49,7 +49,8 @@ class RefCase:
     rhs: Optional[np.ndarray] = None
     # velocity and pressure fields in MAC layout: {"rho", "dt", "u", "v", "w", "p", "p_old" (optional)}; the
     # harness then returns the right-hand side built from them, the increment reference norm (with p_old) and the
-    # projected fields
+    # projected fields; with "t" (time) and "x_tail" (n values standing for a solve's answer) also p after
+    # EnforcePressureBC, after SolveScheme's tail and (with p_old) after AccumulatePressureIncrement
     fields: Optional[dict] = None
 
     def encode(self) -> bytes:
@@ -80,6 +81,8 @@ class RefCase:
                 out.append(np.ascontiguousarray(f[key], np.float64).tobytes())
             if f.get("p_old") is not None:
                 out.append(np.ascontiguousarray(f["p_old"], np.float64).tobytes())
+            out.append(struct.pack("<d", f.get("t", 0.0)))
+            out.append(np.ascontiguousarray(f["x_tail"], np.float64).tobytes())
         return b"".join(out)
 
 
@@ -148,4 +151,9 @@ def run(case: RefCase, timeout: float = 3600.0, threads: Optional[int] = None) -
         res["v"] = b.arr(np.float64, (Ny + 1) * Nx * Nzz)
         if case.dim == 3:
             res["w"] = b.arr(np.float64, Ny * Nx * (Nz + 1))
+        cells = Ny * Nx * Nzz
+        res["p_bc"] = b.arr(np.float64, cells)
+        res["p_tail"] = b.arr(np.float64, cells)
+        if case.fields.get("p_old") is not None:
+            res["p_acc"] = b.arr(np.float64, cells)
     return res

@@ -10,7 +10,9 @@ eigen_solver=CG/DIAGONAL and eigen_solver=AMG (AMGCL).
 The ``fields_*.npz`` fixtures add seeded velocity and pressure fields (MAC layout) for some of those cases and
 what the reference computes from them: the right-hand side with MAC::GetDivergencyAt and DirichletSum
 (PressureSolver{2D,3D}::SolveScheme's expression), IncrementReferenceNorm with a previous pressure, and the
-fields after ProjectPressure.
+fields after ProjectPressure; then p after MAC::EnforcePressureBC with a known boundary pressure, after
+SolveScheme's tail on a given vector standing for the solution, and (incremental) after
+AccumulatePressureIncrement.
 
     python tests/golden/make_fixtures.py
 """
@@ -137,7 +139,8 @@ def main():
 
 
 FIELD_CASES = ["u3_mixed_10", "u3_pressure_7", "s3_hard_mixed", "s3_graded_outlet", "u2_mixed_20",
-               "u2_pressure_12", "s2_ramp_outlet"]
+               "u2_pressure_12", "s2_ramp_outlet", "u3_neumann_8", "s3_equal_neumann", "u2_neumann_16",
+               "u3_freesurface_10"]
 
 
 def main_fields():
@@ -152,15 +155,17 @@ def main_fields():
         f = {"rho": float(rng.uniform(0.5, 2.0)), "dt": float(rng.uniform(1e-4, 1e-2)),
              "u": rng.standard_normal(nu_), "v": rng.standard_normal(nv_),
              "w": rng.standard_normal(nw_) if dim == 3 else None, "p": rng.standard_normal(np_),
-             "p_old": rng.standard_normal(np_) if q % 2 == 0 else None}
+             "p_old": rng.standard_normal(np_) if q % 2 == 0 else None, "t": float(rng.uniform(0.0, 2.0))}
+        f["x_tail"] = rng.standard_normal(int(O.assemble(grid).n)) + 0.7
         out = R.run(R.RefCase(dim=dim, grid_size=gs, domain=dom, non_uniform=nu, refine=refs, types=types,
-                              compat=False, fields=f), threads=1)
+                              compat=True, fields=f), threads=1)
         np.savez_compressed(
             os.path.join(HERE, "fields_" + name + ".npz"),
             case=name, rho=f["rho"], dt=f["dt"], u=f["u"], v=f["v"], w=f["w"] if dim == 3 else np.zeros(0),
             p=f["p"], p_old=f["p_old"] if f["p_old"] is not None else np.zeros(0),
             rhs=out["rhs_built"], ref_norm=out["ref_norm"], u_proj=out["u"], v_proj=out["v"],
-            w_proj=out["w"] if dim == 3 else np.zeros(0))
+            w_proj=out["w"] if dim == 3 else np.zeros(0), t=f["t"], x_tail=f["x_tail"], p_bc=out["p_bc"],
+            p_tail=out["p_tail"], p_acc=out["p_acc"] if f["p_old"] is not None else np.zeros(0))
         print(f"fields_{name}: n={out['n']} incremental={f['p_old'] is not None} ref_norm={out['ref_norm']:.6g}")
 
 
