@@ -185,6 +185,75 @@ uint64_t nppe_estimate_bytes(const nppe_grid *g, int32_t method);
 /* The same for a handle created with *opt (null: the defaults). */
 uint64_t nppe_estimate_bytes_for(const nppe_grid *g, const nppe_options *opt);
 
+/* ---- the pressure step around the solve, on MAC fields ---------------------------------------------------------
+ * The reference's pressure and projection steps (PressureSolver3D::SolveScheme, Project; PressureSolver2D the same)
+ * build the right-hand side from the intermediate velocity, solve, scatter the solution into p, apply the pressure
+ * boundary conditions and project; these entry points do each part on the device, against the committed grid.
+ * Fields are the reference's MAC arrays (MAC.h:235-254), FP64, unpitched, in device memory for the _device calls
+ * and host memory for the others:
+ *   u: ny*(nx+1)*nz values, u[(i*(nx+1) + j)*nz + k]     v: (ny+1)*nx*nz values, v[(i*nx + j)*nz + k]
+ *   w: ny*nx*(nz+1) values, w[(i*nx + j)*(nz+1) + k]     p, p_old, prescribed: ny*nx*nz values (the cell index)
+ * (2D: nz = 1 and w is not used). Vectors over the unknowns (b, x, f) have n entries in IDP order. p_old is the
+ * previous step's pressure (gridAnt's p) under the incremental scheme and NULL under Chorin. prescribed, when not
+ * NULL, holds MAC::PrescribedPressure at the PRESSURE ring cells (the caller evaluates the scene's boundary
+ * pressure there; other entries are not read); when NULL a PRESSURE ring cell keeps the value p holds, which is
+ * p_b(t) once the caller has run its EnforcePressureBC (SolveScheme does so first, PressureSolver3D.cpp:281).
+ * Device pointers must be ready when the call is made; every call returns when its work on the handle's stream is
+ * complete. */
+
+/* The right-hand side of SolveScheme (PressureSolver3D.cpp:288-307; PressureSolver2D.cpp:262-280):
+ * b = rho/dt * div(u) * V - scale * DirichletSum(p, p_old) (MAC::GetDivergencyAt, DirichletSum :248-260), then the
+ * mean removed twice when the system is singular and the compatibility option is on. Bit-identical to the
+ * reference's b before the mean removal. */
+int nppe_rhs_device(nppe_solver *h, const double *d_u, const double *d_v, const double *d_w, const double *d_p,
+                    const double *d_p_old, double rho, double dt, double *d_b);
+int nppe_rhs(nppe_solver *h, const double *u, const double *v, const double *w, const double *p, const double *p_old,
+             double rho, double dt, double *b);
+
+/* ProjectPressure (PressureSolver3D.cpp:403-438; PressureSolver2D.cpp:370-396): u -= dt/rho * grad p on every face
+ * that is not a wall, the gradient times DirichletRingFactor across a Dirichlet ring face; u, v, w in place. */
+int nppe_project_device(nppe_solver *h, double *d_u, double *d_v, double *d_w, const double *d_p, double rho,
+                        double dt);
+int nppe_project(nppe_solver *h, double *u, double *v, double *w, const double *p, double rho, double dt);
+
+/* MAC::EnforcePressureBC (MAC.cpp:866-871) over the ring list in MAC::BuildRingList's order (:1064-1096): EMPTY
+ * ring cells 0, PRESSURE ring cells the prescribed value, every other ring cell its inward neighbour's value. */
+int nppe_enforce_pressure_bc_device(nppe_solver *h, double *d_p, const double *d_prescribed);
+
+/* IncrementReferenceNorm (PressureSolver3D.cpp:262-276; PressureSolver2D.cpp:237-251): ||b + A p^n||_2 with p^n
+ * = p_old at the unknowns, the reference norm of an incremental solve. */
+int nppe_increment_reference_norm_device(nppe_solver *h, const double *d_b, const double *d_p_old, double *norm);
+
+/* SolveScheme's tail after the solve (PressureSolver3D.cpp:316-336; PressureSolver2D.cpp:287-304): with d_p_old
+ * NULL (Chorin) x's mean is removed when the system is singular and the compatibility option is on
+ * (LevelPressure; the level a scene's inflow p= expressions set, :188-211, needs the scene and is the caller's);
+ * x is written into p at the unknowns, the ring BC applied, and with d_p_old every PRESSURE cell has p_old
+ * subtracted (the increment's ring values). */
+int nppe_scatter_pressure_device(nppe_solver *h, const double *d_x, double *d_p, const double *d_p_old,
+                                 const double *d_prescribed);
+
+/* AccumulatePressureIncrement (PressureSolver3D.cpp:374-393; PressureSolver2D.cpp:340-357): p = p_old + phi at
+ * the unknowns (phi: p on entry), the mean removed as LevelPressure does, then the ring BC (without prescribed
+ * values a PRESSURE ring cell gets p_old + phi there, p_b(t) to round-off). */
+int nppe_accumulate_increment_device(nppe_solver *h, const double *d_p_old, double *d_p, const double *d_prescribed);
+
+/* SolveLaplacian (PressureSolver3D.cpp:355-371; PressureSolver2D.cpp:322-337): L x = f on the committed system,
+ * solved as A x = V f from a zero guess with no reference norm (FLIP's density correction reuses the hierarchy). */
+int nppe_solve_laplacian_device(nppe_solver *h, const double *d_f, double *d_x, nppe_result *res);
+int nppe_solve_laplacian(nppe_solver *h, const double *f, double *x, nppe_result *res);
+
+/* The whole pressure step (SolveScheme after its first EnforcePressureBC, PressureSolver3D.cpp:279-336): b from
+ * the fields, the reference norm when incremental, the solve from a zero guess, the scatter into p with the ring
+ * BC and the incremental ring fix. The solve's NPPE_NOT_CONVERGED is returned after p is written, as the
+ * reference warns and goes on. */
+int nppe_pressure_step_device(nppe_solver *h, const double *d_u, const double *d_v, const double *d_w, double *d_p,
+                              const double *d_p_old, double rho, double dt, nppe_result *res);
+
+/* The projection step (Project, PressureSolver3D.cpp:396-401): ProjectPressure, then under the incremental scheme
+ * (d_p_old not NULL) AccumulatePressureIncrement. */
+int nppe_projection_step_device(nppe_solver *h, double *d_u, double *d_v, double *d_w, double *d_p,
+                                const double *d_p_old, double rho, double dt, const double *d_prescribed);
+
 /* ---- many systems at once (no counterpart in the reference, which solves one system per process) ------------- */
 
 /* Solves count independent committed systems concurrently: each handle runs on its own stream, driven by

@@ -17,10 +17,15 @@
 
 #include "../../include/nppe.h"
 #include "../../include/nppe_profile.h"
+#include "handle.h"
 #include "hierarchy.h"
 #include "kernels.h"
 
 using namespace nppe;
+
+// Serialises device allocation and release against stream capture: a capture on one handle's stream must not see
+// another thread's allocator synchronise the device.
+std::mutex nppe::gCaptureMu;
 
 namespace
 {
@@ -40,151 +45,11 @@ struct Traffic
     void add(double b) { bytes += b; }
 };
 
-struct DevLevel
-{
-    HostLevel host;
-    Box box{};
-    uint8_t *types = nullptr;
-    uint8_t *c8 = nullptr;
-    uint16_t *c16 = nullptr;
-    float *x[2] = {nullptr, nullptr};   // ping-pong pair of the smoother (level 0 uses the solver's z pair)
-    float *b = nullptr;
-    float *tab = nullptr;
-    Tables32 t32{};
-    Level32 v{};
-    int tileRows = 6;        // smoother tile: tileRows rows per block (3D: 128 cells wide), planesPerChunk planes
-    int planesPerChunk = 8;
-    double cellsBox = 0.0;   // ns * nm * nf (the box, unpitched)
-};
-
 }  // namespace
 
-struct nppe_solver
-{
-    nppe_options opt{};
-    std::string err;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    bool committed = false;
-    int dim = 3, Nx = 0, Ny = 0, Nz = 0;
-    int64_t n = 0;
-    bool singular = false;
-    bool nonUniform = false;
-    double dh = 0.0;
-
-    std::vector<int32_t> orderHost;   // unpitched box index of each unknown
-    int32_t *dOrder = nullptr;        // pitched box index of each unknown
-    std::vector<DevLevel> levels;
-    double *dTab64 = nullptr;
-    Tables64 t64{};
-    double *dX64[2] = {nullptr, nullptr};
-    double *dB64 = nullptr;
-    float *dE = nullptr, *dR = nullptr, *dP = nullptr, *dP2 = nullptr, *dQ = nullptr, *dZ = nullptr, *dZ2 = nullptr;
-    double *dR64 = nullptr, *dP64 = nullptr, *dQ64 = nullptr, *dP64b = nullptr;
-    bool krylov64 = false;   // MGPCG with the FP64 Krylov recursion (nppe_options::krylov)
-    int nuPre = 2, nuPost = 2;   // sweeps before / after the coarse correction (nppe_options, auto resolved)
-    double *dCompB = nullptr, *dCompX = nullptr, *dCompG = nullptr;
-    double *dCoarseM = nullptr;
-    int32_t *dCoarseOrder = nullptr;
-    int nCoarse = 0;
-    bool coarseDense = false;
-    SolveState *dState = nullptr;
-    SolveState *hState = nullptr;   // pinned
-    double *dPartials = nullptr;
-    int64_t partialsCap = 0;
-    uint64_t bytesHeld = 0;
-    bool budgetHit = false;
-    int coarseNu = 0;   // sweeps on levels >= 2 (0: the same as the fine levels)
-    bool useGraphs = true;                 // replay the inner iterations from a captured hipGraph
-    hipGraphExec_t innerGraph = nullptr;   // one inner iteration, captured at the first solve
-    double graphBytesPerIter = 0.0;
-    double rate = 0.0;   // observed per-iteration reduction of the inner residual norm (0: none yet)
-    std::vector<void *> allocations;
-
-    // profiling (nppe_profile.h)
-    bool profiling = false;
-    uint32_t profMask = ~0u;
-    std::vector<hipEvent_t> evPool;
-    struct Rec
-    {
-        int kind, iter;
-        hipEvent_t a, b;
-        double bytes;
-    };
-    std::vector<Rec> recs;
-    size_t evNext = 0;
-    int curIter = -1;
-    double profMs[NPPE_KERNEL_KINDS] = {};
-    double profBytes[NPPE_KERNEL_KINDS] = {};
-    int64_t profCount[NPPE_KERNEL_KINDS] = {};
-};
 
 namespace
 {
-
-int fail(nppe_solver *h, int code, const std::string &msg)
-{
-    if (h) h->err = msg;
-    return code;
-}
-
-int hip_fail(nppe_solver *h, hipError_t e, const char *what)
-{
-    if (h && h->budgetHit)
-        return fail(h, NPPE_ERR_BUDGET, std::string(what) + ": the HBM budget of the handle is exhausted");
-    return fail(h, NPPE_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-#define HIP_TRY(h, call, what)                                   \
-    do                                                           \
-    {                                                            \
-        hipError_t e_ = (call);                                  \
-        if (e_ != hipSuccess) return hip_fail((h), e_, (what));  \
-    } while (0)
-
-// Serialises device allocation and release against stream capture: a capture on one handle's stream must not see
-// another thread's allocator synchronise the device.
-std::mutex gCaptureMu;
-
-template <class T>
-hipError_t dalloc(nppe_solver *h, T **p, size_t count)
-{
-    const size_t bytes = std::max<size_t>(1, count) * sizeof(T);
-    if (h->opt.hbm_budget && h->bytesHeld + bytes > h->opt.hbm_budget)
-    {
-        h->budgetHit = true;
-        return hipErrorOutOfMemory;
-    }
-    hipError_t e;
-    {
-        std::lock_guard<std::mutex> lock(gCaptureMu);
-        e = hipMalloc((void **)p, bytes);
-    }
-    if (e != hipSuccess) return e;
-    h->allocations.push_back((void *)*p);
-    h->bytesHeld += bytes;
-    return hipMemsetAsync(*p, 0, bytes, h->stream);
-}
-
-// Host-to-device copy ordered on the handle's stream (never on the null stream, which would synchronise with
-// other handles' streams), complete on return.
-hipError_t upload(nppe_solver *h, void *dst, const void *src, size_t bytes)
-{
-    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream);
-    return e == hipSuccess ? hipStreamSynchronize(h->stream) : e;
-}
-
-// A box array (pitch and plane strides multiples of 16 elements) whose element 1 sits on a 16-element boundary:
-// the cell pair (2p + 1, 2p + 2) of every row then starts at an address aligned to the pair's size, so the
-// whole-row sweep reads a pair with one load (rbgs_pairs_aligned checks this before choosing that kernel).
-template <class T>
-hipError_t dalloc_pairs(nppe_solver *h, T **p, size_t count)
-{
-    T *raw = nullptr;
-    const hipError_t e = dalloc(h, &raw, count + 16);
-    *p = raw ? raw + 1 : nullptr;
-    return e;
-}
 
 void free_grid(nppe_solver *h)
 {
@@ -212,6 +77,11 @@ void free_grid(nppe_solver *h)
     h->dCoarseM = nullptr;
     h->dCoarseOrder = nullptr;
     h->dPartials = nullptr;
+    h->dFieldTab = nullptr;
+    h->dRowStart = nullptr;
+    h->dStepScal = nullptr;
+    for (double *&d : h->dStage) d = nullptr;
+    h->fg = FieldGeom{};
     h->committed = false;
     h->budgetHit = false;
 }
@@ -804,6 +674,7 @@ int build(nppe_solver *h, const nppe_grid *g)
     HIP_TRY(h, dalloc(h, &h->dState, 1), "allocate state");
     HIP_TRY(h, hipStreamSynchronize(h->stream), "commit");
     HIP_TRY(h, choose_tiles(h), "smoother tiles");
+    if (int rc = build_fields(h)) return rc;
     h->committed = true;
     return NPPE_OK;
 }
@@ -1169,7 +1040,7 @@ int solve_cg64(nppe_solver *h, const double *dB, double *dX, const double *dGues
     return status;
 }
 
-int solve_device(nppe_solver *h, const double *dB, double *dX, const double *dGuess, double refNorm, nppe_result *res)
+int solve_any(nppe_solver *h, const double *dB, double *dX, const double *dGuess, double refNorm, nppe_result *res)
 {
     if (!h) return NPPE_ERR_ARGUMENT;
     if (!h->committed) return fail(h, NPPE_ERR_NO_GRID, "no grid committed");
@@ -1186,6 +1057,13 @@ int solve_device(nppe_solver *h, const double *dB, double *dX, const double *dGu
 }
 
 }  // namespace
+
+int nppe::solve_device(nppe_solver *h, const double *dB, double *dX, const double *dGuess, double refNorm,
+                       nppe_result *res)
+{
+    return solve_any(h, dB, dX, dGuess, refNorm, res);
+}
+
 
 // ---- C ABI --------------------------------------------------------------------------------------------------------
 

@@ -35,7 +35,10 @@ EXPORTS = [
     "nppe_memory_bytes", "nppe_profile_enable", "nppe_profile_select", "nppe_profile_read", "nppe_synth_cosine_rhs",
     "nppe_set_tolerance", "nppe_estimate_bytes", "nppe_estimate_bytes_for", "nppe_batch_solve", "nppe_server_create", "nppe_server_destroy",
     "nppe_server_add_grid", "nppe_server_submit", "nppe_server_wait", "nppe_server_last_error",
-    "nppe_server_memory",
+    "nppe_server_memory", "nppe_rhs_device", "nppe_rhs", "nppe_project_device", "nppe_project",
+    "nppe_enforce_pressure_bc_device", "nppe_increment_reference_norm_device", "nppe_scatter_pressure_device",
+    "nppe_accumulate_increment_device", "nppe_solve_laplacian_device", "nppe_solve_laplacian",
+    "nppe_pressure_step_device", "nppe_projection_step_device",
 ]
 
 
@@ -140,6 +143,17 @@ def lib():
     L.nppe_server_last_error.restype = C.c_char_p
     L.nppe_server_memory.argtypes = [P, P, P]
     L.nppe_server_memory.restype = C.c_int
+    for name, args in (("nppe_rhs_device", [P, P, P, P, P, P, D, D, P]), ("nppe_rhs", [P, P, P, P, P, P, D, D, P]),
+                       ("nppe_project_device", [P, P, P, P, P, D, D]), ("nppe_project", [P, P, P, P, P, D, D]),
+                       ("nppe_enforce_pressure_bc_device", [P, P, P]),
+                       ("nppe_increment_reference_norm_device", [P, P, P, P]),
+                       ("nppe_scatter_pressure_device", [P, P, P, P, P]),
+                       ("nppe_accumulate_increment_device", [P, P, P, P]),
+                       ("nppe_solve_laplacian_device", [P, P, P, P]), ("nppe_solve_laplacian", [P, P, P, P]),
+                       ("nppe_pressure_step_device", [P, P, P, P, P, P, D, D, P]),
+                       ("nppe_projection_step_device", [P, P, P, P, P, P, D, D, P])):
+        getattr(L, name).argtypes = args
+        getattr(L, name).restype = C.c_int
     _lib = L
     return L
 
@@ -280,6 +294,90 @@ class Solver:
                                                  C.c_void_p(guess_ptr) if guess_ptr else None,
                                                  float(reference_norm), C.byref(res)), "solve_device")
         return res, rc
+
+    # ---- the pressure step around the solve (MAC fields; include/nppe.h) ----------------------------------------
+    def field_sizes(self):
+        """Lengths of the MAC arrays u, v, w, p of the committed grid (w is 0 in 2D)."""
+        Ny, Nx, Nz = self.grid.types.shape
+        if self.grid.dim == 2:
+            return Ny * (Nx + 1), (Ny + 1) * Nx, 0, Ny * Nx
+        return Ny * (Nx + 1) * Nz, (Ny + 1) * Nx * Nz, Ny * Nx * (Nz + 1), Ny * Nx * Nz
+
+    def rhs(self, u, v, w, p, rho, dt, p_old=None):
+        """b (IDP order) from host MAC fields: rho/dt div(u) V - scale DirichletSum, made compatible if singular."""
+        a = [None if q is None else np.ascontiguousarray(q, np.float64) for q in (u, v, w, p, p_old)]
+        b = np.zeros(max(1, self.n), np.float64)
+        self._check(lib().nppe_rhs(self.h, *[_ptr(q) for q in a], float(rho), float(dt), _ptr(b)), "rhs")
+        return b[: self.n]
+
+    def project(self, u, v, w, p, rho, dt):
+        """(u, v, w) after ProjectPressure on host MAC fields (copies)."""
+        u, v = np.array(u, np.float64), np.array(v, np.float64)
+        w = None if w is None else np.array(w, np.float64)
+        p = np.ascontiguousarray(p, np.float64)
+        self._check(lib().nppe_project(self.h, _ptr(u), _ptr(v), _ptr(w), _ptr(p), float(rho), float(dt)),
+                    "project")
+        return u, v, w
+
+    def solve_laplacian(self, f):
+        f = np.ascontiguousarray(f, np.float64)
+        x = np.zeros(max(1, self.n), np.float64)
+        res = Result()
+        rc = self._check(lib().nppe_solve_laplacian(self.h, _ptr(f), _ptr(x), C.byref(res)), "solve_laplacian")
+        return x[: self.n], res, rc
+
+    # device variants take raw device addresses (0 for NULL)
+    @staticmethod
+    def _d(ptr):
+        return C.c_void_p(ptr) if ptr else None
+
+    def rhs_device(self, u, v, w, p, p_old, rho, dt, b):
+        d = self._d
+        self._check(lib().nppe_rhs_device(self.h, d(u), d(v), d(w), d(p), d(p_old), float(rho), float(dt), d(b)),
+                    "rhs_device")
+
+    def project_device(self, u, v, w, p, rho, dt):
+        d = self._d
+        self._check(lib().nppe_project_device(self.h, d(u), d(v), d(w), d(p), float(rho), float(dt)),
+                    "project_device")
+
+    def enforce_pressure_bc_device(self, p, prescribed=0):
+        self._check(lib().nppe_enforce_pressure_bc_device(self.h, self._d(p), self._d(prescribed)),
+                    "enforce_pressure_bc_device")
+
+    def increment_reference_norm_device(self, b, p_old) -> float:
+        out = C.c_double()
+        self._check(lib().nppe_increment_reference_norm_device(self.h, self._d(b), self._d(p_old), C.byref(out)),
+                    "increment_reference_norm_device")
+        return out.value
+
+    def scatter_pressure_device(self, x, p, p_old=0, prescribed=0):
+        d = self._d
+        self._check(lib().nppe_scatter_pressure_device(self.h, d(x), d(p), d(p_old), d(prescribed)),
+                    "scatter_pressure_device")
+
+    def accumulate_increment_device(self, p_old, p, prescribed=0):
+        d = self._d
+        self._check(lib().nppe_accumulate_increment_device(self.h, d(p_old), d(p), d(prescribed)),
+                    "accumulate_increment_device")
+
+    def solve_laplacian_device(self, f, x):
+        res = Result()
+        rc = self._check(lib().nppe_solve_laplacian_device(self.h, self._d(f), self._d(x), C.byref(res)),
+                         "solve_laplacian_device")
+        return res, rc
+
+    def pressure_step_device(self, u, v, w, p, p_old, rho, dt):
+        d = self._d
+        res = Result()
+        rc = self._check(lib().nppe_pressure_step_device(self.h, d(u), d(v), d(w), d(p), d(p_old), float(rho),
+                                                         float(dt), C.byref(res)), "pressure_step_device")
+        return res, rc
+
+    def projection_step_device(self, u, v, w, p, p_old, rho, dt, prescribed=0):
+        d = self._d
+        self._check(lib().nppe_projection_step_device(self.h, d(u), d(v), d(w), d(p), d(p_old), float(rho),
+                                                      float(dt), d(prescribed)), "projection_step_device")
 
     def apply(self, x):
         x = np.ascontiguousarray(x, np.float64)

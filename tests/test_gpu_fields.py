@@ -1,0 +1,288 @@
+"""The pressure step around the solve on the GPU, through the C ABI: the right-hand side from MAC fields, the
+projection, EnforcePressureBC, the increment reference norm, SolveScheme's scatter tail, the increment accumulation
+and SolveLaplacian, against the fields_*.npz fixtures written from the compiled reference (bit-exact where the
+reference's arithmetic is reproduced, within 1e-14 where only a sum's order differs) and against the oracle; the
+composed pressure and projection steps on the fixtures and, at 128^3 and 256^3, through the property the step
+exists for: the projected velocity is divergence-free to the solve's tolerance."""
+import glob
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from nereus_cfd_amd import nppe as N
+
+HERE = os.path.join(os.path.dirname(__file__), "golden")
+FIELDS = sorted(glob.glob(os.path.join(HERE, "fields_*.npz")))
+IDS = [os.path.basename(p)[7:-4] for p in FIELDS]
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def load(path):
+    return dict(np.load(path, allow_pickle=False))
+
+
+def case_of(path):
+    f = load(path)
+    base = load(os.path.join(HERE, str(f["case"]) + ".npz"))
+    dim = int(base["dim"])
+    wz = base["wz"] if dim == 3 else np.array([float(base["dh"])])
+    og = O.Grid(dim, base["types"], base["wx"], base["wy"], wz, float(base["dh"]), bool(base["ref_non_uniform"]))
+    g = N.Grid(dim, base["types"], base["wx"], base["wy"], wz, float(base["dh"]), bool(base["ref_non_uniform"]))
+    return og, O.assemble(og), g, f
+
+
+def opt(a):
+    return a if len(a) else None
+
+
+def prescribed_of(og, f):
+    t = og.types
+    ring = np.zeros(t.shape, bool)
+    ring[0] = ring[-1] = True
+    ring[:, 0] = ring[:, -1] = True
+    if og.dim == 3:
+        ring[:, :, 0] = ring[:, :, -1] = True
+    m = ring.ravel() & (t.ravel() == O.PRESSURE)
+    pr = np.zeros(t.size)
+    pr[m] = f["p_bc"][m]
+    return pr
+
+
+class Dev:
+    """Device copies (torch float64 on cuda:0) of numpy arrays; .ptr(name) gives the address (0 for None)."""
+
+    def __init__(self, **arrays):
+        import torch
+        self.t = {}
+        for k, a in arrays.items():
+            self.t[k] = None if a is None else torch.from_numpy(np.ascontiguousarray(a, np.float64)).to("cuda:0")
+        torch.cuda.synchronize()
+
+    def ptr(self, k):
+        t = self.t.get(k)
+        return 0 if t is None else t.data_ptr()
+
+    def host(self, k):
+        import torch
+        torch.cuda.synchronize()
+        return self.t[k].cpu().numpy()
+
+
+def solver_for(g):
+    s = N.Solver()
+    s.commit(g)
+    return s
+
+
+@pytest.mark.parametrize("path", FIELDS, ids=IDS)
+def test_rhs_matches_reference(gpu, path):
+    og, osys, g, f = case_of(path)
+    say("rhs", IDS[FIELDS.index(path)])
+    with solver_for(g) as s:
+        assert s.n == osys.n
+        rho, dt = float(f["rho"]), float(f["dt"])
+        D = Dev(u=f["u"], v=f["v"], w=opt(f["w"]), p=f["p"], p_old=opt(f["p_old"]), b=np.zeros(max(1, s.n)))
+        s.rhs_device(D.ptr("u"), D.ptr("v"), D.ptr("w"), D.ptr("p"), D.ptr("p_old"), rho, dt, D.ptr("b"))
+        b = D.host("b")[: s.n]
+        bh = s.rhs(f["u"], f["v"], opt(f["w"]), f["p"], rho, dt, opt(f["p_old"]))
+        if s.singular:
+            # the compatibility step (mean removed twice) sums in another order than the reference's loop
+            ref = O.compatibility(f["rhs"])
+            assert np.abs(b - ref).max() <= 1e-14 * np.abs(f["rhs"]).max()
+            assert abs(b.mean()) <= 1e-15 * np.abs(f["rhs"]).max()
+        else:
+            assert np.array_equal(b, f["rhs"]), np.abs(b - f["rhs"]).max()
+        assert np.array_equal(bh, b)
+
+
+@pytest.mark.parametrize("path", FIELDS, ids=IDS)
+def test_projection_matches_reference(gpu, path):
+    og, osys, g, f = case_of(path)
+    say("project", IDS[FIELDS.index(path)])
+    with solver_for(g) as s:
+        rho, dt = float(f["rho"]), float(f["dt"])
+        D = Dev(u=f["u"], v=f["v"], w=opt(f["w"]), p=f["p"])
+        s.project_device(D.ptr("u"), D.ptr("v"), D.ptr("w"), D.ptr("p"), rho, dt)
+        assert np.array_equal(D.host("u"), f["u_proj"])
+        assert np.array_equal(D.host("v"), f["v_proj"])
+        if og.dim == 3:
+            assert np.array_equal(D.host("w"), f["w_proj"])
+        u, v, w = s.project(f["u"], f["v"], opt(f["w"]), f["p"], rho, dt)
+        assert np.array_equal(u, f["u_proj"]) and np.array_equal(v, f["v_proj"])
+
+
+@pytest.mark.parametrize("path", FIELDS, ids=IDS)
+def test_pressure_bc_and_reference_norm(gpu, path):
+    og, osys, g, f = case_of(path)
+    say("bc", IDS[FIELDS.index(path)])
+    with solver_for(g) as s:
+        pr = prescribed_of(og, f)
+        D = Dev(p=f["p"], pr=pr, p2=f["p_bc"], b=f["rhs"], p_old=opt(f["p_old"]))
+        s.enforce_pressure_bc_device(D.ptr("p"), D.ptr("pr"))
+        assert np.array_equal(D.host("p"), f["p_bc"])
+        s.enforce_pressure_bc_device(D.ptr("p2"))
+        assert np.array_equal(D.host("p2"), f["p_bc"])
+        if len(f["p_old"]):
+            ref = s.increment_reference_norm_device(D.ptr("b"), D.ptr("p_old"))
+            assert abs(ref - float(f["ref_norm"])) <= 1e-13 * float(f["ref_norm"])
+
+
+@pytest.mark.parametrize("path", FIELDS, ids=IDS)
+def test_scatter_and_accumulate_match_reference(gpu, path):
+    og, osys, g, f = case_of(path)
+    say("scatter", IDS[FIELDS.index(path)])
+    incremental = len(f["p_old"]) > 0
+    with solver_for(g) as s:
+        pr = prescribed_of(og, f)
+        D = Dev(x=f["x_tail"], p=f["p_bc"], p_old=opt(f["p_old"]), pr=pr)
+        s.scatter_pressure_device(D.ptr("x"), D.ptr("p"), D.ptr("p_old"), D.ptr("pr"))
+        p = D.host("p")
+        if s.singular and not incremental:
+            assert np.abs(p - f["p_tail"]).max() <= 1e-14 * np.abs(f["x_tail"]).max()
+        else:
+            assert np.array_equal(p, f["p_tail"])
+        if not incremental:
+            return
+        D2 = Dev(p=f["p_tail"], p_old=f["p_old"], pr=pr, q=f["p_tail"])
+        s.accumulate_increment_device(D2.ptr("p_old"), D2.ptr("p"), D2.ptr("pr"))
+        pa = D2.host("p")
+        tol = 1e-14 * np.abs(f["p_acc"]).max()
+        if s.singular:
+            assert np.abs(pa - f["p_acc"]).max() <= tol
+        else:
+            assert np.array_equal(pa, f["p_acc"])
+        # without prescribed values a PRESSURE ring cell gets p_old + phi: p_b(t) to round-off
+        s.accumulate_increment_device(D2.ptr("p_old"), D2.ptr("q"))
+        q = D2.host("q")
+        assert np.abs(q - f["p_acc"]).max() <= 4e-16 * max(1.0, np.abs(f["p_acc"]).max()) * 8
+        pres = og.types.ravel() != O.PRESSURE
+        if not s.singular:
+            assert np.array_equal(q[pres], f["p_acc"][pres])
+
+
+@pytest.mark.parametrize("path", FIELDS, ids=IDS)
+def test_solve_laplacian(gpu, path):
+    og, osys, g, f = case_of(path)
+    say("laplacian", IDS[FIELDS.index(path)])
+    rng = np.random.default_rng(7)
+    fl = rng.standard_normal(osys.n)
+    with solver_for(g) as s:
+        if s.singular:
+            fl = fl - fl.mean()
+        x, res, rc = s.solve_laplacian(fl)
+        assert rc == N.OK
+        # A x = V f with the reference's cell volumes (the right-hand side's volume term, pinned by the fixtures)
+        Ny, Nx, Nz = og.types.shape
+        if og.non_uniform:
+            k, j, i = osys.order % Nz, (osys.order // Nz) % Nx, osys.order // (Nz * Nx)
+            vol = og.wx[j] * og.wy[i] * (og.wz[k] if og.dim == 3 else 1.0)
+        else:
+            vol = np.ones(osys.n)
+        b = fl * vol
+        if s.singular:
+            b = O.compatibility(b)
+        assert osys.relres(b, x) <= 1.05e-8
+        D = Dev(f=fl, x=np.zeros(osys.n))
+        res2, rc2 = s.solve_laplacian_device(D.ptr("f"), D.ptr("x"))
+        assert np.array_equal(D.host("x"), x)
+
+
+@pytest.mark.parametrize("path", FIELDS, ids=IDS)
+def test_pressure_and_projection_steps(gpu, path):
+    """The composed steps against the pieces: the solve's x satisfies A x = b (b the oracle's right-hand side),
+    p is the oracle's scatter of that x, and the projection and accumulation are the oracle's on that p."""
+    og, osys, g, f = case_of(path)
+    say("steps", IDS[FIELDS.index(path)])
+    incremental = len(f["p_old"]) > 0
+    rho, dt = float(f["rho"]), float(f["dt"])
+    with solver_for(g) as s:
+        p0 = f["p_bc"]
+        D = Dev(u=f["u"], v=f["v"], w=opt(f["w"]), p=p0, p_old=opt(f["p_old"]))
+        res, rc = s.pressure_step_device(D.ptr("u"), D.ptr("v"), D.ptr("w"), D.ptr("p"), D.ptr("p_old"), rho, dt)
+        assert rc == N.OK
+        p = D.host("p")
+        b = O.rhs(og, osys, f["u"], f["v"], opt(f["w"]), p0, rho, dt, opt(f["p_old"]))
+        if s.singular:
+            b = O.compatibility(b)
+        x = p[osys.order]
+        if incremental:
+            ref = np.linalg.norm(b + osys.matvec(f["p_old"][osys.order]))
+            assert np.linalg.norm(b - osys.matvec(x)) <= 1.05e-8 * max(np.linalg.norm(b), ref)
+        else:
+            assert osys.relres(b, x) <= 1.05e-8 * (1.0 + (1e-6 if s.singular else 0.0))
+        expect = O.scatter_pressure(og, osys, x, p0, opt(f["p_old"]), s.singular)
+        if s.singular and not incremental:
+            # the device levelled x before writing it, so x here is already mean-free: levelling it again moves it
+            # by round-off only
+            assert np.abs(p - expect).max() <= 1e-13 * max(1.0, np.abs(p).max())
+        else:
+            assert np.array_equal(p, expect)
+        s.projection_step_device(D.ptr("u"), D.ptr("v"), D.ptr("w"), D.ptr("p"), D.ptr("p_old"), rho, dt)
+        u, v, w = O.project(og, f["u"], f["v"], opt(f["w"]), p, rho, dt)
+        assert np.array_equal(D.host("u"), u) and np.array_equal(D.host("v"), v)
+        if og.dim == 3:
+            assert np.array_equal(D.host("w"), w)
+        if incremental:
+            # without prescribed values the accumulation puts p_old + phi on the PRESSURE ring cells
+            ring = np.zeros(og.types.shape, bool)
+            ring[0] = ring[-1] = True
+            ring[:, 0] = ring[:, -1] = True
+            if og.dim == 3:
+                ring[:, :, 0] = ring[:, :, -1] = True
+            m = ring.ravel() & (og.types.ravel() == O.PRESSURE)
+            pr = np.zeros(og.types.size)
+            pr[m] = f["p_old"][m] + p[m]
+            pa = O.accumulate_increment(og, osys, f["p_old"], p, s.singular, pr)
+            got = D.host("p")
+            if s.singular:
+                assert np.abs(got - pa).max() <= 1e-13 * max(1.0, np.abs(pa).max())
+            else:
+                assert np.array_equal(got, pa)
+
+
+def _outlet_box(n):
+    g = N.uniform_grid(n, dim=3)
+    g.types[1:-1, -1, 1:-1] = N.EMPTY
+    return g
+
+
+@pytest.mark.parametrize("n", [128, 256])
+def test_projection_leaves_divergence_free_field(gpu, n):
+    """At full size: after the pressure and projection steps the velocity's divergence (the right-hand side of the
+    projected field with p = 0) is the solve's residual, i.e. reduced by the tolerance."""
+    import torch
+    say("divergence", n)
+    g = _outlet_box(n)
+    with solver_for(g) as s:
+        nu, nv, nw, npp = s.field_sizes()
+        gen = torch.Generator(device="cuda:0").manual_seed(11)
+        u = torch.randn(nu, dtype=torch.float64, device="cuda:0", generator=gen)
+        v = torch.randn(nv, dtype=torch.float64, device="cuda:0", generator=gen)
+        w = torch.randn(nw, dtype=torch.float64, device="cuda:0", generator=gen)
+        p = torch.zeros(npp, dtype=torch.float64, device="cuda:0")
+        z = torch.zeros(npp, dtype=torch.float64, device="cuda:0")
+        b0 = torch.zeros(s.n, dtype=torch.float64, device="cuda:0")
+        b1 = torch.zeros(s.n, dtype=torch.float64, device="cuda:0")
+        torch.cuda.synchronize()
+        rho, dt = 1.0, 1e-3
+        s.rhs_device(u.data_ptr(), v.data_ptr(), w.data_ptr(), z.data_ptr(), 0, rho, dt, b0.data_ptr())
+        res, rc = s.pressure_step_device(u.data_ptr(), v.data_ptr(), w.data_ptr(), p.data_ptr(), 0, rho, dt)
+        assert rc == N.OK
+        s.projection_step_device(u.data_ptr(), v.data_ptr(), w.data_ptr(), p.data_ptr(), 0, rho, dt)
+        s.rhs_device(u.data_ptr(), v.data_ptr(), w.data_ptr(), z.data_ptr(), 0, rho, dt, b1.data_ptr())
+        torch.cuda.synchronize()
+        before, after = float(torch.linalg.norm(b0)), float(torch.linalg.norm(b1))
+        say(f"  n={n} iterations={res.iterations} |div| before {before:.3e} after {after:.3e}")
+        # the EMPTY outlet holds p = 0 on the ring: its DirichletSum is zero, so b0 is the divergence term alone;
+        # after the projection, rho/dt div V = b - A p for the solved p
+        assert after <= 1.05e-8 * before
+        assert before > 0.0
