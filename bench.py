@@ -3,13 +3,14 @@
 to 1e-8 true relative residual, on one MI355X.
 
 A step is one complete solve (right-hand side already resident in HBM, solution left in HBM): scatter to the box,
-FP64 residual, FP32 multigrid-preconditioned CG passes inside FP64 iterative refinement, mean projection, gather.
+FP64 residual, CG with its recursion in FP64 preconditioned by an FP32 multigrid V-cycle, the true FP64 residual,
+mean projection, gather.
 Prints one JSON line. Also reports, on the same GPU, the FP64 matrix-free Jacobi-PCG solve of the same system (the
 plain-CG comparison), and on the host the reference's own AMGCL and Eigen CG (oracle/_ref, compiled from the
 reference sources) on a bounded sample.
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--size 256] [--no-cpu] [--no-cg64]
-    python bench.py --config channel|batched|serving      (the other configurations, one JSON line each)
+    python bench.py --config channel|batched|serving|step|layout   (the other configurations, one JSON line each)
 """
 from __future__ import annotations
 
@@ -217,7 +218,8 @@ def channel_leg(args):
             "gbps": res.algorithmic_bytes / (el / args.steps) / 1e9,
             "config": {"workload": "3D stretched channel 256x128x128 cells, neighbour ratio <= 1.05 towards a "
                                    "cylinder r=0.05 (SOLID voxels), inflow x-min, p=0 outlet x-max, SOLID walls; "
-                                   f"N(0,1) RHS seed {SEED}; MGPCG + FP64 refinement to 1e-10",
+                                   f"N(0,1) RHS seed {SEED}; MGPCG (V(4,4) FP32 preconditioner, FP64 CG "
+                                   f"recursion) to 1e-10",
                        "unknowns": int(len(b))}}
 
 
@@ -243,6 +245,145 @@ def copy_leg(args):
     return {"metric": "device copy GB/s", "value": round(moved / ms / 1e6, 1), "unit": "GB/s", "ms_per_step": ms,
             "scaling": "weak", "config": {"workload": "y = 1.0 * x over 2^28 FP32 (1 GiB read + 1 GiB write)",
                                           "bytes_read": 4 * n, "bytes_written": 4 * n}}
+
+
+def step_leg(args):
+    """The pressure step around the solve at 256^3 on MAC fields resident in HBM: a box with SOLID walls and a p = 0
+    (EMPTY) outlet on the x-max ring, N(0,1) velocity; the right-hand side kernel, the projection kernel, and the
+    composed pressure step (right-hand side, solve to 1e-8, scatter with the ring BC) plus projection step. Also the
+    host-array right-hand side and projection (PCIe-inclusive) and the oracle's C loops on the host."""
+    import torch
+    import nereus_cfd_amd as N
+    n = args.size
+    g = N.uniform_grid(n)
+    g.types[1:-1, -1, 1:-1] = N.EMPTY
+    gen = torch.Generator(device="cuda:0").manual_seed(SEED)
+    with N.Solver() as s:
+        s.commit(g)
+        nu, nv, nw, npp = s.field_sizes()
+        u0 = torch.randn(nu, dtype=torch.float64, device="cuda:0", generator=gen)
+        v0 = torch.randn(nv, dtype=torch.float64, device="cuda:0", generator=gen)
+        w0 = torch.randn(nw, dtype=torch.float64, device="cuda:0", generator=gen)
+        u, v, w = u0.clone(), v0.clone(), w0.clone()
+        p = torch.zeros(npp, dtype=torch.float64, device="cuda:0")
+        b = torch.zeros(s.n, dtype=torch.float64, device="cuda:0")
+        torch.cuda.synchronize()
+        rho, dt = 1.0, 1e-3
+        cells = float(npp)
+        faces = float(nu + nv + nw)
+        rhs_bytes = 8.0 * faces + cells + 8.0 * s.n
+        proj_bytes = 16.0 * faces + 9.0 * cells
+
+        def timed(fn, reps):
+            for _ in range(max(1, args.warmup)):
+                fn()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            return (time.perf_counter() - t0) / reps
+
+        reps = max(20, 5 * args.steps)
+        t_rhs = timed(lambda: s.rhs_device(u.data_ptr(), v.data_ptr(), w.data_ptr(), p.data_ptr(), 0, rho, dt,
+                                           b.data_ptr()), reps)
+        t_proj = timed(lambda: s.project_device(u.data_ptr(), v.data_ptr(), w.data_ptr(), p.data_ptr(), rho, dt), reps)
+
+        def step():
+            u.copy_(u0)
+            v.copy_(v0)
+            w.copy_(w0)
+            torch.cuda.synchronize()
+            res, rc = s.pressure_step_device(u.data_ptr(), v.data_ptr(), w.data_ptr(), p.data_ptr(), 0, rho, dt)
+            if rc != 0 or not res.relative_residual < 1e-8:
+                raise SystemExit(f"pressure step did not converge: {res.relative_residual}")
+            s.projection_step_device(u.data_ptr(), v.data_ptr(), w.data_ptr(), p.data_ptr(), 0, rho, dt)
+            return res
+
+        res = step()
+        t_step = timed(step, args.steps)
+        # what the projection left: rho/dt div(u) V with p = 0, against the divergence it started from
+        z = torch.zeros_like(p)
+        s.rhs_device(u.data_ptr(), v.data_ptr(), w.data_ptr(), z.data_ptr(), 0, rho, dt, b.data_ptr())
+        after = float(torch.linalg.norm(b))
+        s.rhs_device(u0.data_ptr(), v0.data_ptr(), w0.data_ptr(), z.data_ptr(), 0, rho, dt, b.data_ptr())
+        before = float(torch.linalg.norm(b))
+        # the host-array variants (PCIe in and out)
+        hu, hv, hw, hp = (t.cpu().numpy() for t in (u0, v0, w0, p))
+        t_rhs_host = timed(lambda: s.rhs(hu, hv, hw, hp, rho, dt), 3)
+        t_proj_host = timed(lambda: s.project(hu, hv, hw, hp, rho, dt), 3)
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        og = O.Grid(3, g.types, g.wx, g.wy, g.wz, g.dh, False)
+        sysm = O.assemble(og)
+        t0 = time.perf_counter()
+        O.rhs(og, sysm, hu, hv, hw, hp, rho, dt)
+        t_cr = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        O.project(og, hu, hv, hw, hp, rho, dt)
+        t_cp = time.perf_counter() - t0
+        cpu = {"rhs_ms": 1e3 * t_cr, "project_ms": 1e3 * t_cp, "cores": 1, "kind": "port",
+               "sample": f"one pass of the oracle's C loops (oracle_rhs, oracle_project) over the {n}^3 fields"}
+    return {"metric": "pressure step (RHS + solve + scatter + projection) per second at 256^3", "value": 1.0 / t_step,
+            "unit": "steps/s", "ms_per_step": 1e3 * t_step,
+            "rhs_ms": 1e3 * t_rhs, "rhs_gbps": rhs_bytes / t_rhs / 1e9, "rhs_bytes": rhs_bytes,
+            "project_ms": 1e3 * t_proj, "project_gbps": proj_bytes / t_proj / 1e9, "project_bytes": proj_bytes,
+            "rhs_host_ms": 1e3 * t_rhs_host, "project_host_ms": 1e3 * t_proj_host,
+            "iterations": int(res.iterations), "relative_residual": float(res.relative_residual),
+            "divergence_before": before, "divergence_after": after, "cpu_baseline": cpu,
+            "config": {"workload": f"{n}^3 uniform box, SOLID walls, EMPTY (p = 0) outlet ring at x-max; N(0,1) "
+                                   f"MAC velocity (torch seed {SEED}); rho = 1, dt = 1e-3; Chorin step: RHS, MGPCG to "
+                                   f"1e-8, scatter + EnforcePressureBC, ProjectPressure",
+                       "unknowns": int(s.n)}}
+
+
+def layout_leg(args):
+    """FLIP's layout change at 128^3: a free-surface box whose liquid moves every step. Times a full commit of each
+    new layout against nppe_update_layout (host types) and nppe_update_layout_device (types in HBM), and the solve
+    that follows (to 1e-8)."""
+    import torch
+    import nereus_cfd_amd as N
+    import nereus_cfd_amd.workloads as W
+    n = 128 if args.size == 256 else args.size
+    shape = (n, n, n)
+    grids = [W.flip_free_surface(shape, np.random.default_rng(SEED + q)) for q in range(args.steps + 1)]
+    dev = [torch.from_numpy(np.ascontiguousarray(g.types)).to("cuda:0") for g in grids]
+    torch.cuda.synchronize()
+    t_commit, t_host, t_dev, t_same, t_solve, codes = [], [], [], [], [], []
+    with N.Solver() as fresh, N.Solver() as s, N.Solver() as d:
+        s.commit(grids[0])
+        d.commit(grids[0])
+        for q in range(1, len(grids)):
+            t0 = time.perf_counter()
+            fresh.commit(grids[q])
+            t_commit.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            codes.append(s.update_layout(grids[q].types))
+            t_host.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            d.update_layout_device(dev[q].data_ptr(), grids[q].types)
+            t_dev.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            d.update_layout_device(dev[q].data_ptr(), grids[q].types)
+            t_same.append(time.perf_counter() - t0)
+            b = W.random_rhs(grids[q], np.random.default_rng(q), zero_mean=False)
+            tb = torch.from_numpy(b).cuda()
+            tx = torch.zeros_like(tb)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            res, rc = d.solve_device(tb.data_ptr(), tx.data_ptr())
+            t_solve.append(time.perf_counter() - t0)
+            if rc != 0 or not res.relative_residual < 1e-8:
+                raise SystemExit(f"solve after the layout update did not converge: {res.relative_residual}")
+    med = lambda v: 1e3 * float(np.median(v))
+    return {"metric": "FLIP layout change + solve per second at 128^3", "value": 1.0 / (np.median(t_dev) +
+                                                                                       np.median(t_solve)),
+            "unit": "steps/s", "ms_per_step": med(t_dev) + med(t_solve),
+            "commit_ms": med(t_commit), "update_host_ms": med(t_host), "update_device_ms": med(t_dev),
+            "unchanged_check_ms": med(t_same), "solve_ms": med(t_solve), "update_codes": codes,
+            "config": {"workload": f"{n}^3 box, SOLID walls, liquid = dam-break block (30-50 % of the volume) + 0-200 "
+                                   f"spheres r = 2-6 cells, EMPTY elsewhere (a new random layout each step, seeds "
+                                   f"{SEED}+q); MGPCG to 1e-8 after each update",
+                       "steps": args.steps}}
 
 
 def batched_leg(args):
@@ -361,7 +502,7 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg64", action="store_true")
-    ap.add_argument("--config", default="headline", choices=["headline", "channel", "batched", "serving", "copy"])
+    ap.add_argument("--config", default="headline", choices=["headline", "channel", "batched", "serving", "copy", "step", "layout"])
     ap.add_argument("--requests", type=int, default=32)
     ap.add_argument("--rate", type=float, default=40.0)
     ap.add_argument("--slots", type=int, default=4)
@@ -372,7 +513,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.config != "headline":
-        leg = {"channel": channel_leg, "batched": batched_leg, "serving": serving_leg, "copy": copy_leg}[args.config]
+        leg = {"channel": channel_leg, "batched": batched_leg, "serving": serving_leg, "copy": copy_leg,
+               "step": step_leg, "layout": layout_leg}[args.config]
         out = leg(args)
         out.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
                     "dtype": "fp32+fp64", "data": "synthetic", "vs_baseline": None})

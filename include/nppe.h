@@ -52,7 +52,8 @@ enum nppe_status
 /* Solve methods. */
 enum nppe_method
 {
-    NPPE_METHOD_MGPCG = 0, /* FP32 geometric-multigrid-preconditioned CG inside FP64 iterative refinement */
+    NPPE_METHOD_MGPCG = 0, /* CG preconditioned by an FP32 geometric-multigrid V-cycle, its Krylov recursion in FP64
+                              or in FP32 passes inside FP64 iterative refinement (nppe_options::krylov) */
     NPPE_METHOD_CG64 = 1   /* FP64 matrix-free Jacobi-PCG (Eigen's ConjugateGradient<..., DiagonalPreconditioner>) */
 };
 
@@ -145,6 +146,19 @@ const char *nppe_last_error(const nppe_solver *h);
  * numbers the unknowns, decides singularity (no Dirichlet face), uploads the cell types and tables and builds the
  * multigrid hierarchy. Returns the number of unknowns (>= 0) or an error. */
 int64_t nppe_commit_grid(nppe_solver *h, const nppe_grid *g);
+
+/* EnsureSystem / PressureLayoutChanged (PressureSolver3D.cpp:120-124; PressureSettings.h:300-303) for a layout
+ * change of the committed box, its spacing unchanged (FLIP's moving liquid, FLIP3D.cpp:784-833): compares
+ * cell_types (nx*ny*nz bytes, MAC order) with the committed layout and returns 0 when they are equal, doing
+ * nothing. Otherwise it renumbers the unknowns and rebuilds every multigrid level's types and stencil codes and the
+ * coarsest solve in the handle's existing buffers -- no device allocation, no tile search -- and returns 1; when
+ * the new layout needs another hierarchy (a different level count, or a coarsest solve the buffers cannot hold) it
+ * commits the grid again in full and returns 2. Solves after an update are bit-identical to those of a handle
+ * committed with the new layout. Negative on error. */
+int nppe_update_layout(nppe_solver *h, const uint8_t *cell_types);
+/* The same with the cell types in device memory: the comparison runs on the device, and the types are read back
+ * only when they differ. */
+int nppe_update_layout_device(nppe_solver *h, const uint8_t *d_cell_types);
 
 /* Unknown count and singularity of the committed system (PressureSystem::A.rows(), ::singular,
  * PressureSettings.h:284-297). */

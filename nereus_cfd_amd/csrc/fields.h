@@ -29,8 +29,10 @@ struct FieldGeom
     int64_t n;                 // unknowns
 };
 
-// Blocks of a row-walking launch (four rows per block), which is also the number of partial sums it writes.
+// Blocks of a row-walking launch (four rows per block), which is also the number of partial sums it writes; the
+// same for a launch over every row of the box (ring rows included).
 int field_row_blocks(const FieldGeom &G);
+int field_box_blocks(const FieldGeom &G);
 
 // b[id] = rho/dt * div(u) * V - scale * DirichletSum(p, pOld) for every unknown (pOld null: Chorin).
 hipError_t launch_rhs(hipStream_t s, const FieldGeom &G, const double *u, const double *v, const double *w,
@@ -39,14 +41,13 @@ hipError_t launch_rhs(hipStream_t s, const FieldGeom &G, const double *u, const 
 hipError_t launch_project(hipStream_t s, const FieldGeom &G, double *u, double *v, double *w, const double *p,
                           double f);
 // The ring list of MAC::EnforcePressureBC, pass by pass (y faces, x faces, z faces): an EMPTY ring cell gets 0, a
-// PRESSURE one prescribed[c] (without prescribed values: pOldBase[c] + p[c] when pOldBase is given, else its own
-// value), any other the inward neighbour's value.
-hipError_t launch_pressure_bc(hipStream_t s, const FieldGeom &G, double *p, const double *prescribed,
-                              const double *pOldBase = nullptr);
+// PRESSURE one prescribed[c] (its own value when prescribed is null), any other the inward neighbour's value.
+hipError_t launch_pressure_bc(hipStream_t s, const FieldGeom &G, double *p, const double *prescribed);
 // p[c] = x[id] - (sumX ? *sumX / n : 0) at the unknowns.
 hipError_t launch_scatter_p(hipStream_t s, const FieldGeom &G, const double *x, const double *sumX, double *p);
-// p[c] -= pOld[c] at every PRESSURE cell of the box.
-hipError_t launch_ring_fix(hipStream_t s, const FieldGeom &G, const double *pOld, double *p);
+// p[c] -= pOld[c] at every PRESSURE cell of the box; with restore, p[c] = pOld[c] + p[c] at the PRESSURE cells of
+// the ring only (each once, where the ring list visits edge and corner cells more than once).
+hipError_t launch_ring_fix(hipStream_t s, const FieldGeom &G, const double *pOld, double *p, bool restore = false);
 // partials[block] = sum over the unknowns of pOld[c] + p[c].
 hipError_t launch_sum_increment(hipStream_t s, const FieldGeom &G, const double *pOld, const double *p,
                                 double *partials);
@@ -58,6 +59,10 @@ hipError_t launch_gather_box(hipStream_t s, const FieldGeom &G, const double *pO
 hipError_t launch_norm2_sum(hipStream_t s, const FieldGeom &G, const double *b, const double *y, double *partials);
 // b[id] = f[id] * V(cell) (V = 1.0 on a uniform grid).
 hipError_t launch_scale_volume(hipStream_t s, const FieldGeom &G, const double *f, double *b);
+
+// partials[block] = the number of cells whose type in newTypes (MAC order, unpitched) differs from the committed
+// one, partials[blocks + block] the number of cells whose code is out of range (> 5); *blocks = the blocks.
+hipError_t launch_diff_types(hipStream_t s, const FieldGeom &G, const uint8_t *newTypes, double *partials, int *blocks);
 
 // Linear vectors of n entries.
 // partials[block] = sum of x over the block's share; returns the block count through *blocks.

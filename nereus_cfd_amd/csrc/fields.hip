@@ -116,7 +116,8 @@ __device__ __forceinline__ double block_sum(double v)
 }
 
 // ---- right-hand side ---------------------------------------------------------------------------------------
-template <int DIM>
+// NU: the stretched (volume-weighted) form, G.nonUniform.
+template <int DIM, bool NU>
 __global__ __launch_bounds__(kThreads) void k_rhs(FieldGeom G, const double *__restrict__ u,
                                                  const double *__restrict__ v, const double *__restrict__ w,
                                                  const double *__restrict__ p, const double *__restrict__ pOld,
@@ -126,18 +127,52 @@ __global__ __launch_bounds__(kThreads) void k_rhs(FieldGeom G, const double *__r
     const int r = xcd_block() * kRowsPerBlock + (threadIdx.x >> 6);
     const Row R = interior_row<DIM>(G, r);
     const int Nx = G.Nx, Ny = G.Ny, Nz = G.Nz;
-    walk_unknowns<DIM>(G, R, r, [&](int f, int64_t id, unsigned code) {
-#pragma clang fp contract(off)
+    if (!R.active) return;   // no barrier in this kernel
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (1ull << lane) - 1ull;
+    int64_t base = G.rowStart[r];
+    for (int f0 = 1; f0 <= R.nfast; f0 += 64)
+    {
+        // the face values are loaded with the stencil code, before the ballot that finds the unknowns (every
+        // interior cell's faces exist), so both loads are in flight together
+        const int f = f0 + lane;
+        unsigned code = 0u;
+        double u0 = 0.0, u1 = 0.0, v0 = 0.0, v1 = 0.0, w0 = 0.0, w1 = 0.0;
+        if (f <= R.nfast)
+        {
+            code = G.c8[R.pitched + f];
+            const int i = R.i;
+            if (DIM == 3)
+            {
+                const int j = R.j, k = f;
+                u0 = u[((int64_t)i * (Nx + 1) + j) * Nz + k];
+                u1 = u[((int64_t)i * (Nx + 1) + j + 1) * Nz + k];
+                v0 = v[((int64_t)i * Nx + j) * Nz + k];
+                v1 = v[((int64_t)(i + 1) * Nx + j) * Nz + k];
+                const int64_t wi = ((int64_t)i * Nx + j) * (Nz + 1) + k;
+                w0 = w[wi];
+                w1 = w[wi + 1];
+            }
+            else
+            {
+                const int j = f;
+                u0 = u[(int64_t)i * (Nx + 1) + j];
+                u1 = u[(int64_t)i * (Nx + 1) + j + 1];
+                v0 = v[(int64_t)i * Nx + j];
+                v1 = v[(int64_t)(i + 1) * Nx + j];
+            }
+        }
+        const bool unk = (code & 64u) != 0u;
+        const uint64_t m = __ballot(unk);
+        const int64_t id = base + __popcll(m & below);
+        base += __popcll(m);
+        if (!unk) continue;
         double div, vol, ds = 0.0;
         const int64_t pc = R.pitched + f;
         if (DIM == 3)
         {
             const int i = R.i, j = R.j, k = f;
-            const double u0 = u[((int64_t)i * (Nx + 1) + j) * Nz + k], u1 = u[((int64_t)i * (Nx + 1) + j + 1) * Nz + k];
-            const double v0 = v[((int64_t)i * Nx + j) * Nz + k], v1 = v[((int64_t)(i + 1) * Nx + j) * Nz + k];
-            const int64_t wi = ((int64_t)i * Nx + j) * (Nz + 1) + k;
-            const double w0 = w[wi], w1 = w[wi + 1];
-            if (G.nonUniform)
+            if (NU)
             {
                 // MAC::GetDivergencyAt, stretched: the net face flux over the cell volume (MAC.cpp:353-360)
                 const double dx = G.wx[j], dy = G.wy[i], dz = G.wz[k];
@@ -158,12 +193,12 @@ __global__ __launch_bounds__(kThreads) void k_rhs(FieldGeom G, const double *__r
                 const int64_t off[6] = {G.B.sS, -G.B.sS, G.B.sM, -G.B.sM, 1, -1};
                 const int64_t moff[6] = {(int64_t)Nx * Nz, -(int64_t)Nx * Nz, Nz, -Nz, 1, -1};
                 const int64_t c = ((int64_t)i * Nx + j) * Nz + k;
-#pragma unroll
+#pragma unroll 1
                 for (int a = 0; a < 6; a++)
                 {
                     if (G.types[pc + off[a]] != kPressure) continue;
                     double cq = 1.0;
-                    if (G.nonUniform)
+                    if (NU)
                     {
                         const double ax = G.wy[i] * G.wz[k], ay = G.wx[j] * G.wz[k], az = G.wx[j] * G.wy[i];
                         cq = a == 0   ? ay / face_gap(G.wy, i)
@@ -185,9 +220,7 @@ __global__ __launch_bounds__(kThreads) void k_rhs(FieldGeom G, const double *__r
         else
         {
             const int i = R.i, j = f;
-            const double u0 = u[(int64_t)i * (Nx + 1) + j], u1 = u[(int64_t)i * (Nx + 1) + j + 1];
-            const double v0 = v[(int64_t)i * Nx + j], v1 = v[(int64_t)(i + 1) * Nx + j];
-            if (G.nonUniform)
+            if (NU)
             {
                 const double dx = G.wx[j], dy = G.wy[i];
                 const double fu = (u1 - u0) * dy, fv = (v1 - v0) * dx;
@@ -206,7 +239,7 @@ __global__ __launch_bounds__(kThreads) void k_rhs(FieldGeom G, const double *__r
                 const int64_t off[4] = {G.B.sM, -G.B.sM, 1, -1};
                 const int64_t moff[4] = {Nx, -Nx, 1, -1};
                 const int64_t c = (int64_t)i * Nx + j;
-#pragma unroll
+#pragma unroll 1
                 for (int a = 0; a < 4; a++)
                 {
                     if (G.types[pc + off[a]] != kPressure) continue;
@@ -221,7 +254,7 @@ __global__ __launch_bounds__(kThreads) void k_rhs(FieldGeom G, const double *__r
             }
         }
         b[id] = rhoDt * div * vol - scale * ds;
-    });
+    }
 }
 
 // ---- projection --------------------------------------------------------------------------------------------
@@ -307,8 +340,7 @@ __global__ __launch_bounds__(kThreads) void k_project(FieldGeom G, double *__res
 // the passes run in the list's order, and a later pass reads what an earlier one wrote (edges and corners).
 // Thread q covers cell (a, b) of the face plane, on side q & 1.
 __global__ __launch_bounds__(256) void k_ring_pass(FieldGeom G, int axis, double *__restrict__ p,
-                                                   const double *__restrict__ prescribed,
-                                                   const double *__restrict__ pOldBase, int64_t count)
+                                                   const double *__restrict__ prescribed, int64_t count)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= count) return;
@@ -343,11 +375,7 @@ __global__ __launch_bounds__(256) void k_ring_pass(FieldGeom G, int axis, double
     const int64_t dst = ((int64_t)i * Nx + j) * Nz + k, src = ((int64_t)si * Nx + sj) * Nz + sk;
     const uint8_t t = G.dim == 3 ? G.types[G.B.at(i, j, k)] : G.types[G.B.at(0, i, j)];
     if (t == kEmpty) p[dst] = 0.0;
-    else if (t == kPressure)
-    {
-        if (prescribed) p[dst] = prescribed[dst];
-        else if (pOldBase) p[dst] = pOldBase[dst] + p[dst];
-    }
+    else if (t == kPressure) { if (prescribed) p[dst] = prescribed[dst]; }
     else p[dst] = p[src];
 }
 
@@ -367,8 +395,9 @@ __global__ __launch_bounds__(kThreads) void k_scatter_p(FieldGeom G, const doubl
     });
 }
 
-// Every cell of the box, one wave per (i, j) row (2D: per i row), lanes along the fast axis.
-template <int DIM>
+// Every cell of the box, one wave per (i, j) row (2D: per i row), lanes along the fast axis. RESTORE = false:
+// p -= pOld at every PRESSURE cell; RESTORE = true: p = pOld + p at the PRESSURE cells of the ring only.
+template <int DIM, bool RESTORE>
 __global__ __launch_bounds__(kThreads) void k_ring_fix(FieldGeom G, const double *__restrict__ pOld,
                                                       double *__restrict__ p, int rows)
 {
@@ -377,12 +406,15 @@ __global__ __launch_bounds__(kThreads) void k_ring_fix(FieldGeom G, const double
     if (r >= rows) return;
     const int lane = threadIdx.x & 63;
     const int nf = DIM == 3 ? G.Nz : G.Nx;
-    const int64_t pitched = DIM == 3 ? G.B.at(r / G.Nx, r % G.Nx, 0) : G.B.at(0, r, 0);
+    const int i = DIM == 3 ? r / G.Nx : r, j = DIM == 3 ? r % G.Nx : 0;
+    const int64_t pitched = DIM == 3 ? G.B.at(i, j, 0) : G.B.at(0, i, 0);
+    const bool ringRow = DIM == 3 ? (i == 0 || i == G.Ny - 1 || j == 0 || j == G.Nx - 1) : (i == 0 || i == G.Ny - 1);
     for (int f = lane; f < nf; f += 64)
         if (G.types[pitched + f] == kPressure)
         {
             const int64_t c = (int64_t)r * nf + f;
-            p[c] = p[c] - pOld[c];
+            if (!RESTORE) p[c] = p[c] - pOld[c];
+            else if (ringRow || f == 0 || f == nf - 1) p[c] = pOld[c] + p[c];
         }
 }
 
@@ -467,6 +499,36 @@ __global__ __launch_bounds__(kThreads) void k_scale_volume(FieldGeom G, const do
     });
 }
 
+// EnsureSystem's layout comparison (PressureSettings.h:300-303): every cell of the box, one wave per (i, j) row.
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void k_diff_types(FieldGeom G, const uint8_t *__restrict__ t, int rows,
+                                                        double *__restrict__ partials)
+{
+    const int lb = xcd_block();
+    const int r = lb * kRowsPerBlock + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    double acc = 0.0, bad = 0.0;
+    if (r < rows)
+    {
+        const int nf = DIM == 3 ? G.Nz : G.Nx;
+        const int64_t pitched = DIM == 3 ? G.B.at(r / G.Nx, r % G.Nx, 0) : G.B.at(0, r, 0);
+        for (int f = lane; f < nf; f += 64)
+        {
+            const uint8_t v = t[(int64_t)r * nf + f];
+            acc += v != G.types[pitched + f] ? 1.0 : 0.0;
+            bad += v > 5 ? 1.0 : 0.0;
+        }
+    }
+    const double s = block_sum(acc);
+    __syncthreads();   // block_sum's LDS is used again
+    const double sb = block_sum(bad);
+    if (threadIdx.x == 0)
+    {
+        partials[lb] = s;
+        partials[gridDim.x + lb] = sb;
+    }
+}
+
 // ---- linear reductions -------------------------------------------------------------------------------------
 constexpr int kLinearBlocks = 1024;
 
@@ -512,12 +574,36 @@ int blocks_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(k
 
 int field_row_blocks(const FieldGeom &G) { return std::max(1, (G.rows + kRowsPerBlock - 1) / kRowsPerBlock); }
 
+int field_box_blocks(const FieldGeom &G)
+{
+    const int rows = G.dim == 3 ? G.Ny * G.Nx : G.Ny;
+    return std::max(1, (rows + kRowsPerBlock - 1) / kRowsPerBlock);
+}
+
+hipError_t launch_diff_types(hipStream_t s, const FieldGeom &G, const uint8_t *newTypes, double *partials, int *blocks)
+{
+    const int rows = G.dim == 3 ? G.Ny * G.Nx : G.Ny;
+    *blocks = field_box_blocks(G);
+    if (G.dim == 3) k_diff_types<3><<<*blocks, kThreads, 0, s>>>(G, newTypes, rows, partials);
+    else k_diff_types<2><<<*blocks, kThreads, 0, s>>>(G, newTypes, rows, partials);
+    return hipGetLastError();
+}
+
 hipError_t launch_rhs(hipStream_t s, const FieldGeom &G, const double *u, const double *v, const double *w,
                       const double *p, const double *pOld, double rhoDt, double scale, double *b)
 {
     if (G.n == 0) return hipSuccess;
-    if (G.dim == 3) k_rhs<3><<<field_row_blocks(G), kThreads, 0, s>>>(G, u, v, w, p, pOld, rhoDt, scale, b);
-    else k_rhs<2><<<field_row_blocks(G), kThreads, 0, s>>>(G, u, v, w, p, pOld, rhoDt, scale, b);
+    const int nb = field_row_blocks(G);
+    if (G.dim == 3)
+    {
+        if (G.nonUniform) k_rhs<3, true><<<nb, kThreads, 0, s>>>(G, u, v, w, p, pOld, rhoDt, scale, b);
+        else k_rhs<3, false><<<nb, kThreads, 0, s>>>(G, u, v, w, p, pOld, rhoDt, scale, b);
+    }
+    else
+    {
+        if (G.nonUniform) k_rhs<2, true><<<nb, kThreads, 0, s>>>(G, u, v, w, p, pOld, rhoDt, scale, b);
+        else k_rhs<2, false><<<nb, kThreads, 0, s>>>(G, u, v, w, p, pOld, rhoDt, scale, b);
+    }
     return hipGetLastError();
 }
 
@@ -531,8 +617,7 @@ hipError_t launch_project(hipStream_t s, const FieldGeom &G, double *u, double *
     return hipGetLastError();
 }
 
-hipError_t launch_pressure_bc(hipStream_t s, const FieldGeom &G, double *p, const double *prescribed,
-                              const double *pOldBase)
+hipError_t launch_pressure_bc(hipStream_t s, const FieldGeom &G, double *p, const double *prescribed)
 {
     const int passes = G.dim == 3 ? 3 : 2;
     for (int a = 0; a < passes; a++)
@@ -541,7 +626,7 @@ hipError_t launch_pressure_bc(hipStream_t s, const FieldGeom &G, double *p, cons
         if (G.dim == 3) plane = a == 0 ? (int64_t)G.Nx * G.Nz : (a == 1 ? (int64_t)G.Ny * G.Nz : (int64_t)G.Ny * G.Nx);
         else plane = a == 0 ? G.Nx : G.Ny;
         const int64_t count = 2 * plane;
-        k_ring_pass<<<(unsigned)((count + 255) / 256), 256, 0, s>>>(G, a, p, prescribed, pOldBase, count);
+        k_ring_pass<<<(unsigned)((count + 255) / 256), 256, 0, s>>>(G, a, p, prescribed, count);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -556,12 +641,20 @@ hipError_t launch_scatter_p(hipStream_t s, const FieldGeom &G, const double *x, 
     return hipGetLastError();
 }
 
-hipError_t launch_ring_fix(hipStream_t s, const FieldGeom &G, const double *pOld, double *p)
+hipError_t launch_ring_fix(hipStream_t s, const FieldGeom &G, const double *pOld, double *p, bool restore)
 {
     const int rows = G.dim == 3 ? G.Ny * G.Nx : G.Ny;
     const int blocks = (rows + kRowsPerBlock - 1) / kRowsPerBlock;
-    if (G.dim == 3) k_ring_fix<3><<<blocks, kThreads, 0, s>>>(G, pOld, p, rows);
-    else k_ring_fix<2><<<blocks, kThreads, 0, s>>>(G, pOld, p, rows);
+    if (G.dim == 3)
+    {
+        if (restore) k_ring_fix<3, true><<<blocks, kThreads, 0, s>>>(G, pOld, p, rows);
+        else k_ring_fix<3, false><<<blocks, kThreads, 0, s>>>(G, pOld, p, rows);
+    }
+    else
+    {
+        if (restore) k_ring_fix<2, true><<<blocks, kThreads, 0, s>>>(G, pOld, p, rows);
+        else k_ring_fix<2, false><<<blocks, kThreads, 0, s>>>(G, pOld, p, rows);
+    }
     return hipGetLastError();
 }
 

@@ -54,6 +54,12 @@ struct nppe_solver
 
     std::vector<int32_t> orderHost;   // unpitched box index of each unknown
     int32_t *dOrder = nullptr;        // pitched box index of each unknown
+    int64_t unknownCap = 0;           // entries of dOrder and the IDP staging vectors: the box's interior cells
+    uint8_t *dLayoutStage = nullptr;  // nppe_update_layout: the new cell types (MAC order)
+    long long *dLayoutInfo = nullptr; // its unknown counts per level (and Dirichlet-face counts), then the total
+    bool chainStopsHere = false;      // coarsening stopped at the last level for a reason the layout cannot change
+    bool hostStale = false;           // the host mirror of the layout (levels' types, orderHost) predates an update
+    double sigma = 0.0;               // A-units of the volume-form coefficients (see build)
     std::vector<nppe::DevLevel> levels;
     double *dTab64 = nullptr;
     nppe::Tables64 t64{};
@@ -174,8 +180,13 @@ hipError_t dalloc_pairs(nppe_solver *h, T **p, size_t count)
     return e;
 }
 
-// Field-kernel data of a committed grid (step.cpp); called at the end of the commit.
+// Slot of the handle's step scalars (dStepScal) that holds nppe_update_layout_device's count of changed cells.
+constexpr int kStepScalDiff = 5;
+
+// Field-kernel data of a committed grid (step.cpp); called at the end of the commit. refresh_fields recomputes the
+// parts that depend on the layout (after nppe_update_layout).
 int build_fields(nppe_solver *h);
+int refresh_fields(nppe_solver *h);
 // The solve on device vectors (solver.cpp).
 int solve_device(nppe_solver *h, const double *dB, double *dX, const double *dGuess, double refNorm, nppe_result *res);
 

@@ -1,0 +1,35 @@
+// Launchers of the layout kernels (layout.hip): a new cell layout of the committed box made into every multigrid
+// level's types and stencil codes, the IDP numbering and the counts that decide the hierarchy's shape, on the
+// device. Every launcher enqueues on the given stream and returns the HIP status of the launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fields.h"
+#include "nppe_internal.h"
+
+namespace nppe
+{
+
+// Coarse types of level c from level f (cell-centred 2:1 coarsening, hierarchy.h coarsen): a coarse cell is EMPTY if
+// any child is EMPTY/PRESSURE, else FLUID if any child is FLUID, else SOLID. Every cell of the coarse box, ring
+// included; the pitch padding is not written.
+hipError_t launch_coarsen_types(hipStream_t s, const Box &Bf, const uint8_t *tf, const Box &Bc, uint8_t *tc);
+
+// code8 / code16 of every interior cell of a level from its types (hierarchy.h stencil_codes; zero for a cell that
+// is not an unknown).
+hipError_t launch_stencil_codes(hipStream_t s, const Box &B, const uint8_t *t, uint8_t *c8, uint16_t *c16);
+
+// counts[0] += unknowns of the level (code8 bit 6), counts[1] += unknowns with a Dirichlet face (bit 7), with
+// partial sums per block in partials (at most 2 * kMaxCountBlocks entries) finished by launch_finish_counts.
+constexpr int kMaxCountBlocks = 1024;
+hipError_t launch_count_codes(hipStream_t s, const Box &B, const uint8_t *c8, unsigned long long *partials,
+                              int *blocks);
+hipError_t launch_finish_counts(hipStream_t s, const unsigned long long *partials, int blocks, long long *out);
+
+// The fine level's IDP numbering: per interior row the unknowns (code8 bit 6), their exclusive prefix sums into
+// G.rowStart (one block), then order[id] = the pitched box index of unknown id. *total gets the unknown count.
+hipError_t launch_number_rows(hipStream_t s, const FieldGeom &G, int32_t *rowStart, long long *total);
+hipError_t launch_write_order(hipStream_t s, const FieldGeom &G, int32_t *order);
+
+}  // namespace nppe

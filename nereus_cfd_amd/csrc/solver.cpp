@@ -20,6 +20,7 @@
 #include "handle.h"
 #include "hierarchy.h"
 #include "kernels.h"
+#include "layout.h"
 
 using namespace nppe;
 
@@ -77,6 +78,9 @@ void free_grid(nppe_solver *h)
     h->dCoarseM = nullptr;
     h->dCoarseOrder = nullptr;
     h->dPartials = nullptr;
+    h->dLayoutStage = nullptr;
+    h->dLayoutInfo = nullptr;
+    h->hostStale = false;
     h->dFieldTab = nullptr;
     h->dRowStart = nullptr;
     h->dStepScal = nullptr;
@@ -447,22 +451,10 @@ hipError_t choose_tiles(nppe_solver *h)
     return err;
 }
 
-int build(nppe_solver *h, const nppe_grid *g)
+// Unknown numbering and singularity (PressureSolver3D.cpp:133-172): interior FLUID cells in index order;
+// singular when no FLUID cell has a Dirichlet face (EMPTY/PRESSURE, or a FLUID ring cell).
+void number_unknowns(nppe_solver *h, const HostLevel &fine)
 {
-    free_grid(h);
-    const int dim = g->dim;
-    const int nz = dim == 2 ? 1 : g->nz;
-    h->dim = dim;
-    h->Nx = g->nx;
-    h->Ny = g->ny;
-    h->Nz = nz;
-    h->nonUniform = g->non_uniform != 0;
-    h->dh = g->dh;
-    HostLevel fine = fine_level(dim, g->nx, g->ny, nz, g->cell_types, g->width_x, g->width_y,
-                                dim == 3 ? g->width_z : nullptr);
-
-    // Unknown numbering and singularity (PressureSolver3D.cpp:133-172): interior FLUID cells in index order;
-    // singular when no FLUID cell has a Dirichlet face (EMPTY/PRESSURE, or a FLUID ring cell).
     h->orderHost.clear();
     bool dirichlet = false;
     {
@@ -489,14 +481,11 @@ int build(nppe_solver *h, const nppe_grid *g)
     }
     h->n = (int64_t)h->orderHost.size();
     h->singular = !dirichlet;
+}
 
-    // Scale factors: the reference's MatrixScale (1/dh^2 uniform, 1 stretched); sigma turns the volume-form
-    // coefficients area/gap into A-units (3D uniform: c = 1 * scale while area/gap = dh).
-    const bool unitCoef = dim == 3 && !h->nonUniform;
-    const double scale = h->nonUniform ? 1.0 : 1.0 / (g->dh * g->dh);
-    const double sigma = unitCoef ? scale / g->dh : scale;
-
-    // Levels: coarsen until the coarsest has at most kCoarsestTarget unknowns or stops shrinking.
+// Levels: coarsen until the coarsest has at most kCoarsestTarget unknowns or stops shrinking.
+std::vector<HostLevel> coarsen_chain(HostLevel fine)
+{
     std::vector<HostLevel> hl;
     hl.push_back(std::move(fine));
     while ((int)hl.size() < kMaxLevels && hl.back().unknowns > kCoarsestTarget)
@@ -507,6 +496,106 @@ int build(nppe_solver *h, const nppe_grid *g)
         if (c.unknowns == 0) break;
         hl.push_back(std::move(c));
     }
+    return hl;
+}
+
+// A level's cell types and stencil codes (pitched) into its device buffers.
+int upload_level_layout(nppe_solver *h, DevLevel &L)
+{
+    const HostLevel &H = L.host;
+    std::vector<uint8_t> pitched((size_t)L.box.cells, 1);
+    for (int s = 0; s < H.ns; s++)
+        for (int m = 0; m < H.nm; m++)
+            std::memcpy(&pitched[(size_t)L.box.at(s, m, 0)], &H.types[((size_t)s * H.nm + m) * H.nf], H.nf);
+    HIP_TRY(h, upload(h, L.types, pitched.data(), pitched.size()), "upload types");
+    std::vector<uint8_t> c8h;
+    std::vector<uint16_t> c16h;
+    stencil_codes(H, c8h, c16h);
+    std::vector<uint8_t> p8((size_t)L.box.cells, 0);
+    std::vector<uint16_t> p16((size_t)L.box.cells, 0);
+    for (int s = 0; s < H.ns; s++)
+        for (int m = 0; m < H.nm; m++)
+        {
+            const size_t src = ((size_t)s * H.nm + m) * H.nf, dst = (size_t)L.box.at(s, m, 0);
+            std::memcpy(&p8[dst], &c8h[src], H.nf);
+            std::memcpy(&p16[dst], &c16h[src], H.nf * sizeof(uint16_t));
+        }
+    HIP_TRY(h, upload(h, L.c8, p8.data(), p8.size()), "upload codes");
+    HIP_TRY(h, upload(h, L.c16, p16.data(), p16.size() * sizeof(uint16_t)), "upload codes");
+    return NPPE_OK;
+}
+
+int64_t interior_cells(const HostLevel &F)
+{
+    return (F.threeD ? (int64_t)(F.ns - 2) : 1) * (F.nm - 2) * (F.nf - 2);
+}
+
+// The IDP order as pitched box indices.
+int upload_order(nppe_solver *h)
+{
+    const DevLevel &F = h->levels[0];
+    std::vector<int32_t> pitched(h->orderHost.size());
+    for (size_t q = 0; q < pitched.size(); q++)
+    {
+        const int64_t c = h->orderHost[q];
+        const int f = (int)(c % F.host.nf), m = (int)((c / F.host.nf) % F.host.nm),
+                  s = (int)(c / ((int64_t)F.host.nf * F.host.nm));
+        pitched[q] = (int32_t)F.box.at(s, m, f);
+    }
+    if (!pitched.empty())
+        HIP_TRY(h, upload(h, h->dOrder, pitched.data(), pitched.size() * sizeof(int32_t)), "upload order");
+    return NPPE_OK;
+}
+
+// The coarsest level's dense pseudo-inverse and unknown order, when it has at most kMaxDenseCoarse unknowns.
+int upload_coarse(nppe_solver *h)
+{
+    const DevLevel &C = h->levels.back();
+    h->coarseDense = false;
+    h->nCoarse = 0;
+    if (C.host.unknowns > kMaxDenseCoarse) return NPPE_OK;
+    std::vector<int64_t> ord;
+    std::vector<double> M;
+    std::string e2;
+    if (!coarse_pseudo_inverse(C.host, h->sigma, ord, M, e2)) return fail(h, NPPE_ERR_INTERNAL, e2);
+    h->nCoarse = (int)ord.size();
+    std::vector<int32_t> pitched(ord.size());
+    for (size_t q = 0; q < ord.size(); q++)
+    {
+        const int64_t c = ord[q];
+        const int f = (int)(c % C.host.nf), m = (int)((c / C.host.nf) % C.host.nm),
+                  s = (int)(c / ((int64_t)C.host.nf * C.host.nm));
+        pitched[q] = (int32_t)C.box.at(s, m, f);
+    }
+    HIP_TRY(h, upload(h, h->dCoarseM, M.data(), M.size() * sizeof(double)), "upload coarse matrix");
+    HIP_TRY(h, upload(h, h->dCoarseOrder, pitched.data(), pitched.size() * sizeof(int32_t)), "upload coarse order");
+    h->coarseDense = true;
+    return NPPE_OK;
+}
+
+int build(nppe_solver *h, const nppe_grid *g)
+{
+    free_grid(h);
+    const int dim = g->dim;
+    const int nz = dim == 2 ? 1 : g->nz;
+    h->dim = dim;
+    h->Nx = g->nx;
+    h->Ny = g->ny;
+    h->Nz = nz;
+    h->nonUniform = g->non_uniform != 0;
+    h->dh = g->dh;
+    HostLevel fine = fine_level(dim, g->nx, g->ny, nz, g->cell_types, g->width_x, g->width_y,
+                                dim == 3 ? g->width_z : nullptr);
+
+    number_unknowns(h, fine);
+
+    // Scale factors: the reference's MatrixScale (1/dh^2 uniform, 1 stretched); sigma turns the volume-form
+    // coefficients area/gap into A-units (3D uniform: c = 1 * scale while area/gap = dh).
+    const bool unitCoef = dim == 3 && !h->nonUniform;
+    const double scale = h->nonUniform ? 1.0 : 1.0 / (g->dh * g->dh);
+    const double sigma = unitCoef ? scale / g->dh : scale;
+
+    std::vector<HostLevel> hl = coarsen_chain(std::move(fine));
     h->coarseNu = std::getenv("NPPE_COARSE_NU") ? std::atoi(std::getenv("NPPE_COARSE_NU")) : 0;
     h->useGraphs = !(std::getenv("NPPE_GRAPHS") && std::atoi(std::getenv("NPPE_GRAPHS")) == 0);
     h->levels.resize(hl.size());
@@ -518,34 +607,9 @@ int build(nppe_solver *h, const nppe_grid *g)
         L.box = make_box(H.ns, H.nm, H.nf, H.threeD);
         L.cellsBox = (double)H.ns * H.nm * H.nf;
         HIP_TRY(h, dalloc(h, &L.types, (size_t)L.box.cells), "allocate types");
-        {
-            std::vector<uint8_t> pitched((size_t)L.box.cells, 1);
-            for (int s = 0; s < H.ns; s++)
-                for (int m = 0; m < H.nm; m++)
-                    std::memcpy(&pitched[(size_t)L.box.at(s, m, 0)], &H.types[((size_t)s * H.nm + m) * H.nf], H.nf);
-            HIP_TRY(h, hipMemcpyAsync(L.types, pitched.data(), pitched.size(), hipMemcpyHostToDevice, h->stream),
-                    "upload types");
-            HIP_TRY(h, hipStreamSynchronize(h->stream), "upload types");
-        }
-        {
-            std::vector<uint8_t> c8h;
-            std::vector<uint16_t> c16h;
-            stencil_codes(H, c8h, c16h);
-            std::vector<uint8_t> p8((size_t)L.box.cells, 0);
-            std::vector<uint16_t> p16((size_t)L.box.cells, 0);
-            for (int s = 0; s < H.ns; s++)
-                for (int m = 0; m < H.nm; m++)
-                {
-                    const size_t src = ((size_t)s * H.nm + m) * H.nf, dst = (size_t)L.box.at(s, m, 0);
-                    std::memcpy(&p8[dst], &c8h[src], H.nf);
-                    std::memcpy(&p16[dst], &c16h[src], H.nf * sizeof(uint16_t));
-                }
-            HIP_TRY(h, dalloc_pairs(h, &L.c8, p8.size()), "allocate codes");
-            HIP_TRY(h, dalloc(h, &L.c16, p16.size()), "allocate codes");
-            HIP_TRY(h, upload(h, L.c8, p8.data(), p8.size()), "upload codes");
-            HIP_TRY(h, upload(h, L.c16, p16.data(), p16.size() * sizeof(uint16_t)),
-                    "upload codes");
-        }
+        HIP_TRY(h, dalloc_pairs(h, &L.c8, (size_t)L.box.cells), "allocate codes");
+        HIP_TRY(h, dalloc(h, &L.c16, (size_t)L.box.cells), "allocate codes");
+        if (int rc = upload_level_layout(h, L)) return rc;
         int off[9];
         std::vector<float> tab = tables32(H, sigma, off);
         HIP_TRY(h, dalloc(h, &L.tab, tab.size()), "allocate tables");
@@ -572,35 +636,15 @@ int build(nppe_solver *h, const nppe_grid *g)
             HIP_TRY(h, dalloc_pairs(h, &L.b, (size_t)L.box.cells), "allocate level b");
         }
     }
-    // Coarsest: dense pseudo-inverse when small enough.
+    // Coarsest: dense pseudo-inverse when small enough (buffers for the largest such level, so a layout update
+    // reuses them).
+    h->sigma = sigma;
+    if (h->levels.back().host.unknowns <= kMaxDenseCoarse)
     {
-        const DevLevel &C = h->levels.back();
-        h->coarseDense = false;
-        h->nCoarse = 0;
-        if (C.host.unknowns <= kMaxDenseCoarse)
-        {
-            std::vector<int64_t> ord;
-            std::vector<double> M;
-            std::string e2;
-            if (!coarse_pseudo_inverse(C.host, sigma, ord, M, e2)) return fail(h, NPPE_ERR_INTERNAL, e2);
-            h->nCoarse = (int)ord.size();
-            std::vector<int32_t> pitched(ord.size());
-            for (size_t q = 0; q < ord.size(); q++)
-            {
-                const int64_t c = ord[q];
-                const int f = (int)(c % C.host.nf), m = (int)((c / C.host.nf) % C.host.nm),
-                          s = (int)(c / ((int64_t)C.host.nf * C.host.nm));
-                pitched[q] = (int32_t)C.box.at(s, m, f);
-            }
-            HIP_TRY(h, dalloc(h, &h->dCoarseM, M.size()), "allocate coarse matrix");
-            HIP_TRY(h, dalloc(h, &h->dCoarseOrder, pitched.size()), "allocate coarse order");
-            HIP_TRY(h, upload(h, h->dCoarseM, M.data(), M.size() * sizeof(double)),
-                    "upload coarse matrix");
-            HIP_TRY(h, upload(h, h->dCoarseOrder, pitched.data(), pitched.size() * sizeof(int32_t)),
-                    "upload coarse order");
-            h->coarseDense = true;
-        }
+        HIP_TRY(h, dalloc(h, &h->dCoarseM, (size_t)kMaxDenseCoarse * kMaxDenseCoarse), "allocate coarse matrix");
+        HIP_TRY(h, dalloc(h, &h->dCoarseOrder, (size_t)kMaxDenseCoarse), "allocate coarse order");
     }
+    if (int rc = upload_coarse(h)) return rc;
     // Fine FP64 data.
     const DevLevel &F = h->levels[0];
     {
@@ -612,19 +656,10 @@ int build(nppe_solver *h, const nppe_grid *g)
         h->t64 = Tables64{h->dTab64 + off[0], h->dTab64 + off[1], h->dTab64 + off[2], h->dTab64 + off[3],
                           h->dTab64 + off[4], h->dTab64 + off[5], scale, unitCoef ? 1 : 0, F.c8, F.c16};
     }
-    {
-        std::vector<int32_t> pitched(h->orderHost.size());
-        for (size_t q = 0; q < pitched.size(); q++)
-        {
-            const int64_t c = h->orderHost[q];
-            const int f = (int)(c % F.host.nf), m = (int)((c / F.host.nf) % F.host.nm),
-                      s = (int)(c / ((int64_t)F.host.nf * F.host.nm));
-            pitched[q] = (int32_t)F.box.at(s, m, f);
-        }
-        HIP_TRY(h, dalloc(h, &h->dOrder, pitched.size()), "allocate order");
-        HIP_TRY(h, upload(h, h->dOrder, pitched.data(), pitched.size() * sizeof(int32_t)),
-                "upload order");
-    }
+    // every vector in IDP order is sized for the most unknowns the box can hold, so a layout update reuses them
+    h->unknownCap = std::max<int64_t>(1, interior_cells(F.host));
+    HIP_TRY(h, dalloc(h, &h->dOrder, (size_t)h->unknownCap), "allocate order");
+    if (int rc = upload_order(h)) return rc;
     const size_t cells = (size_t)F.box.cells;
     HIP_TRY(h, dalloc(h, &h->dX64[0], cells), "allocate x64");
     HIP_TRY(h, dalloc(h, &h->dX64[1], cells), "allocate x64");
@@ -656,7 +691,7 @@ int build(nppe_solver *h, const nppe_grid *g)
         HIP_TRY(h, dalloc(h, &h->dP64, cells), "allocate p64");
         HIP_TRY(h, dalloc(h, &h->dQ64, cells), "allocate q64");
     }
-    const size_t nn = (size_t)std::max<int64_t>(1, h->n);
+    const size_t nn = (size_t)h->unknownCap;
     HIP_TRY(h, dalloc(h, &h->dCompB, nn), "allocate staging b");
     HIP_TRY(h, dalloc(h, &h->dCompX, nn), "allocate staging x");
     HIP_TRY(h, dalloc(h, &h->dCompG, nn), "allocate staging guess");
@@ -668,15 +703,174 @@ int build(nppe_solver *h, const nppe_grid *g)
             most = std::max<int64_t>(most, (int64_t)L.box.tilesF * L.box.tilesM * ((L.box.ns - 2 + 1) / 2 + 1));
         // the fused <r, z> of the level-0 sweep keeps one partial per interior row and plane
         most = std::max<int64_t>(most, (int64_t)(h->levels[0].box.ns - 2) * (h->levels[0].box.nm - 2));
+        // the layout comparison writes two partials per four rows of the box
+        most = std::max<int64_t>(most, (int64_t)h->levels[0].box.ns * h->levels[0].box.nm / 4 + 1);
         h->partialsCap = most * 2;
     }
     HIP_TRY(h, dalloc(h, &h->dPartials, (size_t)h->partialsCap), "allocate partials");
+    // layout updates: the new types, the unknown counts of every level and the fine level's total
+    HIP_TRY(h, dalloc(h, &h->dLayoutStage, (size_t)h->Nx * h->Ny * h->Nz), "allocate layout staging");
+    HIP_TRY(h, dalloc(h, &h->dLayoutInfo, (size_t)2 * kMaxLevels + 1), "allocate layout counts");
+    {
+        // did coarsening stop for a reason the layout cannot change (the level cap, a box that no longer shrinks)?
+        const HostLevel &B = h->levels.back().host;
+        const int cs = B.threeD ? (B.ns - 2 + 1) / 2 + 2 : 1, cm = (B.nm - 2 + 1) / 2 + 2, cf = (B.nf - 2 + 1) / 2 + 2;
+        h->chainStopsHere = (int)h->levels.size() >= kMaxLevels || (cs == B.ns && cm == B.nm && cf == B.nf);
+    }
     HIP_TRY(h, dalloc(h, &h->dState, 1), "allocate state");
     HIP_TRY(h, hipStreamSynchronize(h->stream), "commit");
     HIP_TRY(h, choose_tiles(h), "smoother tiles");
     if (int rc = build_fields(h)) return rc;
     h->committed = true;
     return NPPE_OK;
+}
+
+// Zeroes the box work vectors: the flat kernels rely on zeros outside the unknowns, and a layout change moves the
+// unknowns.
+int zero_work(nppe_solver *h)
+{
+    const size_t cells = (size_t)h->levels[0].box.cells;
+    auto z = [&](void *p, size_t bytes) { return p ? hipMemsetAsync(p, 0, bytes, h->stream) : hipSuccess; };
+    float *f32[] = {h->dE, h->dR, h->dP, h->dQ, h->dP2, h->dZ, h->dZ2};
+    for (float *p : f32) HIP_TRY(h, z(p, cells * sizeof(float)), "clear work vectors");
+    double *f64[] = {h->dX64[0], h->dX64[1], h->dB64, h->dR64, h->dP64, h->dQ64};
+    for (double *p : f64) HIP_TRY(h, z(p, cells * sizeof(double)), "clear work vectors");
+    for (size_t l = 1; l < h->levels.size(); l++)
+    {
+        const DevLevel &L = h->levels[l];
+        const size_t c = (size_t)L.box.cells * sizeof(float);
+        HIP_TRY(h, z(L.x[0], c), "clear work vectors");
+        HIP_TRY(h, z(L.x[1], c), "clear work vectors");
+        HIP_TRY(h, z(L.b, c), "clear work vectors");
+    }
+    return NPPE_OK;
+}
+
+// The widths of the committed grid in MAC axis order (the levels keep them in slow/mid/fast order).
+void mac_widths(const nppe_solver *h, std::vector<double> &wx, std::vector<double> &wy, std::vector<double> &wz)
+{
+    const HostLevel &H = h->levels[0].host;
+    wx = h->dim == 3 ? H.w[1] : H.w[2];
+    wy = h->dim == 3 ? H.w[0] : H.w[1];
+    wz = h->dim == 3 ? H.w[2] : std::vector<double>(1, h->dh);
+}
+
+// Commits the committed box again in full with new cell types (host memory).
+int recommit(nppe_solver *h, const uint8_t *types)
+{
+    std::vector<double> wx, wy, wz;
+    mac_widths(h, wx, wy, wz);
+    nppe_grid g{};
+    g.dim = h->dim;
+    g.nx = h->Nx;
+    g.ny = h->Ny;
+    g.nz = h->Nz;
+    g.non_uniform = h->nonUniform ? 1 : 0;
+    g.dh = h->dh;
+    g.cell_types = types;
+    g.width_x = wx.data();
+    g.width_y = wy.data();
+    g.width_z = wz.data();
+    const int rc = build(h, &g);
+    if (rc)
+    {
+        free_grid(h);
+        return rc;
+    }
+    return 2;
+}
+
+// The host mirror of the layout (every level's types, the IDP order) after a device update, read back on demand.
+int ensure_host_layout(nppe_solver *h)
+{
+    if (!h->hostStale) return NPPE_OK;
+    for (DevLevel &L : h->levels)
+    {
+        HostLevel &H = L.host;
+        HIP_TRY(h, hipMemcpy2DAsync(H.types.data(), (size_t)H.nf, L.types, (size_t)L.box.pitch, (size_t)H.nf,
+                                    (size_t)H.ns * H.nm, hipMemcpyDeviceToHost, h->stream),
+                "read the layout");
+    }
+    HIP_TRY(h, hipStreamSynchronize(h->stream), "read the layout");
+    number_unknowns(h, h->levels[0].host);
+    h->hostStale = false;
+    return NPPE_OK;
+}
+
+// A new layout of the committed box, in device memory (unpitched, MAC order), made into the hierarchy on the
+// device (nppe_update_layout): its types, every coarse level's types and every level's codes in the existing
+// buffers, the unknown counts that decide whether the hierarchy keeps its shape, the IDP numbering, and the
+// coarsest dense solve (on the host, from that level's types). Returns 1, or 2 after a full commit when the shape
+// changes (hostTypes: the same layout in host memory, or null to read it back).
+int layout_from_device(nppe_solver *h, const uint8_t *dTypes, const uint8_t *hostTypes)
+{
+    const int L = (int)h->levels.size();
+    DevLevel &F = h->levels[0];
+    HIP_TRY(h, hipMemcpy2DAsync(F.types, (size_t)F.box.pitch, dTypes, (size_t)F.box.nf, (size_t)F.box.nf,
+                                (size_t)F.box.ns * F.box.nm, hipMemcpyDeviceToDevice, h->stream),
+            "layout: types");
+    for (int l = 1; l < L; l++)
+        HIP_TRY(h, launch_coarsen_types(h->stream, h->levels[l - 1].box, h->levels[l - 1].types, h->levels[l].box,
+                                        h->levels[l].types),
+                "layout: coarsen");
+    unsigned long long *parts = reinterpret_cast<unsigned long long *>(h->dPartials);
+    for (int l = 0; l < L; l++)
+    {
+        DevLevel &D = h->levels[l];
+        HIP_TRY(h, launch_stencil_codes(h->stream, D.box, D.types, D.c8, D.c16), "layout: codes");
+        int nb = 0;
+        HIP_TRY(h, launch_count_codes(h->stream, D.box, D.c8, parts, &nb), "layout: counts");
+        HIP_TRY(h, launch_finish_counts(h->stream, parts, nb, h->dLayoutInfo + 2 * l), "layout: counts");
+    }
+    HIP_TRY(h, launch_number_rows(h->stream, h->fg, h->dRowStart, h->dLayoutInfo + 2 * L), "layout: numbering");
+    std::vector<long long> info((size_t)2 * L + 1, 0);
+    HIP_TRY(h, hipMemcpyAsync(info.data(), h->dLayoutInfo, info.size() * sizeof(long long), hipMemcpyDeviceToHost,
+                              h->stream),
+            "layout: counts");
+    HIP_TRY(h, hipStreamSynchronize(h->stream), "layout: counts");
+    // the hierarchy a fresh commit would build has the same levels iff coarsening continues through every level
+    // but the last and stops there (coarsen_chain's rule)
+    bool same = true;
+    for (int l = 0; l + 1 < L; l++) same = same && info[2 * l] > kCoarsestTarget && info[2 * (l + 1)] > 0;
+    same = same && (info[2 * (L - 1)] <= kCoarsestTarget || h->chainStopsHere);
+    same = same && (info[2 * (L - 1)] > kMaxDenseCoarse || h->dCoarseM);
+    if (!same)
+    {
+        std::vector<uint8_t> t;
+        if (!hostTypes)
+        {
+            t.resize((size_t)h->Nx * h->Ny * h->Nz);
+            HIP_TRY(h, hipMemcpyAsync(t.data(), dTypes, t.size(), hipMemcpyDeviceToHost, h->stream), "read the layout");
+            HIP_TRY(h, hipStreamSynchronize(h->stream), "read the layout");
+            hostTypes = t.data();
+        }
+        return recommit(h, hostTypes);
+    }
+    if (h->innerGraph)
+    {
+        (void)hipGraphExecDestroy(h->innerGraph);
+        h->innerGraph = nullptr;
+    }
+    h->rate = 0.0;
+    h->n = info[2 * L];
+    h->singular = info[1] == 0;
+    h->fg.n = h->n;
+    for (int l = 0; l < L; l++) h->levels[l].host.unknowns = info[2 * l];
+    HIP_TRY(h, launch_write_order(h->stream, h->fg, h->dOrder), "layout: order");
+    if (info[2 * (L - 1)] <= kMaxDenseCoarse)
+    {
+        DevLevel &C = h->levels.back();
+        HostLevel &H = C.host;
+        HIP_TRY(h, hipMemcpy2DAsync(H.types.data(), (size_t)H.nf, C.types, (size_t)C.box.pitch, (size_t)H.nf,
+                                    (size_t)H.ns * H.nm, hipMemcpyDeviceToHost, h->stream),
+                "layout: coarsest types");
+        HIP_TRY(h, hipStreamSynchronize(h->stream), "layout: coarsest types");
+    }
+    if (int rc = upload_coarse(h)) return rc;
+    if (int rc = zero_work(h)) return rc;
+    HIP_TRY(h, hipStreamSynchronize(h->stream), "layout update");
+    h->hostStale = true;
+    return 1;
 }
 
 int validate_grid(nppe_solver *h, const nppe_grid *g)
@@ -1181,6 +1375,57 @@ int64_t nppe_commit_grid(nppe_solver *h, const nppe_grid *g)
     return h->n;
 }
 
+int nppe_update_layout(nppe_solver *h, const uint8_t *cell_types)
+{
+    if (!h || !cell_types) return NPPE_ERR_ARGUMENT;
+    if (!h->committed) return fail(h, NPPE_ERR_NO_GRID, "no grid committed");
+    h->err.clear();
+    HIP_TRY(h, hipSetDevice(h->device), "set device");
+    const size_t cells = (size_t)h->Nx * h->Ny * h->Nz;
+    if (!h->hostStale && std::memcmp(cell_types, h->levels[0].host.types.data(), cells) == 0) return 0;
+    for (size_t c = 0; c < cells; c++)
+        if (cell_types[c] > 5) return fail(h, NPPE_ERR_ARGUMENT, "layout: cell type out of range (0..5)");
+    HIP_TRY(h, upload(h, h->dLayoutStage, cell_types, cells), "layout: upload");
+    if (h->hostStale)
+    {
+        int nb = 0;
+        HIP_TRY(h, launch_diff_types(h->stream, h->fg, h->dLayoutStage, h->dPartials, &nb), "compare layouts");
+        double *count = h->dStepScal + kStepScalDiff;
+        HIP_TRY(h, launch_sum_partials(h->stream, h->dPartials, nb, count), "compare layouts");
+        double differ = 0.0;
+        HIP_TRY(h, hipMemcpyAsync(&differ, count, sizeof(double), hipMemcpyDeviceToHost, h->stream),
+                "compare layouts");
+        HIP_TRY(h, hipStreamSynchronize(h->stream), "compare layouts");
+        if (differ == 0.0) return 0;
+    }
+    const int rc = layout_from_device(h, h->dLayoutStage, cell_types);
+    if (rc < 0 && !h->committed) free_grid(h);
+    return rc;
+}
+
+int nppe_update_layout_device(nppe_solver *h, const uint8_t *d_cell_types)
+{
+    if (!h || !d_cell_types) return NPPE_ERR_ARGUMENT;
+    if (!h->committed) return fail(h, NPPE_ERR_NO_GRID, "no grid committed");
+    h->err.clear();
+    HIP_TRY(h, hipSetDevice(h->device), "set device");
+    // EnsureSystem's comparison on the device: cells whose type differs from the committed one
+    int nb = 0;
+    HIP_TRY(h, launch_diff_types(h->stream, h->fg, d_cell_types, h->dPartials, &nb), "compare layouts");
+    // the partials hold the changed cells, then (from nb on) the cells whose code is out of range
+    double *count = h->dStepScal + kStepScalDiff;
+    HIP_TRY(h, launch_sum_partials(h->stream, h->dPartials, nb, count), "compare layouts");
+    HIP_TRY(h, launch_sum_partials(h->stream, h->dPartials + nb, nb, count + 1), "compare layouts");
+    double counts[2] = {0.0, 0.0};
+    HIP_TRY(h, hipMemcpyAsync(counts, count, sizeof(counts), hipMemcpyDeviceToHost, h->stream), "compare layouts");
+    HIP_TRY(h, hipStreamSynchronize(h->stream), "compare layouts");
+    if (counts[1] != 0.0) return fail(h, NPPE_ERR_ARGUMENT, "layout: cell type out of range (0..5)");
+    if (counts[0] == 0.0) return 0;
+    const int rc = layout_from_device(h, d_cell_types, nullptr);
+    if (rc < 0 && !h->committed) free_grid(h);
+    return rc;
+}
+
 int nppe_system_info(const nppe_solver *h, int64_t *n, int32_t *singular)
 {
     if (!h) return NPPE_ERR_ARGUMENT;
@@ -1194,6 +1439,7 @@ int nppe_unknown_order(const nppe_solver *h, int32_t *order)
 {
     if (!h || !order) return NPPE_ERR_ARGUMENT;
     if (!h->committed) return NPPE_ERR_NO_GRID;
+    if (int rc = ensure_host_layout(const_cast<nppe_solver *>(h))) return rc;
     std::memcpy(order, h->orderHost.data(), h->orderHost.size() * sizeof(int32_t));
     return NPPE_OK;
 }
@@ -1251,6 +1497,7 @@ int nppe_check_csr(nppe_solver *h, int32_t n, int64_t nnz, const int32_t *rowptr
     if (!h || !rowptr || (nnz > 0 && (!col || !val))) return NPPE_ERR_ARGUMENT;
     if (!h->committed) return fail(h, NPPE_ERR_NO_GRID, "no grid committed");
     if ((int64_t)n != h->n) return fail(h, NPPE_ERR_MISMATCH, "row count differs from the committed unknowns");
+    if (int rc = ensure_host_layout(h)) return rc;
     if (rowptr[0] != 0 || (int64_t)rowptr[n] != nnz) return fail(h, NPPE_ERR_ARGUMENT, "malformed rowptr");
     // Column j of the matrix-free operator is A e_j: probe with unit vectors grouped so that no two probes share
     // a row (cells 3 apart along every axis in each of the 27 residue classes).
@@ -1329,6 +1576,7 @@ uint64_t nppe_estimate_bytes_for(const nppe_grid *g, const nppe_options *opt)
             total += cells * (method == NPPE_METHOD_CG64 ? 24.0 : 28.0);   // FP64 r, p, q / FP32 e, r, p pair, q, z pair
             if (k64) total += cells * 8.0;                                  // FP64 Krylov r
             total += (double)(ns - (t3 ? 2 : 0)) * (nm - 2) * (nf - 2) * (24.0 + 4.0);   // staging + order
+            total += (double)ns * nm * 4.0;                                             // field row starts
         }
         const int si = ns - 2, mi = nm - 2, fi = nf - 2;
         const double interior = (t3 ? (double)si : 1.0) * mi * fi;
@@ -1337,7 +1585,7 @@ uint64_t nppe_estimate_bytes_for(const nppe_grid *g, const nppe_options *opt)
         nm = (mi + 1) / 2 + 2;
         nf = (fi + 1) / 2 + 2;
     }
-    total += (double)kMaxDenseCoarse * kMaxDenseCoarse * 8.0 / 4.0 + 1.0e6;   // coarse matrix, partials, state
+    total += (double)kMaxDenseCoarse * kMaxDenseCoarse * 8.0 + 1.0e6;   // coarse matrix, partials, state
     return (uint64_t)(total * 1.05);
 }
 

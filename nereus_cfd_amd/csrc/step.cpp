@@ -19,6 +19,7 @@ constexpr int kScalBC = 0;        // dStepScal slots: the two compatibility sums
 constexpr int kScalMeanX = 2;     // sum of x (LevelPressure of the Chorin scatter)
 constexpr int kScalMeanP = 3;     // sum of p_old + phi (LevelPressure of the accumulation)
 constexpr int kScalNorm = 4;      // ||b + A p^n||^2
+static_assert(kStepScalDiff == 5, "slot 5: the layout comparison's count (nppe_update_layout_device)");
 constexpr int kScalCount = 8;
 
 bool needs_compatibility(const nppe_solver *h) { return h->singular && h->opt.compatibility && h->n > 0; }
@@ -108,7 +109,9 @@ hipError_t accumulate(nppe_solver *h, const double *pOld, double *p, const doubl
         sum = h->dStepScal + kScalMeanP;
     }
     if (e == hipSuccess) e = launch_accumulate(h->stream, h->fg, pOld, sum, p);
-    if (e == hipSuccess) e = launch_pressure_bc(h->stream, h->fg, p, prescribed, prescribed ? nullptr : pOld);
+    // without prescribed values the PRESSURE ring cells get p_old + phi, p_b(t) to round-off
+    if (e == hipSuccess && !prescribed) e = launch_ring_fix(h->stream, h->fg, pOld, p, true);
+    if (e == hipSuccess) e = launch_pressure_bc(h->stream, h->fg, p, prescribed);
     return e;
 }
 
@@ -169,23 +172,8 @@ int nppe::build_fields(nppe_solver *h)
     else tab.push_back(h->dh);
     HIP_TRY(h, dalloc(h, &h->dFieldTab, tab.size()), "allocate field tables");
     HIP_TRY(h, upload(h, h->dFieldTab, tab.data(), tab.size() * sizeof(double)), "upload field tables");
-    // unknowns before each interior row, rows in IDP order (i, then j in 3D)
     const int rows = dim == 3 ? (Ny - 2) * (Nx - 2) : Ny - 2;
-    std::vector<int32_t> start((size_t)std::max(rows, 1), 0);
-    {
-        int64_t count = 0;
-        const int nfast = dim == 3 ? Nz : Nx;
-        for (int r = 0; r < rows; r++)
-        {
-            start[r] = (int32_t)count;
-            const int i = dim == 3 ? 1 + r / (Nx - 2) : 1 + r, j = dim == 3 ? 1 + r % (Nx - 2) : 0;
-            const size_t row = dim == 3 ? ((size_t)i * Nx + j) * Nz : (size_t)i * Nx;
-            for (int f = 1; f <= nfast - 2; f++) count += H.types[row + f] == 0 ? 1 : 0;
-        }
-        if (count != h->n) return fail(h, NPPE_ERR_INTERNAL, "row starts disagree with the unknown count");
-    }
-    HIP_TRY(h, dalloc(h, &h->dRowStart, start.size()), "allocate row starts");
-    HIP_TRY(h, upload(h, h->dRowStart, start.data(), start.size() * sizeof(int32_t)), "upload row starts");
+    HIP_TRY(h, dalloc(h, &h->dRowStart, (size_t)std::max(rows, 1)), "allocate row starts");
     HIP_TRY(h, dalloc(h, &h->dStepScal, kScalCount), "allocate step scalars");
     FieldGeom &G = h->fg;
     G.B = F.box;
@@ -202,10 +190,32 @@ int nppe::build_fields(nppe_solver *h)
     G.c8 = F.c8;
     G.rowStart = h->dRowStart;
     G.rows = rows;
-    G.n = h->n;
     // the row-walking reductions write one partial per block
-    if ((int64_t)field_row_blocks(G) > h->partialsCap || h->partialsCap < 1024)
+    if ((int64_t)field_row_blocks(G) > h->partialsCap || 2 * (int64_t)field_box_blocks(G) > h->partialsCap ||
+        h->partialsCap < 1024)
         return fail(h, NPPE_ERR_INTERNAL, "partials buffer too small for the field reductions");
+    return refresh_fields(h);
+}
+
+int nppe::refresh_fields(nppe_solver *h)
+{
+    // unknowns before each interior row, rows in IDP order (i, then j in 3D)
+    const int Nx = h->Nx, Ny = h->Ny, Nz = h->Nz, dim = h->dim;
+    const HostLevel &H = h->levels[0].host;
+    const int rows = h->fg.rows;
+    std::vector<int32_t> start((size_t)std::max(rows, 1), 0);
+    int64_t count = 0;
+    const int nfast = dim == 3 ? Nz : Nx;
+    for (int r = 0; r < rows; r++)
+    {
+        start[r] = (int32_t)count;
+        const int i = dim == 3 ? 1 + r / (Nx - 2) : 1 + r, j = dim == 3 ? 1 + r % (Nx - 2) : 0;
+        const size_t row = dim == 3 ? ((size_t)i * Nx + j) * Nz : (size_t)i * Nx;
+        for (int f = 1; f <= nfast - 2; f++) count += H.types[row + f] == 0 ? 1 : 0;
+    }
+    if (count != h->n) return fail(h, NPPE_ERR_INTERNAL, "row starts disagree with the unknown count");
+    HIP_TRY(h, upload(h, h->dRowStart, start.data(), start.size() * sizeof(int32_t)), "upload row starts");
+    h->fg.n = h->n;
     return NPPE_OK;
 }
 

@@ -199,6 +199,51 @@ inline int SolvePressureSystemHIP(HipPressureSolver *p, const std::vector<double
     return p->last.iterations;
 }
 
+// SolveLaplacian (PressureSolver3D.cpp:355-371): L x = f on the committed system, solved as A x = V f from zero.
+// False without solving when the system has no unknowns or f has the wrong size, as the reference returns.
+inline bool SolveLaplacianHIP(HipPressureSolver *p, const std::vector<double> &f, std::vector<double> &x)
+{
+    if (p->n == 0 || (int64_t)f.size() != p->n) return false;
+    x.resize((size_t)p->n);
+    const int rc = nppe_solve_laplacian(p->h, f.data(), x.data(), &p->last);
+    Check(rc, p->h, "SolveLaplacianHIP");
+    if (rc == NPPE_NOT_CONVERGED && !p->warned)
+    {
+        p->warned = true;
+        std::cerr << "Warning: the HIP Laplacian solve did not converge (relative residual "
+                  << p->last.relative_residual << "). Reported once." << std::endl;
+    }
+    return true;
+}
+
+// The pressure step of SolveScheme after its first EnforcePressureBC (PressureSolver3D.cpp:279-336) on MAC fields in
+// device memory: b from the velocity, the reference norm when d_pOld is given (incremental), the solve, the scatter
+// into d_p with the ring BC and the incremental ring fix. Returns the iterations (the step's counter).
+inline int PressureStepHIP(HipPressureSolver *p, const double *d_u, const double *d_v, const double *d_w,
+                           double *d_p, const double *d_pOld, double rho, double dt)
+{
+    const int rc = nppe_pressure_step_device(p->h, d_u, d_v, d_w, d_p, d_pOld, rho, dt, &p->last);
+    Check(rc, p->h, "PressureStepHIP");
+    if (rc == NPPE_NOT_CONVERGED && !p->warned)
+    {
+        p->warned = true;
+        std::cerr << "Warning: the HIP pressure solve did not converge (relative residual "
+                  << p->last.relative_residual << " after " << p->last.iterations
+                  << " iterations). Reported once." << std::endl;
+    }
+    return p->last.iterations;
+}
+
+// The projection step (Project, PressureSolver3D.cpp:396-401) on device-resident MAC fields: ProjectPressure, then
+// with d_pOld AccumulatePressureIncrement; d_prescribed (nullable) holds MAC::PrescribedPressure at the PRESSURE
+// ring cells.
+inline void ProjectionStepHIP(HipPressureSolver *p, double *d_u, double *d_v, double *d_w, double *d_p,
+                              const double *d_pOld, double rho, double dt, const double *d_prescribed)
+{
+    Check(nppe_projection_step_device(p->h, d_u, d_v, d_w, d_p, d_pOld, rho, dt, d_prescribed), p->h,
+          "ProjectionStepHIP");
+}
+
 }  // namespace nereus_hip
 
 #endif

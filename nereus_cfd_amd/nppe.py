@@ -38,7 +38,7 @@ EXPORTS = [
     "nppe_server_memory", "nppe_rhs_device", "nppe_rhs", "nppe_project_device", "nppe_project",
     "nppe_enforce_pressure_bc_device", "nppe_increment_reference_norm_device", "nppe_scatter_pressure_device",
     "nppe_accumulate_increment_device", "nppe_solve_laplacian_device", "nppe_solve_laplacian",
-    "nppe_pressure_step_device", "nppe_projection_step_device",
+    "nppe_pressure_step_device", "nppe_projection_step_device", "nppe_update_layout", "nppe_update_layout_device",
 ]
 
 
@@ -151,7 +151,8 @@ def lib():
                        ("nppe_accumulate_increment_device", [P, P, P, P]),
                        ("nppe_solve_laplacian_device", [P, P, P, P]), ("nppe_solve_laplacian", [P, P, P, P]),
                        ("nppe_pressure_step_device", [P, P, P, P, P, P, D, D, P]),
-                       ("nppe_projection_step_device", [P, P, P, P, P, P, D, D, P])):
+                       ("nppe_projection_step_device", [P, P, P, P, P, P, D, D, P]),
+                       ("nppe_update_layout", [P, P]), ("nppe_update_layout_device", [P, P])):
         getattr(L, name).argtypes = args
         getattr(L, name).restype = C.c_int
     _lib = L
@@ -271,6 +272,28 @@ class Solver:
         self.n, self.singular = int(n64.value), bool(sing.value)
         self.grid = grid
         return self.n
+
+    def update_layout(self, types) -> int:
+        """nppe_update_layout: 0 unchanged, 1 updated in place, 2 committed again."""
+        t = np.ascontiguousarray(types, np.uint8)
+        if t.shape != self.grid.types.shape:
+            raise ValueError(f"layout has shape {t.shape}, the committed box {self.grid.types.shape}")
+        rc = self._check(lib().nppe_update_layout(self.h, t.ctypes.data_as(C.c_void_p)), "update_layout")
+        self._after_layout(t)
+        return rc
+
+    def update_layout_device(self, types_ptr: int, types_host=None) -> int:
+        rc = self._check(lib().nppe_update_layout_device(self.h, C.c_void_p(types_ptr)), "update_layout_device")
+        if types_host is not None:
+            self._after_layout(np.ascontiguousarray(types_host, np.uint8))
+        return rc
+
+    def _after_layout(self, t):
+        n64, sing = C.c_int64(), C.c_int32()
+        lib().nppe_system_info(self.h, C.byref(n64), C.byref(sing))
+        self.n, self.singular = int(n64.value), bool(sing.value)
+        self.grid = Grid(self.grid.dim, t.copy(), self.grid.wx, self.grid.wy, self.grid.wz, self.grid.dh,
+                         self.grid.non_uniform)
 
     def order(self) -> np.ndarray:
         o = np.zeros(max(1, self.n), np.int32)

@@ -68,6 +68,15 @@ static void Dirichlet(int dim, int n)
     check(maxAbs > 0.1, "and the field is genuinely non-zero", buf);
     std::snprintf(buf, sizeof(buf), "drift = %.3e", drift);
     check(drift == 0.0, "repeating the solve is deterministic", buf);
+    // SolveLaplacian: on a uniform grid V = 1, so L x = b is the same system (PressureSolver3D.cpp:355-371)
+    std::vector<double> xl;
+    check(nereus_hip::SolveLaplacianHIP(p, b, xl), "SolveLaplacian solves on the committed system");
+    double dl = 0.0;
+    for (size_t q = 0; q < xl.size() && q < x1.size(); q++) dl = std::max(dl, std::fabs(xl[q] - x1[q]));
+    std::snprintf(buf, sizeof(buf), "max|x_laplacian - x_solve| = %.3e", dl);
+    check(xl.size() == x1.size() && dl == 0.0, "and gives the pressure solve's answer bit for bit", buf);
+    std::vector<double> shortF(b.size() > 0 ? b.size() - 1 : 0);
+    check(!nereus_hip::SolveLaplacianHIP(p, shortF, xl), "SolveLaplacian refuses an f of the wrong size");
     nereus_hip::DestroyPressureBackendHIP(p);
     check(p == nullptr, "destroy clears the handle");
 }

@@ -286,3 +286,58 @@ def test_projection_leaves_divergence_free_field(gpu, n):
         # after the projection, rho/dt div V = b - A p for the solved p
         assert after <= 1.05e-8 * before
         assert before > 0.0
+
+
+def _exact(x, y, t):
+    return (1.0 + t) * (0.3 + 1.1 * x - 0.7 * y)
+
+
+@pytest.mark.parametrize("dim,n", [(2, 40), (3, 20)])
+@pytest.mark.parametrize("with_prescribed", [True, False])
+def test_incremental_closed_form(gpu, dim, n, with_prescribed):
+    """The reference's incremental KAT (suite_pressure.cpp:236-325) through the device step: a PRESSURE ring
+    holding p_b = (1+t)(0.3 + 1.1x - 0.7y) (harmonic at every t), p^n the exact field at t = 0, zero velocity; one
+    incremental pressure step and projection step at t1 = 0.5 land on the exact field at t1 in the interior (the
+    reference's bar 1e-7) and put the ring back at p_b(t1) (1e-12)."""
+    import torch
+    say("incremental KAT", dim, n, with_prescribed)
+    dh = 1.0 / n
+    shape = (n + 2, n + 2, n + 2) if dim == 3 else (n + 2, n + 2, 1)
+    t = np.full(shape, N.PRESSURE, np.uint8)
+    if dim == 3:
+        t[1:-1, 1:-1, 1:-1] = N.FLUID
+    else:
+        t[1:-1, 1:-1, 0] = N.FLUID
+    w = np.full(n + 2, dh)
+    g = N.Grid(dim, t, w, w, w if dim == 3 else np.array([dh]), dh, False)
+    Ny, Nx, Nz = shape
+    pos = lambda q, Nq: 0.0 if q == 0 else (1.0 if q == Nq - 1 else (q - 0.5) * dh)   # MAC.cpp:597-601
+    X = np.array([pos(j, Nx) for j in range(Nx)])
+    Y = np.array([pos(i, Ny) for i in range(Ny)])
+    field = lambda tt: np.broadcast_to(_exact(X[None, :, None], Y[:, None, None], tt), shape).ravel().copy()
+    t1 = 0.5
+    p_old = field(0.0)
+    ring = np.ones(shape, bool)
+    if dim == 3:
+        ring[1:-1, 1:-1, 1:-1] = False
+    else:
+        ring[1:-1, 1:-1, 0] = False
+    ring = ring.ravel()
+    presc = np.where(ring, field(t1), 0.0)
+    with N.Solver(tolerance=1e-12, compatibility=0) as s:
+        s.commit(g)
+        nu, nv, nw, npp = s.field_sizes()
+        D = Dev(u=np.zeros(nu), v=np.zeros(nv), w=np.zeros(nw) if dim == 3 else None, p=p_old, p_old=p_old,
+                pr=presc)
+        # SolveScheme's first EnforcePressureBC at t1 (the caller's), then the step
+        s.enforce_pressure_bc_device(D.ptr("p"), D.ptr("pr"))
+        res, rc = s.pressure_step_device(D.ptr("u"), D.ptr("v"), D.ptr("w"), D.ptr("p"), D.ptr("p_old"), 1.0, 1e-3)
+        assert rc == N.OK
+        s.projection_step_device(D.ptr("u"), D.ptr("v"), D.ptr("w"), D.ptr("p"), D.ptr("p_old"), 1.0, 1e-3,
+                                 D.ptr("pr") if with_prescribed else 0)
+        p = D.host("p")
+    e = np.abs(p - field(t1))
+    say(f"  interior {e[~ring].max():.3e} ring {e[ring].max():.3e} iterations {res.iterations}")
+    assert e[~ring].max() < 1e-7
+    assert e[ring].max() < 1e-12
+    assert np.abs(p[~ring]).max() > 0.1
