@@ -286,18 +286,26 @@ __global__ __launch_bounds__(kThreads) void k_project(FieldGeom G, double *__res
     if (DIM == 3)
     {
         const int i = 1 + r / (Nx - 1), j = 1 + r % (Nx - 1);
-        for (int k = 1 + lane; k <= Nz - 1; k += 64)
+        // the faces k = 1 .. Nz-2 one per lane; the last w face (k = Nz-1) goes with the lane of k = Nz-2, so a
+        // row of Nz - 2 = 64 m cells takes m steps, not m + 1
+        for (int k = 1 + lane; k <= Nz - 2; k += 64)
         {
             const int64_t c = ((int64_t)i * Nx + j) * Nz + k, cp = G.B.at(i, j, k);
             const uint8_t tc = t[cp];
             const double pc = p[c];
-            if (i <= Ny - 2 && k <= Nz - 2 && tc != kSolid && t[cp - G.B.sM] != kSolid)
+            if (k == Nz - 2 && i <= Ny - 2 && j <= Nx - 2 && t[cp + 1] != kSolid && tc != kSolid)
+            {
+                const double gr = proj_grad(p[c + 1], pc, G.wz, Nz, Nz - 1, t[G.B.at(i, j, 0)], t[cp + 1]);
+                double *wf = &w[((int64_t)i * Nx + j) * (Nz + 1) + k + 1];
+                *wf = *wf - fac * gr;
+            }
+            if (i <= Ny - 2 && tc != kSolid && t[cp - G.B.sM] != kSolid)
             {
                 const double gr = proj_grad(pc, p[c - Nz], G.wx, Nx, j, t[G.B.at(i, 0, k)], t[G.B.at(i, Nx - 1, k)]);
                 double *uf = &u[((int64_t)i * (Nx + 1) + j) * Nz + k];
                 *uf = *uf - fac * gr;
             }
-            if (j <= Nx - 2 && k <= Nz - 2 && tc != kSolid && t[cp - G.B.sS] != kSolid)
+            if (j <= Nx - 2 && tc != kSolid && t[cp - G.B.sS] != kSolid)
             {
                 const double gr = proj_grad(pc, p[c - (int64_t)Nx * Nz], G.wy, Ny, i, t[G.B.at(0, j, k)],
                                             t[G.B.at(Ny - 1, j, k)]);
@@ -315,17 +323,23 @@ __global__ __launch_bounds__(kThreads) void k_project(FieldGeom G, double *__res
     else
     {
         const int i = 1 + r;
-        for (int j = 1 + lane; j <= Nx - 1; j += 64)
+        // the faces j = 1 .. Nx-2 one per lane; the last u face (j = Nx-1) goes with the lane of j = Nx-2
+        for (int j = 1 + lane; j <= Nx - 2; j += 64)
         {
             const int64_t c = (int64_t)i * Nx + j, cp = G.B.at(0, i, j);
             const uint8_t tc = t[cp];
             const double pc = p[c];
+            if (j == Nx - 2 && i <= Ny - 2 && t[cp + 1] != kSolid && tc != kSolid)
+            {
+                const double gr = proj_grad(p[c + 1], pc, G.wx, Nx, Nx - 1, t[G.B.at(0, i, 0)], t[cp + 1]);
+                u[(int64_t)i * (Nx + 1) + j + 1] = u[(int64_t)i * (Nx + 1) + j + 1] - fac * gr;
+            }
             if (i <= Ny - 2 && tc != kSolid && t[cp - 1] != kSolid)
             {
                 const double gr = proj_grad(pc, p[c - 1], G.wx, Nx, j, t[G.B.at(0, i, 0)], t[G.B.at(0, i, Nx - 1)]);
                 u[(int64_t)i * (Nx + 1) + j] = u[(int64_t)i * (Nx + 1) + j] - fac * gr;
             }
-            if (j <= Nx - 2 && tc != kSolid && t[cp - G.B.sM] != kSolid)
+            if (tc != kSolid && t[cp - G.B.sM] != kSolid)
             {
                 const double gr = proj_grad(pc, p[c - Nx], G.wy, Ny, i, t[G.B.at(0, 0, j)], t[G.B.at(0, Ny - 1, j)]);
                 v[(int64_t)i * Nx + j] = v[(int64_t)i * Nx + j] - fac * gr;

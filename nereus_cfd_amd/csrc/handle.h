@@ -85,6 +85,7 @@ struct nppe_solver
     hipGraphExec_t innerGraph = nullptr;   // one inner iteration, captured at the first solve
     double graphBytesPerIter = 0.0;
     double rate = 0.0;   // observed per-iteration reduction of the inner residual norm (0: none yet)
+    int lastFirstPass = 0;   // iterations of the last FP64 Krylov solve's first pass (the next one's prediction)
     std::vector<void *> allocations;
 
     // MAC-field kernels around the solve (step.cpp): geometry, MAC width tables, row starts of the IDP numbering,

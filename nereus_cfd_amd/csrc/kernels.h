@@ -17,16 +17,19 @@ namespace nppe
 hipError_t launch_scatter64(hipStream_t s, int64_t n, const int32_t *order, const double *src, double *box,
                             SolveState *st, int sumSlot, double *partials);
 // dst[q] = box[order[q]] - (meanSlot >= 0 ? st->sums[meanSlot] / n : 0).
+// gate: only once st->accepted (a true residual met the target).
 hipError_t launch_gather64(hipStream_t s, int64_t n, const int32_t *order, const double *box, double *dst,
-                           const SolveState *st, int meanSlot);
+                           const SolveState *st, int meanSlot, int gate = 0);
 // st->sums[slot] = sum of box[order[q]].
+// gate: only once st->accepted, into sums[2].
 hipError_t launch_sum64(hipStream_t s, int64_t n, const int32_t *order, const double *box, SolveState *st, int slot,
-                        double *partials);
+                        double *partials, int gate = 0);
 // Outer refinement step: xn = x + e (e may be null) is written to xout (a different buffer), then r = b - A xn in
 // FP64 with the reference's exact coefficients, stored as FP32 in r32 and/or FP64 in r64; st->sums[1] = ||r||^2.
+// gate: only once the Krylov loop is done (st->done); then also st->accepted = skip or ||r||^2 < acc2.
 hipError_t launch_update_residual64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty,
                                     const double *x, const float *e, double *xout, const double *b, float *r32,
-                                    double *r64, SolveState *st, double *partials);
+                                    double *r64, SolveState *st, double *partials, int gate = 0);
 // y = A x in FP64 (the reference's SpMV, row sums in ascending column order).
 hipError_t launch_apply64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const double *x,
                           double *y);
@@ -96,7 +99,9 @@ hipError_t launch_cg_update(hipStream_t s, const Level32 &L, float *e, float *r,
                             const float *q, SolveState *st, double *partials);
 // Inner-solve set-up from the outer residual norm st->sums[1]: thr2 = red^2 * sums[1], counters cleared.
 hipError_t launch_inner_setup(hipStream_t s, SolveState *st, double reduction, int maxIters);
-// FP64 CG set-up: thr2, iteration cap, counters cleared.
+// FP64 CG set-up: thr2, iteration cap, counters and the accepted flag cleared.
 hipError_t launch_state_init(hipStream_t s, SolveState *st, double thr2, int maxIters);
+// The FP64 Krylov set-up from the guess's true residual in st->sums (see k_krylov_init).
+hipError_t launch_krylov_init(hipStream_t s, SolveState *st, double tol, double ref, double aimFactor, int maxIters);
 
 }  // namespace nppe

@@ -388,6 +388,8 @@ enum FinOp
     FIN_SUM0 = 0,     // sums[0] = t0
     FIN_SUM1,         // sums[1] = t0
     FIN_SUM2,         // sums[2] = t0
+    FIN_TRUE_RES,     // only after the Krylov loop is done: sums[1] = t0; accepted = skip or t0 < acc2
+    FIN_SUM2_ACC,     // only once a true residual was accepted: sums[2] = t0
     FIN_RZ,           // beta = t0 / rz (0 on the first direction); rz = t0
     FIN_PQ,           // alpha = rz / t0; first = 0
     FIN_RR,           // rr = t0; iters++; done when rr < thr2 (1) or at the cap (2)
@@ -430,6 +432,8 @@ __device__ void fin_apply(SolveState *st, double *partials, int nblk, int nv, in
     if (tid != 0) return;
     __hip_atomic_store(&st->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (fin_in_loop(op) && st->done) return;
+    if (op == FIN_TRUE_RES && !st->done) return;
+    if (op == FIN_SUM2_ACC && !st->accepted) return;
     double t[2] = {0.0, 0.0};
     for (int a = 0; a < nv; a++)
         for (int q = 0; q < nw; q++) t[a] += sh[a][q];
@@ -438,6 +442,11 @@ __device__ void fin_apply(SolveState *st, double *partials, int nblk, int nv, in
     case FIN_SUM0: st->sums[0] = t[0]; break;
     case FIN_SUM1: st->sums[1] = t[0]; break;
     case FIN_SUM2: st->sums[2] = t[0]; break;
+    case FIN_TRUE_RES:
+        st->sums[1] = t[0];
+        st->accepted = (st->skip || t[0] < st->acc2) ? 1 : 0;
+        break;
+    case FIN_SUM2_ACC: st->sums[2] = t[0]; break;
     case FIN_RZ:
         st->beta = (st->first || st->rz == 0.0) ? 0.0 : t[0] / st->rz;
         st->rz = t[0];
@@ -501,16 +510,18 @@ __global__ __launch_bounds__(kBlockThreads) void k_scatter64(int64_t n, const in
 
 __global__ __launch_bounds__(kBlockThreads) void k_gather64(int64_t n, const int32_t *__restrict__ order,
                                                             const double *__restrict__ box, double *__restrict__ dst,
-                                                            const SolveState *st, int meanSlot)
+                                                            const SolveState *st, int meanSlot, int gate)
 {
+    if (gate && !st->accepted) return;
     const double mean = meanSlot >= 0 ? st->sums[meanSlot] / (double)n : 0.0;
     for (int64_t q = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x; q < n; q += (int64_t)gridDim.x * kBlockThreads)
         dst[q] = box[order[q]] - mean;
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_sum64(int64_t n, const int32_t *__restrict__ order,
-                                                         const double *__restrict__ box, Red red)
+                                                         const double *__restrict__ box, Red red, int gate)
 {
+    if (gate && !red.st->accepted) return;   // every block alike: no partial, no ticket
     double acc = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x; q < n; q += (int64_t)gridDim.x * kBlockThreads)
         acc += box[order[q]];
@@ -527,9 +538,10 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64(Box B, Tabl
                                                                      double *__restrict__ xout,
                                                                      const double *__restrict__ b,
                                                                      float *__restrict__ r32, double *__restrict__ r64,
-                                                                     Red red)
+                                                                     Red red, int gate)
 {
 #pragma clang fp contract(off)
+    if (gate && !red.st->done) return;   // every block alike: no partial, no ticket
     const TileRange tr = my_tiles(B.blocks());
     double rr = 0.0;
     for (int t = tr.begin; t < tr.end; t++)
@@ -2146,10 +2158,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64_t(Box B, Ta
                                                                        const double *__restrict__ b,
                                                                        float *__restrict__ r32,
                                                                        double *__restrict__ r64, Red red,
-                                                                       int P)
+                                                                       int P, int gate)
 {
 #pragma clang fp contract(off)
     __shared__ double Xs[kBlockMid + 2][kBlockFast + 2];
+    if (gate && !red.st->done) return;   // every block alike: no partial, no ticket
     const March M = march_setup(B, P);
     const int tx = threadIdx.x & (kBlockFast - 1), tyy = threadIdx.x / kBlockFast;
     const int f0 = M.f - tx, m0 = M.m - tyy;
@@ -2282,6 +2295,27 @@ __global__ void k_state_init(SolveState *st, double thr2, int maxIters)
     st->first = 1;
     st->maxIters = maxIters;
     st->done = maxIters <= 0 ? 2 : 0;
+    st->accepted = 0;
+    st->skip = 0;
+}
+
+// The FP64 Krylov solve's set-up from the true residual of the guess (sums[0] = |b|^2, sums[1] = |r|^2), on the
+// device so that the host needs no round trip first: target = tol * max(|b|, ref), the recursion aims at
+// aimFactor * target; done at once when the guess is the answer (b = 0, |b| <= tol * ref) or already meets the
+// target.
+__global__ void k_krylov_init(SolveState *st, double tol, double ref, double aimFactor, int maxIters)
+{
+    const double bn = sqrt(st->sums[0]), rn = sqrt(st->sums[1]);
+    const double target = tol * fmax(bn, ref), aim = aimFactor * target;
+    st->rr0 = st->sums[1];
+    st->acc2 = target * target;
+    st->thr2 = aim * aim;
+    st->iters = 0;
+    st->first = 1;
+    st->maxIters = maxIters;
+    st->accepted = 0;
+    st->skip = (bn == 0.0 || (ref > 0.0 && bn <= tol * ref)) ? 1 : 0;
+    st->done = (st->skip || rn < target) ? 1 : (maxIters <= 0 ? 2 : 0);
 }
 
 inline int grid_of(const Box &B) { return std::min(B.blocks(), kMaxGrid); }
@@ -2339,41 +2373,42 @@ hipError_t launch_scatter64(hipStream_t s, int64_t n, const int32_t *order, cons
 }
 
 hipError_t launch_gather64(hipStream_t s, int64_t n, const int32_t *order, const double *box, double *dst,
-                           const SolveState *st, int meanSlot)
+                           const SolveState *st, int meanSlot, int gate)
 {
     if (n <= 0) return hipSuccess;
-    k_gather64<<<grid_1d(n), kBlockThreads, 0, s>>>(n, order, box, dst, st, meanSlot);
+    k_gather64<<<grid_1d(n), kBlockThreads, 0, s>>>(n, order, box, dst, st, meanSlot, gate);
     return hipGetLastError();
 }
 
 hipError_t launch_sum64(hipStream_t s, int64_t n, const int32_t *order, const double *box, SolveState *st, int slot,
-                        double *partials)
+                        double *partials, int gate)
 {
     if (n <= 0) return hipSuccess;
     const int G = grid_1d(n);
-    const FinOp op = slot == 0 ? FIN_SUM0 : (slot == 1 ? FIN_SUM1 : FIN_SUM2);
+    const FinOp op = gate ? FIN_SUM2_ACC : (slot == 0 ? FIN_SUM0 : (slot == 1 ? FIN_SUM1 : FIN_SUM2));
     const Red red = red_of(partials, st, op, G);
-    k_sum64<<<G, kBlockThreads, 0, s>>>(n, order, box, red);
+    k_sum64<<<G, kBlockThreads, 0, s>>>(n, order, box, red, gate);
     return finish(s, red, G, 1);
 }
 
 hipError_t launch_update_residual64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty,
                                     const double *x, const float *e, double *xout, const double *b, float *r32,
-                                    double *r64, SolveState *st, double *partials)
+                                    double *r64, SolveState *st, double *partials, int gate)
 {
+    const FinOp op = gate ? FIN_TRUE_RES : FIN_SUM1;
     if (B.threeD)
     {
         const int P = march_planes(B), G = march_grid(B, P);
-        const Red red = red_of(partials, st, FIN_SUM1, G);
+        const Red red = red_of(partials, st, op, G);
         if (T.unitCoef)
-            k_update_residual64_t<true><<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, red, P);
+            k_update_residual64_t<true><<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, red, P, gate);
         else
-            k_update_residual64_t<false><<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, red, P);
+            k_update_residual64_t<false><<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, red, P, gate);
         return finish(s, red, G, 1);
     }
     const int G = grid_of(B);
-    const Red red = red_of(partials, st, FIN_SUM1, G);
-    k_update_residual64<<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, red);
+    const Red red = red_of(partials, st, op, G);
+    k_update_residual64<<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, red, gate);
     return finish(s, red, G, 1);
 }
 
@@ -2497,7 +2532,7 @@ hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float 
     {
         const int K = B.nf - 2 <= 128 ? 1 : (B.nf - 2 <= 256 ? 2 : 4);
         // 14-row tiles for K <= 2 when asked, but not for the prolongation-folding K = 2 sweep (1024-thread blocks
-        // cap its registers at 128, which it exceeds)
+        // cap its registers at 128, which it exceeds; 10-row tiles of it measured slower than 6-row ones at 256^3)
         const int TM = (tileRows >= 10 && (K == 1 || (K == 2 && !prolongFrom))) ? 14 : 6;
         const int tilesM = std::max(1, (B.nm - 2 + TM - 1) / TM);
         const int P = std::max(1, planesPerChunk);
@@ -2697,6 +2732,12 @@ hipError_t launch_inner_setup(hipStream_t s, SolveState *st, double reduction, i
 hipError_t launch_state_init(hipStream_t s, SolveState *st, double thr2, int maxIters)
 {
     k_state_init<<<1, 1, 0, s>>>(st, thr2, maxIters);
+    return hipGetLastError();
+}
+
+hipError_t launch_krylov_init(hipStream_t s, SolveState *st, double tol, double ref, double aimFactor, int maxIters)
+{
+    k_krylov_init<<<1, 1, 0, s>>>(st, tol, ref, aimFactor, maxIters);
     return hipGetLastError();
 }
 

@@ -79,6 +79,12 @@ struct SolveState
     int maxIters;     // inner cap
     int first;        // next direction update is the first of the solve (beta = 0)
     unsigned ticket;  // blocks of the running reduction that have written their partial (0 between kernels)
+    // FP64 Krylov solves scheduled without host round trips (solve_mgpcg): the starting <r, r>, the square of the
+    // target, whether the guess is the answer, and whether the last true residual met the target
+    double rr0;
+    double acc2;
+    int skip;
+    int accepted;
 };
 
 }  // namespace nppe

@@ -60,6 +60,7 @@ void free_grid(nppe_solver *h)
         h->innerGraph = nullptr;
     }
     h->rate = 0.0;
+    h->lastFirstPass = 0;
     {
         std::lock_guard<std::mutex> lock(gCaptureMu);
         for (void *p : h->allocations) (void)hipFree(p);
@@ -852,6 +853,7 @@ int layout_from_device(nppe_solver *h, const uint8_t *dTypes, const uint8_t *hos
         h->innerGraph = nullptr;
     }
     h->rate = 0.0;
+    h->lastFirstPass = 0;
     h->n = info[2 * L];
     h->singular = info[1] == 0;
     h->fg.n = h->n;
@@ -962,6 +964,172 @@ int run_inner(nppe_solver *h, double needRed, double rr0, Traffic &tr, int *ran)
     return NPPE_OK;
 }
 
+// The result of a solve whose events t0, t1 bracket its device work (t1 complete); destroys the events.
+int finish_result(nppe_solver *h, nppe_result *res, hipEvent_t t0, hipEvent_t t1, int inner, int passes, int status,
+                  double bnorm, double rnorm, double refNorm, const Traffic &tr)
+{
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (res)
+    {
+        const double denom = std::max(bnorm, refNorm);
+        res->iterations = inner;
+        res->refinements = passes;
+        res->status = status;
+        res->levels = (int)h->levels.size();
+        res->rhs_norm = bnorm;
+        res->residual_norm = rnorm;
+        res->relative_residual = denom > 0.0 ? rnorm / denom : rnorm;
+        res->device_ms = ms;
+        res->algorithmic_bytes = tr.bytes;
+    }
+    return status;
+}
+
+// The FP64 Krylov solve scheduled without host round trips: after the guess's true residual (already enqueued)
+// the device sets up the recursion itself (k_krylov_init); the host enqueues the iterations it predicts (the last
+// solve's count, or kInnerChunk), then the true residual and the output gather, both gated on the device (the
+// residual only once the recursion is done, the gather only once that residual met the target), and synchronises
+// once. A well-predicted solve makes that its only round trip; a short prediction enqueues more iterations, and a
+// missed target restarts CG from the new x as another pass. The kernels and their order are those of the
+// synchronous schedule, so the result is bit for bit the same.
+int krylov64_scheduled(nppe_solver *h, double refNorm, int cur, double *dX, bool project, Traffic &tr, int *passesOut,
+                       int *innerOut, int *statusOut, double *bnormOut, double *rnormOut)
+{
+    const DevLevel &L0 = h->levels[0];
+    const double N = L0.cellsBox, n = (double)h->n;
+    const bool graphs = h->useGraphs && !h->profiling;
+    if (graphs && !h->innerGraph)
+    {
+        hipGraph_t g = nullptr;
+        Traffic it;
+        std::lock_guard<std::mutex> lock(gCaptureMu);
+        HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+        const hipError_t ce = inner_iteration(h, it);
+        const hipError_t ee = hipStreamEndCapture(h->stream, &g);
+        if (ce != hipSuccess) return hip_fail(h, ce, "graph capture");
+        if (ee != hipSuccess) return hip_fail(h, ee, "graph capture");
+        const hipError_t ie = hipGraphInstantiate(&h->innerGraph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) return hip_fail(h, ie, "graph instantiate");
+        h->graphBytesPerIter = it.bytes;
+    }
+    const double tol = h->opt.tolerance;
+    HIP_TRY(h, launch_krylov_init(h->stream, h->dState, tol, refNorm, 0.8, h->opt.max_iters), "krylov setup");
+    auto predict = [&](double need) {
+        if (!(h->rate > 0.0 && h->rate < 1.0) || !(need < 1.0)) return kInnerChunk;
+        const double k = std::ceil(std::log(need) / std::log(h->rate));
+        return (int)std::max(1.0, std::min((double)kMaxChunk, k));
+    };
+    int chunk = h->lastFirstPass > 0 ? std::min(kMaxChunk, h->lastFirstPass) : kInnerChunk;
+    int passes = 0, inner = 0, status = NPPE_OK, launched = 0;
+    double bnorm = 0.0, rnorm = 0.0, rr0 = 0.0, perIter = 0.0;
+    bool first = true;
+    for (;;)
+    {
+        for (int q = 0; q < chunk; q++)
+        {
+            if (graphs)
+            {
+                HIP_TRY(h, hipGraphLaunch(h->innerGraph, h->stream), "graph launch");
+                perIter = h->graphBytesPerIter;
+            }
+            else
+            {
+                h->curIter = launched;
+                Traffic it;
+                HIP_TRY(h, inner_iteration(h, it), "inner iteration");
+                perIter = it.bytes;
+            }
+            launched++;
+        }
+        h->curIter = -1;
+        {
+            Scope sc(h, NPPE_K_RESIDUAL64, 29.0 * N);
+            HIP_TRY(h, launch_update_residual64(h->stream, L0.box, h->t64, L0.types, h->dX64[cur], nullptr, nullptr,
+                                                h->dB64, h->dR, h->dR64, h->dState, h->dPartials, 1),
+                    "true residual");
+        }
+        if (project)
+        {
+            Scope sc(h, NPPE_K_MEAN, 12.0 * n);
+            HIP_TRY(h, launch_sum64(h->stream, h->n, h->dOrder, h->dX64[cur], h->dState, 2, h->dPartials, 1), "mean");
+        }
+        {
+            Scope sc(h, NPPE_K_GATHER, 20.0 * n);
+            HIP_TRY(h, launch_gather64(h->stream, h->n, h->dOrder, h->dX64[cur], dX, h->dState, project ? 2 : -1, 1),
+                    "gather x");
+        }
+        if (int rc = sync_state(h)) return rc;
+        const SolveState &st = *h->hState;
+        profile_collect(h, st.iters);
+        if (first)
+        {
+            first = false;
+            bnorm = std::sqrt(st.sums[0]);
+            rr0 = st.rr0;
+            if (!std::isfinite(bnorm) || !std::isfinite(rr0))
+                return fail(h, NPPE_ERR_NONFINITE, "non-finite right-hand side or guess");
+            if (bnorm == 0.0)
+            {
+                // Eigen sets x = 0 for a zero right-hand side (ConjugateGradient.h:46-52)
+                HIP_TRY(h, launch_scatter64(h->stream, h->n, h->dOrder, nullptr, h->dX64[cur], h->dState, -1,
+                                            h->dPartials),
+                        "zero x");
+                HIP_TRY(h, launch_gather64(h->stream, h->n, h->dOrder, h->dX64[cur], dX, h->dState, -1), "gather x");
+                rnorm = 0.0;
+                break;
+            }
+        }
+        if (st.iters > 0 && st.rr > 0.0 && st.rr < rr0) h->rate = std::pow(st.rr / rr0, 0.5 / st.iters);
+        if (!st.done)
+        {
+            // the prediction fell short: more iterations of the same pass
+            chunk = st.rr > 0.0 && st.thr2 > 0.0 ? predict(std::sqrt(st.thr2 / st.rr)) : kInnerChunk;
+            continue;
+        }
+        inner += st.iters;
+        tr.add(perIter * st.iters + 29.0 * N);
+        rnorm = std::sqrt(st.sums[1]);
+        if (!std::isfinite(rnorm)) return fail(h, NPPE_ERR_NONFINITE, "non-finite residual during refinement");
+        // a pass that ran iterations counts; the guess accepted as it is does not (as the synchronous loop)
+        if (!(passes == 0 && st.iters == 0 && st.accepted)) passes++;
+        if (passes <= 1) h->lastFirstPass = st.iters;
+        if (st.accepted)
+        {
+            tr.add((project ? 12.0 * n : 0.0) + 20.0 * n);
+            break;
+        }
+        if (passes >= h->opt.max_refinements || inner >= h->opt.max_iters)
+        {
+            status = NPPE_NOT_CONVERGED;
+            if (project)
+            {
+                Scope sc(h, NPPE_K_MEAN, 12.0 * n);
+                HIP_TRY(h, launch_sum64(h->stream, h->n, h->dOrder, h->dX64[cur], h->dState, 2, h->dPartials), "mean");
+            }
+            HIP_TRY(h, launch_gather64(h->stream, h->n, h->dOrder, h->dX64[cur], dX, h->dState, project ? 2 : -1),
+                    "gather x");
+            tr.add((project ? 12.0 * n : 0.0) + 20.0 * n);
+            break;
+        }
+        // the true residual missed the target: CG again from the new x
+        const double aim = 0.8 * tol * std::max(bnorm, refNorm);
+        HIP_TRY(h, launch_state_init(h->stream, h->dState, aim * aim, h->opt.max_iters - inner), "krylov setup");
+        rr0 = rnorm * rnorm;
+        chunk = predict(std::min(0.5, aim / rnorm));
+        launched = 0;
+    }
+    *passesOut = passes;
+    *innerOut = inner;
+    *statusOut = status;
+    *bnormOut = bnorm;
+    *rnormOut = rnorm;
+    return NPPE_OK;
+}
+
 int solve_mgpcg(nppe_solver *h, const double *dB, double *dX, const double *dGuess, double refNorm, nppe_result *res)
 {
     const DevLevel &L0 = h->levels[0];
@@ -992,6 +1160,18 @@ int solve_mgpcg(nppe_solver *h, const double *dB, double *dX, const double *dGue
                 "residual");
     }
     tr.add((h->krylov64 ? 29.0 : 21.0) * N);
+    const bool project = h->singular && h->opt.compatibility && h->n > 0;
+    if (h->krylov64 && h->opt.max_iters > 0 && h->opt.max_refinements > 0)
+    {
+        int passes = 0, inner = 0, status = NPPE_OK;
+        double bnorm = 0.0, rnorm = 0.0;
+        if (int rc = krylov64_scheduled(h, refNorm, cur, dX, project, tr, &passes, &inner, &status, &bnorm, &rnorm))
+            return rc;
+        HIP_TRY(h, hipEventRecord(t1, h->stream), "event");
+        HIP_TRY(h, hipEventSynchronize(t1), "event");
+        profile_collect(h, 0);
+        return finish_result(h, res, t0, t1, inner, passes, status, bnorm, rnorm, refNorm, tr);
+    }
     int rc = sync_state(h);
     if (rc) return rc;
     profile_collect(h, 0);
@@ -1067,7 +1247,6 @@ int solve_mgpcg(nppe_solver *h, const double *dB, double *dX, const double *dGue
     }
     // Output: the pure-Neumann null mode (mean) projected out in FP64 when the system is singular and the
     // compatibility condition is on.
-    const bool project = h->singular && h->opt.compatibility && h->n > 0;
     if (project)
     {
         Scope sc(h, NPPE_K_MEAN, 12.0 * n);
