@@ -34,6 +34,7 @@ struct DevLevel
     Level32 v{};
     int tileRows = 6;        // smoother tile: tileRows rows per block (3D: 128 cells wide), planesPerChunk planes
     int planesPerChunk = 8;
+    int proPlanes = 8;       // planes per chunk of the sweep that folds the coarse correction in
     double cellsBox = 0.0;   // ns * nm * nf (the box, unpitched)
 };
 

@@ -239,9 +239,9 @@ hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel, doubl
             {
                 Scope sc(h, l == 0 ? NPPE_K_SMOOTH0 : NPPE_K_SMOOTH, by);
                 if ((e = launch_rbgs(h->stream, L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l), 1, 0,
-                                     L.tileRows, L.planesPerChunk, h->dState, dot ? rzPartials : nullptr,
-                                     fold ? &C.v : nullptr, fold ? xbuf(l + 1, cur[l + 1]) : nullptr)) !=
-                    hipSuccess)
+                                     L.tileRows, fold ? L.proPlanes : L.planesPerChunk, h->dState,
+                                     dot ? rzPartials : nullptr, fold ? &C.v : nullptr,
+                                     fold ? xbuf(l + 1, cur[l + 1]) : nullptr)) != hipSuccess)
                     return e;
             }
             if (dot)
@@ -354,7 +354,7 @@ struct TileKey
     }
 };
 std::mutex gTileMu;
-std::map<TileKey, std::pair<int, int>> gTiles;
+std::map<TileKey, std::tuple<int, int, int>> gTiles;   // tile rows, planes per chunk, planes of the folding sweep
 
 // Model: rounds of resident blocks (two per CU) times the plane steps of a block (P + 2).
 int model_planes(const Box &B, int family, int tm, int cus)
@@ -398,6 +398,7 @@ hipError_t choose_tiles(nppe_solver *h)
         L.planesPerChunk = model_planes(L.box, family, L.tileRows, cus);
         if (envTM) L.tileRows = std::max(1, std::atoi(envTM));
         if (envP) L.planesPerChunk = std::max(1, std::atoi(envP));
+        L.proPlanes = L.planesPerChunk;
         const int64_t interior = (int64_t)(L.box.threeD ? L.box.ns - 2 : 1) * (L.box.nm - 2) * (L.box.nf - 2);
         if (!tune || interior < kTuneCells) continue;
         const TileKey key{h->opt.device, L.box.ns, L.box.nm, L.box.nf, L.v.uniform, family};
@@ -406,37 +407,80 @@ hipError_t choose_tiles(nppe_solver *h)
             auto it = gTiles.find(key);
             if (it != gTiles.end())
             {
-                L.tileRows = it->second.first;
-                L.planesPerChunk = it->second.second;
+                L.tileRows = std::get<0>(it->second);
+                L.planesPerChunk = std::get<1>(it->second);
+                L.proPlanes = std::get<2>(it->second);
                 continue;
             }
         }
         if (!e0 && ((err = hipEventCreate(&e0)) != hipSuccess || (err = hipEventCreate(&e1)) != hipSuccess)) break;
-        static const int kPlanes[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32};
+        // planes per chunk: a fixed ladder, plus for each tile height the smallest chunk that keeps the launch
+        // within one to four rounds of blocks over the CUs (a launch a little over a round leaves most CUs idle
+        // for its last blocks)
+        auto candidates = [&](int tm) {
+            std::vector<int> ps = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32};
+            if (L.box.threeD)
+                for (int k = 1; k <= 4; k++)
+                    for (int P = 1; P <= L.box.ns - 2; P++)
+                        if (rbgs_blocks(L.box, family, tm, P) <= (int64_t)k * cus)
+                        {
+                            ps.push_back(P);
+                            break;
+                        }
+            std::sort(ps.begin(), ps.end());
+            ps.erase(std::unique(ps.begin(), ps.end()), ps.end());
+            std::vector<int> ok;
+            for (int P : ps)
+                if (L.box.threeD ? P <= std::max(1, L.box.ns - 2) : P == 1) ok.push_back(P);
+            return ok;
+        };
+        // mean time of kReps launches after one untimed
+        auto timeIt = [&](auto &&launch) {
+            constexpr int kReps = 4;
+            for (int r = 0; r <= kReps && err == hipSuccess; r++)
+            {
+                if (r == 1) err = hipEventRecord(e0, h->stream);
+                if (err == hipSuccess) err = launch();
+            }
+            if (err == hipSuccess) err = hipEventRecord(e1, h->stream);
+            if (err == hipSuccess) err = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+            return err == hipSuccess ? ms : 1e30f;
+        };
         float bestMs = 1e30f;
         int bestTM = L.tileRows, bestP = L.planesPerChunk;
         for (int tm : tms)
-            for (int P : kPlanes)
+            for (int P : candidates(tm))
             {
-                if (L.box.threeD ? P > std::max(1, L.box.ns - 2) : P != 1) continue;
-                constexpr int kReps = 4;
-                for (int r = 0; r <= kReps && err == hipSuccess; r++)
-                {
-                    if (r == 1) err = hipEventRecord(e0, h->stream);
-                    if (err == hipSuccess) err = launch_rbgs(h->stream, L.v, xin, xout, b, 1, 0, tm, P, h->dState);
-                }
-                if (err == hipSuccess) err = hipEventRecord(e1, h->stream);
-                if (err == hipSuccess) err = hipEventSynchronize(e1);
-                float ms = 0.f;
-                if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+                const float ms = timeIt([&] { return launch_rbgs(h->stream, L.v, xin, xout, b, 1, 0, tm, P, h->dState); });
                 if (err != hipSuccess) break;
                 if (ms < bestMs) { bestMs = ms; bestTM = tm; bestP = P; }
             }
         if (err != hipSuccess) break;
         L.tileRows = bestTM;
         L.planesPerChunk = bestP;
+        L.proPlanes = bestP;
+        // the first post-smoothing sweep folds the coarse correction in (whole-row family) and may run other
+        // tiles (see launch_rbgs): its chunk length is timed on its own
+        if (family == RBGS_ROW && l + 1 < h->levels.size() && h->levels[l + 1].x[0])
+        {
+            const DevLevel &C = h->levels[l + 1];
+            const int tmPro = (bestTM >= 10 && L.box.nf - 2 <= 128) ? 14 : 6;
+            float bestPro = 1e30f;
+            for (int P : candidates(tmPro))
+            {
+                const float ms = timeIt([&] {
+                    return launch_rbgs(h->stream, L.v, xin, xout, b, 1, 0, bestTM, P, h->dState, nullptr, &C.v,
+                                       C.x[0]);
+                });
+                if (err != hipSuccess) break;
+                if (ms < bestPro) { bestPro = ms; L.proPlanes = P; }
+            }
+            if (err != hipSuccess) break;
+        }
         std::lock_guard<std::mutex> lock(gTileMu);
-        gTiles[key] = {bestTM, bestP};
+        gTiles[key] = {bestTM, bestP, L.proPlanes};
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
