@@ -1177,13 +1177,23 @@ struct Prolong
 // As k_rbgs_march, but a wave spans the whole interior row: lane l owns the pairs (1 + 2p, 2 + 2p) for
 // p = l, l + 64, ... (K pairs), so no tile edge exists along the fast axis and no edge values are recomputed. The
 // block is TM rows plus one halo row above and below (halo waves publish first-colour values only). Plane bases are
-// scalar and per-lane offsets 32-bit; loads outside the interior are clamped into the box and their values masked
-// by the stencil codes, so the loop has no bounds branches. Uniform levels only (constant coefficients).
+// scalar and per-lane offsets 32-bit; loads outside the interior are clamped into the box, so the loop has no bounds
+// branches. Uniform levels only (constant coefficients).
 //
-// Every global load of a plane is issued one (b, codes, the halo waves' outer rows) or two (x) planes ahead of its
-// use: the old x of the mid-axis neighbour rows comes from the neighbouring waves through LDS (each wave publishes
-// its own row), and the halo waves, whose outer neighbour row belongs to no wave of the block, prefetch that row.
-// So no load is waited for in the plane that issues it.
+// An update adds all six neighbours without testing the code's neighbour bits. A neighbour whose bit is clear is a
+// ring cell or an interior cell that is no unknown, and every iterate holds +0 there (the buffers are cleared when a
+// layout is set, every sweep writes 0 at non-unknowns, the prolongation adds at unknowns only), so the sum equals
+// the sum over the coupled neighbours bit for bit. What the kernel makes up itself is zeroed where it stands for such
+// a cell: the pairs past the end of a short row, the LDS columns beyond the row ends, and (prolongation folded in)
+// the coarse correction at a fine cell that is no unknown. 1/n and the unknown mask come from a table in LDS
+// indexed by code & 127. The wave index is made wave-uniform and a row's colour parity alternates with the plane,
+// so the parity is a compile-time constant of each plane step: the kernel holds one plane loop per parity of the
+// first plane, and a wave runs the one its row needs.
+//
+// Every global load of a plane is issued two planes (x, b, codes) or one plane (the halo waves' outer rows) ahead
+// of its use: the old x of the mid-axis neighbour rows and of the neighbouring pairs comes from LDS (each wave
+// publishes its own row), and the halo waves, whose outer neighbour row belongs to no wave of the block, prefetch
+// that row. So no load is waited for in the plane that issues it.
 // With dot non-null the sweep also writes <b, x_out> over the unknowns of each interior row and plane to
 // dot[(plane - 1) * (nm - 2) + row - 1] (the CG's <r, z> when the sweep is the last one of the V-cycle on the
 // finest level): per row, not per block, so the sum over them does not depend on the tile shape.
@@ -1196,15 +1206,19 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
     if (st->done) return;
     const Box &B = L.B;
     constexpr int kCols = 128 * K + 2;
+    constexpr int kSlots = 64 * K + 2;
     __shared__ float red[2][TM + 2][kCols];      // first-colour values by column f, rows m0-1 .. m0+TM
-    __shared__ float old[TM + 2][2][64 * K];     // old x of plane sp by row, element (a = 0, b = 1) and pair
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __shared__ float old[TM + 2][2][kSlots];     // old x of plane sp by row, element (a = 0, b = 1) and pair + 1
+    __shared__ float2 tab[128];                  // by code & 127: (1/n, all ones) for an unknown, (0, 0) otherwise
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lb = logical_block();
     const int tm = lb % tilesM, chunk = lb / tilesM;
     const int m0 = 1 + tm * TM;
     const int m = m0 - 1 + wave;
     const bool haloRow = wave == 0 || wave == TM + 1;
     const bool rowIn = m >= 1 && m <= B.nm - 2;
+    const bool ragged = B.nf - 2 < 128 * K;   // the row ends before the last lane's last pair
     const int mc = min(max(m, 0), B.nm - 1);
     // the row beyond a halo row (outside the block), clamped into the box
     const int mo = min(max(wave == 0 ? m - 1 : m + 1, 0), B.nm - 1);
@@ -1215,6 +1229,7 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
     const int fcMax = ((B.nf - 2) & 1) ? B.nf - 2 : B.nf - 3;
     uint32_t off[K], offO[K];
     bool okA[K], okB[K];
+    unsigned cm[K];   // the pair's codes where its elements are interior cells of the row (zero elsewhere)
     int fa[K];
 #pragma unroll
     for (int k = 0; k < K; k++)
@@ -1222,6 +1237,7 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
         fa[k] = 1 + 2 * (lane + 64 * k);
         okA[k] = rowIn && fa[k] <= B.nf - 2;
         okB[k] = rowIn && fa[k] + 1 <= B.nf - 2;
+        cm[k] = (okA[k] ? 0xFFu : 0u) | (okB[k] ? 0xFF00u : 0u);
         const int fc = min(fa[k], fcMax);
         off[k] = (uint32_t)mc * sM + (uint32_t)fc;
         offO[k] = (uint32_t)mo * sM + (uint32_t)fc;
@@ -1240,20 +1256,48 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
         return pro.xc + (size_t)min(sc, pro.ns - 1) * (size_t)pro.sS;
     };
     for (int i = threadIdx.x; i < 2 * (TM + 2) * kCols; i += 64 * (TM + 2)) (&red[0][0][0])[i] = 0.f;
+    for (int i = threadIdx.x; i < (TM + 2) * 2 * kSlots; i += 64 * (TM + 2)) (&old[0][0][0])[i] = 0.f;
+    if (threadIdx.x < 128)
+    {
+        const unsigned c = threadIdx.x;
+        tab[c] = make_float2((c & 64u) ? inv_count(c) : 0.f, __uint_as_float((c & 64u) ? ~0u : 0u));
+    }
     auto plane = [&](const float *base, int pl) { return base + (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
     auto planeC = [&](int pl) { return L.c8 + (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
     // one aligned load per pair (the arrays are allocated so that odd columns start pair-aligned)
     auto pair = [](const float *row, uint32_t o) { return *reinterpret_cast<const float2 *>(row + o); };
     auto pairC = [](const uint8_t *row, uint32_t o) { return (unsigned)*reinterpret_cast<const uint16_t *>(row + o); };
+    // PRO: x + P x_c at the pair's unknowns (cc: the pair's masked codes)
+    auto fold = [](float2 &x, float xc, unsigned cc) {
+        x.x += (cc & 64u) ? xc : 0.f;
+        x.y += (cc & (64u << 8)) ? xc : 0.f;
+    };
+    // (acc - b / cu) / n at an unknown, +0 elsewhere
+    auto gs = [&](float acc, float bv, unsigned code) {
+        const float2 t = tab[code & 127u];
+        return __uint_as_float(__float_as_uint((acc - bv * L.icu) * t.x) & __float_as_uint(t.y));
+    };
+    // the update of an unknown with a Dirichlet neighbour: the diagonal from the two-bit neighbour states
+    auto gsDirichlet = [&](float acc, float bv, int pl, uint32_t o) {
+        const unsigned c16 = L.c16[(size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS + o];
+        float d = 0.f;
+#pragma unroll
+        for (int q = 0; q < 6; q++)
+        {
+            const unsigned stt = (c16 >> (2 * q)) & 3u;
+            d += stt == 3u ? 2.f : (stt != 0u ? 1.f : 0.f);
+        }
+        return d != 0.f ? (acc - bv * L.icu) / d : 0.f;
+    };
     // Per-plane values live in slots indexed by plane mod 4 (relative to the first plane s0-1), so the march
     // renames registers instead of moving them: a move of a register that a load in flight will write waits for
-    // every load. x is loaded two planes ahead, b, codes and the halo waves' outer row one plane ahead.
-    float2 X[4][K];    // old x of the pair
+    // every load. x, b and codes are loaded two planes ahead, the halo waves' outer row one plane ahead.
+    float2 X[4][K];    // old x of the pair (PRO: plus the coarse correction from the plane step before its use)
     float2 O[2][K];    // halo waves: old x of the outer row's pair
     float2 Bv[4][K];   // b of the pair
-    unsigned C[4][K];  // codes of the pair (low byte: element a), masked where used
+    unsigned C[4][K];  // codes of the pair (low byte: element a), masked with cm where used
     float R[4][K];     // first-colour value of this lane's pair
-    float XC[4][K];    // PRO: the coarse correction of the pair (added to X where X is used)
+    float XC[4][K];    // PRO: the coarse correction of the pair
     float OC[2][K];    // PRO: and of the outer row's pair
 #pragma unroll
     for (int j = 0; j < 4; j++)
@@ -1297,36 +1341,48 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
             }
         }
     }
+#pragma unroll
+    for (int d = PRO ? -1 : 0; d <= 1; d++)
     {
-        const float *pb = plane(b, sp0);
-        const uint8_t *pc = planeC(sp0);
+        const float *pb = plane(b, sp0 + d);
+        const uint8_t *pc = planeC(sp0 + d);
 #pragma unroll
         for (int k = 0; k < K; k++)
         {
-            Bv[0][k] = pair(pb, off[k]);
-            C[0][k] = pairC(pc, off[k]);
+            if (d >= 0) Bv[d][k] = pair(pb, off[k]);
+            C[(d + 4) & 3][k] = pairC(pc, off[k]);
+        }
+    }
+    if (PRO)
+    {
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            fold(X[3][k], XC[3][k], C[3][k] & cm[k]);
+            fold(X[0][k], XC[0][k], C[0][k] & cm[k]);
         }
     }
     __syncthreads();
-    const float icu = L.icu;
     const int wUp = min(wave + 1, TM + 1), wDn = max(wave - 1, 0);
 
-    // One plane step at sp = sp0 + 4g + U: first colour of plane sp, second colour (and store) of plane sp - 1.
-    auto step = [&](int sp, auto phase) {
+    // One plane step at sp = sp0 + 4g + U: first colour of plane sp (element E of every pair of this row), second
+    // colour (and store) of plane sp - 1.
+    auto step = [&](int sp, auto phase, auto colour) {
         constexpr int U = decltype(phase)::value;
+        constexpr int E = decltype(colour)::value;
         constexpr int Sm2 = (U + 2) & 3, Sm1 = (U + 3) & 3, S0 = U & 3, S1 = (U + 1) & 3, S2 = (U + 2) & 3;
         constexpr int O0 = U & 1, O1 = (U + 1) & 1;
         // prefetch
         {
             const float *px = plane(xin, sp + 2);
-            const float *pb = plane(b, sp + 1);
-            const uint8_t *pc = planeC(sp + 1);
+            const float *pb = plane(b, sp + 2);
+            const uint8_t *pc = planeC(sp + 2);
 #pragma unroll
             for (int k = 0; k < K; k++)
             {
                 if (!ZERO) X[S2][k] = pair(px, off[k]);
-                Bv[S1][k] = pair(pb, off[k]);
-                C[S1][k] = pairC(pc, off[k]);
+                Bv[S2][k] = pair(pb, off[k]);
+                C[S2][k] = pairC(pc, off[k]);
             }
             if (!ZERO && haloRow)
             {
@@ -1347,125 +1403,95 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
                 }
             }
         }
-        // the old value of element e of pair k in slot j where it is used (PRO: with the coarse correction)
-#define NPPE_XV(j, k, e) (PRO ? X[j][k].e + XC[j][k] : X[j][k].e)
-#define NPPE_OV(j, k, e) (PRO ? O[j][k].e + OC[j][k] : O[j][k].e)
-        // publish this row's old x of plane sp for the neighbouring rows
+        // publish this row's old x of plane sp for the neighbouring rows and pairs (zero past the row end)
         if (!ZERO)
         {
-#pragma unroll
-            for (int k = 0; k < K; k++)
+            if (ragged)
             {
-                old[wave][0][lane + 64 * k] = NPPE_XV(S0, k, x);
-                old[wave][1][lane + 64 * k] = NPPE_XV(S0, k, y);
+#pragma unroll
+                for (int k = 0; k < K; k++)
+                {
+                    old[wave][0][1 + lane + 64 * k] = okA[k] ? X[S0][k].x : 0.f;
+                    old[wave][1][1 + lane + 64 * k] = okB[k] ? X[S0][k].y : 0.f;
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (int k = 0; k < K; k++)
+                {
+                    old[wave][0][1 + lane + 64 * k] = X[S0][k].x;
+                    old[wave][1][1 + lane + 64 * k] = X[S0][k].y;
+                }
             }
         }
         // (1) old[] is complete; readers of red[sp & 1] (the second colour of plane sp-2) are done
         __syncthreads();
-        const int e1 = (c1 - sp - m - 1) & 1;   // first-colour element of every pair on this row and plane
-        const bool spIn = sp >= 1 && sp <= B.ns - 2;
+        if (PRO)
+        {
+#pragma unroll
+            for (int k = 0; k < K; k++) fold(X[S1][k], XC[S1][k], C[S1][k] & cm[k]);
+        }
         float rnew[K];
 #pragma unroll
         for (int k = 0; k < K; k++)
         {
-            const float xa = NPPE_XV(S0, k, x), xb = NPPE_XV(S0, k, y);
-            // in-row old neighbours: left of fa is the previous pair's b element, right of fa+1 the next pair's a
-            float left = __shfl_up(xb, 1, 64);
-            float right = __shfl_down(xa, 1, 64);
-            if (K > 1)
-            {
-                const float wrapL = k > 0 ? __shfl(NPPE_XV(S0, k > 0 ? k - 1 : 0, y), 63, 64) : 0.f;
-                const float wrapR = k + 1 < K ? __shfl(NPPE_XV(S0, k + 1 < K ? k + 1 : k, x), 0, 64) : 0.f;
-                if (lane == 0) left = wrapL;
-                if (lane == 63) right = wrapR;
-            }
-            float up = 0.f, dn = 0.f;   // old x of rows m+1, m-1 at plane sp, element fa + e1
+            const unsigned code = ((C[S0][k] & cm[k]) >> (8 * E)) & 255u;
+            const float bv = E ? Bv[S0][k].y : Bv[S0][k].x;
+            float acc = 0.f;
             if (!ZERO)
             {
-                const float lu = old[wUp][e1][lane + 64 * k], ld = old[wDn][e1][lane + 64 * k];
-                const float outer = e1 ? NPPE_OV(O0, k, y) : NPPE_OV(O0, k, x);
-                up = wave == TM + 1 ? outer : lu;
-                dn = wave == 0 ? outer : ld;
+                const int p = 1 + lane + 64 * k;
+                const float own = E ? X[S0][k].x : X[S0][k].y;                 // the pair's other element
+                const float nbr = E ? old[wave][0][p + 1] : old[wave][1][p - 1];   // next pair's a / previous b
+                float up = old[wUp][E][p], dn = old[wDn][E][p];   // rows m+1, m-1 at plane sp, element E
+                const float outer = E ? O[O0][k].y : O[O0][k].x;
+                if (wave == TM + 1) up = PRO ? ((code & 4u) ? outer + OC[O0][k] : 0.f) : outer;
+                if (wave == 0) dn = PRO ? ((code & 8u) ? outer + OC[O0][k] : 0.f) : outer;
+                // (no leading 0 + : an iterate is never -0, so the sum is the same)
+                acc = E ? X[S1][k].y : X[S1][k].x;
+                acc += E ? X[Sm1][k].y : X[Sm1][k].x;
+                acc += up;
+                acc += dn;
+                acc += E ? nbr : own;
+                acc += E ? own : nbr;
             }
-            const unsigned code = (spIn && (e1 ? okB[k] : okA[k])) ? ((C[S0][k] >> (8 * e1)) & 255u) : 0u;
-            float v = 0.f;
-            if (code & 64u)
-            {
-                float acc = 0.f;
-                acc += (code & 1u) ? (e1 ? NPPE_XV(S1, k, y) : NPPE_XV(S1, k, x)) : 0.f;
-                acc += (code & 2u) ? (e1 ? NPPE_XV(Sm1, k, y) : NPPE_XV(Sm1, k, x)) : 0.f;
-                acc += (code & 4u) ? up : 0.f;
-                acc += (code & 8u) ? dn : 0.f;
-                acc += (code & 16u) ? (e1 ? right : xb) : 0.f;
-                acc += (code & 32u) ? (e1 ? xa : left) : 0.f;
-                const float bv = e1 ? Bv[S0][k].y : Bv[S0][k].x;
-                if (!(code & 128u))
-                    v = (acc - bv * icu) * inv_count(code);
-                else
-                {
-                    const uint32_t o = off[k] + (uint32_t)e1;
-                    const unsigned c16 = L.c16[(size_t)min(max(sp, 0), B.ns - 1) * (size_t)B.sS + o];
-                    float d = 0.f;
-#pragma unroll
-                    for (int q = 0; q < 6; q++)
-                    {
-                        const unsigned stt = (c16 >> (2 * q)) & 3u;
-                        d += stt == 3u ? 2.f : (stt != 0u ? 1.f : 0.f);
-                    }
-                    v = d != 0.f ? (acc - bv * icu) / d : 0.f;
-                }
-            }
+            float v = gs(acc, bv, code);
+            if (code & 128u) v = gsDirichlet(acc, bv, sp, off[k] + (uint32_t)E);
             rnew[k] = v;
             R[S0][k] = v;
         }
 #pragma unroll
-        for (int k = 0; k < K; k++) red[sp & 1][wave][fa[k] + e1] = rnew[k];
+        for (int k = 0; k < K; k++) red[sp & 1][wave][fa[k] + E] = rnew[k];
         // (2) red[sp & 1] is complete; readers of old[] are done
         __syncthreads();
         const int q = sp - 1;
         if (!haloRow && q >= s0 && q <= s1)
         {
+            // second colour at plane q: the colour flips with the plane, so there the second-colour element is E
             float *po = xout + (size_t)q * (size_t)B.sS;
             const float(*rq)[kCols] = red[q & 1];
             double bx = 0.0;   // <b, x_out> of this lane's cells of the row
 #pragma unroll
             for (int k = 0; k < K; k++)
             {
-                const int eq = e1;   // second-colour element at plane q
-                const unsigned code = (eq ? okB[k] : okA[k]) ? ((C[Sm1][k] >> (8 * eq)) & 255u) : 0u;
-                float out = 0.f;
-                if (code & 64u)
-                {
-                    const int f = fa[k] + eq;
-                    float acc = 0.f;
-                    acc += (code & 1u) ? R[S0][k] : 0.f;
-                    acc += (code & 2u) ? R[Sm2][k] : 0.f;
-                    acc += (code & 4u) ? rq[wave + 1][f] : 0.f;
-                    acc += (code & 8u) ? rq[wave - 1][f] : 0.f;
-                    acc += (code & 16u) ? (eq ? rq[wave][f + 1] : R[Sm1][k]) : 0.f;
-                    acc += (code & 32u) ? (eq ? R[Sm1][k] : rq[wave][f - 1]) : 0.f;
-                    const float bv = eq ? Bv[Sm1][k].y : Bv[Sm1][k].x;
-                    if (!(code & 128u))
-                        out = (acc - bv * icu) * inv_count(code);
-                    else
-                    {
-                        const unsigned c16 = L.c16[(size_t)q * (size_t)B.sS + off[k] + (uint32_t)eq];
-                        float d = 0.f;
-#pragma unroll
-                        for (int qq = 0; qq < 6; qq++)
-                        {
-                            const unsigned stt = (c16 >> (2 * qq)) & 3u;
-                            d += stt == 3u ? 2.f : (stt != 0u ? 1.f : 0.f);
-                        }
-                        out = d != 0.f ? (acc - bv * icu) / d : 0.f;
-                    }
-                }
-                const float va = eq ? R[Sm1][k] : out, vb = eq ? out : R[Sm1][k];
+                const unsigned cc = C[Sm1][k] & cm[k];
+                const unsigned code = (cc >> (8 * E)) & 255u;
+                const int f = fa[k] + E;
+                float acc = R[S0][k];
+                acc += R[Sm2][k];
+                acc += rq[wave + 1][f];
+                acc += rq[wave - 1][f];
+                acc += E ? rq[wave][f + 1] : R[Sm1][k];
+                acc += E ? R[Sm1][k] : rq[wave][f - 1];
+                const float bv = E ? Bv[Sm1][k].y : Bv[Sm1][k].x;
+                float out = gs(acc, bv, code);
+                if (code & 128u) out = gsDirichlet(acc, bv, q, off[k] + (uint32_t)E);
+                const float va = E ? R[Sm1][k] : out, vb = E ? out : R[Sm1][k];
                 if (dot)
                 {
-                    const unsigned ca = okA[k] ? (C[Sm1][k] & 255u) : 0u, cb = okB[k] ? ((C[Sm1][k] >> 8) & 255u) : 0u;
-                    bx += (ca & 64u) ? (double)Bv[Sm1][k].x * (double)va : 0.0;
-                    bx += (cb & 64u) ? (double)Bv[Sm1][k].y * (double)vb : 0.0;
+                    bx += (cc & 64u) ? (double)Bv[Sm1][k].x * (double)va : 0.0;
+                    bx += (cc & (64u << 8)) ? (double)Bv[Sm1][k].y * (double)vb : 0.0;
                 }
                 if (okB[k])
                     *reinterpret_cast<float2 *>(po + off[k]) = make_float2(va, vb);
@@ -1479,18 +1505,25 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
             }
         }
     };
-#undef NPPE_XV
-#undef NPPE_OV
-    for (int sp = sp0; sp <= s1 + 1; sp += 4)
-    {
-        step(sp, std::integral_constant<int, 0>());
-        if (sp + 1 > s1 + 1) break;
-        step(sp + 1, std::integral_constant<int, 1>());
-        if (sp + 2 > s1 + 1) break;
-        step(sp + 2, std::integral_constant<int, 2>());
-        if (sp + 3 > s1 + 1) break;
-        step(sp + 3, std::integral_constant<int, 3>());
-    }
+    // the first-colour element at plane sp is (c1 - sp - m - 1) & 1: constant per wave at the first plane, then
+    // alternating
+    auto march = [&](auto colour0) {
+        constexpr int E0 = decltype(colour0)::value;
+        for (int sp = sp0; sp <= s1 + 1; sp += 4)
+        {
+            step(sp, std::integral_constant<int, 0>(), std::integral_constant<int, E0>());
+            if (sp + 1 > s1 + 1) break;
+            step(sp + 1, std::integral_constant<int, 1>(), std::integral_constant<int, E0 ^ 1>());
+            if (sp + 2 > s1 + 1) break;
+            step(sp + 2, std::integral_constant<int, 2>(), std::integral_constant<int, E0>());
+            if (sp + 3 > s1 + 1) break;
+            step(sp + 3, std::integral_constant<int, 3>(), std::integral_constant<int, E0 ^ 1>());
+        }
+    };
+    if ((c1 - sp0 - m - 1) & 1)
+        march(std::integral_constant<int, 1>());
+    else
+        march(std::integral_constant<int, 0>());
 }
 
 // Coarse cell (S, M, F) gathers the residual b - A x of its fine children (fine indices 2C-1, 2C along each
