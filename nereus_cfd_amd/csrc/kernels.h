@@ -16,6 +16,10 @@ namespace nppe
 // st->sums[sumSlot].
 hipError_t launch_scatter64(hipStream_t s, int64_t n, const int32_t *order, const double *src, double *box,
                             SolveState *st, int sumSlot, double *partials);
+// A solve without a guess: b64[order[q]] = r64[order[q]] = src[q], r32[order[q]] = (float) src[q],
+// x64[order[q]] = 0, and the sum of src^2 into st->sums[0].
+hipError_t launch_scatter_rhs64(hipStream_t s, int64_t n, const int32_t *order, const double *src, double *b64,
+                                double *r64, float *r32, double *x64, SolveState *st, double *partials);
 // dst[q] = box[order[q]] - (meanSlot >= 0 ? st->sums[meanSlot] / n : 0).
 // gate: only once st->accepted (a true residual met the target).
 hipError_t launch_gather64(hipStream_t s, int64_t n, const int32_t *order, const double *box, double *dst,
@@ -101,7 +105,9 @@ hipError_t launch_cg_update(hipStream_t s, const Level32 &L, float *e, float *r,
 hipError_t launch_inner_setup(hipStream_t s, SolveState *st, double reduction, int maxIters);
 // FP64 CG set-up: thr2, iteration cap, counters and the accepted flag cleared.
 hipError_t launch_state_init(hipStream_t s, SolveState *st, double thr2, int maxIters);
-// The FP64 Krylov set-up from the guess's true residual in st->sums (see k_krylov_init).
-hipError_t launch_krylov_init(hipStream_t s, SolveState *st, double tol, double ref, double aimFactor, int maxIters);
+// The FP64 Krylov set-up from |b|^2 in st->sums[0] and the guess's true residual |r|^2 in st->sums[1] (residualIsB:
+// the residual is b itself, no guess, and sums[1] is not read).
+hipError_t launch_krylov_init(hipStream_t s, SolveState *st, double tol, double ref, double aimFactor, int maxIters,
+                              int residualIsB);
 
 }  // namespace nppe

@@ -508,6 +508,29 @@ __global__ __launch_bounds__(kBlockThreads) void k_scatter64(int64_t n, const in
     block_sum<1>(a, red, gridDim.x, blockIdx.x);
 }
 
+// The right-hand side of a solve without a guess: b64 = r64 = b, r32 = (float) b and x = 0 at every unknown (the
+// residual of the zero guess, b - A 0, is b bit for bit), and <b, b> reduced as k_scatter64 reduces it.
+__global__ __launch_bounds__(kBlockThreads) void k_scatter_rhs64(int64_t n, const int32_t *__restrict__ order,
+                                                                 const double *__restrict__ src,
+                                                                 double *__restrict__ b64, double *__restrict__ r64,
+                                                                 float *__restrict__ r32, double *__restrict__ x64,
+                                                                 Red red)
+{
+    double v2 = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x; q < n; q += (int64_t)gridDim.x * kBlockThreads)
+    {
+        const double v = src[q];
+        const int32_t o = order[q];
+        b64[o] = v;
+        r64[o] = v;
+        r32[o] = (float)v;
+        x64[o] = 0.0;
+        v2 += v * v;
+    }
+    double a[1] = {v2};
+    block_sum<1>(a, red, gridDim.x, blockIdx.x);
+}
+
 __global__ __launch_bounds__(kBlockThreads) void k_gather64(int64_t n, const int32_t *__restrict__ order,
                                                             const double *__restrict__ box, double *__restrict__ dst,
                                                             const SolveState *st, int meanSlot, int gate)
@@ -2336,11 +2359,12 @@ __global__ void k_state_init(SolveState *st, double thr2, int maxIters)
 // device so that the host needs no round trip first: target = tol * max(|b|, ref), the recursion aims at
 // aimFactor * target; done at once when the guess is the answer (b = 0, |b| <= tol * ref) or already meets the
 // target.
-__global__ void k_krylov_init(SolveState *st, double tol, double ref, double aimFactor, int maxIters)
+__global__ void k_krylov_init(SolveState *st, double tol, double ref, double aimFactor, int maxIters, int residualIsB)
 {
-    const double bn = sqrt(st->sums[0]), rn = sqrt(st->sums[1]);
+    const double rr = residualIsB ? st->sums[0] : st->sums[1];
+    const double bn = sqrt(st->sums[0]), rn = sqrt(rr);
     const double target = tol * fmax(bn, ref), aim = aimFactor * target;
-    st->rr0 = st->sums[1];
+    st->rr0 = rr;
     st->acc2 = target * target;
     st->thr2 = aim * aim;
     st->iters = 0;
@@ -2402,6 +2426,16 @@ hipError_t launch_scatter64(hipStream_t s, int64_t n, const int32_t *order, cons
     const Red red = red_of(partials, st, op, G);
     k_scatter64<<<G, kBlockThreads, 0, s>>>(n, order, src, box, red, sumSlot >= 0);
     if (sumSlot < 0) return hipGetLastError();
+    return finish(s, red, G, 1);
+}
+
+hipError_t launch_scatter_rhs64(hipStream_t s, int64_t n, const int32_t *order, const double *src, double *b64,
+                                double *r64, float *r32, double *x64, SolveState *st, double *partials)
+{
+    if (n <= 0) return hipSuccess;
+    const int G = grid_1d(n);
+    const Red red = red_of(partials, st, FIN_SUM0, G);
+    k_scatter_rhs64<<<G, kBlockThreads, 0, s>>>(n, order, src, b64, r64, r32, x64, red);
     return finish(s, red, G, 1);
 }
 
@@ -2768,9 +2802,10 @@ hipError_t launch_state_init(hipStream_t s, SolveState *st, double thr2, int max
     return hipGetLastError();
 }
 
-hipError_t launch_krylov_init(hipStream_t s, SolveState *st, double tol, double ref, double aimFactor, int maxIters)
+hipError_t launch_krylov_init(hipStream_t s, SolveState *st, double tol, double ref, double aimFactor, int maxIters,
+                              int residualIsB)
 {
-    k_krylov_init<<<1, 1, 0, s>>>(st, tol, ref, aimFactor, maxIters);
+    k_krylov_init<<<1, 1, 0, s>>>(st, tol, ref, aimFactor, maxIters, residualIsB);
     return hipGetLastError();
 }
 

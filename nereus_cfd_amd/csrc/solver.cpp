@@ -1032,15 +1032,16 @@ int finish_result(nppe_solver *h, nppe_result *res, hipEvent_t t0, hipEvent_t t1
     return status;
 }
 
-// The FP64 Krylov solve scheduled without host round trips: after the guess's true residual (already enqueued)
-// the device sets up the recursion itself (k_krylov_init); the host enqueues the iterations it predicts (the last
+// The FP64 Krylov solve scheduled without host round trips: after the guess's true residual (already enqueued;
+// residualIsB: no guess, the residual is b, scattered with it) the device sets up the recursion itself
+// (k_krylov_init); the host enqueues the iterations it predicts (the last
 // solve's count, or kInnerChunk), then the true residual and the output gather, both gated on the device (the
 // residual only once the recursion is done, the gather only once that residual met the target), and synchronises
 // once. A well-predicted solve makes that its only round trip; a short prediction enqueues more iterations, and a
 // missed target restarts CG from the new x as another pass. The kernels and their order are those of the
 // synchronous schedule, so the result is bit for bit the same.
-int krylov64_scheduled(nppe_solver *h, double refNorm, int cur, double *dX, bool project, Traffic &tr, int *passesOut,
-                       int *innerOut, int *statusOut, double *bnormOut, double *rnormOut)
+int krylov64_scheduled(nppe_solver *h, double refNorm, int cur, double *dX, bool project, bool residualIsB,
+                       Traffic &tr, int *passesOut, int *innerOut, int *statusOut, double *bnormOut, double *rnormOut)
 {
     const DevLevel &L0 = h->levels[0];
     const double N = L0.cellsBox, n = (double)h->n;
@@ -1061,7 +1062,8 @@ int krylov64_scheduled(nppe_solver *h, double refNorm, int cur, double *dX, bool
         h->graphBytesPerIter = it.bytes;
     }
     const double tol = h->opt.tolerance;
-    HIP_TRY(h, launch_krylov_init(h->stream, h->dState, tol, refNorm, 0.8, h->opt.max_iters), "krylov setup");
+    HIP_TRY(h, launch_krylov_init(h->stream, h->dState, tol, refNorm, 0.8, h->opt.max_iters, residualIsB ? 1 : 0),
+            "krylov setup");
     auto predict = [&](double need) {
         if (!(h->rate > 0.0 && h->rate < 1.0) || !(need < 1.0)) return kInnerChunk;
         const double k = std::ceil(std::log(need) / std::log(h->rate));
@@ -1186,6 +1188,27 @@ int solve_mgpcg(nppe_solver *h, const double *dB, double *dX, const double *dGue
     HIP_TRY(h, hipEventRecord(t0, h->stream), "event");
     h->curIter = -1;
     int cur = 0;
+    const bool project = h->singular && h->opt.compatibility && h->n > 0;
+    if (h->krylov64 && h->opt.max_iters > 0 && h->opt.max_refinements > 0 && !dGuess)
+    {
+        // no guess: x = 0, and its residual is b (one scatter instead of the guess and the residual passes)
+        {
+            Scope sc(h, NPPE_K_SCATTER, 40.0 * n);
+            HIP_TRY(h, launch_scatter_rhs64(h->stream, h->n, h->dOrder, dB, h->dB64, h->dR64, h->dR, h->dX64[cur],
+                                            h->dState, h->dPartials),
+                    "scatter b");
+        }
+        tr.add(40.0 * n);
+        int passes = 0, inner = 0, status = NPPE_OK;
+        double bnorm = 0.0, rnorm = 0.0;
+        if (int rc = krylov64_scheduled(h, refNorm, cur, dX, project, true, tr, &passes, &inner, &status, &bnorm,
+                                        &rnorm))
+            return rc;
+        HIP_TRY(h, hipEventRecord(t1, h->stream), "event");
+        HIP_TRY(h, hipEventSynchronize(t1), "event");
+        profile_collect(h, 0);
+        return finish_result(h, res, t0, t1, inner, passes, status, bnorm, rnorm, refNorm, tr);
+    }
     {
         Scope sc(h, NPPE_K_SCATTER, 20.0 * n);
         HIP_TRY(h, launch_scatter64(h->stream, h->n, h->dOrder, dB, h->dB64, h->dState, 0, h->dPartials), "scatter b");
@@ -1204,12 +1227,12 @@ int solve_mgpcg(nppe_solver *h, const double *dB, double *dX, const double *dGue
                 "residual");
     }
     tr.add((h->krylov64 ? 29.0 : 21.0) * N);
-    const bool project = h->singular && h->opt.compatibility && h->n > 0;
     if (h->krylov64 && h->opt.max_iters > 0 && h->opt.max_refinements > 0)
     {
         int passes = 0, inner = 0, status = NPPE_OK;
         double bnorm = 0.0, rnorm = 0.0;
-        if (int rc = krylov64_scheduled(h, refNorm, cur, dX, project, tr, &passes, &inner, &status, &bnorm, &rnorm))
+        if (int rc = krylov64_scheduled(h, refNorm, cur, dX, project, false, tr, &passes, &inner, &status, &bnorm,
+                                        &rnorm))
             return rc;
         HIP_TRY(h, hipEventRecord(t1, h->stream), "event");
         HIP_TRY(h, hipEventSynchronize(t1), "event");
