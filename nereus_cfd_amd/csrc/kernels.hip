@@ -2111,6 +2111,349 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_update64_t(Box B, Tables64
     block_sum<1>(a, red, gridDim.x, M.lb);
 }
 
+// ---- FP64 Krylov passes, whole-row waves (3D, unit coefficients) ----------------------------------------------
+//
+// k_cg_dir64_t and k_cg_update64_t on the block layout of k_rbgs_row: one wave per row (lane l owns the cell pairs
+// (1 + 2p, 2 + 2p), p = l + 64k, so a wave reads and writes whole rows), TM interior rows plus a halo row above and
+// below, a march over P planes with every load issued two planes ahead (x and r of the own cells one plane ahead),
+// and the neighbouring rows' and pairs' p through LDS, double-buffered by plane parity (one barrier per plane).
+// Loads outside the interior are clamped into the box; apply64u only adds coupled neighbours, so what they read
+// there is never used. The per-cell arithmetic is the tile kernels' (the same FP32 rounding of p, apply64u in the
+// same order), so p, x and r match theirs bit for bit; only the order in which <p, A p> and <r, r> are summed
+// differs (it depends on the box size alone).
+// apply64u for an operand that is +0 at every neighbour that is no coupled unknown (p of the FP64 Krylov passes:
+// 0 at every cell but the unknowns, and the LDS columns beyond the row ends): the same products added in the same
+// order without testing the code's neighbour bits. Bit for bit the same: a skipped term is v * (+0) = +-0, and
+// adding +-0 leaves a sum that is never -0 unchanged.
+__device__ __forceinline__ double apply64u_z(const Tables64 &T, int64_t idx, double xc, const double (&nb)[6],
+                                             unsigned code)
+{
+#pragma clang fp contract(off)
+    unsigned n;
+    if (!(code & 128u))
+        n = __popc(code & 63u);
+    else
+    {
+        const unsigned c16 = T.c16[idx];
+        n = 0;
+#pragma unroll
+        for (int q = 0; q < 6; q++)
+        {
+            const unsigned st = (c16 >> (2 * q)) & 3u;
+            n += st == 3u ? 2u : (st != 0u ? 1u : 0u);
+        }
+    }
+    const double v = T.scale * 1.0;
+    const double dv = T.scale * (-(double)n);
+    double t = 0.0;
+    t += v * nb[1];
+    t += v * nb[3];
+    t += v * nb[5];
+    t += dv * xc;
+    t += v * nb[4];
+    t += v * nb[2];
+    t += v * nb[0];
+    return t;
+}
+
+template <int TM, int K>
+struct RowMarch
+{
+    int lane, wave, m, s0, s1;
+    bool halo, rowIn;
+    uint32_t off[K];
+    bool okA[K], okB[K];
+};
+
+template <int TM, int K>
+__device__ __forceinline__ RowMarch<TM, K> row_march(const Box &B, int P, int tilesM)
+{
+    RowMarch<TM, K> R;
+    R.lane = threadIdx.x & 63;
+    R.wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lb = logical_block();
+    const int tm = lb % tilesM, chunk = lb / tilesM;
+    R.m = tm * TM + R.wave;   // m0 - 1 + wave with m0 = 1 + tm * TM
+    R.halo = R.wave == 0 || R.wave == TM + 1;
+    R.rowIn = R.m >= 1 && R.m <= B.nm - 2;
+    R.s0 = 1 + chunk * P;
+    R.s1 = min(R.s0 + P - 1, B.ns - 2);
+    const int mc = min(max(R.m, 0), B.nm - 1);
+    const int fcMax = ((B.nf - 2) & 1) ? B.nf - 2 : B.nf - 3;
+#pragma unroll
+    for (int k = 0; k < K; k++)
+    {
+        const int fa = 1 + 2 * (R.lane + 64 * k);
+        R.okA[k] = R.rowIn && fa <= B.nf - 2;
+        R.okB[k] = R.rowIn && fa + 1 <= B.nf - 2;
+        R.off[k] = (uint32_t)mc * (uint32_t)B.sM + (uint32_t)min(fa, fcMax);
+    }
+    return R;
+}
+
+// Part 1 (see k_cg_dir64_t): p = (float)(z + beta p_old) (z on the first iteration) at the unknowns, and <p, A p>.
+template <int TM, int K>
+__global__ __launch_bounds__(64 * (TM + 2)) void k_cg_dir64_row(Box B, Tables64 T, const float *__restrict__ z,
+                                                                float *__restrict__ pA, float *__restrict__ pB,
+                                                                const SolveState *st, Red red, int P, int tilesM)
+{
+#pragma clang fp contract(off)
+    if (st->done) return;
+    const int first = st->first;
+    const double beta = st->beta;
+    const bool odd = st->iters & 1;
+    float *__restrict__ pn = odd ? pB : pA;
+    const float *__restrict__ po = odd ? pA : pB;
+    constexpr int kSlots = 64 * K + 2;
+    __shared__ float ldsP[2][TM + 2][2][kSlots];   // new p of a plane by parity, row, element and pair + 1
+    const RowMarch<TM, K> R = row_march<TM, K>(B, P, tilesM);
+    for (int i = threadIdx.x; i < 2 * (TM + 2) * 2 * kSlots; i += 64 * (TM + 2)) (&ldsP[0][0][0][0])[i] = 0.f;
+    auto plane = [&](int pl) { return (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
+    auto pair = [](const float *row, uint32_t o) { return *reinterpret_cast<const float2 *>(row + o); };
+    auto pairC = [](const uint8_t *row, uint32_t o) { return (unsigned)*reinterpret_cast<const uint16_t *>(row + o); };
+    auto pv = [&](float zv, float pov) { return (float)(first ? (double)zv : (double)zv + beta * (double)pov); };
+    // per-plane values in slots by (plane - (s0 - 1)) mod 4
+    float2 Zv[4][K], Po[4][K], Pn[4][K];
+    unsigned C[4][K];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            Zv[j][k] = Po[j][k] = Pn[j][k] = make_float2(0.f, 0.f);
+            C[j][k] = 0u;
+        }
+    const int sp0 = R.s0 - 1;
+#pragma unroll
+    for (int d = 0; d <= 2; d++)
+    {
+        const size_t pl = plane(sp0 + d);
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            Zv[d][k] = pair(z + pl, R.off[k]);
+            if (!first) Po[d][k] = pair(po + pl, R.off[k]);
+            C[d][k] = pairC(T.c8 + pl, R.off[k]);
+        }
+    }
+#pragma unroll
+    for (int d = 0; d <= 1; d++)
+#pragma unroll
+        for (int k = 0; k < K; k++) Pn[d][k] = make_float2(pv(Zv[d][k].x, Po[d][k].x), pv(Zv[d][k].y, Po[d][k].y));
+    __syncthreads();
+    const int wUp = min(R.wave + 1, TM + 1), wDn = max(R.wave - 1, 0);
+    double pq = 0.0;
+    auto step = [&](int sp, auto phase) {
+        constexpr int U = decltype(phase)::value;
+        constexpr int Sm1 = (U + 3) & 3, S0 = U, S1 = (U + 1) & 3, S2 = (U + 2) & 3;
+        {
+            const size_t pl = plane(sp + 2);
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                Zv[S2][k] = pair(z + pl, R.off[k]);
+                if (!first) Po[S2][k] = pair(po + pl, R.off[k]);
+                C[S2][k] = pairC(T.c8 + pl, R.off[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            Pn[S1][k] = make_float2(pv(Zv[S1][k].x, Po[S1][k].x), pv(Zv[S1][k].y, Po[S1][k].y));
+        float(*buf)[2][kSlots] = ldsP[sp & 1];
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            // (past a short row's end the clamped loads are no cells of the row: 0 there)
+            buf[R.wave][0][1 + R.lane + 64 * k] = R.okA[k] || !R.rowIn ? Pn[S0][k].x : 0.f;
+            buf[R.wave][1][1 + R.lane + 64 * k] = R.okB[k] || !R.rowIn ? Pn[S0][k].y : 0.f;
+        }
+        __syncthreads();
+        if (R.halo) return;
+        const size_t pl = plane(sp);
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            const int q = 1 + R.lane + 64 * k;
+#pragma unroll
+            for (int e = 0; e < 2; e++)
+            {
+                const unsigned code = (e ? R.okB[k] : R.okA[k]) ? ((C[S0][k] >> (8 * e)) & 255u) : 0u;
+                if (!(code & 64u)) continue;
+                const float pc = e ? Pn[S0][k].y : Pn[S0][k].x;
+                const double nb[6] = {(double)(e ? Pn[S1][k].y : Pn[S1][k].x), (double)(e ? Pn[Sm1][k].y : Pn[Sm1][k].x),
+                                      (double)buf[wUp][e][q], (double)buf[wDn][e][q],
+                                      (double)(e ? buf[R.wave][0][q + 1] : Pn[S0][k].y),
+                                      (double)(e ? Pn[S0][k].x : buf[R.wave][1][q - 1])};
+                const double v = apply64u_z(T, (int64_t)pl + R.off[k] + e, (double)pc, nb, code);
+                pq += (double)pc * v;
+            }
+            // p of both cells of the pair (a non-unknown's p is 0, as every iteration writes it)
+            if (R.okB[k])
+                *reinterpret_cast<float2 *>(pn + pl + R.off[k]) = Pn[S0][k];
+            else if (R.okA[k])
+                pn[pl + R.off[k]] = Pn[S0][k].x;
+        }
+    };
+    for (int sp = R.s0; sp <= R.s1; sp += 4)
+    {
+        step(sp, std::integral_constant<int, 1>());
+        if (sp + 1 > R.s1) break;
+        step(sp + 1, std::integral_constant<int, 2>());
+        if (sp + 2 > R.s1) break;
+        step(sp + 2, std::integral_constant<int, 3>());
+        if (sp + 3 > R.s1) break;
+        step(sp + 3, std::integral_constant<int, 0>());
+    }
+    double a[1] = {pq};
+    block_sum<1>(a, red, gridDim.x, logical_block());
+}
+
+// Part 2 (see k_cg_update64_t): q = A p, x += alpha p, r -= alpha q, r32 = (float) r at the unknowns, and <r, r>.
+template <int TM, int K>
+__global__ __launch_bounds__(64 * (TM + 2), 4) void k_cg_update64_row(Box B, Tables64 T, const float *__restrict__ pA,
+                                                                   const float *__restrict__ pB,
+                                                                   double *__restrict__ x, double *__restrict__ r,
+                                                                   float *__restrict__ r32, const SolveState *st,
+                                                                   Red red, int P, int tilesM)
+{
+#pragma clang fp contract(off)
+    if (st->done) return;
+    const double alpha = st->alpha;
+    const float *__restrict__ p = (st->iters & 1) ? pB : pA;
+    constexpr int kSlots = 64 * K + 2;
+    __shared__ float ldsP[2][TM + 2][2][kSlots];
+    const RowMarch<TM, K> R = row_march<TM, K>(B, P, tilesM);
+    for (int i = threadIdx.x; i < 2 * (TM + 2) * 2 * kSlots; i += 64 * (TM + 2)) (&ldsP[0][0][0][0])[i] = 0.f;
+    auto plane = [&](int pl) { return (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
+    auto pair = [](const float *row, uint32_t o) { return *reinterpret_cast<const float2 *>(row + o); };
+    auto pair64 = [](const double *row, uint32_t o) { return *reinterpret_cast<const double2 *>(row + o); };
+    auto pairC = [](const uint8_t *row, uint32_t o) { return (unsigned)*reinterpret_cast<const uint16_t *>(row + o); };
+    float2 Pv[4][K];
+    unsigned C[4][K];
+    double2 Xv[4][K], Rv[4][K];   // own x and r (interior waves), loaded two planes ahead
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+        {
+            Pv[j][k] = make_float2(0.f, 0.f);
+            C[j][k] = 0u;
+            Xv[j][k] = Rv[j][k] = make_double2(0.0, 0.0);
+        }
+    const int sp0 = R.s0 - 1;
+#pragma unroll
+    for (int d = 0; d <= 2; d++)
+    {
+        const size_t pl = plane(sp0 + d);
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            Pv[d][k] = pair(p + pl, R.off[k]);
+            C[d][k] = pairC(T.c8 + pl, R.off[k]);
+        }
+    }
+    if (!R.halo)
+    {
+#pragma unroll
+        for (int d = 1; d <= 2; d++)
+        {
+            const size_t pl = plane(sp0 + d);
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                Xv[d][k] = pair64(x + pl, R.off[k]);
+                Rv[d][k] = pair64(r + pl, R.off[k]);
+            }
+        }
+    }
+    __syncthreads();
+    const int wUp = min(R.wave + 1, TM + 1), wDn = max(R.wave - 1, 0);
+    double rr = 0.0;
+    auto step = [&](int sp, auto phase) {
+        constexpr int U = decltype(phase)::value;
+        constexpr int Sm1 = (U + 3) & 3, S0 = U, S1 = (U + 1) & 3, S2 = (U + 2) & 3;
+        {
+            const size_t pl = plane(sp + 2);
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                Pv[S2][k] = pair(p + pl, R.off[k]);
+                C[S2][k] = pairC(T.c8 + pl, R.off[k]);
+            }
+        }
+        if (!R.halo && sp + 2 <= R.s1)
+        {
+            const size_t pl = plane(sp + 2);
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                Xv[S2][k] = pair64(x + pl, R.off[k]);
+                Rv[S2][k] = pair64(r + pl, R.off[k]);
+            }
+        }
+        float(*buf)[2][kSlots] = ldsP[sp & 1];
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            buf[R.wave][0][1 + R.lane + 64 * k] = R.okA[k] || !R.rowIn ? Pv[S0][k].x : 0.f;
+            buf[R.wave][1][1 + R.lane + 64 * k] = R.okB[k] || !R.rowIn ? Pv[S0][k].y : 0.f;
+        }
+        __syncthreads();
+        if (R.halo) return;
+        const size_t pl = plane(sp);
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            const int q = 1 + R.lane + 64 * k;
+            double xo[2] = {Xv[S0][k].x, Xv[S0][k].y}, ro[2] = {Rv[S0][k].x, Rv[S0][k].y};
+            bool any = false;
+#pragma unroll
+            for (int e = 0; e < 2; e++)
+            {
+                const unsigned code = (e ? R.okB[k] : R.okA[k]) ? ((C[S0][k] >> (8 * e)) & 255u) : 0u;
+                if (!(code & 64u)) continue;
+                any = true;
+                const double pc = (double)(e ? Pv[S0][k].y : Pv[S0][k].x);
+                const double nb[6] = {(double)(e ? Pv[S1][k].y : Pv[S1][k].x), (double)(e ? Pv[Sm1][k].y : Pv[Sm1][k].x),
+                                      (double)buf[wUp][e][q], (double)buf[wDn][e][q],
+                                      (double)(e ? buf[R.wave][0][q + 1] : Pv[S0][k].y),
+                                      (double)(e ? Pv[S0][k].x : buf[R.wave][1][q - 1])};
+                const double qv = apply64u_z(T, (int64_t)pl + R.off[k] + e, pc, nb, code);
+                xo[e] = xo[e] + alpha * pc;
+                ro[e] = ro[e] - alpha * qv;
+                rr += ro[e] * ro[e];
+            }
+            if (!any) continue;
+            // a pair with one unknown writes the other element back unchanged (r and r32 are 0 at non-unknowns)
+            const int64_t idx = (int64_t)pl + R.off[k];
+            if (R.okB[k])
+            {
+                *reinterpret_cast<double2 *>(x + idx) = make_double2(xo[0], xo[1]);
+                *reinterpret_cast<double2 *>(r + idx) = make_double2(ro[0], ro[1]);
+                *reinterpret_cast<float2 *>(r32 + idx) = make_float2((float)ro[0], (float)ro[1]);
+            }
+            else
+            {
+                x[idx] = xo[0];
+                r[idx] = ro[0];
+                r32[idx] = (float)ro[0];
+            }
+        }
+    };
+    for (int sp = R.s0; sp <= R.s1; sp += 4)
+    {
+        step(sp, std::integral_constant<int, 1>());
+        if (sp + 1 > R.s1) break;
+        step(sp + 1, std::integral_constant<int, 2>());
+        if (sp + 2 > R.s1) break;
+        step(sp + 2, std::integral_constant<int, 3>());
+        if (sp + 3 > R.s1) break;
+        step(sp + 3, std::integral_constant<int, 0>());
+    }
+    double a[1] = {rr};
+    block_sum<1>(a, red, gridDim.x, logical_block());
+}
+
 // Residual of every fine cell, summed over each 2 x 2 x 2 family into the coarse right-hand side: fast pairs are
 // neighbouring lanes (shuffle), mid pairs neighbouring rows of the block (LDS), slow pairs consecutive planes of
 // the march (P even, chunks start on an odd plane). A family cut by the end of an odd interior keeps its 1 or 2
@@ -2743,9 +3086,47 @@ hipError_t launch_cg_dirspmv(hipStream_t s, const Level32 &L, const float *z, fl
     return finish(s, red, G, 1);
 }
 
+// The whole-row FP64 passes: 3D boxes with unit coefficients, rows of at most 256 cells, and every array with
+// element 1 of each row pair-aligned. Tile rows and the chunk length follow from the box alone (never from timing),
+// so the order of the partial sums is a function of the box.
+struct RowPlan
+{
+    bool ok;
+    int K, TM, P, tilesM, G;
+};
+
+RowPlan row_plan(const Box &B, const Tables64 &T, std::initializer_list<std::pair<const void *, uintptr_t>> arrays,
+                 int tm, int blocksPerCU)
+{
+    RowPlan R{};
+    auto oddAligned = [](const void *p, uintptr_t a) { return p == nullptr || (((uintptr_t)p + (a >> 1)) % a) == 0; };
+    R.ok = B.threeD && T.unitCoef && B.nf - 2 <= 256 && (B.sM % 16) == 0 && (B.sS % 16) == 0 && oddAligned(T.c8, 2);
+    for (const auto &a : arrays) R.ok = R.ok && oddAligned(a.first, a.second);
+    if (!R.ok) return R;
+    constexpr int kCUs = 256;
+    R.K = B.nf - 2 <= 128 ? 1 : 2;
+    R.TM = tm;
+    R.tilesM = std::max(1, (B.nm - 2 + tm - 1) / tm);
+    const int planes = B.ns - 2;
+    const int chunksMax = std::max(1, kCUs * blocksPerCU / R.tilesM);
+    R.P = std::max(1, (planes + chunksMax - 1) / chunksMax);
+    R.G = R.tilesM * ((planes + R.P - 1) / R.P);
+    return R;
+}
+
 hipError_t launch_cg_dir64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const float *z,
                            float *pA, float *pB, SolveState *st, double *partials)
 {
+    const RowPlan rp = row_plan(B, T, {{z, 8}, {pA, 8}, {pB, 8}}, 14, 1);
+    if (rp.ok)
+    {
+        const Red red = red_of(partials, st, FIN_PQ, rp.G);
+        if (rp.K == 1)
+            k_cg_dir64_row<14, 1><<<rp.G, 64 * 16, 0, s>>>(B, T, z, pA, pB, st, red, rp.P, rp.tilesM);
+        else
+            k_cg_dir64_row<14, 2><<<rp.G, 64 * 16, 0, s>>>(B, T, z, pA, pB, st, red, rp.P, rp.tilesM);
+        return finish(s, red, rp.G, 1);
+    }
     const int P = march_planes(B), G = march_grid(B, P);
     const Red red = red_of(partials, st, FIN_PQ, G);
     if (T.unitCoef)
@@ -2758,6 +3139,16 @@ hipError_t launch_cg_dir64(hipStream_t s, const Box &B, const Tables64 &T, const
 hipError_t launch_cg_update64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const float *pA,
                               const float *pB, double *x, double *r, float *r32, SolveState *st, double *partials)
 {
+    const RowPlan rp = row_plan(B, T, {{pA, 8}, {pB, 8}, {x, 16}, {r, 16}, {r32, 8}}, 6, 2);
+    if (rp.ok)
+    {
+        const Red red = red_of(partials, st, FIN_RR, rp.G);
+        if (rp.K == 1)
+            k_cg_update64_row<6, 1><<<rp.G, 64 * 8, 0, s>>>(B, T, pA, pB, x, r, r32, st, red, rp.P, rp.tilesM);
+        else
+            k_cg_update64_row<6, 2><<<rp.G, 64 * 8, 0, s>>>(B, T, pA, pB, x, r, r32, st, red, rp.P, rp.tilesM);
+        return finish(s, red, rp.G, 1);
+    }
     const int P = march_planes(B), G = march_grid(B, P);
     const Red red = red_of(partials, st, FIN_RR, G);
     if (T.unitCoef)

@@ -706,9 +706,10 @@ int build(nppe_solver *h, const nppe_grid *g)
     HIP_TRY(h, dalloc(h, &h->dOrder, (size_t)h->unknownCap), "allocate order");
     if (int rc = upload_order(h)) return rc;
     const size_t cells = (size_t)F.box.cells;
-    HIP_TRY(h, dalloc(h, &h->dX64[0], cells), "allocate x64");
-    HIP_TRY(h, dalloc(h, &h->dX64[1], cells), "allocate x64");
-    HIP_TRY(h, dalloc(h, &h->dB64, cells), "allocate b64");
+    // (odd columns pair-aligned, like the FP32 vectors: the whole-row FP64 passes move x and r in pairs)
+    HIP_TRY(h, dalloc_pairs(h, &h->dX64[0], cells), "allocate x64");
+    HIP_TRY(h, dalloc_pairs(h, &h->dX64[1], cells), "allocate x64");
+    HIP_TRY(h, dalloc_pairs(h, &h->dB64, cells), "allocate b64");
     if (h->opt.method == NPPE_METHOD_MGPCG)
     {
         HIP_TRY(h, dalloc_pairs(h, &h->dE, cells), "allocate e");
@@ -725,7 +726,7 @@ int build(nppe_solver *h, const nppe_grid *g)
         h->nuPost = h->opt.post_smooth == NPPE_SMOOTH_AUTO ? autoNu : h->opt.post_smooth;
         if (h->krylov64)
         {
-            HIP_TRY(h, dalloc(h, &h->dR64, cells), "allocate r64");
+            HIP_TRY(h, dalloc_pairs(h, &h->dR64, cells), "allocate r64");
         }
         HIP_TRY(h, dalloc_pairs(h, &h->dZ, cells), "allocate z");
         HIP_TRY(h, dalloc_pairs(h, &h->dZ2, cells), "allocate z");
