@@ -2454,6 +2454,157 @@ __global__ __launch_bounds__(64 * (TM + 2), 4) void k_cg_update64_row(Box B, Tab
     block_sum<1>(a, red, gridDim.x, logical_block());
 }
 
+// Residual of every fine cell summed over each 2 x 2 x 2 family into the coarse right-hand side (k_residual_restrict_m)
+// on the layout of k_rbgs_row: one wave per row (TM even, so a block's interior rows start on an odd row and hold
+// whole row pairs; P even, so chunks start on an odd plane), x through LDS double-buffered by plane, every load two
+// planes ahead. The sums are k_residual_restrict_m's in its order: the pair (r_a + r_b), then the two rows (the odd
+// row first), then the two planes added to 0 -- bit for bit the same coarse right-hand side. The fast cells' six
+// neighbours are added without testing the code's bits (x is +0 at every cell that is no unknown, and at the LDS
+// columns beyond the row ends); the first row of each family reads its partner row's pair sums one plane step later.
+template <int TM, int K>
+__global__ __launch_bounds__(64 * (TM + 2)) void k_restrict_row(Level32 Lf, const float *__restrict__ x,
+                                                                const float *__restrict__ b, Level32 Lc,
+                                                                float *__restrict__ bc, const SolveState *st, int P,
+                                                                int tilesM)
+{
+    static_assert((TM & 1) == 0, "rows are summed in pairs");
+    if (st->done) return;
+    const Box &B = Lf.B;
+    constexpr int kSlots = 64 * K + 2;
+    __shared__ float ldsX[2][TM + 2][2][kSlots];   // x of a plane by parity, row, element and pair + 1
+    __shared__ float ldsR[2][TM][64 * K];          // the rows' pair sums of the residual by plane parity
+    const RowMarch<TM, K> R = row_march<TM, K>(B, P, tilesM);
+    for (int i = threadIdx.x; i < 2 * (TM + 2) * 2 * kSlots; i += 64 * (TM + 2)) (&ldsX[0][0][0][0])[i] = 0.f;
+    auto plane = [&](int pl) { return (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
+    auto pair = [](const float *row, uint32_t o) { return *reinterpret_cast<const float2 *>(row + o); };
+    auto pairC = [](const uint8_t *row, uint32_t o) { return (unsigned)*reinterpret_cast<const uint16_t *>(row + o); };
+    unsigned cm[K];   // the pair's codes inside the row
+#pragma unroll
+    for (int k = 0; k < K; k++) cm[k] = (R.okA[k] ? 0xFFu : 0u) | (R.okB[k] ? 0xFF00u : 0u);
+    float2 X[4][K], Bv[4][K];
+    unsigned C[4][K];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            X[j][k] = Bv[j][k] = make_float2(0.f, 0.f);
+            C[j][k] = 0u;
+        }
+    const int sp0 = R.s0 - 1;
+#pragma unroll
+    for (int d = 0; d <= 2; d++)
+    {
+        const size_t pl = plane(sp0 + d);
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            X[d][k] = pair(x + pl, R.off[k]);
+            if (d >= 1 && !R.halo)
+            {
+                Bv[d][k] = pair(b + pl, R.off[k]);
+                C[d][k] = pairC(Lf.c8 + pl, R.off[k]);
+            }
+        }
+    }
+    __syncthreads();
+    const int wUp = min(R.wave + 1, TM + 1), wDn = max(R.wave - 1, 0);
+    const int i = R.wave - 1;   // interior row index of the block
+    const bool famRow = !R.halo && !(i & 1);   // the odd row of a row pair: writes the family sums
+    float acc[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) acc[k] = 0.f;
+    // the family sums of plane q from the rows' pair sums in ldsR[q & 1]
+    auto family = [&](int q) {
+        if (!famRow || q < R.s0) return;
+        const size_t cpl = (size_t)((q + 1) >> 1) * (size_t)Lc.B.sS + (size_t)((R.m + 1) >> 1) * (size_t)Lc.B.sM;
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            const int pq = R.lane + 64 * k;
+            acc[k] += ldsR[q & 1][i][pq] + ldsR[q & 1][i + 1][pq];
+            if (!(q & 1) || q == R.s1)
+            {
+                if (R.okA[k]) bc[cpl + pq + 1] = acc[k];
+                acc[k] = 0.f;
+            }
+        }
+    };
+    auto step = [&](int sp, auto phase) {
+        constexpr int U = decltype(phase)::value;
+        constexpr int Sm1 = (U + 3) & 3, S0 = U, S1 = (U + 1) & 3, S2 = (U + 2) & 3;
+        {
+            const size_t pl = plane(sp + 2);
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                X[S2][k] = pair(x + pl, R.off[k]);
+                if (!R.halo)
+                {
+                    Bv[S2][k] = pair(b + pl, R.off[k]);
+                    C[S2][k] = pairC(Lf.c8 + pl, R.off[k]);
+                }
+            }
+        }
+        float(*buf)[2][kSlots] = ldsX[sp & 1];
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            buf[R.wave][0][1 + R.lane + 64 * k] = R.okA[k] || !R.rowIn ? X[S0][k].x : 0.f;
+            buf[R.wave][1][1 + R.lane + 64 * k] = R.okB[k] || !R.rowIn ? X[S0][k].y : 0.f;
+        }
+        __syncthreads();
+        family(sp - 1);
+        if (R.halo) return;
+        const size_t pl = plane(sp);
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            const int q = 1 + R.lane + 64 * k;
+            float r[2];
+#pragma unroll
+            for (int e = 0; e < 2; e++)
+            {
+                const unsigned code = ((C[S0][k] & cm[k]) >> (8 * e)) & 255u;
+                const float xc = e ? X[S0][k].y : X[S0][k].x;
+                const float nb[6] = {e ? X[S1][k].y : X[S1][k].x, e ? X[Sm1][k].y : X[Sm1][k].x, buf[wUp][e][q],
+                                     buf[wDn][e][q], e ? buf[R.wave][0][q + 1] : X[S0][k].y,
+                                     e ? X[S0][k].x : buf[R.wave][1][q - 1]};
+                const float bv = e ? Bv[S0][k].y : Bv[S0][k].x;
+                float v = 0.f;
+                if (code & 64u)
+                {
+                    if (!(code & 128u))
+                    {
+#pragma clang fp contract(off)
+                        // k_residual_restrict_m's rounding: the neighbour sum less n x in one fused step, A x
+                        // rounded on its own, then b - A x
+                        const float s6 = nb[0] + nb[1] + nb[2] + nb[3] + nb[4] + nb[5];
+                        const float ax = Lf.cu * __builtin_fmaf(-(float)__popc(code & 63u), xc, s6);
+                        v = bv - ax;
+                    }
+                    else
+                        v = bv - apply32v(Lf, sp, R.m, 2 * q - 1 + e, (int64_t)pl + R.off[k] + e, code, xc, nb);
+                }
+                r[e] = v;
+            }
+            ldsR[sp & 1][i][R.lane + 64 * k] = r[0] + r[1];
+        }
+    };
+    for (int sp = R.s0; sp <= R.s1; sp += 4)
+    {
+        step(sp, std::integral_constant<int, 1>());
+        if (sp + 1 > R.s1) break;
+        step(sp + 1, std::integral_constant<int, 2>());
+        if (sp + 2 > R.s1) break;
+        step(sp + 2, std::integral_constant<int, 3>());
+        if (sp + 3 > R.s1) break;
+        step(sp + 3, std::integral_constant<int, 0>());
+    }
+    __syncthreads();
+    family(R.s1);
+}
+
 // Residual of every fine cell, summed over each 2 x 2 x 2 family into the coarse right-hand side: fast pairs are
 // neighbouring lanes (shuffle), mid pairs neighbouring rows of the block (LDS), slow pairs consecutive planes of
 // the march (P even, chunks start on an odd plane). A family cut by the end of an odd interior keeps its 1 or 2
@@ -3007,11 +3158,51 @@ hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float 
     return hipGetLastError();
 }
 
+// The whole-row marches of the FP64 Krylov passes and the restriction: 3D boxes with uniform spacing (unit
+// coefficients), rows of at most 256 cells, and every array with element 1 of each row pair-aligned. Tile rows and
+// the chunk length follow from the box alone (never from timing), so the order of the partial sums is a function
+// of the box.
+struct RowPlan
+{
+    bool ok;
+    int K, TM, P, tilesM, G;
+};
+
+RowPlan row_plan(const Box &B, const uint8_t *c8, bool eligible,
+                 std::initializer_list<std::pair<const void *, uintptr_t>> arrays, int tm, int blocksPerCU,
+                 bool evenP = false)
+{
+    RowPlan R{};
+    auto oddAligned = [](const void *p, uintptr_t a) { return p == nullptr || (((uintptr_t)p + (a >> 1)) % a) == 0; };
+    R.ok = eligible && B.threeD && B.nf - 2 <= 256 && (B.sM % 16) == 0 && (B.sS % 16) == 0 && oddAligned(c8, 2);
+    for (const auto &a : arrays) R.ok = R.ok && oddAligned(a.first, a.second);
+    if (!R.ok) return R;
+    constexpr int kCUs = 256;
+    R.K = B.nf - 2 <= 128 ? 1 : 2;
+    R.TM = tm;
+    R.tilesM = std::max(1, (B.nm - 2 + tm - 1) / tm);
+    const int planes = B.ns - 2;
+    const int chunksMax = std::max(1, kCUs * blocksPerCU / R.tilesM);
+    R.P = std::max(1, (planes + chunksMax - 1) / chunksMax);
+    if (evenP) R.P += R.P & 1;
+    R.G = R.tilesM * ((planes + R.P - 1) / R.P);
+    return R;
+}
+
 hipError_t launch_residual_restrict(hipStream_t s, const Level32 &Lf, const float *x, const float *b,
                                     const Level32 &Lc, float *bc, const SolveState *st)
 {
     if (Lf.B.threeD)
     {
+        const RowPlan rp = row_plan(Lf.B, Lf.c8, Lf.uniform != 0, {{x, 8}, {b, 8}}, 14, 1, true);
+        if (rp.ok)
+        {
+            if (rp.K == 1)
+                k_restrict_row<14, 1><<<rp.G, 64 * 16, 0, s>>>(Lf, x, b, Lc, bc, st, rp.P, rp.tilesM);
+            else
+                k_restrict_row<14, 2><<<rp.G, 64 * 16, 0, s>>>(Lf, x, b, Lc, bc, st, rp.P, rp.tilesM);
+            return hipGetLastError();
+        }
         const int P = march_planes(Lf.B);
         k_residual_restrict_m<<<march_grid(Lf.B, P), kBlockThreads, 0, s>>>(Lf, x, b, Lc, bc, st, P);
         return hipGetLastError();
@@ -3086,38 +3277,10 @@ hipError_t launch_cg_dirspmv(hipStream_t s, const Level32 &L, const float *z, fl
     return finish(s, red, G, 1);
 }
 
-// The whole-row FP64 passes: 3D boxes with unit coefficients, rows of at most 256 cells, and every array with
-// element 1 of each row pair-aligned. Tile rows and the chunk length follow from the box alone (never from timing),
-// so the order of the partial sums is a function of the box.
-struct RowPlan
-{
-    bool ok;
-    int K, TM, P, tilesM, G;
-};
-
-RowPlan row_plan(const Box &B, const Tables64 &T, std::initializer_list<std::pair<const void *, uintptr_t>> arrays,
-                 int tm, int blocksPerCU)
-{
-    RowPlan R{};
-    auto oddAligned = [](const void *p, uintptr_t a) { return p == nullptr || (((uintptr_t)p + (a >> 1)) % a) == 0; };
-    R.ok = B.threeD && T.unitCoef && B.nf - 2 <= 256 && (B.sM % 16) == 0 && (B.sS % 16) == 0 && oddAligned(T.c8, 2);
-    for (const auto &a : arrays) R.ok = R.ok && oddAligned(a.first, a.second);
-    if (!R.ok) return R;
-    constexpr int kCUs = 256;
-    R.K = B.nf - 2 <= 128 ? 1 : 2;
-    R.TM = tm;
-    R.tilesM = std::max(1, (B.nm - 2 + tm - 1) / tm);
-    const int planes = B.ns - 2;
-    const int chunksMax = std::max(1, kCUs * blocksPerCU / R.tilesM);
-    R.P = std::max(1, (planes + chunksMax - 1) / chunksMax);
-    R.G = R.tilesM * ((planes + R.P - 1) / R.P);
-    return R;
-}
-
 hipError_t launch_cg_dir64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const float *z,
                            float *pA, float *pB, SolveState *st, double *partials)
 {
-    const RowPlan rp = row_plan(B, T, {{z, 8}, {pA, 8}, {pB, 8}}, 14, 1);
+    const RowPlan rp = row_plan(B, T.c8, T.unitCoef, {{z, 8}, {pA, 8}, {pB, 8}}, 14, 1);
     if (rp.ok)
     {
         const Red red = red_of(partials, st, FIN_PQ, rp.G);
@@ -3139,7 +3302,7 @@ hipError_t launch_cg_dir64(hipStream_t s, const Box &B, const Tables64 &T, const
 hipError_t launch_cg_update64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const float *pA,
                               const float *pB, double *x, double *r, float *r32, SolveState *st, double *partials)
 {
-    const RowPlan rp = row_plan(B, T, {{pA, 8}, {pB, 8}, {x, 16}, {r, 16}, {r32, 8}}, 6, 2);
+    const RowPlan rp = row_plan(B, T.c8, T.unitCoef, {{pA, 8}, {pB, 8}, {x, 16}, {r, 16}, {r32, 8}}, 6, 2);
     if (rp.ok)
     {
         const Red red = red_of(partials, st, FIN_RR, rp.G);
