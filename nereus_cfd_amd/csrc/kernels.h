@@ -30,10 +30,11 @@ hipError_t launch_sum64(hipStream_t s, int64_t n, const int32_t *order, const do
                         double *partials, int gate = 0);
 // Outer refinement step: xn = x + e (e may be null) is written to xout (a different buffer), then r = b - A xn in
 // FP64 with the reference's exact coefficients, stored as FP32 in r32 and/or FP64 in r64; st->sums[1] = ||r||^2.
-// gate: only once the Krylov loop is done (st->done); then also st->accepted = skip or ||r||^2 < acc2.
+// gate: only once the Krylov loop is done (st->done); then also st->accepted = skip or ||r||^2 < acc2, and with
+// sumX (3D) st->sums[2] = the sum of x over the unknowns (the mean projection's sum, in the same pass).
 hipError_t launch_update_residual64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty,
                                     const double *x, const float *e, double *xout, const double *b, float *r32,
-                                    double *r64, SolveState *st, double *partials, int gate = 0);
+                                    double *r64, SolveState *st, double *partials, int gate = 0, int sumX = 0);
 // y = A x in FP64 (the reference's SpMV, row sums in ascending column order).
 hipError_t launch_apply64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const double *x,
                           double *y);

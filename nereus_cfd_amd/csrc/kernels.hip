@@ -390,6 +390,7 @@ enum FinOp
     FIN_SUM2,         // sums[2] = t0
     FIN_TRUE_RES,     // only after the Krylov loop is done: sums[1] = t0; accepted = skip or t0 < acc2
     FIN_SUM2_ACC,     // only once a true residual was accepted: sums[2] = t0
+    FIN_TRUE_RES_SUM, // FIN_TRUE_RES, and sums[2] = t1 (the sum of x over the unknowns, for the mean projection)
     FIN_RZ,           // beta = t0 / rz (0 on the first direction); rz = t0
     FIN_PQ,           // alpha = rz / t0; first = 0
     FIN_RR,           // rr = t0; iters++; done when rr < thr2 (1) or at the cap (2)
@@ -432,7 +433,7 @@ __device__ void fin_apply(SolveState *st, double *partials, int nblk, int nv, in
     if (tid != 0) return;
     __hip_atomic_store(&st->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (fin_in_loop(op) && st->done) return;
-    if (op == FIN_TRUE_RES && !st->done) return;
+    if ((op == FIN_TRUE_RES || op == FIN_TRUE_RES_SUM) && !st->done) return;
     if (op == FIN_SUM2_ACC && !st->accepted) return;
     double t[2] = {0.0, 0.0};
     for (int a = 0; a < nv; a++)
@@ -443,8 +444,10 @@ __device__ void fin_apply(SolveState *st, double *partials, int nblk, int nv, in
     case FIN_SUM1: st->sums[1] = t[0]; break;
     case FIN_SUM2: st->sums[2] = t[0]; break;
     case FIN_TRUE_RES:
+    case FIN_TRUE_RES_SUM:
         st->sums[1] = t[0];
         st->accepted = (st->skip || t[0] < st->acc2) ? 1 : 0;
+        if (op == FIN_TRUE_RES_SUM) st->sums[2] = t[1];
         break;
     case FIN_SUM2_ACC: st->sums[2] = t[0]; break;
     case FIN_RZ:
@@ -2708,7 +2711,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64_t(Box B, Ta
                                                                        const double *__restrict__ b,
                                                                        float *__restrict__ r32,
                                                                        double *__restrict__ r64, Red red,
-                                                                       int P, int gate)
+                                                                       int P, int gate, int sumX)
 {
 #pragma clang fp contract(off)
     __shared__ double Xs[kBlockMid + 2][kBlockFast + 2];
@@ -2730,7 +2733,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64_t(Box B, Ta
                               : own;
     auto X = [&](int64_t i) { return e ? x[i] + (double)e[i] : x[i]; };
     auto plane = [&](int s) { return (int64_t)min(max(s, 0), B.ns - 1) * B.sS; };
-    double rr = 0.0;
+    double rr = 0.0, xs = 0.0;   // <r, r>; with sumX the sum of x over the unknowns
     // registers: own X at s-1, s, s+1; halo X at s; b and code at s
     double Xm = X(plane(M.s0 - 1) + own), Xc = X(plane(M.s0) + own), Xp = X(plane(M.s0 + 1) + own);
     double H = halo ? X(plane(M.s0) + hoff) : 0.0;
@@ -2753,6 +2756,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64_t(Box B, Ta
             const double nb[6] = {Xp, Xm, Xs[tyy + 2][tx + 1], Xs[tyy][tx + 1], Xs[tyy + 1][tx + 2], Xs[tyy + 1][tx]};
             r = bc - (UNI ? apply64u(T, idx, Xc, nb, cc) : apply64v(B, T, ty, s, M.m, M.f, idx, Xc, nb, cc));
             if (e) xout[idx] = Xc;
+            xs += Xc;
         }
         if (M.in)
         {
@@ -2768,8 +2772,16 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64_t(Box B, Ta
         bc = bn;
         cc = cn;
     }
-    double a[1] = {rr};
-    block_sum<1>(a, red, gridDim.x, M.lb);
+    if (sumX)
+    {
+        double a[2] = {rr, xs};
+        block_sum<2>(a, red, gridDim.x, M.lb);
+    }
+    else
+    {
+        double a[1] = {rr};
+        block_sum<1>(a, red, gridDim.x, M.lb);
+    }
 }
 
 // UNI: the unit-coefficient row (apply64u), the one the FP64 Krylov and residual kernels use on uniform grids.
@@ -2954,19 +2966,24 @@ hipError_t launch_sum64(hipStream_t s, int64_t n, const int32_t *order, const do
 
 hipError_t launch_update_residual64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty,
                                     const double *x, const float *e, double *xout, const double *b, float *r32,
-                                    double *r64, SolveState *st, double *partials, int gate)
+                                    double *r64, SolveState *st, double *partials, int gate, int sumX)
 {
-    const FinOp op = gate ? FIN_TRUE_RES : FIN_SUM1;
     if (B.threeD)
     {
+        sumX = gate && sumX ? 1 : 0;
+        const FinOp op = gate ? (sumX ? FIN_TRUE_RES_SUM : FIN_TRUE_RES) : FIN_SUM1;
         const int P = march_planes(B), G = march_grid(B, P);
         const Red red = red_of(partials, st, op, G);
         if (T.unitCoef)
-            k_update_residual64_t<true><<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, red, P, gate);
+            k_update_residual64_t<true><<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, red, P, gate,
+                                                                    sumX);
         else
-            k_update_residual64_t<false><<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, red, P, gate);
-        return finish(s, red, G, 1);
+            k_update_residual64_t<false><<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, red, P,
+                                                                     gate, sumX);
+        return finish(s, red, G, sumX ? 2 : 1);
     }
+    if (sumX) return hipErrorInvalidValue;   // (the 2D residual has no x sum)
+    const FinOp op = gate ? FIN_TRUE_RES : FIN_SUM1;
     const int G = grid_of(B);
     const Red red = red_of(partials, st, op, G);
     k_update_residual64<<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, red, gate);

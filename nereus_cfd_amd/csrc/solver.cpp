@@ -1095,14 +1095,10 @@ int krylov64_scheduled(nppe_solver *h, double refNorm, int cur, double *dX, bool
         h->curIter = -1;
         {
             Scope sc(h, NPPE_K_RESIDUAL64, 29.0 * N);
+            // (with the projection it also sums x over the unknowns for the mean)
             HIP_TRY(h, launch_update_residual64(h->stream, L0.box, h->t64, L0.types, h->dX64[cur], nullptr, nullptr,
-                                                h->dB64, h->dR, h->dR64, h->dState, h->dPartials, 1),
+                                                h->dB64, h->dR, h->dR64, h->dState, h->dPartials, 1, project ? 1 : 0),
                     "true residual");
-        }
-        if (project)
-        {
-            Scope sc(h, NPPE_K_MEAN, 12.0 * n);
-            HIP_TRY(h, launch_sum64(h->stream, h->n, h->dOrder, h->dX64[cur], h->dState, 2, h->dPartials, 1), "mean");
         }
         {
             Scope sc(h, NPPE_K_GATHER, 20.0 * n);
@@ -1146,7 +1142,7 @@ int krylov64_scheduled(nppe_solver *h, double refNorm, int cur, double *dX, bool
         if (passes <= 1) h->lastFirstPass = st.iters;
         if (st.accepted)
         {
-            tr.add((project ? 12.0 * n : 0.0) + 20.0 * n);
+            tr.add(20.0 * n);   // the gather (the mean's sum came with the residual)
             break;
         }
         if (passes >= h->opt.max_refinements || inner >= h->opt.max_iters)
