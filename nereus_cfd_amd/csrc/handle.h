@@ -83,6 +83,7 @@ struct nppe_solver
     bool budgetHit = false;
     int coarseNu = 0;   // sweeps on levels >= 2 (0: the same as the fine levels)
     bool useGraphs = true;                 // replay the inner iterations from a captured hipGraph
+    bool tileTiming = true;                // time the sweep tile candidates at commit (see choose_tiles)
     hipGraphExec_t innerGraph = nullptr;   // one inner iteration, captured at the first solve
     double graphBytesPerIter = 0.0;
     double rate = 0.0;   // observed per-iteration reduction of the inner residual norm (0: none yet)

@@ -14,6 +14,7 @@
 #include <thread>
 #include <vector>
 
+#include "handle.h"
 #include "../../include/nppe.h"
 
 namespace
@@ -153,6 +154,8 @@ struct nppe_server
                 rc = nppe_create(&h, &o);
                 if (rc == NPPE_OK)
                 {
+                    // a commit here lies on the request's path: no tile timing (a shape timed before, or the model)
+                    h->tileTiming = false;
                     const int64_t n = nppe_commit_grid(h, &sg->g);
                     if (n < 0) rc = (int)n;
                 }

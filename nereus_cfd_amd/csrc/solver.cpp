@@ -342,7 +342,9 @@ int sync_state(nppe_solver *h)
 // results (the updates of one colour are independent), so the shape is chosen for speed alone: levels of at least
 // kTuneCells interior cells time every candidate shape on the device at commit (an autotuner; choices are cached
 // for the process per device and level shape), smaller levels take the shape a model picks. NPPE_RBGS_TM /
-// NPPE_RBGS_P force a shape and NPPE_AUTOTUNE=0 turns the timing off.
+// NPPE_RBGS_P force a shape; NPPE_AUTOTUNE=0, or a handle's tileTiming off (the server's handles, whose commits
+// lie on a request's path), turns the timing off: such a handle takes a shape another handle timed for the same
+// level, else the model's.
 constexpr int64_t kTuneCells = 32 * 32 * 32;
 
 struct TileKey
@@ -381,7 +383,8 @@ hipError_t choose_tiles(nppe_solver *h)
     }
     const char *envTM = std::getenv("NPPE_RBGS_TM"), *envP = std::getenv("NPPE_RBGS_P");
     const char *envTune = std::getenv("NPPE_AUTOTUNE");
-    const bool tune = !(envTune && std::atoi(envTune) == 0) && !envTM && !envP;
+    const bool forced = envTM || envP;
+    const bool tune = h->tileTiming && !(envTune && std::atoi(envTune) == 0);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t err = hipSuccess;
     for (size_t l = 0; l < h->levels.size(); l++)
@@ -400,7 +403,7 @@ hipError_t choose_tiles(nppe_solver *h)
         if (envP) L.planesPerChunk = std::max(1, std::atoi(envP));
         L.proPlanes = L.planesPerChunk;
         const int64_t interior = (int64_t)(L.box.threeD ? L.box.ns - 2 : 1) * (L.box.nm - 2) * (L.box.nf - 2);
-        if (!tune || interior < kTuneCells) continue;
+        if (forced || interior < kTuneCells) continue;
         const TileKey key{h->opt.device, L.box.ns, L.box.nm, L.box.nf, L.v.uniform, family};
         {
             std::lock_guard<std::mutex> lock(gTileMu);
@@ -413,6 +416,7 @@ hipError_t choose_tiles(nppe_solver *h)
                 continue;
             }
         }
+        if (!tune) continue;
         if (!e0 && ((err = hipEventCreate(&e0)) != hipSuccess || (err = hipEventCreate(&e1)) != hipSuccess)) break;
         // planes per chunk: a fixed ladder, plus for each tile height the smallest chunk that keeps the launch
         // within one to four rounds of blocks over the CUs (a launch a little over a round leaves most CUs idle
