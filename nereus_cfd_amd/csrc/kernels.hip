@@ -402,7 +402,8 @@ enum FinOp
 __device__ __forceinline__ bool fin_in_loop(int op) { return op >= FIN_RZ; }
 
 // Run by every thread of one block (the last block of a fused reducing kernel, or k_finalize): sums the nblk
-// partials of each value in a fixed order (thread t takes entries t, t + 4 bd, ... into four interleaved sums),
+// partials of each value in a fixed order (thread t takes entries t, t + 4 bd, ... into four interleaved sums;
+// a separate launch reads them in 16-byte pairs),
 // applies op, and resets the ticket for the next reduction. coherent: read with agent-scope atomic loads (the
 // partials were written by the same kernel's other blocks).
 __device__ void fin_apply(SolveState *st, double *partials, int nblk, int nv, int op, bool coherent)
@@ -415,17 +416,45 @@ __device__ void fin_apply(SolveState *st, double *partials, int nblk, int nv, in
     for (int a = 0; a < nv; a++)
     {
         const double *pa = partials + (size_t)a * nblk;
-        double x0 = 0.0, x1 = 0.0, x2 = 0.0, x3 = 0.0;
-        int q = tid;
-        for (; q + 3 * bd < nblk; q += 4 * bd)
+        double x;
+        if (!coherent && ((uintptr_t)pa & 15u) == 0)
         {
-            x0 += ld(pa + q);
-            x1 += ld(pa + q + bd);
-            x2 += ld(pa + q + 2 * bd);
-            x3 += ld(pa + q + 3 * bd);
+            // a separate launch over many partials: 16-byte loads (pairs t, t + bd, ... into four pair sums), twice
+            // the bytes in flight of the scalar loop; a lone last partial is added by thread 0
+            const double2 *p2 = reinterpret_cast<const double2 *>(pa);
+            const int n2 = nblk >> 1;
+            double2 y0 = make_double2(0.0, 0.0), y1 = y0, y2 = y0, y3 = y0;
+            int q = tid;
+            for (; q + 3 * bd < n2; q += 4 * bd)
+            {
+                const double2 a0 = p2[q], a1 = p2[q + bd], a2 = p2[q + 2 * bd], a3 = p2[q + 3 * bd];
+                y0.x += a0.x; y0.y += a0.y;
+                y1.x += a1.x; y1.y += a1.y;
+                y2.x += a2.x; y2.y += a2.y;
+                y3.x += a3.x; y3.y += a3.y;
+            }
+            for (; q < n2; q += bd)
+            {
+                const double2 a0 = p2[q];
+                y0.x += a0.x; y0.y += a0.y;
+            }
+            x = ((y0.x + y0.y) + (y1.x + y1.y)) + ((y2.x + y2.y) + (y3.x + y3.y));
+            if (tid == 0 && (nblk & 1)) x += pa[nblk - 1];
         }
-        for (; q < nblk; q += bd) x0 += ld(pa + q);
-        double x = (x0 + x1) + (x2 + x3);
+        else
+        {
+            double x0 = 0.0, x1 = 0.0, x2 = 0.0, x3 = 0.0;
+            int q = tid;
+            for (; q + 3 * bd < nblk; q += 4 * bd)
+            {
+                x0 += ld(pa + q);
+                x1 += ld(pa + q + bd);
+                x2 += ld(pa + q + 2 * bd);
+                x3 += ld(pa + q + 3 * bd);
+            }
+            for (; q < nblk; q += bd) x0 += ld(pa + q);
+            x = (x0 + x1) + (x2 + x3);
+        }
         x = wave_sum(x);
         if (lane == 0) sh[a][w] = x;
     }
@@ -3211,13 +3240,14 @@ hipError_t launch_residual_restrict(hipStream_t s, const Level32 &Lf, const floa
 {
     if (Lf.B.threeD)
     {
-        const RowPlan rp = row_plan(Lf.B, Lf.c8, Lf.uniform != 0, {{x, 8}, {b, 8}}, 14, 1, true);
+        // 6-row blocks, two per CU (the same time at 256^3, a little less on the small levels than 14-row blocks)
+        const RowPlan rp = row_plan(Lf.B, Lf.c8, Lf.uniform != 0, {{x, 8}, {b, 8}}, 6, 2, true);
         if (rp.ok)
         {
             if (rp.K == 1)
-                k_restrict_row<14, 1><<<rp.G, 64 * 16, 0, s>>>(Lf, x, b, Lc, bc, st, rp.P, rp.tilesM);
+                k_restrict_row<6, 1><<<rp.G, 64 * 8, 0, s>>>(Lf, x, b, Lc, bc, st, rp.P, rp.tilesM);
             else
-                k_restrict_row<14, 2><<<rp.G, 64 * 16, 0, s>>>(Lf, x, b, Lc, bc, st, rp.P, rp.tilesM);
+                k_restrict_row<6, 2><<<rp.G, 64 * 8, 0, s>>>(Lf, x, b, Lc, bc, st, rp.P, rp.tilesM);
             return hipGetLastError();
         }
         const int P = march_planes(Lf.B);
@@ -3297,14 +3327,15 @@ hipError_t launch_cg_dirspmv(hipStream_t s, const Level32 &L, const float *z, fl
 hipError_t launch_cg_dir64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const float *z,
                            float *pA, float *pB, SolveState *st, double *partials)
 {
-    const RowPlan rp = row_plan(B, T.c8, T.unitCoef, {{z, 8}, {pA, 8}, {pB, 8}}, 14, 1);
+    // 6-row blocks, two per CU (measured at 256^3: 57 us against 61 us for one 14-row block per CU)
+    const RowPlan rp = row_plan(B, T.c8, T.unitCoef, {{z, 8}, {pA, 8}, {pB, 8}}, 6, 2);
     if (rp.ok)
     {
         const Red red = red_of(partials, st, FIN_PQ, rp.G);
         if (rp.K == 1)
-            k_cg_dir64_row<14, 1><<<rp.G, 64 * 16, 0, s>>>(B, T, z, pA, pB, st, red, rp.P, rp.tilesM);
+            k_cg_dir64_row<6, 1><<<rp.G, 64 * 8, 0, s>>>(B, T, z, pA, pB, st, red, rp.P, rp.tilesM);
         else
-            k_cg_dir64_row<14, 2><<<rp.G, 64 * 16, 0, s>>>(B, T, z, pA, pB, st, red, rp.P, rp.tilesM);
+            k_cg_dir64_row<6, 2><<<rp.G, 64 * 8, 0, s>>>(B, T, z, pA, pB, st, red, rp.P, rp.tilesM);
         return finish(s, red, rp.G, 1);
     }
     const int P = march_planes(B), G = march_grid(B, P);
