@@ -85,6 +85,8 @@ struct nppe_solver
     bool useGraphs = true;                 // replay the inner iterations from a captured hipGraph
     bool tileTiming = true;                // time the sweep tile candidates at commit (see choose_tiles)
     hipGraphExec_t innerGraph = nullptr;   // one inner iteration, captured at the first solve
+    hipGraphExec_t chunkGraph = nullptr;   // chunkGraphIters inner iterations back to back (a solve's first chunk)
+    int chunkGraphIters = 0;
     double graphBytesPerIter = 0.0;
     double rate = 0.0;   // observed per-iteration reduction of the inner residual norm (0: none yet)
     int lastFirstPass = 0;   // iterations of the last FP64 Krylov solve's first pass (the next one's prediction)

@@ -59,6 +59,12 @@ void free_grid(nppe_solver *h)
         (void)hipGraphExecDestroy(h->innerGraph);
         h->innerGraph = nullptr;
     }
+    if (h->chunkGraph)
+    {
+        (void)hipGraphExecDestroy(h->chunkGraph);
+        h->chunkGraph = nullptr;
+        h->chunkGraphIters = 0;
+    }
     h->rate = 0.0;
     h->lastFirstPass = 0;
     {
@@ -901,6 +907,12 @@ int layout_from_device(nppe_solver *h, const uint8_t *dTypes, const uint8_t *hos
         (void)hipGraphExecDestroy(h->innerGraph);
         h->innerGraph = nullptr;
     }
+    if (h->chunkGraph)
+    {
+        (void)hipGraphExecDestroy(h->chunkGraph);
+        h->chunkGraph = nullptr;
+        h->chunkGraphIters = 0;
+    }
     h->rate = 0.0;
     h->lastFirstPass = 0;
     h->n = info[2 * L];
@@ -1066,6 +1078,32 @@ int krylov64_scheduled(nppe_solver *h, double refNorm, int cur, double *dX, bool
         if (ie != hipSuccess) return hip_fail(h, ie, "graph instantiate");
         h->graphBytesPerIter = it.bytes;
     }
+    // The first chunk of a solve (as many iterations as the last solve's first pass needed) replays as one graph of
+    // that many iterations: consecutive graph launches leave a gap of several microseconds between them. Captured
+    // again only when that count changes; shorter later chunks replay the one-iteration graph.
+    auto chunkGraph = [&](int iters) -> int {
+        if (h->chunkGraph && h->chunkGraphIters == iters) return NPPE_OK;
+        if (h->chunkGraph)
+        {
+            (void)hipGraphExecDestroy(h->chunkGraph);
+            h->chunkGraph = nullptr;
+            h->chunkGraphIters = 0;
+        }
+        hipGraph_t g = nullptr;
+        Traffic it;
+        std::lock_guard<std::mutex> lock(gCaptureMu);
+        HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+        hipError_t ce = hipSuccess;
+        for (int q = 0; q < iters && ce == hipSuccess; q++) ce = inner_iteration(h, it);
+        const hipError_t ee = hipStreamEndCapture(h->stream, &g);
+        if (ce != hipSuccess) return hip_fail(h, ce, "graph capture");
+        if (ee != hipSuccess) return hip_fail(h, ee, "graph capture");
+        const hipError_t ie = hipGraphInstantiate(&h->chunkGraph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) return hip_fail(h, ie, "graph instantiate");
+        h->chunkGraphIters = iters;
+        return NPPE_OK;
+    };
     const double tol = h->opt.tolerance;
     HIP_TRY(h, launch_krylov_init(h->stream, h->dState, tol, refNorm, 0.8, h->opt.max_iters, residualIsB ? 1 : 0),
             "krylov setup");
@@ -1080,21 +1118,31 @@ int krylov64_scheduled(nppe_solver *h, double refNorm, int cur, double *dX, bool
     bool first = true;
     for (;;)
     {
-        for (int q = 0; q < chunk; q++)
+        if (graphs && first && chunk > 1)
         {
-            if (graphs)
+            if (int rc = chunkGraph(chunk)) return rc;
+            HIP_TRY(h, hipGraphLaunch(h->chunkGraph, h->stream), "graph launch");
+            perIter = h->graphBytesPerIter;
+            launched += chunk;
+        }
+        else
+        {
+            for (int q = 0; q < chunk; q++)
             {
-                HIP_TRY(h, hipGraphLaunch(h->innerGraph, h->stream), "graph launch");
-                perIter = h->graphBytesPerIter;
+                if (graphs)
+                {
+                    HIP_TRY(h, hipGraphLaunch(h->innerGraph, h->stream), "graph launch");
+                    perIter = h->graphBytesPerIter;
+                }
+                else
+                {
+                    h->curIter = launched;
+                    Traffic it;
+                    HIP_TRY(h, inner_iteration(h, it), "inner iteration");
+                    perIter = it.bytes;
+                }
+                launched++;
             }
-            else
-            {
-                h->curIter = launched;
-                Traffic it;
-                HIP_TRY(h, inner_iteration(h, it), "inner iteration");
-                perIter = it.bytes;
-            }
-            launched++;
         }
         h->curIter = -1;
         {
