@@ -16,6 +16,7 @@
 #include <type_traits>
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "kernels.h"
 
@@ -3220,7 +3221,10 @@ RowPlan row_plan(const Box &B, const uint8_t *c8, bool eligible,
 {
     RowPlan R{};
     auto oddAligned = [](const void *p, uintptr_t a) { return p == nullptr || (((uintptr_t)p + (a >> 1)) % a) == 0; };
-    R.ok = eligible && B.threeD && B.nf - 2 <= 256 && (B.sM % 16) == 0 && (B.sS % 16) == 0 && oddAligned(c8, 2);
+    // NPPE_TILE_PASSES=1 runs the tile kernels instead (the tests compare the two)
+    const char *tiles = std::getenv("NPPE_TILE_PASSES");
+    R.ok = eligible && !(tiles && std::atoi(tiles) != 0) && B.threeD && B.nf - 2 <= 256 && (B.sM % 16) == 0 &&
+           (B.sS % 16) == 0 && oddAligned(c8, 2);
     for (const auto &a : arrays) R.ok = R.ok && oddAligned(a.first, a.second);
     if (!R.ok) return R;
     constexpr int kCUs = 256;
