@@ -1253,7 +1253,7 @@ struct Prolong
 // With dot non-null the sweep also writes <b, x_out> over the unknowns of each interior row and plane to
 // dot[(plane - 1) * (nm - 2) + row - 1] (the CG's <r, z> when the sweep is the last one of the V-cycle on the
 // finest level): per row, not per block, so the sum over them does not depend on the tile shape.
-template <int TM, int K, bool ZERO, bool PRO>
+template <int TM, int K, bool ZERO, bool PRO, bool NU>
 __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const float *__restrict__ xin,
                                                             float *__restrict__ xout, const float *__restrict__ b,
                                                             int c1, int P, int tilesM, const SolveState *st,
@@ -1320,6 +1320,7 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
     }
     auto plane = [&](const float *base, int pl) { return base + (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
     auto planeC = [&](int pl) { return L.c8 + (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
+    auto plane_off = [&](int pl) { return (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
     // one aligned load per pair (the arrays are allocated so that odd columns start pair-aligned)
     auto pair = [](const float *row, uint32_t o) { return *reinterpret_cast<const float2 *>(row + o); };
     auto pairC = [](const uint8_t *row, uint32_t o) { return (unsigned)*reinterpret_cast<const uint16_t *>(row + o); };
@@ -1494,7 +1495,7 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
         {
             const unsigned code = ((C[S0][k] & cm[k]) >> (8 * E)) & 255u;
             const float bv = E ? Bv[S0][k].y : Bv[S0][k].x;
-            float acc = 0.f;
+            float nb[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};   // s+1, s-1, m+1, m-1, f+1, f-1
             if (!ZERO)
             {
                 const int p = 1 + lane + 64 * k;
@@ -1504,16 +1505,34 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
                 const float outer = E ? O[O0][k].y : O[O0][k].x;
                 if (wave == TM + 1) up = PRO ? ((code & 4u) ? outer + OC[O0][k] : 0.f) : outer;
                 if (wave == 0) dn = PRO ? ((code & 8u) ? outer + OC[O0][k] : 0.f) : outer;
-                // (no leading 0 + : an iterate is never -0, so the sum is the same)
-                acc = E ? X[S1][k].y : X[S1][k].x;
-                acc += E ? X[Sm1][k].y : X[Sm1][k].x;
-                acc += up;
-                acc += dn;
-                acc += E ? nbr : own;
-                acc += E ? own : nbr;
+                nb[0] = E ? X[S1][k].y : X[S1][k].x;
+                nb[1] = E ? X[Sm1][k].y : X[Sm1][k].x;
+                nb[2] = up;
+                nb[3] = dn;
+                nb[4] = E ? nbr : own;
+                nb[5] = E ? own : nbr;
             }
-            float v = gs(acc, bv, code);
-            if (code & 128u) v = gsDirichlet(acc, bv, sp, off[k] + (uint32_t)E);
+            float v;
+            if (NU)
+                v = (code & 64u) ? gs_update<false>(L, sp, m, fa[k] + E, (int64_t)plane_off(sp) + off[k] + E, code,
+                                                    bv, nb)
+                                 : 0.f;
+            else
+            {
+                // (no leading 0 + : an iterate is never -0, so the sum is the same)
+                float acc = 0.f;
+                if (!ZERO)
+                {
+                    acc = nb[0];
+                    acc += nb[1];
+                    acc += nb[2];
+                    acc += nb[3];
+                    acc += nb[4];
+                    acc += nb[5];
+                }
+                v = gs(acc, bv, code);
+                if (code & 128u) v = gsDirichlet(acc, bv, sp, off[k] + (uint32_t)E);
+            }
             rnew[k] = v;
             R[S0][k] = v;
         }
@@ -1534,15 +1553,26 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
                 const unsigned cc = C[Sm1][k] & cm[k];
                 const unsigned code = (cc >> (8 * E)) & 255u;
                 const int f = fa[k] + E;
-                float acc = R[S0][k];
-                acc += R[Sm2][k];
-                acc += rq[wave + 1][f];
-                acc += rq[wave - 1][f];
-                acc += E ? rq[wave][f + 1] : R[Sm1][k];
-                acc += E ? R[Sm1][k] : rq[wave][f - 1];
                 const float bv = E ? Bv[Sm1][k].y : Bv[Sm1][k].x;
-                float out = gs(acc, bv, code);
-                if (code & 128u) out = gsDirichlet(acc, bv, q, off[k] + (uint32_t)E);
+                float out;
+                if (NU)
+                {
+                    const float nb[6] = {R[S0][k], R[Sm2][k], rq[wave + 1][f], rq[wave - 1][f],
+                                         E ? rq[wave][f + 1] : R[Sm1][k], E ? R[Sm1][k] : rq[wave][f - 1]};
+                    out = (code & 64u) ? gs_update<false>(L, q, m, f, (int64_t)q * B.sS + off[k] + E, code, bv, nb)
+                                       : 0.f;
+                }
+                else
+                {
+                    float acc = R[S0][k];
+                    acc += R[Sm2][k];
+                    acc += rq[wave + 1][f];
+                    acc += rq[wave - 1][f];
+                    acc += E ? rq[wave][f + 1] : R[Sm1][k];
+                    acc += E ? R[Sm1][k] : rq[wave][f - 1];
+                    out = gs(acc, bv, code);
+                    if (code & 128u) out = gsDirichlet(acc, bv, q, off[k] + (uint32_t)E);
+                }
                 const float va = E ? R[Sm1][k] : out, vb = E ? out : R[Sm1][k];
                 if (dot)
                 {
@@ -3089,7 +3119,7 @@ bool rbgs_pairs_aligned(const Level32 &L, const float *xin, const float *xout, c
 int rbgs_family(const Level32 &L, const float *xin, const float *xout, const float *b)
 {
     const Box &B = L.B;
-    if (B.threeD && L.uniform && B.nf - 2 <= 512 && rbgs_pairs_aligned(L, xin, xout, b)) return RBGS_ROW;
+    if (B.threeD && B.nf - 2 <= 512 && rbgs_pairs_aligned(L, xin, xout, b)) return RBGS_ROW;
     return B.threeD ? RBGS_MARCH : RBGS_2D;
 }
 
@@ -3150,16 +3180,19 @@ hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float 
         Prolong pro{nullptr, 0, 0, 1, 1};
         if (prolongFrom) pro = Prolong{prolongXc, prolongFrom->B.sS, (int)prolongFrom->B.sM, prolongFrom->B.ns,
                                        prolongFrom->B.nf};
-#define NPPE_ROW(TMv, Kv)                                                                                            \
-    (zero ? (k_rbgs_row<TMv, Kv, true, false><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st,             \
-                                                                       rzPartials, pro),                              \
-             0)                                                                                                      \
-          : (prolongFrom ? (k_rbgs_row<TMv, Kv, false, true><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, \
-                                                                                     st, rzPartials, pro),            \
-                            0)                                                                                       \
-                         : (k_rbgs_row<TMv, Kv, false, false><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, \
-                                                                                       st, rzPartials, pro),          \
+#define NPPE_ROW_NU(TMv, Kv, NUv)                                                                                     \
+    (zero ? (k_rbgs_row<TMv, Kv, true, false, NUv><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st,        \
+                                                                            rzPartials, pro),                         \
+             0)                                                                                                       \
+          : (prolongFrom ? (k_rbgs_row<TMv, Kv, false, true, NUv><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P,      \
+                                                                                          tilesM, st, rzPartials,      \
+                                                                                          pro),                        \
+                            0)                                                                                        \
+                         : (k_rbgs_row<TMv, Kv, false, false, NUv><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P,     \
+                                                                                            tilesM, st, rzPartials,    \
+                                                                                            pro),                      \
                             0)))
+#define NPPE_ROW(TMv, Kv) (L.uniform ? NPPE_ROW_NU(TMv, Kv, false) : NPPE_ROW_NU(TMv, Kv, true))
         if (TM == 14)
         {
             if (K == 1) NPPE_ROW(14, 1);
@@ -3169,6 +3202,7 @@ hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float 
         else if (K == 2) NPPE_ROW(6, 2);
         else NPPE_ROW(6, 4);
 #undef NPPE_ROW
+#undef NPPE_ROW_NU
         return hipGetLastError();
     }
     if (B.threeD)
