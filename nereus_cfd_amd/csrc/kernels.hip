@@ -1219,6 +1219,40 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_march(Level32 L, const f
     }
 }
 
+// gs_update<false> (op32's coefficients and diagonal, then the Gauss-Seidel quotient) from width-table entries the
+// caller already holds: tS = (wS, gS[s], gS[s-1], hS) of the plane, tM of the row, tF of the column. The same
+// products, sums and quotient in the same order, so the same value bit for bit.
+__device__ __forceinline__ float gs_stretched(const uint16_t *c16, int64_t idx, unsigned code, float bv,
+                                             const float (&nb)[6], float4 tS, float4 tM, float4 tF)
+{
+    const float wS = tS.x, wM = tM.x, wF = tF.x;
+    const float aS = wM * wF, aM = wS * wF, aF = wS * wM;
+    const float c[6] = {aS * tS.y, aS * tS.z, aM * tM.y, aM * tM.z, aF * tF.y, aF * tF.z};
+    const unsigned coupled = code & 63u;
+    float diag = 0.f;
+    if (!(code & 128u))
+    {
+#pragma unroll
+        for (int q = 0; q < 6; q++) diag -= ((coupled >> q) & 1u) ? c[q] : 0.f;
+    }
+    else
+    {
+        const float rS = aS * tS.w, rM = aM * tM.w, rF = aF * tF.w;
+        const float ring[6] = {rS, rS, rM, rM, rF, rF};
+        const unsigned cc = c16[idx];
+#pragma unroll
+        for (int q = 0; q < 6; q++)
+        {
+            const unsigned st = (cc >> (2 * q)) & 3u;
+            diag -= st == 3u ? ring[q] : (st != 0u ? c[q] : 0.f);
+        }
+    }
+    float acc = 0.f;
+#pragma unroll
+    for (int q = 0; q < 6; q++) acc += ((coupled >> q) & 1u) ? c[q] * nb[q] : 0.f;
+    return diag != 0.f ? (bv - acc) / diag : 0.f;
+}
+
 // The coarse correction a whole-row sweep folds in (x_in + P x_c where x is used): the coarse level's iterate and
 // box strides.
 struct Prolong
@@ -1310,6 +1344,27 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
     auto planeXC = [&](int pl) {
         const int sc = (min(max(pl, 0), B.ns - 1) + 1) >> 1;
         return pro.xc + (size_t)min(sc, pro.ns - 1) * (size_t)pro.sS;
+    };
+    // NU: the width-table entries of this lane's columns and of this wave's row, fixed over the march (indices
+    // clamped into the interior: a cell outside it is no unknown and never uses them)
+    float4 tF[K][2], tM = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (NU)
+    {
+        const Tables32 &T = L.T;
+#pragma unroll
+        for (int k = 0; k < K; k++)
+#pragma unroll
+            for (int e = 0; e < 2; e++)
+            {
+                const int f = min(max(fa[k] + e, 1), B.nf - 2);
+                tF[k][e] = make_float4(T.wF[f], T.gF[f], T.gF[f - 1], T.hF[f]);
+            }
+        const int mr = min(max(m, 1), B.nm - 2);
+        tM = make_float4(T.wM[mr], T.gM[mr], T.gM[mr - 1], T.hM[mr]);
+    }
+    auto tablesS = [&](int pl) {
+        const int sr = min(max(pl, 1), B.ns - 2);
+        return make_float4(L.T.wS[sr], L.T.gS[sr], L.T.gS[sr - 1], L.T.hS[sr]);
     };
     for (int i = threadIdx.x; i < 2 * (TM + 2) * kCols; i += 64 * (TM + 2)) (&red[0][0][0])[i] = 0.f;
     for (int i = threadIdx.x; i < (TM + 2) * 2 * kSlots; i += 64 * (TM + 2)) (&old[0][0][0])[i] = 0.f;
@@ -1514,8 +1569,8 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
             }
             float v;
             if (NU)
-                v = (code & 64u) ? gs_update<false>(L, sp, m, fa[k] + E, (int64_t)plane_off(sp) + off[k] + E, code,
-                                                    bv, nb)
+                v = (code & 64u) ? gs_stretched(L.c16, (int64_t)plane_off(sp) + off[k] + E, code, bv, nb, tablesS(sp),
+                                                tM, tF[k][E])
                                  : 0.f;
             else
             {
@@ -1559,7 +1614,8 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
                 {
                     const float nb[6] = {R[S0][k], R[Sm2][k], rq[wave + 1][f], rq[wave - 1][f],
                                          E ? rq[wave][f + 1] : R[Sm1][k], E ? R[Sm1][k] : rq[wave][f - 1]};
-                    out = (code & 64u) ? gs_update<false>(L, q, m, f, (int64_t)q * B.sS + off[k] + E, code, bv, nb)
+                    out = (code & 64u) ? gs_stretched(L.c16, (int64_t)q * B.sS + off[k] + E, code, bv, nb, tablesS(q),
+                                                      tM, tF[k][E])
                                        : 0.f;
                 }
                 else
