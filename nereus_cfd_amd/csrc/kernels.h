@@ -66,7 +66,11 @@ enum { RBGS_ROW = 0, RBGS_MARCH = 1, RBGS_2D = 2 };
 int rbgs_family(const Level32 &L, const float *xin, const float *xout, const float *b);
 void rbgs_tile_candidates(int family, std::vector<int> &tileRows);
 // Blocks of one sweep launch of that family and tile shape.
-int64_t rbgs_blocks(const Box &B, int family, int tileRows, int planesPerChunk);
+// (folding: the sweep that folds the prolongation in, whose tiles the whole-row family picks on its own)
+int64_t rbgs_blocks(const Box &B, int family, int tileRows, int planesPerChunk, bool folding = false,
+                    bool uniform = true);
+// The tile rows the whole-row sweep runs for the tile height asked.
+int rbgs_row_tile(const Box &B, int tileRows, bool folding, bool uniform);
 hipError_t launch_residual_restrict(hipStream_t s, const Level32 &Lf, const float *x, const float *b,
                                     const Level32 &Lc, float *bc, const SolveState *st);
 hipError_t launch_prolong(hipStream_t s, const Level32 &Lf, float *x, const Level32 &Lc, const float *xc,

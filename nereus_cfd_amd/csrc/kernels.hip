@@ -3179,13 +3179,20 @@ int rbgs_family(const Level32 &L, const float *xin, const float *xout, const flo
     return B.threeD ? RBGS_MARCH : RBGS_2D;
 }
 
-int64_t rbgs_blocks(const Box &B, int family, int tileRows, int planesPerChunk)
+int rbgs_row_tile(const Box &B, int tileRows, bool folding, bool uniform)
+{
+    const int K = B.nf - 2 <= 128 ? 1 : (B.nf - 2 <= 256 ? 2 : 4);
+    if (K == 2 && folding && uniform) return 10;
+    return (tileRows >= 10 && (K == 1 || (K == 2 && !folding))) ? 14 : 6;
+}
+
+int64_t rbgs_blocks(const Box &B, int family, int tileRows, int planesPerChunk, bool folding, bool uniform)
 {
     const int P = std::max(1, planesPerChunk);
     const int64_t chunks = B.threeD ? std::max(1, (B.ns - 2 + P - 1) / P) : 1;
     if (family == RBGS_ROW)
     {
-        const int TM = (tileRows >= 10 && B.nf - 2 <= 256) ? 14 : 6;
+        const int TM = rbgs_row_tile(B, tileRows, folding, uniform);
         return (int64_t)std::max(1, (B.nm - 2 + TM - 1) / TM) * chunks;
     }
     if (family == RBGS_MARCH)
@@ -3225,9 +3232,10 @@ hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float 
     if (rbgs_family(L, xin, xout, b) == RBGS_ROW)
     {
         const int K = B.nf - 2 <= 128 ? 1 : (B.nf - 2 <= 256 ? 2 : 4);
-        // 14-row tiles for K <= 2 when asked, but not for the prolongation-folding K = 2 sweep (1024-thread blocks
-        // cap its registers at 128, which it exceeds; 10-row tiles of it measured slower than 6-row ones at 256^3)
-        const int TM = (tileRows >= 10 && (K == 1 || (K == 2 && !prolongFrom))) ? 14 : 6;
+        // 14-row tiles for K <= 2 when asked; the prolongation-folding K = 2 sweep in 10-row tiles on uniform levels
+        // (1024-thread blocks cap its registers at 128, which it exceeds; its 157 fit 768-thread blocks, and at 256^3
+        // it took 61 us there against 69 us in 6-row tiles)
+        const int TM = rbgs_row_tile(B, tileRows, prolongFrom != nullptr, L.uniform != 0);
         const int tilesM = std::max(1, (B.nm - 2 + TM - 1) / TM);
         const int P = std::max(1, planesPerChunk);
         const int chunks = std::max(1, (B.ns - 2 + P - 1) / P);
@@ -3249,7 +3257,9 @@ hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float 
                                                                                             pro),                      \
                             0)))
 #define NPPE_ROW(TMv, Kv) (L.uniform ? NPPE_ROW_NU(TMv, Kv, false) : NPPE_ROW_NU(TMv, Kv, true))
-        if (TM == 14)
+        if (TM == 10)
+            NPPE_ROW(10, 2);
+        else if (TM == 14)
         {
             if (K == 1) NPPE_ROW(14, 1);
             else NPPE_ROW(14, 2);

@@ -427,12 +427,12 @@ hipError_t choose_tiles(nppe_solver *h)
         // planes per chunk: a fixed ladder, plus for each tile height the smallest chunk that keeps the launch
         // within one to four rounds of blocks over the CUs (a launch a little over a round leaves most CUs idle
         // for its last blocks)
-        auto candidates = [&](int tm) {
+        auto candidates = [&](int tm, bool folding) {
             std::vector<int> ps = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32};
             if (L.box.threeD)
                 for (int k = 1; k <= 4; k++)
                     for (int P = 1; P <= L.box.ns - 2; P++)
-                        if (rbgs_blocks(L.box, family, tm, P) <= (int64_t)k * cus)
+                        if (rbgs_blocks(L.box, family, tm, P, folding, L.v.uniform != 0) <= (int64_t)k * cus)
                         {
                             ps.push_back(P);
                             break;
@@ -461,7 +461,7 @@ hipError_t choose_tiles(nppe_solver *h)
         float bestMs = 1e30f;
         int bestTM = L.tileRows, bestP = L.planesPerChunk;
         for (int tm : tms)
-            for (int P : candidates(tm))
+            for (int P : candidates(tm, false))
             {
                 const float ms = timeIt([&] { return launch_rbgs(h->stream, L.v, xin, xout, b, 1, 0, tm, P, h->dState); });
                 if (err != hipSuccess) break;
@@ -476,9 +476,8 @@ hipError_t choose_tiles(nppe_solver *h)
         if (family == RBGS_ROW && l + 1 < h->levels.size() && h->levels[l + 1].x[0])
         {
             const DevLevel &C = h->levels[l + 1];
-            const int tmPro = (bestTM >= 10 && L.box.nf - 2 <= 128) ? 14 : 6;
             float bestPro = 1e30f;
-            for (int P : candidates(tmPro))
+            for (int P : candidates(bestTM, true))
             {
                 const float ms = timeIt([&] {
                     return launch_rbgs(h->stream, L.v, xin, xout, b, 1, 0, bestTM, P, h->dState, nullptr, &C.v,
