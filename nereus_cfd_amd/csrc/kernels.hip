@@ -520,6 +520,19 @@ __global__ __launch_bounds__(1024) void k_finalize(SolveState *st, double *parti
     fin_apply(st, partials, nblk, nv, op, false);
 }
 
+// The <r, z> of the whole-row sweep from its partials, one per interior row and plane, in two fixed stages: block q
+// sums the rows [q seg, (q + 1) seg) (thread t the entries t, t + 256, ... of them), and the last block to finish
+// sums the block sums in index order and applies FIN_RZ (block_sum's fused finalisation). The order depends on the
+// row count only, never on the sweep's tiles.
+__global__ __launch_bounds__(kBlockThreads) void k_rz_rows_sum(SolveState *st, const double *__restrict__ rows,
+                                                               int nrows, int seg, double *blockSums)
+{
+    const int blk = blockIdx.x, e = min(nrows, (blk + 1) * seg);
+    double v[1] = {0.0};
+    for (int q = blk * seg + (int)threadIdx.x; q < e; q += blockDim.x) v[0] += rows[q];
+    block_sum<1>(v, Red{blockSums, st, (int)FIN_RZ, 1}, gridDim.x, blk);
+}
+
 // One block for a separate finalisation: 256 threads, or 1024 for many partials.
 inline int fin_threads(int nblk) { return nblk > 8192 ? 1024 : kBlockThreads; }
 
@@ -3382,7 +3395,10 @@ hipError_t launch_coarse_solve(hipStream_t s, int n, const int32_t *order, const
 hipError_t launch_rz_rows_finalize(hipStream_t s, const Box &B, SolveState *st, double *partials)
 {
     const int nrows = (B.ns - 2) * (B.nm - 2);
-    k_finalize<<<1, fin_threads(nrows), 0, s>>>(st, partials, nrows, 1, (int)FIN_RZ);
+    const int nblk = std::min(kFuseBlocks, std::max(1, (nrows + kBlockThreads - 1) / kBlockThreads));
+    const int seg = (nrows + nblk - 1) / nblk;
+    // the block sums follow the row partials (the partials buffer holds at least twice the row count)
+    k_rz_rows_sum<<<nblk, kBlockThreads, 0, s>>>(st, partials, nrows, seg, partials + ((nrows + 1) & ~1));
     return hipGetLastError();
 }
 
