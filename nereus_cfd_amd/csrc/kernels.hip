@@ -1279,9 +1279,10 @@ struct Prolong
 //
 // As k_rbgs_march, but a wave spans the whole interior row: lane l owns the pairs (1 + 2p, 2 + 2p) for
 // p = l, l + 64, ... (K pairs), so no tile edge exists along the fast axis and no edge values are recomputed. The
-// block is TM rows plus one halo row above and below (halo waves publish first-colour values only). Plane bases are
-// scalar and per-lane offsets 32-bit; loads outside the interior are clamped into the box, so the loop has no bounds
-// branches. Uniform levels only (constant coefficients).
+// block is TM rows plus one halo row above and below (halo waves publish first-colour values only). The old x and
+// the first-colour values pass between waves through LDS double-buffered by plane parity, one barrier per plane
+// step. Plane bases are scalar and per-lane offsets 32-bit; loads outside the interior are clamped into the box, so
+// the loop has no bounds branches. Uniform levels only (constant coefficients).
 //
 // An update adds all six neighbours without testing the code's neighbour bits. A neighbour whose bit is clear is a
 // ring cell or an interior cell that is no unknown, and every iterate holds +0 there (the buffers are cleared when a
@@ -1311,7 +1312,7 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
     constexpr int kCols = 128 * K + 2;
     constexpr int kSlots = 64 * K + 2;
     __shared__ float red[2][TM + 2][kCols];      // first-colour values by column f, rows m0-1 .. m0+TM
-    __shared__ float old[TM + 2][2][kSlots];     // old x of plane sp by row, element (a = 0, b = 1) and pair + 1
+    __shared__ float old[2][TM + 2][2][kSlots];  // old x of plane sp by parity, row, element (a = 0, b = 1), pair + 1
     __shared__ float2 tab[128];                  // by code & 127: (1/n, all ones) for an unknown, (0, 0) otherwise
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -1380,7 +1381,7 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
         return make_float4(L.T.wS[sr], L.T.gS[sr], L.T.gS[sr - 1], L.T.hS[sr]);
     };
     for (int i = threadIdx.x; i < 2 * (TM + 2) * kCols; i += 64 * (TM + 2)) (&red[0][0][0])[i] = 0.f;
-    for (int i = threadIdx.x; i < (TM + 2) * 2 * kSlots; i += 64 * (TM + 2)) (&old[0][0][0])[i] = 0.f;
+    for (int i = threadIdx.x; i < 2 * (TM + 2) * 2 * kSlots; i += 64 * (TM + 2)) (&old[0][0][0][0])[i] = 0.f;
     if (threadIdx.x < 128)
     {
         const unsigned c = threadIdx.x;
@@ -1529,6 +1530,7 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
             }
         }
         // publish this row's old x of plane sp for the neighbouring rows and pairs (zero past the row end)
+        float(*ob)[2][kSlots] = old[sp & 1];
         if (!ZERO)
         {
             if (ragged)
@@ -1536,8 +1538,8 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
 #pragma unroll
                 for (int k = 0; k < K; k++)
                 {
-                    old[wave][0][1 + lane + 64 * k] = okA[k] ? X[S0][k].x : 0.f;
-                    old[wave][1][1 + lane + 64 * k] = okB[k] ? X[S0][k].y : 0.f;
+                    ob[wave][0][1 + lane + 64 * k] = okA[k] ? X[S0][k].x : 0.f;
+                    ob[wave][1][1 + lane + 64 * k] = okB[k] ? X[S0][k].y : 0.f;
                 }
             }
             else
@@ -1545,12 +1547,14 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
 #pragma unroll
                 for (int k = 0; k < K; k++)
                 {
-                    old[wave][0][1 + lane + 64 * k] = X[S0][k].x;
-                    old[wave][1][1 + lane + 64 * k] = X[S0][k].y;
+                    ob[wave][0][1 + lane + 64 * k] = X[S0][k].x;
+                    ob[wave][1][1 + lane + 64 * k] = X[S0][k].y;
                 }
             }
         }
-        // (1) old[] is complete; readers of red[sp & 1] (the second colour of plane sp-2) are done
+        // The one barrier of the plane step: old[sp & 1] is complete, and so is red[(sp - 1) & 1], written in the
+        // step before; a wave past it knows that every wave has finished the step before, whose readers of
+        // old[(sp + 1) & 1] (its first colour) and of red[sp & 1] (its second colour, at plane sp - 2) are done.
         __syncthreads();
         if (PRO)
         {
@@ -1568,8 +1572,8 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
             {
                 const int p = 1 + lane + 64 * k;
                 const float own = E ? X[S0][k].x : X[S0][k].y;                 // the pair's other element
-                const float nbr = E ? old[wave][0][p + 1] : old[wave][1][p - 1];   // next pair's a / previous b
-                float up = old[wUp][E][p], dn = old[wDn][E][p];   // rows m+1, m-1 at plane sp, element E
+                const float nbr = E ? ob[wave][0][p + 1] : ob[wave][1][p - 1];   // next pair's a / previous b
+                float up = ob[wUp][E][p], dn = ob[wDn][E][p];   // rows m+1, m-1 at plane sp, element E
                 const float outer = E ? O[O0][k].y : O[O0][k].x;
                 if (wave == TM + 1) up = PRO ? ((code & 4u) ? outer + OC[O0][k] : 0.f) : outer;
                 if (wave == 0) dn = PRO ? ((code & 8u) ? outer + OC[O0][k] : 0.f) : outer;
@@ -1606,8 +1610,6 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
         }
 #pragma unroll
         for (int k = 0; k < K; k++) red[sp & 1][wave][fa[k] + E] = rnew[k];
-        // (2) red[sp & 1] is complete; readers of old[] are done
-        __syncthreads();
         const int q = sp - 1;
         if (!haloRow && q >= s0 && q <= s1)
         {
