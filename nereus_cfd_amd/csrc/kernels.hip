@@ -1739,14 +1739,21 @@ __global__ __launch_bounds__(kBlockThreads) void k_coarse_solve(int n, const int
 {
     if (st->done) return;
     __shared__ double rb[kMaxDenseCoarse];
-    for (int q = threadIdx.x; q < n; q += kBlockThreads) rb[q] = (double)b[order[q]];
-    __syncthreads();
+    constexpr int kPer = kMaxDenseCoarse / 64;
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * (kBlockThreads / 64) + (threadIdx.x >> 6);
+    // this lane's entries of the row, loaded before the gather of b so that the two loads overlap
+    const double *Mr = M + (size_t)min(row, n - 1) * n;
+    double mr[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; j++) mr[j] = (row < n && lane + 64 * j < n) ? Mr[lane + 64 * j] : 0.0;
+    for (int q = threadIdx.x; q < n; q += kBlockThreads) rb[q] = (double)b[order[q]];
+    __syncthreads();
     if (row >= n) return;
-    const double *Mr = M + (size_t)row * n;
     double acc = 0.0;
-    for (int q = lane; q < n; q += 64) acc += Mr[q] * rb[q];
+#pragma unroll
+    for (int j = 0; j < kPer; j++)
+        if (lane + 64 * j < n) acc += mr[j] * rb[lane + 64 * j];
     acc = wave_sum(acc);
     if (lane == 0) x[order[row]] = (float)acc;
 }
