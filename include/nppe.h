@@ -34,7 +34,9 @@ extern "C" {
 #define NPPE_ABI_VERSION 1
 
 /* Return codes. NPPE_NOT_CONVERGED is a warning: the result is filled and x holds the last iterate (the reference
- * warns once and still returns its iterations, PressureSettings.h:472-479). */
+ * warns once and still returns its iterations, PressureSettings.h:472-479). NPPE_ERR_NONFINITE: a non-finite
+ * right-hand side or guess, or a NaN/infinity that arose inside the Krylov recursion (the solve stops there; x is
+ * not written). */
 enum nppe_status
 {
     NPPE_OK = 0,
@@ -139,7 +141,8 @@ int nppe_create(nppe_solver **out, const nppe_options *opt);
 /* DestroyPressureBackend (PressureSettings.h:516-538). NULL is accepted. */
 int nppe_destroy(nppe_solver *h);
 
-/* Message of the last error on this handle (empty when none); a NULL handle gives the last creation error. */
+/* Message of the last error on this handle (empty when none); a NULL handle gives the calling thread's last
+ * creation error. */
 const char *nppe_last_error(const nppe_solver *h);
 
 /* CommitPressureMatrix (PressureSettings.h:542-574) and the assembly it follows (PressureSolver3D.cpp:128-178):
@@ -163,6 +166,29 @@ int nppe_update_layout_device(nppe_solver *h, const uint8_t *d_cell_types);
 /* Unknown count and singularity of the committed system (PressureSystem::A.rows(), ::singular,
  * PressureSettings.h:284-297). */
 int nppe_system_info(const nppe_solver *h, int64_t *n, int32_t *singular);
+
+/* What the committed system's graph holds beyond PressureSystem::singular (no counterpart in the reference, which
+ * assembles such systems without a word): its unknowns split into connected components (unknowns joined by a FLUID
+ * face); a component with no Dirichlet face anywhere is "floating" -- A is singular on it, with the component's
+ * indicator as null vector, so b must sum to zero over it -- and an unknown with no FLUID and no Dirichlet neighbour
+ * at all is an empty row of A (the reference skips its zero diagonal, PressureSolver3D.cpp:168-169): b must be 0
+ * there. The reference's compatibility fix removes the mean of b only when the whole system is singular
+ * (PressureSolver3D.cpp:71-74, :300-307), so on a system with a Dirichlet face somewhere, an isolated unknown or a
+ * floating pocket with nonzero b is an inconsistent system no solver converges on (MGPCG then runs to its cap and
+ * returns NPPE_NOT_CONVERGED; CG64's recursion goes non-finite and returns NPPE_ERR_NONFINITE). Computed on the
+ * host from the committed layout when asked. */
+typedef struct nppe_system_report
+{
+    int64_t unknowns;            /* PressureSystem::A.rows() */
+    int32_t singular;            /* PressureSystem::singular: no Dirichlet face anywhere */
+    int32_t levels;              /* multigrid levels */
+    int64_t components;          /* connected components of the unknowns */
+    int64_t floating_components; /* components without a Dirichlet face (each one a null vector of A) */
+    int64_t isolated_unknowns;   /* unknowns whose row of A is empty (no FLUID, no Dirichlet neighbour) */
+    int64_t coarsest_unknowns;   /* unknowns of the coarsest level (its dense solve) */
+    uint64_t device_bytes;       /* device memory the handle holds */
+} nppe_system_report;
+int nppe_describe_system(nppe_solver *h, nppe_system_report *out);
 
 /* The IDP numbering of the committed system: order[q] = box index of unknown q (n entries). */
 int nppe_unknown_order(const nppe_solver *h, int32_t *order);

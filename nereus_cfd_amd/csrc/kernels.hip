@@ -468,6 +468,13 @@ __device__ void fin_apply(SolveState *st, double *partials, int nblk, int nv, in
     double t[2] = {0.0, 0.0};
     for (int a = 0; a < nv; a++)
         for (int q = 0; q < nw; q++) t[a] += sh[a][q];
+    if (fin_in_loop(op) && (!isfinite(t[0]) || !isfinite(t[1])))
+    {
+        // a NaN or infinity in the recursion (an inconsistent singular system drives p^T A p to round-off and
+        // alpha past the range): stop the loop; no comparison with a NaN would ever end it
+        st->done = kDoneNonFinite;
+        return;
+    }
     switch (op)
     {
     case FIN_SUM0: st->sums[0] = t[0]; break;
@@ -1777,6 +1784,33 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_rz(Level32 L, const float 
     }
     double a[1] = {v};
     block_sum<1>(a, red, gridDim.x, tr.lb);
+}
+
+// <r, z> over the unknowns of each interior row and plane of a 3D box, as the whole-row sweep writes it when it is
+// the V-cycle's last one (one wave per row; lane l sums the cell pairs (1 + 2p, 2 + 2p), p = l, l + 64, ..., in
+// that order, then the wave sums its lanes), so a level swept by another kernel gets the same partials bit for bit
+// and k_rz_rows_sum the same <r, z>.
+__global__ __launch_bounds__(kBlockThreads) void k_rz_rows(Level32 L, const float *__restrict__ r,
+                                                           const float *__restrict__ z, const SolveState *st,
+                                                           double *__restrict__ rows)
+{
+    if (st->done) return;
+    const Box &B = L.B;
+    const int lane = threadIdx.x & 63;
+    const int nrm = B.nm - 2;
+    const int64_t row = (int64_t)blockIdx.x * (kBlockThreads / 64) + (threadIdx.x >> 6);
+    if (row >= (int64_t)(B.ns - 2) * nrm) return;   // whole waves: the row is wave-uniform
+    const int q = (int)(row / nrm) + 1, m = (int)(row % nrm) + 1;
+    const int64_t base = B.at(q, m, 0);
+    double bx = 0.0;
+    for (int p = lane; 2 * p + 1 <= B.nf - 2; p += 64)
+    {
+        const int64_t a = base + 1 + 2 * p;
+        bx += (L.c8[a] & 64u) ? (double)r[a] * (double)z[a] : 0.0;
+        if (2 * p + 2 <= B.nf - 2) bx += (L.c8[a + 1] & 64u) ? (double)r[a + 1] * (double)z[a + 1] : 0.0;
+    }
+    const double v = wave_sum(bx);
+    if (lane == 0) rows[row] = v;
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_cg_direction(Level32 L, const float *__restrict__ z,
@@ -3197,6 +3231,9 @@ bool rbgs_pairs_aligned(const Level32 &L, const float *xin, const float *xout, c
 int rbgs_family(const Level32 &L, const float *xin, const float *xout, const float *b)
 {
     const Box &B = L.B;
+    // NPPE_RBGS_MARCH=1 keeps 3D levels on the marching sweep (the tests compare it with the whole-row sweep)
+    const char *march = std::getenv("NPPE_RBGS_MARCH");
+    if (march && std::atoi(march) != 0 && B.threeD) return RBGS_MARCH;
     if (B.threeD && B.nf - 2 <= 512 && rbgs_pairs_aligned(L, xin, xout, b)) return RBGS_ROW;
     return B.threeD ? RBGS_MARCH : RBGS_2D;
 }
@@ -3248,7 +3285,13 @@ hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float 
     {
         hipError_t e = launch_rbgs(s, L, xin, xout, b, c1, zero, tileRows, planesPerChunk, st, nullptr);
         if (e != hipSuccess) return e;
-        return launch_cg_rz(s, L, b, xout, st, rzPartials);
+        if (!L.B.threeD) return launch_cg_rz(s, L, b, xout, st, rzPartials);
+        // 3D: the row partials of the whole-row sweep and their two-stage sum (the same <r, z> bit for bit)
+        const int64_t nrows = (int64_t)(L.B.ns - 2) * (L.B.nm - 2);
+        const int G = (int)((nrows + kBlockThreads / 64 - 1) / (kBlockThreads / 64));
+        k_rz_rows<<<G, kBlockThreads, 0, s>>>(L, b, xout, st, rzPartials);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        return launch_rz_rows_finalize(s, L.B, st, rzPartials);
     }
     const Box &B = L.B;
     if (rbgs_family(L, xin, xout, b) == RBGS_ROW)

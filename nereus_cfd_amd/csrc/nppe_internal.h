@@ -66,6 +66,10 @@ struct Tables64
 };
 
 // Device-resident scalars of one solve. Written by the last block of each reduction.
+// SolveState::done when a reduction inside the Krylov loop came out non-finite (NaN or infinity): the loop stops
+// there, and the host reports NPPE_ERR_NONFINITE instead of running on to the iteration cap.
+constexpr int kDoneNonFinite = 3;
+
 struct SolveState
 {
     double rz;        // <r, z> of the current search direction
@@ -74,7 +78,7 @@ struct SolveState
     double rr;        // <r, r> after the last update
     double thr2;      // inner stop: rr < thr2
     double sums[4];   // FP64 reductions: [0] ||b||^2, [1] ||r||^2, [2] sum x, [3] spare
-    int done;         // inner loop finished (converged or capped)
+    int done;         // inner loop finished: converged (1), capped (2) or a non-finite reduction (kDoneNonFinite)
     int iters;        // inner iterations taken
     int maxIters;     // inner cap
     int first;        // next direction update is the first of the solve (beta = 0)

@@ -32,7 +32,8 @@ namespace
 {
 
 std::mutex g_create_mutex;
-std::string g_create_error;
+// the message of the calling thread's last failed nppe_create (creates on other threads never overwrite it)
+thread_local std::string g_create_error;
 
 constexpr int kInnerChunk = 4;      // inner iterations enqueued before the first host check of a solve
 constexpr int kMaxChunk = 64;       // most inner iterations enqueued between two host checks
@@ -1013,6 +1014,8 @@ int run_inner(nppe_solver *h, double needRed, double rr0, Traffic &tr, int *ran)
         const int rc = sync_state(h);
         if (rc) return rc;
         const SolveState &st = *h->hState;
+        if (st.done == kDoneNonFinite)
+            return fail(h, NPPE_ERR_NONFINITE, "non-finite value in the Krylov recursion (inconsistent system?)");
         if (st.iters > 0 && st.rr > 0.0 && st.rr < rr0) h->rate = std::pow(st.rr / rr0, 0.5 / st.iters);
         if (st.done) break;
         chunk = st.rr > 0.0 && st.thr2 > 0.0 ? predict(std::sqrt(st.thr2 / st.rr)) : kInnerChunk;
@@ -1177,6 +1180,8 @@ int krylov64_scheduled(nppe_solver *h, double refNorm, int cur, double *dX, bool
                 break;
             }
         }
+        if (st.done == kDoneNonFinite)
+            return fail(h, NPPE_ERR_NONFINITE, "non-finite value in the Krylov recursion (inconsistent system?)");
         if (st.iters > 0 && st.rr > 0.0 && st.rr < rr0) h->rate = std::pow(st.rr / rr0, 0.5 / st.iters);
         if (!st.done)
         {
@@ -1479,6 +1484,8 @@ int solve_cg64(nppe_solver *h, const double *dB, double *dX, const double *dGues
             if (rc) return rc;
             if (h->hState->done) break;
         }
+        if (h->hState->done == kDoneNonFinite)
+            return fail(h, NPPE_ERR_NONFINITE, "non-finite value in the CG recursion (inconsistent system?)");
         iters = h->hState->iters;
         // passes that ran: iters + 1 when the last one converged (Eigen does not count it), spmv+update only
         const int ran = iters + (h->hState->done == 1 ? 1 : 0);
@@ -1512,6 +1519,7 @@ int solve_cg64(nppe_solver *h, const double *dB, double *dX, const double *dGues
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     const double rnorm = std::sqrt(h->hState->sums[1]);
+    if (!std::isfinite(rnorm)) return fail(h, NPPE_ERR_NONFINITE, "non-finite residual of the result");
     if (res)
     {
         res->iterations = iters;
@@ -1726,6 +1734,81 @@ int nppe_system_info(const nppe_solver *h, int64_t *n, int32_t *singular)
     if (!h->committed) return NPPE_ERR_NO_GRID;
     if (n) *n = h->n;
     if (singular) *singular = h->singular ? 1 : 0;
+    return NPPE_OK;
+}
+
+int nppe_describe_system(nppe_solver *h, nppe_system_report *out)
+{
+    if (!h || !out) return NPPE_ERR_ARGUMENT;
+    if (!h->committed) return NPPE_ERR_NO_GRID;
+    if (int rc = ensure_host_layout(h)) return rc;
+    const HostLevel &F = h->levels[0].host;
+    const int ns = F.ns, nm = F.nm, nf = F.nf;
+    auto at = [&](int s, int m, int f) { return ((int64_t)s * nm + m) * nf + f; };
+    // unknown index of every cell (-1: none), union-find over the FLUID faces between unknowns
+    std::vector<int32_t> id((size_t)ns * nm * nf, -1);
+    const int64_t n = (int64_t)h->orderHost.size();
+    for (int64_t q = 0; q < n; q++) id[(size_t)h->orderHost[q]] = (int32_t)q;
+    std::vector<int32_t> parent((size_t)n);
+    for (int64_t q = 0; q < n; q++) parent[(size_t)q] = (int32_t)q;
+    auto root = [&](int32_t a) {
+        while (parent[(size_t)a] != a)
+        {
+            parent[(size_t)a] = parent[(size_t)parent[(size_t)a]];
+            a = parent[(size_t)a];
+        }
+        return a;
+    };
+    std::vector<uint8_t> dirichlet((size_t)n, 0);
+    int64_t isolated = 0;
+    const int ds[6] = {1, -1, 0, 0, 0, 0}, dm[6] = {0, 0, 1, -1, 0, 0}, df[6] = {0, 0, 0, 0, 1, -1};
+    for (int64_t q = 0; q < n; q++)
+    {
+        const int64_t c = h->orderHost[(size_t)q];
+        const int f = (int)(c % nf), m = (int)((c / nf) % nm), s = (int)(c / ((int64_t)nf * nm));
+        bool any = false;
+        for (int k = F.threeD ? 0 : 2; k < 6; k++)
+        {
+            const int S = s + ds[k], M = m + dm[k], Fq = f + df[k];
+            const uint8_t t = F.types[(size_t)at(S, M, Fq)];
+            if (t == 0)
+            {
+                any = true;   // a FLUID neighbour: a coupled unknown, or a ring cell (Dirichlet, as assembled)
+                const int32_t j = id[(size_t)at(S, M, Fq)];
+                if (j < 0) dirichlet[(size_t)q] = 1;
+                else if (k % 2 == 0)
+                {
+                    const int32_t a = root((int32_t)q), b = root(j);
+                    if (a != b) parent[(size_t)std::max(a, b)] = std::min(a, b);
+                }
+            }
+            else if (t == 2 || t == 4)
+            {
+                any = true;
+                dirichlet[(size_t)q] = 1;
+            }
+        }
+        if (!any) isolated++;
+    }
+    std::vector<uint8_t> compDirichlet((size_t)n, 0);
+    int64_t comps = 0, floating = 0;
+    for (int64_t q = 0; q < n; q++)
+        if (dirichlet[(size_t)q]) compDirichlet[(size_t)root((int32_t)q)] = 1;
+    for (int64_t q = 0; q < n; q++)
+        if (root((int32_t)q) == q)
+        {
+            comps++;
+            if (!compDirichlet[(size_t)q]) floating++;
+        }
+    *out = nppe_system_report{};
+    out->unknowns = h->n;
+    out->singular = h->singular ? 1 : 0;
+    out->levels = (int32_t)h->levels.size();
+    out->components = comps;
+    out->floating_components = floating;
+    out->isolated_unknowns = isolated;
+    out->coarsest_unknowns = h->levels.back().host.unknowns;
+    out->device_bytes = h->bytesHeld;
     return NPPE_OK;
 }
 

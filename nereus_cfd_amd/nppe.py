@@ -31,7 +31,7 @@ KERNEL_KINDS = [
 # Every symbol the headers in include/ declare.
 EXPORTS = [
     "nppe_default_options", "nppe_create", "nppe_destroy", "nppe_last_error", "nppe_commit_grid",
-    "nppe_system_info", "nppe_unknown_order", "nppe_check_csr", "nppe_solve", "nppe_solve_device", "nppe_apply",
+    "nppe_system_info", "nppe_describe_system", "nppe_unknown_order", "nppe_check_csr", "nppe_solve", "nppe_solve_device", "nppe_apply",
     "nppe_memory_bytes", "nppe_profile_enable", "nppe_profile_select", "nppe_profile_read", "nppe_synth_cosine_rhs",
     "nppe_set_tolerance", "nppe_estimate_bytes", "nppe_estimate_bytes_for", "nppe_batch_solve", "nppe_server_create", "nppe_server_destroy",
     "nppe_server_add_grid", "nppe_server_submit", "nppe_server_wait", "nppe_server_last_error",
@@ -76,6 +76,17 @@ class Result(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class SystemReport(C.Structure):
+    _fields_ = [
+        ("unknowns", C.c_int64), ("singular", C.c_int32), ("levels", C.c_int32), ("components", C.c_int64),
+        ("floating_components", C.c_int64), ("isolated_unknowns", C.c_int64), ("coarsest_unknowns", C.c_int64),
+        ("device_bytes", C.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -101,6 +112,8 @@ def lib():
     L.nppe_commit_grid.restype = I64
     L.nppe_system_info.argtypes = [P, P, P]
     L.nppe_system_info.restype = C.c_int
+    L.nppe_describe_system.argtypes = [P, P]
+    L.nppe_describe_system.restype = C.c_int
     L.nppe_unknown_order.argtypes = [P, P]
     L.nppe_unknown_order.restype = C.c_int
     L.nppe_check_csr.argtypes = [P, I32, I64, P, P, P]
@@ -294,6 +307,12 @@ class Solver:
         self.n, self.singular = int(n64.value), bool(sing.value)
         self.grid = Grid(self.grid.dim, t.copy(), self.grid.wx, self.grid.wy, self.grid.wz, self.grid.dh,
                          self.grid.non_uniform)
+
+    def describe(self) -> dict:
+        """nppe_describe_system: components, floating (Dirichlet-free) components, isolated unknowns."""
+        r = SystemReport()
+        self._check(lib().nppe_describe_system(self.h, C.byref(r)), "describe_system")
+        return r.as_dict()
 
     def order(self) -> np.ndarray:
         o = np.zeros(max(1, self.n), np.int32)

@@ -65,8 +65,9 @@ struct Writer
 
 struct SolveSpec
 {
-    int solver, precond, maxIters, repeats;
+    int solver, precond, maxIters, repeats, hasGuess;
     double tol, refNorm;
+    Eigen::VectorXd guess;
 };
 
 MAC GRID;
@@ -208,9 +209,17 @@ int main(int argc, char **argv)
         s.maxIters = r.get<int32_t>();
         s.refNorm = r.get<double>();
         s.repeats = std::max(1, (int)r.get<int32_t>());
+        s.hasGuess = r.get<int32_t>();
     }
     Eigen::VectorXd b(n);
     if (nsolves > 0) r.get(b.data(), (size_t)n);
+    // the guesses of the solves that have one, in order
+    for (SolveSpec &s : specs)
+        if (s.hasGuess)
+        {
+            s.guess.resize(n);
+            r.get(s.guess.data(), (size_t)n);
+        }
     // optional fields section
     const int hasFields = r.get<int32_t>();
     double rho = 1.0, dt = 1e-3;
@@ -256,7 +265,7 @@ int main(int argc, char **argv)
         for (int rep = 0; rep < s.repeats; rep++)
         {
             const double t0 = omp_get_wtime();
-            iters = SolvePressureSystem(*sys, b, x, nullptr, s.refNorm);
+            iters = SolvePressureSystem(*sys, b, x, s.hasGuess ? &s.guess : nullptr, s.refNorm);
             times.push_back(omp_get_wtime() - t0);
         }
         w.put<int32_t>(iters);

@@ -34,6 +34,7 @@ class SolveSpec:
     max_iters: int = 2000
     ref_norm: float = 0.0
     repeats: int = 1
+    guess: Optional[np.ndarray] = None   # SolvePressureSystem's guess (n values), or none
 
 
 @dataclass
@@ -70,9 +71,12 @@ class RefCase:
         out.append(struct.pack("<i", len(self.solves)))
         for s in self.solves:
             out.append(struct.pack("<iidid", s.solver, s.precond, s.tol, s.max_iters, s.ref_norm))
-            out.append(struct.pack("<i", s.repeats))
+            out.append(struct.pack("<ii", s.repeats, 0 if s.guess is None else 1))
         if self.solves:
             out.append(np.ascontiguousarray(self.rhs, np.float64).tobytes())
+        for s in self.solves:
+            if s.guess is not None:
+                out.append(np.ascontiguousarray(s.guess, np.float64).tobytes())
         f = self.fields
         out.append(struct.pack("<i", 1 if f else 0))
         if f:

@@ -14,7 +14,11 @@ fields after ProjectPressure; then p after MAC::EnforcePressureBC with a known b
 SolveScheme's tail on a given vector standing for the solution, and (incremental) after
 AccumulatePressureIncrement.
 
-    python tests/golden/make_fixtures.py
+The ``floor_*.npz`` fixtures hold solves with a reference norm (the absolute floor of an incremental solve,
+SolveEigenPressure) on some of those systems: their iteration counts and solutions.
+
+    python tests/golden/make_fixtures.py            # everything
+    python tests/golden/make_fixtures.py --floor    # the floor fixtures only
 """
 from __future__ import annotations
 
@@ -169,7 +173,62 @@ def main_fields():
         print(f"fields_{name}: n={out['n']} incremental={f['p_old'] is not None} ref_norm={out['ref_norm']:.6g}")
 
 
+# SolveEigenPressure's absolute floor (PressureSettings.h:411-427) on uniform, stretched, singular and 2D systems:
+# the early return (||b|| <= tol * ref: the guess, or 0, with 0 iterations), the min(0.5, tol * ref / ||b||) branch
+# and its 0.5 cap, from a zero start and from a guess; eigen_solver = CG with the DIAGONAL preconditioner.
+FLOOR_CASES = ["u3_outlet_9", "s3_graded_outlet", "u3_neumann_8", "u2_pressure_12", "u3_mixed_10"]
+
+
+def floor_specs(b, guess, tol):
+    bn = float(np.linalg.norm(b))
+    return [
+        ("early_zero", R.SolveSpec(R.CG, R.DIAGONAL, tol, 5000, ref_norm=2.0 * bn / tol)),
+        ("early_guess", R.SolveSpec(R.CG, R.DIAGONAL, tol, 5000, ref_norm=1.25 * bn / tol, guess=guess)),
+        ("min_zero", R.SolveSpec(R.CG, R.DIAGONAL, tol, 5000, ref_norm=1e4 * bn)),
+        ("min_guess", R.SolveSpec(R.CG, R.DIAGONAL, tol, 5000, ref_norm=3e3 * bn, guess=guess)),
+        ("cap_zero", R.SolveSpec(R.CG, R.DIAGONAL, tol, 5000, ref_norm=0.999 * bn / tol)),
+        ("cap_guess", R.SolveSpec(R.CG, R.DIAGONAL, tol, 5000, ref_norm=0.99 * bn / tol, guess=guess)),
+        ("plain_guess", R.SolveSpec(R.CG, R.DIAGONAL, tol, 5000, ref_norm=0.0, guess=guess)),
+    ]
+
+
+def main_floor():
+    tol = 1e-10
+    for q, name in enumerate(FLOOR_CASES):
+        (_, dim, gs, dom, nu, refs, kind, compat) = next(c for c in CASES if c[0] == name)
+        base = np.load(os.path.join(HERE, name + ".npz"))
+        # a smooth right-hand side (low modes plus a little noise), so that one CG step leaves more than half of
+        # it and the 0.5 cap decides when the floor's tolerance stops
+        order = base["order"].astype(np.int64)
+        Ny, Nx = base["types"].shape[:2]
+        Nz = base["types"].shape[2]
+        i, j, k = order // (Nx * Nz), (order // Nz) % Nx, order % Nz
+        rng = np.random.default_rng(3000 + q)
+        b = (np.cos(np.pi * (j + 0.5) / Nx) * np.cos(np.pi * (i + 0.5) / Ny) * (1.0 + 0.5 * np.cos(np.pi * (k + 0.5) / Nz))
+             + 0.05 * rng.standard_normal(order.size))
+        if compat:
+            b = O.compatibility(b)
+        grid = O.Grid(dim, base["types"], base["wx"], base["wy"],
+                      base["wz"] if dim == 3 else np.array([float(base["dh"])]), float(base["dh"]), bool(nu))
+        xs, _, _ = O.assemble(grid).cg(b, 1e-13, 100000)
+        guess = xs + 0.2 * rng.standard_normal(b.size) * float(np.abs(xs).max())
+        specs = floor_specs(b, guess, tol)
+        out = R.run(R.RefCase(dim=dim, grid_size=gs, domain=dom, non_uniform=nu, refine=refs, types=base["types"],
+                              compat=compat, solves=[s for _, s in specs], rhs=b), threads=1)
+        arrays = {"case": name, "tol": tol, "rhs": b, "guess": guess, "names": np.array([k for k, _ in specs]),
+                  "ref_norms": np.array([s.ref_norm for _, s in specs]),
+                  "has_guess": np.array([s.guess is not None for _, s in specs], np.int32),
+                  "iters": np.array([o["iters"] for o in out["solves"]], np.int32),
+                  "x": np.stack([o["x"] for o in out["solves"]])}
+        np.savez_compressed(os.path.join(HERE, "floor_" + name + ".npz"), **arrays)
+        print(f"floor_{name}: n={out['n']} iters={arrays['iters'].tolist()}")
+
+
 if __name__ == "__main__":
+    if "--floor" in sys.argv:
+        main_floor()
+        raise SystemExit(0)
     if "--fields" not in sys.argv:
         main()
     main_fields()
+    main_floor()

@@ -14,7 +14,7 @@ SHAPES = [("6", "1"), ("6", "5"), ("14", "3"), ("14", "16"), ("4", "2"), ("8", "
 
 
 def _solve(g, b, env):
-    saved = {k: os.environ.get(k) for k in ("NPPE_RBGS_TM", "NPPE_RBGS_P", "NPPE_AUTOTUNE")}
+    saved = {k: os.environ.get(k) for k in ("NPPE_RBGS_TM", "NPPE_RBGS_P", "NPPE_AUTOTUNE", "NPPE_RBGS_MARCH")}
     try:
         for k in saved:
             os.environ.pop(k, None)
@@ -33,10 +33,12 @@ def _solve(g, b, env):
 
 
 @pytest.mark.parametrize("dim,n,kind,non_uniform", [
-    (3, 40, "mixed", False),        # whole-row sweep (uniform), Dirichlet codes, solids
-    (3, 36, "outlet", True),        # marching sweep (stretched)
-    (3, 72, "neumann", False),      # large enough for the timed autotuning
-    (2, 96, "mixed", False),        # 2D sweep
+    (3, 40, "mixed", False),             # whole-row sweep (uniform, one pair per lane), Dirichlet codes, solids
+    (3, 36, "outlet", True),             # whole-row sweep on a stretched grid (width tables in registers)
+    (3, 72, "neumann", False),           # large enough for the timed autotuning
+    (3, (24, 20, 140), "pressure", False),   # rows of 140 cells: two pairs per lane
+    (3, (20, 16, 300), "pressure", False),   # rows of 300 cells: four pairs per lane
+    (2, 96, "mixed", False),             # 2D sweep
 ])
 def test_tile_shapes_bit_identical(dim, n, kind, non_uniform):
     g, _ = make(dim, n, kind, non_uniform, seed=3)
@@ -49,4 +51,20 @@ def test_tile_shapes_bit_identical(dim, n, kind, non_uniform):
         assert res.iterations == rres.iterations
         assert np.array_equal(x, ref), (tm, p)
     x, _ = _solve(g, b, {"NPPE_AUTOTUNE": "0"})
+    assert np.array_equal(x, ref)
+
+
+@pytest.mark.parametrize("n,kind,non_uniform", [
+    (36, "outlet", True),                # stretched: the row sweep's register tables against the march's
+    ((24, 20, 140), "mixed", False),     # two pairs per lane, Dirichlet codes
+    ((20, 16, 300), "pressure", False),  # four pairs per lane
+])
+def test_march_equals_row_sweep(n, kind, non_uniform):
+    """The marching sweep (tiles of 128 cells, forced by NPPE_RBGS_MARCH) and the whole-row sweep compute every
+    update from the same values in the same order, so whole solves agree bit for bit."""
+    g, _ = make(3, n, kind, non_uniform, seed=4)
+    b = np.random.default_rng(6).standard_normal(g.unknowns)
+    ref, rres = _solve(g, b, {})
+    x, res = _solve(g, b, {"NPPE_RBGS_MARCH": "1"})
+    assert res.iterations == rres.iterations
     assert np.array_equal(x, ref)
