@@ -264,6 +264,14 @@ int nppe_enforce_pressure_bc_device(nppe_solver *h, double *d_p, const double *d
  * = p_old at the unknowns, the reference norm of an incremental solve. */
 int nppe_increment_reference_norm_device(nppe_solver *h, const double *d_b, const double *d_p_old, double *norm);
 
+/* nppe_increment_reference_norm_device without any allocation: no counterpart in the reference. For callers that
+ * cannot allocate device memory (code under stream capture, tight serving loops, harnesses that forbid it). One fused
+ * pass computes A p^n from the MAC array directly (no box copy of p^n) and reduces into a fixed device array of the
+ * library (512 KiB per device) instead of the handle's buffers: grids of more than 262,144 interior rows (3D:
+ * (ny-2)*(nx-2); 2D: ny-2) get NPPE_ERR_SIZE. The norm is bit-identical to nppe_increment_reference_norm_device's.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU race (one call at a time per GPU). */
+int nppe_increment_reference_norm_noalloc(nppe_solver *h, const double *d_b, const double *d_p_old, double *norm);
+
 /* SolveScheme's tail after the solve (PressureSolver3D.cpp:316-336; PressureSolver2D.cpp:287-304): with d_p_old
  * NULL (Chorin) x's mean is removed when the system is singular and the compatibility option is on
  * (LevelPressure; the level a scene's inflow p= expressions set, :188-211, needs the scene and is the caller's);

@@ -77,6 +77,7 @@ struct nppe_solver
     bool coarseDense = false;
     nppe::SolveState *dState = nullptr;
     nppe::SolveState *hState = nullptr;   // pinned
+    double *hScal = nullptr;              // pinned: scalars read back by the allocation-free entry points
     double *dPartials = nullptr;
     int64_t partialsCap = 0;
     uint64_t bytesHeld = 0;
@@ -187,6 +188,9 @@ hipError_t dalloc_pairs(nppe_solver *h, T **p, size_t count)
 
 // Slot of the handle's step scalars (dStepScal) that holds nppe_update_layout_device's count of changed cells.
 constexpr int kStepScalDiff = 5;
+
+// The allocation-free entry points' scratch array on `device` (current), looked up once per device (noalloc.hip).
+int noalloc_scratch(int device, double **out);
 
 // Field-kernel data of a committed grid (step.cpp); called at the end of the commit. refresh_fields recomputes the
 // parts that depend on the layout (after nppe_update_layout).

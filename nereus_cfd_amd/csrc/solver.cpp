@@ -1627,14 +1627,28 @@ int nppe_create(nppe_solver **out, const nppe_options *opt)
     std::unique_lock<std::mutex> devLock(gCaptureMu);
     if ((e = hipSetDevice(h->device)) != hipSuccess ||
         (e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess ||
-        (e = hipHostMalloc((void **)&h->hState, sizeof(SolveState), hipHostMallocDefault)) != hipSuccess)
+        (e = hipHostMalloc((void **)&h->hState, sizeof(SolveState), hipHostMallocDefault)) != hipSuccess ||
+        (e = hipHostMalloc((void **)&h->hScal, 8 * sizeof(double), hipHostMallocDefault)) != hipSuccess)
     {
         g_create_error = std::string("device set-up failed: ") + hipGetErrorString(e);
+        if (h->hState) (void)hipHostFree(h->hState);
         if (h->stream) (void)hipStreamDestroy(h->stream);
         delete h;
         return NPPE_ERR_DEVICE;
     }
     std::memset(h->hState, 0, sizeof(SolveState));
+    std::memset(h->hScal, 0, 8 * sizeof(double));
+    // the allocation-free entry points' scratch address on this device, looked up here at set-up
+    double *scratchPtr = nullptr;
+    if (noalloc_scratch(h->device, &scratchPtr) != NPPE_OK)
+    {
+        g_create_error = "device set-up failed: scratch symbol address";
+        (void)hipHostFree(h->hScal);
+        (void)hipHostFree(h->hState);
+        (void)hipStreamDestroy(h->stream);
+        delete h;
+        return NPPE_ERR_DEVICE;
+    }
     *out = h;
     return NPPE_OK;
 }
@@ -1649,6 +1663,7 @@ int nppe_destroy(nppe_solver *h)
     {
         std::lock_guard<std::mutex> lock(gCaptureMu);
         if (h->hState) (void)hipHostFree(h->hState);
+        if (h->hScal) (void)hipHostFree(h->hScal);
         if (h->stream) (void)hipStreamDestroy(h->stream);
     }
     delete h;

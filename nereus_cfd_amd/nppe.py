@@ -39,6 +39,7 @@ EXPORTS = [
     "nppe_enforce_pressure_bc_device", "nppe_increment_reference_norm_device", "nppe_scatter_pressure_device",
     "nppe_accumulate_increment_device", "nppe_solve_laplacian_device", "nppe_solve_laplacian",
     "nppe_pressure_step_device", "nppe_projection_step_device", "nppe_update_layout", "nppe_update_layout_device",
+    "nppe_increment_reference_norm_noalloc",
 ]
 
 
@@ -160,6 +161,7 @@ def lib():
                        ("nppe_project_device", [P, P, P, P, P, D, D]), ("nppe_project", [P, P, P, P, P, D, D]),
                        ("nppe_enforce_pressure_bc_device", [P, P, P]),
                        ("nppe_increment_reference_norm_device", [P, P, P, P]),
+                       ("nppe_increment_reference_norm_noalloc", [P, P, P, P]),
                        ("nppe_scatter_pressure_device", [P, P, P, P, P]),
                        ("nppe_accumulate_increment_device", [P, P, P, P]),
                        ("nppe_solve_laplacian_device", [P, P, P, P]), ("nppe_solve_laplacian", [P, P, P, P]),
@@ -391,6 +393,13 @@ class Solver:
         out = C.c_double()
         self._check(lib().nppe_increment_reference_norm_device(self.h, self._d(b), self._d(p_old), C.byref(out)),
                     "increment_reference_norm_device")
+        return out.value
+
+    def increment_reference_norm_noalloc(self, b, p_old) -> float:
+        """nppe_increment_reference_norm_noalloc: the same norm, no allocation (the library's fixed scratch)."""
+        out = C.c_double()
+        self._check(lib().nppe_increment_reference_norm_noalloc(self.h, self._d(b), self._d(p_old), C.byref(out)),
+                    "increment_reference_norm_noalloc")
         return out.value
 
     def scatter_pressure_device(self, x, p, p_old=0, prescribed=0):
