@@ -302,6 +302,56 @@ int nppe_pressure_step_device(nppe_solver *h, const double *d_u, const double *d
 int nppe_projection_step_device(nppe_solver *h, double *d_u, double *d_v, double *d_w, double *d_p,
                                 const double *d_p_old, double rho, double dt, const double *d_prescribed);
 
+/* ---- warm start and the inflow pressure level ------------------------------------------------------------------
+ * The two parts of SolveScheme that need the previous step or the scene: BuildWarmStartGuess
+ * (PressureSolver3D.cpp:215-244; PressureSolver2D.cpp:182-212) and LevelPressure's inflow shift (:15-30, :181-211;
+ * 2D :13-29, :156-178). The scene's inlet levels are expressions the caller evaluates: it passes, per MAC cell, the
+ * value SCENE.PressureLevelAt gives at each INFLOW ring cell's centre (NaN where it gives none; other cells are not
+ * read) and the MAC::CentrePos tables the extrapolation uses. */
+typedef struct nppe_inflow_level
+{
+    const double *d_level;   /* device, nx*ny*nz values (MAC cell index): the inlet's level at INFLOW ring cells */
+    const double *centre_x;  /* host, nx values: MAC::CentrePos(X, n) for n = 0 .. nx-1 (ring cells included) */
+    const double *centre_y;  /* host, ny values */
+    const double *centre_z;  /* host, nz values (3D only) */
+} nppe_inflow_level;
+
+/* BuildWarmStartGuess once the caller's part of WarmStartEnabled holds (PRESSURE_SETTINGS.warmStart and a method
+ * whose fluid region is fixed, MethodFixedFluidRegion): d_guess (n entries, IDP order) = the previous pressure
+ * d_p_prev (MAC array) at the unknowns, its mean removed when the system is singular and the compatibility option is
+ * on. *built = 0 (no guess; d_guess undefined) on a stretched grid or when d_p_prev is zero at every unknown. */
+int nppe_warm_start_guess_device(nppe_solver *h, const double *d_p_prev, double *d_guess, int32_t *built);
+
+/* nppe_scatter_pressure_device with LevelPressure's inflow shift after the mean removal (level may be NULL: no scene). */
+int nppe_scatter_pressure_level_device(nppe_solver *h, const double *d_x, double *d_p, const double *d_p_old,
+                                       const double *d_prescribed, const nppe_inflow_level *level);
+/* nppe_accumulate_increment_device with LevelPressure's inflow shift (AccumulatePressureIncrement levels p^n + phi,
+ * PressureSolver3D.cpp:374-393). */
+int nppe_accumulate_increment_level_device(nppe_solver *h, const double *d_p_old, double *d_p,
+                                           const double *d_prescribed, const nppe_inflow_level *level);
+
+typedef struct nppe_step_options
+{
+    const double *d_p_prev;               /* gridAnt's p (MAC array, device): the warm start's source; NULL: none */
+    int32_t warm_start;                   /* the caller's part of WarmStartEnabled (see above) */
+    int32_t reserved0;
+    const double *d_prescribed;           /* as nppe_scatter_pressure_device (NULL: PRESSURE ring cells keep p's) */
+    const nppe_inflow_level *inflow_level;/* NULL: no scene (LevelPressure removes the mean only) */
+    int32_t *warm_started;                /* out, may be NULL: 1 when the solve started from the warm-start guess */
+} nppe_step_options;
+
+/* nppe_pressure_step_device with the rest of SolveScheme (PressureSolver3D.cpp:279-336): under Chorin (d_p_old NULL)
+ * the solve starts from BuildWarmStartGuess's guess when o->warm_start is set and a guess is built (:309-314; the
+ * AMGX-only closed-box gate does not apply to this backend), and LevelPressure applies the scene's inflow level;
+ * o may be NULL (then it is nppe_pressure_step_device). */
+int nppe_pressure_step_ex_device(nppe_solver *h, const double *d_u, const double *d_v, const double *d_w, double *d_p,
+                                 const double *d_p_old, double rho, double dt, const nppe_step_options *o,
+                                 nppe_result *res);
+/* nppe_projection_step_device whose accumulation applies the scene's inflow level (level may be NULL). */
+int nppe_projection_step_ex_device(nppe_solver *h, double *d_u, double *d_v, double *d_w, double *d_p,
+                                   const double *d_p_old, double rho, double dt, const double *d_prescribed,
+                                   const nppe_inflow_level *level);
+
 /* ---- many systems at once (no counterpart in the reference, which solves one system per process) ------------- */
 
 /* Solves count independent committed systems concurrently: each handle runs on its own stream, driven by

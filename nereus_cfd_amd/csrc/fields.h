@@ -60,6 +60,17 @@ hipError_t launch_norm2_sum(hipStream_t s, const FieldGeom &G, const double *b, 
 // b[id] = f[id] * V(cell) (V = 1.0 on a uniform grid).
 hipError_t launch_scale_volume(hipStream_t s, const FieldGeom &G, const double *f, double *b);
 
+// LevelPressure's inflow shift on p (MAC array, mean already removed at the unknowns): partials[block] = the sum
+// of the INFLOW ring cells' terms, partials[blocks + block] their count, over inflow_level_blocks(G) blocks; level
+// is the scene's value per MAC cell (NaN: none), centres MAC::CentrePos of the x, y, z axes back to back.
+int inflow_level_blocks(const FieldGeom &G);
+hipError_t launch_inflow_level(hipStream_t s, const FieldGeom &G, const double *p, const double *level,
+                               const double *centres, double *partials);
+// p[c] += sumCount[0] / sumCount[1] at the unknowns when sumCount[1] > 0.
+hipError_t launch_add_shift(hipStream_t s, const FieldGeom &G, const double *sumCount, double *p);
+// x[id] = p[c] at the unknowns; partials[block] = the block's sum of p[c]^2 (field_row_blocks(G) blocks).
+hipError_t launch_gather_idp(hipStream_t s, const FieldGeom &G, const double *p, double *x, double *partials);
+
 // partials[block] = the number of cells whose type in newTypes (MAC order, unpitched) differs from the committed
 // one, partials[blocks + block] the number of cells whose code is out of range (> 5); *blocks = the blocks.
 hipError_t launch_diff_types(hipStream_t s, const FieldGeom &G, const uint8_t *newTypes, double *partials, int *blocks);

@@ -441,6 +441,125 @@ __global__ __launch_bounds__(kThreads) void k_scale_volume(FieldGeom G, const do
     });
 }
 
+// LevelPressure's inflow shift (PressureSolver3D.cpp:15-30, :181-211; PressureSolver2D.cpp:13-29, :156-178) on
+// p at the unknowns, after its mean was removed there: every ring cell of the x, y (3D: and z) faces whose cell is
+// INFLOW, whose two inward neighbours are FLUID unknowns and where the scene gives a level (level[c] not NaN) adds
+// level - (p1 + (p1 - p2) (sR - s1) / (s1 - s2)), the reference's expression with its operation order (contraction
+// off); centres: MAC::CentrePos of the x, y, z axes back to back. Block partials: the sums, then the counts.
+constexpr uint8_t kInflow = 3;
+constexpr int kLevelBlocks = 256;
+
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void k_inflow_level(FieldGeom G, const double *__restrict__ p,
+                                                          const double *__restrict__ level,
+                                                          const double *__restrict__ centres,
+                                                          double *__restrict__ partials)
+{
+#pragma clang fp contract(off)
+    const int Nx = G.Nx, Ny = G.Ny, Nz = G.Nz;
+    const double *cx = centres, *cy = centres + Nx, *cz = centres + Nx + Ny;
+    // ring-face items: x faces (i, k), y faces (j, k), z faces (i, j), two sides each (2D: x faces i, y faces j)
+    const int64_t segX = DIM == 3 ? (int64_t)(Ny - 2) * (Nz - 2) : (int64_t)(Ny - 2);
+    const int64_t segY = DIM == 3 ? (int64_t)(Nx - 2) * (Nz - 2) : (int64_t)(Nx - 2);
+    const int64_t segZ = DIM == 3 ? (int64_t)(Ny - 2) * (Nx - 2) : 0;
+    const int64_t total = 2 * (segX + segY + segZ);
+    double sum = 0.0, cnt = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < total; t += (int64_t)gridDim.x * kThreads)
+    {
+        int64_t u = t >> 1;
+        const bool hi = (t & 1) != 0;
+        int ri, rj, rk, di = 0, dj = 0, dk = 0, N;
+        const double *cen;
+        if (u < segX)
+        {
+            ri = 1 + (int)(DIM == 3 ? u / (Nz - 2) : u);
+            rk = DIM == 3 ? 1 + (int)(u % (Nz - 2)) : 0;
+            rj = hi ? Nx - 1 : 0;
+            dj = hi ? -1 : 1;
+            N = Nx;
+            cen = cx;
+        }
+        else if ((u -= segX) < segY)
+        {
+            rj = 1 + (int)(DIM == 3 ? u / (Nz - 2) : u);
+            rk = DIM == 3 ? 1 + (int)(u % (Nz - 2)) : 0;
+            ri = hi ? Ny - 1 : 0;
+            di = hi ? -1 : 1;
+            N = Ny;
+            cen = cy;
+        }
+        else
+        {
+            u -= segY;
+            ri = 1 + (int)(u / (Nx - 2));
+            rj = 1 + (int)(u % (Nx - 2));
+            rk = hi ? Nz - 1 : 0;
+            dk = hi ? -1 : 1;
+            N = Nz;
+            cen = cz;
+        }
+        if (N < 4) continue;   // the second inward cell is the far ring cell: no unknown
+        const int n0 = di ? ri : (dj ? rj : rk);
+        auto typ = [&](int i, int j, int k) { return G.types[G.B.at(DIM == 3 ? i : 0, DIM == 3 ? j : i, DIM == 3 ? k : j)]; };
+        auto mac = [&](int i, int j, int k) { return ((int64_t)i * Nx + j) * Nz + k; };
+        if (typ(ri, rj, rk) != kInflow || typ(ri + di, rj + dj, rk + dk) != 0 ||
+            typ(ri + 2 * di, rj + 2 * dj, rk + 2 * dk) != 0)
+            continue;
+        const double lv = level[mac(ri, rj, rk)];
+        if (lv != lv) continue;   // NaN: the scene sets no level here
+        const double p1 = p[mac(ri + di, rj + dj, rk + dk)], p2 = p[mac(ri + 2 * di, rj + 2 * dj, rk + 2 * dk)];
+        const int step = di + dj + dk;
+        const double sR = cen[n0], s1 = cen[n0 + step], s2 = cen[n0 + 2 * step];
+        sum += lv - (p1 + (p1 - p2) * (sR - s1) / (s1 - s2));
+        cnt += 1.0;
+    }
+    const double bs = block_sum(sum);
+    __syncthreads();   // block_sum's LDS is used again
+    const double bc = block_sum(cnt);
+    if (threadIdx.x == 0)
+    {
+        partials[blockIdx.x] = bs;
+        partials[gridDim.x + blockIdx.x] = bc;
+    }
+}
+
+// p[c] += sum / count at the unknowns when count > 0 (LevelPressure's p.array() += sum / n).
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void k_add_shift(FieldGeom G, const double *__restrict__ sumCount,
+                                                       double *__restrict__ p)
+{
+#pragma clang fp contract(off)
+    const double cnt = sumCount[1];
+    if (!(cnt > 0.0)) return;
+    const double shift = sumCount[0] / cnt;
+    const int r = xcd_block() * kRowsPerBlock + (threadIdx.x >> 6);
+    const Row R = interior_row<DIM>(G, r);
+    walk_unknowns<DIM>(G, R, r, [&](int f, int64_t, unsigned) {
+        const int64_t c = DIM == 3 ? ((int64_t)R.i * G.Nx + R.j) * G.Nz + f : (int64_t)R.i * G.Nx + f;
+        p[c] += shift;
+    });
+}
+
+// x[id] = p[c] at the unknowns (IDP order), with the block partials of sum p[c]^2 (BuildWarmStartGuess's gather
+// and its zero test).
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void k_gather_idp(FieldGeom G, const double *__restrict__ p,
+                                                        double *__restrict__ x, double *__restrict__ partials)
+{
+    const int lb = xcd_block();
+    const int r = lb * kRowsPerBlock + (threadIdx.x >> 6);
+    const Row R = interior_row<DIM>(G, r);
+    double acc = 0.0;
+    walk_unknowns<DIM>(G, R, r, [&](int f, int64_t id, unsigned) {
+        const int64_t c = DIM == 3 ? ((int64_t)R.i * G.Nx + R.j) * G.Nz + f : (int64_t)R.i * G.Nx + f;
+        const double v = p[c];
+        x[id] = v;
+        acc += v * v;
+    });
+    const double sum = block_sum(acc);
+    if (threadIdx.x == 0) partials[lb] = sum;
+}
+
 // EnsureSystem's layout comparison (PressureSettings.h:300-303): every cell of the box, one wave per (i, j) row.
 template <int DIM>
 __global__ __launch_bounds__(kThreads) void k_diff_types(FieldGeom G, const uint8_t *__restrict__ t, int rows,
@@ -657,6 +776,39 @@ hipError_t launch_sub_mean(hipStream_t s, double *x, int64_t n, const double *su
 hipError_t launch_sum_partials(hipStream_t s, const double *partials, int count, double *dst)
 {
     k_sum_partials<<<1, 1024, 0, s>>>(partials, count, dst);
+    return hipGetLastError();
+}
+
+
+int inflow_level_blocks(const FieldGeom &G)
+{
+    const int64_t segX = G.dim == 3 ? (int64_t)(G.Ny - 2) * (G.Nz - 2) : (int64_t)(G.Ny - 2);
+    const int64_t segY = G.dim == 3 ? (int64_t)(G.Nx - 2) * (G.Nz - 2) : (int64_t)(G.Nx - 2);
+    const int64_t segZ = G.dim == 3 ? (int64_t)(G.Ny - 2) * (G.Nx - 2) : 0;
+    const int64_t total = 2 * (segX + segY + segZ);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(kLevelBlocks, (total + kThreads - 1) / kThreads));
+}
+
+hipError_t launch_inflow_level(hipStream_t s, const FieldGeom &G, const double *p, const double *level,
+                               const double *centres, double *partials)
+{
+    const int nb = inflow_level_blocks(G);
+    if (G.dim == 3) k_inflow_level<3><<<nb, kThreads, 0, s>>>(G, p, level, centres, partials);
+    else k_inflow_level<2><<<nb, kThreads, 0, s>>>(G, p, level, centres, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_shift(hipStream_t s, const FieldGeom &G, const double *sumCount, double *p)
+{
+    if (G.dim == 3) k_add_shift<3><<<field_row_blocks(G), kThreads, 0, s>>>(G, sumCount, p);
+    else k_add_shift<2><<<field_row_blocks(G), kThreads, 0, s>>>(G, sumCount, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_idp(hipStream_t s, const FieldGeom &G, const double *p, double *x, double *partials)
+{
+    if (G.dim == 3) k_gather_idp<3><<<field_row_blocks(G), kThreads, 0, s>>>(G, p, x, partials);
+    else k_gather_idp<2><<<field_row_blocks(G), kThreads, 0, s>>>(G, p, x, partials);
     return hipGetLastError();
 }
 

@@ -99,6 +99,7 @@ struct nppe_solver
     double *dFieldTab = nullptr;
     int32_t *dRowStart = nullptr;
     double *dStepScal = nullptr;
+    double *dCentre = nullptr;   // MAC::CentrePos of the x, y, z axes (LevelPressure's inflow shift), per call
     double *dStage[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
 
     // profiling (nppe_profile.h)

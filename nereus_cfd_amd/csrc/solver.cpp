@@ -92,6 +92,7 @@ void free_grid(nppe_solver *h)
     h->dFieldTab = nullptr;
     h->dRowStart = nullptr;
     h->dStepScal = nullptr;
+    h->dCentre = nullptr;
     for (double *&d : h->dStage) d = nullptr;
     h->fg = FieldGeom{};
     h->committed = false;

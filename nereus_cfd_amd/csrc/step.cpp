@@ -19,8 +19,11 @@ constexpr int kScalBC = 0;        // dStepScal slots: the two compatibility sums
 constexpr int kScalMeanX = 2;     // sum of x (LevelPressure of the Chorin scatter)
 constexpr int kScalMeanP = 3;     // sum of p_old + phi (LevelPressure of the accumulation)
 constexpr int kScalNorm = 4;      // ||b + A p^n||^2
-static_assert(kStepScalDiff == 5, "slot 5: the layout comparison's count (nppe_update_layout_device)");
-constexpr int kScalCount = 8;
+static_assert(kStepScalDiff == 5, "slots 5, 6: the layout comparison's counts (nppe_update_layout_device)");
+constexpr int kScalLevel = 7;     // LevelPressure's inflow sum and count (slots 7, 8)
+constexpr int kScalWarm = 9;      // the warm start's ||p_prev||^2 at the unknowns
+constexpr int kScalWarmMean = 10; // the sum of the warm-start guess (its mean removal)
+constexpr int kScalCount = 12;
 
 bool needs_compatibility(const nppe_solver *h) { return h->singular && h->opt.compatibility && h->n > 0; }
 
@@ -80,11 +83,38 @@ hipError_t rhs(nppe_solver *h, const double *u, const double *v, const double *w
     return e;
 }
 
-hipError_t scatter(nppe_solver *h, const double *x, double *p, const double *pOld, const double *prescribed)
+// LevelPressure's inflow shift (PressureSolver3D.cpp:15-30, :196-211) on p at the unknowns, whose mean is already
+// removed: the ring terms reduced in a fixed order, then p += sum / count when any ring cell contributed.
+hipError_t level_inflow(nppe_solver *h, double *p, const nppe_inflow_level *lv)
+{
+    const size_t Nx = h->Nx, Ny = h->Ny, Nz = h->Nz;
+    hipError_t e = hipMemcpyAsync(h->dCentre, lv->centre_x, Nx * sizeof(double), hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h->dCentre + Nx, lv->centre_y, Ny * sizeof(double), hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess && h->dim == 3)
+        e = hipMemcpyAsync(h->dCentre + Nx + Ny, lv->centre_z, Nz * sizeof(double), hipMemcpyHostToDevice, h->stream);
+    const int nb = inflow_level_blocks(h->fg);
+    if (e == hipSuccess) e = launch_inflow_level(h->stream, h->fg, p, lv->d_level, h->dCentre, h->dPartials);
+    if (e == hipSuccess) e = launch_sum_partials(h->stream, h->dPartials, nb, h->dStepScal + kScalLevel);
+    if (e == hipSuccess) e = launch_sum_partials(h->stream, h->dPartials + nb, nb, h->dStepScal + kScalLevel + 1);
+    if (e == hipSuccess) e = launch_add_shift(h->stream, h->fg, h->dStepScal + kScalLevel, p);
+    return e;
+}
+
+int level_args(nppe_solver *h, const nppe_inflow_level *lv)
+{
+    if (lv && (!lv->d_level || !lv->centre_x || !lv->centre_y || (h->dim == 3 && !lv->centre_z)))
+        return fail(h, NPPE_ERR_ARGUMENT, "inflow level: null level array or centre table");
+    return NPPE_OK;
+}
+
+hipError_t scatter(nppe_solver *h, const double *x, double *p, const double *pOld, const double *prescribed,
+                   const nppe_inflow_level *lv = nullptr)
 {
     const double *sumX = nullptr;
     hipError_t e = hipSuccess;
-    if (!pOld && needs_compatibility(h))
+    const bool level = !pOld && needs_compatibility(h);
+    if (level)
     {
         int nb = 0;
         e = launch_sum_linear(h->stream, x, h->n, h->dPartials, &nb);
@@ -92,12 +122,14 @@ hipError_t scatter(nppe_solver *h, const double *x, double *p, const double *pOl
         sumX = h->dStepScal + kScalMeanX;
     }
     if (e == hipSuccess) e = launch_scatter_p(h->stream, h->fg, x, sumX, p);
+    if (e == hipSuccess && level && lv) e = level_inflow(h, p, lv);
     if (e == hipSuccess) e = launch_pressure_bc(h->stream, h->fg, p, prescribed);
     if (e == hipSuccess && pOld) e = launch_ring_fix(h->stream, h->fg, pOld, p);
     return e;
 }
 
-hipError_t accumulate(nppe_solver *h, const double *pOld, double *p, const double *prescribed)
+hipError_t accumulate(nppe_solver *h, const double *pOld, double *p, const double *prescribed,
+                      const nppe_inflow_level *lv = nullptr)
 {
     const double *sum = nullptr;
     hipError_t e = hipSuccess;
@@ -109,6 +141,7 @@ hipError_t accumulate(nppe_solver *h, const double *pOld, double *p, const doubl
         sum = h->dStepScal + kScalMeanP;
     }
     if (e == hipSuccess) e = launch_accumulate(h->stream, h->fg, pOld, sum, p);
+    if (e == hipSuccess && sum && lv) e = level_inflow(h, p, lv);
     // without prescribed values the PRESSURE ring cells get p_old + phi, p_b(t) to round-off
     if (e == hipSuccess && !prescribed) e = launch_ring_fix(h->stream, h->fg, pOld, p, true);
     if (e == hipSuccess) e = launch_pressure_bc(h->stream, h->fg, p, prescribed);
@@ -131,6 +164,33 @@ int reference_norm(nppe_solver *h, const double *b, const double *pOld, double *
             "reference norm");
     HIP_TRY(h, hipStreamSynchronize(h->stream), "reference norm");
     *norm = std::sqrt(s);
+    return NPPE_OK;
+}
+
+// BuildWarmStartGuess (PressureSolver3D.cpp:222-244; PressureSolver2D.cpp:191-212) once the caller's part of
+// WarmStartEnabled holds (warm_start on, a method whose fluid region is fixed): the previous pressure at the
+// unknowns, its mean removed when the compatibility condition applies; no guess on a stretched grid
+// (WarmStartEnabled, :215-220) or when that pressure is zero at every unknown.
+int warm_guess(nppe_solver *h, const double *pPrev, double *guess, bool *built)
+{
+    *built = false;
+    if (h->nonUniform || h->n == 0) return NPPE_OK;
+    HIP_TRY(h, launch_gather_idp(h->stream, h->fg, pPrev, guess, h->dPartials), "warm start");
+    HIP_TRY(h, launch_sum_partials(h->stream, h->dPartials, field_row_blocks(h->fg), h->dStepScal + kScalWarm),
+            "warm start");
+    HIP_TRY(h, hipMemcpyAsync(h->hScal + 1, h->dStepScal + kScalWarm, sizeof(double), hipMemcpyDeviceToHost,
+                              h->stream),
+            "warm start");
+    HIP_TRY(h, hipStreamSynchronize(h->stream), "warm start");
+    if (h->hScal[1] == 0.0) return NPPE_OK;   // the first step: solve cold
+    if (needs_compatibility(h))
+    {
+        int nb = 0;
+        HIP_TRY(h, launch_sum_linear(h->stream, guess, h->n, h->dPartials, &nb), "warm start");
+        HIP_TRY(h, launch_sum_partials(h->stream, h->dPartials, nb, h->dStepScal + kScalWarmMean), "warm start");
+        HIP_TRY(h, launch_sub_mean(h->stream, guess, h->n, h->dStepScal + kScalWarmMean), "warm start");
+    }
+    *built = true;
     return NPPE_OK;
 }
 
@@ -175,6 +235,7 @@ int nppe::build_fields(nppe_solver *h)
     const int rows = dim == 3 ? (Ny - 2) * (Nx - 2) : Ny - 2;
     HIP_TRY(h, dalloc(h, &h->dRowStart, (size_t)std::max(rows, 1)), "allocate row starts");
     HIP_TRY(h, dalloc(h, &h->dStepScal, kScalCount), "allocate step scalars");
+    HIP_TRY(h, dalloc(h, &h->dCentre, (size_t)Nx + Ny + Nz), "allocate centre tables");
     FieldGeom &G = h->fg;
     G.B = F.box;
     G.dim = dim;
@@ -192,6 +253,7 @@ int nppe::build_fields(nppe_solver *h)
     G.rows = rows;
     // the row-walking reductions write one partial per block
     if ((int64_t)field_row_blocks(G) > h->partialsCap || 2 * (int64_t)field_box_blocks(G) > h->partialsCap ||
+        2 * (int64_t)inflow_level_blocks(G) > h->partialsCap ||
         h->partialsCap < 1024)
         return fail(h, NPPE_ERR_INTERNAL, "partials buffer too small for the field reductions");
     return refresh_fields(h);
@@ -362,6 +424,79 @@ int nppe_pressure_step_device(nppe_solver *h, const double *d_u, const double *d
     HIP_TRY(h, scatter(h, h->dCompX, d_p, d_p_old, nullptr), "scatter pressure");
     HIP_TRY(h, hipStreamSynchronize(h->stream), "pressure step");
     return rc;
+}
+
+int nppe_warm_start_guess_device(nppe_solver *h, const double *d_p_prev, double *d_guess, int32_t *built)
+{
+    if (int rc = ready(h)) return rc;
+    if (!built || !d_p_prev || (h->n > 0 && !d_guess)) return fail(h, NPPE_ERR_ARGUMENT, "null p_prev, guess or flag");
+    bool b = false;
+    if (int rc = warm_guess(h, d_p_prev, d_guess, &b)) return rc;
+    HIP_TRY(h, hipStreamSynchronize(h->stream), "warm start");
+    *built = b ? 1 : 0;
+    return NPPE_OK;
+}
+
+int nppe_scatter_pressure_level_device(nppe_solver *h, const double *d_x, double *d_p, const double *d_p_old,
+                                       const double *d_prescribed, const nppe_inflow_level *level)
+{
+    if (int rc = ready(h)) return rc;
+    if (!d_p || (h->n > 0 && !d_x)) return fail(h, NPPE_ERR_ARGUMENT, "null x or p");
+    if (int rc = level_args(h, level)) return rc;
+    HIP_TRY(h, scatter(h, d_x, d_p, d_p_old, d_prescribed, level), "scatter pressure");
+    HIP_TRY(h, hipStreamSynchronize(h->stream), "scatter pressure");
+    return NPPE_OK;
+}
+
+int nppe_accumulate_increment_level_device(nppe_solver *h, const double *d_p_old, double *d_p,
+                                           const double *d_prescribed, const nppe_inflow_level *level)
+{
+    if (int rc = ready(h)) return rc;
+    if (!d_p || !d_p_old) return fail(h, NPPE_ERR_ARGUMENT, "null p or p_old");
+    if (int rc = level_args(h, level)) return rc;
+    HIP_TRY(h, accumulate(h, d_p_old, d_p, d_prescribed, level), "accumulate increment");
+    HIP_TRY(h, hipStreamSynchronize(h->stream), "accumulate increment");
+    return NPPE_OK;
+}
+
+int nppe_pressure_step_ex_device(nppe_solver *h, const double *d_u, const double *d_v, const double *d_w, double *d_p,
+                                 const double *d_p_old, double rho, double dt, const nppe_step_options *o,
+                                 nppe_result *res)
+{
+    if (int rc = ready(h)) return rc;
+    if (int rc = fields_args(h, d_u, d_v, d_w, d_p)) return rc;
+    if (int rc = scalars_args(h, rho, dt)) return rc;
+    const nppe_inflow_level *lv = o ? o->inflow_level : nullptr;
+    if (int rc = level_args(h, lv)) return rc;
+    if (o && o->warm_started) *o->warm_started = 0;
+    HIP_TRY(h, rhs(h, d_u, d_v, d_w, d_p, d_p_old, rho, dt, h->dCompB), "right-hand side");
+    double ref = 0.0;
+    if (d_p_old)
+        if (int rc = reference_norm(h, h->dCompB, d_p_old, &ref)) return rc;
+    // SolveScheme (PressureSolver3D.cpp:309-314): only a Chorin step warm starts
+    bool warm = false;
+    if (o && o->warm_start && o->d_p_prev && !d_p_old)
+        if (int rc = warm_guess(h, o->d_p_prev, h->dCompG, &warm)) return rc;
+    const int rc = solve_device(h, h->dCompB, h->dCompX, warm ? h->dCompG : nullptr, ref, res);
+    if (rc < 0) return rc;
+    HIP_TRY(h, scatter(h, h->dCompX, d_p, d_p_old, o ? o->d_prescribed : nullptr, lv), "scatter pressure");
+    HIP_TRY(h, hipStreamSynchronize(h->stream), "pressure step");
+    if (o && o->warm_started) *o->warm_started = warm ? 1 : 0;
+    return rc;
+}
+
+int nppe_projection_step_ex_device(nppe_solver *h, double *d_u, double *d_v, double *d_w, double *d_p,
+                                   const double *d_p_old, double rho, double dt, const double *d_prescribed,
+                                   const nppe_inflow_level *level)
+{
+    if (int rc = ready(h)) return rc;
+    if (int rc = fields_args(h, d_u, d_v, d_w, d_p)) return rc;
+    if (int rc = scalars_args(h, rho, dt)) return rc;
+    if (int rc = level_args(h, level)) return rc;
+    HIP_TRY(h, launch_project(h->stream, h->fg, d_u, d_v, d_w, d_p, dt / rho), "projection");
+    if (d_p_old) HIP_TRY(h, accumulate(h, d_p_old, d_p, d_prescribed, level), "accumulate increment");
+    HIP_TRY(h, hipStreamSynchronize(h->stream), "projection step");
+    return NPPE_OK;
 }
 
 int nppe_projection_step_device(nppe_solver *h, double *d_u, double *d_v, double *d_w, double *d_p,

@@ -39,7 +39,8 @@ EXPORTS = [
     "nppe_enforce_pressure_bc_device", "nppe_increment_reference_norm_device", "nppe_scatter_pressure_device",
     "nppe_accumulate_increment_device", "nppe_solve_laplacian_device", "nppe_solve_laplacian",
     "nppe_pressure_step_device", "nppe_projection_step_device", "nppe_update_layout", "nppe_update_layout_device",
-    "nppe_increment_reference_norm_noalloc",
+    "nppe_increment_reference_norm_noalloc", "nppe_warm_start_guess_device", "nppe_scatter_pressure_level_device",
+    "nppe_accumulate_increment_level_device", "nppe_pressure_step_ex_device", "nppe_projection_step_ex_device",
 ]
 
 
@@ -75,6 +76,15 @@ class Result(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class InflowLevel(C.Structure):
+    _fields_ = [("d_level", C.c_void_p), ("centre_x", C.c_void_p), ("centre_y", C.c_void_p), ("centre_z", C.c_void_p)]
+
+
+class StepOptions(C.Structure):
+    _fields_ = [("d_p_prev", C.c_void_p), ("warm_start", C.c_int32), ("reserved0", C.c_int32),
+                ("d_prescribed", C.c_void_p), ("inflow_level", C.c_void_p), ("warm_started", C.c_void_p)]
 
 
 class SystemReport(C.Structure):
@@ -167,7 +177,12 @@ def lib():
                        ("nppe_solve_laplacian_device", [P, P, P, P]), ("nppe_solve_laplacian", [P, P, P, P]),
                        ("nppe_pressure_step_device", [P, P, P, P, P, P, D, D, P]),
                        ("nppe_projection_step_device", [P, P, P, P, P, P, D, D, P]),
-                       ("nppe_update_layout", [P, P]), ("nppe_update_layout_device", [P, P])):
+                       ("nppe_update_layout", [P, P]), ("nppe_update_layout_device", [P, P]),
+                       ("nppe_warm_start_guess_device", [P, P, P, P]),
+                       ("nppe_scatter_pressure_level_device", [P, P, P, P, P, P]),
+                       ("nppe_accumulate_increment_level_device", [P, P, P, P, P]),
+                       ("nppe_pressure_step_ex_device", [P, P, P, P, P, P, D, D, P, P]),
+                       ("nppe_projection_step_ex_device", [P, P, P, P, P, P, D, D, P, P])):
         getattr(L, name).argtypes = args
         getattr(L, name).restype = C.c_int
     _lib = L
@@ -401,6 +416,51 @@ class Solver:
         self._check(lib().nppe_increment_reference_norm_noalloc(self.h, self._d(b), self._d(p_old), C.byref(out)),
                     "increment_reference_norm_noalloc")
         return out.value
+
+    def _level(self, level):
+        """An nppe_inflow_level from (d_level pointer, cx, cy, cz host arrays), kept alive on the handle."""
+        if level is None:
+            return None
+        ptr, cx, cy, cz = level
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (cx, cy, cz if cz is not None else np.zeros(1))]
+        lv = InflowLevel(C.c_void_p(ptr), arrs[0].ctypes.data, arrs[1].ctypes.data, arrs[2].ctypes.data)
+        self._level_keep = (arrs, lv)
+        return C.byref(lv)
+
+    def warm_start_guess_device(self, p_prev, guess) -> bool:
+        built = C.c_int32()
+        self._check(lib().nppe_warm_start_guess_device(self.h, self._d(p_prev), self._d(guess), C.byref(built)),
+                    "warm_start_guess_device")
+        return bool(built.value)
+
+    def scatter_pressure_level_device(self, x, p, p_old=0, prescribed=0, level=None):
+        self._check(lib().nppe_scatter_pressure_level_device(self.h, self._d(x), self._d(p), self._d(p_old),
+                                                             self._d(prescribed), self._level(level)),
+                    "scatter_pressure_level_device")
+
+    def accumulate_increment_level_device(self, p_old, p, prescribed=0, level=None):
+        self._check(lib().nppe_accumulate_increment_level_device(self.h, self._d(p_old), self._d(p),
+                                                                 self._d(prescribed), self._level(level)),
+                    "accumulate_increment_level_device")
+
+    def pressure_step_ex_device(self, u, v, w, p, p_old, rho, dt, p_prev=0, warm_start=False, prescribed=0,
+                                level=None):
+        """nppe_pressure_step_ex_device: (result, rc, warm_started)."""
+        res = Result()
+        ws = C.c_int32()
+        lvp = self._level(level)
+        o = StepOptions(C.c_void_p(p_prev) if p_prev else None, int(bool(warm_start)), 0,
+                        C.c_void_p(prescribed) if prescribed else None,
+                        C.cast(lvp, C.c_void_p) if lvp is not None else None, C.cast(C.byref(ws), C.c_void_p))
+        rc = self._check(lib().nppe_pressure_step_ex_device(self.h, self._d(u), self._d(v), self._d(w), self._d(p),
+                                                            self._d(p_old), float(rho), float(dt), C.byref(o),
+                                                            C.byref(res)), "pressure_step_ex_device")
+        return res, rc, bool(ws.value)
+
+    def projection_step_ex_device(self, u, v, w, p, p_old, rho, dt, prescribed=0, level=None):
+        self._check(lib().nppe_projection_step_ex_device(self.h, self._d(u), self._d(v), self._d(w), self._d(p),
+                                                         self._d(p_old), float(rho), float(dt), self._d(prescribed),
+                                                         self._level(level)), "projection_step_ex_device")
 
     def scatter_pressure_device(self, x, p, p_old=0, prescribed=0):
         d = self._d

@@ -72,6 +72,10 @@ def lib():
         L.oracle_enforce_pressure_bc.argtypes = [P, P, P]
         L.oracle_scatter_pressure.argtypes = [P, P, C.c_int64, P, P, P, C.c_int, P]
         L.oracle_accumulate_increment.argtypes = [P, P, C.c_int64, P, P, C.c_int, P]
+        L.oracle_level_inflow.argtypes = [P, P, C.c_int64, P, P, P, P, P]
+        L.oracle_level_inflow.restype = C.c_int
+        L.oracle_warm_start_guess.argtypes = [P, P, C.c_int64, P, C.c_int, P]
+        L.oracle_warm_start_guess.restype = C.c_int
         _lib = L
     return _lib
 
@@ -267,3 +271,27 @@ def accumulate_increment(grid: Grid, system: System, p_old, p, level=False, pres
     lib().oracle_accumulate_increment(C.byref(g), _p(system.order), system.n, _p(po), _p(p), int(bool(level)),
                                       _p(pr))
     return p
+
+
+def level_pressure(grid: Grid, system: System, x, level=None, centres=None):
+    """LevelPressure (PressureSolver3D.cpp:181-211) on a copy of x with the compatibility condition in force: the
+    mean removed, then (level: the scene's value at each ring cell, NaN where none; centres: (cx, cy, cz) of
+    MAC::CentrePos) the inflow shift (oracle_level_inflow). Returns (x, contributing ring cells)."""
+    x = np.array(x, np.float64)
+    x -= x.mean() if x.size else 0.0
+    cnt = 0
+    if level is not None:
+        g = grid._c()
+        lv = np.ascontiguousarray(level, np.float64)
+        cx, cy, cz = (np.ascontiguousarray(c, np.float64) for c in centres)
+        cnt = lib().oracle_level_inflow(C.byref(g), _p(system.order), system.n, _p(x), _p(lv), _p(cx), _p(cy), _p(cz))
+    return x, cnt
+
+
+def warm_start_guess(grid: Grid, system: System, p_prev, level):
+    """BuildWarmStartGuess once enabled (oracle_warm_start_guess): (built, x0)."""
+    g = grid._c()
+    x0 = np.zeros(max(1, system.n), np.float64)
+    pp = np.ascontiguousarray(p_prev, np.float64)
+    built = lib().oracle_warm_start_guess(C.byref(g), _p(system.order), system.n, _p(pp), int(bool(level)), _p(x0))
+    return bool(built), x0[: system.n]

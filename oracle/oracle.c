@@ -688,3 +688,99 @@ void oracle_accumulate_increment(const oracle_grid *g, const int32_t *order, int
     free(v);
     oracle_enforce_pressure_bc(g, p, prescribed);
 }
+
+/* LevelPressure's inflow part (PressureSolver3D.cpp:15-30, :181-211; PressureSolver2D.cpp:13-29, :156-178), on x
+ * (IDP order) AFTER its mean has been removed: for each ring cell of the x, y (and z) faces in the reference's loop
+ * order, an INFLOW ring cell whose two inward neighbours are FLUID unknowns and where the scene gives a level
+ * (level[c] not NaN) adds level - (p1 + (p1 - p2) (sR - s1) / (s1 - s2)); x += sum / count when any did. centre:
+ * MAC::CentrePos(a, n), n = 0 .. N-1, per axis. Returns the count. */
+static int64_t unknown_id(const int32_t *inv, int64_t c) { return inv[c]; }
+
+int oracle_level_inflow(const oracle_grid *g, const int32_t *order, int64_t n, double *x, const double *level,
+                        const double *cx, const double *cy, const double *cz)
+{
+    const int Nx = g->Nx, Ny = g->Ny, Nz = g->dim == 2 ? 1 : g->Nz;
+    const int64_t cells = (int64_t)Nx * Ny * Nz;
+    int32_t *inv = (int32_t *)malloc(sizeof(int32_t) * (size_t)cells);
+    for (int64_t c = 0; c < cells; c++) inv[c] = -1;
+    for (int64_t q = 0; q < n; q++) inv[order[q]] = (int32_t)q;
+    double sum = 0.0;
+    int cnt = 0;
+#define AT(i, j, k) (((int64_t)(i) * Nx + (j)) * Nz + (k))
+#define ACC(ri, rj, rk, i1, j1, k1, i2, j2, k2, sR, s1, s2)                                                      \
+    do                                                                                                          \
+    {                                                                                                           \
+        const int64_t cR = AT(ri, rj, rk), c1 = AT(i1, j1, k1), c2 = AT(i2, j2, k2);                            \
+        if (g->types[cR] == ORACLE_INFLOW && g->types[c1] == ORACLE_FLUID && g->types[c2] == ORACLE_FLUID)      \
+        {                                                                                                       \
+            const int64_t id1 = unknown_id(inv, c1), id2 = unknown_id(inv, c2);                                 \
+            if (id1 >= 0 && id2 >= 0 && level[cR] == level[cR])                                                 \
+            {                                                                                                   \
+                const double p1 = x[id1], p2 = x[id2];                                                          \
+                sum += level[cR] - (p1 + (p1 - p2) * ((sR) - (s1)) / ((s1) - (s2)));                            \
+                cnt++;                                                                                          \
+            }                                                                                                   \
+        }                                                                                                       \
+    } while (0)
+    if (g->dim == 3)
+    {
+        for (int i = 1; i < Ny - 1; i++)
+            for (int k = 1; k < Nz - 1; k++)
+            {
+                ACC(i, 0, k, i, 1, k, i, 2, k, cx[0], cx[1], cx[2]);
+                ACC(i, Nx - 1, k, i, Nx - 2, k, i, Nx - 3, k, cx[Nx - 1], cx[Nx - 2], cx[Nx - 3]);
+            }
+        for (int j = 1; j < Nx - 1; j++)
+            for (int k = 1; k < Nz - 1; k++)
+            {
+                ACC(0, j, k, 1, j, k, 2, j, k, cy[0], cy[1], cy[2]);
+                ACC(Ny - 1, j, k, Ny - 2, j, k, Ny - 3, j, k, cy[Ny - 1], cy[Ny - 2], cy[Ny - 3]);
+            }
+        for (int i = 1; i < Ny - 1; i++)
+            for (int j = 1; j < Nx - 1; j++)
+            {
+                ACC(i, j, 0, i, j, 1, i, j, 2, cz[0], cz[1], cz[2]);
+                ACC(i, j, Nz - 1, i, j, Nz - 2, i, j, Nz - 3, cz[Nz - 1], cz[Nz - 2], cz[Nz - 3]);
+            }
+    }
+    else
+    {
+        for (int i = 1; i < Ny - 1; i++)
+        {
+            ACC(i, 0, 0, i, 1, 0, i, 2, 0, cx[0], cx[1], cx[2]);
+            ACC(i, Nx - 1, 0, i, Nx - 2, 0, i, Nx - 3, 0, cx[Nx - 1], cx[Nx - 2], cx[Nx - 3]);
+        }
+        for (int j = 1; j < Nx - 1; j++)
+        {
+            ACC(0, j, 0, 1, j, 0, 2, j, 0, cy[0], cy[1], cy[2]);
+            ACC(Ny - 1, j, 0, Ny - 2, j, 0, Ny - 3, j, 0, cy[Ny - 1], cy[Ny - 2], cy[Ny - 3]);
+        }
+    }
+#undef ACC
+#undef AT
+    free(inv);
+    if (cnt > 0)
+    {
+        const double shift = sum / cnt;
+        for (int64_t q = 0; q < n; q++) x[q] += shift;
+    }
+    return cnt;
+}
+
+/* BuildWarmStartGuess (PressureSolver3D.cpp:222-244; PressureSolver2D.cpp:191-212) once WarmStartEnabled holds:
+ * x0 = p_prev at the unknowns; no guess (returns 0) when that is zero everywhere; the mean removed when `level`
+ * (NeedsCompatibility). */
+int oracle_warm_start_guess(const oracle_grid *g, const int32_t *order, int64_t n, const double *pPrev, int level,
+                            double *x0)
+{
+    (void)g;
+    double sq = 0.0;
+    for (int64_t q = 0; q < n; q++)
+    {
+        x0[q] = pPrev[order[q]];
+        sq += x0[q] * x0[q];
+    }
+    if (sq == 0.0) return 0;
+    if (level) remove_mean(x0, n);
+    return 1;
+}

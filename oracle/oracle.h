@@ -23,6 +23,11 @@
  *                           incremental ring fix) where no mean is removed; within 1e-14 of it where LevelPressure
  *                           removes the mean (Eigen's vectorised mean sums in another order).
  *   oracle_accumulate_increment - the same bars against the compiled reference's AccumulatePressureIncrement.
+ *   oracle_level_inflow   - against the compiled reference's LevelPressure with a scene of inlet p= levels
+ *                           (level_*.npz: 2D/3D, uniform/stretched), to 1e-14 (the mean sums in another order);
+ *                           a constant level c on every inlet makes the extrapolated ring values average c.
+ *   oracle_warm_start_guess - against the compiled reference's BuildWarmStartGuess (level_*.npz), bit-exact where
+ *                           no mean is removed, 1e-14 where it is; a zero previous pressure gives no guess.
  */
 #ifndef NPPE_ORACLE_H
 #define NPPE_ORACLE_H
@@ -112,6 +117,16 @@ void oracle_scatter_pressure(const oracle_grid *g, const int32_t *order, int64_t
  * unknowns (phi: p on entry), the mean removed when `level` is set, then the ring BC. */
 void oracle_accumulate_increment(const oracle_grid *g, const int32_t *order, int64_t n, const double *pOld,
                                  double *p, int level, const double *prescribed);
+
+/* LevelPressure's inflow shift (PressureSolver3D.cpp:15-30, :181-211; PressureSolver2D.cpp:13-29, :156-178) on a
+ * mean-free x (IDP order): level[c] is the scene's level at ring cell c (NaN: none), centre_* MAC::CentrePos per axis.
+ * Returns the number of contributing ring cells. */
+int oracle_level_inflow(const oracle_grid *g, const int32_t *order, int64_t n, double *x, const double *level,
+                        const double *centre_x, const double *centre_y, const double *centre_z);
+
+/* BuildWarmStartGuess (PressureSolver3D.cpp:222-244) once WarmStartEnabled holds: returns 1 and x0, or 0 (no guess). */
+int oracle_warm_start_guess(const oracle_grid *g, const int32_t *order, int64_t n, const double *p_prev, int level,
+                            double *x0);
 
 /* |b - A x|_2 / |b|_2 in FP64. */
 double oracle_relres(int32_t n, const int32_t *rowptr, const int32_t *col, const double *val, const double *b,

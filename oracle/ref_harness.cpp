@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -250,6 +251,29 @@ int main(int argc, char **argv)
         xTail.resize((size_t)n);
         r.get(xTail.data(), (size_t)n);
     }
+    // optional extras: a scene (inlet faces with p= levels) for LevelPressure on a given vector, and a previous
+    // pressure for BuildWarmStartGuess
+    int hasExtras = 0;
+    std::string sceneText;
+    double tLevel = 0.0;
+    std::vector<double> xLevel, pPrev;
+    int hasWarm = 0;
+    if (std::fread(&hasExtras, sizeof(int32_t), 1, fi) != 1) hasExtras = 0;
+    if (hasExtras)
+    {
+        const int len = r.get<int32_t>();
+        sceneText.resize((size_t)len);
+        if (len) r.get(&sceneText[0], (size_t)len);
+        tLevel = r.get<double>();
+        xLevel.resize((size_t)n);
+        r.get(xLevel.data(), (size_t)n);
+        hasWarm = r.get<int32_t>();
+        if (hasWarm)
+        {
+            pPrev.resize(cells);
+            r.get(pPrev.data(), cells);
+        }
+    }
     std::fclose(fi);
 
     w.put<int32_t>(nsolves);
@@ -387,6 +411,58 @@ int main(int argc, char **argv)
             if (dim == 2) PressureSolver2D::AccumulatePressureIncrement(&OLD, &GRID);
             else PressureSolver3D::AccumulatePressureIncrement(&OLD, &GRID);
             putP(GRID);
+        }
+    }
+    if (hasExtras)
+    {
+        // LevelPressure (PressureSolver{2D,3D}.cpp) with the scene's inlet levels: the per-cell level the scene gives
+        // at each ring cell's centre (NaN where it gives none), the centre tables, and the levelled vector
+        const bool loaded = !sceneText.empty() && SCENE.LoadText(sceneText, "<harness>");
+        if (!sceneText.empty() && !loaded) std::fprintf(stderr, "ref_harness: scene: %s\n", SCENE.error.c_str());
+        w.put<int32_t>(loaded ? 1 : 0);
+        SIMULATION.time = tLevel;
+        for (int i = 0; i < Ny; i++)
+            for (int j = 0; j < Nx; j++)
+                for (int k = 0; k < Nz; k++)
+                {
+                    double lv = std::numeric_limits<double>::quiet_NaN();
+                    const bool ring = i == 0 || i == Ny - 1 || j == 0 || j == Nx - 1 ||
+                                      (dim == 3 && (k == 0 || k == Nz - 1));
+                    double v = 0.0;
+                    if (loaded && ring &&
+                        SCENE.PressureLevelAt(GRID.CentrePos(MAC::X, j), GRID.CentrePos(MAC::Y, i),
+                                              dim == 3 ? GRID.CentrePos(MAC::Z, k) : 0.0, tLevel, v))
+                        lv = v;
+                    w.put<double>(lv);
+                }
+        for (int j = 0; j < Nx; j++) w.put<double>(GRID.CentrePos(MAC::X, j));
+        for (int i = 0; i < Ny; i++) w.put<double>(GRID.CentrePos(MAC::Y, i));
+        if (dim == 3)
+            for (int k = 0; k < Nz; k++) w.put<double>(GRID.CentrePos(MAC::Z, k));
+        Eigen::VectorXd xl = Eigen::Map<Eigen::VectorXd>(xLevel.data(), n);
+        if (dim == 2) PressureSolver2D::LevelPressure(&GRID, xl);
+        else PressureSolver3D::LevelPressure(&GRID, xl);
+        w.put(xl.data(), (size_t)n);
+        // BuildWarmStartGuess from a previous pressure, with warm_start on under a fixed-region method (ADI)
+        w.put<int32_t>(hasWarm);
+        if (hasWarm)
+        {
+            MAC ANT = GRID;
+            for (int i = 0; i < Ny; i++)
+                for (int j = 0; j < Nx; j++)
+                    for (int k = 0; k < Nz; k++)
+                    {
+                        const double v = pPrev[((size_t)i * Nx + j) * Nz + k];
+                        if (dim == 2) ANT.SetP(i, j, v);
+                        else ANT.SetP(i, j, k, v);
+                    }
+            PRESSURE_SETTINGS.warmStart = true;
+            Eigen::VectorXd x0(n);
+            const bool built = dim == 2 ? PressureSolver2D::BuildWarmStartGuess(&ANT, x0)
+                                        : PressureSolver3D::BuildWarmStartGuess(&ANT, x0);
+            PRESSURE_SETTINGS.warmStart = false;
+            w.put<int32_t>(built ? 1 : 0);
+            w.put(x0.data(), (size_t)n);
         }
     }
     std::fclose(fo);

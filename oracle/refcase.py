@@ -53,6 +53,10 @@ class RefCase:
     # projected fields; with "t" (time) and "x_tail" (n values standing for a solve's answer) also p after
     # EnforcePressureBC, after SolveScheme's tail and (with p_old) after AccumulatePressureIncrement
     fields: Optional[dict] = None
+    # LevelPressure with a scene's inlet levels and BuildWarmStartGuess: {"scene": text (inlet faces with p=),
+    # "t": time, "x": n values to level, "p_prev": MAC pressure for the warm start (optional)}; the harness returns
+    # the scene's level at every ring cell, the centre tables, the levelled x and the warm-start guess
+    extras: Optional[dict] = None
 
     def encode(self) -> bytes:
         out = [b"NPPECASE", struct.pack("<ii", self.dim, self.grid_size), struct.pack("<6d", *self.domain),
@@ -87,6 +91,17 @@ class RefCase:
                 out.append(np.ascontiguousarray(f["p_old"], np.float64).tobytes())
             out.append(struct.pack("<d", f.get("t", 0.0)))
             out.append(np.ascontiguousarray(f["x_tail"], np.float64).tobytes())
+        e = self.extras
+        out.append(struct.pack("<i", 1 if e else 0))
+        if e:
+            text = e.get("scene", "").encode()
+            out.append(struct.pack("<i", len(text)))
+            out.append(text)
+            out.append(struct.pack("<d", e.get("t", 0.0)))
+            out.append(np.ascontiguousarray(e["x"], np.float64).tobytes())
+            out.append(struct.pack("<i", 1 if e.get("p_prev") is not None else 0))
+            if e.get("p_prev") is not None:
+                out.append(np.ascontiguousarray(e["p_prev"], np.float64).tobytes())
         return b"".join(out)
 
 
@@ -160,4 +175,15 @@ def run(case: RefCase, timeout: float = 3600.0, threads: Optional[int] = None) -
         res["p_tail"] = b.arr(np.float64, cells)
         if case.fields.get("p_old") is not None:
             res["p_acc"] = b.arr(np.float64, cells)
+    if case.extras:
+        Nzz = Nz if case.dim == 3 else 1
+        res["scene_loaded"] = bool(b.take("<i")[0])
+        res["inflow_level"] = b.arr(np.float64, Ny * Nx * Nzz)
+        res["cx"] = b.arr(np.float64, Nx)
+        res["cy"] = b.arr(np.float64, Ny)
+        res["cz"] = b.arr(np.float64, Nz) if case.dim == 3 else np.zeros(1)
+        res["x_levelled"] = b.arr(np.float64, n)
+        if b.take("<i")[0]:
+            res["warm_built"] = bool(b.take("<i")[0])
+            res["warm_guess"] = b.arr(np.float64, n)
     return res

@@ -19,6 +19,8 @@ SolveEigenPressure) on some of those systems: their iteration counts and solutio
 
     python tests/golden/make_fixtures.py            # everything
     python tests/golden/make_fixtures.py --floor    # the floor fixtures only
+    python tests/golden/make_fixtures.py --level    # the level_*.npz fixtures only: LevelPressure with a scene's
+                                                    # inlet levels and BuildWarmStartGuess
 """
 from __future__ import annotations
 
@@ -224,7 +226,62 @@ def main_floor():
         print(f"floor_{name}: n={out['n']} iters={arrays['iters'].tolist()}")
 
 
+# LevelPressure's inflow shift (PressureSolver3D.cpp:15-30, :181-211) and BuildWarmStartGuess (:215-244): inlet
+# faces with scene p= levels on an INFLOW ring (singular, compatibility on), 2D/3D, uniform/stretched, interior
+# solids beside the inlets; a previous pressure for the warm start (zero in one case: no guess), and a non-singular
+# case where LevelPressure does nothing and the guess keeps its mean.
+SCENE_A = ('boundary xmin inlet u="1" p="0.5+y*z+t"\nboundary xmax inlet u="1" p="-0.25+x*y"\n'
+           'boundary ymin noslip\nboundary ymax noslip\nboundary zmin inlet w="0.1" p="2*x-t"\nboundary zmax noslip\n')
+SCENE_B = ('boundary xmin noslip\nboundary xmax noslip\nboundary ymin inlet v="1" p="1.5-x"\n'
+           'boundary ymax inlet v="-1" p="0.75+x*z"\nboundary zmin noslip\nboundary zmax inlet w="-0.5" p="3+t*y"\n')
+SCENE_2D = ('boundary xmin inlet u="1" p="0.5+y+t"\nboundary xmax noslip\nboundary ymin noslip\n'
+            'boundary ymax inlet v="-1" p="2-x*x"\n')
+LEVEL_CASES = [
+    # name, dim, GRID_SIZE, non_uniform, refinements, scene, ring kind, interior solids, warm start
+    ("level_u3_inflow", 3, 8, False, ((), (), ()), SCENE_A, "inflow", False, "random"),
+    ("level_s3_inflow", 3, 10, True,
+     ((REF(0.3, 0.7, 3, SY),), (REF(0.0, 0.4, 2, RA), REF(0.6, 1.0, 2, RB)), (REF(0.2, 0.5, 2, H),)), SCENE_B, "inflow",
+     True, "random"),
+    ("level_u3_solids", 3, 9, False, ((), (), ()), SCENE_A, "inflow", True, "zero"),
+    ("level_u2_inflow", 2, 12, False, ((), (), ()), SCENE_2D, "inflow", True, "random"),
+    ("level_u3_outlet", 3, 8, False, ((), (), ()), SCENE_A, "outlet", False, "random"),
+]
+
+
+def main_level():
+    for q, (name, dim, gs, nu, refs, scene, kind, solids, warm) in enumerate(LEVEL_CASES):
+        probe = R.run(R.RefCase(dim=dim, grid_size=gs, non_uniform=nu, refine=refs))
+        shape = (probe["Ny"], probe["Nx"], probe["Nz"])
+        rng = np.random.default_rng(4000 + q)
+        t = np.zeros(shape, np.uint8)
+        ring = ring_mask(shape, dim)
+        t[ring] = INFLOW if kind == "inflow" else SOLID
+        if kind == "outlet":
+            t[:, -1][ring[:, -1]] = EMPTY
+        if solids:
+            inner = ~ring
+            t[inner & (rng.random(shape) < 0.12)] = SOLID
+        inner_t = t[1:-1, 1:-1, 1:-1] if dim == 3 else t[1:-1, 1:-1, :]
+        n = int((inner_t == FLUID).sum())
+        x = rng.standard_normal(n) + 2.5
+        p_prev = rng.standard_normal(t.size) + 1.0 if warm == "random" else np.zeros(t.size)
+        out = R.run(R.RefCase(dim=dim, grid_size=gs, non_uniform=nu, refine=refs, types=t, compat=True,
+                              extras={"scene": scene, "t": 0.3, "x": x, "p_prev": p_prev}), threads=1)
+        wz = out["wz"] if dim == 3 else np.array([out["dh"]])
+        np.savez_compressed(
+            os.path.join(HERE, name + ".npz"),
+            dim=dim, non_uniform=int(out["non_uniform"]), dh=out["dh"], wx=out["wx"], wy=out["wy"], wz=wz, types=t,
+            singular=int(out["singular"]), order=out["order"], x=x, t=0.3, scene=scene,
+            inflow_level=out["inflow_level"], cx=out["cx"], cy=out["cy"], cz=out["cz"], x_levelled=out["x_levelled"],
+            p_prev=p_prev, warm_built=int(out["warm_built"]), warm_guess=out["warm_guess"])
+        print(f"{name}: n={out['n']} singular={out['singular']} levels={int(np.isfinite(out['inflow_level']).sum())} "
+              f"warm={out['warm_built']}")
+
+
 if __name__ == "__main__":
+    if "--level" in sys.argv:
+        main_level()
+        raise SystemExit(0)
     if "--floor" in sys.argv:
         main_floor()
         raise SystemExit(0)
@@ -232,3 +289,4 @@ if __name__ == "__main__":
         main()
     main_fields()
     main_floor()
+    main_level()

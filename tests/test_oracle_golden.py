@@ -11,7 +11,7 @@ import pytest
 import oracle as O
 
 GOLDEN = sorted(p for p in glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz"))
-                if not os.path.basename(p).startswith(("fields_", "floor_")))
+                if not os.path.basename(p).startswith(("fields_", "floor_", "level_")))
 
 
 def load(path):
