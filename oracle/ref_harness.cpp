@@ -195,10 +195,6 @@ int main(int argc, char **argv)
                 const int id = dim == 2 ? PressureSolver2D::GetIDP(i, j) : PressureSolver3D::GetIDP(i, j, k);
                 if (id >= 0) order[id] = (int32_t)(((size_t)i * Nx + j) * Nz + k);
             }
-    w.put(order.data(), order.size());
-    w.put(A->outerIndexPtr(), (size_t)n + 1);
-    w.put(A->innerIndexPtr(), (size_t)nnz);
-    w.put(A->valuePtr(), (size_t)nnz);
 
     const int nsolves = r.get<int32_t>();
     std::vector<SolveSpec> specs(nsolves);
@@ -274,7 +270,18 @@ int main(int argc, char **argv)
             r.get(pPrev.data(), cells);
         }
     }
+    // output flags (timing runs of large systems): bit 0 leaves the unknown order and the matrix out of the output,
+    // bit 1 the solutions (iterations and times are always written)
+    int outFlags = 0;
+    if (std::fread(&outFlags, sizeof(int32_t), 1, fi) != 1) outFlags = 0;
     std::fclose(fi);
+    if (!(outFlags & 1))
+    {
+        w.put(order.data(), order.size());
+        w.put(A->outerIndexPtr(), (size_t)n + 1);
+        w.put(A->innerIndexPtr(), (size_t)nnz);
+        w.put(A->valuePtr(), (size_t)nnz);
+    }
 
     w.put<int32_t>(nsolves);
     for (const SolveSpec &s : specs)
@@ -295,7 +302,7 @@ int main(int argc, char **argv)
         w.put<int32_t>(iters);
         w.put<int32_t>((int32_t)times.size());
         w.put(times.data(), times.size());
-        w.put(x.data(), (size_t)n);
+        if (!(outFlags & 2)) w.put(x.data(), (size_t)n);
     }
     w.put<int32_t>(omp_get_max_threads());
 
@@ -457,10 +464,11 @@ int main(int argc, char **argv)
                         else ANT.SetP(i, j, k, v);
                     }
             PRESSURE_SETTINGS.warmStart = true;
-            Eigen::VectorXd x0(n);
+            Eigen::VectorXd x0 = Eigen::VectorXd::Zero(n);
             const bool built = dim == 2 ? PressureSolver2D::BuildWarmStartGuess(&ANT, x0)
                                         : PressureSolver3D::BuildWarmStartGuess(&ANT, x0);
             PRESSURE_SETTINGS.warmStart = false;
+            if (!built) x0.setZero();   // no guess: the vector is not meaningful
             w.put<int32_t>(built ? 1 : 0);
             w.put(x0.data(), (size_t)n);
         }

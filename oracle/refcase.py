@@ -57,6 +57,9 @@ class RefCase:
     # "t": time, "x": n values to level, "p_prev": MAC pressure for the warm start (optional)}; the harness returns
     # the scene's level at every ring cell, the centre tables, the levelled x and the warm-start guess
     extras: Optional[dict] = None
+    # timing runs of large systems: leave the unknown order and the matrix (and the solutions) out of the output
+    write_matrix: bool = True
+    write_solutions: bool = True
 
     def encode(self) -> bytes:
         out = [b"NPPECASE", struct.pack("<ii", self.dim, self.grid_size), struct.pack("<6d", *self.domain),
@@ -102,6 +105,7 @@ class RefCase:
             out.append(struct.pack("<i", 1 if e.get("p_prev") is not None else 0))
             if e.get("p_prev") is not None:
                 out.append(np.ascontiguousarray(e["p_prev"], np.float64).tobytes())
+        out.append(struct.pack("<i", (0 if self.write_matrix else 1) | (0 if self.write_solutions else 2)))
         return b"".join(out)
 
 
@@ -150,16 +154,17 @@ def run(case: RefCase, timeout: float = 3600.0, threads: Optional[int] = None) -
         return res
     n, nnz, singular = b.take("<iii")
     res.update(n=n, nnz=nnz, singular=bool(singular))
-    res["order"] = b.arr(np.int32, n)
-    res["rowptr"] = b.arr(np.int32, n + 1)
-    res["col"] = b.arr(np.int32, nnz)
-    res["val"] = b.arr(np.float64, nnz)
+    if case.write_matrix:
+        res["order"] = b.arr(np.int32, n)
+        res["rowptr"] = b.arr(np.int32, n + 1)
+        res["col"] = b.arr(np.int32, nnz)
+        res["val"] = b.arr(np.float64, nnz)
     nsolves = b.take("<i")[0]
     res["solves"] = []
     for _ in range(nsolves):
         iters, nt = b.take("<ii")
         times = b.arr(np.float64, nt)
-        x = b.arr(np.float64, n)
+        x = b.arr(np.float64, n) if case.write_solutions else None
         res["solves"].append({"iters": iters, "times": times, "x": x})
     res["threads"] = b.take("<i")[0]
     if case.fields:
