@@ -479,68 +479,93 @@ def batched_leg(args):
                        "cells": cells}}
 
 
+# The serving stream's resident grid set: (dim, cells per axis, boundary condition), spanning the stated mix.
+SERVING_GRIDS = [(2, 256, "outlet"), (2, 512, "neumann"), (2, 1024, "freesurface"), (2, 2048, "outlet"),
+                 (3, 64, "freesurface"), (3, 96, "neumann"), (3, 128, "outlet"), (3, 192, "freesurface"),
+                 (3, 256, "neumann")]
+
+
 def serving_leg(args):
-    """Poisson arrivals of solve requests (2D 256^2-2048^2 / 3D 64^3-256^3, Neumann / outlet / free surface,
-    tolerance 1e-6..1e-10) through the solve server; the same stream through a server running the FP64 CG."""
+    """The serving stream on a fixed set of resident grids (2D 256^2-2048^2, 3D 64^3-256^3; Neumann / p=0 outlet /
+    free surface), each request a seeded right-hand side and a tolerance 10^U(-10,-6) on a uniformly drawn grid:
+    1. first use: one request per grid, its latency including the commit (reported apart, as commit latency);
+    2. capacity: every request submitted at once (closed loop), solves/s;
+    3. serving: Poisson arrivals at 70% of that capacity (open loop), solves/s and p50/p99 latency from submit to
+       completion; the FP64 CG server on the same stream at the same rate, beside it."""
     import threading
     import nereus_cfd_amd as N
     import nereus_cfd_amd.workloads as W
     rng = np.random.default_rng(SEED)
-    reqs = W.serving_mix(args.requests, rng)
-    rate = args.rate
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(reqs)))
-    keys = {}
-    grids = {}
-    for (dim, n, bc, tol, seed) in reqs:
-        k = (dim, n, bc, seed % 4)
-        if k not in keys:
-            keys[k] = len(keys)
-            grids[keys[k]] = W.serving_grid(dim, n, bc, seed % 4)
-    rhs = []
-    for (dim, n, bc, tol, seed) in reqs:
-        g = grids[keys[(dim, n, bc, seed % 4)]]
-        rhs.append(W.random_rhs(g, np.random.default_rng(seed), zero_mean=(bc == "neumann")))
+    grids = [W.serving_grid(d, n, bc, seed=k) for k, (d, n, bc) in enumerate(SERVING_GRIDS)]
+    pool = [[W.random_rhs(g, np.random.default_rng(SEED + 7 * k + v), zero_mean=(SERVING_GRIDS[k][2] == "neumann"))
+             for v in range(2)] for k, g in enumerate(grids)]
+    nreq = args.requests
+    reqs = [(int(rng.integers(0, len(grids))), float(10.0 ** rng.uniform(-10.0, -6.0)), int(rng.integers(0, 2)))
+            for _ in range(nreq)]
+    gaps = rng.exponential(1.0, size=nreq)   # unit-rate gaps, scaled to the serving rate below
 
-    def run(method):
-        out = []
-        with N.Server(slots=args.slots, hbm_budget=int(args.budget_gb * 2 ** 30), method=method,
-                      max_iters=100000) as srv:
-            for k, g in grids.items():
-                srv.add_grid(k, g)
-            tickets = [None] * len(reqs)
-            t0 = time.perf_counter()
+    def stream(srv, rate):
+        """Submits reqs with Poisson gaps at `rate` (inf: all at once); returns (latencies, wall, all converged)."""
+        tickets = [None] * nreq
+        arrivals = np.cumsum(gaps / rate) if np.isfinite(rate) else np.zeros(nreq)
+        t0 = time.perf_counter()
 
-            def submitter():
-                for q, (dim, n, bc, tol, seed) in enumerate(reqs):
-                    delay = arrivals[q] - (time.perf_counter() - t0)
-                    if delay > 0:
-                        time.sleep(delay)
-                    tickets[q] = srv.submit(keys[(dim, n, bc, seed % 4)], rhs[q], tol)
+        def submitter():
+            for q, (k, tol, v) in enumerate(reqs):
+                delay = arrivals[q] - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+                tickets[q] = srv.submit(k, pool[k][v], tol)
 
-            th = threading.Thread(target=submitter)
-            th.start()
-            for q in range(len(reqs)):
-                while tickets[q] is None:
-                    time.sleep(0.0005)
-                x, res, lat, rc = srv.wait(tickets[q])
-                out.append((lat, res.relative_residual <= reqs[q][3] * 1.0000001, rc))
-            th.join()
-            el = time.perf_counter() - t0
-            peak = srv.memory()[1]
-        lats = np.array([o[0] for o in out])
-        return {"solves_per_s": len(reqs) / el, "p50_ms": float(np.percentile(lats, 50)),
-                "p99_ms": float(np.percentile(lats, 99)), "all_converged": all(o[1] and o[2] == 0 for o in out),
-                "peak_hbm_gb": peak / 2 ** 30, "wall_s": el}
+        th = threading.Thread(target=submitter)
+        th.start()
+        lats, ok = [], True
+        for q in range(nreq):
+            while tickets[q] is None:
+                time.sleep(0.0002)
+            x, res, lat, rc = srv.wait(tickets[q])
+            lats.append(lat)
+            ok = ok and rc == 0 and res.relative_residual <= reqs[q][1] * 1.0000001
+        th.join()
+        return np.array(lats), time.perf_counter() - t0, ok
 
-    mg = run(N.METHOD_MGPCG)
-    cg = run(N.METHOD_CG64) if not args.no_cg64 else None
+    def server(method):
+        srv = N.Server(slots=args.slots, hbm_budget=int(args.budget_gb * 2 ** 30), method=method, max_iters=100000)
+        first = []
+        for k, g in enumerate(grids):
+            srv.add_grid(k, g)
+            t = srv.submit(k, pool[k][0], 1e-8)
+            x, res, lat, rc = srv.wait(t)
+            first.append(lat)
+        return srv, first
+
+    srv, first = server(N.METHOD_MGPCG)
+    lat_c, wall_c, ok_c = stream(srv, float("inf"))
+    capacity = nreq / wall_c
+    rate = args.rate if args.rate > 0 else 0.7 * capacity
+    lat, wall, ok = stream(srv, rate)
+    peak = srv.memory()[1]
+    srv.close()
+    log(f"[bench] serving: capacity {capacity:.1f} solves/s; at {rate:.1f}/s p50 {np.percentile(lat, 50):.1f} ms "
+        f"p99 {np.percentile(lat, 99):.1f} ms")
+    cg = None
+    if not args.no_cg64:
+        srv, first_cg = server(N.METHOD_CG64)
+        lat_g, wall_g, ok_g = stream(srv, rate)
+        srv.close()
+        cg = {"solves_per_s": nreq / wall_g, "p50_ms": float(np.percentile(lat_g, 50)),
+              "p99_ms": float(np.percentile(lat_g, 99)), "all_converged": ok_g, "first_request_ms": first_cg}
     return {"metric": "serving solves/s and p50/p99 latency at a fixed Poisson arrival rate",
-            "value": mg["solves_per_s"], "unit": "solves/s", "p50_ms": mg["p50_ms"], "p99_ms": mg["p99_ms"],
-            "all_converged": mg["all_converged"], "peak_hbm_gb": mg["peak_hbm_gb"], "fp64_cg": cg,
-            "config": {"workload": f"{len(reqs)} requests, Poisson arrivals at {rate}/s; 2D 256^2-2048^2 or 3D "
-                                   f"64^3-256^3; Neumann / p=0 outlet / free surface; tolerance 10^U(-10,-6); "
-                                   f"{len(grids)} distinct resident grids; {args.slots} slots, HBM budget "
-                                   f"{args.budget_gb} GiB", "seed": SEED}}
+            "value": nreq / wall, "unit": "solves/s", "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "arrival_rate_per_s": rate, "capacity_solves_per_s": capacity,
+            "capacity_p50_ms": float(np.percentile(lat_c, 50)), "all_converged": ok and ok_c,
+            "first_request_ms": {f"{d}D {n} {bc}": round(v, 1) for (d, n, bc), v in zip(SERVING_GRIDS, first)},
+            "peak_hbm_gb": peak / 2 ** 30, "fp64_cg": cg,
+            "config": {"workload": f"{nreq} requests on {len(grids)} resident grids (2D 256^2-2048^2, 3D 64^3-256^3; "
+                                   f"Neumann / p=0 outlet / free surface), tolerance 10^U(-10,-6); Poisson arrivals "
+                                   f"at {rate:.1f}/s = {rate / capacity:.0%} of the measured capacity; {args.slots} "
+                                   f"slots, HBM budget {args.budget_gb} GiB; first use (commit) reported apart",
+                       "seed": SEED}}
 
 
 def main():
@@ -552,8 +577,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg64", action="store_true")
     ap.add_argument("--config", default="headline", choices=["headline", "channel", "batched", "serving", "copy", "step", "layout"])
-    ap.add_argument("--requests", type=int, default=32)
-    ap.add_argument("--rate", type=float, default=40.0)
+    ap.add_argument("--requests", type=int, default=64)
+    ap.add_argument("--rate", type=float, default=0.0, help="serving arrival rate (0: 70%% of the measured capacity)")
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--budget-gb", type=float, default=64.0)
     args = ap.parse_args()

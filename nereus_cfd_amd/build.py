@@ -16,8 +16,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnppe.so")
 
-SOURCES = ["kernels.hip", "fields.hip", "layout.hip", "noalloc.hip", "solver.cpp", "step.cpp", "hierarchy.cpp",
-           "synth.cpp", "serve.cpp"]
+SOURCES = ["kernels.hip", "fields.hip", "layout.hip", "noalloc.hip", "coarse.hip", "solver.cpp", "step.cpp",
+           "hierarchy.cpp", "synth.cpp", "serve.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NPPE_OFFLOAD_ARCH", "gfx950")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I" + os.path.join(ROOT, "include")]

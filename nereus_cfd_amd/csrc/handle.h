@@ -73,6 +73,15 @@ struct nppe_solver
     double *dCompB = nullptr, *dCompX = nullptr, *dCompG = nullptr;
     double *dCoarseM = nullptr;
     int32_t *dCoarseOrder = nullptr;
+    // building the coarsest pseudo-inverse on the device (coarse.hip): Gauss-Jordan's second matrix, the level's
+    // unknown map over its box, FP64 widths, component labels, counts, shifts, status
+    double *dCoarseWork = nullptr;
+    int32_t *dCoarseId = nullptr;
+    double *dCoarseW = nullptr;
+    int32_t *dCoarseLab = nullptr;
+    int32_t *dCoarseAux = nullptr;
+    double *dCoarseShift = nullptr;
+    int32_t *dCoarseState = nullptr;
     int nCoarse = 0;
     bool coarseDense = false;
     nppe::SolveState *dState = nullptr;
@@ -189,6 +198,10 @@ hipError_t dalloc_pairs(nppe_solver *h, T **p, size_t count)
 
 // Slot of the handle's step scalars (dStepScal) that holds nppe_update_layout_device's count of changed cells.
 constexpr int kStepScalDiff = 5;
+
+// The coarsest level's dense pseudo-inverse (n unknowns) built on the device into dCoarseM / dCoarseOrder
+// (coarse.hip).
+int coarse_pinv_device(nppe_solver *h, int n);
 
 // The allocation-free entry points' scratch array on `device` (current), looked up once per device (noalloc.hip).
 int noalloc_scratch(int device, double **out);

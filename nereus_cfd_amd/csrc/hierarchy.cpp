@@ -3,7 +3,6 @@
 
 #include <algorithm>
 #include <cmath>
-#include <queue>
 
 namespace nppe
 {
@@ -152,41 +151,6 @@ HostLevel coarsen(const HostLevel &f)
     return c;
 }
 
-void stencil_codes(const HostLevel &L, std::vector<uint8_t> &code8, std::vector<uint16_t> &code16)
-{
-    const int ns = L.ns, nm = L.nm, nf = L.nf;
-    const size_t cells = (size_t)ns * nm * nf;
-    code8.assign(cells, 0);
-    code16.assign(cells, 0);
-    auto at = [&](int s, int m, int f) { return ((size_t)s * nm + m) * nf + f; };
-    const int s0 = L.threeD ? 1 : 0, s1 = L.threeD ? ns - 2 : 0;
-    const int ds[6] = {1, -1, 0, 0, 0, 0}, dm[6] = {0, 0, 1, -1, 0, 0}, df[6] = {0, 0, 0, 0, 1, -1};
-    for (int s = s0; s <= s1; s++)
-        for (int m = 1; m <= nm - 2; m++)
-            for (int f = 1; f <= nf - 2; f++)
-            {
-                const size_t c = at(s, m, f);
-                if (L.types[c] != kFluid) continue;
-                uint8_t c8 = 1u << 6;
-                uint16_t c16 = 1u << 12;
-                for (int q = L.threeD ? 0 : 2; q < 6; q++)
-                {
-                    const int S = s + ds[q], M = m + dm[q], F = f + df[q];
-                    const bool ring = (L.threeD && (S == 0 || S == ns - 1)) || M == 0 || M == nm - 1 || F == 0 ||
-                                      F == nf - 1;
-                    const uint8_t t = L.types[at(S, M, F)];
-                    unsigned st = 0;
-                    if (t == kFluid) st = ring ? 2u : 1u;
-                    else if (is_dirichlet(t)) st = ring ? 3u : 2u;
-                    if (st == 1u) c8 |= (uint8_t)(1u << q);
-                    if (st >= 2u) c8 |= (uint8_t)(1u << 7);
-                    c16 |= (uint16_t)(st << (2 * q));
-                }
-                code8[c] = c8;
-                code16[c] = c16;
-            }
-}
-
 std::vector<float> tables32(const HostLevel &L, double sigma, int off[9])
 {
     std::vector<float> out;
@@ -228,162 +192,6 @@ std::vector<double> tables64(const HostLevel &L, int off[6])
         for (int m = 0; m < std::max(1, N - 1); m++) out.push_back(N >= 2 ? 0.5 * (w[m] + w[m + 1]) : 0.0);
     }
     return out;
-}
-
-bool coarse_pseudo_inverse(const HostLevel &L, double sigma, std::vector<int64_t> &order, std::vector<double> &M,
-                           std::string &err)
-{
-    const int ns = L.ns, nm = L.nm, nf = L.nf;
-    auto at = [&](int s, int m, int f) { return ((int64_t)s * nm + m) * nf + f; };
-    std::vector<int> id((size_t)ns * nm * nf, -1);
-    order.clear();
-    const int s0 = L.threeD ? 1 : 0, s1 = L.threeD ? ns - 2 : 0;
-    for (int s = s0; s <= s1; s++)
-        for (int m = 1; m <= nm - 2; m++)
-            for (int f = 1; f <= nf - 2; f++)
-                if (L.types[at(s, m, f)] == kFluid)
-                {
-                    id[at(s, m, f)] = (int)order.size();
-                    order.push_back(at(s, m, f));
-                }
-    const int n = (int)order.size();
-    if (n > kMaxDenseCoarse)
-    {
-        err = "coarsest level too large for the dense solve";
-        return false;
-    }
-    // B = -A (positive semi-definite) in the sigma-scaled volume form of the FP32 levels.
-    std::vector<double> B((size_t)n * n, 0.0);
-    std::vector<char> touchesDirichlet(n, 0);
-    const std::vector<double> &wS = L.w[0], &wM = L.w[1], &wF = L.w[2];
-    for (int r = 0; r < n; r++)
-    {
-        const int64_t c = order[r];
-        const int f = (int)(c % nf), m = (int)((c / nf) % nm), s = (int)(c / ((int64_t)nf * nm));
-        const double aS = wM[m] * wF[f], aM = wS[s] * wF[f], aF = wS[s] * wM[m];
-        auto gap = [](const std::vector<double> &w, int q) { return 0.5 * (w[q] + w[q + 1]); };
-        double cq[6] = {0, 0, 0, 0, 0, 0}, ring[6] = {0, 0, 0, 0, 0, 0};
-        if (L.threeD)
-        {
-            cq[0] = aS * sigma / gap(wS, s);
-            cq[1] = aS * sigma / gap(wS, s - 1);
-            ring[0] = ring[1] = aS * 2.0 * sigma / wS[s];
-        }
-        cq[2] = aM * sigma / gap(wM, m);
-        cq[3] = aM * sigma / gap(wM, m - 1);
-        cq[4] = aF * sigma / gap(wF, f);
-        cq[5] = aF * sigma / gap(wF, f - 1);
-        ring[2] = ring[3] = aM * 2.0 * sigma / wM[m];
-        ring[4] = ring[5] = aF * 2.0 * sigma / wF[f];
-        const int ds[6] = {1, -1, 0, 0, 0, 0}, dm[6] = {0, 0, 1, -1, 0, 0}, df[6] = {0, 0, 0, 0, 1, -1};
-        double diag = 0.0;
-        for (int q = 0; q < 6; q++)
-        {
-            if (q < 2 && !L.threeD) continue;
-            const int S = s + ds[q], Mm = m + dm[q], F = f + df[q];
-            const bool onRing = (L.threeD && (S == 0 || S == ns - 1)) || Mm == 0 || Mm == nm - 1 || F == 0 || F == nf - 1;
-            const uint8_t t = L.types[at(S, Mm, F)];
-            if (t == kFluid)
-            {
-                diag += cq[q];
-                const int col = id[at(S, Mm, F)];
-                if (!onRing && col >= 0) B[(size_t)r * n + col] -= cq[q];
-                else touchesDirichlet[r] = 1;
-            }
-            else if (is_dirichlet(t))
-            {
-                diag += onRing ? ring[q] : cq[q];
-                touchesDirichlet[r] = 1;
-            }
-        }
-        B[(size_t)r * n + r] += diag;
-    }
-    // Components of the coupling graph; a component with no Dirichlet face has the constant as its null mode.
-    std::vector<int> comp(n, -1);
-    std::vector<std::vector<int>> members;
-    std::vector<char> singular;
-    for (int r = 0; r < n; r++)
-    {
-        if (comp[r] >= 0) continue;
-        const int cid = (int)members.size();
-        members.emplace_back();
-        singular.push_back(1);
-        std::queue<int> q;
-        q.push(r);
-        comp[r] = cid;
-        while (!q.empty())
-        {
-            const int u = q.front();
-            q.pop();
-            members[cid].push_back(u);
-            if (touchesDirichlet[u]) singular[cid] = 0;
-            for (int v = 0; v < n; v++)
-                if (v != u && B[(size_t)u * n + v] != 0.0 && comp[v] < 0)
-                {
-                    comp[v] = cid;
-                    q.push(v);
-                }
-        }
-    }
-    double dmean = 0.0;
-    for (int r = 0; r < n; r++) dmean += B[(size_t)r * n + r];
-    dmean = n > 0 ? dmean / n : 1.0;
-    if (!(dmean > 0.0)) dmean = 1.0;
-    for (size_t cid = 0; cid < members.size(); cid++)
-    {
-        if (!singular[cid]) continue;
-        const double v = dmean / (double)members[cid].size();
-        for (int a : members[cid])
-            for (int b : members[cid]) B[(size_t)a * n + b] += v;
-    }
-    // Cholesky B = L L^T, then B^-1 column by column.
-    std::vector<double> Lc(B);
-    for (int j = 0; j < n; j++)
-    {
-        double d = Lc[(size_t)j * n + j];
-        for (int k = 0; k < j; k++) d -= Lc[(size_t)j * n + k] * Lc[(size_t)j * n + k];
-        if (!(d > 0.0))
-        {
-            err = "coarsest operator is not positive definite after the null-mode shift";
-            return false;
-        }
-        d = std::sqrt(d);
-        Lc[(size_t)j * n + j] = d;
-        for (int i = j + 1; i < n; i++)
-        {
-            double v = Lc[(size_t)i * n + j];
-            for (int k = 0; k < j; k++) v -= Lc[(size_t)i * n + k] * Lc[(size_t)j * n + k];
-            Lc[(size_t)i * n + j] = v / d;
-        }
-    }
-    M.assign((size_t)n * n, 0.0);
-    std::vector<double> y(n);
-    for (int col = 0; col < n; col++)
-    {
-        for (int i = 0; i < n; i++)
-        {
-            double v = i == col ? 1.0 : 0.0;
-            for (int k = 0; k < i; k++) v -= Lc[(size_t)i * n + k] * y[k];
-            y[i] = v / Lc[(size_t)i * n + i];
-        }
-        for (int i = n - 1; i >= 0; i--)
-        {
-            double v = y[i];
-            for (int k = i + 1; k < n; k++) v -= Lc[(size_t)k * n + i] * y[k];
-            y[i] = v / Lc[(size_t)i * n + i];
-        }
-        for (int i = 0; i < n; i++) M[(size_t)i * n + col] = y[i];
-    }
-    // A^+ = -(B^-1 - sum_c 1_c 1_c^T / (dmean |c|)): remove the shift on each singular component, flip the sign.
-    for (size_t cid = 0; cid < members.size(); cid++)
-    {
-        if (!singular[cid]) continue;
-        const double shift = 1.0 / dmean / (double)members[cid].size();
-        for (int a : members[cid])
-            for (int b : members[cid]) M[(size_t)a * n + b] -= shift;
-    }
-    for (double &v : M) v = -v;
-    return true;
 }
 
 }  // namespace nppe

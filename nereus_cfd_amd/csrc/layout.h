@@ -16,8 +16,12 @@ namespace nppe
 // included; the pitch padding is not written.
 hipError_t launch_coarsen_types(hipStream_t s, const Box &Bf, const uint8_t *tf, const Box &Bc, uint8_t *tc);
 
-// code8 / code16 of every interior cell of a level from its types (hierarchy.h stencil_codes; zero for a cell that
-// is not an unknown).
+// Stencil codes of every interior cell of a level from its types (zero for a cell that is not an unknown). code8:
+// bits 0-5 = neighbour q (slow+, slow-, mid+, mid-, fast+, fast-) is a coupled unknown; bit 6 = the cell is an
+// unknown; bit 7 = the cell has a Dirichlet neighbour (read code16 for it). code16: two bits per neighbour q at bit
+// 2q -- 0 Neumann, 1 coupled unknown, 2 Dirichlet with the face coefficient (an interior EMPTY/PRESSURE cell, or a
+// FLUID ring cell), 3 Dirichlet ring cell (EMPTY/PRESSURE on the ring: coefficient times FaceGap/(Width/2)); bit 12 =
+// the cell is an unknown. Built at commit and after every layout update.
 hipError_t launch_stencil_codes(hipStream_t s, const Box &B, const uint8_t *t, uint8_t *c8, uint16_t *c16);
 
 // counts[0] += unknowns of the level (code8 bit 6), counts[1] += unknowns with a Dirichlet face (bit 7), with

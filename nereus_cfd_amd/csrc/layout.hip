@@ -1,8 +1,9 @@
 // Layout kernels for gfx950: a new cell layout of the committed box (FLIP's moving liquid) made into the multigrid
 // hierarchy on the device -- every coarse level's types by the 2:1 coarsening rule, every level's stencil codes,
 // the unknown counts that decide whether the hierarchy keeps its shape, and the fine level's IDP numbering. The
-// rules are those of the host construction (hierarchy.cpp coarsen, stencil_codes; the numbering of
-// PressureSolver3D.cpp:133-139), so a handle updated here solves exactly as one committed with the new layout.
+// coarsening rule is the host construction's (hierarchy.cpp coarsen), the codes and the numbering
+// (PressureSolver3D.cpp:133-139) are built by these same kernels at commit, so a handle updated here solves exactly
+// as one committed with the new layout.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -41,8 +41,8 @@ struct Tables32
     const float *hS, *hM, *hF;
 };
 
-// One FP32 multigrid level as the kernels see it: box, tables, stencil codes (see stencil_codes in
-// hierarchy.h) and, when every width of the level is equal, the constant face coefficient cu and ring-Dirichlet
+// One FP32 multigrid level as the kernels see it: box, tables, stencil codes (see launch_stencil_codes in
+// layout.h) and, when every width of the level is equal, the constant face coefficient cu and ring-Dirichlet
 // coefficient cr = 2 cu that replace the tables.
 struct Level32
 {

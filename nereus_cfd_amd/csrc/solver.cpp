@@ -85,6 +85,13 @@ void free_grid(nppe_solver *h)
     h->dCompB = h->dCompX = h->dCompG = nullptr;
     h->dCoarseM = nullptr;
     h->dCoarseOrder = nullptr;
+    h->dCoarseWork = nullptr;
+    h->dCoarseId = nullptr;
+    h->dCoarseW = nullptr;
+    h->dCoarseLab = nullptr;
+    h->dCoarseAux = nullptr;
+    h->dCoarseShift = nullptr;
+    h->dCoarseState = nullptr;
     h->dPartials = nullptr;
     h->dLayoutStage = nullptr;
     h->dLayoutInfo = nullptr;
@@ -508,33 +515,49 @@ hipError_t choose_tiles(nppe_solver *h)
 }
 
 // Unknown numbering and singularity (PressureSolver3D.cpp:133-172): interior FLUID cells in index order;
-// singular when no FLUID cell has a Dirichlet face (EMPTY/PRESSURE, or a FLUID ring cell).
+// singular when no FLUID cell has a Dirichlet face (EMPTY/PRESSURE, or a FLUID ring cell). The Dirichlet test looks
+// from the (usually few) Dirichlet cells at their interior FLUID neighbours instead of from every unknown.
 void number_unknowns(nppe_solver *h, const HostLevel &fine)
 {
     h->orderHost.clear();
+    const int ns = fine.ns, nm = fine.nm, nf = fine.nf;
+    auto at = [&](int s, int m, int f) { return ((int64_t)s * nm + m) * nf + f; };
+    const int s0 = fine.threeD ? 1 : 0, s1 = fine.threeD ? ns - 2 : 0;
+    h->orderHost.reserve((size_t)fine.unknowns);
+    for (int s = s0; s <= s1; s++)
+        for (int m = 1; m <= nm - 2; m++)
+        {
+            const uint8_t *row = &fine.types[(size_t)at(s, m, 0)];
+            for (int f = 1; f <= nf - 2; f++)
+                if (row[f] == 0) h->orderHost.push_back((int32_t)at(s, m, f));
+        }
     bool dirichlet = false;
-    {
-        const int ns = fine.ns, nm = fine.nm, nf = fine.nf;
-        auto at = [&](int s, int m, int f) { return ((int64_t)s * nm + m) * nf + f; };
-        const int s0 = fine.threeD ? 1 : 0, s1 = fine.threeD ? ns - 2 : 0;
-        for (int s = s0; s <= s1; s++)
-            for (int m = 1; m <= nm - 2; m++)
-                for (int f = 1; f <= nf - 2; f++)
+    const int sLo = fine.threeD ? 0 : 0, sHi = fine.threeD ? ns - 1 : 0;
+    for (int s = sLo; s <= sHi && !dirichlet; s++)
+        for (int m = 0; m < nm && !dirichlet; m++)
+        {
+            const uint8_t *row = &fine.types[(size_t)at(s, m, 0)];
+            const bool ringRow = (fine.threeD && (s == 0 || s == ns - 1)) || m == 0 || m == nm - 1;
+            for (int f = 0; f < nf; f++)
+            {
+                const uint8_t t = row[f];
+                const bool ring = ringRow || f == 0 || f == nf - 1;
+                if (!(t == 2 || t == 4 || (t == 0 && ring))) continue;
+                // an interior FLUID neighbour makes this a Dirichlet face of an unknown
+                const int ds[6] = {1, -1, 0, 0, 0, 0}, dm[6] = {0, 0, 1, -1, 0, 0}, df[6] = {0, 0, 0, 0, 1, -1};
+                for (int q = fine.threeD ? 0 : 2; q < 6; q++)
                 {
-                    const int64_t c = at(s, m, f);
-                    if (fine.types[c] != 0) continue;
-                    h->orderHost.push_back((int32_t)c);
-                    const int ds[6] = {1, -1, 0, 0, 0, 0}, dm[6] = {0, 0, 1, -1, 0, 0}, df[6] = {0, 0, 0, 0, 1, -1};
-                    for (int q = fine.threeD ? 0 : 2; q < 6; q++)
+                    const int S = s + ds[q], M = m + dm[q], F = f + df[q];
+                    if ((fine.threeD && (S < 1 || S > ns - 2)) || M < 1 || M > nm - 2 || F < 1 || F > nf - 2) continue;
+                    if (fine.types[(size_t)at(S, M, F)] == 0)
                     {
-                        const int S = s + ds[q], M = m + dm[q], F = f + df[q];
-                        const uint8_t t = fine.types[at(S, M, F)];
-                        const bool ring = (fine.threeD && (S == 0 || S == ns - 1)) || M == 0 || M == nm - 1 ||
-                                          F == 0 || F == nf - 1;
-                        if (t == 2 || t == 4 || (t == 0 && ring)) dirichlet = true;
+                        dirichlet = true;
+                        break;
                     }
                 }
-    }
+                if (dirichlet) break;
+            }
+        }
     h->n = (int64_t)h->orderHost.size();
     h->singular = !dirichlet;
 }
@@ -555,29 +578,19 @@ std::vector<HostLevel> coarsen_chain(HostLevel fine)
     return hl;
 }
 
-// A level's cell types and stencil codes (pitched) into its device buffers.
+// A level's cell types (pitched) into its device buffer, and its stencil codes built from them on the device (the
+// layout updates build them the same way).
 int upload_level_layout(nppe_solver *h, DevLevel &L)
 {
     const HostLevel &H = L.host;
+    // one contiguous upload of the pitched types (a strided copy from pageable memory moves row by row)
     std::vector<uint8_t> pitched((size_t)L.box.cells, 1);
     for (int s = 0; s < H.ns; s++)
         for (int m = 0; m < H.nm; m++)
             std::memcpy(&pitched[(size_t)L.box.at(s, m, 0)], &H.types[((size_t)s * H.nm + m) * H.nf], H.nf);
     HIP_TRY(h, upload(h, L.types, pitched.data(), pitched.size()), "upload types");
-    std::vector<uint8_t> c8h;
-    std::vector<uint16_t> c16h;
-    stencil_codes(H, c8h, c16h);
-    std::vector<uint8_t> p8((size_t)L.box.cells, 0);
-    std::vector<uint16_t> p16((size_t)L.box.cells, 0);
-    for (int s = 0; s < H.ns; s++)
-        for (int m = 0; m < H.nm; m++)
-        {
-            const size_t src = ((size_t)s * H.nm + m) * H.nf, dst = (size_t)L.box.at(s, m, 0);
-            std::memcpy(&p8[dst], &c8h[src], H.nf);
-            std::memcpy(&p16[dst], &c16h[src], H.nf * sizeof(uint16_t));
-        }
-    HIP_TRY(h, upload(h, L.c8, p8.data(), p8.size()), "upload codes");
-    HIP_TRY(h, upload(h, L.c16, p16.data(), p16.size() * sizeof(uint16_t)), "upload codes");
+    HIP_TRY(h, launch_stencil_codes(h->stream, L.box, L.types, L.c8, L.c16), "stencil codes");
+    HIP_TRY(h, hipStreamSynchronize(h->stream), "stencil codes");
     return NPPE_OK;
 }
 
@@ -586,52 +599,38 @@ int64_t interior_cells(const HostLevel &F)
     return (F.threeD ? (int64_t)(F.ns - 2) : 1) * (F.nm - 2) * (F.nf - 2);
 }
 
-// The IDP order as pitched box indices.
-int upload_order(nppe_solver *h)
+// The coarsest level's dense pseudo-inverse and unknown order, built on the device, when it has at most
+// kMaxDenseCoarse unknowns (else the coarsest level is smoothed instead).
+int upload_coarse(nppe_solver *h, int64_t unknowns)
 {
-    const DevLevel &F = h->levels[0];
-    std::vector<int32_t> pitched(h->orderHost.size());
-    for (size_t q = 0; q < pitched.size(); q++)
-    {
-        const int64_t c = h->orderHost[q];
-        const int f = (int)(c % F.host.nf), m = (int)((c / F.host.nf) % F.host.nm),
-                  s = (int)(c / ((int64_t)F.host.nf * F.host.nm));
-        pitched[q] = (int32_t)F.box.at(s, m, f);
-    }
-    if (!pitched.empty())
-        HIP_TRY(h, upload(h, h->dOrder, pitched.data(), pitched.size() * sizeof(int32_t)), "upload order");
-    return NPPE_OK;
-}
-
-// The coarsest level's dense pseudo-inverse and unknown order, when it has at most kMaxDenseCoarse unknowns.
-int upload_coarse(nppe_solver *h)
-{
-    const DevLevel &C = h->levels.back();
     h->coarseDense = false;
     h->nCoarse = 0;
-    if (C.host.unknowns > kMaxDenseCoarse) return NPPE_OK;
-    std::vector<int64_t> ord;
-    std::vector<double> M;
-    std::string e2;
-    if (!coarse_pseudo_inverse(C.host, h->sigma, ord, M, e2)) return fail(h, NPPE_ERR_INTERNAL, e2);
-    h->nCoarse = (int)ord.size();
-    std::vector<int32_t> pitched(ord.size());
-    for (size_t q = 0; q < ord.size(); q++)
-    {
-        const int64_t c = ord[q];
-        const int f = (int)(c % C.host.nf), m = (int)((c / C.host.nf) % C.host.nm),
-                  s = (int)(c / ((int64_t)C.host.nf * C.host.nm));
-        pitched[q] = (int32_t)C.box.at(s, m, f);
-    }
-    HIP_TRY(h, upload(h, h->dCoarseM, M.data(), M.size() * sizeof(double)), "upload coarse matrix");
-    HIP_TRY(h, upload(h, h->dCoarseOrder, pitched.data(), pitched.size() * sizeof(int32_t)), "upload coarse order");
+    if (unknowns > kMaxDenseCoarse || !h->dCoarseM) return NPPE_OK;
+    if (int rc = coarse_pinv_device(h, (int)unknowns)) return rc;
     h->coarseDense = true;
     return NPPE_OK;
 }
 
+// NPPE_COMMIT_TIMES=1: the commit's phases on stderr (where a commit spends its time).
+struct PhaseTimer
+{
+    bool on = std::getenv("NPPE_COMMIT_TIMES") && std::atoi(std::getenv("NPPE_COMMIT_TIMES")) != 0;
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    void mark(nppe_solver *h, const char *what)
+    {
+        if (!on) return;
+        (void)hipStreamSynchronize(h->stream);
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[commit] %-22s %8.2f ms\n", what, std::chrono::duration<double, std::milli>(now - t).count());
+        t = now;
+    }
+};
+
 int build(nppe_solver *h, const nppe_grid *g)
 {
+    PhaseTimer pt;
     free_grid(h);
+    pt.mark(h, "free");
     const int dim = g->dim;
     const int nz = dim == 2 ? 1 : g->nz;
     h->dim = dim;
@@ -643,7 +642,9 @@ int build(nppe_solver *h, const nppe_grid *g)
     HostLevel fine = fine_level(dim, g->nx, g->ny, nz, g->cell_types, g->width_x, g->width_y,
                                 dim == 3 ? g->width_z : nullptr);
 
+    pt.mark(h, "fine level");
     number_unknowns(h, fine);
+    pt.mark(h, "numbering");
 
     // Scale factors: the reference's MatrixScale (1/dh^2 uniform, 1 stretched); sigma turns the volume-form
     // coefficients area/gap into A-units (3D uniform: c = 1 * scale while area/gap = dh).
@@ -652,6 +653,7 @@ int build(nppe_solver *h, const nppe_grid *g)
     const double sigma = unitCoef ? scale / g->dh : scale;
 
     std::vector<HostLevel> hl = coarsen_chain(std::move(fine));
+    pt.mark(h, "coarsening");
     h->coarseNu = std::getenv("NPPE_COARSE_NU") ? std::atoi(std::getenv("NPPE_COARSE_NU")) : 0;
     h->useGraphs = !(std::getenv("NPPE_GRAPHS") && std::atoi(std::getenv("NPPE_GRAPHS")) == 0);
     h->levels.resize(hl.size());
@@ -692,15 +694,29 @@ int build(nppe_solver *h, const nppe_grid *g)
             HIP_TRY(h, dalloc_pairs(h, &L.b, (size_t)L.box.cells), "allocate level b");
         }
     }
+    pt.mark(h, "levels (codes, tables)");
     // Coarsest: dense pseudo-inverse when small enough (buffers for the largest such level, so a layout update
     // reuses them).
     h->sigma = sigma;
     if (h->levels.back().host.unknowns <= kMaxDenseCoarse)
     {
-        HIP_TRY(h, dalloc(h, &h->dCoarseM, (size_t)kMaxDenseCoarse * kMaxDenseCoarse), "allocate coarse matrix");
+        const DevLevel &C = h->levels.back();
+        const size_t nn = (size_t)kMaxDenseCoarse * kMaxDenseCoarse;
+        HIP_TRY(h, dalloc(h, &h->dCoarseM, nn), "allocate coarse matrix");
+        HIP_TRY(h, dalloc(h, &h->dCoarseWork, nn), "allocate coarse matrix");
         HIP_TRY(h, dalloc(h, &h->dCoarseOrder, (size_t)kMaxDenseCoarse), "allocate coarse order");
+        HIP_TRY(h, dalloc(h, &h->dCoarseId, (size_t)C.box.cells), "allocate coarse numbering");
+        HIP_TRY(h, dalloc(h, &h->dCoarseLab, (size_t)kMaxDenseCoarse), "allocate coarse labels");
+        HIP_TRY(h, dalloc(h, &h->dCoarseAux, (size_t)3 * kMaxDenseCoarse), "allocate coarse labels");
+        HIP_TRY(h, dalloc(h, &h->dCoarseShift, (size_t)kMaxDenseCoarse), "allocate coarse shifts");
+        HIP_TRY(h, dalloc(h, &h->dCoarseState, 4), "allocate coarse state");
+        std::vector<double> w;
+        for (int a = 0; a < 3; a++) w.insert(w.end(), C.host.w[a].begin(), C.host.w[a].end());
+        HIP_TRY(h, dalloc(h, &h->dCoarseW, w.size()), "allocate coarse widths");
+        HIP_TRY(h, upload(h, h->dCoarseW, w.data(), w.size() * sizeof(double)), "upload coarse widths");
     }
-    if (int rc = upload_coarse(h)) return rc;
+    if (int rc = upload_coarse(h, h->levels.back().host.unknowns)) return rc;
+    pt.mark(h, "coarse pseudo-inverse");
     // Fine FP64 data.
     const DevLevel &F = h->levels[0];
     {
@@ -715,7 +731,6 @@ int build(nppe_solver *h, const nppe_grid *g)
     // every vector in IDP order is sized for the most unknowns the box can hold, so a layout update reuses them
     h->unknownCap = std::max<int64_t>(1, interior_cells(F.host));
     HIP_TRY(h, dalloc(h, &h->dOrder, (size_t)h->unknownCap), "allocate order");
-    if (int rc = upload_order(h)) return rc;
     const size_t cells = (size_t)F.box.cells;
     // (odd columns pair-aligned, like the FP32 vectors: the whole-row FP64 passes move x and r in pairs)
     HIP_TRY(h, dalloc_pairs(h, &h->dX64[0], cells), "allocate x64");
@@ -776,8 +791,14 @@ int build(nppe_solver *h, const nppe_grid *g)
     }
     HIP_TRY(h, dalloc(h, &h->dState, 1), "allocate state");
     HIP_TRY(h, hipStreamSynchronize(h->stream), "commit");
+    pt.mark(h, "vectors, order");
     HIP_TRY(h, choose_tiles(h), "smoother tiles");
+    pt.mark(h, "tile timing");
     if (int rc = build_fields(h)) return rc;
+    // the IDP order on the device from the row starts (the layout updates number the unknowns the same way)
+    HIP_TRY(h, launch_write_order(h->stream, h->fg, h->dOrder), "order");
+    HIP_TRY(h, hipStreamSynchronize(h->stream), "order");
+    pt.mark(h, "field data, order");
     h->committed = true;
     return NPPE_OK;
 }
@@ -921,16 +942,8 @@ int layout_from_device(nppe_solver *h, const uint8_t *dTypes, const uint8_t *hos
     h->fg.n = h->n;
     for (int l = 0; l < L; l++) h->levels[l].host.unknowns = info[2 * l];
     HIP_TRY(h, launch_write_order(h->stream, h->fg, h->dOrder), "layout: order");
-    if (info[2 * (L - 1)] <= kMaxDenseCoarse)
-    {
-        DevLevel &C = h->levels.back();
-        HostLevel &H = C.host;
-        HIP_TRY(h, hipMemcpy2DAsync(H.types.data(), (size_t)H.nf, C.types, (size_t)C.box.pitch, (size_t)H.nf,
-                                    (size_t)H.ns * H.nm, hipMemcpyDeviceToHost, h->stream),
-                "layout: coarsest types");
-        HIP_TRY(h, hipStreamSynchronize(h->stream), "layout: coarsest types");
-    }
-    if (int rc = upload_coarse(h)) return rc;
+    // the coarsest pseudo-inverse from the level's types on the device (nothing read back)
+    if (int rc = upload_coarse(h, info[2 * (L - 1)])) return rc;
     if (int rc = zero_work(h)) return rc;
     HIP_TRY(h, hipStreamSynchronize(h->stream), "layout update");
     h->hostStale = true;
@@ -1978,7 +1991,8 @@ uint64_t nppe_estimate_bytes_for(const nppe_grid *g, const nppe_options *opt)
         nm = (mi + 1) / 2 + 2;
         nf = (fi + 1) / 2 + 2;
     }
-    total += (double)kMaxDenseCoarse * kMaxDenseCoarse * 8.0 + 1.0e6;   // coarse matrix, partials, state
+    // the coarse matrix and its Gauss-Jordan partner, partials, state and the smaller buffers
+    total += 2.0 * kMaxDenseCoarse * kMaxDenseCoarse * 8.0 + 2.0e6;
     return (uint64_t)(total * 1.05);
 }
 

@@ -9,6 +9,7 @@
 
 #include "handle.h"
 #include "kernels.h"
+#include "layout.h"
 
 using namespace nppe;
 
@@ -261,22 +262,13 @@ int nppe::build_fields(nppe_solver *h)
 
 int nppe::refresh_fields(nppe_solver *h)
 {
-    // unknowns before each interior row, rows in IDP order (i, then j in 3D)
-    const int Nx = h->Nx, Ny = h->Ny, Nz = h->Nz, dim = h->dim;
-    const HostLevel &H = h->levels[0].host;
-    const int rows = h->fg.rows;
-    std::vector<int32_t> start((size_t)std::max(rows, 1), 0);
-    int64_t count = 0;
-    const int nfast = dim == 3 ? Nz : Nx;
-    for (int r = 0; r < rows; r++)
-    {
-        start[r] = (int32_t)count;
-        const int i = dim == 3 ? 1 + r / (Nx - 2) : 1 + r, j = dim == 3 ? 1 + r % (Nx - 2) : 0;
-        const size_t row = dim == 3 ? ((size_t)i * Nx + j) * Nz : (size_t)i * Nx;
-        for (int f = 1; f <= nfast - 2; f++) count += H.types[row + f] == 0 ? 1 : 0;
-    }
+    // unknowns before each interior row (rows in IDP order: i, then j in 3D), counted and scanned on the device
+    long long *total = h->dLayoutInfo;
+    HIP_TRY(h, launch_number_rows(h->stream, h->fg, h->dRowStart, total), "row starts");
+    long long count = 0;
+    HIP_TRY(h, hipMemcpyAsync(&count, total, sizeof(count), hipMemcpyDeviceToHost, h->stream), "row starts");
+    HIP_TRY(h, hipStreamSynchronize(h->stream), "row starts");
     if (count != h->n) return fail(h, NPPE_ERR_INTERNAL, "row starts disagree with the unknown count");
-    HIP_TRY(h, upload(h, h->dRowStart, start.data(), start.size() * sizeof(int32_t)), "upload row starts");
     h->fg.n = h->n;
     return NPPE_OK;
 }
