@@ -436,7 +436,9 @@ def layout_leg(args):
 
 
 def batched_leg(args):
-    """32 FLIP-like free-surface systems of sizes U{64..160}^3, solved concurrently to 1e-8."""
+    """32 FLIP-like free-surface systems of sizes U{64..160}^3, solved together to 1e-8: nppe_batch_solve with device
+    vectors, which shares every launch of the inner iteration between the systems (one per V-cycle place and kernel,
+    replayed as one graph)."""
     import torch
     import nereus_cfd_amd as N
     import nereus_cfd_amd.workloads as W
@@ -460,7 +462,11 @@ def batched_leg(args):
         if rc != 0:
             raise SystemExit(f"batched solve failed: {rc}")
     el = time.perf_counter() - t0
-    # the same systems one after another on one stream, for comparison
+    # the same systems one after another, each on its own handle's stream, for comparison (one untimed pass first:
+    # a handle solving alone captures its own iteration graphs at its first solve)
+    for h, b, x in zip(solvers, bp, xp):
+        h.solve_device(b, x)
+    torch.cuda.synchronize()
     t1 = time.perf_counter()
     for h, b, x in zip(solvers, bp, xp):
         h.solve_device(b, x)

@@ -354,11 +354,16 @@ int nppe_projection_step_ex_device(nppe_solver *h, double *d_u, double *d_v, dou
 
 /* ---- many systems at once (no counterpart in the reference, which solves one system per process) ------------- */
 
-/* Solves count independent committed systems concurrently: each handle runs on its own stream, driven by
- * `workers` host threads (0: one per system, at most 64). b[i], x[i] are host (device == 0) or device
- * (device == 1) FP64 vectors in IDP order of hs[i]; results go to res[i] (may be NULL). Returns NPPE_OK when every
- * solve returned NPPE_OK, the first negative code otherwise, or NPPE_NOT_CONVERGED. Each result is bit-identical to
- * the same solve run alone. */
+/* Solves count independent committed systems together. b[i], x[i] are host (device == 0) or device (device == 1)
+ * FP64 vectors in IDP order of hs[i]; results go to res[i] (may be NULL). With device vectors and every system an
+ * MGPCG system with the FP64 Krylov recursion whose V-cycle runs the whole-row kernels (3D, unit coefficients, rows
+ * of at most 256 cells), the inner iterations are batched: one launch per V-cycle level and kernel covers all
+ * systems, the batch's iteration replays as one graph, all on hs[0]'s stream, and res[i].device_ms is the whole
+ * batch's time. The first call on a set of handles (or after one of them changed its grid or layout) records and
+ * merges the launches and times the tile shapes of the shared sweeps and restrictions (tens of milliseconds); later
+ * calls reuse that plan. Otherwise each handle runs on its own stream, driven by `workers` host threads (0: one per system,
+ * at most 64). Returns NPPE_OK when every solve returned NPPE_OK, the first negative code otherwise, or
+ * NPPE_NOT_CONVERGED. Each result is bit-identical to the same solve run alone. */
 int nppe_batch_solve(nppe_solver *const *hs, const double *const *b, double *const *x, int32_t count,
                      int32_t workers, int32_t device, nppe_result *res);
 

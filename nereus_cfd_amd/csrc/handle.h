@@ -19,6 +19,8 @@
 namespace nppe
 {
 
+struct BatchPlan;
+
 // One multigrid level on the device.
 struct DevLevel
 {
@@ -98,6 +100,9 @@ struct nppe_solver
     hipGraphExec_t chunkGraph = nullptr;   // chunkGraphIters inner iterations back to back (a solve's first chunk)
     int chunkGraphIters = 0;
     double graphBytesPerIter = 0.0;
+    uint64_t graphEpoch = 0;              // advanced whenever the captured iterations are dropped (drop_graphs)
+    uint64_t serial = 0;                  // unique per handle of the process (a batched plan's key)
+    nppe::BatchPlan *batch = nullptr;     // the batched solve's plan last built with this handle first (batch.cpp)
     double rate = 0.0;   // observed per-iteration reduction of the inner residual norm (0: none yet)
     int lastFirstPass = 0;   // iterations of the last FP64 Krylov solve's first pass (the next one's prediction)
     std::vector<void *> allocations;
@@ -212,5 +217,13 @@ int build_fields(nppe_solver *h);
 int refresh_fields(nppe_solver *h);
 // The solve on device vectors (solver.cpp).
 int solve_device(nppe_solver *h, const double *dB, double *dX, const double *dGuess, double refNorm, nppe_result *res);
+// MGPCG (FP64 Krylov) solves of several committed systems from no guess with their inner iterations batched: one
+// launch per place of the iteration and kernel covers every system, each getting the bits of its solve alone
+// (solver.cpp). Returns 1, having done nothing, when the systems do not all qualify (then solve them one by one);
+// otherwise NPPE_OK with each system's status in rcs[q] and its result in res[q] (device_ms: the whole batch's), or a
+// negative error that ended the batch.
+int batch_solve_krylov64(nppe_solver *const *hs, const double *const *b, double *const *x, int count,
+                         nppe_result *res, int *rcs);
+void free_batch_plan(BatchPlan *p);
 
 }  // namespace nppe

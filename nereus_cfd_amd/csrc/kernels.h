@@ -11,6 +11,45 @@
 namespace nppe
 {
 
+// ---- batched launches -----------------------------------------------------------------------------------------
+// While a LaunchRecorder is set on the calling thread, the launches of the kernels that have a batched form (the
+// whole-row sweep, the whole-row and marching restrictions, the coarsest solve, the row sum of <r, z>, the whole-row
+// FP64 Krylov passes and the separate finalisation) append a record -- the batched kernel, the system's grid and block, its arguments --
+// instead of launching. A batched solve concatenates one record of each of several systems into one launch
+// (launch_batched), whose block g - start[q] of system q does what that block does in the system's own launch.
+// Kinds of records: a sweep or a restriction gives the same bits for any tiling, so a batch may re-tile it
+// (retile_record); the other kernels' reductions follow their grid, which stays as recorded.
+enum { REC_FIXED = 0, REC_ROW_SWEEP = 1, REC_RESTRICT_ROW = 2, REC_RESTRICT_MARCH = 3 };
+struct LaunchRecord
+{
+    const void *fn = nullptr;   // the batched kernel: kernel(const Args *args, const int *start, int nsys)
+    int grid = 0, threads = 0;
+    int phase = 0, level = 0, seq = 0;   // where in the inner iteration (record_slot), then the order there
+    int kind = REC_FIXED;
+    int tm = 0, k = 0, flags = 0;        // whole-row kernels: tile rows, 64-lane steps per row, variant bits
+    std::vector<unsigned char> args;     // the system's Args
+};
+struct LaunchRecorder
+{
+    int phase = 0, level = 0, seq = 0;
+    std::vector<LaunchRecord> recs;
+};
+// Places of the inner iteration, in the order it runs them: the V-cycle's way down (level ascending), the coarsest
+// solve, the way up (level descending), the Krylov update.
+enum { SLOT_DOWN = 0, SLOT_COARSE = 1, SLOT_UP = 2, SLOT_KRYLOV = 3 };
+// The record re-tiled: tm tile rows (whole-row sweeps; 0 keeps them) and p planes per chunk (made even for the
+// restrictions, which sum plane pairs). False, and the record unchanged, for a fixed record or a tile shape the
+// kernel has no form for.
+bool retile_record(LaunchRecord &r, int tm, int p);
+// Tile rows a batch may give the whole-row sweep record r.
+void retile_rows(const LaunchRecord &r, std::vector<int> &tms);
+void set_launch_recorder(LaunchRecorder *r);   // null: launch again
+void record_slot(int phase, int level);        // the place of the next records (nothing without a recorder)
+// args: nsys records' Args back to back (the kernel's Args type); start: nsys + 1 block offsets, start[0] = 0,
+// start[nsys] = grid, each system at least one block.
+hipError_t launch_batched(hipStream_t s, const void *fn, int grid, int threads, const void *args, const int *start,
+                          int nsys);
+
 // ---- FP64 fine level ------------------------------------------------------------------------------------------
 // box[order[q]] = src[q] (0 when src is null) for the n unknowns; with sumSlot >= 0 also sums src^2 into
 // st->sums[sumSlot].

@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <cstring>
 
 #include "kernels.h"
 #include "op64.h"
@@ -399,13 +400,19 @@ __global__ __launch_bounds__(1024) void k_finalize(SolveState *st, double *parti
 // sums the rows [q seg, (q + 1) seg) (thread t the entries t, t + 256, ... of them), and the last block to finish
 // sums the block sums in index order and applies FIN_RZ (block_sum's fused finalisation). The order depends on the
 // row count only, never on the sweep's tiles.
+__device__ __forceinline__ void rz_rows_sum_body(SolveState *st, const double *__restrict__ rows, int nrows, int seg,
+                                                 double *blockSums, int blk, int nblk)
+{
+    const int e = min(nrows, (blk + 1) * seg);
+    double v[1] = {0.0};
+    for (int q = blk * seg + (int)threadIdx.x; q < e; q += blockDim.x) v[0] += rows[q];
+    block_sum<1>(v, Red{blockSums, st, (int)FIN_RZ, 1}, nblk, blk);
+}
+
 __global__ __launch_bounds__(kBlockThreads) void k_rz_rows_sum(SolveState *st, const double *__restrict__ rows,
                                                                int nrows, int seg, double *blockSums)
 {
-    const int blk = blockIdx.x, e = min(nrows, (blk + 1) * seg);
-    double v[1] = {0.0};
-    for (int q = blk * seg + (int)threadIdx.x; q < e; q += blockDim.x) v[0] += rows[q];
-    block_sum<1>(v, Red{blockSums, st, (int)FIN_RZ, 1}, gridDim.x, blk);
+    rz_rows_sum_body(st, rows, nrows, seg, blockSums, blockIdx.x, gridDim.x);
 }
 
 // One block for a separate finalisation: 256 threads, or 1024 for many partials.
@@ -1177,10 +1184,10 @@ struct Prolong
 // dot[(plane - 1) * (nm - 2) + row - 1] (the CG's <r, z> when the sweep is the last one of the V-cycle on the
 // finest level): per row, not per block, so the sum over them does not depend on the tile shape.
 template <int TM, int K, bool ZERO, bool PRO, bool NU>
-__global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const float *__restrict__ xin,
-                                                            float *__restrict__ xout, const float *__restrict__ b,
-                                                            int c1, int P, int tilesM, const SolveState *st,
-                                                            double *__restrict__ dot, Prolong pro)
+__device__ __forceinline__ void rbgs_row_body(const Level32 &L, const float *__restrict__ xin,
+                                              float *__restrict__ xout, const float *__restrict__ b, int c1, int P,
+                                              int tilesM, const SolveState *st, double *__restrict__ dot,
+                                              const Prolong &pro, int lb)
 {
     if (st->done) return;
     const Box &B = L.B;
@@ -1191,7 +1198,6 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const flo
     __shared__ float2 tab[128];                  // by code & 127: (1/n, all ones) for an unknown, (0, 0) otherwise
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int lb = logical_block();
     const int tm = lb % tilesM, chunk = lb / tilesM;
     const int m0 = 1 + tm * TM;
     const int m = m0 - 1 + wave;
@@ -1607,16 +1613,15 @@ __global__ __launch_bounds__(kBlockThreads) void k_prolong(Level32 Lf, float *__
 }
 
 // Coarsest level: dense FP64 pseudo-inverse times the gathered right-hand side. One wavefront per row.
-__global__ __launch_bounds__(kBlockThreads) void k_coarse_solve(int n, const int32_t *__restrict__ order,
-                                                                const double *__restrict__ M,
-                                                                const float *__restrict__ b, float *__restrict__ x,
-                                                                const SolveState *st)
+__device__ __forceinline__ void coarse_solve_body(int n, const int32_t *__restrict__ order,
+                                                  const double *__restrict__ M, const float *__restrict__ b,
+                                                  float *__restrict__ x, const SolveState *st, int blk)
 {
     if (st->done) return;
     __shared__ double rb[kMaxDenseCoarse];
     constexpr int kPer = kMaxDenseCoarse / 64;
     const int lane = threadIdx.x & 63;
-    const int row = blockIdx.x * (kBlockThreads / 64) + (threadIdx.x >> 6);
+    const int row = blk * (kBlockThreads / 64) + (threadIdx.x >> 6);
     // this lane's entries of the row, loaded before the gather of b so that the two loads overlap
     const double *Mr = M + (size_t)min(row, n - 1) * n;
     double mr[kPer];
@@ -1821,10 +1826,10 @@ struct March
     bool in;
 };
 
-__device__ __forceinline__ March march_setup(const Box &B, int P)
+__device__ __forceinline__ March march_setup(const Box &B, int P, int lb)
 {
     March M;
-    M.lb = logical_block();
+    M.lb = lb;
     const int tf = M.lb % B.tilesF, t2 = M.lb / B.tilesF, tm = t2 % B.tilesM, chunk = t2 / B.tilesM;
     M.f = 1 + tf * kBlockFast + (threadIdx.x & (kBlockFast - 1));
     M.m = 1 + tm * kBlockMid + (threadIdx.x / kBlockFast);
@@ -1833,6 +1838,8 @@ __device__ __forceinline__ March march_setup(const Box &B, int P)
     M.in = M.f <= B.nf - 2 && M.m <= B.nm - 2;
     return M;
 }
+
+__device__ __forceinline__ March march_setup(const Box &B, int P) { return march_setup(B, P, logical_block()); }
 
 // (A x) of one cell in FP32 from its value and its six neighbours (slow+, slow-, mid+, mid-, fast+, fast-).
 __device__ __forceinline__ float apply32v(const Level32 &L, int s, int m, int f, int64_t idx, unsigned code, float xc,
@@ -2209,12 +2216,11 @@ struct RowMarch
 };
 
 template <int TM, int K>
-__device__ __forceinline__ RowMarch<TM, K> row_march(const Box &B, int P, int tilesM)
+__device__ __forceinline__ RowMarch<TM, K> row_march(const Box &B, int P, int tilesM, int lb)
 {
     RowMarch<TM, K> R;
     R.lane = threadIdx.x & 63;
     R.wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int lb = logical_block();
     const int tm = lb % tilesM, chunk = lb / tilesM;
     R.m = tm * TM + R.wave;   // m0 - 1 + wave with m0 = 1 + tm * TM
     R.halo = R.wave == 0 || R.wave == TM + 1;
@@ -2236,9 +2242,9 @@ __device__ __forceinline__ RowMarch<TM, K> row_march(const Box &B, int P, int ti
 
 // Part 1 (see k_cg_dir64_t): p = (float)(z + beta p_old) (z on the first iteration) at the unknowns, and <p, A p>.
 template <int TM, int K>
-__global__ __launch_bounds__(64 * (TM + 2)) void k_cg_dir64_row(Box B, Tables64 T, const float *__restrict__ z,
-                                                                float *__restrict__ pA, float *__restrict__ pB,
-                                                                const SolveState *st, Red red, int P, int tilesM)
+__device__ __forceinline__ void cg_dir64_row_body(const Box &B, const Tables64 &T, const float *__restrict__ z,
+                                                  float *__restrict__ pA, float *__restrict__ pB, const SolveState *st,
+                                                  const Red &red, int P, int tilesM, int lb, int nblk)
 {
 #pragma clang fp contract(off)
     if (st->done) return;
@@ -2249,7 +2255,7 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_cg_dir64_row(Box B, Tables64 
     const float *__restrict__ po = odd ? pA : pB;
     constexpr int kSlots = 64 * K + 2;
     __shared__ float ldsP[2][TM + 2][2][kSlots];   // new p of a plane by parity, row, element and pair + 1
-    const RowMarch<TM, K> R = row_march<TM, K>(B, P, tilesM);
+    const RowMarch<TM, K> R = row_march<TM, K>(B, P, tilesM, lb);
     for (int i = threadIdx.x; i < 2 * (TM + 2) * 2 * kSlots; i += 64 * (TM + 2)) (&ldsP[0][0][0][0])[i] = 0.f;
     auto plane = [&](int pl) { return (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
     auto pair = [](const float *row, uint32_t o) { return *reinterpret_cast<const float2 *>(row + o); };
@@ -2348,16 +2354,16 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_cg_dir64_row(Box B, Tables64 
         step(sp + 3, std::integral_constant<int, 0>());
     }
     double a[1] = {pq};
-    block_sum<1>(a, red, gridDim.x, logical_block());
+    block_sum<1>(a, red, nblk, lb);
 }
 
 // Part 2 (see k_cg_update64_t): q = A p, x += alpha p, r -= alpha q, r32 = (float) r at the unknowns, and <r, r>.
 template <int TM, int K>
-__global__ __launch_bounds__(64 * (TM + 2), 4) void k_cg_update64_row(Box B, Tables64 T, const float *__restrict__ pA,
-                                                                   const float *__restrict__ pB,
-                                                                   double *__restrict__ x, double *__restrict__ r,
-                                                                   float *__restrict__ r32, const SolveState *st,
-                                                                   Red red, int P, int tilesM)
+__device__ __forceinline__ void cg_update64_row_body(const Box &B, const Tables64 &T, const float *__restrict__ pA,
+                                                     const float *__restrict__ pB, double *__restrict__ x,
+                                                     double *__restrict__ r, float *__restrict__ r32,
+                                                     const SolveState *st, const Red &red, int P, int tilesM, int lb,
+                                                     int nblk)
 {
 #pragma clang fp contract(off)
     if (st->done) return;
@@ -2365,7 +2371,7 @@ __global__ __launch_bounds__(64 * (TM + 2), 4) void k_cg_update64_row(Box B, Tab
     const float *__restrict__ p = (st->iters & 1) ? pB : pA;
     constexpr int kSlots = 64 * K + 2;
     __shared__ float ldsP[2][TM + 2][2][kSlots];
-    const RowMarch<TM, K> R = row_march<TM, K>(B, P, tilesM);
+    const RowMarch<TM, K> R = row_march<TM, K>(B, P, tilesM, lb);
     for (int i = threadIdx.x; i < 2 * (TM + 2) * 2 * kSlots; i += 64 * (TM + 2)) (&ldsP[0][0][0][0])[i] = 0.f;
     auto plane = [&](int pl) { return (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
     auto pair = [](const float *row, uint32_t o) { return *reinterpret_cast<const float2 *>(row + o); };
@@ -2494,7 +2500,7 @@ __global__ __launch_bounds__(64 * (TM + 2), 4) void k_cg_update64_row(Box B, Tab
         step(sp + 3, std::integral_constant<int, 0>());
     }
     double a[1] = {rr};
-    block_sum<1>(a, red, gridDim.x, logical_block());
+    block_sum<1>(a, red, nblk, lb);
 }
 
 // Residual of every fine cell summed over each 2 x 2 x 2 family into the coarse right-hand side (k_residual_restrict_m)
@@ -2505,10 +2511,10 @@ __global__ __launch_bounds__(64 * (TM + 2), 4) void k_cg_update64_row(Box B, Tab
 // neighbours are added without testing the code's bits (x is +0 at every cell that is no unknown, and at the LDS
 // columns beyond the row ends); the first row of each family reads its partner row's pair sums one plane step later.
 template <int TM, int K>
-__global__ __launch_bounds__(64 * (TM + 2)) void k_restrict_row(Level32 Lf, const float *__restrict__ x,
-                                                                const float *__restrict__ b, Level32 Lc,
-                                                                float *__restrict__ bc, const SolveState *st, int P,
-                                                                int tilesM)
+__device__ __forceinline__ void restrict_row_body(const Level32 &Lf, const float *__restrict__ x,
+                                                  const float *__restrict__ b, const Level32 &Lc,
+                                                  float *__restrict__ bc, const SolveState *st, int P, int tilesM,
+                                                  int lb)
 {
     static_assert((TM & 1) == 0, "rows are summed in pairs");
     if (st->done) return;
@@ -2516,7 +2522,7 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_restrict_row(Level32 Lf, cons
     constexpr int kSlots = 64 * K + 2;
     __shared__ float ldsX[2][TM + 2][2][kSlots];   // x of a plane by parity, row, element and pair + 1
     __shared__ float ldsR[2][TM][64 * K];          // the rows' pair sums of the residual by plane parity
-    const RowMarch<TM, K> R = row_march<TM, K>(B, P, tilesM);
+    const RowMarch<TM, K> R = row_march<TM, K>(B, P, tilesM, lb);
     for (int i = threadIdx.x; i < 2 * (TM + 2) * 2 * kSlots; i += 64 * (TM + 2)) (&ldsX[0][0][0][0])[i] = 0.f;
     auto plane = [&](int pl) { return (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
     auto pair = [](const float *row, uint32_t o) { return *reinterpret_cast<const float2 *>(row + o); };
@@ -2652,14 +2658,13 @@ __global__ __launch_bounds__(64 * (TM + 2)) void k_restrict_row(Level32 Lf, cons
 // neighbouring lanes (shuffle), mid pairs neighbouring rows of the block (LDS), slow pairs consecutive planes of
 // the march (P even, chunks start on an odd plane). A family cut by the end of an odd interior keeps its 1 or 2
 // children (the same clipping as the coarse widths).
-__global__ __launch_bounds__(kBlockThreads) void k_residual_restrict_m(Level32 Lf, const float *__restrict__ x,
-                                                                       const float *__restrict__ b, Level32 Lc,
-                                                                       float *__restrict__ bc, const SolveState *st,
-                                                                       int P)
+__device__ __forceinline__ void residual_restrict_m_body(const Level32 &Lf, const float *__restrict__ x,
+                                                         const float *__restrict__ b, const Level32 &Lc,
+                                                         float *__restrict__ bc, const SolveState *st, int P, int lb)
 {
     if (st->done) return;
     const Box &B = Lf.B;
-    const March M = march_setup(B, P);
+    const March M = march_setup(B, P, lb);
     __shared__ float sh[kBlockMid][kBlockFast];
     const int tx = threadIdx.x & (kBlockFast - 1), ty = threadIdx.x / kBlockFast;
     int64_t idx = B.at(M.s0, M.in ? M.m : 1, M.in ? M.f : 1);
@@ -2879,6 +2884,236 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg64_spmv_m(Box B, Tables64 T
     block_sum<1>(a, red, gridDim.x, M.lb);
 }
 
+// ---- the inner iteration's kernels, for one system and batched over several -----------------------------------
+//
+// A batched launch runs one kernel for several systems at once. System q owns the logical blocks [start[q],
+// start[q + 1]) of the launch (numbered after the XCD remap over the whole launch), and logical block g of the launch
+// runs block g - start[q] of the system's own launch with the system's own arguments: the same work in the same
+// order, so a system solved in a batch gives the bits it gives alone. Reductions stay per system (its partials, its
+// ticket counter, its block count).
+
+struct RowSweepArgs
+{
+    Level32 L;
+    const float *xin;
+    float *xout;
+    const float *b;
+    int c1, P, tilesM;
+    const SolveState *st;
+    double *dot;
+    Prolong pro;
+};
+
+struct RestrictRowArgs
+{
+    Level32 Lf;
+    const float *x;
+    const float *b;
+    Level32 Lc;
+    float *bc;
+    const SolveState *st;
+    int P, tilesM;
+};
+
+struct RestrictMarchArgs
+{
+    Level32 Lf;
+    const float *x;
+    const float *b;
+    Level32 Lc;
+    float *bc;
+    const SolveState *st;
+    int P;
+};
+
+struct Dir64RowArgs
+{
+    Box B;
+    Tables64 T;
+    const float *z;
+    float *pA, *pB;
+    const SolveState *st;
+    Red red;
+    int P, tilesM;
+};
+
+struct Update64RowArgs
+{
+    Box B;
+    Tables64 T;
+    const float *pA, *pB;
+    double *x, *r;
+    float *r32;
+    const SolveState *st;
+    Red red;
+    int P, tilesM;
+};
+
+struct CoarseArgs
+{
+    int n;
+    const int32_t *order;
+    const double *M;
+    const float *b;
+    float *x;
+    const SolveState *st;
+};
+
+struct RzSumArgs
+{
+    SolveState *st;
+    const double *rows;
+    int nrows, seg;
+    double *blockSums;
+};
+
+struct FinalizeArgs
+{
+    SolveState *st;
+    double *partials;
+    int nblk, nv, op;
+};
+
+struct BatchBlock
+{
+    int q, lb, nb;
+};
+
+// This block's system and its logical block in that system's launch (start: nsys + 1 increasing block offsets,
+// start[0] = 0 and start[nsys] = the launch's grid).
+__device__ __forceinline__ BatchBlock batch_block(const int *__restrict__ start, int nsys)
+{
+    const int g = logical_block();
+    int lo = 0, hi = nsys - 1;   // the last system whose first block is at most g
+    while (lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return BatchBlock{lo, g - start[lo], start[lo + 1] - start[lo]};
+}
+
+template <int TM, int K, bool ZERO, bool PRO, bool NU>
+__global__ __launch_bounds__(64 * (TM + 2)) void k_rbgs_row(Level32 L, const float *__restrict__ xin,
+                                                            float *__restrict__ xout, const float *__restrict__ b,
+                                                            int c1, int P, int tilesM, const SolveState *st,
+                                                            double *__restrict__ dot, Prolong pro)
+{
+    rbgs_row_body<TM, K, ZERO, PRO, NU>(L, xin, xout, b, c1, P, tilesM, st, dot, pro, logical_block());
+}
+
+template <int TM, int K, bool ZERO, bool PRO, bool NU>
+__global__ __launch_bounds__(64 * (TM + 2)) void kb_rbgs_row(const RowSweepArgs *__restrict__ args,
+                                                             const int *__restrict__ start, int nsys)
+{
+    const BatchBlock bb = batch_block(start, nsys);
+    const RowSweepArgs a = args[bb.q];
+    rbgs_row_body<TM, K, ZERO, PRO, NU>(a.L, a.xin, a.xout, a.b, a.c1, a.P, a.tilesM, a.st, a.dot, a.pro, bb.lb);
+}
+
+template <int TM, int K>
+__global__ __launch_bounds__(64 * (TM + 2)) void k_restrict_row(Level32 Lf, const float *__restrict__ x,
+                                                                const float *__restrict__ b, Level32 Lc,
+                                                                float *__restrict__ bc, const SolveState *st, int P,
+                                                                int tilesM)
+{
+    restrict_row_body<TM, K>(Lf, x, b, Lc, bc, st, P, tilesM, logical_block());
+}
+
+template <int TM, int K>
+__global__ __launch_bounds__(64 * (TM + 2)) void kb_restrict_row(const RestrictRowArgs *__restrict__ args,
+                                                                 const int *__restrict__ start, int nsys)
+{
+    const BatchBlock bb = batch_block(start, nsys);
+    const RestrictRowArgs a = args[bb.q];
+    restrict_row_body<TM, K>(a.Lf, a.x, a.b, a.Lc, a.bc, a.st, a.P, a.tilesM, bb.lb);
+}
+
+__global__ __launch_bounds__(kBlockThreads) void k_residual_restrict_m(Level32 Lf, const float *__restrict__ x,
+                                                                       const float *__restrict__ b, Level32 Lc,
+                                                                       float *__restrict__ bc, const SolveState *st,
+                                                                       int P)
+{
+    residual_restrict_m_body(Lf, x, b, Lc, bc, st, P, logical_block());
+}
+
+__global__ __launch_bounds__(kBlockThreads) void kb_residual_restrict_m(const RestrictMarchArgs *__restrict__ args,
+                                                                        const int *__restrict__ start, int nsys)
+{
+    const BatchBlock bb = batch_block(start, nsys);
+    const RestrictMarchArgs a = args[bb.q];
+    residual_restrict_m_body(a.Lf, a.x, a.b, a.Lc, a.bc, a.st, a.P, bb.lb);
+}
+
+template <int TM, int K>
+__global__ __launch_bounds__(64 * (TM + 2)) void k_cg_dir64_row(Box B, Tables64 T, const float *__restrict__ z,
+                                                                float *__restrict__ pA, float *__restrict__ pB,
+                                                                const SolveState *st, Red red, int P, int tilesM)
+{
+    cg_dir64_row_body<TM, K>(B, T, z, pA, pB, st, red, P, tilesM, logical_block(), gridDim.x);
+}
+
+template <int TM, int K>
+__global__ __launch_bounds__(64 * (TM + 2)) void kb_cg_dir64_row(const Dir64RowArgs *__restrict__ args,
+                                                                 const int *__restrict__ start, int nsys)
+{
+    const BatchBlock bb = batch_block(start, nsys);
+    const Dir64RowArgs a = args[bb.q];
+    cg_dir64_row_body<TM, K>(a.B, a.T, a.z, a.pA, a.pB, a.st, a.red, a.P, a.tilesM, bb.lb, bb.nb);
+}
+
+template <int TM, int K>
+__global__ __launch_bounds__(64 * (TM + 2), 4) void k_cg_update64_row(Box B, Tables64 T, const float *__restrict__ pA,
+                                                                   const float *__restrict__ pB,
+                                                                   double *__restrict__ x, double *__restrict__ r,
+                                                                   float *__restrict__ r32, const SolveState *st,
+                                                                   Red red, int P, int tilesM)
+{
+    cg_update64_row_body<TM, K>(B, T, pA, pB, x, r, r32, st, red, P, tilesM, logical_block(), gridDim.x);
+}
+
+template <int TM, int K>
+__global__ __launch_bounds__(64 * (TM + 2), 4) void kb_cg_update64_row(const Update64RowArgs *__restrict__ args,
+                                                                    const int *__restrict__ start, int nsys)
+{
+    const BatchBlock bb = batch_block(start, nsys);
+    const Update64RowArgs a = args[bb.q];
+    cg_update64_row_body<TM, K>(a.B, a.T, a.pA, a.pB, a.x, a.r, a.r32, a.st, a.red, a.P, a.tilesM, bb.lb, bb.nb);
+}
+
+__global__ __launch_bounds__(kBlockThreads) void k_coarse_solve(int n, const int32_t *__restrict__ order,
+                                                                const double *__restrict__ M,
+                                                                const float *__restrict__ b, float *__restrict__ x,
+                                                                const SolveState *st)
+{
+    coarse_solve_body(n, order, M, b, x, st, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kBlockThreads) void kb_coarse_solve(const CoarseArgs *__restrict__ args,
+                                                                 const int *__restrict__ start, int nsys)
+{
+    const BatchBlock bb = batch_block(start, nsys);
+    const CoarseArgs a = args[bb.q];
+    coarse_solve_body(a.n, a.order, a.M, a.b, a.x, a.st, bb.lb);
+}
+
+__global__ __launch_bounds__(kBlockThreads) void kb_rz_rows_sum(const RzSumArgs *__restrict__ args,
+                                                                const int *__restrict__ start, int nsys)
+{
+    const BatchBlock bb = batch_block(start, nsys);
+    const RzSumArgs a = args[bb.q];
+    rz_rows_sum_body(a.st, a.rows, a.nrows, a.seg, a.blockSums, bb.lb, bb.nb);
+}
+
+__global__ __launch_bounds__(1024) void kb_finalize(const FinalizeArgs *__restrict__ args,
+                                                    const int *__restrict__ start, int nsys)
+{
+    const BatchBlock bb = batch_block(start, nsys);
+    const FinalizeArgs a = args[bb.q];
+    fin_apply(a.st, a.partials, a.nblk, a.nv, a.op, false);
+}
+
 __global__ void k_inner_setup(SolveState *st, double reduction, int maxIters)
 {
     const double r2 = st->sums[1];
@@ -2940,6 +3175,98 @@ inline int grid_1d(int64_t n) { return (int)std::min<int64_t>((n + kBlockThreads
 // followed by a k_finalize launch (the in-kernel ticket wait would lengthen every block).
 constexpr int kFuseBlocks = 512;
 
+// ---- launch recording (batched solves) -------------------------------------------------------------------------
+
+namespace
+{
+thread_local LaunchRecorder *tRecorder = nullptr;
+
+// With a recorder set on this thread: the launch appended to it (the batched kernel, the system's grid and block,
+// its arguments) instead of made. Returns whether it was recorded.
+template <class A>
+bool record(void (*fn)(const A *, const int *, int), int grid, int threads, const A &a, int kind = REC_FIXED,
+            int tm = 0, int k = 0, int flags = 0)
+{
+    LaunchRecorder *r = tRecorder;
+    if (!r) return false;
+    LaunchRecord q;
+    q.fn = reinterpret_cast<const void *>(fn);
+    q.grid = grid;
+    q.threads = threads;
+    q.phase = r->phase;
+    q.level = r->level;
+    q.seq = r->seq++;
+    q.kind = kind;
+    q.tm = tm;
+    q.k = k;
+    q.flags = flags;
+    q.args.assign(reinterpret_cast<const unsigned char *>(&a), reinterpret_cast<const unsigned char *>(&a) + sizeof(A));
+    r->recs.push_back(std::move(q));
+    return true;
+}
+
+// variant bits of the whole-row sweep: 1 ZERO, 2 PRO, 4 NU
+template <int TM, int K>
+const void *row_sweep_variant(int flags)
+{
+    switch (flags)
+    {
+    case 0: return reinterpret_cast<const void *>(kb_rbgs_row<TM, K, false, false, false>);
+    case 1: return reinterpret_cast<const void *>(kb_rbgs_row<TM, K, true, false, false>);
+    case 2: return reinterpret_cast<const void *>(kb_rbgs_row<TM, K, false, true, false>);
+    case 4: return reinterpret_cast<const void *>(kb_rbgs_row<TM, K, false, false, true>);
+    case 5: return reinterpret_cast<const void *>(kb_rbgs_row<TM, K, true, false, true>);
+    case 6: return reinterpret_cast<const void *>(kb_rbgs_row<TM, K, false, true, true>);
+    default: return nullptr;
+    }
+}
+
+// The batched whole-row sweep of that shape and variant (the shapes launch_rbgs instantiates), or null.
+const void *row_sweep_batched(int tm, int k, int flags)
+{
+    if (tm == 6 && k == 1) return row_sweep_variant<6, 1>(flags);
+    if (tm == 14 && k == 1) return row_sweep_variant<14, 1>(flags);
+    if (tm == 6 && k == 2) return row_sweep_variant<6, 2>(flags);
+    if (tm == 14 && k == 2) return row_sweep_variant<14, 2>(flags);
+    if (tm == 10 && k == 2) return row_sweep_variant<10, 2>(flags);
+    if (tm == 6 && k == 4) return row_sweep_variant<6, 4>(flags);
+    return nullptr;
+}
+
+template <int TM, int K, bool ZERO, bool PRO, bool NU>
+void row_sweep(hipStream_t s, int grid, int threads, const RowSweepArgs &a)
+{
+    if (record(kb_rbgs_row<TM, K, ZERO, PRO, NU>, grid, threads, a, REC_ROW_SWEEP, TM, K,
+               (ZERO ? 1 : 0) | (PRO ? 2 : 0) | (NU ? 4 : 0)))
+        return;
+    k_rbgs_row<TM, K, ZERO, PRO, NU><<<grid, threads, 0, s>>>(a.L, a.xin, a.xout, a.b, a.c1, a.P, a.tilesM, a.st,
+                                                              a.dot, a.pro);
+}
+
+template <int TM, int K>
+void restrict_row(hipStream_t s, int grid, int threads, const RestrictRowArgs &a)
+{
+    if (record(kb_restrict_row<TM, K>, grid, threads, a, REC_RESTRICT_ROW, TM, K)) return;
+    k_restrict_row<TM, K><<<grid, threads, 0, s>>>(a.Lf, a.x, a.b, a.Lc, a.bc, a.st, a.P, a.tilesM);
+}
+
+template <int TM, int K>
+void dir64_row(hipStream_t s, int grid, int threads, const Dir64RowArgs &a)
+{
+    if (record(kb_cg_dir64_row<TM, K>, grid, threads, a)) return;
+    k_cg_dir64_row<TM, K><<<grid, threads, 0, s>>>(a.B, a.T, a.z, a.pA, a.pB, a.st, a.red, a.P, a.tilesM);
+}
+
+template <int TM, int K>
+void update64_row(hipStream_t s, int grid, int threads, const Update64RowArgs &a)
+{
+    if (record(kb_cg_update64_row<TM, K>, grid, threads, a)) return;
+    k_cg_update64_row<TM, K><<<grid, threads, 0, s>>>(a.B, a.T, a.pA, a.pB, a.x, a.r, a.r32, a.st, a.red, a.P,
+                                                      a.tilesM);
+}
+}   // namespace
+
+
 Red red_of(double *partials, SolveState *st, FinOp op, int blocks)
 {
     return Red{partials, st, (int)op, blocks <= kFuseBlocks ? 1 : 0};
@@ -2950,6 +3277,8 @@ hipError_t finish(hipStream_t s, const Red &red, int nblk, int nv)
 {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || red.fused) return e;
+    if (record(kb_finalize, 1, fin_threads(nblk), FinalizeArgs{red.st, red.partials, nblk, nv, red.op}))
+        return hipSuccess;
     k_finalize<<<1, fin_threads(nblk), 0, s>>>(red.st, red.partials, nblk, nv, red.op);
     return hipGetLastError();
 }
@@ -2962,6 +3291,81 @@ hipError_t finish(hipStream_t s, const Red &red, int nblk, int nv)
     } while (0)
 
 }  // namespace
+
+void set_launch_recorder(LaunchRecorder *r) { tRecorder = r; }
+
+void record_slot(int phase, int level)
+{
+    if (!tRecorder) return;
+    tRecorder->phase = phase;
+    tRecorder->level = level;
+    tRecorder->seq = 0;
+}
+
+void retile_rows(const LaunchRecord &r, std::vector<int> &tms)
+{
+    tms.clear();
+    if (r.kind != REC_ROW_SWEEP) return;
+    // 14-row tiles need 1024-thread blocks, whose register limit the prolongation-folding K = 2 sweep exceeds (10
+    // rows there on uniform levels); K = 4 rows only in 6-row tiles
+    if (r.k == 4) tms = {6};
+    else if (r.k == 2 && (r.flags & 2)) tms = (r.flags & 4) ? std::vector<int>{6} : std::vector<int>{6, 10};
+    else tms = {6, 14};
+}
+
+bool retile_record(LaunchRecord &r, int tm, int p)
+{
+    const int P = std::max(1, std::min(64, p));
+    const int Pe = std::max(2, P + (P & 1));
+    switch (r.kind)
+    {
+    case REC_ROW_SWEEP:
+    {
+        RowSweepArgs a;
+        std::memcpy(&a, r.args.data(), sizeof a);
+        const int TM = tm > 0 ? tm : r.tm;
+        std::vector<int> ok;
+        retile_rows(r, ok);
+        const void *fn = row_sweep_batched(TM, r.k, r.flags);
+        if (!fn || std::find(ok.begin(), ok.end(), TM) == ok.end()) return false;
+        const Box &B = a.L.B;
+        a.P = P;
+        a.tilesM = std::max(1, (B.nm - 2 + TM - 1) / TM);
+        r.grid = a.tilesM * std::max(1, (B.ns - 2 + P - 1) / P);
+        r.threads = 64 * (TM + 2);
+        r.fn = fn;
+        r.tm = TM;
+        std::memcpy(r.args.data(), &a, sizeof a);
+        return true;
+    }
+    case REC_RESTRICT_ROW:
+    {
+        RestrictRowArgs a;
+        std::memcpy(&a, r.args.data(), sizeof a);
+        a.P = Pe;
+        r.grid = a.tilesM * std::max(1, (a.Lf.B.ns - 2 + Pe - 1) / Pe);
+        std::memcpy(r.args.data(), &a, sizeof a);
+        return true;
+    }
+    case REC_RESTRICT_MARCH:
+    {
+        RestrictMarchArgs a;
+        std::memcpy(&a, r.args.data(), sizeof a);
+        a.P = Pe;
+        r.grid = march_grid(a.Lf.B, Pe);
+        std::memcpy(r.args.data(), &a, sizeof a);
+        return true;
+    }
+    default: return false;
+    }
+}
+
+hipError_t launch_batched(hipStream_t s, const void *fn, int grid, int threads, const void *args, const int *start,
+                          int nsys)
+{
+    void *kargs[] = {(void *)&args, (void *)&start, (void *)&nsys};
+    return hipLaunchKernel(fn, dim3(grid), dim3(threads), kargs, 0, s);
+}
 
 hipError_t launch_scatter64(hipStream_t s, int64_t n, const int32_t *order, const double *src, double *box,
                             SolveState *st, int sumSlot, double *partials)
@@ -3177,18 +3581,11 @@ hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float 
         Prolong pro{nullptr, 0, 0, 1, 1};
         if (prolongFrom) pro = Prolong{prolongXc, prolongFrom->B.sS, (int)prolongFrom->B.sM, prolongFrom->B.ns,
                                        prolongFrom->B.nf};
-#define NPPE_ROW_NU(TMv, Kv, NUv)                                                                                     \
-    (zero ? (k_rbgs_row<TMv, Kv, true, false, NUv><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P, tilesM, st,        \
-                                                                            rzPartials, pro),                         \
-             0)                                                                                                       \
-          : (prolongFrom ? (k_rbgs_row<TMv, Kv, false, true, NUv><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P,      \
-                                                                                          tilesM, st, rzPartials,      \
-                                                                                          pro),                        \
-                            0)                                                                                        \
-                         : (k_rbgs_row<TMv, Kv, false, false, NUv><<<grid, threads, 0, s>>>(L, xin, xout, b, c1, P,     \
-                                                                                            tilesM, st, rzPartials,    \
-                                                                                            pro),                      \
-                            0)))
+        const RowSweepArgs ra{L, xin, xout, b, c1, P, tilesM, st, rzPartials, pro};
+#define NPPE_ROW_NU(TMv, Kv, NUv)                                                                                  \
+    (zero ? row_sweep<TMv, Kv, true, false, NUv>(s, grid, threads, ra)                                             \
+          : (prolongFrom ? row_sweep<TMv, Kv, false, true, NUv>(s, grid, threads, ra)                              \
+                         : row_sweep<TMv, Kv, false, false, NUv>(s, grid, threads, ra)))
 #define NPPE_ROW(TMv, Kv) (L.uniform ? NPPE_ROW_NU(TMv, Kv, false) : NPPE_ROW_NU(TMv, Kv, true))
         if (TM == 10)
             NPPE_ROW(10, 2);
@@ -3266,7 +3663,11 @@ RowPlan row_plan(const Box &B, const uint8_t *c8, bool eligible,
     R.tilesM = std::max(1, (B.nm - 2 + tm - 1) / tm);
     const int planes = B.ns - 2;
     const int chunksMax = std::max(1, kCUs * blocksPerCU / R.tilesM);
-    R.P = std::max(1, (planes + chunksMax - 1) / chunksMax);
+    // at least kMinPlanes planes per chunk (as many as the box has): a shorter chunk spends most of its loads on the
+    // planes it reads ahead, which costs little on a small box alone but much when many systems share a launch
+    constexpr int kMinPlanes = 8;
+    R.P = std::max(std::min(kMinPlanes, planes), (planes + chunksMax - 1) / chunksMax);
+    R.P = std::max(1, R.P);
     if (evenP) R.P += R.P & 1;
     R.G = R.tilesM * ((planes + R.P - 1) / R.P);
     return R;
@@ -3281,14 +3682,18 @@ hipError_t launch_residual_restrict(hipStream_t s, const Level32 &Lf, const floa
         const RowPlan rp = row_plan(Lf.B, Lf.c8, Lf.uniform != 0, {{x, 8}, {b, 8}}, 6, 2, true);
         if (rp.ok)
         {
+            const RestrictRowArgs a{Lf, x, b, Lc, bc, st, rp.P, rp.tilesM};
             if (rp.K == 1)
-                k_restrict_row<6, 1><<<rp.G, 64 * 8, 0, s>>>(Lf, x, b, Lc, bc, st, rp.P, rp.tilesM);
+                restrict_row<6, 1>(s, rp.G, 64 * 8, a);
             else
-                k_restrict_row<6, 2><<<rp.G, 64 * 8, 0, s>>>(Lf, x, b, Lc, bc, st, rp.P, rp.tilesM);
+                restrict_row<6, 2>(s, rp.G, 64 * 8, a);
             return hipGetLastError();
         }
-        const int P = march_planes(Lf.B);
-        k_residual_restrict_m<<<march_grid(Lf.B, P), kBlockThreads, 0, s>>>(Lf, x, b, Lc, bc, st, P);
+        const int P = march_planes(Lf.B), G = march_grid(Lf.B, P);
+        if (record(kb_residual_restrict_m, G, kBlockThreads, RestrictMarchArgs{Lf, x, b, Lc, bc, st, P},
+                   REC_RESTRICT_MARCH))
+            return hipSuccess;
+        k_residual_restrict_m<<<G, kBlockThreads, 0, s>>>(Lf, x, b, Lc, bc, st, P);
         return hipGetLastError();
     }
     k_residual_restrict<<<grid_of(Lc.B), kBlockThreads, 0, s>>>(Lf, x, b, Lc, bc, st);
@@ -3307,8 +3712,9 @@ hipError_t launch_coarse_solve(hipStream_t s, int n, const int32_t *order, const
 {
     if (n <= 0) return hipSuccess;
     if (n > kMaxDenseCoarse) return hipErrorInvalidValue;
-    const int rowsPerBlock = kBlockThreads / 64;
-    k_coarse_solve<<<(n + rowsPerBlock - 1) / rowsPerBlock, kBlockThreads, 0, s>>>(n, order, M, b, x, st);
+    const int rowsPerBlock = kBlockThreads / 64, grid = (n + rowsPerBlock - 1) / rowsPerBlock;
+    if (record(kb_coarse_solve, grid, kBlockThreads, CoarseArgs{n, order, M, b, x, st})) return hipSuccess;
+    k_coarse_solve<<<grid, kBlockThreads, 0, s>>>(n, order, M, b, x, st);
     return hipGetLastError();
 }
 
@@ -3318,7 +3724,9 @@ hipError_t launch_rz_rows_finalize(hipStream_t s, const Box &B, SolveState *st, 
     const int nblk = std::min(kFuseBlocks, std::max(1, (nrows + kBlockThreads - 1) / kBlockThreads));
     const int seg = (nrows + nblk - 1) / nblk;
     // the block sums follow the row partials (the partials buffer holds at least twice the row count)
-    k_rz_rows_sum<<<nblk, kBlockThreads, 0, s>>>(st, partials, nrows, seg, partials + ((nrows + 1) & ~1));
+    const RzSumArgs a{st, partials, nrows, seg, partials + ((nrows + 1) & ~1)};
+    if (record(kb_rz_rows_sum, nblk, kBlockThreads, a)) return hipSuccess;
+    k_rz_rows_sum<<<nblk, kBlockThreads, 0, s>>>(a.st, a.rows, a.nrows, a.seg, a.blockSums);
     return hipGetLastError();
 }
 
@@ -3372,10 +3780,11 @@ hipError_t launch_cg_dir64(hipStream_t s, const Box &B, const Tables64 &T, const
     if (rp.ok)
     {
         const Red red = red_of(partials, st, FIN_PQ, rp.G);
+        const Dir64RowArgs a{B, T, z, pA, pB, st, red, rp.P, rp.tilesM};
         if (rp.K == 1)
-            k_cg_dir64_row<6, 1><<<rp.G, 64 * 8, 0, s>>>(B, T, z, pA, pB, st, red, rp.P, rp.tilesM);
+            dir64_row<6, 1>(s, rp.G, 64 * 8, a);
         else
-            k_cg_dir64_row<6, 2><<<rp.G, 64 * 8, 0, s>>>(B, T, z, pA, pB, st, red, rp.P, rp.tilesM);
+            dir64_row<6, 2>(s, rp.G, 64 * 8, a);
         return finish(s, red, rp.G, 1);
     }
     const int P = march_planes(B), G = march_grid(B, P);
@@ -3394,10 +3803,11 @@ hipError_t launch_cg_update64(hipStream_t s, const Box &B, const Tables64 &T, co
     if (rp.ok)
     {
         const Red red = red_of(partials, st, FIN_RR, rp.G);
+        const Update64RowArgs a{B, T, pA, pB, x, r, r32, st, red, rp.P, rp.tilesM};
         if (rp.K == 1)
-            k_cg_update64_row<6, 1><<<rp.G, 64 * 8, 0, s>>>(B, T, pA, pB, x, r, r32, st, red, rp.P, rp.tilesM);
+            update64_row<6, 1>(s, rp.G, 64 * 8, a);
         else
-            k_cg_update64_row<6, 2><<<rp.G, 64 * 8, 0, s>>>(B, T, pA, pB, x, r, r32, st, red, rp.P, rp.tilesM);
+            update64_row<6, 2>(s, rp.G, 64 * 8, a);
         return finish(s, red, rp.G, 1);
     }
     const int P = march_planes(B), G = march_grid(B, P);

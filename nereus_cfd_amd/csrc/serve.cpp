@@ -1,11 +1,14 @@
-// Many systems at once: nppe_batch_solve (a fixed set of committed systems solved concurrently) and nppe_server
-// (requests admitted against an HBM budget into solver slots as they arrive, continuous batching). Both run every
-// system on its own handle and stream, so each result is bit-identical to the same solve run alone: no kernel,
-// buffer or reduction is shared between systems.
+// Many systems at once: nppe_batch_solve (a fixed set of committed systems solved together) and nppe_server
+// (requests admitted against an HBM budget into solver slots as they arrive, continuous batching). The batch shares
+// launches between systems where they qualify (batch_solve_krylov64: every block still does one system's work of
+// its own launch); otherwise, and in the server, every system runs on its own handle and stream. Either way each
+// result is bit-identical to the same solve run alone: no buffer or reduction is shared between systems.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -208,6 +211,22 @@ int nppe_batch_solve(nppe_solver *const *hs, const double *const *b, double *con
 {
     if (!hs || !b || !x || count < 0) return NPPE_ERR_ARGUMENT;
     if (count == 0) return NPPE_OK;
+    if (device)
+    {
+        // the inner iterations of all systems batched into shared launches, where every system qualifies
+        std::vector<int> rcs(count, NPPE_OK);
+        const int rc = nppe::batch_solve_krylov64(hs, b, x, count, res, rcs.data());
+        if (rc < 0) return rc;
+        if (rc == 1 && std::getenv("NPPE_BATCH_VERBOSE")) std::fprintf(stderr, "nppe: %s\n", hs[0]->err.c_str());
+        if (rc == NPPE_OK)
+        {
+            int worst = NPPE_OK;
+            for (int r : rcs)
+                if (r < 0) return r;
+                else if (r != NPPE_OK) worst = r;
+            return worst;
+        }
+    }
     const int nw = std::max(1, std::min<int>(workers > 0 ? workers : count, std::min(count, 64)));
     std::atomic<int> next(0);
     std::vector<int> rcs(count, NPPE_OK);

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -53,7 +54,9 @@ struct Traffic
 namespace
 {
 
-void free_grid(nppe_solver *h)
+// The captured inner iterations no longer match the system (a new grid or layout): dropped, and the handle's graph
+// epoch advanced so that batched plans built over it are rebuilt too.
+void drop_graphs(nppe_solver *h)
 {
     if (h->innerGraph)
     {
@@ -66,6 +69,12 @@ void free_grid(nppe_solver *h)
         h->chunkGraph = nullptr;
         h->chunkGraphIters = 0;
     }
+    h->graphEpoch++;
+}
+
+void free_grid(nppe_solver *h)
+{
+    drop_graphs(h);
     h->rate = 0.0;
     h->lastFirstPass = 0;
     {
@@ -173,6 +182,7 @@ hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel, doubl
         DevLevel &L = h->levels[l];
         const double N = L.cellsBox;
         const int nu1 = (coarseNu > 0 && l >= 2) ? coarseNu : h->nuPre;
+        record_slot(SLOT_DOWN, l);
         for (int sw = 0; sw < nu1; sw++)
         {
             const bool zero = sw == 0;
@@ -199,6 +209,7 @@ hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel, doubl
     {
         const int l = nl - 1;
         DevLevel &L = h->levels[l];
+        record_slot(SLOT_COARSE, 0);
         if (h->coarseDense)
         {
             const double cb = 8.0 * h->nCoarse * (double)h->nCoarse + 8.0 * h->nCoarse * 2;
@@ -232,6 +243,7 @@ hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel, doubl
         const double N = L.cellsBox;
         const double pb = s * C.cellsBox + 2 * s * N + f * N;
         const int nu2 = (coarseNu > 0 && l >= 2) ? coarseNu : h->nuPost;
+        record_slot(SLOT_UP, l);
         // The whole-row sweep folds the prolongation into its first post-smoothing sweep (it reads x + P x_c
         // where it reads x); other sweeps get the correction as a pass of its own.
         const bool foldProlong =
@@ -288,6 +300,7 @@ hipError_t inner_iteration(nppe_solver *h, Traffic &tr)
     int zsel = 0;
     bool rzDone = false;
     if ((e = vcycle(h, tr, h->dR, &zsel, h->dPartials, &rzDone)) != hipSuccess) return e;
+    record_slot(SLOT_KRYLOV, 0);
     const float *z = zsel ? h->dZ2 : h->dZ;
     if (!rzDone)
     {
@@ -924,17 +937,7 @@ int layout_from_device(nppe_solver *h, const uint8_t *dTypes, const uint8_t *hos
         }
         return recommit(h, hostTypes);
     }
-    if (h->innerGraph)
-    {
-        (void)hipGraphExecDestroy(h->innerGraph);
-        h->innerGraph = nullptr;
-    }
-    if (h->chunkGraph)
-    {
-        (void)hipGraphExecDestroy(h->chunkGraph);
-        h->chunkGraph = nullptr;
-        h->chunkGraphIters = 0;
-    }
+    drop_graphs(h);
     h->rate = 0.0;
     h->lastFirstPass = 0;
     h->n = info[2 * L];
@@ -1067,181 +1070,665 @@ int finish_result(nppe_solver *h, nppe_result *res, hipEvent_t t0, hipEvent_t t1
 
 // The FP64 Krylov solve scheduled without host round trips: after the guess's true residual (already enqueued;
 // residualIsB: no guess, the residual is b, scattered with it) the device sets up the recursion itself
-// (k_krylov_init); the host enqueues the iterations it predicts (the last
-// solve's count, or kInnerChunk), then the true residual and the output gather, both gated on the device (the
-// residual only once the recursion is done, the gather only once that residual met the target), and synchronises
-// once. A well-predicted solve makes that its only round trip; a short prediction enqueues more iterations, and a
-// missed target restarts CG from the new x as another pass. The kernels and their order are those of the
-// synchronous schedule, so the result is bit for bit the same.
+// (k_krylov_init); the host enqueues the iterations it predicts (the last solve's count, or kInnerChunk), then the
+// true residual and the output gather, both gated on the device (the residual only once the recursion is done, the
+// gather only once that residual met the target), and synchronises once. A well-predicted solve makes that its only
+// round trip; a short prediction enqueues more iterations, and a missed target restarts CG from the new x as another
+// pass. The kernels and their order are those of the synchronous schedule, so the result is bit for bit the same.
+//
+// K64Run is that schedule stepped by its caller, which enqueues the iterations itself (the handle's own graphs in
+// krylov64_scheduled, a batch's launches in batch_krylov64): k64_begin enqueues the set-up, k64_tail the gated
+// residual and gather after the iterations, and k64_advance reads the state once it is on the host and decides --
+// finished, more iterations of the same pass (chunk), or a new pass. Iterations enqueued past convergence exit on
+// the device's done flag, so how many are enqueued never changes the result.
+struct K64Run
+{
+    nppe_solver *h = nullptr;
+    double refNorm = 0.0;
+    int cur = 0;
+    double *dX = nullptr;
+    bool project = false;
+    Traffic *tr = nullptr;
+    int chunk = 0, passes = 0, inner = 0, status = NPPE_OK, launched = 0;
+    double bnorm = 0.0, rnorm = 0.0, rr0 = 0.0, perIter = 0.0;
+    bool first = true, finished = false;
+};
+
+int k64_predict(const nppe_solver *h, double need)
+{
+    if (!(h->rate > 0.0 && h->rate < 1.0) || !(need < 1.0)) return kInnerChunk;
+    const double k = std::ceil(std::log(need) / std::log(h->rate));
+    return (int)std::max(1.0, std::min((double)kMaxChunk, k));
+}
+
+int k64_begin(K64Run &R, bool residualIsB)
+{
+    nppe_solver *h = R.h;
+    HIP_TRY(h, launch_krylov_init(h->stream, h->dState, h->opt.tolerance, R.refNorm, 0.8, h->opt.max_iters,
+                                  residualIsB ? 1 : 0),
+            "krylov setup");
+    R.chunk = h->lastFirstPass > 0 ? std::min(kMaxChunk, h->lastFirstPass) : kInnerChunk;
+    return NPPE_OK;
+}
+
+int k64_tail(K64Run &R)
+{
+    nppe_solver *h = R.h;
+    const DevLevel &L0 = h->levels[0];
+    const double N = L0.cellsBox, n = (double)h->n;
+    {
+        Scope sc(h, NPPE_K_RESIDUAL64, 29.0 * N);
+        // (with the projection it also sums x over the unknowns for the mean)
+        HIP_TRY(h, launch_update_residual64(h->stream, L0.box, h->t64, L0.types, h->dX64[R.cur], nullptr, nullptr,
+                                            h->dB64, h->dR, h->dR64, h->dState, h->dPartials, 1, R.project ? 1 : 0),
+                "true residual");
+    }
+    {
+        Scope sc(h, NPPE_K_GATHER, 20.0 * n);
+        HIP_TRY(h, launch_gather64(h->stream, h->n, h->dOrder, h->dX64[R.cur], R.dX, h->dState, R.project ? 2 : -1, 1),
+                "gather x");
+    }
+    return NPPE_OK;
+}
+
+int k64_advance(K64Run &R)
+{
+    nppe_solver *h = R.h;
+    const DevLevel &L0 = h->levels[0];
+    const double N = L0.cellsBox, n = (double)h->n;
+    const SolveState &st = *h->hState;
+    if (R.first)
+    {
+        R.first = false;
+        R.bnorm = std::sqrt(st.sums[0]);
+        R.rr0 = st.rr0;
+        if (!std::isfinite(R.bnorm) || !std::isfinite(R.rr0))
+            return fail(h, NPPE_ERR_NONFINITE, "non-finite right-hand side or guess");
+        if (R.bnorm == 0.0)
+        {
+            // Eigen sets x = 0 for a zero right-hand side (ConjugateGradient.h:46-52)
+            HIP_TRY(h, launch_scatter64(h->stream, h->n, h->dOrder, nullptr, h->dX64[R.cur], h->dState, -1,
+                                        h->dPartials),
+                    "zero x");
+            HIP_TRY(h, launch_gather64(h->stream, h->n, h->dOrder, h->dX64[R.cur], R.dX, h->dState, -1), "gather x");
+            R.rnorm = 0.0;
+            R.finished = true;
+            return NPPE_OK;
+        }
+    }
+    if (st.done == kDoneNonFinite)
+        return fail(h, NPPE_ERR_NONFINITE, "non-finite value in the Krylov recursion (inconsistent system?)");
+    if (st.iters > 0 && st.rr > 0.0 && st.rr < R.rr0) h->rate = std::pow(st.rr / R.rr0, 0.5 / st.iters);
+    if (!st.done)
+    {
+        // the prediction fell short: more iterations of the same pass
+        R.chunk = st.rr > 0.0 && st.thr2 > 0.0 ? k64_predict(h, std::sqrt(st.thr2 / st.rr)) : kInnerChunk;
+        return NPPE_OK;
+    }
+    R.inner += st.iters;
+    R.tr->add(R.perIter * st.iters + 29.0 * N);
+    R.rnorm = std::sqrt(st.sums[1]);
+    if (!std::isfinite(R.rnorm)) return fail(h, NPPE_ERR_NONFINITE, "non-finite residual during refinement");
+    // a pass that ran iterations counts; the guess accepted as it is does not (as the synchronous loop)
+    if (!(R.passes == 0 && st.iters == 0 && st.accepted)) R.passes++;
+    if (R.passes <= 1) h->lastFirstPass = st.iters;
+    if (st.accepted)
+    {
+        R.tr->add(20.0 * n);   // the gather (the mean's sum came with the residual)
+        R.finished = true;
+        return NPPE_OK;
+    }
+    if (R.passes >= h->opt.max_refinements || R.inner >= h->opt.max_iters)
+    {
+        R.status = NPPE_NOT_CONVERGED;
+        if (R.project)
+        {
+            Scope sc(h, NPPE_K_MEAN, 12.0 * n);
+            HIP_TRY(h, launch_sum64(h->stream, h->n, h->dOrder, h->dX64[R.cur], h->dState, 2, h->dPartials), "mean");
+        }
+        HIP_TRY(h, launch_gather64(h->stream, h->n, h->dOrder, h->dX64[R.cur], R.dX, h->dState, R.project ? 2 : -1),
+                "gather x");
+        R.tr->add((R.project ? 12.0 * n : 0.0) + 20.0 * n);
+        R.finished = true;
+        return NPPE_OK;
+    }
+    // the true residual missed the target: CG again from the new x
+    const double aim = 0.8 * h->opt.tolerance * std::max(R.bnorm, R.refNorm);
+    HIP_TRY(h, launch_state_init(h->stream, h->dState, aim * aim, h->opt.max_iters - R.inner), "krylov setup");
+    R.rr0 = R.rnorm * R.rnorm;
+    R.chunk = k64_predict(h, std::min(0.5, aim / R.rnorm));
+    R.launched = 0;
+    return NPPE_OK;
+}
+
+// Captures `iters` inner iterations of the handle into *out (replacing what it held).
+int capture_iterations(nppe_solver *h, int iters, hipGraphExec_t *out, double *bytesPerIter)
+{
+    if (*out)
+    {
+        (void)hipGraphExecDestroy(*out);
+        *out = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    Traffic it;
+    std::lock_guard<std::mutex> lock(gCaptureMu);
+    HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+    hipError_t ce = hipSuccess;
+    for (int q = 0; q < iters && ce == hipSuccess; q++) ce = inner_iteration(h, it);
+    const hipError_t ee = hipStreamEndCapture(h->stream, &g);
+    if (ce != hipSuccess) return hip_fail(h, ce, "graph capture");
+    if (ee != hipSuccess) return hip_fail(h, ee, "graph capture");
+    const hipError_t ie = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) return hip_fail(h, ie, "graph instantiate");
+    if (bytesPerIter) *bytesPerIter = it.bytes / std::max(1, iters);
+    return NPPE_OK;
+}
+
 int krylov64_scheduled(nppe_solver *h, double refNorm, int cur, double *dX, bool project, bool residualIsB,
                        Traffic &tr, int *passesOut, int *innerOut, int *statusOut, double *bnormOut, double *rnormOut)
 {
-    const DevLevel &L0 = h->levels[0];
-    const double N = L0.cellsBox, n = (double)h->n;
     const bool graphs = h->useGraphs && !h->profiling;
+    // One iteration captured once: its launches, their arguments and the z ping-pong parity are the same every
+    // time, and every kernel reads the device-side done flag, so one graph serves every iteration.
     if (graphs && !h->innerGraph)
+        if (int rc = capture_iterations(h, 1, &h->innerGraph, &h->graphBytesPerIter)) return rc;
+    K64Run R;
+    R.h = h;
+    R.refNorm = refNorm;
+    R.cur = cur;
+    R.dX = dX;
+    R.project = project;
+    R.tr = &tr;
+    if (int rc = k64_begin(R, residualIsB)) return rc;
+    while (!R.finished)
     {
-        hipGraph_t g = nullptr;
-        Traffic it;
-        std::lock_guard<std::mutex> lock(gCaptureMu);
-        HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "graph capture");
-        const hipError_t ce = inner_iteration(h, it);
-        const hipError_t ee = hipStreamEndCapture(h->stream, &g);
-        if (ce != hipSuccess) return hip_fail(h, ce, "graph capture");
-        if (ee != hipSuccess) return hip_fail(h, ee, "graph capture");
-        const hipError_t ie = hipGraphInstantiate(&h->innerGraph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (ie != hipSuccess) return hip_fail(h, ie, "graph instantiate");
-        h->graphBytesPerIter = it.bytes;
-    }
-    // The first chunk of a solve (as many iterations as the last solve's first pass needed) replays as one graph of
-    // that many iterations: consecutive graph launches leave a gap of several microseconds between them. Captured
-    // again only when that count changes; shorter later chunks replay the one-iteration graph.
-    auto chunkGraph = [&](int iters) -> int {
-        if (h->chunkGraph && h->chunkGraphIters == iters) return NPPE_OK;
-        if (h->chunkGraph)
+        // The first chunk of a solve (as many iterations as the last solve's first pass needed) replays as one
+        // graph of that many iterations: consecutive graph launches leave a gap of several microseconds between
+        // them. Captured again only when that count changes; shorter later chunks replay the one-iteration graph.
+        if (graphs && R.first && R.chunk > 1)
         {
-            (void)hipGraphExecDestroy(h->chunkGraph);
-            h->chunkGraph = nullptr;
-            h->chunkGraphIters = 0;
-        }
-        hipGraph_t g = nullptr;
-        Traffic it;
-        std::lock_guard<std::mutex> lock(gCaptureMu);
-        HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "graph capture");
-        hipError_t ce = hipSuccess;
-        for (int q = 0; q < iters && ce == hipSuccess; q++) ce = inner_iteration(h, it);
-        const hipError_t ee = hipStreamEndCapture(h->stream, &g);
-        if (ce != hipSuccess) return hip_fail(h, ce, "graph capture");
-        if (ee != hipSuccess) return hip_fail(h, ee, "graph capture");
-        const hipError_t ie = hipGraphInstantiate(&h->chunkGraph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (ie != hipSuccess) return hip_fail(h, ie, "graph instantiate");
-        h->chunkGraphIters = iters;
-        return NPPE_OK;
-    };
-    const double tol = h->opt.tolerance;
-    HIP_TRY(h, launch_krylov_init(h->stream, h->dState, tol, refNorm, 0.8, h->opt.max_iters, residualIsB ? 1 : 0),
-            "krylov setup");
-    auto predict = [&](double need) {
-        if (!(h->rate > 0.0 && h->rate < 1.0) || !(need < 1.0)) return kInnerChunk;
-        const double k = std::ceil(std::log(need) / std::log(h->rate));
-        return (int)std::max(1.0, std::min((double)kMaxChunk, k));
-    };
-    int chunk = h->lastFirstPass > 0 ? std::min(kMaxChunk, h->lastFirstPass) : kInnerChunk;
-    int passes = 0, inner = 0, status = NPPE_OK, launched = 0;
-    double bnorm = 0.0, rnorm = 0.0, rr0 = 0.0, perIter = 0.0;
-    bool first = true;
-    for (;;)
-    {
-        if (graphs && first && chunk > 1)
-        {
-            if (int rc = chunkGraph(chunk)) return rc;
+            if (!(h->chunkGraph && h->chunkGraphIters == R.chunk))
+            {
+                h->chunkGraphIters = 0;
+                if (int rc = capture_iterations(h, R.chunk, &h->chunkGraph, nullptr)) return rc;
+                h->chunkGraphIters = R.chunk;
+            }
             HIP_TRY(h, hipGraphLaunch(h->chunkGraph, h->stream), "graph launch");
-            perIter = h->graphBytesPerIter;
-            launched += chunk;
+            R.perIter = h->graphBytesPerIter;
+            R.launched += R.chunk;
         }
         else
-        {
-            for (int q = 0; q < chunk; q++)
+            for (int q = 0; q < R.chunk; q++)
             {
                 if (graphs)
                 {
                     HIP_TRY(h, hipGraphLaunch(h->innerGraph, h->stream), "graph launch");
-                    perIter = h->graphBytesPerIter;
+                    R.perIter = h->graphBytesPerIter;
                 }
                 else
                 {
-                    h->curIter = launched;
+                    h->curIter = R.launched;
                     Traffic it;
                     HIP_TRY(h, inner_iteration(h, it), "inner iteration");
-                    perIter = it.bytes;
+                    R.perIter = it.bytes;
                 }
-                launched++;
+                R.launched++;
             }
-        }
         h->curIter = -1;
-        {
-            Scope sc(h, NPPE_K_RESIDUAL64, 29.0 * N);
-            // (with the projection it also sums x over the unknowns for the mean)
-            HIP_TRY(h, launch_update_residual64(h->stream, L0.box, h->t64, L0.types, h->dX64[cur], nullptr, nullptr,
-                                                h->dB64, h->dR, h->dR64, h->dState, h->dPartials, 1, project ? 1 : 0),
-                    "true residual");
-        }
-        {
-            Scope sc(h, NPPE_K_GATHER, 20.0 * n);
-            HIP_TRY(h, launch_gather64(h->stream, h->n, h->dOrder, h->dX64[cur], dX, h->dState, project ? 2 : -1, 1),
-                    "gather x");
-        }
+        if (int rc = k64_tail(R)) return rc;
         if (int rc = sync_state(h)) return rc;
-        const SolveState &st = *h->hState;
-        profile_collect(h, st.iters);
-        if (first)
-        {
-            first = false;
-            bnorm = std::sqrt(st.sums[0]);
-            rr0 = st.rr0;
-            if (!std::isfinite(bnorm) || !std::isfinite(rr0))
-                return fail(h, NPPE_ERR_NONFINITE, "non-finite right-hand side or guess");
-            if (bnorm == 0.0)
-            {
-                // Eigen sets x = 0 for a zero right-hand side (ConjugateGradient.h:46-52)
-                HIP_TRY(h, launch_scatter64(h->stream, h->n, h->dOrder, nullptr, h->dX64[cur], h->dState, -1,
-                                            h->dPartials),
-                        "zero x");
-                HIP_TRY(h, launch_gather64(h->stream, h->n, h->dOrder, h->dX64[cur], dX, h->dState, -1), "gather x");
-                rnorm = 0.0;
-                break;
-            }
-        }
-        if (st.done == kDoneNonFinite)
-            return fail(h, NPPE_ERR_NONFINITE, "non-finite value in the Krylov recursion (inconsistent system?)");
-        if (st.iters > 0 && st.rr > 0.0 && st.rr < rr0) h->rate = std::pow(st.rr / rr0, 0.5 / st.iters);
-        if (!st.done)
-        {
-            // the prediction fell short: more iterations of the same pass
-            chunk = st.rr > 0.0 && st.thr2 > 0.0 ? predict(std::sqrt(st.thr2 / st.rr)) : kInnerChunk;
-            continue;
-        }
-        inner += st.iters;
-        tr.add(perIter * st.iters + 29.0 * N);
-        rnorm = std::sqrt(st.sums[1]);
-        if (!std::isfinite(rnorm)) return fail(h, NPPE_ERR_NONFINITE, "non-finite residual during refinement");
-        // a pass that ran iterations counts; the guess accepted as it is does not (as the synchronous loop)
-        if (!(passes == 0 && st.iters == 0 && st.accepted)) passes++;
-        if (passes <= 1) h->lastFirstPass = st.iters;
-        if (st.accepted)
-        {
-            tr.add(20.0 * n);   // the gather (the mean's sum came with the residual)
-            break;
-        }
-        if (passes >= h->opt.max_refinements || inner >= h->opt.max_iters)
-        {
-            status = NPPE_NOT_CONVERGED;
-            if (project)
-            {
-                Scope sc(h, NPPE_K_MEAN, 12.0 * n);
-                HIP_TRY(h, launch_sum64(h->stream, h->n, h->dOrder, h->dX64[cur], h->dState, 2, h->dPartials), "mean");
-            }
-            HIP_TRY(h, launch_gather64(h->stream, h->n, h->dOrder, h->dX64[cur], dX, h->dState, project ? 2 : -1),
-                    "gather x");
-            tr.add((project ? 12.0 * n : 0.0) + 20.0 * n);
-            break;
-        }
-        // the true residual missed the target: CG again from the new x
-        const double aim = 0.8 * tol * std::max(bnorm, refNorm);
-        HIP_TRY(h, launch_state_init(h->stream, h->dState, aim * aim, h->opt.max_iters - inner), "krylov setup");
-        rr0 = rnorm * rnorm;
-        chunk = predict(std::min(0.5, aim / rnorm));
-        launched = 0;
+        profile_collect(h, h->hState->iters);
+        if (int rc = k64_advance(R)) return rc;
     }
-    *passesOut = passes;
-    *innerOut = inner;
-    *statusOut = status;
-    *bnormOut = bnorm;
-    *rnormOut = rnorm;
+    *passesOut = R.passes;
+    *innerOut = R.inner;
+    *statusOut = R.status;
+    *bnormOut = R.bnorm;
+    *rnormOut = R.rnorm;
     return NPPE_OK;
 }
+
+}  // namespace
+
+// ---- batched solves -----------------------------------------------------------------------------------------------
+//
+// Several committed systems solved together from no guess: each system's inner iteration is recorded once
+// (set_launch_recorder) as its list of launches, the lists are merged by place in the iteration (the V-cycle's way
+// down by level, the coarsest solve, the way up, the Krylov update) and, at each place, by kernel, and every merged
+// group becomes one batched launch whose blocks cover all its systems (launch_batched). The batch's iteration is
+// captured as one graph. Each system keeps its own state, reductions and host schedule (K64Run): the set-up, the
+// gated true residual and gather stay per system, around chunks of batched iterations that the systems already done
+// skip on the device. Every kernel does, per system, what it does in the system's own launch, so each result is bit
+// for bit the result of the same solve alone.
+
+struct nppe::BatchPlan
+{
+    std::vector<std::pair<uint64_t, uint64_t>> key;   // (serial, graph epoch) of each handle, in order
+    bool eligible = false;
+    struct Launch
+    {
+        const void *fn;
+        int grid, threads, nsys;
+        const void *args;
+        const int *start;
+    };
+    std::vector<Launch> launches;
+    std::vector<double> perIter;   // algorithmic bytes of one inner iteration, per system
+    void *dMem = nullptr;
+    hipGraphExec_t iterGraph = nullptr, chunkGraph = nullptr;
+    int chunkIters = 0;
+};
+
+void nppe::free_batch_plan(BatchPlan *p)
+{
+    if (!p) return;
+    if (p->iterGraph) (void)hipGraphExecDestroy(p->iterGraph);
+    if (p->chunkGraph) (void)hipGraphExecDestroy(p->chunkGraph);
+    if (p->dMem)
+    {
+        std::lock_guard<std::mutex> lock(gCaptureMu);
+        (void)hipFree(p->dMem);
+    }
+    delete p;
+}
+
+namespace
+{
+
+// One batched launch being planned: records of several systems (one each) that become one launch.
+struct PlanGroup
+{
+    std::vector<LaunchRecord *> recs;
+    std::vector<unsigned char> host;   // the records' Args back to back, then (from startOff) the block offsets
+    size_t startOff = 0;
+    int grid = 0;
+
+    bool lay_out()
+    {
+        const size_t argSize = recs[0]->args.size();
+        startOff = (argSize * recs.size() + 255) & ~(size_t)255;
+        host.assign(startOff + sizeof(int) * (recs.size() + 1), 0);
+        int acc = 0;
+        for (size_t k = 0; k < recs.size(); k++)
+        {
+            const LaunchRecord &r = *recs[k];
+            if (r.args.size() != argSize || r.grid < 1 || r.fn != recs[0]->fn || r.threads != recs[0]->threads)
+                return false;
+            std::memcpy(host.data() + argSize * k, r.args.data(), argSize);
+            reinterpret_cast<int *>(host.data() + startOff)[k] = acc;
+            acc += r.grid;
+        }
+        reinterpret_cast<int *>(host.data() + startOff)[recs.size()] = acc;
+        grid = acc;
+        return true;
+    }
+};
+
+// A sweep or restriction group's tile shape chosen by timing: the systems' own tiles were chosen for each system
+// alone, where the launch is short of blocks and small chunks win; sharing a launch, the systems fill the device,
+// and fewer warm-up planes per block pay. Every candidate gives the same bits. The systems' states are set running
+// (k_state_init) so that the timed kernels do their work; every solve sets its state up again.
+int tune_group(nppe_solver *h0, PlanGroup &G, void *scratch)
+{
+    const LaunchRecord &r0 = *G.recs[0];
+    std::vector<int> tms;
+    if (r0.kind == REC_ROW_SWEEP) retile_rows(r0, tms);
+    else tms = {0};
+    const std::vector<int> ps = r0.kind == REC_ROW_SWEEP ? std::vector<int>{1, 2, 4, 8, 12, 16, 24, 32}
+                                                         : std::vector<int>{2, 4, 8, 16, 32};
+    const hipStream_t s = h0->stream;
+    hipEvent_t a, b;
+    HIP_TRY(h0, hipEventCreate(&a), "batch: tune");
+    HIP_TRY(h0, hipEventCreate(&b), "batch: tune");
+    float best = 0.f;
+    std::vector<LaunchRecord> keep;
+    for (int tm : tms)
+        for (int p : ps)
+        {
+            std::vector<LaunchRecord> cand;
+            for (LaunchRecord *r : G.recs) cand.push_back(*r);
+            bool ok = true;
+            for (LaunchRecord &r : cand) ok = ok && retile_record(r, tm, p);
+            if (!ok) continue;
+            PlanGroup T;
+            for (LaunchRecord &r : cand) T.recs.push_back(&r);
+            if (!T.lay_out()) continue;
+            HIP_TRY(h0, hipMemcpyAsync(scratch, T.host.data(), T.host.size(), hipMemcpyHostToDevice, s), "batch: tune");
+            const unsigned char *base = static_cast<const unsigned char *>(scratch);
+            const int *start = reinterpret_cast<const int *>(base + T.startOff);
+            const int nsys = (int)cand.size();
+            float ms = 0.f;
+            for (int rep = 0; rep < 3; rep++)
+            {
+                HIP_TRY(h0, hipEventRecord(a, s), "batch: tune");
+                HIP_TRY(h0, launch_batched(s, cand[0].fn, T.grid, cand[0].threads, scratch, start, nsys), "batch: tune");
+                HIP_TRY(h0, hipEventRecord(b, s), "batch: tune");
+                HIP_TRY(h0, hipEventSynchronize(b), "batch: tune");
+                float t = 0.f;
+                (void)hipEventElapsedTime(&t, a, b);
+                if (rep == 1 || (rep == 2 && t < ms)) ms = t;   // the first run warms up
+            }
+            if (keep.empty() || ms < best)
+            {
+                best = ms;
+                keep = cand;
+            }
+        }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (keep.empty()) return fail(h0, NPPE_ERR_INTERNAL, "batch: no tile shape for a sweep");
+    for (size_t k = 0; k < G.recs.size(); k++) *G.recs[k] = keep[k];
+    return NPPE_OK;
+}
+
+// The systems' inner iterations recorded and merged into batched launches. Leaves plan->eligible false when a
+// system's iteration runs a kernel without a batched form (a 2D, stretched-coefficient or long-row system).
+int build_batch_plan(nppe_solver *const *hs, int count, BatchPlan *plan)
+{
+    nppe_solver *h0 = hs[0];
+    std::vector<LaunchRecorder> recs(count);
+    plan->perIter.assign(count, 0.0);
+    for (int q = 0; q < count; q++)
+    {
+        nppe_solver *h = hs[q];
+        hipGraph_t g = nullptr;
+        Traffic it;
+        hipError_t ce, ee;
+        {
+            // recorded inside a capture: a launch without a batched form lands in the graph instead of running
+            std::lock_guard<std::mutex> lock(gCaptureMu);
+            HIP_TRY(h0, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "batch: record");
+            set_launch_recorder(&recs[q]);
+            ce = inner_iteration(h, it);
+            set_launch_recorder(nullptr);
+            ee = hipStreamEndCapture(h->stream, &g);
+        }
+        size_t nodes = 0;
+        if (g) (void)hipGraphGetNodes(g, nullptr, &nodes);
+        if (g) (void)hipGraphDestroy(g);
+        if (ce != hipSuccess) return hip_fail(h0, ce, "batch: record");
+        if (ee != hipSuccess) return hip_fail(h0, ee, "batch: record");
+        if (nodes != 0 || recs[q].recs.empty())
+        {
+            h0->err = "batch: system " + std::to_string(q) + " runs " + std::to_string(nodes) +
+                      " launches without a batched form (" + std::to_string(recs[q].recs.size()) + " with one)";
+            return NPPE_OK;   // not eligible
+        }
+        plan->perIter[q] = it.bytes;
+    }
+    // merge: by place in the iteration, then by kernel (the re-tileable kinds by variant, whatever their tiles)
+    struct Item
+    {
+        int phase, level, seq, q;
+        LaunchRecord *r;
+    };
+    std::vector<Item> items;
+    for (int q = 0; q < count; q++)
+        for (LaunchRecord &r : recs[q].recs)
+            items.push_back({r.phase, r.phase == SLOT_UP ? -r.level : r.level, r.seq, q, &r});
+    auto key = [](const Item &a) {
+        const bool fixed = a.r->kind == REC_FIXED;
+        return std::make_tuple(a.phase, a.level, a.seq, a.r->kind, a.r->k, a.r->flags,
+                               fixed ? reinterpret_cast<uintptr_t>(a.r->fn) : (uintptr_t)0, fixed ? a.r->threads : 0);
+    };
+    std::stable_sort(items.begin(), items.end(), [&](const Item &a, const Item &b) {
+        if (key(a) != key(b)) return key(a) < key(b);
+        return a.q < b.q;
+    });
+    std::vector<PlanGroup> groups;
+    for (size_t i = 0; i < items.size();)
+    {
+        size_t j = i;
+        PlanGroup G;
+        while (j < items.size() && key(items[j]) == key(items[i])) G.recs.push_back(items[j++].r);
+        groups.push_back(std::move(G));
+        i = j;
+    }
+    // tiles of the sweeps and restrictions by timing, on a scratch copy of each group's arguments
+    size_t scratchBytes = 256;
+    for (const PlanGroup &G : groups)
+        scratchBytes = std::max(scratchBytes, G.recs[0]->args.size() * G.recs.size() + 512 + 4 * G.recs.size());
+    void *scratch = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(gCaptureMu);
+        HIP_TRY(h0, hipMalloc(&scratch, scratchBytes), "batch: tune");
+    }
+    int rc = NPPE_OK;
+    for (int q = 0; q < count && rc == NPPE_OK; q++)
+        if (launch_state_init(h0->stream, hs[q]->dState, 0.0, 1) != hipSuccess)
+            rc = fail(h0, NPPE_ERR_DEVICE, "batch: tune");
+    for (PlanGroup &G : groups)
+        if (rc == NPPE_OK && G.recs[0]->kind != REC_FIXED) rc = tune_group(h0, G, scratch);
+    if (hipStreamSynchronize(h0->stream) != hipSuccess && rc == NPPE_OK) rc = fail(h0, NPPE_ERR_DEVICE, "batch: tune");
+    {
+        std::lock_guard<std::mutex> lock(gCaptureMu);
+        (void)hipFree(scratch);
+    }
+    if (rc != NPPE_OK) return rc;
+    // device memory: each group's Args then its offsets
+    size_t bytes = 0;
+    std::vector<size_t> offs;
+    for (PlanGroup &G : groups)
+    {
+        if (!G.lay_out()) return fail(h0, NPPE_ERR_INTERNAL, "batch: inconsistent group");
+        offs.push_back(bytes);
+        bytes = (bytes + G.host.size() + 255) & ~(size_t)255;
+    }
+    std::vector<unsigned char> host(bytes, 0);
+    for (size_t g = 0; g < groups.size(); g++)
+        std::memcpy(host.data() + offs[g], groups[g].host.data(), groups[g].host.size());
+    {
+        std::lock_guard<std::mutex> lock(gCaptureMu);
+        HIP_TRY(h0, hipMalloc(&plan->dMem, std::max<size_t>(bytes, 256)), "batch: plan memory");
+    }
+    HIP_TRY(h0, upload(h0, plan->dMem, host.data(), bytes), "batch: plan upload");
+    const unsigned char *base = static_cast<const unsigned char *>(plan->dMem);
+    for (size_t g = 0; g < groups.size(); g++)
+    {
+        const PlanGroup &G = groups[g];
+        plan->launches.push_back({G.recs[0]->fn, G.grid, G.recs[0]->threads, (int)G.recs.size(), base + offs[g],
+                                  reinterpret_cast<const int *>(base + offs[g] + G.startOff)});
+    }
+    plan->eligible = true;
+    return NPPE_OK;
+}
+
+hipError_t batch_iteration(const BatchPlan &p, hipStream_t s)
+{
+    for (const BatchPlan::Launch &l : p.launches)
+    {
+        const hipError_t e = launch_batched(s, l.fn, l.grid, l.threads, l.args, l.start, l.nsys);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+int capture_batch(nppe_solver *h0, const BatchPlan &p, int iters, hipGraphExec_t *out)
+{
+    if (*out)
+    {
+        (void)hipGraphExecDestroy(*out);
+        *out = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    std::lock_guard<std::mutex> lock(gCaptureMu);
+    HIP_TRY(h0, hipStreamBeginCapture(h0->stream, hipStreamCaptureModeThreadLocal), "batch: capture");
+    hipError_t ce = hipSuccess;
+    for (int q = 0; q < iters && ce == hipSuccess; q++) ce = batch_iteration(p, h0->stream);
+    const hipError_t ee = hipStreamEndCapture(h0->stream, &g);
+    if (ce != hipSuccess) return hip_fail(h0, ce, "batch: capture");
+    if (ee != hipSuccess) return hip_fail(h0, ee, "batch: capture");
+    const hipError_t ie = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) return hip_fail(h0, ie, "batch: instantiate");
+    return NPPE_OK;
+}
+
+// The batch's solves, every handle's work on the first handle's stream.
+int batch_run(nppe_solver *const *hs, const double *const *b, double *const *x, int count, BatchPlan &plan,
+              nppe_result *res, int *rcs)
+{
+    nppe_solver *h0 = hs[0];
+    const hipStream_t s = h0->stream;
+    const bool graphs = h0->useGraphs;
+    std::vector<K64Run> R(count);
+    std::vector<Traffic> tr(count);
+    hipEvent_t t0, t1;
+    HIP_TRY(h0, hipEventCreate(&t0), "event");
+    HIP_TRY(h0, hipEventCreate(&t1), "event");
+    HIP_TRY(h0, hipEventRecord(t0, s), "event");
+    for (int q = 0; q < count; q++)
+    {
+        nppe_solver *h = hs[q];
+        const double n = (double)h->n;
+        rcs[q] = NPPE_OK;
+        h->curIter = -1;
+        // no guess: x = 0, and its residual is b (solve_mgpcg's schedule)
+        HIP_TRY(h0, launch_scatter_rhs64(s, h->n, h->dOrder, b[q], h->dB64, h->dR64, h->dR, h->dX64[0], h->dState,
+                                         h->dPartials),
+                "batch: scatter b");
+        tr[q].add(40.0 * n);
+        K64Run &r = R[q];
+        r.h = h;
+        r.refNorm = 0.0;
+        r.cur = 0;
+        r.dX = x[q];
+        r.project = h->singular && h->opt.compatibility && h->n > 0;
+        r.tr = &tr[q];
+        if ((rcs[q] = k64_begin(r, true)) != NPPE_OK) r.finished = true;
+    }
+    bool firstRound = true;
+    for (;;)
+    {
+        int chunk = 0;
+        for (int q = 0; q < count; q++)
+            if (!R[q].finished) chunk = std::max(chunk, R[q].chunk);
+        if (chunk == 0) break;
+        if (graphs && firstRound && chunk > 1)
+        {
+            if (!(plan.chunkGraph && plan.chunkIters == chunk))
+            {
+                plan.chunkIters = 0;
+                if (int rc = capture_batch(h0, plan, chunk, &plan.chunkGraph)) return rc;
+                plan.chunkIters = chunk;
+            }
+            HIP_TRY(h0, hipGraphLaunch(plan.chunkGraph, s), "batch: graph launch");
+        }
+        else
+            for (int k = 0; k < chunk; k++)
+            {
+                if (graphs)
+                {
+                    if (!plan.iterGraph)
+                        if (int rc = capture_batch(h0, plan, 1, &plan.iterGraph)) return rc;
+                    HIP_TRY(h0, hipGraphLaunch(plan.iterGraph, s), "batch: graph launch");
+                }
+                else
+                    HIP_TRY(h0, batch_iteration(plan, s), "batch: iteration");
+            }
+        firstRound = false;
+        for (int q = 0; q < count; q++)
+        {
+            if (R[q].finished) continue;
+            R[q].perIter = plan.perIter[q];
+            R[q].launched += chunk;
+            if ((rcs[q] = k64_tail(R[q])) != NPPE_OK)
+            {
+                R[q].finished = true;
+                continue;
+            }
+            HIP_TRY(h0, hipMemcpyAsync(hs[q]->hState, hs[q]->dState, sizeof(SolveState), hipMemcpyDeviceToHost, s),
+                    "batch: state copy");
+        }
+        HIP_TRY(h0, hipStreamSynchronize(s), "batch: synchronize");
+        for (int q = 0; q < count; q++)
+        {
+            if (R[q].finished) continue;
+            if ((rcs[q] = k64_advance(R[q])) != NPPE_OK) R[q].finished = true;
+        }
+    }
+    HIP_TRY(h0, hipEventRecord(t1, s), "event");
+    HIP_TRY(h0, hipEventSynchronize(t1), "event");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    for (int q = 0; q < count; q++)
+    {
+        if (rcs[q] == NPPE_OK) rcs[q] = R[q].status;
+        if (!res) continue;
+        nppe_result &o = res[q];
+        o = nppe_result{};
+        const double denom = R[q].bnorm;
+        o.iterations = R[q].inner;
+        o.refinements = R[q].passes;
+        o.status = rcs[q];
+        o.levels = (int)hs[q]->levels.size();
+        o.rhs_norm = R[q].bnorm;
+        o.residual_norm = R[q].rnorm;
+        o.relative_residual = denom > 0.0 ? R[q].rnorm / denom : R[q].rnorm;
+        o.device_ms = ms;   // the whole batch's device time
+        o.algorithmic_bytes = tr[q].bytes;
+    }
+    return NPPE_OK;
+}
+
+}  // namespace
+
+int nppe::batch_solve_krylov64(nppe_solver *const *hs, const double *const *b, double *const *x, int count,
+                               nppe_result *res, int *rcs)
+{
+    if (count < 2 || !hs[0]) return 1;
+    nppe_solver *h0 = hs[0];
+    for (int q = 0; q < count; q++)
+    {
+        const nppe_solver *h = hs[q];
+        bool ok = h && h->committed && h->n > 0 && b[q] && x[q] && h->krylov64 &&
+                  h->opt.method != NPPE_METHOD_CG64 && h->opt.max_iters > 0 && h->opt.max_refinements > 0 &&
+                  !h->profiling && h->device == h0->device && h->useGraphs == h0->useGraphs;
+        for (int k = 0; k < q && ok; k++) ok = hs[k] != h;
+        if (!ok)
+        {
+            h0->err = "batch: system " + std::to_string(q) + " is solved on its own (not an FP64-Krylov MGPCG solve "
+                      "of a distinct committed handle on the first handle's device)";
+            return 1;
+        }
+    }
+    HIP_TRY(h0, hipSetDevice(h0->device), "set device");
+    std::vector<std::pair<uint64_t, uint64_t>> key(count);
+    for (int q = 0; q < count; q++) key[q] = {hs[q]->serial, hs[q]->graphEpoch};
+    if (!h0->batch || h0->batch->key != key)
+    {
+        free_batch_plan(h0->batch);
+        h0->batch = new BatchPlan();
+        h0->batch->key = key;
+        // pending work of every handle first (the recording captures on each handle's stream)
+        for (int q = 0; q < count; q++) HIP_TRY(h0, hipStreamSynchronize(hs[q]->stream), "batch: synchronize");
+        if (int rc = build_batch_plan(hs, count, h0->batch))
+        {
+            free_batch_plan(h0->batch);
+            h0->batch = nullptr;
+            return rc;
+        }
+    }
+    if (!h0->batch->eligible) return 1;
+    // every handle's work goes to the first handle's stream for the batch (restored on every return)
+    std::vector<hipStream_t> own(count);
+    for (int q = 0; q < count; q++)
+    {
+        own[q] = hs[q]->stream;
+        if (own[q] != h0->stream) HIP_TRY(h0, hipStreamSynchronize(own[q]), "batch: synchronize");
+        hs[q]->err.clear();
+    }
+    for (int q = 1; q < count; q++) hs[q]->stream = h0->stream;
+    const int rc = batch_run(hs, b, x, count, *h0->batch, res, rcs);
+    for (int q = 1; q < count; q++) hs[q]->stream = own[q];
+    return rc;
+}
+
+namespace
+{
 
 int solve_mgpcg(nppe_solver *h, const double *dB, double *dX, const double *dGuess, double refNorm, nppe_result *res)
 {
@@ -1637,6 +2124,10 @@ int nppe_create(nppe_solver **out, const nppe_options *opt)
     }
     h->opt = o;
     h->device = o.device;
+    {
+        static std::atomic<uint64_t> serials(0);
+        h->serial = ++serials;
+    }
     // non-blocking: no implicit synchronisation with the null stream (default priority)
     std::unique_lock<std::mutex> devLock(gCaptureMu);
     if ((e = hipSetDevice(h->device)) != hipSuccess ||
@@ -1672,6 +2163,8 @@ int nppe_destroy(nppe_solver *h)
     if (!h) return NPPE_OK;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    free_batch_plan(h->batch);
+    h->batch = nullptr;
     free_grid(h);
     for (hipEvent_t e : h->evPool) (void)hipEventDestroy(e);
     {

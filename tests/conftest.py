@@ -21,6 +21,15 @@ def _has_gpu():
         return False
 
 
+@pytest.fixture(scope="session", autouse=True)
+def _torch_before_library(request):
+    """When GPU tests are selected, torch's HIP runtime starts before the library's: tests that hand torch device
+    tensors to the library need torch to have found the device first (started after the library, it finds none)."""
+    if any(item.get_closest_marker("gpu") for item in request.session.items):
+        _has_gpu()
+    yield
+
+
 @pytest.fixture(scope="session")
 def gpu():
     if not _has_gpu():

@@ -65,6 +65,17 @@ def test_batch_32_free_surface_64_to_160(gpu):
         assert rc == 0
         for (x, res), xa in zip(out, alone):
             assert np.array_equal(x, xa)
+        # device vectors: the inner iterations batched into shared launches
+        import torch
+        tb = [torch.from_numpy(b).cuda() for b in bs]
+        tx = [torch.zeros_like(t) for t in tb]
+        torch.cuda.synchronize()
+        res, rc = N.batch_solve_device(solvers, [t.data_ptr() for t in tb], [t.data_ptr() for t in tx])
+        torch.cuda.synchronize()
+        assert rc == 0 and len({r.device_ms for r in res}) == 1
+        say(f"[batch] batched launches: {res[0].device_ms:.1f} ms for the 32 solves")
+        for t, xa in zip(tx, alone):
+            assert np.array_equal(t.cpu().numpy(), xa)
     finally:
         for h in solvers:
             h.close()

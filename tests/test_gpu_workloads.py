@@ -40,10 +40,13 @@ def test_flip_free_surface_to_1e8():
     assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-5
 
 
-def _mix(count=6, seed=14):
+def _mix(count=6, seed=14, shapes=None):
     rng = np.random.default_rng(seed)
-    grids = [W.flip_free_surface(shape, rng) for shape in W.batch_shapes(count, rng, 24, 48)]
+    drawn = W.batch_shapes(count, rng, 24, 48)
+    grids = [W.flip_free_surface(shape, rng) for shape in drawn]
     bs = [W.random_rhs(g, rng, False) for g in grids]
+    if shapes is not None:
+        shapes.extend(drawn)
     return grids, bs
 
 
@@ -65,6 +68,82 @@ def test_batch_equals_alone():
         assert np.array_equal(x, xa)
     for h in solvers:
         h.close()
+
+
+def _device_batch(solvers, bs):
+    import torch
+    tb = [torch.from_numpy(b).cuda() for b in bs]
+    tx = [torch.zeros_like(t) for t in tb]
+    torch.cuda.synchronize()
+    res, rc = N.batch_solve_device(solvers, [t.data_ptr() for t in tb], [t.data_ptr() for t in tx])
+    torch.cuda.synchronize()
+    return [t.cpu().numpy() for t in tx], res, rc
+
+
+def _alone(grids, bs):
+    out = []
+    for g, b in zip(grids, bs):
+        with N.Solver() as h:
+            h.commit(g)
+            x, res, rc = h.solve(b)
+            assert rc == 0
+            out.append((x, res))
+    return out
+
+
+def test_batched_launches_equal_alone():
+    """Device vectors: the systems' inner iterations share launches (one per V-cycle place and kernel); every result
+    is the alone result bit for bit, on the call that builds the batch's plan, on the call that reuses it, and after
+    one system's layout changed (the plan is rebuilt)."""
+    shapes = []
+    grids, bs = _mix(8, 16, shapes)
+    alone = _alone(grids, bs)
+    solvers = [N.Solver() for _ in grids]
+    try:
+        for h, g in zip(solvers, grids):
+            h.commit(g)
+        for call in range(2):
+            xs, res, rc = _device_batch(solvers, bs)
+            assert rc == 0
+            # one batch on one stream: every system reports the batch's device time
+            assert len({r.device_ms for r in res}) == 1
+            for x, r, (xa, ra) in zip(xs, res, alone):
+                assert np.array_equal(x, xa), call
+                assert r.iterations == ra.iterations and r.relative_residual == ra.relative_residual
+        g2 = W.flip_free_surface(shapes[3], np.random.default_rng(99))
+        solvers[3].update_layout(g2.types)
+        b2 = W.random_rhs(g2, np.random.default_rng(98), False)
+        assert b2.size == solvers[3].n
+        grids[3], bs[3] = g2, b2
+        alone[3] = _alone([g2], [b2])[0]
+        xs, res, rc = _device_batch(solvers, bs)
+        assert rc == 0
+        for x, (xa, _) in zip(xs, alone):
+            assert np.array_equal(x, xa)
+    finally:
+        for h in solvers:
+            h.close()
+
+
+def test_batched_launches_fall_back_for_other_systems():
+    """A batch holding a system without the batched kernels (a 2D grid) is solved system by system, still equal to
+    alone."""
+    grids, bs = _mix(3, 17)
+    g2 = W.serving_grid(2, 96, "outlet", seed=1)
+    grids.append(g2)
+    bs.append(W.random_rhs(g2, np.random.default_rng(5), False))
+    alone = _alone(grids, bs)
+    solvers = [N.Solver() for _ in grids]
+    try:
+        for h, g in zip(solvers, grids):
+            h.commit(g)
+        xs, res, rc = _device_batch(solvers, bs)
+        assert rc == 0
+        for x, (xa, _) in zip(xs, alone):
+            assert np.array_equal(x, xa)
+    finally:
+        for h in solvers:
+            h.close()
 
 
 def test_graph_replay_equals_direct_launches():
