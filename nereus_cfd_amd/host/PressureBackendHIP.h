@@ -14,11 +14,15 @@
 #ifndef NEREUS_PRESSURE_BACKEND_HIP_H
 #define NEREUS_PRESSURE_BACKEND_HIP_H
 
+#include <algorithm>
 #include <cctype>
+#include <climits>
+#include <cstdlib>
 #include <cstdint>
 #include <iostream>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/nppe.h"
@@ -48,56 +52,87 @@ inline std::string Trim(const std::string &s)
     return s.substr(a, b - a);
 }
 
-// Splits the block on newlines and commas, drops blanks and # comments, reads the hip_* keys (values are
-// case-insensitive); unknown hip_* keys and bad values warn and keep the default, as ReadEigenSettings does.
+// The PRESSURE block as (key, VALUE) entries: entries end at a newline or a comma, '#' starts a comment, an entry
+// without '=' is skipped; keys are trimmed, values trimmed and upper-cased (values are case-insensitive).
+inline std::vector<std::pair<std::string, std::string>> PressureEntries(const std::string &text)
+{
+    std::vector<std::pair<std::string, std::string>> entries;
+    size_t begin = 0;
+    while (begin <= text.size())
+    {
+        size_t end = text.find_first_of(",\n", begin);
+        if (end == std::string::npos) end = text.size();
+        std::string entry = text.substr(begin, end - begin);
+        entry.erase(std::min(entry.size(), entry.find('#')));
+        const size_t eq = entry.find('=');
+        if (eq != std::string::npos)
+        {
+            std::string value = Trim(entry.substr(eq + 1));
+            for (char &c : value) c = (char)std::toupper((unsigned char)c);
+            entries.emplace_back(Trim(entry.substr(0, eq)), value);
+        }
+        begin = end + 1;
+    }
+    return entries;
+}
+
+// The hip_* keys of the PRESSURE block (the other backends' keys are theirs). A key this backend does not know, a
+// word it does not know or a number that does not parse (or is out of range) keeps the default and says so on
+// stderr; reading never throws.
 inline HipSettings ReadHipSettings(const std::string &text)
 {
     HipSettings s;
-    std::string token;
-    for (size_t n = 0; n <= text.size(); n++)
+    auto ignored = [](const std::string &key, const std::string &value, const char *why) {
+        std::cerr << "[nereus_hip] PRESSURE " << key << " = '" << value << "' ignored: " << why << std::endl;
+    };
+    auto real = [](const std::string &v, double &out) {
+        char *end = nullptr;
+        const double d = std::strtod(v.c_str(), &end);
+        if (v.empty() || *end != '\0') return false;
+        out = d;
+        return true;
+    };
+    auto whole = [](const std::string &v, int &out) {
+        char *end = nullptr;
+        const long d = std::strtol(v.c_str(), &end, 10);
+        if (v.empty() || *end != '\0' || d < INT32_MIN || d > INT32_MAX) return false;
+        out = (int)d;
+        return true;
+    };
+    // pre/post smoothing: a sweep count or AUTO
+    auto sweeps = [&](const std::string &v, int &out) {
+        if (v == "AUTO")
+        {
+            out = NPPE_SMOOTH_AUTO;
+            return true;
+        }
+        return whole(v, out);
+    };
+    for (const auto &kv : PressureEntries(text))
     {
-        const char c = n < text.size() ? text[n] : '\n';
-        if (c != ',' && c != '\n')
+        const std::string &key = kv.first, &v = kv.second;
+        if (key.compare(0, 4, "hip_") != 0) continue;
+        bool ok = true;
+        if (key == "hip_solver")
         {
-            token.push_back(c);
-            continue;
+            if (v == "MGPCG" || v == "CG") s.method = v == "CG" ? NPPE_METHOD_CG64 : NPPE_METHOD_MGPCG;
+            else ignored(key, v, "expected MGPCG or CG; MGPCG kept");
         }
-        const size_t hash = token.find('#');
-        if (hash != std::string::npos) token.erase(hash);
-        const std::string t = Trim(token);
-        token.clear();
-        const size_t eq = t.find('=');
-        if (t.empty() || eq == std::string::npos || t.rfind("hip_", 0) != 0) continue;
-        const std::string key = Trim(t.substr(0, eq));
-        std::string val = Trim(t.substr(eq + 1));
-        for (char &ch : val) ch = (char)std::toupper((unsigned char)ch);
-        try
+        else if (key == "hip_krylov")
         {
-            if (key == "hip_solver")
-            {
-                if (val == "MGPCG") s.method = NPPE_METHOD_MGPCG;
-                else if (val == "CG") s.method = NPPE_METHOD_CG64;
-                else std::cerr << "Warning: unknown hip_solver '" << val << "', using MGPCG." << std::endl;
-            }
-            else if (key == "hip_tolerance") s.tolerance = std::stod(val);
-            else if (key == "hip_max_iters") s.maxIters = std::stoi(val);
-            else if (key == "hip_pre_smooth") s.preSmooth = val == "AUTO" ? NPPE_SMOOTH_AUTO : std::stoi(val);
-            else if (key == "hip_post_smooth") s.postSmooth = val == "AUTO" ? NPPE_SMOOTH_AUTO : std::stoi(val);
-            else if (key == "hip_krylov")
-            {
-                if (val == "FP64") s.krylov = NPPE_KRYLOV_FP64;
-                else if (val == "REFINE") s.krylov = NPPE_KRYLOV_REFINE;
-                else std::cerr << "Warning: unknown hip_krylov '" << val << "', using FP64." << std::endl;
-            }
-            else if (key == "hip_inner_reduction") s.innerReduction = std::stod(val);
-            else if (key == "hip_device") s.device = std::stoi(val);
-            else std::cerr << "Warning: unknown pressure key '" << key << "'." << std::endl;
+            if (v == "FP64" || v == "REFINE") s.krylov = v == "REFINE" ? NPPE_KRYLOV_REFINE : NPPE_KRYLOV_FP64;
+            else ignored(key, v, "expected FP64 or REFINE; FP64 kept");
         }
-        catch (...)
-        {
-            std::cerr << "Warning: could not read " << key << " = " << val << "." << std::endl;
-        }
+        else if (key == "hip_tolerance") ok = real(v, s.tolerance);
+        else if (key == "hip_inner_reduction") ok = real(v, s.innerReduction);
+        else if (key == "hip_max_iters") ok = whole(v, s.maxIters);
+        else if (key == "hip_device") ok = whole(v, s.device);
+        else if (key == "hip_pre_smooth") ok = sweeps(v, s.preSmooth);
+        else if (key == "hip_post_smooth") ok = sweeps(v, s.postSmooth);
+        else ignored(key, v, "not a key of this backend");
+        if (!ok) ignored(key, v, "not a number; default kept");
     }
+    // values outside their ranges fall back to the defaults
     if (!(s.tolerance > 0.0)) s.tolerance = 1e-8;
     if (s.maxIters <= 0) s.maxIters = 2000;
     if (s.preSmooth < 1 && s.preSmooth != NPPE_SMOOTH_AUTO) s.preSmooth = NPPE_SMOOTH_AUTO;
