@@ -19,7 +19,7 @@ namespace nppe
 // (launch_batched), whose block g - start[q] of system q does what that block does in the system's own launch.
 // Kinds of records: a sweep or a restriction gives the same bits for any tiling, so a batch may re-tile it
 // (retile_record); the other kernels' reductions follow their grid, which stays as recorded.
-enum { REC_FIXED = 0, REC_ROW_SWEEP = 1, REC_RESTRICT_ROW = 2, REC_RESTRICT_MARCH = 3 };
+enum { REC_FIXED = 0, REC_ROW_SWEEP = 1, REC_RESTRICT_ROW = 2, REC_RESTRICT_MARCH = 3, REC_SWEEP_RESTRICT = 4 };
 struct LaunchRecord
 {
     const void *fn = nullptr;   // the batched kernel: kernel(const Args *args, const int *start, int nsys)
@@ -112,6 +112,17 @@ int64_t rbgs_blocks(const Box &B, int family, int tileRows, int planesPerChunk, 
 int rbgs_row_tile(const Box &B, int tileRows, bool folding, bool uniform);
 hipError_t launch_residual_restrict(hipStream_t s, const Level32 &Lf, const float *x, const float *b,
                                     const Level32 &Lc, float *bc, const SolveState *st);
+// A plain whole-row sweep x_out = S(x_in) and the restriction of its residual b - A x_out into bc in one march
+// (the last pre-smoothing sweep of a uniform 3D level with rows of at most 256 cells): x_out and bc bit for bit
+// those of launch_rbgs followed by launch_residual_restrict. Tiles: the largest of rbgs_restrict_rows at most
+// tileRows, planesPerChunk made even. rbgs_restrict_fusable says whether the level has it (NPPE_FUSE_RESTRICT=0
+// turns it off: the tests compare the two).
+bool rbgs_restrict_fusable(const Level32 &Lf, const float *xin, const float *xout, const float *b, const float *bc);
+void rbgs_restrict_rows(int k, std::vector<int> &tms);
+hipError_t launch_rbgs_restrict(hipStream_t s, const Level32 &Lf, const float *xin, float *xout, const float *b,
+                                int c1, const Level32 &Lc, float *bc, int tileRows, int planesPerChunk,
+                                const SolveState *st);
+int64_t rbgs_restrict_blocks(const Box &B, int tileRows, int planesPerChunk);
 hipError_t launch_prolong(hipStream_t s, const Level32 &Lf, float *x, const Level32 &Lc, const float *xc,
                           const SolveState *st);
 // Coarsest level: x[order[i]] = sum_j M[i][j] b[order[j]] in FP64 (M: the dense pseudo-inverse, n x n).

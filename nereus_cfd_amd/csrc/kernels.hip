@@ -2654,6 +2654,328 @@ __device__ __forceinline__ void restrict_row_body(const Level32 &Lf, const float
     family(R.s1);
 }
 
+// The last pre-smoothing sweep and the restriction of its residual in one march (3D uniform levels, rows of at most
+// 256 cells): k_rbgs_row's plain sweep on a block of TM rows with two halo rows each side, and two planes behind it
+// k_restrict_row's residual and family sums, on the sweep's results while they are still in registers and LDS. The
+// outer halo rows compute the first colour only (as the sweep's halo rows do), the inner ones both colours without
+// storing them, so the residual of the block's first and last rows has its row neighbours; each chunk also sweeps the
+// plane before and after it (not stored) for the residual of its first and last plane. Every value is computed by
+// the same operations as in the two kernels: x and the coarse right-hand side are bit for bit theirs, for one pass
+// over x, b and codes instead of two (TM and P even, so families lie within a block and a chunk).
+template <int TM, int K>
+__device__ __forceinline__ void rbgs_restrict_row_body(const Level32 &L, const float *__restrict__ xin,
+                                                       float *__restrict__ xout, const float *__restrict__ b, int c1,
+                                                       const Level32 &Lc, float *__restrict__ bc, int P, int tilesM,
+                                                       const SolveState *st, int lb)
+{
+    static_assert((TM & 1) == 0, "rows are summed in pairs");
+    if (st->done) return;
+    const Box &B = L.B;
+    constexpr int W = TM + 4;   // waves: rows m0 - 2 .. m0 + TM + 1
+    constexpr int kCols = 128 * K + 2;
+    constexpr int kSlots = 64 * K + 2;
+    __shared__ float red[2][W][kCols];        // first-colour values by column f
+    __shared__ float old[2][W][2][kSlots];    // old x of a plane by parity, row, element, pair + 1
+    __shared__ float fin[4][W][2][kSlots];    // the swept x of a plane by slot (plane - sp0) & 3, row, element, pair + 1
+    __shared__ float ldsR[2][TM][64 * K];     // the interior rows' pair sums of the residual by plane parity
+    __shared__ float2 tab[128];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int tm = lb % tilesM, chunk = lb / tilesM;
+    const int m0 = 1 + tm * TM;
+    const int m = m0 - 2 + wave;
+    const bool outerRow = wave == 0 || wave == W - 1;
+    const bool interior = wave >= 2 && wave <= W - 3;
+    const int i = wave - 2;                       // interior row index of the block
+    const bool famRow = interior && !(i & 1);     // the odd row of a row pair: writes the family sums
+    const bool rowIn = m >= 1 && m <= B.nm - 2;
+    const bool ragged = B.nf - 2 < 128 * K;
+    const int mc = min(max(m, 0), B.nm - 1);
+    const int mo = min(max(wave == 0 ? m - 1 : m + 1, 0), B.nm - 1);
+    const int s0 = 1 + chunk * P, s1 = min(s0 + P - 1, B.ns - 2);
+    const uint32_t sM = (uint32_t)B.sM;
+    const int fcMax = ((B.nf - 2) & 1) ? B.nf - 2 : B.nf - 3;
+    uint32_t off[K], offO[K];
+    bool okA[K], okB[K];
+    unsigned cm[K];
+    int fa[K];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+    {
+        fa[k] = 1 + 2 * (lane + 64 * k);
+        okA[k] = rowIn && fa[k] <= B.nf - 2;
+        okB[k] = rowIn && fa[k] + 1 <= B.nf - 2;
+        cm[k] = (okA[k] ? 0xFFu : 0u) | (okB[k] ? 0xFF00u : 0u);
+        const int fc = min(fa[k], fcMax);
+        off[k] = (uint32_t)mc * sM + (uint32_t)fc;
+        offO[k] = (uint32_t)mo * sM + (uint32_t)fc;
+    }
+    for (int q = threadIdx.x; q < 2 * W * kCols; q += 64 * W) (&red[0][0][0])[q] = 0.f;
+    for (int q = threadIdx.x; q < 2 * W * 2 * kSlots; q += 64 * W) (&old[0][0][0][0])[q] = 0.f;
+    for (int q = threadIdx.x; q < 4 * W * 2 * kSlots; q += 64 * W) (&fin[0][0][0][0])[q] = 0.f;
+    if (threadIdx.x < 128)
+    {
+        const unsigned c = threadIdx.x;
+        tab[c] = make_float2((c & 64u) ? inv_count(c) : 0.f, __uint_as_float((c & 64u) ? ~0u : 0u));
+    }
+    auto plane = [&](const float *base, int pl) { return base + (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
+    auto planeC = [&](int pl) { return L.c8 + (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
+    auto plane_off = [&](int pl) { return (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
+    auto pair = [](const float *row, uint32_t o) { return *reinterpret_cast<const float2 *>(row + o); };
+    auto pairC = [](const uint8_t *row, uint32_t o) { return (unsigned)*reinterpret_cast<const uint16_t *>(row + o); };
+    auto gs = [&](float acc, float bv, unsigned code) {
+        const float2 t = tab[code & 127u];
+        return __uint_as_float(__float_as_uint((acc - bv * L.icu) * t.x) & __float_as_uint(t.y));
+    };
+    auto gsDirichlet = [&](float acc, float bv, int pl, uint32_t o) {
+        const unsigned c16 = L.c16[(size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS + o];
+        float d = 0.f;
+#pragma unroll
+        for (int q = 0; q < 6; q++)
+        {
+            const unsigned stt = (c16 >> (2 * q)) & 3u;
+            d += stt == 3u ? 2.f : (stt != 0u ? 1.f : 0.f);
+        }
+        return d != 0.f ? (acc - bv * L.icu) / d : 0.f;
+    };
+    // slots by plane mod 4 relative to sp0 (as k_rbgs_row): x, b and codes two planes ahead, the outer rows one
+    float2 X[4][K], O[2][K], Bv[4][K];
+    unsigned C[4][K];
+    float R[4][K];    // first-colour value of the pair
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            X[j][k] = Bv[j][k] = make_float2(0.f, 0.f);
+            O[j & 1][k] = make_float2(0.f, 0.f);
+            R[j][k] = 0.f;
+            C[j][k] = 0u;
+        }
+    float acc[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) acc[k] = 0.f;
+    const int sp0 = s0 - 2;
+#pragma unroll
+    for (int d = -1; d <= 1; d++)
+    {
+        const float *px = plane(xin, sp0 + d);
+#pragma unroll
+        for (int k = 0; k < K; k++) X[(d + 4) & 3][k] = pair(px, off[k]);
+    }
+    if (outerRow)
+    {
+        const float *px = plane(xin, sp0);
+#pragma unroll
+        for (int k = 0; k < K; k++) O[0][k] = pair(px, offO[k]);
+    }
+#pragma unroll
+    for (int d = 0; d <= 1; d++)
+    {
+        const float *pb = plane(b, sp0 + d);
+        const uint8_t *pc = planeC(sp0 + d);
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            Bv[d][k] = pair(pb, off[k]);
+            C[d][k] = pairC(pc, off[k]);
+        }
+    }
+    __syncthreads();
+    const int wUp = min(wave + 1, W - 1), wDn = max(wave - 1, 0);
+    // the family sums of plane q from the rows' pair sums in ldsR[q & 1]
+    auto family = [&](int q) {
+        if (!famRow || q < s0 || q > s1) return;
+        const size_t cpl = (size_t)((q + 1) >> 1) * (size_t)Lc.B.sS + (size_t)((m + 1) >> 1) * (size_t)Lc.B.sM;
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            const int pq = lane + 64 * k;
+            acc[k] += ldsR[q & 1][i][pq] + ldsR[q & 1][i + 1][pq];
+            if (!(q & 1) || q == s1)
+            {
+                if (okA[k]) bc[cpl + pq + 1] = acc[k];
+                acc[k] = 0.f;
+            }
+        }
+    };
+    // One plane step at sp: first colour of plane sp, second colour of plane q = sp - 1 (stored on the interior rows
+    // inside the chunk), residual of plane t = sp - 2 (interior rows), family sums of plane t - 1.
+    auto step = [&](int sp, auto phase, auto colour) {
+        constexpr int U = decltype(phase)::value;
+        constexpr int E = decltype(colour)::value;
+        constexpr int Sm2 = (U + 2) & 3, Sm1 = (U + 3) & 3, S0 = U & 3, S1 = (U + 1) & 3, S2 = (U + 2) & 3;
+        constexpr int O0 = U & 1, O1 = (U + 1) & 1;
+        // x two planes ahead now; b and codes of plane sp + 2 at the end of the step, once the residual of plane
+        // t = sp - 2 has used that slot
+        {
+            const float *px = plane(xin, sp + 2);
+#pragma unroll
+            for (int k = 0; k < K; k++) X[S2][k] = pair(px, off[k]);
+            if (outerRow)
+            {
+                const float *po = plane(xin, sp + 1);
+#pragma unroll
+                for (int k = 0; k < K; k++) O[O1][k] = pair(po, offO[k]);
+            }
+        }
+        float(*ob)[2][kSlots] = old[sp & 1];
+        if (ragged)
+        {
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                ob[wave][0][1 + lane + 64 * k] = okA[k] ? X[S0][k].x : 0.f;
+                ob[wave][1][1 + lane + 64 * k] = okB[k] ? X[S0][k].y : 0.f;
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                ob[wave][0][1 + lane + 64 * k] = X[S0][k].x;
+                ob[wave][1][1 + lane + 64 * k] = X[S0][k].y;
+            }
+        }
+        // the one barrier of the step: old[sp & 1], red[(sp - 1) & 1], fin of plane sp - 2 and ldsR[(sp - 3) & 1],
+        // written in this step or the one before, are complete; their readers of a step before are done (fin holds
+        // four planes: a row's own residual reads its planes t - 1 .. t + 1, the last written by itself this step)
+        __syncthreads();
+        family(sp - 3);
+        float rnew[K];
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            const unsigned code = ((C[S0][k] & cm[k]) >> (8 * E)) & 255u;
+            const float bv = E ? Bv[S0][k].y : Bv[S0][k].x;
+            const int p = 1 + lane + 64 * k;
+            const float own = E ? X[S0][k].x : X[S0][k].y;
+            const float nbr = E ? ob[wave][0][p + 1] : ob[wave][1][p - 1];
+            float up = ob[wUp][E][p], dn = ob[wDn][E][p];
+            const float outer = E ? O[O0][k].y : O[O0][k].x;
+            if (wave == W - 1) up = outer;
+            if (wave == 0) dn = outer;
+            // (no leading 0 + : an iterate is never -0, so the sum is the same)
+            float a = E ? X[S1][k].y : X[S1][k].x;
+            a += E ? X[Sm1][k].y : X[Sm1][k].x;
+            a += up;
+            a += dn;
+            a += E ? nbr : own;
+            a += E ? own : nbr;
+            float v = gs(a, bv, code);
+            if (code & 128u) v = gsDirichlet(a, bv, sp, off[k] + (uint32_t)E);
+            rnew[k] = v;
+            R[S0][k] = v;
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) red[sp & 1][wave][fa[k] + E] = rnew[k];
+        const int q = sp - 1;
+        if (!outerRow && q >= s0 - 1 && q <= s1 + 1)
+        {
+            // second colour at plane q (there the second-colour element is E)
+            float *po = xout + (size_t)q * (size_t)B.sS;
+            const float(*rq)[kCols] = red[q & 1];
+            const bool store = interior && q >= s0 && q <= s1;
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                const unsigned cc = C[Sm1][k] & cm[k];
+                const unsigned code = (cc >> (8 * E)) & 255u;
+                const int f = fa[k] + E;
+                const float bv = E ? Bv[Sm1][k].y : Bv[Sm1][k].x;
+                float a = R[S0][k];
+                a += R[Sm2][k];
+                a += rq[wave + 1][f];
+                a += rq[wave - 1][f];
+                a += E ? rq[wave][f + 1] : R[Sm1][k];
+                a += E ? R[Sm1][k] : rq[wave][f - 1];
+                float out = gs(a, bv, code);
+                if (code & 128u) out = gsDirichlet(a, bv, q, off[k] + (uint32_t)E);
+                const float va = E ? R[Sm1][k] : out, vb = E ? out : R[Sm1][k];
+                fin[Sm1][wave][0][1 + lane + 64 * k] = va;
+                fin[Sm1][wave][1][1 + lane + 64 * k] = vb;
+                if (store)
+                {
+                    if (okB[k])
+                        *reinterpret_cast<float2 *>(po + off[k]) = make_float2(va, vb);
+                    else if (okA[k])
+                        po[off[k]] = va;
+                }
+            }
+        }
+        const int t = sp - 2;
+        if (interior && t >= s0 && t <= s1)
+        {
+            // residual of plane t (k_restrict_row's operations) and the row's pair sums
+            const float(*ft)[2][kSlots] = fin[Sm2];
+            const float(*fp)[2][kSlots] = fin[Sm1];   // plane t + 1
+            const float(*fm)[2][kSlots] = fin[S1];    // plane t - 1
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                const int p = 1 + lane + 64 * k;
+                float r[2];
+#pragma unroll
+                for (int e = 0; e < 2; e++)
+                {
+                    const unsigned code = ((C[S2][k] & cm[k]) >> (8 * e)) & 255u;
+                    const float xc = ft[wave][e][p];
+                    const float nb[6] = {fp[wave][e][p], fm[wave][e][p], ft[wave + 1][e][p], ft[wave - 1][e][p],
+                                         e ? ft[wave][0][p + 1] : ft[wave][1][p], e ? ft[wave][0][p] : ft[wave][1][p - 1]};
+                    const float bv = e ? Bv[S2][k].y : Bv[S2][k].x;
+                    float v = 0.f;
+                    if (code & 64u)
+                    {
+                        if (!(code & 128u))
+                        {
+#pragma clang fp contract(off)
+                            const float s6 = nb[0] + nb[1] + nb[2] + nb[3] + nb[4] + nb[5];
+                            const float ax = L.cu * __builtin_fmaf(-(float)__popc(code & 63u), xc, s6);
+                            v = bv - ax;
+                        }
+                        else
+                            v = bv - apply32v(L, t, m, 2 * p - 1 + e, (int64_t)plane_off(t) + off[k] + e, code, xc,
+                                              nb);
+                    }
+                    r[e] = v;
+                }
+                ldsR[t & 1][i][lane + 64 * k] = r[0] + r[1];
+            }
+        }
+        {
+            const float *pb = plane(b, sp + 2);
+            const uint8_t *pc = planeC(sp + 2);
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                Bv[S2][k] = pair(pb, off[k]);
+                C[S2][k] = pairC(pc, off[k]);
+            }
+        }
+    };
+    // the first-colour element at plane sp is (c1 - sp - m - 1) & 1
+    auto march = [&](auto colour0) {
+        constexpr int E0 = decltype(colour0)::value;
+        for (int sp = sp0; sp <= s1 + 2; sp += 4)
+        {
+            step(sp, std::integral_constant<int, 0>(), std::integral_constant<int, E0>());
+            if (sp + 1 > s1 + 2) break;
+            step(sp + 1, std::integral_constant<int, 1>(), std::integral_constant<int, E0 ^ 1>());
+            if (sp + 2 > s1 + 2) break;
+            step(sp + 2, std::integral_constant<int, 2>(), std::integral_constant<int, E0>());
+            if (sp + 3 > s1 + 2) break;
+            step(sp + 3, std::integral_constant<int, 3>(), std::integral_constant<int, E0 ^ 1>());
+        }
+    };
+    if ((c1 - sp0 - m - 1) & 1)
+        march(std::integral_constant<int, 1>());
+    else
+        march(std::integral_constant<int, 0>());
+    __syncthreads();
+    family(s1);
+}
+
 // Residual of every fine cell, summed over each 2 x 2 x 2 family into the coarse right-hand side: fast pairs are
 // neighbouring lanes (shuffle), mid pairs neighbouring rows of the block (LDS), slow pairs consecutive planes of
 // the march (P even, chunks start on an odd plane). A family cut by the end of an odd interior keeps its 1 or 2
@@ -2915,6 +3237,19 @@ struct RestrictRowArgs
     int P, tilesM;
 };
 
+struct SweepRestrictArgs
+{
+    Level32 L;
+    const float *xin;
+    float *xout;
+    const float *b;
+    int c1;
+    Level32 Lc;
+    float *bc;
+    int P, tilesM;
+    const SolveState *st;
+};
+
 struct RestrictMarchArgs
 {
     Level32 Lf;
@@ -3036,6 +3371,25 @@ __global__ __launch_bounds__(kBlockThreads) void k_residual_restrict_m(Level32 L
                                                                        int P)
 {
     residual_restrict_m_body(Lf, x, b, Lc, bc, st, P, logical_block());
+}
+
+template <int TM, int K>
+__global__ __launch_bounds__(64 * (TM + 4)) void k_rbgs_restrict_row(Level32 L, const float *__restrict__ xin,
+                                                                     float *__restrict__ xout,
+                                                                     const float *__restrict__ b, int c1, Level32 Lc,
+                                                                     float *__restrict__ bc, int P, int tilesM,
+                                                                     const SolveState *st)
+{
+    rbgs_restrict_row_body<TM, K>(L, xin, xout, b, c1, Lc, bc, P, tilesM, st, logical_block());
+}
+
+template <int TM, int K>
+__global__ __launch_bounds__(64 * (TM + 4)) void kb_rbgs_restrict_row(const SweepRestrictArgs *__restrict__ args,
+                                                                      const int *__restrict__ start, int nsys)
+{
+    const BatchBlock bb = batch_block(start, nsys);
+    const SweepRestrictArgs a = args[bb.q];
+    rbgs_restrict_row_body<TM, K>(a.L, a.xin, a.xout, a.b, a.c1, a.Lc, a.bc, a.P, a.tilesM, a.st, bb.lb);
 }
 
 __global__ __launch_bounds__(kBlockThreads) void kb_residual_restrict_m(const RestrictMarchArgs *__restrict__ args,
@@ -3244,6 +3598,38 @@ void row_sweep(hipStream_t s, int grid, int threads, const RowSweepArgs &a)
 }
 
 template <int TM, int K>
+void sweep_restrict(hipStream_t s, int grid, const SweepRestrictArgs &a)
+{
+    if (record(kb_rbgs_restrict_row<TM, K>, grid, 64 * (TM + 4), a, REC_SWEEP_RESTRICT, TM, K)) return;
+    k_rbgs_restrict_row<TM, K><<<grid, 64 * (TM + 4), 0, s>>>(a.L, a.xin, a.xout, a.b, a.c1, a.Lc, a.bc, a.P,
+                                                                 a.tilesM, a.st);
+}
+
+// The fused sweep and restriction of that shape: launched (or recorded), false for a shape it has no form for.
+bool sweep_restrict_shape(hipStream_t s, int tm, int k, int grid, const SweepRestrictArgs &a)
+{
+    if (k == 1 && tm == 6) sweep_restrict<6, 1>(s, grid, a);
+    else if (k == 1 && tm == 8) sweep_restrict<8, 1>(s, grid, a);
+    else if (k == 1 && tm == 10) sweep_restrict<10, 1>(s, grid, a);
+    else if (k == 1 && tm == 12) sweep_restrict<12, 1>(s, grid, a);
+    else if (k == 2 && tm == 6) sweep_restrict<6, 2>(s, grid, a);
+    else if (k == 2 && tm == 8) sweep_restrict<8, 2>(s, grid, a);
+    else return false;
+    return true;
+}
+
+const void *sweep_restrict_batched(int tm, int k)
+{
+    if (k == 1 && tm == 6) return reinterpret_cast<const void *>(kb_rbgs_restrict_row<6, 1>);
+    if (k == 1 && tm == 8) return reinterpret_cast<const void *>(kb_rbgs_restrict_row<8, 1>);
+    if (k == 1 && tm == 10) return reinterpret_cast<const void *>(kb_rbgs_restrict_row<10, 1>);
+    if (k == 1 && tm == 12) return reinterpret_cast<const void *>(kb_rbgs_restrict_row<12, 1>);
+    if (k == 2 && tm == 6) return reinterpret_cast<const void *>(kb_rbgs_restrict_row<6, 2>);
+    if (k == 2 && tm == 8) return reinterpret_cast<const void *>(kb_rbgs_restrict_row<8, 2>);
+    return nullptr;
+}
+
+template <int TM, int K>
 void restrict_row(hipStream_t s, int grid, int threads, const RestrictRowArgs &a)
 {
     if (record(kb_restrict_row<TM, K>, grid, threads, a, REC_RESTRICT_ROW, TM, K)) return;
@@ -3305,6 +3691,11 @@ void record_slot(int phase, int level)
 void retile_rows(const LaunchRecord &r, std::vector<int> &tms)
 {
     tms.clear();
+    if (r.kind == REC_SWEEP_RESTRICT)
+    {
+        rbgs_restrict_rows(r.k, tms);
+        return;
+    }
     if (r.kind != REC_ROW_SWEEP) return;
     // 14-row tiles need 1024-thread blocks, whose register limit the prolongation-folding K = 2 sweep exceeds (10
     // rows there on uniform levels); K = 4 rows only in 6-row tiles
@@ -3333,6 +3724,22 @@ bool retile_record(LaunchRecord &r, int tm, int p)
         a.tilesM = std::max(1, (B.nm - 2 + TM - 1) / TM);
         r.grid = a.tilesM * std::max(1, (B.ns - 2 + P - 1) / P);
         r.threads = 64 * (TM + 2);
+        r.fn = fn;
+        r.tm = TM;
+        std::memcpy(r.args.data(), &a, sizeof a);
+        return true;
+    }
+    case REC_SWEEP_RESTRICT:
+    {
+        SweepRestrictArgs a;
+        std::memcpy(&a, r.args.data(), sizeof a);
+        const int TM = tm > 0 ? tm : r.tm;
+        const void *fn = sweep_restrict_batched(TM, r.k);
+        if (!fn) return false;
+        a.P = Pe;
+        a.tilesM = std::max(1, (a.L.B.nm - 2 + TM - 1) / TM);
+        r.grid = a.tilesM * std::max(1, (a.L.B.ns - 2 + Pe - 1) / Pe);
+        r.threads = 64 * (TM + 4);
         r.fn = fn;
         r.tm = TM;
         std::memcpy(r.args.data(), &a, sizeof a);
@@ -3698,6 +4105,57 @@ hipError_t launch_residual_restrict(hipStream_t s, const Level32 &Lf, const floa
     }
     k_residual_restrict<<<grid_of(Lc.B), kBlockThreads, 0, s>>>(Lf, x, b, Lc, bc, st);
     return hipGetLastError();
+}
+
+void rbgs_restrict_rows(int k, std::vector<int> &tms)
+{
+    // (two pairs per lane: larger tiles exceed the registers of their blocks)
+    if (k == 1) tms = {6, 8, 10, 12};
+    else if (k == 2) tms = {6, 8};
+    else tms.clear();
+}
+
+bool rbgs_restrict_fusable(const Level32 &Lf, const float *xin, const float *xout, const float *b, const float *bc)
+{
+    const char *env = std::getenv("NPPE_FUSE_RESTRICT");
+    if (env && std::atoi(env) == 0) return false;
+    const char *tiles = std::getenv("NPPE_TILE_PASSES");
+    if (tiles && std::atoi(tiles) != 0) return false;
+    return Lf.B.threeD && Lf.uniform && Lf.B.nf - 2 <= 256 && bc && rbgs_family(Lf, xin, xout, b) == RBGS_ROW;
+}
+
+hipError_t launch_rbgs_restrict(hipStream_t s, const Level32 &Lf, const float *xin, float *xout, const float *b,
+                                int c1, const Level32 &Lc, float *bc, int tileRows, int planesPerChunk,
+                                const SolveState *st)
+{
+    const Box &B = Lf.B;
+    const int K = B.nf - 2 <= 128 ? 1 : 2;
+    std::vector<int> tms;
+    rbgs_restrict_rows(K, tms);
+    int TM = tms.front();
+    for (int t : tms)
+        if (t <= tileRows) TM = t;
+    int P = std::max(2, planesPerChunk);
+    P += P & 1;
+    const int tilesM = std::max(1, (B.nm - 2 + TM - 1) / TM);
+    const int grid = tilesM * std::max(1, (B.ns - 2 + P - 1) / P);
+    const SweepRestrictArgs a{Lf, xin, xout, b, c1, Lc, bc, P, tilesM, st};
+    if (!sweep_restrict_shape(s, TM, K, grid, a)) return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+int64_t rbgs_restrict_blocks(const Box &B, int tileRows, int planesPerChunk)
+{
+    const int K = B.nf - 2 <= 128 ? 1 : 2;
+    std::vector<int> tms;
+    rbgs_restrict_rows(K, tms);
+    if (tms.empty()) return 0;
+    int TM = tms.front();
+    for (int t : tms)
+        if (t <= tileRows) TM = t;
+    int P = std::max(2, planesPerChunk);
+    P += P & 1;
+    return (int64_t)std::max(1, (B.nm - 2 + TM - 1) / TM) * std::max(1, (B.ns - 2 + P - 1) / P);
 }
 
 hipError_t launch_prolong(hipStream_t s, const Level32 &Lf, float *x, const Level32 &Lc, const float *xc,

@@ -183,28 +183,44 @@ hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel, doubl
         const double N = L.cellsBox;
         const int nu1 = (coarseNu > 0 && l >= 2) ? coarseNu : h->nuPre;
         record_slot(SLOT_DOWN, l);
+        DevLevel &C = h->levels[l + 1];
+        // the last sweep (a plain one) and the restriction of its residual in one march where the level has it
+        const bool fuse = nu1 >= 2 && rbgs_restrict_fusable(L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l), C.b);
         for (int sw = 0; sw < nu1; sw++)
         {
             const bool zero = sw == 0;
-            const double by = zero ? (2 * s + f) * N : (3 * s + f) * N;
-            Scope sc(h, l == 0 ? NPPE_K_SMOOTH0 : NPPE_K_SMOOTH, by);
             float *xin = xbuf(l, cur[l]);
             float *xout = zero ? xin : xbuf(l, cur[l] ^ 1);
+            if (fuse && sw == nu1 - 1)
+            {
+                const double by = (3 * s + f) * N + s * C.cellsBox;
+                Scope sc(h, l == 0 ? NPPE_K_SMOOTH0 : NPPE_K_SMOOTH, by);
+                if ((e = launch_rbgs_restrict(h->stream, L.v, xin, xout, bbuf(l), 0, C.v, C.b, L.fuseRows,
+                                              L.fusePlanes, h->dState)) != hipSuccess)
+                    return e;
+                cur[l] ^= 1;
+                tr.add(by);
+                break;
+            }
+            const double by = zero ? (2 * s + f) * N : (3 * s + f) * N;
+            Scope sc(h, l == 0 ? NPPE_K_SMOOTH0 : NPPE_K_SMOOTH, by);
             if ((e = launch_rbgs(h->stream, L.v, zero ? nullptr : xin, xout, bbuf(l), 0, zero ? 1 : 0, L.tileRows,
                                  L.planesPerChunk, h->dState)) != hipSuccess)
                 return e;
             if (!zero) cur[l] ^= 1;
             tr.add(by);
         }
-        DevLevel &C = h->levels[l + 1];
-        const double rb = (2 * s + f) * N + s * C.cellsBox;
+        if (!fuse)
         {
-            Scope sc(h, NPPE_K_RESTRICT, rb);
-            if ((e = launch_residual_restrict(h->stream, L.v, xbuf(l, cur[l]), bbuf(l), C.v, C.b, h->dState)) !=
-                hipSuccess)
-                return e;
+            const double rb = (2 * s + f) * N + s * C.cellsBox;
+            {
+                Scope sc(h, NPPE_K_RESTRICT, rb);
+                if ((e = launch_residual_restrict(h->stream, L.v, xbuf(l, cur[l]), bbuf(l), C.v, C.b, h->dState)) !=
+                    hipSuccess)
+                    return e;
+            }
+            tr.add(rb);
         }
-        tr.add(rb);
     }
     {
         const int l = nl - 1;
@@ -384,7 +400,8 @@ struct TileKey
     }
 };
 std::mutex gTileMu;
-std::map<TileKey, std::tuple<int, int, int>> gTiles;   // tile rows, planes per chunk, planes of the folding sweep
+// tile rows, planes per chunk, planes of the folding sweep, tile rows and planes of the sweep with the restriction
+std::map<TileKey, std::tuple<int, int, int, int, int>> gTiles;
 
 // Model: rounds of resident blocks (two per CU) times the plane steps of a block (P + 2).
 int model_planes(const Box &B, int family, int tm, int cus)
@@ -430,8 +447,31 @@ hipError_t choose_tiles(nppe_solver *h)
         if (envTM) L.tileRows = std::max(1, std::atoi(envTM));
         if (envP) L.planesPerChunk = std::max(1, std::atoi(envP));
         L.proPlanes = L.planesPerChunk;
+        // the sweep with the restriction folded in: the tallest tile, the chunk of the same block-round model
+        // (even: the restriction sums plane pairs)
+        {
+            std::vector<int> fr;
+            rbgs_restrict_rows(L.box.nf - 2 <= 128 ? 1 : 2, fr);
+            L.fuseRows = fr.empty() ? 6 : fr.back();
+            double bestCost = 1e300;
+            for (int P = 2; P <= std::max(2, std::min(64, L.box.ns - 1)); P += 2)
+            {
+                const int64_t blocks = rbgs_restrict_blocks(L.box, L.fuseRows, P);
+                const int64_t cap = 2LL * std::max(1, cus);
+                const double cost = (double)((blocks + cap - 1) / cap) * (P + 4);
+                if (cost < bestCost)
+                {
+                    bestCost = cost;
+                    L.fusePlanes = P;
+                }
+            }
+        }
+        // NPPE_FUSE_TM / NPPE_FUSE_P force the fused sweep's shape (the tests compare shapes)
+        const char *envFTM = std::getenv("NPPE_FUSE_TM"), *envFP = std::getenv("NPPE_FUSE_P");
+        if (envFTM) L.fuseRows = std::max(1, std::atoi(envFTM));
+        if (envFP) L.fusePlanes = std::max(1, std::atoi(envFP));
         const int64_t interior = (int64_t)(L.box.threeD ? L.box.ns - 2 : 1) * (L.box.nm - 2) * (L.box.nf - 2);
-        if (forced || interior < kTuneCells) continue;
+        if (forced || envFTM || envFP || interior < kTuneCells) continue;
         const TileKey key{h->opt.device, L.box.ns, L.box.nm, L.box.nf, L.v.uniform, family};
         {
             std::lock_guard<std::mutex> lock(gTileMu);
@@ -441,6 +481,8 @@ hipError_t choose_tiles(nppe_solver *h)
                 L.tileRows = std::get<0>(it->second);
                 L.planesPerChunk = std::get<1>(it->second);
                 L.proPlanes = std::get<2>(it->second);
+                L.fuseRows = std::get<3>(it->second);
+                L.fusePlanes = std::get<4>(it->second);
                 continue;
             }
         }
@@ -510,8 +552,32 @@ hipError_t choose_tiles(nppe_solver *h)
             }
             if (err != hipSuccess) break;
         }
+        // the last pre-smoothing sweep with the restriction folded in: its tile rows and chunk timed on their own
+        if (l + 1 < h->levels.size() && rbgs_restrict_fusable(L.v, xin, xout, b, h->levels[l + 1].b))
+        {
+            const DevLevel &C = h->levels[l + 1];
+            std::vector<int> fr;
+            rbgs_restrict_rows(L.box.nf - 2 <= 128 ? 1 : 2, fr);
+            float bestFuse = 1e30f;
+            for (int tm : fr)
+                for (int P : candidates(tm, false))
+                {
+                    if (P & 1) continue;
+                    const float ms = timeIt([&] {
+                        return launch_rbgs_restrict(h->stream, L.v, xin, xout, b, 0, C.v, C.b, tm, P, h->dState);
+                    });
+                    if (err != hipSuccess) break;
+                    if (ms < bestFuse)
+                    {
+                        bestFuse = ms;
+                        L.fuseRows = tm;
+                        L.fusePlanes = P;
+                    }
+                }
+            if (err != hipSuccess) break;
+        }
         std::lock_guard<std::mutex> lock(gTileMu);
-        gTiles[key] = {bestTM, bestP, L.proPlanes};
+        gTiles[key] = {bestTM, bestP, L.proPlanes, L.fuseRows, L.fusePlanes};
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -1373,7 +1439,7 @@ int tune_group(nppe_solver *h0, PlanGroup &G, void *scratch)
 {
     const LaunchRecord &r0 = *G.recs[0];
     std::vector<int> tms;
-    if (r0.kind == REC_ROW_SWEEP) retile_rows(r0, tms);
+    if (r0.kind == REC_ROW_SWEEP || r0.kind == REC_SWEEP_RESTRICT) retile_rows(r0, tms);
     else tms = {0};
     const std::vector<int> ps = r0.kind == REC_ROW_SWEEP ? std::vector<int>{1, 2, 4, 8, 12, 16, 24, 32}
                                                          : std::vector<int>{2, 4, 8, 16, 32};

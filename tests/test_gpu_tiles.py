@@ -14,7 +14,8 @@ SHAPES = [("6", "1"), ("6", "5"), ("14", "3"), ("14", "16"), ("4", "2"), ("8", "
 
 
 def _solve(g, b, env):
-    saved = {k: os.environ.get(k) for k in ("NPPE_RBGS_TM", "NPPE_RBGS_P", "NPPE_AUTOTUNE", "NPPE_RBGS_MARCH")}
+    saved = {k: os.environ.get(k) for k in ("NPPE_RBGS_TM", "NPPE_RBGS_P", "NPPE_AUTOTUNE", "NPPE_RBGS_MARCH",
+                                            "NPPE_FUSE_RESTRICT", "NPPE_FUSE_TM", "NPPE_FUSE_P")}
     try:
         for k in saved:
             os.environ.pop(k, None)
@@ -68,3 +69,35 @@ def test_march_equals_row_sweep(n, kind, non_uniform):
     x, res = _solve(g, b, {"NPPE_RBGS_MARCH": "1"})
     assert res.iterations == rres.iterations
     assert np.array_equal(x, ref)
+
+
+@pytest.mark.parametrize("n,kind,non_uniform", [
+    (40, "mixed", False),                # one pair per lane, Dirichlet codes, solids
+    ((24, 20, 140), "pressure", False),  # rows of 140 cells: two pairs per lane
+    (72, "neumann", False),              # large enough for the timed autotuning
+    ((33, 27, 61), "outlet", False),     # odd sizes: coarse levels with a half-width last cell (no fused form)
+    ((31, 45, 97), "mixed", False),      # odd plane count: a chunk ending in a lone plane
+    (36, "outlet", True),                # stretched: no fused form at any level
+])
+def test_fused_restriction_equals_separate_passes(n, kind, non_uniform):
+    """The last pre-smoothing sweep with the restriction folded in computes every swept value and every coarse
+    right-hand side with the operations of the two passes it replaces, so whole solves agree bit for bit."""
+    g, _ = make(3, n, kind, non_uniform, seed=7)
+    b = np.random.default_rng(8).standard_normal(g.unknowns)
+    if kind == "neumann":
+        b -= b.mean()
+    ref, rres = _solve(g, b, {"NPPE_FUSE_RESTRICT": "0"})
+    x, res = _solve(g, b, {})
+    assert res.iterations == rres.iterations
+    assert np.array_equal(x, ref)
+
+
+@pytest.mark.parametrize("tm,p", [("6", "2"), ("8", "4"), ("10", "6"), ("12", "16"), ("12", "2"), ("6", "64")])
+def test_fused_restriction_tile_shapes(tm, p):
+    """Every tile shape of the fused sweep (tile rows, planes per chunk) gives the same bits."""
+    g, _ = make(3, (30, 26, 120), "mixed", False, seed=9)
+    b = np.random.default_rng(10).standard_normal(g.unknowns)
+    ref, rres = _solve(g, b, {"NPPE_FUSE_RESTRICT": "0"})
+    x, res = _solve(g, b, {"NPPE_FUSE_TM": tm, "NPPE_FUSE_P": p})
+    assert res.iterations == rres.iterations
+    assert np.array_equal(x, ref), (tm, p)
