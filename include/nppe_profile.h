@@ -32,7 +32,8 @@ enum nppe_kernel_kind
     NPPE_K_CG64_UPDATE = 14, /* FP64 x, r update with <r, r>, <r, M^-1 r> */
     NPPE_K_CG64_DIRECTION = 15,
     NPPE_K_SMOOTH0 = 16,     /* red-black Gauss-Seidel sweep (both colours), FP32, the finest level */
-    NPPE_KERNEL_KINDS = 17
+    NPPE_K_SMOOTH_RESTRICT = 17, /* the last pre-smoothing sweep with the residual restriction folded in, FP32 */
+    NPPE_KERNEL_KINDS = 18
 };
 
 struct nppe_solver;

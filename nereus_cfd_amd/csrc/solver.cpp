@@ -194,7 +194,7 @@ hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel, doubl
             if (fuse && sw == nu1 - 1)
             {
                 const double by = (3 * s + f) * N + s * C.cellsBox;
-                Scope sc(h, l == 0 ? NPPE_K_SMOOTH0 : NPPE_K_SMOOTH, by);
+                Scope sc(h, NPPE_K_SMOOTH_RESTRICT, by);
                 if ((e = launch_rbgs_restrict(h->stream, L.v, xin, xout, bbuf(l), 0, C.v, C.b, L.fuseRows,
                                               L.fusePlanes, h->dState)) != hipSuccess)
                     return e;

@@ -26,6 +26,7 @@ FLUID, SOLID, EMPTY, INFLOW, PRESSURE, OUTFLOW = 0, 1, 2, 3, 4, 5
 KERNEL_KINDS = [
     "smooth", "restrict", "prolong", "coarse", "cg_dot", "cg_direction", "cg_spmv", "cg_update", "residual64",
     "scatter", "gather", "mean", "cg64_init", "cg64_spmv", "cg64_update", "cg64_direction", "smooth0",
+    "smooth_restrict",
 ]
 
 # Every symbol the headers in include/ declare.
