@@ -146,6 +146,42 @@ def test_batched_launches_fall_back_for_other_systems():
             h.close()
 
 
+@pytest.mark.parametrize("other", ["cg64", "profiling", "stretched"])
+def test_batched_launches_with_a_system_of_another_kind(other):
+    """A CG64 handle, a handle that is profiling, or a stretched grid in the batch: every system is solved on its
+    own stream instead, and still gets its alone result."""
+    grids, bs = _mix(3, 18)
+    solvers = [N.Solver() for _ in grids]
+    if other == "stretched":
+        g = W.stretched_channel(32, 16, 16)
+        grids.append(g)
+        bs.append(W.random_rhs(g, np.random.default_rng(3), False))
+        solvers.append(N.Solver())
+    else:
+        grids.append(grids[0])
+        bs.append(bs[0])
+        solvers.append(N.Solver(method=N.METHOD_CG64) if other == "cg64" else N.Solver())
+    alone = []
+    for g, b, h in zip(grids, bs, solvers):
+        with N.Solver(method=h.opts.method) as a:
+            a.commit(g)
+            x, res, rc = a.solve(b)
+            assert rc == 0
+            alone.append(x)
+    try:
+        for h, g in zip(solvers, grids):
+            h.commit(g)
+        if other == "profiling":
+            solvers[-1].profile(True)
+        xs, res, rc = _device_batch(solvers, bs)
+        assert rc == 0
+        for x, xa in zip(xs, alone):
+            assert np.array_equal(x, xa)
+    finally:
+        for h in solvers:
+            h.close()
+
+
 def test_graph_replay_equals_direct_launches():
     g = N.uniform_grid(40)
     b = N.synth_rhs(g)
