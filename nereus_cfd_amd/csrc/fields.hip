@@ -188,13 +188,14 @@ __global__ __launch_bounds__(kThreads) void k_rhs(FieldGeom G, const double *__r
 // ---- projection --------------------------------------------------------------------------------------------
 // ProjectionGradient (PressureSolver3D.cpp:405-414): the face gradient over FaceGap, times DirichletRingFactor when
 // the face's outer cell is a Dirichlet ring cell. n: the face's index along the axis, N: the axis length.
-__device__ __forceinline__ double proj_grad(double pc, double pm, const double *w, int N, int n, uint8_t tRing0,
-                                            uint8_t tRingN)
+// The ring cells' types (t[ring0], t[ringN]) are read only for the first and last face of the axis.
+__device__ __forceinline__ double proj_grad(double pc, double pm, const double *w, int N, int n,
+                                            const uint8_t *__restrict__ t, int64_t ring0, int64_t ringN)
 {
 #pragma clang fp contract(off)
     const double grad = (pc - pm) / face_gap(w, n - 1);
-    if (n == 1 && dirichlet(tRing0)) return grad * ring_factor(w, 1, -1);
-    if (n == N - 1 && dirichlet(tRingN)) return grad * ring_factor(w, N - 2, +1);
+    if (n == 1 && dirichlet(t[ring0])) return grad * ring_factor(w, 1, -1);
+    if (n == N - 1 && dirichlet(t[ringN])) return grad * ring_factor(w, N - 2, +1);
     return grad;
 }
 
@@ -223,26 +224,26 @@ __global__ __launch_bounds__(kThreads) void k_project(FieldGeom G, double *__res
             const double pc = p[c];
             if (k == Nz - 2 && i <= Ny - 2 && j <= Nx - 2 && t[cp + 1] != kSolid && tc != kSolid)
             {
-                const double gr = proj_grad(p[c + 1], pc, G.wz, Nz, Nz - 1, t[G.B.at(i, j, 0)], t[cp + 1]);
+                const double gr = proj_grad(p[c + 1], pc, G.wz, Nz, Nz - 1, t, G.B.at(i, j, 0), cp + 1);
                 double *wf = &w[((int64_t)i * Nx + j) * (Nz + 1) + k + 1];
                 *wf = *wf - fac * gr;
             }
             if (i <= Ny - 2 && tc != kSolid && t[cp - G.B.sM] != kSolid)
             {
-                const double gr = proj_grad(pc, p[c - Nz], G.wx, Nx, j, t[G.B.at(i, 0, k)], t[G.B.at(i, Nx - 1, k)]);
+                const double gr = proj_grad(pc, p[c - Nz], G.wx, Nx, j, t, G.B.at(i, 0, k), G.B.at(i, Nx - 1, k));
                 double *uf = &u[((int64_t)i * (Nx + 1) + j) * Nz + k];
                 *uf = *uf - fac * gr;
             }
             if (j <= Nx - 2 && tc != kSolid && t[cp - G.B.sS] != kSolid)
             {
-                const double gr = proj_grad(pc, p[c - (int64_t)Nx * Nz], G.wy, Ny, i, t[G.B.at(0, j, k)],
-                                            t[G.B.at(Ny - 1, j, k)]);
+                const double gr =
+                    proj_grad(pc, p[c - (int64_t)Nx * Nz], G.wy, Ny, i, t, G.B.at(0, j, k), G.B.at(Ny - 1, j, k));
                 double *vf = &v[((int64_t)i * Nx + j) * Nz + k];
                 *vf = *vf - fac * gr;
             }
             if (i <= Ny - 2 && j <= Nx - 2 && tc != kSolid && t[cp - 1] != kSolid)
             {
-                const double gr = proj_grad(pc, p[c - 1], G.wz, Nz, k, t[G.B.at(i, j, 0)], t[G.B.at(i, j, Nz - 1)]);
+                const double gr = proj_grad(pc, p[c - 1], G.wz, Nz, k, t, G.B.at(i, j, 0), G.B.at(i, j, Nz - 1));
                 double *wf = &w[((int64_t)i * Nx + j) * (Nz + 1) + k];
                 *wf = *wf - fac * gr;
             }
@@ -259,17 +260,17 @@ __global__ __launch_bounds__(kThreads) void k_project(FieldGeom G, double *__res
             const double pc = p[c];
             if (j == Nx - 2 && i <= Ny - 2 && t[cp + 1] != kSolid && tc != kSolid)
             {
-                const double gr = proj_grad(p[c + 1], pc, G.wx, Nx, Nx - 1, t[G.B.at(0, i, 0)], t[cp + 1]);
+                const double gr = proj_grad(p[c + 1], pc, G.wx, Nx, Nx - 1, t, G.B.at(0, i, 0), cp + 1);
                 u[(int64_t)i * (Nx + 1) + j + 1] = u[(int64_t)i * (Nx + 1) + j + 1] - fac * gr;
             }
             if (i <= Ny - 2 && tc != kSolid && t[cp - 1] != kSolid)
             {
-                const double gr = proj_grad(pc, p[c - 1], G.wx, Nx, j, t[G.B.at(0, i, 0)], t[G.B.at(0, i, Nx - 1)]);
+                const double gr = proj_grad(pc, p[c - 1], G.wx, Nx, j, t, G.B.at(0, i, 0), G.B.at(0, i, Nx - 1));
                 u[(int64_t)i * (Nx + 1) + j] = u[(int64_t)i * (Nx + 1) + j] - fac * gr;
             }
             if (tc != kSolid && t[cp - G.B.sM] != kSolid)
             {
-                const double gr = proj_grad(pc, p[c - Nx], G.wy, Ny, i, t[G.B.at(0, 0, j)], t[G.B.at(0, Ny - 1, j)]);
+                const double gr = proj_grad(pc, p[c - Nx], G.wy, Ny, i, t, G.B.at(0, 0, j), G.B.at(0, Ny - 1, j));
                 v[(int64_t)i * Nx + j] = v[(int64_t)i * Nx + j] - fac * gr;
             }
         }
