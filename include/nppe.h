@@ -67,8 +67,7 @@ enum nppe_krylov
 {
     NPPE_KRYLOV_REFINE = 0, /* FP32 CG passes inside FP64 iterative refinement (true FP64 residual between passes) */
     NPPE_KRYLOV_FP64 = 1    /* one CG with x, r, alpha, beta and A p in FP64 (directions stored in FP32) around the
-                               FP32 V-cycle; the true FP64 residual is checked at its end (3D grids; 2D grids use
-                               NPPE_KRYLOV_REFINE) */
+                               FP32 V-cycle; the true FP64 residual is checked at its end (2D and 3D grids) */
 };
 
 /* Cell type codes (Definitions.h:191-196). */

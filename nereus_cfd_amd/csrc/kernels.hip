@@ -489,12 +489,12 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64(Box B, Tabl
                                                                      double *__restrict__ xout,
                                                                      const double *__restrict__ b,
                                                                      float *__restrict__ r32, double *__restrict__ r64,
-                                                                     Red red, int gate)
+                                                                     Red red, int gate, int sumX)
 {
 #pragma clang fp contract(off)
     if (gate && !red.st->done) return;   // every block alike: no partial, no ticket
     const TileRange tr = my_tiles(B.blocks());
-    double rr = 0.0;
+    double rr = 0.0, xs = 0.0;   // <r, r>; with sumX the sum of x over the unknowns
     for (int t = tr.begin; t < tr.end; t++)
     {
         int s, m, f;
@@ -514,10 +514,83 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64(Box B, Tabl
                 ax = apply64(B, T, ty, s, m, f, idx, [&](int64_t d) { return x[idx + d]; });
             }
             r = b[idx] - ax;
+            xs += e ? x[idx] + (double)e[idx] : x[idx];
         }
         if (r32) r32[idx] = (float)r;
         if (r64) r64[idx] = r;
         rr += r * r;
+    }
+    if (sumX)
+    {
+        double a[2] = {rr, xs};
+        block_sum<2>(a, red, gridDim.x, tr.lb);
+    }
+    else
+    {
+        double a[1] = {rr};
+        block_sum<1>(a, red, gridDim.x, tr.lb);
+    }
+}
+
+// The FP64 Krylov recursion's two passes on any box (2D grids), one thread per cell over the tiles, with the
+// reference's FP64 operator (apply64). Part 1: p = (float)(z + beta p_old) at the unknowns and <p, A p>; p is
+// pointwise, so the neighbours' directions are formed from their z and p_old as well and no pass reads a p it
+// writes. Part 2: q = A p, x += alpha p, r -= alpha q, r32 = (float) r and <r, r>.
+__global__ __launch_bounds__(kBlockThreads) void k_cg_dir64_g(Box B, Tables64 T, const uint8_t *__restrict__ ty,
+                                                              const float *__restrict__ z, float *__restrict__ pA,
+                                                              float *__restrict__ pB, const SolveState *st, Red red)
+{
+#pragma clang fp contract(off)
+    if (st->done) return;
+    const int first = st->first;
+    const double beta = st->beta;
+    const bool odd = st->iters & 1;
+    float *__restrict__ pn = odd ? pB : pA;
+    const float *__restrict__ po = odd ? pA : pB;
+    auto PV = [&](int64_t i) { return (double)(float)(first ? (double)z[i] : (double)z[i] + beta * (double)po[i]); };
+    const TileRange tr = my_tiles(B.blocks());
+    double pq = 0.0;
+    for (int t = tr.begin; t < tr.end; t++)
+    {
+        int s, m, f;
+        if (!tile_cell(B, t, s, m, f)) continue;
+        const int64_t idx = B.at(s, m, f);
+        if (!(T.c8[idx] & 64u)) continue;
+        const double pc = PV(idx);
+        const double v = apply64(B, T, ty, s, m, f, idx, [&](int64_t d) { return PV(idx + d); });
+        pn[idx] = (float)pc;
+        pq += pc * v;
+    }
+    double a[1] = {pq};
+    block_sum<1>(a, red, gridDim.x, tr.lb);
+}
+
+__global__ __launch_bounds__(kBlockThreads) void k_cg_update64_g(Box B, Tables64 T, const uint8_t *__restrict__ ty,
+                                                                 const float *__restrict__ pA,
+                                                                 const float *__restrict__ pB,
+                                                                 double *__restrict__ x, double *__restrict__ r,
+                                                                 float *__restrict__ r32, const SolveState *st,
+                                                                 Red red)
+{
+#pragma clang fp contract(off)
+    if (st->done) return;
+    const double alpha = st->alpha;
+    const float *__restrict__ p = (st->iters & 1) ? pB : pA;
+    const TileRange tr = my_tiles(B.blocks());
+    double rr = 0.0;
+    for (int t = tr.begin; t < tr.end; t++)
+    {
+        int s, m, f;
+        if (!tile_cell(B, t, s, m, f)) continue;
+        const int64_t idx = B.at(s, m, f);
+        if (!(T.c8[idx] & 64u)) continue;
+        const double pc = p[idx];
+        const double q = apply64(B, T, ty, s, m, f, idx, [&](int64_t d) { return (double)p[idx + d]; });
+        const double xv = x[idx] + alpha * pc, rv = r[idx] - alpha * q;
+        x[idx] = xv;
+        r[idx] = rv;
+        r32[idx] = (float)rv;
+        rr += rv * rv;
     }
     double a[1] = {rr};
     block_sum<1>(a, red, gridDim.x, tr.lb);
@@ -3833,12 +3906,12 @@ hipError_t launch_update_residual64(hipStream_t s, const Box &B, const Tables64 
                                                                      gate, sumX);
         return finish(s, red, G, sumX ? 2 : 1);
     }
-    if (sumX) return hipErrorInvalidValue;   // (the 2D residual has no x sum)
-    const FinOp op = gate ? FIN_TRUE_RES : FIN_SUM1;
+    sumX = gate && sumX ? 1 : 0;
+    const FinOp op = gate ? (sumX ? FIN_TRUE_RES_SUM : FIN_TRUE_RES) : FIN_SUM1;
     const int G = grid_of(B);
     const Red red = red_of(partials, st, op, G);
-    k_update_residual64<<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, red, gate);
-    return finish(s, red, G, 1);
+    k_update_residual64<<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, red, gate, sumX);
+    return finish(s, red, G, sumX ? 2 : 1);
 }
 
 hipError_t launch_apply64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const double *x,
@@ -4234,6 +4307,13 @@ hipError_t launch_cg_dir64(hipStream_t s, const Box &B, const Tables64 &T, const
                            float *pA, float *pB, SolveState *st, double *partials)
 {
     // 6-row blocks, two per CU (measured at 256^3: 57 us against 61 us for one 14-row block per CU)
+    if (!B.threeD)
+    {
+        const int G = grid_of(B);
+        const Red red = red_of(partials, st, FIN_PQ, G);
+        k_cg_dir64_g<<<G, kBlockThreads, 0, s>>>(B, T, ty, z, pA, pB, st, red);
+        return finish(s, red, G, 1);
+    }
     const RowPlan rp = row_plan(B, T.c8, T.unitCoef, {{z, 8}, {pA, 8}, {pB, 8}}, 6, 2);
     if (rp.ok)
     {
@@ -4257,6 +4337,13 @@ hipError_t launch_cg_dir64(hipStream_t s, const Box &B, const Tables64 &T, const
 hipError_t launch_cg_update64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const float *pA,
                               const float *pB, double *x, double *r, float *r32, SolveState *st, double *partials)
 {
+    if (!B.threeD)
+    {
+        const int G = grid_of(B);
+        const Red red = red_of(partials, st, FIN_RR, G);
+        k_cg_update64_g<<<G, kBlockThreads, 0, s>>>(B, T, ty, pA, pB, x, r, r32, st, red);
+        return finish(s, red, G, 1);
+    }
     const RowPlan rp = row_plan(B, T.c8, T.unitCoef, {{pA, 8}, {pB, 8}, {x, 16}, {r, 16}, {r32, 8}}, 6, 2);
     if (rp.ok)
     {

@@ -822,8 +822,8 @@ int build(nppe_solver *h, const nppe_grid *g)
         HIP_TRY(h, dalloc_pairs(h, &h->dP, cells), "allocate p");
         HIP_TRY(h, dalloc_pairs(h, &h->dQ, cells), "allocate q");
         HIP_TRY(h, dalloc_pairs(h, &h->dP2, cells), "allocate p");
-        // FP64 Krylov (3D): x stays in dX64[0], r in FP64; the directions (FP32) in dP / dP2
-        h->krylov64 = h->opt.krylov == NPPE_KRYLOV_FP64 && F.box.threeD;
+        // FP64 Krylov: x stays in dX64[0], r in FP64; the directions (FP32) in dP / dP2
+        h->krylov64 = h->opt.krylov == NPPE_KRYLOV_FP64;
         // automatic sweep counts: V(2,2) on uniform grids; V(4,4) on stretched ones, whose anisotropic cells
         // point smoothing reduces slowly
         const int autoNu = F.host.uniform ? 2 : 4;
