@@ -72,6 +72,8 @@ struct nppe_solver
     double *dB64 = nullptr;
     float *dE = nullptr, *dR = nullptr, *dP = nullptr, *dP2 = nullptr, *dQ = nullptr, *dZ = nullptr, *dZ2 = nullptr;
     double *dR64 = nullptr, *dP64 = nullptr, *dQ64 = nullptr, *dP64b = nullptr;
+    float *dPRing = nullptr;   // FP64 Krylov: the ring of kXRing FP32 directions, slot j at dPRing + j * ringStride
+    int64_t ringStride = 0;
     bool krylov64 = false;   // MGPCG with the FP64 Krylov recursion (nppe_options::krylov)
     int nuPre = 2, nuPost = 2;   // sweeps before / after the coarse correction (nppe_options, auto resolved)
     double *dCompB = nullptr, *dCompX = nullptr, *dCompG = nullptr;

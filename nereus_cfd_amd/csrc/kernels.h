@@ -144,14 +144,20 @@ bool cg_fused_direction(const Level32 &L);
 // other (pA written on even inner iterations).
 hipError_t launch_cg_dirspmv(hipStream_t s, const Level32 &L, const float *z, float *pA, float *pB, float *q,
                              SolveState *st, double *partials);
-// FP64 Krylov around the FP32 preconditioner (3D boxes). Part 1: p = z + beta p formed in FP64, stored in FP32
-// (pA written on even inner iterations, pB on odd), <p, A p> -> alpha, A p in the reference's FP64 arithmetic.
+// FP64 Krylov around the FP32 preconditioner. The directions live in a ring of kXRing FP32 box vectors (slot j at
+// ring + j * stride; stride a multiple of 16): iteration k writes slot k mod kXRing. Part 1: p = z + beta p formed
+// in FP64, stored in FP32, <p, A p> -> alpha (kept in st->alphas too), A p in the reference's FP64 arithmetic.
 hipError_t launch_cg_dir64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const float *z,
-                           float *pA, float *pB, SolveState *st, double *partials);
-// Part 2: q = A p from the same FP32 p, x += alpha p, r -= alpha q (x, r FP64), r32 = (float) r, <r, r> ->
-// convergence flag.
-hipError_t launch_cg_update64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const float *pA,
-                              const float *pB, double *x, double *r, float *r32, SolveState *st, double *partials);
+                           float *ring, int64_t stride, SolveState *st, double *partials);
+// Part 2: q = A p from the same FP32 p, r -= alpha q (FP64), r32 = (float) r, <r, r> -> convergence flag; on the
+// iteration that fills the ring (k mod kXRing = kXRing - 1) also x += alpha_j p_j for the ring's iterations, oldest
+// first -- the per-iteration updates, in their order.
+hipError_t launch_cg_update64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const float *ring,
+                              int64_t stride, double *x, double *r, float *r32, SolveState *st, double *partials);
+// The rest of the deferred update once the recursion is done: x += alpha_j p_j for the slots j < iters mod kXRing
+// (the iterations since the last full ring), at the unknowns. Nothing before st->done.
+hipError_t launch_flush_x(hipStream_t s, const Box &B, const Tables64 &T, const float *ring, int64_t stride, double *x,
+                          const SolveState *st);
 // e += alpha p (e = alpha p on the first iteration), r -= alpha q, <r, r> -> convergence flag; p is pA on even
 // inner iterations and pB on odd ones (pA == pB for an in-place direction).
 hipError_t launch_cg_update(hipStream_t s, const Level32 &L, float *e, float *r, const float *pA, const float *pB,

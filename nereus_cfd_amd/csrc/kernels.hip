@@ -362,6 +362,7 @@ __device__ void fin_apply(SolveState *st, double *partials, int nblk, int nv, in
         break;
     case FIN_PQ:
         st->alpha = t[0] != 0.0 ? st->rz / t[0] : 0.0;
+        st->alphas[st->iters % kXRing] = st->alpha;
         st->first = 0;
         break;
     case FIN_RR:
@@ -532,21 +533,41 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64(Box B, Tabl
     }
 }
 
+// The deferred update's rest at the end of a pass (launch_flush_x): x += alpha_j p_j for the ring slots j < iters mod
+// kXRing, at the unknowns, oldest first; a flat grid-stride pass over the box.
+__global__ __launch_bounds__(kBlockThreads) void k_flush_x(int64_t cells, const uint8_t *__restrict__ c8,
+                                                           const float *__restrict__ ring, int64_t stride,
+                                                           double *__restrict__ x, const SolveState *st)
+{
+#pragma clang fp contract(off)
+    if (!st->done) return;
+    const int rem = st->iters % kXRing;
+    if (rem == 0) return;
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cells; i += step)
+    {
+        if (!(c8[i] & 64u)) continue;
+        double xv = x[i];
+        for (int j = 0; j < rem; j++) xv = xv + st->alphas[j] * (double)ring[(size_t)j * (size_t)stride + i];
+        x[i] = xv;
+    }
+}
+
 // The FP64 Krylov recursion's two passes on any box (2D grids), one thread per cell over the tiles, with the
 // reference's FP64 operator (apply64). Part 1: p = (float)(z + beta p_old) at the unknowns and <p, A p>; p is
 // pointwise, so the neighbours' directions are formed from their z and p_old as well and no pass reads a p it
 // writes. Part 2: q = A p, x += alpha p, r -= alpha q, r32 = (float) r and <r, r>.
 __global__ __launch_bounds__(kBlockThreads) void k_cg_dir64_g(Box B, Tables64 T, const uint8_t *__restrict__ ty,
-                                                              const float *__restrict__ z, float *__restrict__ pA,
-                                                              float *__restrict__ pB, const SolveState *st, Red red)
+                                                              const float *__restrict__ z, float *__restrict__ ring,
+                                                              int64_t stride, const SolveState *st, Red red)
 {
 #pragma clang fp contract(off)
     if (st->done) return;
     const int first = st->first;
     const double beta = st->beta;
-    const bool odd = st->iters & 1;
-    float *__restrict__ pn = odd ? pB : pA;
-    const float *__restrict__ po = odd ? pA : pB;
+    const int slot = st->iters % kXRing;
+    float *__restrict__ pn = ring + (size_t)slot * (size_t)stride;
+    const float *__restrict__ po = ring + (size_t)((slot + kXRing - 1) % kXRing) * (size_t)stride;
     auto PV = [&](int64_t i) { return (double)(float)(first ? (double)z[i] : (double)z[i] + beta * (double)po[i]); };
     const TileRange tr = my_tiles(B.blocks());
     double pq = 0.0;
@@ -566,8 +587,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_dir64_g(Box B, Tables64 T,
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_cg_update64_g(Box B, Tables64 T, const uint8_t *__restrict__ ty,
-                                                                 const float *__restrict__ pA,
-                                                                 const float *__restrict__ pB,
+                                                                 const float *__restrict__ ring, int64_t stride,
                                                                  double *__restrict__ x, double *__restrict__ r,
                                                                  float *__restrict__ r32, const SolveState *st,
                                                                  Red red)
@@ -575,7 +595,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_update64_g(Box B, Tables64
 #pragma clang fp contract(off)
     if (st->done) return;
     const double alpha = st->alpha;
-    const float *__restrict__ p = (st->iters & 1) ? pB : pA;
+    const int slot = st->iters % kXRing;
+    const bool flush = slot == kXRing - 1;
+    const float *__restrict__ p = ring + (size_t)slot * (size_t)stride;
     const TileRange tr = my_tiles(B.blocks());
     double rr = 0.0;
     for (int t = tr.begin; t < tr.end; t++)
@@ -586,11 +608,18 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_update64_g(Box B, Tables64
         if (!(T.c8[idx] & 64u)) continue;
         const double pc = p[idx];
         const double q = apply64(B, T, ty, s, m, f, idx, [&](int64_t d) { return (double)p[idx + d]; });
-        const double xv = x[idx] + alpha * pc, rv = r[idx] - alpha * q;
-        x[idx] = xv;
+        const double rv = r[idx] - alpha * q;
         r[idx] = rv;
         r32[idx] = (float)rv;
         rr += rv * rv;
+        if (flush)
+        {
+            double xv = x[idx];
+#pragma unroll
+            for (int j = 0; j < kXRing; j++)
+                xv = xv + st->alphas[j] * (j == kXRing - 1 ? pc : (double)ring[(size_t)j * (size_t)stride + idx]);
+            x[idx] = xv;
+        }
     }
     double a[1] = {rr};
     block_sum<1>(a, red, gridDim.x, tr.lb);
@@ -2116,17 +2145,16 @@ __device__ __forceinline__ void tile_halo(int t, int &hy, int &hx)
 // (odd); plane p staged in LDS, slow neighbours in registers, loads a plane ahead.
 template <bool UNI>
 __global__ __launch_bounds__(kBlockThreads) void k_cg_dir64_t(Box B, Tables64 T, const uint8_t *__restrict__ ty,
-                                                              const float *__restrict__ z, float *__restrict__ pA,
-                                                              float *__restrict__ pB, const SolveState *st, Red red,
-                                                              int P)
+                                                              const float *__restrict__ z, float *__restrict__ ring,
+                                                              int64_t stride, const SolveState *st, Red red, int P)
 {
 #pragma clang fp contract(off)
     if (st->done) return;
     const int first = st->first;
     const double beta = st->beta;
-    const bool odd = st->iters & 1;
-    float *__restrict__ pn = odd ? pB : pA;
-    const float *__restrict__ po = odd ? pA : pB;
+    const int slot = st->iters % kXRing;
+    float *__restrict__ pn = ring + (size_t)slot * (size_t)stride;
+    const float *__restrict__ po = ring + (size_t)((slot + kXRing - 1) % kXRing) * (size_t)stride;
     __shared__ double Ps[kBlockMid + 2][kBlockFast + 2];
     const March M = march_setup(B, P);
     const int tx = threadIdx.x & (kBlockFast - 1), tyy = threadIdx.x / kBlockFast;
@@ -2174,8 +2202,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_dir64_t(Box B, Tables64 T,
 // Part 2: q = A p (FP64, from the FP32 p of part 1), x += alpha p, r -= alpha q, r32 = (float) r, and <r, r>.
 template <bool UNI>
 __global__ __launch_bounds__(kBlockThreads) void k_cg_update64_t(Box B, Tables64 T, const uint8_t *__restrict__ ty,
-                                                                 const float *__restrict__ pA,
-                                                                 const float *__restrict__ pB,
+                                                                 const float *__restrict__ ring, int64_t stride,
                                                                  double *__restrict__ x, double *__restrict__ r,
                                                                  float *__restrict__ r32, const SolveState *st,
                                                                  Red red, int P)
@@ -2183,7 +2210,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_update64_t(Box B, Tables64
 #pragma clang fp contract(off)
     if (st->done) return;
     const double alpha = st->alpha;
-    const float *__restrict__ p = (st->iters & 1) ? pB : pA;
+    const int slot = st->iters % kXRing;
+    const bool flush = slot == kXRing - 1;
+    const float *__restrict__ p = ring + (size_t)slot * (size_t)stride;
     __shared__ double Ps[kBlockMid + 2][kBlockFast + 2];
     const March M = march_setup(B, P);
     const int tx = threadIdx.x & (kBlockFast - 1), tyy = threadIdx.x / kBlockFast;
@@ -2200,13 +2229,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_update64_t(Box B, Tables64
     double Pm = p[plane(M.s0 - 1) + own], Pc = p[plane(M.s0) + own], Pp = p[plane(M.s0 + 1) + own];
     double H = halo ? (double)p[plane(M.s0) + hoff] : 0.0;
     unsigned cc = T.c8[plane(M.s0) + own];
-    double xc = x[plane(M.s0) + own], rc = r[plane(M.s0) + own];
+    double rc = r[plane(M.s0) + own];
     for (int sl = M.s0; sl <= M.s1; sl++)
     {
         const double Pn = p[plane(sl + 2) + own];
         const double Hn = halo ? (double)p[plane(sl + 1) + hoff] : 0.0;
         const unsigned cn = T.c8[plane(sl + 1) + own];
-        const double xn = x[plane(sl + 1) + own], rn = r[plane(sl + 1) + own];
+        const double rn = r[plane(sl + 1) + own];
         Ps[tyy + 1][tx + 1] = Pc;
         if (halo) Ps[hy][hx] = H;
         __syncthreads();
@@ -2215,11 +2244,19 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_update64_t(Box B, Tables64
         {
             const double nb[6] = {Pp, Pm, Ps[tyy + 2][tx + 1], Ps[tyy][tx + 1], Ps[tyy + 1][tx + 2], Ps[tyy + 1][tx]};
             const double q = UNI ? apply64u(T, idx, Pc, nb, cc) : apply64v(B, T, ty, sl, M.m, M.f, idx, Pc, nb, cc);
-            const double xv = xc + alpha * Pc, rv = rc - alpha * q;
-            x[idx] = xv;
+            const double rv = rc - alpha * q;
             r[idx] = rv;
             r32[idx] = (float)rv;
             rr += rv * rv;
+            if (flush)
+            {
+                // x += alpha_j p_j over the ring's iterations, oldest first (this iteration's p last)
+                double xv = x[idx];
+#pragma unroll
+                for (int j = 0; j < kXRing; j++)
+                    xv = xv + st->alphas[j] * (j == kXRing - 1 ? Pc : (double)ring[(size_t)j * (size_t)stride + idx]);
+                x[idx] = xv;
+            }
         }
         __syncthreads();
         Pm = Pc;
@@ -2227,7 +2264,6 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_update64_t(Box B, Tables64
         Pp = Pn;
         H = Hn;
         cc = cn;
-        xc = xn;
         rc = rn;
     }
     double a[1] = {rr};
@@ -2316,16 +2352,16 @@ __device__ __forceinline__ RowMarch<TM, K> row_march(const Box &B, int P, int ti
 // Part 1 (see k_cg_dir64_t): p = (float)(z + beta p_old) (z on the first iteration) at the unknowns, and <p, A p>.
 template <int TM, int K>
 __device__ __forceinline__ void cg_dir64_row_body(const Box &B, const Tables64 &T, const float *__restrict__ z,
-                                                  float *__restrict__ pA, float *__restrict__ pB, const SolveState *st,
+                                                  float *__restrict__ ring, int64_t stride, const SolveState *st,
                                                   const Red &red, int P, int tilesM, int lb, int nblk)
 {
 #pragma clang fp contract(off)
     if (st->done) return;
     const int first = st->first;
     const double beta = st->beta;
-    const bool odd = st->iters & 1;
-    float *__restrict__ pn = odd ? pB : pA;
-    const float *__restrict__ po = odd ? pA : pB;
+    const int slot = st->iters % kXRing;
+    float *__restrict__ pn = ring + (size_t)slot * (size_t)stride;
+    const float *__restrict__ po = ring + (size_t)((slot + kXRing - 1) % kXRing) * (size_t)stride;
     constexpr int kSlots = 64 * K + 2;
     __shared__ float ldsP[2][TM + 2][2][kSlots];   // new p of a plane by parity, row, element and pair + 1
     const RowMarch<TM, K> R = row_march<TM, K>(B, P, tilesM, lb);
@@ -2430,10 +2466,11 @@ __device__ __forceinline__ void cg_dir64_row_body(const Box &B, const Tables64 &
     block_sum<1>(a, red, nblk, lb);
 }
 
-// Part 2 (see k_cg_update64_t): q = A p, x += alpha p, r -= alpha q, r32 = (float) r at the unknowns, and <r, r>.
+// Part 2 (see k_cg_update64_t): q = A p, r -= alpha q, r32 = (float) r at the unknowns, and <r, r>; on the
+// iteration that fills the ring of directions, x += alpha_j p_j for the ring's iterations in order.
 template <int TM, int K>
-__device__ __forceinline__ void cg_update64_row_body(const Box &B, const Tables64 &T, const float *__restrict__ pA,
-                                                     const float *__restrict__ pB, double *__restrict__ x,
+__device__ __forceinline__ void cg_update64_row_body(const Box &B, const Tables64 &T, const float *__restrict__ ring,
+                                                     int64_t stride, double *__restrict__ x,
                                                      double *__restrict__ r, float *__restrict__ r32,
                                                      const SolveState *st, const Red &red, int P, int tilesM, int lb,
                                                      int nblk)
@@ -2441,7 +2478,9 @@ __device__ __forceinline__ void cg_update64_row_body(const Box &B, const Tables6
 #pragma clang fp contract(off)
     if (st->done) return;
     const double alpha = st->alpha;
-    const float *__restrict__ p = (st->iters & 1) ? pB : pA;
+    const int slot = st->iters % kXRing;
+    const bool flush = slot == kXRing - 1;
+    const float *__restrict__ p = ring + (size_t)slot * (size_t)stride;
     constexpr int kSlots = 64 * K + 2;
     __shared__ float ldsP[2][TM + 2][2][kSlots];
     const RowMarch<TM, K> R = row_march<TM, K>(B, P, tilesM, lb);
@@ -2452,7 +2491,7 @@ __device__ __forceinline__ void cg_update64_row_body(const Box &B, const Tables6
     auto pairC = [](const uint8_t *row, uint32_t o) { return (unsigned)*reinterpret_cast<const uint16_t *>(row + o); };
     float2 Pv[4][K];
     unsigned C[4][K];
-    double2 Xv[4][K], Rv[4][K];   // own x and r (interior waves), loaded two planes ahead
+    double2 Rv[4][K];   // own r (interior waves), loaded two planes ahead
 #pragma unroll
     for (int k = 0; k < K; k++)
 #pragma unroll
@@ -2460,7 +2499,7 @@ __device__ __forceinline__ void cg_update64_row_body(const Box &B, const Tables6
         {
             Pv[j][k] = make_float2(0.f, 0.f);
             C[j][k] = 0u;
-            Xv[j][k] = Rv[j][k] = make_double2(0.0, 0.0);
+            Rv[j][k] = make_double2(0.0, 0.0);
         }
     const int sp0 = R.s0 - 1;
 #pragma unroll
@@ -2481,11 +2520,7 @@ __device__ __forceinline__ void cg_update64_row_body(const Box &B, const Tables6
         {
             const size_t pl = plane(sp0 + d);
 #pragma unroll
-            for (int k = 0; k < K; k++)
-            {
-                Xv[d][k] = pair64(x + pl, R.off[k]);
-                Rv[d][k] = pair64(r + pl, R.off[k]);
-            }
+            for (int k = 0; k < K; k++) Rv[d][k] = pair64(r + pl, R.off[k]);
         }
     }
     __syncthreads();
@@ -2507,11 +2542,7 @@ __device__ __forceinline__ void cg_update64_row_body(const Box &B, const Tables6
         {
             const size_t pl = plane(sp + 2);
 #pragma unroll
-            for (int k = 0; k < K; k++)
-            {
-                Xv[S2][k] = pair64(x + pl, R.off[k]);
-                Rv[S2][k] = pair64(r + pl, R.off[k]);
-            }
+            for (int k = 0; k < K; k++) Rv[S2][k] = pair64(r + pl, R.off[k]);
         }
         float(*buf)[2][kSlots] = ldsP[sp & 1];
 #pragma unroll
@@ -2527,7 +2558,7 @@ __device__ __forceinline__ void cg_update64_row_body(const Box &B, const Tables6
         for (int k = 0; k < K; k++)
         {
             const int q = 1 + R.lane + 64 * k;
-            double xo[2] = {Xv[S0][k].x, Xv[S0][k].y}, ro[2] = {Rv[S0][k].x, Rv[S0][k].y};
+            double ro[2] = {Rv[S0][k].x, Rv[S0][k].y};
             bool any = false;
 #pragma unroll
             for (int e = 0; e < 2; e++)
@@ -2541,7 +2572,6 @@ __device__ __forceinline__ void cg_update64_row_body(const Box &B, const Tables6
                                       (double)(e ? buf[R.wave][0][q + 1] : Pv[S0][k].y),
                                       (double)(e ? Pv[S0][k].x : buf[R.wave][1][q - 1])};
                 const double qv = apply64u_z(T, (int64_t)pl + R.off[k] + e, pc, nb, code);
-                xo[e] = xo[e] + alpha * pc;
                 ro[e] = ro[e] - alpha * qv;
                 rr += ro[e] * ro[e];
             }
@@ -2550,15 +2580,48 @@ __device__ __forceinline__ void cg_update64_row_body(const Box &B, const Tables6
             const int64_t idx = (int64_t)pl + R.off[k];
             if (R.okB[k])
             {
-                *reinterpret_cast<double2 *>(x + idx) = make_double2(xo[0], xo[1]);
                 *reinterpret_cast<double2 *>(r + idx) = make_double2(ro[0], ro[1]);
                 *reinterpret_cast<float2 *>(r32 + idx) = make_float2((float)ro[0], (float)ro[1]);
             }
             else
             {
-                x[idx] = xo[0];
                 r[idx] = ro[0];
                 r32[idx] = (float)ro[0];
+            }
+            if (flush)
+            {
+                // x += alpha_j p_j over the ring's iterations, oldest first (this iteration's p last)
+                double xo[2];
+                if (R.okB[k])
+                {
+                    const double2 xv = *reinterpret_cast<const double2 *>(x + idx);
+                    xo[0] = xv.x;
+                    xo[1] = xv.y;
+                }
+                else
+                {
+                    xo[0] = x[idx];
+                    xo[1] = 0.0;
+                }
+#pragma unroll
+                for (int j = 0; j < kXRing; j++)
+                {
+                    const double aj = st->alphas[j];
+                    const float *rj = ring + (size_t)j * (size_t)stride + idx;
+                    const float2 pj = j == kXRing - 1 ? Pv[S0][k]
+                                                      : (R.okB[k] ? *reinterpret_cast<const float2 *>(rj)
+                                                                  : make_float2(rj[0], 0.f));
+#pragma unroll
+                    for (int e = 0; e < 2; e++)
+                    {
+                        const unsigned code = (e ? R.okB[k] : R.okA[k]) ? ((C[S0][k] >> (8 * e)) & 255u) : 0u;
+                        if (code & 64u) xo[e] = xo[e] + aj * (double)(e ? pj.y : pj.x);
+                    }
+                }
+                if (R.okB[k])
+                    *reinterpret_cast<double2 *>(x + idx) = make_double2(xo[0], xo[1]);
+                else
+                    x[idx] = xo[0];
             }
         }
     };
@@ -3339,7 +3402,8 @@ struct Dir64RowArgs
     Box B;
     Tables64 T;
     const float *z;
-    float *pA, *pB;
+    float *ring;
+    int64_t stride;
     const SolveState *st;
     Red red;
     int P, tilesM;
@@ -3349,7 +3413,8 @@ struct Update64RowArgs
 {
     Box B;
     Tables64 T;
-    const float *pA, *pB;
+    const float *ring;
+    int64_t stride;
     double *x, *r;
     float *r32;
     const SolveState *st;
@@ -3475,10 +3540,10 @@ __global__ __launch_bounds__(kBlockThreads) void kb_residual_restrict_m(const Re
 
 template <int TM, int K>
 __global__ __launch_bounds__(64 * (TM + 2)) void k_cg_dir64_row(Box B, Tables64 T, const float *__restrict__ z,
-                                                                float *__restrict__ pA, float *__restrict__ pB,
+                                                                float *__restrict__ ring, int64_t stride,
                                                                 const SolveState *st, Red red, int P, int tilesM)
 {
-    cg_dir64_row_body<TM, K>(B, T, z, pA, pB, st, red, P, tilesM, logical_block(), gridDim.x);
+    cg_dir64_row_body<TM, K>(B, T, z, ring, stride, st, red, P, tilesM, logical_block(), gridDim.x);
 }
 
 template <int TM, int K>
@@ -3487,17 +3552,17 @@ __global__ __launch_bounds__(64 * (TM + 2)) void kb_cg_dir64_row(const Dir64RowA
 {
     const BatchBlock bb = batch_block(start, nsys);
     const Dir64RowArgs a = args[bb.q];
-    cg_dir64_row_body<TM, K>(a.B, a.T, a.z, a.pA, a.pB, a.st, a.red, a.P, a.tilesM, bb.lb, bb.nb);
+    cg_dir64_row_body<TM, K>(a.B, a.T, a.z, a.ring, a.stride, a.st, a.red, a.P, a.tilesM, bb.lb, bb.nb);
 }
 
 template <int TM, int K>
-__global__ __launch_bounds__(64 * (TM + 2), 4) void k_cg_update64_row(Box B, Tables64 T, const float *__restrict__ pA,
-                                                                   const float *__restrict__ pB,
+__global__ __launch_bounds__(64 * (TM + 2), 4) void k_cg_update64_row(Box B, Tables64 T,
+                                                                   const float *__restrict__ ring, int64_t stride,
                                                                    double *__restrict__ x, double *__restrict__ r,
                                                                    float *__restrict__ r32, const SolveState *st,
                                                                    Red red, int P, int tilesM)
 {
-    cg_update64_row_body<TM, K>(B, T, pA, pB, x, r, r32, st, red, P, tilesM, logical_block(), gridDim.x);
+    cg_update64_row_body<TM, K>(B, T, ring, stride, x, r, r32, st, red, P, tilesM, logical_block(), gridDim.x);
 }
 
 template <int TM, int K>
@@ -3506,7 +3571,8 @@ __global__ __launch_bounds__(64 * (TM + 2), 4) void kb_cg_update64_row(const Upd
 {
     const BatchBlock bb = batch_block(start, nsys);
     const Update64RowArgs a = args[bb.q];
-    cg_update64_row_body<TM, K>(a.B, a.T, a.pA, a.pB, a.x, a.r, a.r32, a.st, a.red, a.P, a.tilesM, bb.lb, bb.nb);
+    cg_update64_row_body<TM, K>(a.B, a.T, a.ring, a.stride, a.x, a.r, a.r32, a.st, a.red, a.P, a.tilesM, bb.lb,
+                                bb.nb);
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_coarse_solve(int n, const int32_t *__restrict__ order,
@@ -3713,15 +3779,15 @@ template <int TM, int K>
 void dir64_row(hipStream_t s, int grid, int threads, const Dir64RowArgs &a)
 {
     if (record(kb_cg_dir64_row<TM, K>, grid, threads, a)) return;
-    k_cg_dir64_row<TM, K><<<grid, threads, 0, s>>>(a.B, a.T, a.z, a.pA, a.pB, a.st, a.red, a.P, a.tilesM);
+    k_cg_dir64_row<TM, K><<<grid, threads, 0, s>>>(a.B, a.T, a.z, a.ring, a.stride, a.st, a.red, a.P, a.tilesM);
 }
 
 template <int TM, int K>
 void update64_row(hipStream_t s, int grid, int threads, const Update64RowArgs &a)
 {
     if (record(kb_cg_update64_row<TM, K>, grid, threads, a)) return;
-    k_cg_update64_row<TM, K><<<grid, threads, 0, s>>>(a.B, a.T, a.pA, a.pB, a.x, a.r, a.r32, a.st, a.red, a.P,
-                                                      a.tilesM);
+    k_cg_update64_row<TM, K><<<grid, threads, 0, s>>>(a.B, a.T, a.ring, a.stride, a.x, a.r, a.r32, a.st, a.red,
+                                                      a.P, a.tilesM);
 }
 }   // namespace
 
@@ -4304,21 +4370,21 @@ hipError_t launch_cg_dirspmv(hipStream_t s, const Level32 &L, const float *z, fl
 }
 
 hipError_t launch_cg_dir64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const float *z,
-                           float *pA, float *pB, SolveState *st, double *partials)
+                           float *ring, int64_t stride, SolveState *st, double *partials)
 {
     // 6-row blocks, two per CU (measured at 256^3: 57 us against 61 us for one 14-row block per CU)
     if (!B.threeD)
     {
         const int G = grid_of(B);
         const Red red = red_of(partials, st, FIN_PQ, G);
-        k_cg_dir64_g<<<G, kBlockThreads, 0, s>>>(B, T, ty, z, pA, pB, st, red);
+        k_cg_dir64_g<<<G, kBlockThreads, 0, s>>>(B, T, ty, z, ring, stride, st, red);
         return finish(s, red, G, 1);
     }
-    const RowPlan rp = row_plan(B, T.c8, T.unitCoef, {{z, 8}, {pA, 8}, {pB, 8}}, 6, 2);
+    const RowPlan rp = row_plan(B, T.c8, T.unitCoef, {{z, 8}, {ring, 8}}, 6, 2);
     if (rp.ok)
     {
         const Red red = red_of(partials, st, FIN_PQ, rp.G);
-        const Dir64RowArgs a{B, T, z, pA, pB, st, red, rp.P, rp.tilesM};
+        const Dir64RowArgs a{B, T, z, ring, stride, st, red, rp.P, rp.tilesM};
         if (rp.K == 1)
             dir64_row<6, 1>(s, rp.G, 64 * 8, a);
         else
@@ -4328,27 +4394,27 @@ hipError_t launch_cg_dir64(hipStream_t s, const Box &B, const Tables64 &T, const
     const int P = march_planes(B), G = march_grid(B, P);
     const Red red = red_of(partials, st, FIN_PQ, G);
     if (T.unitCoef)
-        k_cg_dir64_t<true><<<G, kBlockThreads, 0, s>>>(B, T, ty, z, pA, pB, st, red, P);
+        k_cg_dir64_t<true><<<G, kBlockThreads, 0, s>>>(B, T, ty, z, ring, stride, st, red, P);
     else
-        k_cg_dir64_t<false><<<G, kBlockThreads, 0, s>>>(B, T, ty, z, pA, pB, st, red, P);
+        k_cg_dir64_t<false><<<G, kBlockThreads, 0, s>>>(B, T, ty, z, ring, stride, st, red, P);
     return finish(s, red, G, 1);
 }
 
-hipError_t launch_cg_update64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const float *pA,
-                              const float *pB, double *x, double *r, float *r32, SolveState *st, double *partials)
+hipError_t launch_cg_update64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const float *ring,
+                              int64_t stride, double *x, double *r, float *r32, SolveState *st, double *partials)
 {
     if (!B.threeD)
     {
         const int G = grid_of(B);
         const Red red = red_of(partials, st, FIN_RR, G);
-        k_cg_update64_g<<<G, kBlockThreads, 0, s>>>(B, T, ty, pA, pB, x, r, r32, st, red);
+        k_cg_update64_g<<<G, kBlockThreads, 0, s>>>(B, T, ty, ring, stride, x, r, r32, st, red);
         return finish(s, red, G, 1);
     }
-    const RowPlan rp = row_plan(B, T.c8, T.unitCoef, {{pA, 8}, {pB, 8}, {x, 16}, {r, 16}, {r32, 8}}, 6, 2);
+    const RowPlan rp = row_plan(B, T.c8, T.unitCoef, {{ring, 8}, {x, 16}, {r, 16}, {r32, 8}}, 6, 2);
     if (rp.ok)
     {
         const Red red = red_of(partials, st, FIN_RR, rp.G);
-        const Update64RowArgs a{B, T, pA, pB, x, r, r32, st, red, rp.P, rp.tilesM};
+        const Update64RowArgs a{B, T, ring, stride, x, r, r32, st, red, rp.P, rp.tilesM};
         if (rp.K == 1)
             update64_row<6, 1>(s, rp.G, 64 * 8, a);
         else
@@ -4358,10 +4424,18 @@ hipError_t launch_cg_update64(hipStream_t s, const Box &B, const Tables64 &T, co
     const int P = march_planes(B), G = march_grid(B, P);
     const Red red = red_of(partials, st, FIN_RR, G);
     if (T.unitCoef)
-        k_cg_update64_t<true><<<G, kBlockThreads, 0, s>>>(B, T, ty, pA, pB, x, r, r32, st, red, P);
+        k_cg_update64_t<true><<<G, kBlockThreads, 0, s>>>(B, T, ty, ring, stride, x, r, r32, st, red, P);
     else
-        k_cg_update64_t<false><<<G, kBlockThreads, 0, s>>>(B, T, ty, pA, pB, x, r, r32, st, red, P);
+        k_cg_update64_t<false><<<G, kBlockThreads, 0, s>>>(B, T, ty, ring, stride, x, r, r32, st, red, P);
     return finish(s, red, G, 1);
+}
+
+hipError_t launch_flush_x(hipStream_t s, const Box &B, const Tables64 &T, const float *ring, int64_t stride, double *x,
+                          const SolveState *st)
+{
+    const int G = grid_1d(B.cells);
+    k_flush_x<<<G, kBlockThreads, 0, s>>>(B.cells, T.c8, ring, stride, x, st);
+    return hipGetLastError();
 }
 
 hipError_t launch_cg_update(hipStream_t s, const Level32 &L, float *e, float *r, const float *pA, const float *pB,

@@ -70,6 +70,12 @@ struct Tables64
 // there, and the host reports NPPE_ERR_NONFINITE instead of running on to the iteration cap.
 constexpr int kDoneNonFinite = 3;
 
+// Directions the FP64 Krylov recursion keeps for its deferred solution update: x += alpha p is applied for
+// kXRing iterations at once (every kXRing-th iteration, and once more at the end of a pass), from the directions
+// in a ring of kXRing FP32 box vectors and their alphas, in iteration order -- the same operations, in the same
+// order, as one update per iteration, for a sixth of the x traffic.
+constexpr int kXRing = 8;
+
 struct SolveState
 {
     double rz;        // <r, z> of the current search direction
@@ -89,6 +95,7 @@ struct SolveState
     double acc2;
     int skip;
     int accepted;
+    double alphas[kXRing];   // alpha of the iteration whose direction sits in ring slot (iteration mod kXRing)
 };
 
 }  // namespace nppe

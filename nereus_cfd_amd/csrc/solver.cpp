@@ -90,6 +90,8 @@ void free_grid(nppe_solver *h)
     h->dB64 = nullptr;
     h->dE = h->dR = h->dP = h->dP2 = h->dQ = h->dZ = h->dZ2 = nullptr;
     h->dR64 = h->dP64 = h->dQ64 = h->dP64b = nullptr;
+    h->dPRing = nullptr;
+    h->ringStride = 0;
     h->krylov64 = false;
     h->dCompB = h->dCompX = h->dCompG = nullptr;
     h->dCoarseM = nullptr;
@@ -329,17 +331,18 @@ hipError_t inner_iteration(nppe_solver *h, Traffic &tr)
         // r32 refreshed for the V-cycle
         {
             Scope sc(h, NPPE_K_CG_SPMV, 13.0 * N);   // z, p_old, codes in; p out
-            if ((e = launch_cg_dir64(h->stream, L0.box, h->t64, L0.types, z, h->dP, h->dP2, h->dState,
+            if ((e = launch_cg_dir64(h->stream, L0.box, h->t64, L0.types, z, h->dPRing, h->ringStride, h->dState,
                                      h->dPartials)) != hipSuccess)
                 return e;
         }
         {
-            Scope sc(h, NPPE_K_CG_UPDATE, 41.0 * N);   // p, codes, x, r in; x, r, r32 out
-            if ((e = launch_cg_update64(h->stream, L0.box, h->t64, L0.types, h->dP, h->dP2, h->dX64[0], h->dR64,
-                                        h->dR, h->dState, h->dPartials)) != hipSuccess)
+            // p, codes, r in; r, r32 out (x: every kXRing-th iteration, counted when the pass ends)
+            Scope sc(h, NPPE_K_CG_UPDATE, 25.0 * N);
+            if ((e = launch_cg_update64(h->stream, L0.box, h->t64, L0.types, h->dPRing, h->ringStride, h->dX64[0],
+                                        h->dR64, h->dR, h->dState, h->dPartials)) != hipSuccess)
                 return e;
         }
-        tr.add((rzDone ? 0.0 : (2 * s + f) * N) + 13.0 * N + 41.0 * N);
+        tr.add((rzDone ? 0.0 : (2 * s + f) * N) + 13.0 * N + 25.0 * N);
         return hipSuccess;
     }
     const bool fused = cg_fused_direction(L0.v);
@@ -817,13 +820,21 @@ int build(nppe_solver *h, const nppe_grid *g)
     HIP_TRY(h, dalloc_pairs(h, &h->dB64, cells), "allocate b64");
     if (h->opt.method == NPPE_METHOD_MGPCG)
     {
-        HIP_TRY(h, dalloc_pairs(h, &h->dE, cells), "allocate e");
-        HIP_TRY(h, dalloc_pairs(h, &h->dR, cells), "allocate r");
-        HIP_TRY(h, dalloc_pairs(h, &h->dP, cells), "allocate p");
-        HIP_TRY(h, dalloc_pairs(h, &h->dQ, cells), "allocate q");
-        HIP_TRY(h, dalloc_pairs(h, &h->dP2, cells), "allocate p");
-        // FP64 Krylov: x stays in dX64[0], r in FP64; the directions (FP32) in dP / dP2
+        // FP64 Krylov: x stays in dX64[0], r in FP64; the directions (FP32) in a ring of kXRing box vectors
         h->krylov64 = h->opt.krylov == NPPE_KRYLOV_FP64;
+        HIP_TRY(h, dalloc_pairs(h, &h->dR, cells), "allocate r");
+        if (h->krylov64)
+        {
+            h->ringStride = (int64_t)((cells + 16 + 15) / 16 * 16);
+            HIP_TRY(h, dalloc_pairs(h, &h->dPRing, (size_t)kXRing * (size_t)h->ringStride), "allocate directions");
+        }
+        else
+        {
+            HIP_TRY(h, dalloc_pairs(h, &h->dE, cells), "allocate e");
+            HIP_TRY(h, dalloc_pairs(h, &h->dP, cells), "allocate p");
+            HIP_TRY(h, dalloc_pairs(h, &h->dQ, cells), "allocate q");
+            HIP_TRY(h, dalloc_pairs(h, &h->dP2, cells), "allocate p");
+        }
         // automatic sweep counts: V(2,2) on uniform grids; V(4,4) on stretched ones, whose anisotropic cells
         // point smoothing reduces slowly
         const int autoNu = F.host.uniform ? 2 : 4;
@@ -890,6 +901,7 @@ int zero_work(nppe_solver *h)
     auto z = [&](void *p, size_t bytes) { return p ? hipMemsetAsync(p, 0, bytes, h->stream) : hipSuccess; };
     float *f32[] = {h->dE, h->dR, h->dP, h->dQ, h->dP2, h->dZ, h->dZ2};
     for (float *p : f32) HIP_TRY(h, z(p, cells * sizeof(float)), "clear work vectors");
+    HIP_TRY(h, z(h->dPRing, (size_t)kXRing * (size_t)h->ringStride * sizeof(float)), "clear work vectors");
     double *f64[] = {h->dX64[0], h->dX64[1], h->dB64, h->dR64, h->dP64, h->dQ64};
     for (double *p : f64) HIP_TRY(h, z(p, cells * sizeof(double)), "clear work vectors");
     for (size_t l = 1; l < h->levels.size(); l++)
@@ -1182,6 +1194,8 @@ int k64_tail(K64Run &R)
     nppe_solver *h = R.h;
     const DevLevel &L0 = h->levels[0];
     const double N = L0.cellsBox, n = (double)h->n;
+    HIP_TRY(h, launch_flush_x(h->stream, L0.box, h->t64, h->dPRing, h->ringStride, h->dX64[R.cur], h->dState),
+            "solution update");
     {
         Scope sc(h, NPPE_K_RESIDUAL64, 29.0 * N);
         // (with the projection it also sums x over the unknowns for the mean)
@@ -1232,7 +1246,10 @@ int k64_advance(K64Run &R)
         return NPPE_OK;
     }
     R.inner += st.iters;
-    R.tr->add(R.perIter * st.iters + 29.0 * N);
+    // the deferred solution updates: a full ring every kXRing iterations, the rest at the end of the pass
+    const int fulls = st.iters / kXRing, rest = st.iters % kXRing;
+    R.tr->add(R.perIter * st.iters + 29.0 * N + fulls * (4.0 * (kXRing - 1) + 16.0) * N +
+              (rest ? (4.0 * rest + 16.0 + 1.0) * N : 0.0));
     R.rnorm = std::sqrt(st.sums[1]);
     if (!std::isfinite(R.rnorm)) return fail(h, NPPE_ERR_NONFINITE, "non-finite residual during refinement");
     // a pass that ran iterations counts; the guess accepted as it is does not (as the synchronous loop)
@@ -1897,6 +1914,8 @@ int solve_mgpcg(nppe_solver *h, const double *dB, double *dX, const double *dGue
             int done = 0;
             if ((rc = run_inner(h, std::min(0.5, aim / rnorm), rnorm * rnorm, tr, &done))) return rc;
             inner += done;
+            HIP_TRY(h, launch_flush_x(h->stream, L0.box, h->t64, h->dPRing, h->ringStride, h->dX64[cur], h->dState),
+                    "solution update");
             {
                 Scope sc(h, NPPE_K_RESIDUAL64, 29.0 * N);
                 HIP_TRY(h, launch_update_residual64(h->stream, L0.box, h->t64, L0.types, h->dX64[cur], nullptr,
@@ -2525,7 +2544,7 @@ uint64_t nppe_estimate_bytes_for(const nppe_grid *g, const nppe_options *opt)
     nppe_default_options(&o);
     if (opt) o = *opt;
     const int32_t method = o.method;
-    const bool k64 = method == NPPE_METHOD_MGPCG && o.krylov == NPPE_KRYLOV_FP64 && g && g->dim == 3;
+    const bool k64 = method == NPPE_METHOD_MGPCG && o.krylov == NPPE_KRYLOV_FP64;
     if (!g || (g->dim != 2 && g->dim != 3) || g->nx < 3 || g->ny < 3) return 0;
     const bool t3 = g->dim == 3;
     int ns = t3 ? g->ny : 1, nm = t3 ? g->nx : g->ny, nf = t3 ? g->nz : g->nx;
@@ -2538,8 +2557,9 @@ uint64_t nppe_estimate_bytes_for(const nppe_grid *g, const nppe_options *opt)
         if (l == 0)
         {
             total += cells * 24.0;                                      // x64 pair, b64
-            total += cells * (method == NPPE_METHOD_CG64 ? 24.0 : 28.0);   // FP64 r, p, q / FP32 e, r, p pair, q, z pair
-            if (k64) total += cells * 8.0;                                  // FP64 Krylov r
+            // FP64 r, p, q / FP32 e, r, p pair, q, z pair / FP32 r, z pair, the ring of directions and FP64 r
+            total += cells * (method == NPPE_METHOD_CG64 ? 24.0 : (k64 ? 12.0 + 4.0 * kXRing + 8.0 : 28.0));
+            if (k64) total += 4.0 * kXRing * 32.0;
             total += (double)(ns - (t3 ? 2 : 0)) * (nm - 2) * (nf - 2) * (24.0 + 4.0);   // staging + order
             total += (double)ns * nm * 4.0;                                             // field row starts
         }
