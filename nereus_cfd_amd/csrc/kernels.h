@@ -155,9 +155,10 @@ hipError_t launch_cg_dir64(hipStream_t s, const Box &B, const Tables64 &T, const
 hipError_t launch_cg_update64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const float *ring,
                               int64_t stride, double *x, double *r, float *r32, SolveState *st, double *partials);
 // The rest of the deferred update once the recursion is done: x += alpha_j p_j for the slots j < iters mod kXRing
-// (the iterations since the last full ring), at the unknowns. Nothing before st->done.
+// (the iterations since the last full ring), at the unknowns. Nothing before st->done. xz: the pass began at
+// x = 0, so x is not read while no full ring has been applied (the same sums from 0.0).
 hipError_t launch_flush_x(hipStream_t s, const Box &B, const Tables64 &T, const float *ring, int64_t stride, double *x,
-                          const SolveState *st);
+                          const SolveState *st, int xz);
 // e += alpha p (e = alpha p on the first iteration), r -= alpha q, <r, r> -> convergence flag; p is pA on even
 // inner iterations and pB on odd ones (pA == pB for an in-place direction).
 hipError_t launch_cg_update(hipStream_t s, const Level32 &L, float *e, float *r, const float *pA, const float *pB,

@@ -534,22 +534,44 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64(Box B, Tabl
 }
 
 // The deferred update's rest at the end of a pass (launch_flush_x): x += alpha_j p_j for the ring slots j < iters mod
-// kXRing, at the unknowns, oldest first; a flat grid-stride pass over the box.
-__global__ __launch_bounds__(kBlockThreads) void k_flush_x(int64_t cells, const uint8_t *__restrict__ c8,
+// kXRing, at the unknowns, oldest first. Four cells per thread (elements 4q - 1 .. 4q + 2: element -1 of every
+// box array is 16-byte aligned, and the allocations extend past the box), a float4 of each direction and two
+// double2 of x. xz: the pass began at x = 0 (a solve without a guess); while no full ring has been applied x is
+// still that zero, so the sums start from 0.0 instead of reading it -- the same additions, the same bits.
+__global__ __launch_bounds__(kBlockThreads) void k_flush_x(int64_t quads, const uint8_t *__restrict__ c8,
                                                            const float *__restrict__ ring, int64_t stride,
-                                                           double *__restrict__ x, const SolveState *st)
+                                                           double *__restrict__ x, const SolveState *st, int xz)
 {
 #pragma clang fp contract(off)
     if (!st->done) return;
-    const int rem = st->iters % kXRing;
+    const int iters = st->iters, rem = iters % kXRing;
     if (rem == 0) return;
+    const bool zero = xz && iters < kXRing;
     const int64_t step = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cells; i += step)
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += step)
     {
-        if (!(c8[i] & 64u)) continue;
-        double xv = x[i];
-        for (int j = 0; j < rem; j++) xv = xv + st->alphas[j] * (double)ring[(size_t)j * (size_t)stride + i];
-        x[i] = xv;
+        const int64_t i = 4 * q - 1;
+        const uchar4 c = *reinterpret_cast<const uchar4 *>(c8 + i);
+        const bool u0 = c.x & 64u, u1 = c.y & 64u, u2 = c.z & 64u, u3 = c.w & 64u;
+        if (!(u0 || u1 || u2 || u3)) continue;
+        double2 xa = make_double2(0.0, 0.0), xb = make_double2(0.0, 0.0);
+        if (!zero)
+        {
+            xa = *reinterpret_cast<const double2 *>(x + i);
+            xb = *reinterpret_cast<const double2 *>(x + i + 2);
+        }
+        double v0 = xa.x, v1 = xa.y, v2 = xb.x, v3 = xb.y;
+        for (int j = 0; j < rem; j++)
+        {
+            const double a = st->alphas[j];
+            const float4 p = *reinterpret_cast<const float4 *>(ring + (size_t)j * (size_t)stride + i);
+            v0 = v0 + a * (double)p.x;
+            v1 = v1 + a * (double)p.y;
+            v2 = v2 + a * (double)p.z;
+            v3 = v3 + a * (double)p.w;
+        }
+        *reinterpret_cast<double2 *>(x + i) = make_double2(u0 ? v0 : xa.x, u1 ? v1 : xa.y);
+        *reinterpret_cast<double2 *>(x + i + 2) = make_double2(u2 ? v2 : xb.x, u3 ? v3 : xb.y);
     }
 }
 
@@ -4431,10 +4453,12 @@ hipError_t launch_cg_update64(hipStream_t s, const Box &B, const Tables64 &T, co
 }
 
 hipError_t launch_flush_x(hipStream_t s, const Box &B, const Tables64 &T, const float *ring, int64_t stride, double *x,
-                          const SolveState *st)
+                          const SolveState *st, int xz)
 {
-    const int G = grid_1d(B.cells);
-    k_flush_x<<<G, kBlockThreads, 0, s>>>(B.cells, T.c8, ring, stride, x, st);
+    // quads covering elements -1 .. cells - 1 (the allocations extend at least 15 elements past the box)
+    const int64_t nq = (B.cells + 1 + 3) / 4;
+    const int G = grid_1d(nq);
+    k_flush_x<<<G, kBlockThreads, 0, s>>>(nq, T.c8, ring, stride, x, st, xz);
     return hipGetLastError();
 }
 

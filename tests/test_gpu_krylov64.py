@@ -109,3 +109,52 @@ def test_krylov64_2d_matches_oracle(n, kind, stretched, tol):
     if s.singular:
         x2 = x2 - x2.mean()
     assert np.linalg.norm(x - x2) / np.linalg.norm(xr) < 2e3 * tol
+
+
+@pytest.mark.parametrize("n,kind,batched", [(40, "neumann", False), (36, "outlet", False), (36, "outlet", True)])
+def test_missed_first_pass_takes_a_second(monkeypatch, n, kind, batched):
+    """The first pass's end forms only the true residual's norm. A first pass aimed at 30x the target on its
+    recursive residual (NPPE_K64_AIM) ends with a true residual that misses the target; the solve then forms the
+    residual vectors again from the new x and takes a second pass, which meets the tolerance -- alone and in a batch,
+    where the other system takes one pass."""
+    g, og = make(3, n, kind, seed=7)
+    s = O.assemble(og)
+    b = np.random.default_rng(8).standard_normal(s.n)
+    if s.singular:
+        b -= b.mean()
+        b -= b.mean()
+    tol = 1e-9
+    with N.Solver(tolerance=tol) as h:
+        h.commit(g)
+        x1, r1, rc1 = h.solve(b)
+    assert rc1 == 0 and r1.refinements == 1
+    monkeypatch.setenv("NPPE_K64_AIM", "30")
+    if not batched:
+        with N.Solver(tolerance=tol) as h:
+            h.commit(g)
+            x2, r2, rc2 = h.solve(b)
+    else:
+        g2, og2 = make(3, 28, "neumann", seed=9)
+        s2 = O.assemble(og2)
+        b2 = np.random.default_rng(10).standard_normal(s2.n)
+        b2 -= b2.mean()
+        b2 -= b2.mean()
+        import torch
+        with N.Solver(tolerance=tol) as h, N.Solver(tolerance=tol) as h2:
+            h.commit(g)
+            h2.commit(g2)
+            tb = [torch.from_numpy(v).cuda() for v in (b, b2)]
+            tx = [torch.zeros_like(t) for t in tb]
+            torch.cuda.synchronize()
+            (r2, r3), rc = N.batch_solve_device([h, h2], [t.data_ptr() for t in tb], [t.data_ptr() for t in tx])
+            torch.cuda.synchronize()
+            assert rc == 0
+            x2, x3 = (t.cpu().numpy() for t in tx)
+            assert s2.relres(b2, x3) < tol * 1.01
+        rc2 = 0
+    assert rc2 == 0 and r2.refinements == 2, (r2.refinements, r2.iterations)
+    assert r2.relative_residual < tol and s.relres(b, x2) < tol * 1.01
+    if s.singular:
+        x1 = x1 - x1.mean()
+        x2 = x2 - x2.mean()
+    assert np.linalg.norm(x2 - x1) / np.linalg.norm(x1) < 1e3 * tol
