@@ -55,8 +55,9 @@ hipError_t launch_batched(hipStream_t s, const void *fn, int grid, int threads, 
 // st->sums[sumSlot].
 hipError_t launch_scatter64(hipStream_t s, int64_t n, const int32_t *order, const double *src, double *box,
                             SolveState *st, int sumSlot, double *partials);
-// A solve without a guess: b64[order[q]] = r64[order[q]] = src[q], r32[order[q]] = (float) src[q],
-// x64[order[q]] = 0, and the sum of src^2 into st->sums[0].
+// A solve without a guess: b64[order[q]] = src[q], r32[order[q]] = (float) src[q], where given r64[order[q]] =
+// src[q] and x64[order[q]] = 0 (a pass begun by launch_krylov_init with residualIsB reads b and 0.0 for them), and
+// the sum of src^2 into st->sums[0].
 hipError_t launch_scatter_rhs64(hipStream_t s, int64_t n, const int32_t *order, const double *src, double *b64,
                                 double *r64, float *r32, double *x64, SolveState *st, double *partials);
 // dst[q] = box[order[q]] - (meanSlot >= 0 ? st->sums[meanSlot] / n : 0).
@@ -152,13 +153,17 @@ hipError_t launch_cg_dir64(hipStream_t s, const Box &B, const Tables64 &T, const
 // Part 2: q = A p from the same FP32 p, r -= alpha q (FP64), r32 = (float) r, <r, r> -> convergence flag; on the
 // iteration that fills the ring (k mod kXRing = kXRing - 1) also x += alpha_j p_j for the ring's iterations, oldest
 // first -- the per-iteration updates, in their order.
+// b: the right-hand side, read for r on the first update of a pass that began at x = 0 (SolveState::zeroStart, the
+// residual then still b); x is read as 0.0 by the first full ring of such a pass.
 hipError_t launch_cg_update64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const float *ring,
-                              int64_t stride, double *x, double *r, float *r32, SolveState *st, double *partials);
+                              int64_t stride, double *x, double *r, float *r32, const double *b, SolveState *st,
+                              double *partials);
 // The rest of the deferred update once the recursion is done: x += alpha_j p_j for the slots j < iters mod kXRing
-// (the iterations since the last full ring), at the unknowns. Nothing before st->done. xz: the pass began at
-// x = 0, so x is not read while no full ring has been applied (the same sums from 0.0).
+// (the iterations since the last full ring), at the unknowns. Nothing before st->done. In a pass that began at
+// x = 0 (SolveState::zeroStart) x is not read while no full ring has been applied (the same sums from 0.0), and a
+// pass without iterations writes that 0.
 hipError_t launch_flush_x(hipStream_t s, const Box &B, const Tables64 &T, const float *ring, int64_t stride, double *x,
-                          const SolveState *st, int xz);
+                          const SolveState *st);
 // e += alpha p (e = alpha p on the first iteration), r -= alpha q, <r, r> -> convergence flag; p is pA on even
 // inner iterations and pB on odd ones (pA == pB for an in-place direction).
 hipError_t launch_cg_update(hipStream_t s, const Level32 &L, float *e, float *r, const float *pA, const float *pB,

@@ -437,8 +437,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_scatter64(int64_t n, const in
     block_sum<1>(a, red, gridDim.x, blockIdx.x);
 }
 
-// The right-hand side of a solve without a guess: b64 = r64 = b, r32 = (float) b and x = 0 at every unknown (the
-// residual of the zero guess, b - A 0, is b bit for bit), and <b, b> reduced as k_scatter64 reduces it.
+// The right-hand side of a solve without a guess: b64 = b, r32 = (float) b, and (where given) r64 = b and x = 0 at
+// every unknown (the residual of the zero guess, b - A 0, is b bit for bit; a pass that starts from the zero guess
+// reads b for it and 0.0 for x instead, SolveState::zeroStart), and <b, b> reduced as k_scatter64 reduces it.
 __global__ __launch_bounds__(kBlockThreads) void k_scatter_rhs64(int64_t n, const int32_t *__restrict__ order,
                                                                  const double *__restrict__ src,
                                                                  double *__restrict__ b64, double *__restrict__ r64,
@@ -451,9 +452,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_scatter_rhs64(int64_t n, cons
         const double v = src[q];
         const int32_t o = order[q];
         b64[o] = v;
-        r64[o] = v;
+        if (r64) r64[o] = v;
         r32[o] = (float)v;
-        x64[o] = 0.0;
+        if (x64) x64[o] = 0.0;
         v2 += v * v;
     }
     double a[1] = {v2};
@@ -536,17 +537,18 @@ __global__ __launch_bounds__(kBlockThreads) void k_update_residual64(Box B, Tabl
 // The deferred update's rest at the end of a pass (launch_flush_x): x += alpha_j p_j for the ring slots j < iters mod
 // kXRing, at the unknowns, oldest first. Four cells per thread (elements 4q - 1 .. 4q + 2: element -1 of every
 // box array is 16-byte aligned, and the allocations extend past the box), a float4 of each direction and two
-// double2 of x. xz: the pass began at x = 0 (a solve without a guess); while no full ring has been applied x is
-// still that zero, so the sums start from 0.0 instead of reading it -- the same additions, the same bits.
+// double2 of x. A pass that began at x = 0 (SolveState::zeroStart) has not written x while no full ring has been
+// applied: the sums then start from 0.0 instead of reading it -- the same additions, the same bits -- and a pass
+// that took no iteration writes that 0 at the unknowns.
 __global__ __launch_bounds__(kBlockThreads) void k_flush_x(int64_t quads, const uint8_t *__restrict__ c8,
                                                            const float *__restrict__ ring, int64_t stride,
-                                                           double *__restrict__ x, const SolveState *st, int xz)
+                                                           double *__restrict__ x, const SolveState *st)
 {
 #pragma clang fp contract(off)
     if (!st->done) return;
     const int iters = st->iters, rem = iters % kXRing;
-    if (rem == 0) return;
-    const bool zero = xz && iters < kXRing;
+    const bool zero = st->zeroStart && iters < kXRing;
+    if (rem == 0 && !zero) return;
     const int64_t step = (int64_t)gridDim.x * blockDim.x;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += step)
     {
@@ -611,14 +613,17 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_dir64_g(Box B, Tables64 T,
 __global__ __launch_bounds__(kBlockThreads) void k_cg_update64_g(Box B, Tables64 T, const uint8_t *__restrict__ ty,
                                                                  const float *__restrict__ ring, int64_t stride,
                                                                  double *__restrict__ x, double *__restrict__ r,
-                                                                 float *__restrict__ r32, const SolveState *st,
-                                                                 Red red)
+                                                                 float *__restrict__ r32, const double *__restrict__ b0,
+                                                                 const SolveState *st, Red red)
 {
 #pragma clang fp contract(off)
     if (st->done) return;
     const double alpha = st->alpha;
     const int slot = st->iters % kXRing;
     const bool flush = slot == kXRing - 1;
+    // a pass begun at x = 0: the residual is still b before its first update, x still 0 before its first ring
+    const double *__restrict__ rin = st->zeroStart && st->iters == 0 ? b0 : r;
+    const bool x0 = st->zeroStart && st->iters < kXRing;
     const float *__restrict__ p = ring + (size_t)slot * (size_t)stride;
     const TileRange tr = my_tiles(B.blocks());
     double rr = 0.0;
@@ -630,13 +635,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_update64_g(Box B, Tables64
         if (!(T.c8[idx] & 64u)) continue;
         const double pc = p[idx];
         const double q = apply64(B, T, ty, s, m, f, idx, [&](int64_t d) { return (double)p[idx + d]; });
-        const double rv = r[idx] - alpha * q;
+        const double rv = rin[idx] - alpha * q;
         r[idx] = rv;
         r32[idx] = (float)rv;
         rr += rv * rv;
         if (flush)
         {
-            double xv = x[idx];
+            double xv = x0 ? 0.0 : x[idx];
 #pragma unroll
             for (int j = 0; j < kXRing; j++)
                 xv = xv + st->alphas[j] * (j == kXRing - 1 ? pc : (double)ring[(size_t)j * (size_t)stride + idx]);
@@ -2226,14 +2231,17 @@ template <bool UNI>
 __global__ __launch_bounds__(kBlockThreads) void k_cg_update64_t(Box B, Tables64 T, const uint8_t *__restrict__ ty,
                                                                  const float *__restrict__ ring, int64_t stride,
                                                                  double *__restrict__ x, double *__restrict__ r,
-                                                                 float *__restrict__ r32, const SolveState *st,
-                                                                 Red red, int P)
+                                                                 float *__restrict__ r32, const double *__restrict__ b0,
+                                                                 const SolveState *st, Red red, int P)
 {
 #pragma clang fp contract(off)
     if (st->done) return;
     const double alpha = st->alpha;
     const int slot = st->iters % kXRing;
     const bool flush = slot == kXRing - 1;
+    // a pass begun at x = 0: the residual is still b before its first update, x still 0 before its first ring
+    const double *__restrict__ rin = st->zeroStart && st->iters == 0 ? b0 : r;
+    const bool x0 = st->zeroStart && st->iters < kXRing;
     const float *__restrict__ p = ring + (size_t)slot * (size_t)stride;
     __shared__ double Ps[kBlockMid + 2][kBlockFast + 2];
     const March M = march_setup(B, P);
@@ -2251,13 +2259,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_update64_t(Box B, Tables64
     double Pm = p[plane(M.s0 - 1) + own], Pc = p[plane(M.s0) + own], Pp = p[plane(M.s0 + 1) + own];
     double H = halo ? (double)p[plane(M.s0) + hoff] : 0.0;
     unsigned cc = T.c8[plane(M.s0) + own];
-    double rc = r[plane(M.s0) + own];
+    double rc = rin[plane(M.s0) + own];
     for (int sl = M.s0; sl <= M.s1; sl++)
     {
         const double Pn = p[plane(sl + 2) + own];
         const double Hn = halo ? (double)p[plane(sl + 1) + hoff] : 0.0;
         const unsigned cn = T.c8[plane(sl + 1) + own];
-        const double rn = r[plane(sl + 1) + own];
+        const double rn = rin[plane(sl + 1) + own];
         Ps[tyy + 1][tx + 1] = Pc;
         if (halo) Ps[hy][hx] = H;
         __syncthreads();
@@ -2273,7 +2281,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_cg_update64_t(Box B, Tables64
             if (flush)
             {
                 // x += alpha_j p_j over the ring's iterations, oldest first (this iteration's p last)
-                double xv = x[idx];
+                double xv = x0 ? 0.0 : x[idx];
 #pragma unroll
                 for (int j = 0; j < kXRing; j++)
                     xv = xv + st->alphas[j] * (j == kXRing - 1 ? Pc : (double)ring[(size_t)j * (size_t)stride + idx]);
@@ -2494,14 +2502,17 @@ template <int TM, int K>
 __device__ __forceinline__ void cg_update64_row_body(const Box &B, const Tables64 &T, const float *__restrict__ ring,
                                                      int64_t stride, double *__restrict__ x,
                                                      double *__restrict__ r, float *__restrict__ r32,
-                                                     const SolveState *st, const Red &red, int P, int tilesM, int lb,
-                                                     int nblk)
+                                                     const double *__restrict__ b0, const SolveState *st,
+                                                     const Red &red, int P, int tilesM, int lb, int nblk)
 {
 #pragma clang fp contract(off)
     if (st->done) return;
     const double alpha = st->alpha;
     const int slot = st->iters % kXRing;
     const bool flush = slot == kXRing - 1;
+    // a pass begun at x = 0: the residual is still b before its first update, x still 0 before its first ring
+    const double *__restrict__ rin = st->zeroStart && st->iters == 0 ? b0 : r;
+    const bool x0 = st->zeroStart && st->iters < kXRing;
     const float *__restrict__ p = ring + (size_t)slot * (size_t)stride;
     constexpr int kSlots = 64 * K + 2;
     __shared__ float ldsP[2][TM + 2][2][kSlots];
@@ -2542,7 +2553,7 @@ __device__ __forceinline__ void cg_update64_row_body(const Box &B, const Tables6
         {
             const size_t pl = plane(sp0 + d);
 #pragma unroll
-            for (int k = 0; k < K; k++) Rv[d][k] = pair64(r + pl, R.off[k]);
+            for (int k = 0; k < K; k++) Rv[d][k] = pair64(rin + pl, R.off[k]);
         }
     }
     __syncthreads();
@@ -2564,7 +2575,7 @@ __device__ __forceinline__ void cg_update64_row_body(const Box &B, const Tables6
         {
             const size_t pl = plane(sp + 2);
 #pragma unroll
-            for (int k = 0; k < K; k++) Rv[S2][k] = pair64(r + pl, R.off[k]);
+            for (int k = 0; k < K; k++) Rv[S2][k] = pair64(rin + pl, R.off[k]);
         }
         float(*buf)[2][kSlots] = ldsP[sp & 1];
 #pragma unroll
@@ -2613,18 +2624,17 @@ __device__ __forceinline__ void cg_update64_row_body(const Box &B, const Tables6
             if (flush)
             {
                 // x += alpha_j p_j over the ring's iterations, oldest first (this iteration's p last)
-                double xo[2];
-                if (R.okB[k])
+                double xo[2] = {0.0, 0.0};
+                if (x0)
+                    ;   // still the zero the pass began at
+                else if (R.okB[k])
                 {
                     const double2 xv = *reinterpret_cast<const double2 *>(x + idx);
                     xo[0] = xv.x;
                     xo[1] = xv.y;
                 }
                 else
-                {
                     xo[0] = x[idx];
-                    xo[1] = 0.0;
-                }
 #pragma unroll
                 for (int j = 0; j < kXRing; j++)
                 {
@@ -3439,6 +3449,7 @@ struct Update64RowArgs
     int64_t stride;
     double *x, *r;
     float *r32;
+    const double *b0;
     const SolveState *st;
     Red red;
     int P, tilesM;
@@ -3581,10 +3592,11 @@ template <int TM, int K>
 __global__ __launch_bounds__(64 * (TM + 2), 4) void k_cg_update64_row(Box B, Tables64 T,
                                                                    const float *__restrict__ ring, int64_t stride,
                                                                    double *__restrict__ x, double *__restrict__ r,
-                                                                   float *__restrict__ r32, const SolveState *st,
+                                                                   float *__restrict__ r32,
+                                                                   const double *__restrict__ b0, const SolveState *st,
                                                                    Red red, int P, int tilesM)
 {
-    cg_update64_row_body<TM, K>(B, T, ring, stride, x, r, r32, st, red, P, tilesM, logical_block(), gridDim.x);
+    cg_update64_row_body<TM, K>(B, T, ring, stride, x, r, r32, b0, st, red, P, tilesM, logical_block(), gridDim.x);
 }
 
 template <int TM, int K>
@@ -3593,7 +3605,7 @@ __global__ __launch_bounds__(64 * (TM + 2), 4) void kb_cg_update64_row(const Upd
 {
     const BatchBlock bb = batch_block(start, nsys);
     const Update64RowArgs a = args[bb.q];
-    cg_update64_row_body<TM, K>(a.B, a.T, a.ring, a.stride, a.x, a.r, a.r32, a.st, a.red, a.P, a.tilesM, bb.lb,
+    cg_update64_row_body<TM, K>(a.B, a.T, a.ring, a.stride, a.x, a.r, a.r32, a.b0, a.st, a.red, a.P, a.tilesM, bb.lb,
                                 bb.nb);
 }
 
@@ -3649,6 +3661,7 @@ __global__ void k_state_init(SolveState *st, double thr2, int maxIters)
     st->done = maxIters <= 0 ? 2 : 0;
     st->accepted = 0;
     st->skip = 0;
+    st->zeroStart = 0;
 }
 
 // The FP64 Krylov solve's set-up from the true residual of the guess (sums[0] = |b|^2, sums[1] = |r|^2), on the
@@ -3669,6 +3682,7 @@ __global__ void k_krylov_init(SolveState *st, double tol, double ref, double aim
     st->accepted = 0;
     st->skip = (bn == 0.0 || (ref > 0.0 && bn <= tol * ref)) ? 1 : 0;
     st->done = (st->skip || rn < target) ? 1 : (maxIters <= 0 ? 2 : 0);
+    st->zeroStart = residualIsB;
 }
 
 inline int grid_of(const Box &B) { return std::min(B.blocks(), kMaxGrid); }
@@ -3808,8 +3822,8 @@ template <int TM, int K>
 void update64_row(hipStream_t s, int grid, int threads, const Update64RowArgs &a)
 {
     if (record(kb_cg_update64_row<TM, K>, grid, threads, a)) return;
-    k_cg_update64_row<TM, K><<<grid, threads, 0, s>>>(a.B, a.T, a.ring, a.stride, a.x, a.r, a.r32, a.st, a.red,
-                                                      a.P, a.tilesM);
+    k_cg_update64_row<TM, K><<<grid, threads, 0, s>>>(a.B, a.T, a.ring, a.stride, a.x, a.r, a.r32, a.b0, a.st,
+                                                      a.red, a.P, a.tilesM);
 }
 }   // namespace
 
@@ -4423,20 +4437,21 @@ hipError_t launch_cg_dir64(hipStream_t s, const Box &B, const Tables64 &T, const
 }
 
 hipError_t launch_cg_update64(hipStream_t s, const Box &B, const Tables64 &T, const uint8_t *ty, const float *ring,
-                              int64_t stride, double *x, double *r, float *r32, SolveState *st, double *partials)
+                              int64_t stride, double *x, double *r, float *r32, const double *b, SolveState *st,
+                              double *partials)
 {
     if (!B.threeD)
     {
         const int G = grid_of(B);
         const Red red = red_of(partials, st, FIN_RR, G);
-        k_cg_update64_g<<<G, kBlockThreads, 0, s>>>(B, T, ty, ring, stride, x, r, r32, st, red);
+        k_cg_update64_g<<<G, kBlockThreads, 0, s>>>(B, T, ty, ring, stride, x, r, r32, b, st, red);
         return finish(s, red, G, 1);
     }
-    const RowPlan rp = row_plan(B, T.c8, T.unitCoef, {{ring, 8}, {x, 16}, {r, 16}, {r32, 8}}, 6, 2);
+    const RowPlan rp = row_plan(B, T.c8, T.unitCoef, {{ring, 8}, {x, 16}, {r, 16}, {r32, 8}, {b, 16}}, 6, 2);
     if (rp.ok)
     {
         const Red red = red_of(partials, st, FIN_RR, rp.G);
-        const Update64RowArgs a{B, T, ring, stride, x, r, r32, st, red, rp.P, rp.tilesM};
+        const Update64RowArgs a{B, T, ring, stride, x, r, r32, b, st, red, rp.P, rp.tilesM};
         if (rp.K == 1)
             update64_row<6, 1>(s, rp.G, 64 * 8, a);
         else
@@ -4446,19 +4461,19 @@ hipError_t launch_cg_update64(hipStream_t s, const Box &B, const Tables64 &T, co
     const int P = march_planes(B), G = march_grid(B, P);
     const Red red = red_of(partials, st, FIN_RR, G);
     if (T.unitCoef)
-        k_cg_update64_t<true><<<G, kBlockThreads, 0, s>>>(B, T, ty, ring, stride, x, r, r32, st, red, P);
+        k_cg_update64_t<true><<<G, kBlockThreads, 0, s>>>(B, T, ty, ring, stride, x, r, r32, b, st, red, P);
     else
-        k_cg_update64_t<false><<<G, kBlockThreads, 0, s>>>(B, T, ty, ring, stride, x, r, r32, st, red, P);
+        k_cg_update64_t<false><<<G, kBlockThreads, 0, s>>>(B, T, ty, ring, stride, x, r, r32, b, st, red, P);
     return finish(s, red, G, 1);
 }
 
 hipError_t launch_flush_x(hipStream_t s, const Box &B, const Tables64 &T, const float *ring, int64_t stride, double *x,
-                          const SolveState *st, int xz)
+                          const SolveState *st)
 {
     // quads covering elements -1 .. cells - 1 (the allocations extend at least 15 elements past the box)
     const int64_t nq = (B.cells + 1 + 3) / 4;
     const int G = grid_1d(nq);
-    k_flush_x<<<G, kBlockThreads, 0, s>>>(nq, T.c8, ring, stride, x, st, xz);
+    k_flush_x<<<G, kBlockThreads, 0, s>>>(nq, T.c8, ring, stride, x, st);
     return hipGetLastError();
 }
 

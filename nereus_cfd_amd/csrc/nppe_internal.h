@@ -95,6 +95,9 @@ struct SolveState
     double acc2;
     int skip;
     int accepted;
+    // the pass began without a guess and nothing has written x or the FP64 residual yet: x is still 0 (read as 0.0
+    // by the deferred updates until the first full ring) and the residual still b (read from b by the first update)
+    int zeroStart;
     double alphas[kXRing];   // alpha of the iteration whose direction sits in ring slot (iteration mod kXRing)
 };
 

@@ -339,7 +339,7 @@ hipError_t inner_iteration(nppe_solver *h, Traffic &tr)
             // p, codes, r in; r, r32 out (x: every kXRing-th iteration, counted when the pass ends)
             Scope sc(h, NPPE_K_CG_UPDATE, 25.0 * N);
             if ((e = launch_cg_update64(h->stream, L0.box, h->t64, L0.types, h->dPRing, h->ringStride, h->dX64[0],
-                                        h->dR64, h->dR, h->dState, h->dPartials)) != hipSuccess)
+                                        h->dR64, h->dR, h->dB64, h->dState, h->dPartials)) != hipSuccess)
                 return e;
         }
         tr.add((rzDone ? 0.0 : (2 * s + f) * N) + 13.0 * N + 25.0 * N);
@@ -1200,8 +1200,7 @@ int k64_tail(K64Run &R)
     nppe_solver *h = R.h;
     const DevLevel &L0 = h->levels[0];
     const double N = L0.cellsBox, n = (double)h->n;
-    HIP_TRY(h, launch_flush_x(h->stream, L0.box, h->t64, h->dPRing, h->ringStride, h->dX64[R.cur], h->dState,
-                              R.xZero ? 1 : 0),
+    HIP_TRY(h, launch_flush_x(h->stream, L0.box, h->t64, h->dPRing, h->ringStride, h->dX64[R.cur], h->dState),
             "solution update");
     {
         // Only its norm (and with the projection the sum of x over the unknowns, for the mean): a pass that misses
@@ -1258,8 +1257,8 @@ int k64_advance(K64Run &R)
     // (x is not read by the flush while a pass that began at x = 0 has applied no full ring)
     const int fulls = st.iters / kXRing, rest = st.iters % kXRing;
     const double xRead = R.xZero && st.iters < kXRing ? 0.0 : 8.0;
-    R.tr->add(R.perIter * st.iters + 17.0 * N + fulls * (4.0 * (kXRing - 1) + 16.0) * N +
-              (rest ? (4.0 * rest + 8.0 + xRead + 1.0) * N : 0.0));
+    R.tr->add(R.perIter * st.iters + 17.0 * N + fulls * (4.0 * (kXRing - 1) + 16.0) * N -
+              (R.xZero && fulls > 0 ? 8.0 * N : 0.0) + (rest ? (4.0 * rest + 8.0 + xRead + 1.0) * N : 0.0));
     R.rnorm = std::sqrt(st.sums[1]);
     if (!std::isfinite(R.rnorm)) return fail(h, NPPE_ERR_NONFINITE, "non-finite residual during refinement");
     // a pass that ran iterations counts; the guess accepted as it is does not (as the synchronous loop)
@@ -1687,10 +1686,10 @@ int batch_run(nppe_solver *const *hs, const double *const *b, double *const *x, 
         rcs[q] = NPPE_OK;
         h->curIter = -1;
         // no guess: x = 0, and its residual is b (solve_mgpcg's schedule)
-        HIP_TRY(h0, launch_scatter_rhs64(s, h->n, h->dOrder, b[q], h->dB64, h->dR64, h->dR, h->dX64[0], h->dState,
+        HIP_TRY(h0, launch_scatter_rhs64(s, h->n, h->dOrder, b[q], h->dB64, nullptr, h->dR, nullptr, h->dState,
                                          h->dPartials),
                 "batch: scatter b");
-        tr[q].add(40.0 * n);
+        tr[q].add(24.0 * n);
         K64Run &r = R[q];
         r.h = h;
         r.refNorm = 0.0;
@@ -1847,14 +1846,15 @@ int solve_mgpcg(nppe_solver *h, const double *dB, double *dX, const double *dGue
     const bool project = h->singular && h->opt.compatibility && h->n > 0;
     if (h->krylov64 && h->opt.max_iters > 0 && h->opt.max_refinements > 0 && !dGuess)
     {
-        // no guess: x = 0, and its residual is b (one scatter instead of the guess and the residual passes)
+        // no guess: x = 0, and its residual is b (one scatter instead of the guess and the residual passes; the pass
+        // reads b for its first residual and 0.0 for x until it writes them, so neither is written here)
         {
-            Scope sc(h, NPPE_K_SCATTER, 40.0 * n);
-            HIP_TRY(h, launch_scatter_rhs64(h->stream, h->n, h->dOrder, dB, h->dB64, h->dR64, h->dR, h->dX64[cur],
+            Scope sc(h, NPPE_K_SCATTER, 24.0 * n);
+            HIP_TRY(h, launch_scatter_rhs64(h->stream, h->n, h->dOrder, dB, h->dB64, nullptr, h->dR, nullptr,
                                             h->dState, h->dPartials),
                     "scatter b");
         }
-        tr.add(40.0 * n);
+        tr.add(24.0 * n);
         int passes = 0, inner = 0, status = NPPE_OK;
         double bnorm = 0.0, rnorm = 0.0;
         if (int rc = krylov64_scheduled(h, refNorm, cur, dX, project, true, tr, &passes, &inner, &status, &bnorm,
@@ -1933,8 +1933,7 @@ int solve_mgpcg(nppe_solver *h, const double *dB, double *dX, const double *dGue
             int done = 0;
             if ((rc = run_inner(h, std::min(0.5, aim / rnorm), rnorm * rnorm, tr, &done))) return rc;
             inner += done;
-            HIP_TRY(h, launch_flush_x(h->stream, L0.box, h->t64, h->dPRing, h->ringStride, h->dX64[cur], h->dState,
-                                      0),
+            HIP_TRY(h, launch_flush_x(h->stream, L0.box, h->t64, h->dPRing, h->ringStride, h->dX64[cur], h->dState),
                     "solution update");
             {
                 Scope sc(h, NPPE_K_RESIDUAL64, 29.0 * N);
