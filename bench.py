@@ -5,7 +5,7 @@ to 1e-8 true relative residual, on one MI355X.
 A step is one complete solve (right-hand side already resident in HBM, solution left in HBM): scatter to the box
 (with no guess the residual is b itself), CG with its recursion in FP64 preconditioned by an FP32 multigrid
 V-cycle, the true FP64 residual, mean projection, gather. The dominant kernel's launch durations come from HIP
-events around it in every tenth timed solve.
+events that its own dispatches record (start and end of each finest-level sweep) in every tenth timed solve.
 Prints one JSON line. Also reports, on the same GPU, the FP64 matrix-free Jacobi-PCG solve of the same system (the
 plain-CG comparison), and on the host the reference's own AMGCL and Eigen CG (oracle/_ref, compiled from the
 reference sources) on a bounded sample.
@@ -57,6 +57,21 @@ def measured_traffic():
     return None
 
 
+def profiled_launch_us():
+    """The dominant kernel's average duration per launch in the 256^3 kernel trace summarised in
+    profiles/k_rbgs_row_256cube_mgpcg_kernel_trace.md (rocprofv3 --kernel-trace of this benchmark), as a cross-check
+    of the live figure; None when that summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_rbgs_row_256cube_mgpcg_kernel_trace.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("rocprof_avg_launch_us ="):
+                    return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def gpu_leg(args):
     import torch
     import nereus_cfd_amd as N
@@ -78,10 +93,10 @@ def gpu_leg(args):
         res, rc = s.solve_device(tb.data_ptr(), tx.data_ptr())
         log(f"[bench] warmup {w}: iters={res.iterations} passes={res.refinements} relres={res.relative_residual:.3e} "
             f"device_ms={res.device_ms:.3f}")
-    # HIP events bracket the finest-level red-black sweeps (the dominant kernel) in every tenth timed solve (the
-    # last of each ten); every other launch, and every launch of the other solves, runs without events around it
-    # (event records between launches stall the stream, and a profiled solve runs its iterations without the
-    # captured graph).
+    # In every tenth timed solve (the last of each ten) each finest-level red-black sweep (the dominant kernel)
+    # records a pair of HIP events with its own dispatch (hipExtLaunchKernel's start and stop events, on the
+    # solver's stream): the kernel's duration, as a kernel trace measures it. The other solves run without events
+    # (a profiled solve launches its iterations directly instead of replaying the captured graph).
     every = 10 if args.steps >= 10 else args.steps
     dom_launches, dom_ms, dom_bytes = 0, 0.0, 0.0
     lat = []
@@ -142,7 +157,10 @@ def gpu_leg(args):
                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                        "launches": dom[1]["launches"],
                        "avg_launch_us": round(1e3 * dom[1]["ms"] / dom[1]["launches"], 2),
-                       "bytes_per_launch": round(dom[1]["bytes"] / dom[1]["launches"])}
+                       "bytes_per_launch": round(dom[1]["bytes"] / dom[1]["launches"]),
+                       "timing": "HIP start/stop events recorded by each sampled launch's own dispatch "
+                                 "(hipExtLaunchKernel) on the solver's stream",
+                       "rocprof_avg_launch_us": profiled_launch_us() if n == 256 else None}
     s.close()
 
     if not args.no_cg64:

@@ -44,6 +44,11 @@ bool retile_record(LaunchRecord &r, int tm, int p);
 // Tile rows a batch may give the whole-row sweep record r.
 void retile_rows(const LaunchRecord &r, std::vector<int> &tms);
 void set_launch_recorder(LaunchRecorder *r);   // null: launch again
+// Kernel-attached timing: the next whole-row sweep launched on this thread records start on its dispatch's start and
+// stop on its end (hipExtLaunchKernel's events: the kernel's own duration, with no event packets around it) and
+// clears the pair; launch_events_pending says whether a pair set here is still waiting for such a launch.
+void set_launch_events(hipEvent_t start, hipEvent_t stop);
+bool launch_events_pending();
 void record_slot(int phase, int level);        // the place of the next records (nothing without a recorder)
 // args: nsys records' Args back to back (the kernel's Args type); start: nsys + 1 block offsets, start[0] = 0,
 // start[nsys] = grid, each system at least one block.

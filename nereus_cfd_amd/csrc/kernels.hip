@@ -12,6 +12,7 @@
 //   FLUID neighbour             -> diag -= c; off-diagonal c when the neighbour is an unknown (not on the ring)
 //   EMPTY / PRESSURE neighbour  -> diag -= c * rf, rf = FaceGap / (Width/2) on the domain ring, 1 inside
 //   SOLID / INFLOW / OUTFLOW    -> nothing (Neumann)
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <type_traits>
 
@@ -3762,12 +3763,21 @@ const void *row_sweep_batched(int tm, int k, int flags)
     return nullptr;
 }
 
+thread_local hipEvent_t tEvStart = nullptr, tEvStop = nullptr;
+
 template <int TM, int K, bool ZERO, bool PRO, bool NU>
 void row_sweep(hipStream_t s, int grid, int threads, const RowSweepArgs &a)
 {
     if (record(kb_rbgs_row<TM, K, ZERO, PRO, NU>, grid, threads, a, REC_ROW_SWEEP, TM, K,
                (ZERO ? 1 : 0) | (PRO ? 2 : 0) | (NU ? 4 : 0)))
         return;
+    if (tEvStart)
+    {
+        hipExtLaunchKernelGGL(k_rbgs_row<TM, K, ZERO, PRO, NU>, dim3(grid), dim3(threads), 0, s, tEvStart, tEvStop, 0,
+                              a.L, a.xin, a.xout, a.b, a.c1, a.P, a.tilesM, a.st, a.dot, a.pro);
+        tEvStart = tEvStop = nullptr;
+        return;
+    }
     k_rbgs_row<TM, K, ZERO, PRO, NU><<<grid, threads, 0, s>>>(a.L, a.xin, a.xout, a.b, a.c1, a.P, a.tilesM, a.st,
                                                               a.dot, a.pro);
 }
@@ -3854,6 +3864,14 @@ hipError_t finish(hipStream_t s, const Red &red, int nblk, int nv)
 }  // namespace
 
 void set_launch_recorder(LaunchRecorder *r) { tRecorder = r; }
+
+void set_launch_events(hipEvent_t start, hipEvent_t stop)
+{
+    tEvStart = start;
+    tEvStop = stop;
+}
+
+bool launch_events_pending() { return tEvStart != nullptr; }
 
 void record_slot(int phase, int level)
 {

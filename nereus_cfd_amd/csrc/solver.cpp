@@ -118,13 +118,17 @@ void free_grid(nppe_solver *h)
 }
 
 // ---- profiling -----------------------------------------------------------------------------------------------
+// attach: the scope holds one whole-row sweep, which records the pair itself on its dispatch (set_launch_events:
+// the kernel's own duration); otherwise events are recorded on the stream before and after the launches.
 struct Scope
 {
     nppe_solver *h;
     int kind;
     double bytes;
+    bool attach;
     hipEvent_t b = nullptr;
-    Scope(nppe_solver *h_, int kind_, double bytes_) : h(h_), kind(kind_), bytes(bytes_)
+    Scope(nppe_solver *h_, int kind_, double bytes_, bool attach_ = false)
+        : h(h_), kind(kind_), bytes(bytes_), attach(attach_)
     {
         if (!h->profiling || !((h->profMask >> kind) & 1u)) return;
         if (h->evNext + 2 > h->evPool.size())
@@ -136,12 +140,23 @@ struct Scope
             }
         hipEvent_t a = h->evPool[h->evNext++];
         b = h->evPool[h->evNext++];
-        (void)hipEventRecord(a, h->stream);
+        if (attach)
+            set_launch_events(a, b);
+        else
+            (void)hipEventRecord(a, h->stream);
         h->recs.push_back({kind, h->curIter, a, b, bytes});
     }
     ~Scope()
     {
-        if (h->profiling && b) (void)hipEventRecord(b, h->stream);
+        if (!h->profiling || !b) return;
+        if (!attach)
+            (void)hipEventRecord(b, h->stream);
+        else if (launch_events_pending())
+        {
+            // no whole-row sweep took the pair: drop the record
+            set_launch_events(nullptr, nullptr);
+            h->recs.back().a = nullptr;
+        }
     }
 };
 
@@ -152,7 +167,7 @@ void profile_collect(nppe_solver *h, int itersDone)
     if (!h->profiling) return;
     for (const auto &r : h->recs)
     {
-        if (r.iter >= itersDone) continue;
+        if (r.iter >= itersDone || !r.a) continue;
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) continue;
         h->profMs[r.kind] += ms;
@@ -205,7 +220,8 @@ hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel, doubl
                 break;
             }
             const double by = zero ? (2 * s + f) * N : (3 * s + f) * N;
-            Scope sc(h, l == 0 ? NPPE_K_SMOOTH0 : NPPE_K_SMOOTH, by);
+            Scope sc(h, l == 0 ? NPPE_K_SMOOTH0 : NPPE_K_SMOOTH, by,
+                     l == 0 && rbgs_family(L.v, zero ? nullptr : xin, xout, bbuf(l)) == RBGS_ROW);
             if ((e = launch_rbgs(h->stream, L.v, zero ? nullptr : xin, xout, bbuf(l), 0, zero ? 1 : 0, L.tileRows,
                                  L.planesPerChunk, h->dState)) != hipSuccess)
                 return e;
@@ -282,7 +298,8 @@ hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel, doubl
             const double by = (3 * s + f) * N + (fold ? s * C.cellsBox : 0.0);
             const bool dot = l == 0 && sw == nu2 - 1 && rzPartials;
             {
-                Scope sc(h, l == 0 ? NPPE_K_SMOOTH0 : NPPE_K_SMOOTH, by);
+                Scope sc(h, l == 0 ? NPPE_K_SMOOTH0 : NPPE_K_SMOOTH, by,
+                         l == 0 && rbgs_family(L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l)) == RBGS_ROW);
                 if ((e = launch_rbgs(h->stream, L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l), 1, 0,
                                      L.tileRows, fold ? L.proPlanes : L.planesPerChunk, h->dState,
                                      dot ? rzPartials : nullptr, fold ? &C.v : nullptr,
