@@ -37,6 +37,7 @@ struct DevLevel
     int tileRows = 6;        // smoother tile: tileRows rows per block (3D: 128 cells wide), planesPerChunk planes
     int planesPerChunk = 8;
     int proPlanes = 8;       // planes per chunk of the sweep that folds the coarse correction in
+    int post2Planes = 8;     // planes per chunk of the post-smoothing pair in one march (launch_rbgs2)
     int fuseRows = 12;       // tile rows and planes per chunk of the last pre-smoothing sweep with the restriction
     int fusePlanes = 8;      // folded in (launch_rbgs_restrict)
     double cellsBox = 0.0;   // ns * nm * nf (the box, unpitched)

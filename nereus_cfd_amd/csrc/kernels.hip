@@ -1694,6 +1694,347 @@ __device__ __forceinline__ void rbgs_row_body(const Level32 &L, const float *__r
         march(std::integral_constant<int, 0>());
 }
 
+// ---- two sweeps in one march (whole-row waves, 3D uniform levels) ----------------------------------------------
+//
+// Both post-smoothing sweeps of a level as one launch (the first folding the coarse correction in, as
+// k_rbgs_row<..., PRO> does): the second sweep marches three planes behind the first. A plane step sp computes the
+// first sweep's first colour at plane sp (A1), its second colour at sp - 1 (A2), the second sweep's first colour at
+// sp - 2 (B1) and its second colour at sp - 3 (B2, stored). A block is TM output rows and three halo rows each side
+// (waves 0 and NW - 1 load their outer row, as the single sweep's halo waves do): A1 runs on every wave, A2 on all
+// but the outermost, B1 on all but the two outermost, B2 on the TM output rows. Each stage's values pass to the next
+// through LDS, double-buffered by plane parity, with the one barrier of the step; each stage's update is the single
+// sweep's, from the same values in the same order, so x (and <b, x> per row) is bit for bit that of the two launches.
+// b and codes stay in registers from their load (two planes ahead) to their last use by B2 (slots by plane mod 8).
+template <int TM, int K>
+__device__ __forceinline__ void rbgs2_row_body(const Level32 &L, const float *__restrict__ xin, float *__restrict__ xout,
+                                               const float *__restrict__ b, int c1, int P, int tilesM,
+                                               const SolveState *st, double *__restrict__ dot, const Prolong &pro,
+                                               int lb)
+{
+    if (st->done) return;
+    constexpr int NW = TM + 6;
+    const Box &B = L.B;
+    constexpr int kCols = 128 * K + 2;
+    constexpr int kSlots = 64 * K + 2;
+    __shared__ float old[2][NW][2][kSlots];   // old x (plus the correction) of plane sp by parity, row, element, pair+1
+    __shared__ float a1s[2][NW][kCols];       // first sweep, first colour, by column
+    __shared__ float a2s[2][NW][kCols];       // first sweep, second colour
+    __shared__ float b1s[2][NW][kCols];       // second sweep, first colour
+    __shared__ float2 tab[128];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int tm = lb % tilesM, chunk = lb / tilesM;
+    const int m = tm * TM + wave - 2;   // m0 - 3 + wave with m0 = 1 + tm * TM
+    const bool outerW = wave == 0 || wave == NW - 1;
+    const bool doA2 = wave >= 1 && wave <= NW - 2, doB1 = wave >= 2 && wave <= NW - 3;
+    const bool doB2 = wave >= 3 && wave <= NW - 4;
+    const bool rowIn = m >= 1 && m <= B.nm - 2;
+    const bool ragged = B.nf - 2 < 128 * K;
+    const int mc = min(max(m, 0), B.nm - 1);
+    const int mo = min(max(wave == 0 ? m - 1 : m + 1, 0), B.nm - 1);
+    const int s0 = 1 + chunk * P, s1 = min(s0 + P - 1, B.ns - 2);
+    const uint32_t sM = (uint32_t)B.sM;
+    const int fcMax = ((B.nf - 2) & 1) ? B.nf - 2 : B.nf - 3;
+    uint32_t off[K], offO[K], offC[K], offCO[K];
+    bool okA[K], okB[K];
+    unsigned cm[K];
+    int fa[K];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+    {
+        fa[k] = 1 + 2 * (lane + 64 * k);
+        okA[k] = rowIn && fa[k] <= B.nf - 2;
+        okB[k] = rowIn && fa[k] + 1 <= B.nf - 2;
+        cm[k] = (okA[k] ? 0xFFu : 0u) | (okB[k] ? 0xFF00u : 0u);
+        const int fc = min(fa[k], fcMax);
+        off[k] = (uint32_t)mc * sM + (uint32_t)fc;
+        offO[k] = (uint32_t)mo * sM + (uint32_t)fc;
+        const int Fc = min(lane + 64 * k + 1, pro.nf - 1);
+        offC[k] = (uint32_t)((mc + 1) >> 1) * (uint32_t)pro.sM + (uint32_t)Fc;
+        offCO[k] = (uint32_t)((mo + 1) >> 1) * (uint32_t)pro.sM + (uint32_t)Fc;
+    }
+    auto planeXC = [&](int pl) {
+        const int sc = (min(max(pl, 0), B.ns - 1) + 1) >> 1;
+        return pro.xc + (size_t)min(sc, pro.ns - 1) * (size_t)pro.sS;
+    };
+    for (int i = threadIdx.x; i < 2 * NW * kCols; i += 64 * NW)
+    {
+        (&a1s[0][0][0])[i] = 0.f;
+        (&a2s[0][0][0])[i] = 0.f;
+        (&b1s[0][0][0])[i] = 0.f;
+    }
+    for (int i = threadIdx.x; i < 2 * NW * 2 * kSlots; i += 64 * NW) (&old[0][0][0][0])[i] = 0.f;
+    if (threadIdx.x < 128)
+    {
+        const unsigned c = threadIdx.x;
+        tab[c] = make_float2((c & 64u) ? inv_count(c) : 0.f, __uint_as_float((c & 64u) ? ~0u : 0u));
+    }
+    auto plane = [&](const float *base, int pl) { return base + (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
+    auto planeC = [&](int pl) { return L.c8 + (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
+    auto pair = [](const float *row, uint32_t o) { return *reinterpret_cast<const float2 *>(row + o); };
+    auto pairC = [](const uint8_t *row, uint32_t o) { return (unsigned)*reinterpret_cast<const uint16_t *>(row + o); };
+    auto fold = [](float2 &x, float xc, unsigned cc) {
+        x.x += (cc & 64u) ? xc : 0.f;
+        x.y += (cc & (64u << 8)) ? xc : 0.f;
+    };
+    // acc - b / cu as one fma, as the single sweep's contracted form: here each plane's b feeds two stages, and a
+    // product b / cu shared by both would otherwise be rounded before the subtraction
+    auto gs = [&](float acc, float bv, unsigned code) {
+        const float2 t = tab[code & 127u];
+        return __uint_as_float(__float_as_uint(__builtin_fmaf(-bv, L.icu, acc) * t.x) & __float_as_uint(t.y));
+    };
+    auto gsDirichlet = [&](float acc, float bv, int pl, uint32_t o) {
+        const unsigned c16 = L.c16[(size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS + o];
+        float d = 0.f;
+#pragma unroll
+        for (int q = 0; q < 6; q++)
+        {
+            const unsigned stt = (c16 >> (2 * q)) & 3u;
+            d += stt == 3u ? 2.f : (stt != 0u ? 1.f : 0.f);
+        }
+        return d != 0.f ? __builtin_fmaf(-bv, L.icu, acc) / d : 0.f;
+    };
+    // update of element E of a pair from its six neighbours (s+1, s-1, m+1, m-1, f+1, f-1), as the single sweep
+    auto update = [&](const float (&nb)[6], float bv, unsigned code, int pl, uint32_t o) {
+        float acc = nb[0];
+        acc += nb[1];
+        acc += nb[2];
+        acc += nb[3];
+        acc += nb[4];
+        acc += nb[5];
+        float v = gs(acc, bv, code);
+        if (code & 128u) v = gsDirichlet(acc, bv, pl, o);
+        return v;
+    };
+    // slots: X, XC by plane mod 4 (sp - 1 .. sp + 2), O, OC by plane mod 2, b and codes by plane mod 8 (sp + 2 ..
+    // sp - 3), the stages' own values by plane mod 4
+    float2 X[4][K], O[2][K], Bv[8][K];
+    unsigned C[8][K];
+    float XC[4][K], OC[2][K];
+    float R1[4][K], R2[4][K], RB[4][K];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+    {
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+        {
+            Bv[j][k] = make_float2(0.f, 0.f);
+            C[j][k] = 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+        {
+            X[j][k] = make_float2(0.f, 0.f);
+            XC[j][k] = R1[j][k] = R2[j][k] = RB[j][k] = 0.f;
+        }
+        O[0][k] = O[1][k] = make_float2(0.f, 0.f);
+        OC[0][k] = OC[1][k] = 0.f;
+    }
+    const int sp0 = s0 - 3;
+    // plane sp0 + d -> slot d & 3 (d & 7 for b and codes, d & 1 for O)
+#pragma unroll
+    for (int d = -1; d <= 1; d++)
+    {
+        const float *px = plane(xin, sp0 + d);
+        const float *pc = planeXC(sp0 + d);
+        const float *pb = plane(b, sp0 + d);
+        const uint8_t *pcc = planeC(sp0 + d);
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            X[(d + 4) & 3][k] = pair(px, off[k]);
+            XC[(d + 4) & 3][k] = pc[offC[k]];
+            C[(d + 8) & 7][k] = pairC(pcc, off[k]);
+            if (d >= 0) Bv[d][k] = pair(pb, off[k]);
+        }
+    }
+    if (outerW)
+    {
+        const float *px = plane(xin, sp0);
+        const float *pc = planeXC(sp0);
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            O[0][k] = pair(px, offO[k]);
+            OC[0][k] = pc[offCO[k]];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++)
+    {
+        fold(X[3][k], XC[3][k], C[7][k] & cm[k]);
+        fold(X[0][k], XC[0][k], C[0][k] & cm[k]);
+    }
+    __syncthreads();
+    const int wUp = min(wave + 1, NW - 1), wDn = max(wave - 1, 0);
+
+    auto step = [&](int sp, auto phase, auto colour) {
+        constexpr int U = decltype(phase)::value;
+        constexpr int E = decltype(colour)::value;
+        constexpr int Xm1 = (U + 3) & 3, X0 = U & 3, X1 = (U + 1) & 3, X2 = (U + 2) & 3;
+        constexpr int B0 = U & 7, Bm1 = (U + 7) & 7, Bm2 = (U + 6) & 7, Bm3 = (U + 5) & 7, B2s = (U + 2) & 7;
+        constexpr int O0 = U & 1, O1 = (U + 1) & 1;
+        // R slots: plane p -> (p - sp0) & 3
+        constexpr int R0 = U & 3, Rm1 = (U + 3) & 3, Rm2 = (U + 2) & 3, Rm3 = (U + 1) & 3, Rm4 = U & 3;
+        // prefetch: plane sp + 2 (x, b, codes, coarse x); the outer row of plane sp + 1
+        {
+            const float *px = plane(xin, sp + 2);
+            const float *pb = plane(b, sp + 2);
+            const uint8_t *pcc = planeC(sp + 2);
+            const float *pc2 = planeXC(sp + 2);
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                X[X2][k] = pair(px, off[k]);
+                Bv[B2s][k] = pair(pb, off[k]);
+                C[B2s][k] = pairC(pcc, off[k]);
+                XC[X2][k] = pc2[offC[k]];
+            }
+            if (outerW)
+            {
+                const float *po = plane(xin, sp + 1);
+                const float *pc1 = planeXC(sp + 1);
+#pragma unroll
+                for (int k = 0; k < K; k++)
+                {
+                    O[O1][k] = pair(po, offO[k]);
+                    OC[O1][k] = pc1[offCO[k]];
+                }
+            }
+        }
+        // publish this row's old x (with the correction) of plane sp
+        float(*ob)[2][kSlots] = old[sp & 1];
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            ob[wave][0][1 + lane + 64 * k] = (!ragged || okA[k]) ? X[X0][k].x : 0.f;
+            ob[wave][1][1 + lane + 64 * k] = (!ragged || okB[k]) ? X[X0][k].y : 0.f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; k++) fold(X[X1][k], XC[X1][k], C[(U + 1) & 7][k] & cm[k]);
+        // A1: first sweep, first colour, plane sp (every wave)
+        {
+            float v[K];
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                const unsigned code = ((C[B0][k] & cm[k]) >> (8 * E)) & 255u;
+                const int p = 1 + lane + 64 * k;
+                const float own = E ? X[X0][k].x : X[X0][k].y;
+                const float nbr = E ? ob[wave][0][p + 1] : ob[wave][1][p - 1];
+                float up = ob[wUp][E][p], dn = ob[wDn][E][p];
+                const float outer = E ? O[O0][k].y : O[O0][k].x;
+                if (wave == NW - 1) up = (code & 4u) ? outer + OC[O0][k] : 0.f;
+                if (wave == 0) dn = (code & 8u) ? outer + OC[O0][k] : 0.f;
+                const float nb[6] = {E ? X[X1][k].y : X[X1][k].x, E ? X[Xm1][k].y : X[Xm1][k].x, up, dn,
+                                     E ? nbr : own, E ? own : nbr};
+                v[k] = update(nb, E ? Bv[B0][k].y : Bv[B0][k].x, code, sp, off[k] + (uint32_t)E);
+                R1[R0][k] = v[k];
+            }
+#pragma unroll
+            for (int k = 0; k < K; k++) a1s[sp & 1][wave][fa[k] + E] = v[k];
+        }
+        // A2: first sweep, second colour, plane sp - 1 (element E there)
+        if (doA2)
+        {
+            const int q = sp - 1;
+            const float(*rq)[kCols] = a1s[q & 1];
+            float v[K];
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                const unsigned code = ((C[Bm1][k] & cm[k]) >> (8 * E)) & 255u;
+                const int f = fa[k] + E;
+                const float nb[6] = {R1[R0][k], R1[Rm2][k], rq[wave + 1][f], rq[wave - 1][f],
+                                     E ? rq[wave][f + 1] : R1[Rm1][k], E ? R1[Rm1][k] : rq[wave][f - 1]};
+                v[k] = update(nb, E ? Bv[Bm1][k].y : Bv[Bm1][k].x, code, q, off[k] + (uint32_t)E);
+                R2[Rm1][k] = v[k];
+            }
+#pragma unroll
+            for (int k = 0; k < K; k++) a2s[q & 1][wave][fa[k] + E] = v[k];
+        }
+        // B1: second sweep, first colour, plane sp - 2 (element E there): from the first sweep's values
+        if (doB1)
+        {
+            const int q = sp - 2;
+            const float(*rq)[kCols] = a2s[q & 1];
+            float v[K];
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                const unsigned code = ((C[Bm2][k] & cm[k]) >> (8 * E)) & 255u;
+                const int f = fa[k] + E;
+                const float nb[6] = {R2[Rm1][k], R2[Rm3][k], rq[wave + 1][f], rq[wave - 1][f],
+                                     E ? rq[wave][f + 1] : R2[Rm2][k], E ? R2[Rm2][k] : rq[wave][f - 1]};
+                v[k] = update(nb, E ? Bv[Bm2][k].y : Bv[Bm2][k].x, code, q, off[k] + (uint32_t)E);
+                RB[Rm2][k] = v[k];
+            }
+#pragma unroll
+            for (int k = 0; k < K; k++) b1s[q & 1][wave][fa[k] + E] = v[k];
+        }
+        // B2: second sweep, second colour, plane sp - 3 (element E there), stored
+        const int q = sp - 3;
+        if (doB2 && q >= s0 && q <= s1)
+        {
+            float *po = xout + (size_t)q * (size_t)B.sS;
+            const float(*rq)[kCols] = b1s[q & 1];
+            double bx = 0.0;
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                const unsigned cc = C[Bm3][k] & cm[k];
+                const unsigned code = (cc >> (8 * E)) & 255u;
+                const int f = fa[k] + E;
+                const float nb[6] = {RB[Rm2][k], RB[Rm4][k], rq[wave + 1][f], rq[wave - 1][f],
+                                     E ? rq[wave][f + 1] : RB[Rm3][k], E ? RB[Rm3][k] : rq[wave][f - 1]};
+                const float out = update(nb, E ? Bv[Bm3][k].y : Bv[Bm3][k].x, code, q, off[k] + (uint32_t)E);
+                const float va = E ? RB[Rm3][k] : out, vb = E ? out : RB[Rm3][k];
+                if (dot)
+                {
+                    bx += (cc & 64u) ? (double)Bv[Bm3][k].x * (double)va : 0.0;
+                    bx += (cc & (64u << 8)) ? (double)Bv[Bm3][k].y * (double)vb : 0.0;
+                }
+                if (okB[k])
+                    *reinterpret_cast<float2 *>(po + off[k]) = make_float2(va, vb);
+                else if (okA[k])
+                    po[off[k]] = va;
+            }
+            if (dot)
+            {
+                const double rowDot = wave_sum(bx);
+                if (lane == 0 && rowIn) dot[(size_t)(q - 1) * (size_t)(B.nm - 2) + (size_t)(m - 1)] = rowDot;
+            }
+        }
+    };
+    // the first-colour element at plane sp is (c1 - sp - m - 1) & 1
+    auto march = [&](auto colour0) {
+        constexpr int E0 = decltype(colour0)::value;
+        for (int sp = sp0; sp <= s1 + 3; sp += 8)
+        {
+            step(sp, std::integral_constant<int, 0>(), std::integral_constant<int, E0>());
+            if (sp + 1 > s1 + 3) break;
+            step(sp + 1, std::integral_constant<int, 1>(), std::integral_constant<int, E0 ^ 1>());
+            if (sp + 2 > s1 + 3) break;
+            step(sp + 2, std::integral_constant<int, 2>(), std::integral_constant<int, E0>());
+            if (sp + 3 > s1 + 3) break;
+            step(sp + 3, std::integral_constant<int, 3>(), std::integral_constant<int, E0 ^ 1>());
+            if (sp + 4 > s1 + 3) break;
+            step(sp + 4, std::integral_constant<int, 4>(), std::integral_constant<int, E0>());
+            if (sp + 5 > s1 + 3) break;
+            step(sp + 5, std::integral_constant<int, 5>(), std::integral_constant<int, E0 ^ 1>());
+            if (sp + 6 > s1 + 3) break;
+            step(sp + 6, std::integral_constant<int, 6>(), std::integral_constant<int, E0>());
+            if (sp + 7 > s1 + 3) break;
+            step(sp + 7, std::integral_constant<int, 7>(), std::integral_constant<int, E0 ^ 1>());
+        }
+    };
+    if ((c1 - sp0 - m - 1) & 1)
+        march(std::integral_constant<int, 1>());
+    else
+        march(std::integral_constant<int, 0>());
+}
+
 // Coarse cell (S, M, F) gathers the residual b - A x of its fine children (fine indices 2C-1, 2C along each
 // coarsened axis, clipped to the fine interior): bc = sum (R = P^T for the injection prolongation).
 __global__ __launch_bounds__(kBlockThreads) void k_residual_restrict(Level32 Lf, const float *__restrict__ x,
@@ -3520,6 +3861,24 @@ __global__ __launch_bounds__(64 * (TM + 2)) void kb_rbgs_row(const RowSweepArgs 
 }
 
 template <int TM, int K>
+__global__ __launch_bounds__(64 * (TM + 6)) void k_rbgs2_row(Level32 L, const float *__restrict__ xin,
+                                                             float *__restrict__ xout, const float *__restrict__ b,
+                                                             int c1, int P, int tilesM, const SolveState *st,
+                                                             double *__restrict__ dot, Prolong pro)
+{
+    rbgs2_row_body<TM, K>(L, xin, xout, b, c1, P, tilesM, st, dot, pro, logical_block());
+}
+
+template <int TM, int K>
+__global__ __launch_bounds__(64 * (TM + 6)) void kb_rbgs2_row(const RowSweepArgs *__restrict__ args,
+                                                              const int *__restrict__ start, int nsys)
+{
+    const BatchBlock bb = batch_block(start, nsys);
+    const RowSweepArgs a = args[bb.q];
+    rbgs2_row_body<TM, K>(a.L, a.xin, a.xout, a.b, a.c1, a.P, a.tilesM, a.st, a.dot, a.pro, bb.lb);
+}
+
+template <int TM, int K>
 __global__ __launch_bounds__(64 * (TM + 2)) void k_restrict_row(Level32 Lf, const float *__restrict__ x,
                                                                 const float *__restrict__ b, Level32 Lc,
                                                                 float *__restrict__ bc, const SolveState *st, int P,
@@ -3780,6 +4139,14 @@ void row_sweep(hipStream_t s, int grid, int threads, const RowSweepArgs &a)
     }
     k_rbgs_row<TM, K, ZERO, PRO, NU><<<grid, threads, 0, s>>>(a.L, a.xin, a.xout, a.b, a.c1, a.P, a.tilesM, a.st,
                                                               a.dot, a.pro);
+}
+
+template <int TM, int K>
+void row_sweep2(hipStream_t s, int grid, const RowSweepArgs &a)
+{
+    if (record(kb_rbgs2_row<TM, K>, grid, 64 * (TM + 6), a)) return;
+    k_rbgs2_row<TM, K><<<grid, 64 * (TM + 6), 0, s>>>(a.L, a.xin, a.xout, a.b, a.c1, a.P, a.tilesM, a.st, a.dot,
+                                                       a.pro);
 }
 
 template <int TM, int K>
@@ -4140,6 +4507,38 @@ void rbgs_tile_candidates(int family, std::vector<int> &tileRows)
     if (family == RBGS_ROW) tileRows = {6, 14};
     else if (family == RBGS_MARCH) tileRows = {4, 6, 8};
     else tileRows = {8, 16};
+}
+
+// Tile rows of the two-sweep march: three halo rows each side of 16 waves, rows of at most 128 cells.
+constexpr int kRbgs2Rows = 10;
+
+bool rbgs2_level(const Level32 &L, const float *xin, const float *xout, const float *b)
+{
+    const char *env = std::getenv("NPPE_FUSE_POST");
+    if (env && std::atoi(env) == 0) return false;
+    const char *tiles = std::getenv("NPPE_TILE_PASSES");
+    if (tiles && std::atoi(tiles) != 0) return false;
+    return L.B.threeD && L.uniform && L.B.nf - 2 <= 128 && rbgs_family(L, xin, xout, b) == RBGS_ROW;
+}
+
+int64_t rbgs2_blocks(const Box &B, int planesPerChunk)
+{
+    const int P = std::max(1, planesPerChunk);
+    return (int64_t)((B.nm - 2 + kRbgs2Rows - 1) / kRbgs2Rows) * ((B.ns - 2 + P - 1) / P);
+}
+
+hipError_t launch_rbgs2(hipStream_t s, const Level32 &L, const float *xin, float *xout, const float *b, int c1,
+                        int planesPerChunk, SolveState *st, double *rzPartials, const Level32 &Lc, const float *xc)
+{
+    const Box &B = L.B;
+    if (!rbgs2_level(L, xin, xout, b) || xin == xout || !xc) return hipErrorInvalidValue;
+    const int P = std::max(1, planesPerChunk);
+    const int tilesM = (B.nm - 2 + kRbgs2Rows - 1) / kRbgs2Rows;
+    const int grid = tilesM * ((B.ns - 2 + P - 1) / P);
+    const Prolong pro{xc, Lc.B.sS, (int)Lc.B.sM, Lc.B.ns, Lc.B.nf};
+    const RowSweepArgs ra{L, xin, xout, b, c1, P, tilesM, st, rzPartials, pro};
+    row_sweep2<kRbgs2Rows, 1>(s, grid, ra);
+    return hipGetLastError();
 }
 
 hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float *xout, const float *b, int c1,

@@ -278,6 +278,28 @@ hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel, doubl
         const double pb = s * C.cellsBox + 2 * s * N + f * N;
         const int nu2 = (coarseNu > 0 && l >= 2) ? coarseNu : h->nuPost;
         record_slot(SLOT_UP, l);
+        if (nu2 == 2 && rbgs2_level(L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l)))
+        {
+            // rows of at most 128 cells: the correction and both sweeps in one march
+            const bool dot = l == 0 && rzPartials;
+            const double by = (3 * s + f) * N + s * C.cellsBox;
+            {
+                Scope sc(h, l == 0 ? NPPE_K_SMOOTH0 : NPPE_K_SMOOTH, by);
+                if ((e = launch_rbgs2(h->stream, L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l), 1, L.post2Planes,
+                                      h->dState, dot ? rzPartials : nullptr, C.v, xbuf(l + 1, cur[l + 1]))) !=
+                    hipSuccess)
+                    return e;
+            }
+            if (dot)
+            {
+                *rzDone = true;
+                Scope sc(h, NPPE_K_CG_DOT, 0.0);
+                if ((e = launch_rz_rows_finalize(h->stream, L.box, h->dState, rzPartials)) != hipSuccess) return e;
+            }
+            cur[l] ^= 1;
+            tr.add(by);
+            continue;
+        }
         // The whole-row sweep folds the prolongation into its first post-smoothing sweep (it reads x + P x_c
         // where it reads x); other sweeps get the correction as a pass of its own.
         const bool foldProlong =
@@ -467,6 +489,21 @@ hipError_t choose_tiles(nppe_solver *h)
         if (envTM) L.tileRows = std::max(1, std::atoi(envTM));
         if (envP) L.planesPerChunk = std::max(1, std::atoi(envP));
         L.proPlanes = L.planesPerChunk;
+        // the post-smoothing pair in one march: rounds of resident blocks (one per CU) times its plane steps (P + 6)
+        {
+            double bestCost = 1e300;
+            for (int P = 1; P <= std::max(1, std::min(64, L.box.ns - 2)); P++)
+            {
+                const double cost = (double)((rbgs2_blocks(L.box, P) + cus - 1) / cus) * (P + 6);
+                if (cost < bestCost)
+                {
+                    bestCost = cost;
+                    L.post2Planes = P;
+                }
+            }
+            const char *env2 = std::getenv("NPPE_POST2_P");
+            if (env2) L.post2Planes = std::max(1, std::atoi(env2));
+        }
         // the sweep with the restriction folded in: the tallest tile, the chunk of the same block-round model
         // (even: the restriction sums plane pairs)
         {

@@ -15,7 +15,8 @@ SHAPES = [("6", "1"), ("6", "5"), ("14", "3"), ("14", "16"), ("4", "2"), ("8", "
 
 def _solve(g, b, env):
     saved = {k: os.environ.get(k) for k in ("NPPE_RBGS_TM", "NPPE_RBGS_P", "NPPE_AUTOTUNE", "NPPE_RBGS_MARCH",
-                                            "NPPE_FUSE_RESTRICT", "NPPE_FUSE_TM", "NPPE_FUSE_P")}
+                                            "NPPE_FUSE_RESTRICT", "NPPE_FUSE_TM", "NPPE_FUSE_P", "NPPE_FUSE_POST",
+                                            "NPPE_POST2_P")}
     try:
         for k in saved:
             os.environ.pop(k, None)
@@ -101,3 +102,53 @@ def test_fused_restriction_tile_shapes(tm, p):
     x, res = _solve(g, b, {"NPPE_FUSE_TM": tm, "NPPE_FUSE_P": p})
     assert res.iterations == rres.iterations
     assert np.array_equal(x, ref), (tm, p)
+
+
+@pytest.mark.parametrize("n,kind", [
+    (40, "mixed"),                       # level 0 too (rows of 40 cells): the march also writes <b, z> per row
+    ((24, 20, 140), "pressure"),         # level 0 of 140-cell rows keeps two launches; levels 1 and 2 march both
+    ((33, 27, 61), "outlet"),            # odd sizes, Dirichlet codes
+    ((31, 45, 97), "mixed"),             # odd plane count, solids
+    ((100, 90, 24), "neumann"),          # many rows and planes: several row tiles and chunks
+])
+def test_post_smoothing_pair_equals_two_sweeps(n, kind):
+    """Both post-smoothing sweeps of a level with rows of at most 128 cells run as one march (the second sweep
+    three planes behind the first, three halo rows each side of a tile); every chunk length gives the bits of the
+    two separate launches."""
+    g, _ = make(3, n, kind, False, seed=11)
+    b = np.random.default_rng(12).standard_normal(g.unknowns)
+    if kind == "neumann":
+        b -= b.mean()
+    ref, rres = _solve(g, b, {"NPPE_FUSE_POST": "0"})
+    for p in (None, "1", "2", "5", "64"):
+        x, res = _solve(g, b, {} if p is None else {"NPPE_POST2_P": p})
+        assert res.iterations == rres.iterations
+        assert np.array_equal(x, ref), p
+
+
+def test_post_smoothing_pair_batched():
+    """The batched form of the march (systems of different shapes in shared launches) gives each system the bits
+    of its own solve with the two separate post-smoothing launches."""
+    import torch
+    cases = [make(3, 40, "mixed", False, seed=13)[0], make(3, (24, 20, 140), "pressure", False, seed=14)[0],
+             make(3, 28, "neumann", False, seed=15)[0]]
+    bs = []
+    for i, g in enumerate(cases):
+        b = np.random.default_rng(20 + i).standard_normal(g.unknowns)
+        bs.append(b - b.mean() if i == 2 else b)
+    refs = [_solve(g, b, {"NPPE_FUSE_POST": "0"})[0] for g, b in zip(cases, bs)]
+    solvers = [N.Solver(tolerance=1e-9) for _ in cases]
+    try:
+        for h, g in zip(solvers, cases):
+            h.commit(g)
+        tb = [torch.from_numpy(b).cuda() for b in bs]
+        tx = [torch.zeros_like(t) for t in tb]
+        torch.cuda.synchronize()
+        _, rc = N.batch_solve_device(solvers, [t.data_ptr() for t in tb], [t.data_ptr() for t in tx])
+        torch.cuda.synchronize()
+        assert rc == 0
+        for t, xr in zip(tx, refs):
+            assert np.array_equal(t.cpu().numpy(), xr)
+    finally:
+        for h in solvers:
+            h.close()
