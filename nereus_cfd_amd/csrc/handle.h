@@ -38,6 +38,7 @@ struct DevLevel
     int planesPerChunk = 8;
     int proPlanes = 8;       // planes per chunk of the sweep that folds the coarse correction in
     int post2Planes = 8;     // planes per chunk of the post-smoothing pair in one march (launch_rbgs2)
+    int pre2Planes = 8;      // planes per chunk of the pre-smoothing and restriction in one march
     int fuseRows = 12;       // tile rows and planes per chunk of the last pre-smoothing sweep with the restriction
     int fusePlanes = 8;      // folded in (launch_rbgs_restrict)
     double cellsBox = 0.0;   // ns * nm * nf (the box, unpitched)

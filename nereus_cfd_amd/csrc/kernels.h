@@ -138,6 +138,15 @@ hipError_t launch_rbgs_restrict(hipStream_t s, const Level32 &Lf, const float *x
                                 int c1, const Level32 &Lc, float *bc, int tileRows, int planesPerChunk,
                                 const SolveState *st);
 int64_t rbgs_restrict_blocks(const Box &B, int tileRows, int planesPerChunk);
+// Both pre-smoothing sweeps from x = 0 and the restriction of the residual in one march: x_out = S(S_0(b)) (colour c1
+// first in both) and bc = R (b - A x_out), bit for bit those of launch_rbgs (zero guess) followed by
+// launch_rbgs_restrict; x_out is the only x written or read. rbgs_pre_restrict_level says whether a level has it
+// (rows of at most 128 cells where rbgs_restrict_fusable holds; NPPE_FUSE_PRE=0 turns it off: the tests compare the
+// two); planesPerChunk is made even.
+bool rbgs_pre_restrict_level(const Level32 &Lf, const float *xin, const float *xout, const float *b, const float *bc);
+int64_t rbgs_pre_restrict_blocks(const Box &B, int planesPerChunk);
+hipError_t launch_rbgs_pre_restrict(hipStream_t s, const Level32 &Lf, float *xout, const float *b, int c1,
+                                    const Level32 &Lc, float *bc, int planesPerChunk, const SolveState *st);
 hipError_t launch_prolong(hipStream_t s, const Level32 &Lf, float *x, const Level32 &Lc, const float *xc,
                           const SolveState *st);
 // Coarsest level: x[order[i]] = sum_j M[i][j] b[order[j]] in FP64 (M: the dense pseudo-inverse, n x n).

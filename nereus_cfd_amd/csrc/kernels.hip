@@ -3486,6 +3486,275 @@ __device__ __forceinline__ void rbgs_restrict_row_body(const Level32 &L, const f
     family(s1);
 }
 
+// ---- the whole pre-smoothing and the restriction in one march (whole-row waves, 3D uniform levels) -------------
+//
+// Both pre-smoothing sweeps of a level with rows of at most 128 cells, from x = 0, and the restriction of the residual
+// as one launch: a plane step sp computes the zero-guess sweep's first colour at plane sp (from b alone: with x = 0
+// its neighbours are zero) and its second colour at sp - 1, the plain sweep's first colour at sp - 2 and its second
+// colour at sp - 3 (stored), the residual of plane sp - 4 and the family sums of plane sp - 5. Each stage computes
+// what the kernel it replaces computes, from the same values in the same order (k_rbgs_row's zero-guess and plain
+// sweeps, k_rbgs_restrict_row's residual and family sums), so x and the coarse right-hand side are bit for bit
+// theirs. A block is TM output rows and four halo rows each side: the zero-guess first colour runs on every wave, each
+// later stage on one row fewer each side. No x is read: b and codes once, x and the coarse b written once. Values pass between stages through LDS, double-buffered by plane parity, with the one barrier of the step;
+// b and codes stay in registers from their load (two planes ahead) to the residual (slots by plane mod 8).
+template <int TM>
+__device__ __forceinline__ void rbgs_pre_restrict_row_body(const Level32 &L, float *__restrict__ xout,
+                                                           const float *__restrict__ b, int c1, const Level32 &Lc,
+                                                           float *__restrict__ bc, int P, int tilesM,
+                                                           const SolveState *st, int lb)
+{
+    static_assert((TM & 1) == 0, "rows are summed in pairs");
+    if (st->done) return;
+    constexpr int W = TM + 8;   // waves: rows m0 - 4 .. m0 + TM + 3
+    constexpr int kCols = 130;
+    constexpr int kSlots = 66;
+    const Box &B = L.B;
+    __shared__ float z1s[2][W][kCols];       // zero-guess sweep, first colour, by column
+    __shared__ float z2s[2][W][kCols];       // zero-guess sweep, second colour
+    __shared__ float s1s[2][W][kCols];       // plain sweep, first colour
+    __shared__ float fin[4][W][2][kSlots];   // the swept x of a plane by slot (plane - sp0) & 3, row, element, pair + 1
+    __shared__ float ldsR[2][TM][64];        // the output rows' pair sums of the residual by plane parity
+    __shared__ float2 tab[128];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int tm = lb % tilesM, chunk = lb / tilesM;
+    const int m0 = 1 + tm * TM;
+    const int m = m0 - 4 + wave;
+    const bool doZ2 = wave >= 1 && wave <= W - 2, doS1 = wave >= 2 && wave <= W - 3;
+    const bool doS2 = wave >= 3 && wave <= W - 4, interior = wave >= 4 && wave <= W - 5;
+    const int i = wave - 4;                       // output row index of the block
+    const bool famRow = interior && !(i & 1);     // the odd row of a row pair: writes the family sums
+    const bool rowIn = m >= 1 && m <= B.nm - 2;
+    const int mc = min(max(m, 0), B.nm - 1);
+    const int s0 = 1 + chunk * P, s1 = min(s0 + P - 1, B.ns - 2);
+    const uint32_t sM = (uint32_t)B.sM;
+    const int fcMax = ((B.nf - 2) & 1) ? B.nf - 2 : B.nf - 3;
+    const int fa = 1 + 2 * lane;
+    const bool colA = fa <= B.nf - 2, colB = fa + 1 <= B.nf - 2;
+    const bool okA = rowIn && colA, okB = rowIn && colB;
+    const unsigned cm = (okA ? 0xFFu : 0u) | (okB ? 0xFF00u : 0u);
+    const int fc = min(fa, fcMax);
+    const uint32_t off = (uint32_t)mc * sM + (uint32_t)fc;
+    for (int q = threadIdx.x; q < 2 * W * kCols; q += 64 * W)
+    {
+        (&z1s[0][0][0])[q] = 0.f;
+        (&z2s[0][0][0])[q] = 0.f;
+        (&s1s[0][0][0])[q] = 0.f;
+    }
+    for (int q = threadIdx.x; q < 4 * W * 2 * kSlots; q += 64 * W) (&fin[0][0][0][0])[q] = 0.f;
+    if (threadIdx.x < 128)
+    {
+        const unsigned c = threadIdx.x;
+        tab[c] = make_float2((c & 64u) ? inv_count(c) : 0.f, __uint_as_float((c & 64u) ? ~0u : 0u));
+    }
+    auto plane = [&](const float *base, int pl) { return base + (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
+    auto planeC = [&](int pl) { return L.c8 + (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
+    auto plane_off = [&](int pl) { return (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
+    auto pair = [](const float *row, uint32_t o) { return *reinterpret_cast<const float2 *>(row + o); };
+    auto pairC = [](const uint8_t *row, uint32_t o) { return (unsigned)*reinterpret_cast<const uint16_t *>(row + o); };
+    // acc - b / cu as one fma, the single sweep's contracted form (each plane's b feeds several stages here, and a
+    // product b / cu shared by them would otherwise be rounded before the subtraction)
+    auto gs = [&](float acc, float bv, unsigned code) {
+        const float2 t = tab[code & 127u];
+        return __uint_as_float(__float_as_uint(__builtin_fmaf(-bv, L.icu, acc) * t.x) & __float_as_uint(t.y));
+    };
+    auto gsDirichlet = [&](float acc, float bv, int pl, uint32_t o) {
+        const unsigned c16 = L.c16[(size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS + o];
+        float d = 0.f;
+#pragma unroll
+        for (int q = 0; q < 6; q++)
+        {
+            const unsigned stt = (c16 >> (2 * q)) & 3u;
+            d += stt == 3u ? 2.f : (stt != 0u ? 1.f : 0.f);
+        }
+        return d != 0.f ? __builtin_fmaf(-bv, L.icu, acc) / d : 0.f;
+    };
+    // the update of element E of a pair from its six neighbours (s+1, s-1, m+1, m-1, f+1, f-1), as the single sweep
+    auto update = [&](const float (&nb)[6], float bv, unsigned code, int pl, uint32_t o) {
+        float acc = nb[0];
+        acc += nb[1];
+        acc += nb[2];
+        acc += nb[3];
+        acc += nb[4];
+        acc += nb[5];
+        float v = gs(acc, bv, code);
+        if (code & 128u) v = gsDirichlet(acc, bv, pl, o);
+        return v;
+    };
+    // the zero-guess first colour (no neighbours: an iterate is never -0, so 0 stands for their sum)
+    auto first0 = [&](float bv, unsigned code, int pl, uint32_t o) {
+        float v = gs(0.f, bv, code);
+        if (code & 128u) v = gsDirichlet(0.f, bv, pl, o);
+        return v;
+    };
+    // slots: b and codes by plane mod 8 (sp + 2 .. sp - 4), the stages' own values by plane mod 4 (relative to sp0)
+    float2 Bv[8];
+    unsigned C[8];
+    float R1[4], R2[4], RS[4];
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+    {
+        Bv[j] = make_float2(0.f, 0.f);
+        C[j] = 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) R1[j] = R2[j] = RS[j] = 0.f;
+    float acc = 0.f;
+    const int sp0 = s0 - 4;
+#pragma unroll
+    for (int d = 0; d <= 1; d++)
+    {
+        Bv[d] = pair(plane(b, sp0 + d), off);
+        C[d] = pairC(planeC(sp0 + d), off);
+    }
+    __syncthreads();
+    // the family sums of plane q from the rows' pair sums in ldsR[q & 1]
+    auto family = [&](int q) {
+        if (!famRow || q < s0 || q > s1) return;
+        const size_t cpl = (size_t)((q + 1) >> 1) * (size_t)Lc.B.sS + (size_t)((m + 1) >> 1) * (size_t)Lc.B.sM;
+        acc += ldsR[q & 1][i][lane] + ldsR[q & 1][i + 1][lane];
+        if (!(q & 1) || q == s1)
+        {
+            if (okA) bc[cpl + lane + 1] = acc;
+            acc = 0.f;
+        }
+    };
+    auto step = [&](int sp, auto phase, auto colour) {
+        constexpr int U = decltype(phase)::value;
+        constexpr int E = decltype(colour)::value;
+        constexpr int B0 = U & 7, Bm1 = (U + 7) & 7, Bm2 = (U + 6) & 7, Bm3 = (U + 5) & 7, Bm4 = (U + 4) & 7;
+        constexpr int B2s = (U + 2) & 7;
+        constexpr int R0 = U & 3, Rm1 = (U + 3) & 3, Rm2 = (U + 2) & 3, Rm3 = (U + 1) & 3, Rm4 = U & 3;
+        // fin slots: plane t = sp - 4, t + 1, t - 1
+        constexpr int F0 = U & 3, Fp = (U + 1) & 3, Fm = (U + 3) & 3;
+        // prefetch: b and codes of plane sp + 2
+        Bv[B2s] = pair(plane(b, sp + 2), off);
+        C[B2s] = pairC(planeC(sp + 2), off);
+        // the one barrier of the step: what the stages of the step before wrote (z1s of plane sp - 1, z2s of sp - 2,
+        // s1s of sp - 3, fin of sp - 4, ldsR of sp - 5) is complete, and their readers of a step before are done
+        __syncthreads();
+        family(sp - 5);
+        // Z1: zero-guess sweep, first colour, plane sp (element E)
+        {
+            const unsigned code = ((C[B0] & cm) >> (8 * E)) & 255u;
+            const float v = first0(E ? Bv[B0].y : Bv[B0].x, code, sp, off + (uint32_t)E);
+            R1[R0] = v;
+            z1s[sp & 1][wave][fa + E] = v;
+        }
+        // Z2: zero-guess sweep, second colour, plane sp - 1 (element E there)
+        if (doZ2)
+        {
+            const int q = sp - 1;
+            const float(*rq)[kCols] = z1s[q & 1];
+            const unsigned code = ((C[Bm1] & cm) >> (8 * E)) & 255u;
+            const int f = fa + E;
+            const float nb[6] = {R1[R0], R1[Rm2], rq[wave + 1][f], rq[wave - 1][f],
+                                 E ? rq[wave][f + 1] : R1[Rm1], E ? R1[Rm1] : rq[wave][f - 1]};
+            const float v = update(nb, E ? Bv[Bm1].y : Bv[Bm1].x, code, q, off + (uint32_t)E);
+            R2[Rm1] = v;
+            z2s[q & 1][wave][f] = v;
+        }
+        // S1: plain sweep, first colour, plane sp - 2 (element E there), from the zero-guess sweep's second colour
+        if (doS1)
+        {
+            const int q = sp - 2;
+            const float(*rq)[kCols] = z2s[q & 1];
+            const unsigned code = ((C[Bm2] & cm) >> (8 * E)) & 255u;
+            const int f = fa + E;
+            const float nb[6] = {R2[Rm1], R2[Rm3], rq[wave + 1][f], rq[wave - 1][f],
+                                 E ? rq[wave][f + 1] : R2[Rm2], E ? R2[Rm2] : rq[wave][f - 1]};
+            const float v = update(nb, E ? Bv[Bm2].y : Bv[Bm2].x, code, q, off + (uint32_t)E);
+            RS[Rm2] = v;
+            s1s[q & 1][wave][f] = v;
+        }
+        // S2: plain sweep, second colour, plane sp - 3 (element E there): the swept x of that plane, kept for the
+        // residual and stored on the output rows
+        if (doS2)
+        {
+            const int q = sp - 3;
+            const float(*rq)[kCols] = s1s[q & 1];
+            const unsigned code = ((C[Bm3] & cm) >> (8 * E)) & 255u;
+            const int f = fa + E;
+            const float nb[6] = {RS[Rm2], RS[Rm4], rq[wave + 1][f], rq[wave - 1][f],
+                                 E ? rq[wave][f + 1] : RS[Rm3], E ? RS[Rm3] : rq[wave][f - 1]};
+            const float out = update(nb, E ? Bv[Bm3].y : Bv[Bm3].x, code, q, off + (uint32_t)E);
+            const float va = E ? RS[Rm3] : out, vb = E ? out : RS[Rm3];
+            fin[Fp][wave][0][1 + lane] = va;
+            fin[Fp][wave][1][1 + lane] = vb;
+            if (interior && q >= s0 && q <= s1)
+            {
+                float *po = xout + (size_t)q * (size_t)B.sS;
+                if (okB)
+                    *reinterpret_cast<float2 *>(po + off) = make_float2(va, vb);
+                else if (okA)
+                    po[off] = va;
+            }
+        }
+        // residual of plane t = sp - 4 on the output rows (k_rbgs_restrict_row's operations) and the row's pair sums
+        const int t = sp - 4;
+        if (interior && t >= s0 && t <= s1)
+        {
+            const float(*ft)[2][kSlots] = fin[F0];
+            const float(*fp)[2][kSlots] = fin[Fp];
+            const float(*fm)[2][kSlots] = fin[Fm];
+            const int p = 1 + lane;
+            float r[2];
+#pragma unroll
+            for (int e = 0; e < 2; e++)
+            {
+                const unsigned code = ((C[Bm4] & cm) >> (8 * e)) & 255u;
+                const float xc = ft[wave][e][p];
+                const float nb[6] = {fp[wave][e][p], fm[wave][e][p], ft[wave + 1][e][p], ft[wave - 1][e][p],
+                                     e ? ft[wave][0][p + 1] : ft[wave][1][p], e ? ft[wave][0][p] : ft[wave][1][p - 1]};
+                const float bv = e ? Bv[Bm4].y : Bv[Bm4].x;
+                float v = 0.f;
+                if (code & 64u)
+                {
+                    if (!(code & 128u))
+                    {
+#pragma clang fp contract(off)
+                        const float s6 = nb[0] + nb[1] + nb[2] + nb[3] + nb[4] + nb[5];
+                        const float ax = L.cu * __builtin_fmaf(-(float)__popc(code & 63u), xc, s6);
+                        v = bv - ax;
+                    }
+                    else
+                        v = bv - apply32v(L, t, m, 2 * p - 1 + e, (int64_t)plane_off(t) + off + e, code, xc, nb);
+                }
+                r[e] = v;
+            }
+            ldsR[t & 1][i][lane] = r[0] + r[1];
+        }
+    };
+    // the first-colour element at plane sp is (c1 - sp - m - 1) & 1
+    auto march = [&](auto colour0) {
+        constexpr int E0 = decltype(colour0)::value;
+        for (int sp = sp0; sp <= s1 + 4; sp += 8)
+        {
+            step(sp, std::integral_constant<int, 0>(), std::integral_constant<int, E0>());
+            if (sp + 1 > s1 + 4) break;
+            step(sp + 1, std::integral_constant<int, 1>(), std::integral_constant<int, E0 ^ 1>());
+            if (sp + 2 > s1 + 4) break;
+            step(sp + 2, std::integral_constant<int, 2>(), std::integral_constant<int, E0>());
+            if (sp + 3 > s1 + 4) break;
+            step(sp + 3, std::integral_constant<int, 3>(), std::integral_constant<int, E0 ^ 1>());
+            if (sp + 4 > s1 + 4) break;
+            step(sp + 4, std::integral_constant<int, 4>(), std::integral_constant<int, E0>());
+            if (sp + 5 > s1 + 4) break;
+            step(sp + 5, std::integral_constant<int, 5>(), std::integral_constant<int, E0 ^ 1>());
+            if (sp + 6 > s1 + 4) break;
+            step(sp + 6, std::integral_constant<int, 6>(), std::integral_constant<int, E0>());
+            if (sp + 7 > s1 + 4) break;
+            step(sp + 7, std::integral_constant<int, 7>(), std::integral_constant<int, E0 ^ 1>());
+        }
+    };
+    if ((c1 - sp0 - m - 1) & 1)
+        march(std::integral_constant<int, 1>());
+    else
+        march(std::integral_constant<int, 0>());
+    __syncthreads();
+    family(s1);
+}
+
 // Residual of every fine cell, summed over each 2 x 2 x 2 family into the coarse right-hand side: fast pairs are
 // neighbouring lanes (shuffle), mid pairs neighbouring rows of the block (LDS), slow pairs consecutive planes of
 // the march (P even, chunks start on an odd plane). A family cut by the end of an odd interior keeps its 1 or 2
@@ -3923,6 +4192,24 @@ __global__ __launch_bounds__(64 * (TM + 4)) void kb_rbgs_restrict_row(const Swee
     rbgs_restrict_row_body<TM, K>(a.L, a.xin, a.xout, a.b, a.c1, a.Lc, a.bc, a.P, a.tilesM, a.st, bb.lb);
 }
 
+template <int TM>
+__global__ __launch_bounds__(64 * (TM + 8)) void k_rbgs_pre_restrict_row(Level32 L, float *__restrict__ xout,
+                                                                         const float *__restrict__ b, int c1,
+                                                                         Level32 Lc, float *__restrict__ bc, int P,
+                                                                         int tilesM, const SolveState *st)
+{
+    rbgs_pre_restrict_row_body<TM>(L, xout, b, c1, Lc, bc, P, tilesM, st, logical_block());
+}
+
+template <int TM>
+__global__ __launch_bounds__(64 * (TM + 8)) void kb_rbgs_pre_restrict_row(const SweepRestrictArgs *__restrict__ args,
+                                                                          const int *__restrict__ start, int nsys)
+{
+    const BatchBlock bb = batch_block(start, nsys);
+    const SweepRestrictArgs a = args[bb.q];
+    rbgs_pre_restrict_row_body<TM>(a.L, a.xout, a.b, a.c1, a.Lc, a.bc, a.P, a.tilesM, a.st, bb.lb);
+}
+
 __global__ __launch_bounds__(kBlockThreads) void kb_residual_restrict_m(const RestrictMarchArgs *__restrict__ args,
                                                                         const int *__restrict__ start, int nsys)
 {
@@ -4155,6 +4442,14 @@ void sweep_restrict(hipStream_t s, int grid, const SweepRestrictArgs &a)
     if (record(kb_rbgs_restrict_row<TM, K>, grid, 64 * (TM + 4), a, REC_SWEEP_RESTRICT, TM, K)) return;
     k_rbgs_restrict_row<TM, K><<<grid, 64 * (TM + 4), 0, s>>>(a.L, a.xin, a.xout, a.b, a.c1, a.Lc, a.bc, a.P,
                                                                  a.tilesM, a.st);
+}
+
+template <int TM>
+void sweep_pre_restrict(hipStream_t s, int grid, const SweepRestrictArgs &a)
+{
+    if (record(kb_rbgs_pre_restrict_row<TM>, grid, 64 * (TM + 8), a)) return;
+    k_rbgs_pre_restrict_row<TM><<<grid, 64 * (TM + 8), 0, s>>>(a.L, a.xout, a.b, a.c1, a.Lc, a.bc, a.P, a.tilesM,
+                                                                 a.st);
 }
 
 // The fused sweep and restriction of that shape: launched (or recorded), false for a shape it has no form for.
@@ -4733,6 +5028,37 @@ hipError_t launch_rbgs_restrict(hipStream_t s, const Level32 &Lf, const float *x
     const int grid = tilesM * std::max(1, (B.ns - 2 + P - 1) / P);
     const SweepRestrictArgs a{Lf, xin, xout, b, c1, Lc, bc, P, tilesM, st};
     if (!sweep_restrict_shape(s, TM, K, grid, a)) return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+// Output rows of the whole pre-smoothing march: four halo rows each side in 16 waves (even: rows are summed in pairs).
+constexpr int kPreRows = 8;
+
+bool rbgs_pre_restrict_level(const Level32 &Lf, const float *xin, const float *xout, const float *b, const float *bc)
+{
+    const char *env = std::getenv("NPPE_FUSE_PRE");
+    if (env && std::atoi(env) == 0) return false;
+    return Lf.B.nf - 2 <= 128 && rbgs_restrict_fusable(Lf, xin, xout, b, bc);
+}
+
+int64_t rbgs_pre_restrict_blocks(const Box &B, int planesPerChunk)
+{
+    int P = std::max(2, planesPerChunk);
+    P += P & 1;
+    return (int64_t)((B.nm - 2 + kPreRows - 1) / kPreRows) * ((B.ns - 2 + P - 1) / P);
+}
+
+hipError_t launch_rbgs_pre_restrict(hipStream_t s, const Level32 &Lf, float *xout, const float *b, int c1,
+                                    const Level32 &Lc, float *bc, int planesPerChunk, const SolveState *st)
+{
+    const Box &B = Lf.B;
+    if (!B.threeD || !Lf.uniform || B.nf - 2 > 128 || !bc || !xout || !b) return hipErrorInvalidValue;
+    int P = std::max(2, planesPerChunk);
+    P += P & 1;
+    const int tilesM = (B.nm - 2 + kPreRows - 1) / kPreRows;
+    const int grid = tilesM * ((B.ns - 2 + P - 1) / P);
+    const SweepRestrictArgs a{Lf, nullptr, xout, b, c1, Lc, bc, P, tilesM, st};
+    sweep_pre_restrict<kPreRows>(s, grid, a);
     return hipGetLastError();
 }
 

@@ -201,6 +201,21 @@ hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel, doubl
         const int nu1 = (coarseNu > 0 && l >= 2) ? coarseNu : h->nuPre;
         record_slot(SLOT_DOWN, l);
         DevLevel &C = h->levels[l + 1];
+        if (nu1 == 2 && rbgs_pre_restrict_level(L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l), C.b))
+        {
+            // rows of at most 128 cells: both sweeps from x = 0 and the restriction in one march (b and codes in;
+            // x and the coarse b out)
+            const double by = (2 * s + f) * N + s * C.cellsBox;
+            {
+                Scope sc(h, NPPE_K_SMOOTH_RESTRICT, by);
+                if ((e = launch_rbgs_pre_restrict(h->stream, L.v, xbuf(l, cur[l] ^ 1), bbuf(l), 0, C.v, C.b,
+                                                  L.pre2Planes, h->dState)) != hipSuccess)
+                    return e;
+            }
+            cur[l] ^= 1;
+            tr.add(by);
+            continue;
+        }
         // the last sweep (a plain one) and the restriction of its residual in one march where the level has it
         const bool fuse = nu1 >= 2 && rbgs_restrict_fusable(L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l), C.b);
         for (int sw = 0; sw < nu1; sw++)
@@ -503,6 +518,19 @@ hipError_t choose_tiles(nppe_solver *h)
             }
             const char *env2 = std::getenv("NPPE_POST2_P");
             if (env2) L.post2Planes = std::max(1, std::atoi(env2));
+            // the pre-smoothing and restriction in one march: the same model, P + 9 plane steps, P even
+            bestCost = 1e300;
+            for (int P = 2; P <= std::max(2, std::min(64, L.box.ns - 1)); P += 2)
+            {
+                const double cost = (double)((rbgs_pre_restrict_blocks(L.box, P) + cus - 1) / cus) * (P + 9);
+                if (cost < bestCost)
+                {
+                    bestCost = cost;
+                    L.pre2Planes = P;
+                }
+            }
+            const char *env3 = std::getenv("NPPE_PRE2_P");
+            if (env3) L.pre2Planes = std::max(1, std::atoi(env3));
         }
         // the sweep with the restriction folded in: the tallest tile, the chunk of the same block-round model
         // (even: the restriction sums plane pairs)

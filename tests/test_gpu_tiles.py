@@ -16,7 +16,7 @@ SHAPES = [("6", "1"), ("6", "5"), ("14", "3"), ("14", "16"), ("4", "2"), ("8", "
 def _solve(g, b, env):
     saved = {k: os.environ.get(k) for k in ("NPPE_RBGS_TM", "NPPE_RBGS_P", "NPPE_AUTOTUNE", "NPPE_RBGS_MARCH",
                                             "NPPE_FUSE_RESTRICT", "NPPE_FUSE_TM", "NPPE_FUSE_P", "NPPE_FUSE_POST",
-                                            "NPPE_POST2_P")}
+                                            "NPPE_POST2_P", "NPPE_FUSE_PRE", "NPPE_PRE2_P")}
     try:
         for k in saved:
             os.environ.pop(k, None)
@@ -95,11 +95,12 @@ def test_fused_restriction_equals_separate_passes(n, kind, non_uniform):
 
 @pytest.mark.parametrize("tm,p", [("6", "2"), ("8", "4"), ("10", "6"), ("12", "16"), ("12", "2"), ("6", "64")])
 def test_fused_restriction_tile_shapes(tm, p):
-    """Every tile shape of the fused sweep (tile rows, planes per chunk) gives the same bits."""
+    """Every tile shape of the fused sweep (tile rows, planes per chunk) gives the same bits (rows of 120 cells: the
+    whole pre-smoothing march turned off, so that every level runs the fused sweep)."""
     g, _ = make(3, (30, 26, 120), "mixed", False, seed=9)
     b = np.random.default_rng(10).standard_normal(g.unknowns)
     ref, rres = _solve(g, b, {"NPPE_FUSE_RESTRICT": "0"})
-    x, res = _solve(g, b, {"NPPE_FUSE_TM": tm, "NPPE_FUSE_P": p})
+    x, res = _solve(g, b, {"NPPE_FUSE_TM": tm, "NPPE_FUSE_P": p, "NPPE_FUSE_PRE": "0"})
     assert res.iterations == rres.iterations
     assert np.array_equal(x, ref), (tm, p)
 
@@ -126,9 +127,31 @@ def test_post_smoothing_pair_equals_two_sweeps(n, kind):
         assert np.array_equal(x, ref), p
 
 
+@pytest.mark.parametrize("n,kind", [
+    (40, "mixed"),                       # level 0 too (rows of 40 cells)
+    ((24, 20, 140), "pressure"),         # level 0 of 140-cell rows keeps the two launches; coarser levels march
+    ((33, 27, 61), "outlet"),            # odd sizes: families cut by an odd interior, Dirichlet codes
+    ((31, 45, 97), "mixed"),             # odd plane count, solids
+    ((100, 90, 24), "neumann"),          # many rows and planes: several row tiles and chunks
+])
+def test_pre_smoothing_and_restriction_equal_separate_passes(n, kind):
+    """Both pre-smoothing sweeps from x = 0 and the restriction of the residual run as one march on levels with rows
+    of at most 128 cells (b and codes read once, no x read); every chunk length gives the bits of the zero-guess
+    sweep followed by the sweep with the restriction folded in."""
+    g, _ = make(3, n, kind, False, seed=16)
+    b = np.random.default_rng(17).standard_normal(g.unknowns)
+    if kind == "neumann":
+        b -= b.mean()
+    ref, rres = _solve(g, b, {"NPPE_FUSE_PRE": "0"})
+    for p in (None, "2", "4", "6", "64"):
+        x, res = _solve(g, b, {} if p is None else {"NPPE_PRE2_P": p})
+        assert res.iterations == rres.iterations
+        assert np.array_equal(x, ref), p
+
+
 def test_post_smoothing_pair_batched():
-    """The batched form of the march (systems of different shapes in shared launches) gives each system the bits
-    of its own solve with the two separate post-smoothing launches."""
+    """The batched forms of the two marches (systems of different shapes in shared launches) give each system the
+    bits of its own solve with the separate pre- and post-smoothing launches."""
     import torch
     cases = [make(3, 40, "mixed", False, seed=13)[0], make(3, (24, 20, 140), "pressure", False, seed=14)[0],
              make(3, 28, "neumann", False, seed=15)[0]]
@@ -136,7 +159,7 @@ def test_post_smoothing_pair_batched():
     for i, g in enumerate(cases):
         b = np.random.default_rng(20 + i).standard_normal(g.unknowns)
         bs.append(b - b.mean() if i == 2 else b)
-    refs = [_solve(g, b, {"NPPE_FUSE_POST": "0"})[0] for g, b in zip(cases, bs)]
+    refs = [_solve(g, b, {"NPPE_FUSE_POST": "0", "NPPE_FUSE_PRE": "0"})[0] for g, b in zip(cases, bs)]
     solvers = [N.Solver(tolerance=1e-9) for _ in cases]
     try:
         for h, g in zip(solvers, cases):
