@@ -29,6 +29,7 @@ struct DevLevel
     uint8_t *types = nullptr;
     uint8_t *c8 = nullptr;
     uint16_t *c16 = nullptr;
+    uint32_t *rowc = nullptr;   // per-row code descriptors (launch_row_codes), ns * nm
     float *x[2] = {nullptr, nullptr};   // ping-pong pair of the smoother (level 0 uses the solver's z pair)
     float *b = nullptr;
     float *tab = nullptr;

@@ -1287,6 +1287,44 @@ struct Prolong
     int sM, ns, nf;
 };
 
+// ---- per-row code descriptors ---------------------------------------------------------------------------------
+//
+// A row whose interior cells 2 .. nf - 3 share one stencil code (most rows of a box: every row of an all-Neumann or
+// all-outlet box) has its codes built from its descriptor (launch_row_codes) instead of read: a wave holds the
+// descriptors of its row for the planes it marches over, lane j that of plane p0 + j (and p0 + 64 + j in the second
+// word), and reads one out per plane with a wave-uniform lane index. A row without one (descriptor bit 31 clear, or
+// no table) reads its codes as before.
+struct RowCodes
+{
+    uint32_t a = 0u, b = 0u;
+    int p0 = 0;
+    // planes p0 .. pEnd of row mc (clamped into the box as the sweeps clamp their plane index)
+    __device__ __forceinline__ void load(const Level32 &L, int mc, int p0_, int pEnd)
+    {
+        p0 = p0_;
+        if (!L.rowc) return;
+        const uint32_t *rr = L.rowc + (size_t)mc * (size_t)L.B.ns;
+        const int lane = threadIdx.x & 63;
+        a = rr[min(max(p0 + lane, 0), L.B.ns - 1)];
+        if (pEnd - p0 >= 64) b = rr[min(max(p0 + 64 + lane, 0), L.B.ns - 1)];
+    }
+    __device__ __forceinline__ uint32_t at(int pl) const
+    {
+        const int j = pl - p0;
+        return j < 64 ? (uint32_t)__builtin_amdgcn_readlane((int)a, j)
+                      : (j < 128 ? (uint32_t)__builtin_amdgcn_readlane((int)b, j - 64) : 0u);
+    }
+};
+
+// The pair (f, f + 1) codes (low byte: f) of a row with descriptor d: the first, middle or last code by column.
+__device__ __forceinline__ unsigned row_pair_code(uint32_t d, bool first, bool loLast, bool hiLast)
+{
+    const unsigned cf = d & 255u, cmid = (d >> 8) & 255u, cl = (d >> 16) & 255u;
+    const unsigned lo = first ? cf : (loLast ? cl : cmid);
+    const unsigned hi = hiLast ? cl : cmid;
+    return lo | (hi << 8);
+}
+
 // ---- red-black Gauss-Seidel sweep, whole-row waves (3D uniform levels) -----------------------------------------
 //
 // As k_rbgs_march, but a wave spans the whole interior row: lane l owns the pairs (1 + 2p, 2 + 2p) for
@@ -1449,6 +1487,23 @@ __device__ __forceinline__ void rbgs_row_body(const Level32 &L, const float *__r
             OC[j & 1][k] = 0.f;
         }
     const int sp0 = s0 - 1;
+    RowCodes rc;
+    rc.load(L, mc, sp0 - 1, s1 + 3);
+    // the codes of plane pl into slot c: built from the row's descriptor where it has one, else read
+    auto loadCodes = [&](int c, int pl) {
+        const uint32_t d = rc.at(pl);
+        if (d >> 31)
+        {
+#pragma unroll
+            for (int k = 0; k < K; k++) C[c][k] = row_pair_code(d, fa[k] == 1, fa[k] == B.nf - 2, fa[k] + 1 == B.nf - 2);
+        }
+        else
+        {
+            const uint8_t *pc = planeC(pl);
+#pragma unroll
+            for (int k = 0; k < K; k++) C[c][k] = pairC(pc, off[k]);
+        }
+    };
     // slots: plane sp0 + d -> d & 3 (d & 1 for O)
     if (!ZERO)
     {
@@ -1482,13 +1537,10 @@ __device__ __forceinline__ void rbgs_row_body(const Level32 &L, const float *__r
     for (int d = PRO ? -1 : 0; d <= 1; d++)
     {
         const float *pb = plane(b, sp0 + d);
-        const uint8_t *pc = planeC(sp0 + d);
 #pragma unroll
         for (int k = 0; k < K; k++)
-        {
             if (d >= 0) Bv[d][k] = pair(pb, off[k]);
-            C[(d + 4) & 3][k] = pairC(pc, off[k]);
-        }
+        loadCodes((d + 4) & 3, sp0 + d);
     }
     if (PRO)
     {
@@ -1513,14 +1565,13 @@ __device__ __forceinline__ void rbgs_row_body(const Level32 &L, const float *__r
         {
             const float *px = plane(xin, sp + 2);
             const float *pb = plane(b, sp + 2);
-            const uint8_t *pc = planeC(sp + 2);
 #pragma unroll
             for (int k = 0; k < K; k++)
             {
                 if (!ZERO) X[S2][k] = pair(px, off[k]);
                 Bv[S2][k] = pair(pb, off[k]);
-                C[S2][k] = pairC(pc, off[k]);
             }
+            loadCodes(S2, sp + 2);
             if (!ZERO && haloRow)
             {
                 const float *po = plane(xin, sp + 1);

@@ -24,6 +24,13 @@ hipError_t launch_coarsen_types(hipStream_t s, const Box &Bf, const uint8_t *tf,
 // the cell is an unknown. Built at commit and after every layout update.
 hipError_t launch_stencil_codes(hipStream_t s, const Box &B, const uint8_t *t, uint8_t *c8, uint16_t *c16);
 
+// Per row (s, m) of the box, ring rows included, at rowc[m * ns + s]: bits 0-7 the code8 of the row's first
+// interior cell (f = 1), 8-15 that of its cells f = 2 .. nf - 3 when they all have one code (else the code of f = 2,
+// or 0 for rows of fewer than 3 cells), 16-23 that of its last interior cell (f = nf - 2), bit 31 set when cells
+// 2 .. nf - 3 do share one code. A sweep then builds such a row's codes instead of reading them. Built after the
+// stencil codes, at commit and after every layout update.
+hipError_t launch_row_codes(hipStream_t s, const Box &B, const uint8_t *c8, uint32_t *rowc);
+
 // counts[0] += unknowns of the level (code8 bit 6), counts[1] += unknowns with a Dirichlet face (bit 7), with
 // partial sums per block in partials (at most 2 * kMaxCountBlocks entries) finished by launch_finish_counts.
 constexpr int kMaxCountBlocks = 1024;

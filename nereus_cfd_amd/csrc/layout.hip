@@ -86,6 +86,24 @@ __global__ __launch_bounds__(256) void k_stencil_codes(Box B, const uint8_t *__r
     c16[c] = (uint16_t)code16;
 }
 
+// One wave per row (s, m) of the box, ring rows included: the row's code descriptor (see launch_row_codes).
+__global__ __launch_bounds__(256) void k_row_codes(Box B, const uint8_t *__restrict__ c8, uint32_t *__restrict__ rowc)
+{
+    const int64_t row = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    const int rows = B.ns * B.nm;
+    if (row >= rows) return;
+    const int s = (int)(row / B.nm), m = (int)(row % B.nm);
+    const uint8_t *r = c8 + (size_t)B.at(s, m, 0);
+    const int last = B.nf - 2;
+    const unsigned mid = last >= 3 ? r[2] : 0u;
+    bool same = true;
+    for (int f = 2 + lane; f <= last - 1; f += 64) same = same && r[f] == mid;
+    same = __all(same);
+    if (lane == 0)
+        rowc[(size_t)m * B.ns + s] = (same ? 0x80000000u : 0u) | (unsigned)r[1] | (mid << 8) | ((unsigned)r[last] << 16);
+}
+
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
 {
 #pragma unroll
@@ -247,6 +265,12 @@ hipError_t launch_stencil_codes(hipStream_t s, const Box &B, const uint8_t *t, u
 {
     const int64_t interior = (int64_t)(B.threeD ? B.ns - 2 : 1) * (B.nm - 2) * (B.nf - 2);
     k_stencil_codes<<<grid_for(interior, 256), 256, 0, s>>>(B, t, c8, c16);
+    return hipGetLastError();
+}
+
+hipError_t launch_row_codes(hipStream_t s, const Box &B, const uint8_t *c8, uint32_t *rowc)
+{
+    k_row_codes<<<grid_for((int64_t)B.ns * B.nm * 64, 256), 256, 0, s>>>(B, c8, rowc);
     return hipGetLastError();
 }
 

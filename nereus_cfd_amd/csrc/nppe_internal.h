@@ -50,6 +50,7 @@ struct Level32
     Tables32 T;
     const uint8_t *c8;
     const uint16_t *c16;
+    const uint32_t *rowc;   // per-row code descriptors (launch_row_codes), or null
     float cu, cr, icu;   // icu = 1 / cu
     int uniform;
 };

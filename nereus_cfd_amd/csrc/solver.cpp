@@ -754,6 +754,7 @@ int upload_level_layout(nppe_solver *h, DevLevel &L)
             std::memcpy(&pitched[(size_t)L.box.at(s, m, 0)], &H.types[((size_t)s * H.nm + m) * H.nf], H.nf);
     HIP_TRY(h, upload(h, L.types, pitched.data(), pitched.size()), "upload types");
     HIP_TRY(h, launch_stencil_codes(h->stream, L.box, L.types, L.c8, L.c16), "stencil codes");
+    HIP_TRY(h, launch_row_codes(h->stream, L.box, L.c8, L.rowc), "row codes");
     HIP_TRY(h, hipStreamSynchronize(h->stream), "stencil codes");
     return NPPE_OK;
 }
@@ -831,6 +832,7 @@ int build(nppe_solver *h, const nppe_grid *g)
         HIP_TRY(h, dalloc(h, &L.types, (size_t)L.box.cells), "allocate types");
         HIP_TRY(h, dalloc_pairs(h, &L.c8, (size_t)L.box.cells), "allocate codes");
         HIP_TRY(h, dalloc(h, &L.c16, (size_t)L.box.cells), "allocate codes");
+        HIP_TRY(h, dalloc(h, &L.rowc, (size_t)L.box.ns * L.box.nm), "allocate row codes");
         if (int rc = upload_level_layout(h, L)) return rc;
         int off[9];
         std::vector<float> tab = tables32(H, sigma, off);
@@ -842,6 +844,11 @@ int build(nppe_solver *h, const nppe_grid *g)
         L.v.T = L.t32;
         L.v.c8 = L.c8;
         L.v.c16 = L.c16;
+        {
+            // NPPE_ROW_CODES=0: every sweep reads its codes (the tests compare the two)
+            const char *env = std::getenv("NPPE_ROW_CODES");
+            L.v.rowc = env && std::atoi(env) == 0 ? nullptr : L.rowc;
+        }
         L.v.uniform = H.uniform ? 1 : 0;
         {
             // constant coefficient of a uniform level: sigma * area / gap = sigma * w (3D) or sigma (2D)
@@ -1069,6 +1076,7 @@ int layout_from_device(nppe_solver *h, const uint8_t *dTypes, const uint8_t *hos
     {
         DevLevel &D = h->levels[l];
         HIP_TRY(h, launch_stencil_codes(h->stream, D.box, D.types, D.c8, D.c16), "layout: codes");
+        HIP_TRY(h, launch_row_codes(h->stream, D.box, D.c8, D.rowc), "layout: row codes");
         int nb = 0;
         HIP_TRY(h, launch_count_codes(h->stream, D.box, D.c8, parts, &nb), "layout: counts");
         HIP_TRY(h, launch_finish_counts(h->stream, parts, nb, h->dLayoutInfo + 2 * l), "layout: counts");

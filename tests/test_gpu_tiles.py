@@ -16,7 +16,7 @@ SHAPES = [("6", "1"), ("6", "5"), ("14", "3"), ("14", "16"), ("4", "2"), ("8", "
 def _solve(g, b, env):
     saved = {k: os.environ.get(k) for k in ("NPPE_RBGS_TM", "NPPE_RBGS_P", "NPPE_AUTOTUNE", "NPPE_RBGS_MARCH",
                                             "NPPE_FUSE_RESTRICT", "NPPE_FUSE_TM", "NPPE_FUSE_P", "NPPE_FUSE_POST",
-                                            "NPPE_POST2_P", "NPPE_FUSE_PRE", "NPPE_PRE2_P")}
+                                            "NPPE_POST2_P", "NPPE_FUSE_PRE", "NPPE_PRE2_P", "NPPE_ROW_CODES")}
     try:
         for k in saved:
             os.environ.pop(k, None)
@@ -175,3 +175,23 @@ def test_post_smoothing_pair_batched():
     finally:
         for h in solvers:
             h.close()
+
+
+@pytest.mark.parametrize("n,kind,non_uniform", [
+    (40, "mixed", False),                # rows with interior solids and air cells (codes read) beside uniform rows
+    ((24, 20, 140), "pressure", False),  # two pairs per lane; Dirichlet codes at every row end
+    ((20, 16, 300), "outlet", False),    # four pairs per lane
+    ((33, 27, 61), "neumann", False),    # odd row length: the last pair half outside the row
+    (36, "outlet", True),                # stretched levels
+])
+def test_row_code_descriptors_equal_read_codes(n, kind, non_uniform):
+    """Rows whose interior cells share one code have their codes built from a per-row descriptor instead of read
+    (NPPE_ROW_CODES=0 reads them everywhere): the same codes, so the same bits."""
+    g, _ = make(3, n, kind, non_uniform, seed=18)
+    b = np.random.default_rng(19).standard_normal(g.unknowns)
+    if kind == "neumann":
+        b -= b.mean()
+    ref, rres = _solve(g, b, {"NPPE_ROW_CODES": "0"})
+    x, res = _solve(g, b, {})
+    assert res.iterations == rres.iterations
+    assert np.array_equal(x, ref)
