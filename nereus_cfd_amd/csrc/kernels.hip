@@ -1298,15 +1298,19 @@ struct RowCodes
 {
     uint32_t a = 0u, b = 0u;
     int p0 = 0;
-    // planes p0 .. pEnd of row mc (clamped into the box as the sweeps clamp their plane index)
-    __device__ __forceinline__ void load(const Level32 &L, int mc, int p0_, int pEnd)
+    // planes p0 .. pEnd of row mc of a box of ns planes (clamped into the box as the sweeps clamp their plane index)
+    __device__ __forceinline__ void load(const uint32_t *rowc, int ns, int mc, int p0_, int pEnd)
     {
         p0 = p0_;
-        if (!L.rowc) return;
-        const uint32_t *rr = L.rowc + (size_t)mc * (size_t)L.B.ns;
+        if (!rowc) return;
+        const uint32_t *rr = rowc + (size_t)mc * (size_t)ns;
         const int lane = threadIdx.x & 63;
-        a = rr[min(max(p0 + lane, 0), L.B.ns - 1)];
-        if (pEnd - p0 >= 64) b = rr[min(max(p0 + 64 + lane, 0), L.B.ns - 1)];
+        a = rr[min(max(p0 + lane, 0), ns - 1)];
+        if (pEnd - p0 >= 64) b = rr[min(max(p0 + 64 + lane, 0), ns - 1)];
+    }
+    __device__ __forceinline__ void load(const Level32 &L, int mc, int p0_, int pEnd)
+    {
+        load(L.rowc, L.B.ns, mc, p0_, pEnd);
     }
     __device__ __forceinline__ uint32_t at(int pl) const
     {
@@ -1323,6 +1327,29 @@ __device__ __forceinline__ unsigned row_pair_code(uint32_t d, bool first, bool l
     const unsigned lo = first ? cf : (loLast ? cl : cmid);
     const unsigned hi = hiLast ? cl : cmid;
     return lo | (hi << 8);
+}
+
+// The codes of plane pl's pairs (first cells 1 + 2 (lane + 64 k), read at c8pl + off[k] for a row without a
+// descriptor) into c: built where the row has a descriptor, else read.
+template <int K>
+__device__ __forceinline__ void row_codes_into(unsigned (&c)[K], const RowCodes &rc, int pl, const uint8_t *c8pl,
+                                               const uint32_t (&off)[K], int lane, int nf)
+{
+    const uint32_t d = rc.at(pl);
+    if (d >> 31)
+    {
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            const int fa = 1 + 2 * (lane + 64 * k);
+            c[k] = row_pair_code(d, fa == 1, fa == nf - 2, fa + 1 == nf - 2);
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int k = 0; k < K; k++) c[k] = (unsigned)*reinterpret_cast<const uint16_t *>(c8pl + off[k]);
+    }
 }
 
 // ---- red-black Gauss-Seidel sweep, whole-row waves (3D uniform levels) -----------------------------------------
@@ -2741,7 +2768,7 @@ __device__ __forceinline__ double apply64u_z(const Tables64 &T, int64_t idx, dou
 template <int TM, int K>
 struct RowMarch
 {
-    int lane, wave, m, s0, s1;
+    int lane, wave, m, mc, s0, s1;
     bool halo, rowIn;
     uint32_t off[K];
     bool okA[K], okB[K];
@@ -2760,6 +2787,7 @@ __device__ __forceinline__ RowMarch<TM, K> row_march(const Box &B, int P, int ti
     R.s0 = 1 + chunk * P;
     R.s1 = min(R.s0 + P - 1, B.ns - 2);
     const int mc = min(max(R.m, 0), B.nm - 1);
+    R.mc = mc;
     const int fcMax = ((B.nf - 2) & 1) ? B.nf - 2 : B.nf - 3;
 #pragma unroll
     for (int k = 0; k < K; k++)
@@ -2805,16 +2833,18 @@ __device__ __forceinline__ void cg_dir64_row_body(const Box &B, const Tables64 &
             C[j][k] = 0u;
         }
     const int sp0 = R.s0 - 1;
+    RowCodes rc;
+    rc.load(T.rowc, B.ns, R.mc, sp0, R.s1 + 3);
 #pragma unroll
     for (int d = 0; d <= 2; d++)
     {
         const size_t pl = plane(sp0 + d);
+        row_codes_into<K>(C[d], rc, sp0 + d, T.c8 + pl, R.off, R.lane, B.nf);
 #pragma unroll
         for (int k = 0; k < K; k++)
         {
             Zv[d][k] = pair(z + pl, R.off[k]);
             if (!first) Po[d][k] = pair(po + pl, R.off[k]);
-            C[d][k] = pairC(T.c8 + pl, R.off[k]);
         }
     }
 #pragma unroll
@@ -2829,12 +2859,12 @@ __device__ __forceinline__ void cg_dir64_row_body(const Box &B, const Tables64 &
         constexpr int Sm1 = (U + 3) & 3, S0 = U, S1 = (U + 1) & 3, S2 = (U + 2) & 3;
         {
             const size_t pl = plane(sp + 2);
+            row_codes_into<K>(C[S2], rc, sp + 2, T.c8 + pl, R.off, R.lane, B.nf);
 #pragma unroll
             for (int k = 0; k < K; k++)
             {
                 Zv[S2][k] = pair(z + pl, R.off[k]);
                 if (!first) Po[S2][k] = pair(po + pl, R.off[k]);
-                C[S2][k] = pairC(T.c8 + pl, R.off[k]);
             }
         }
 #pragma unroll
@@ -2928,15 +2958,17 @@ __device__ __forceinline__ void cg_update64_row_body(const Box &B, const Tables6
             Rv[j][k] = make_double2(0.0, 0.0);
         }
     const int sp0 = R.s0 - 1;
+    RowCodes rc;
+    rc.load(T.rowc, B.ns, R.mc, sp0, R.s1 + 3);
 #pragma unroll
     for (int d = 0; d <= 2; d++)
     {
         const size_t pl = plane(sp0 + d);
+        row_codes_into<K>(C[d], rc, sp0 + d, T.c8 + pl, R.off, R.lane, B.nf);
 #pragma unroll
         for (int k = 0; k < K; k++)
         {
             Pv[d][k] = pair(p + pl, R.off[k]);
-            C[d][k] = pairC(T.c8 + pl, R.off[k]);
         }
     }
     if (!R.halo)
@@ -2957,11 +2989,11 @@ __device__ __forceinline__ void cg_update64_row_body(const Box &B, const Tables6
         constexpr int Sm1 = (U + 3) & 3, S0 = U, S1 = (U + 1) & 3, S2 = (U + 2) & 3;
         {
             const size_t pl = plane(sp + 2);
+            row_codes_into<K>(C[S2], rc, sp + 2, T.c8 + pl, R.off, R.lane, B.nf);
 #pragma unroll
             for (int k = 0; k < K; k++)
             {
                 Pv[S2][k] = pair(p + pl, R.off[k]);
-                C[S2][k] = pairC(T.c8 + pl, R.off[k]);
             }
         }
         if (!R.halo && sp + 2 <= R.s1)
@@ -3317,6 +3349,8 @@ __device__ __forceinline__ void rbgs_restrict_row_body(const Level32 &L, const f
 #pragma unroll
     for (int k = 0; k < K; k++) acc[k] = 0.f;
     const int sp0 = s0 - 2;
+    RowCodes rc;
+    rc.load(L, mc, sp0, s1 + 4);
 #pragma unroll
     for (int d = -1; d <= 1; d++)
     {
@@ -3334,13 +3368,9 @@ __device__ __forceinline__ void rbgs_restrict_row_body(const Level32 &L, const f
     for (int d = 0; d <= 1; d++)
     {
         const float *pb = plane(b, sp0 + d);
-        const uint8_t *pc = planeC(sp0 + d);
 #pragma unroll
-        for (int k = 0; k < K; k++)
-        {
-            Bv[d][k] = pair(pb, off[k]);
-            C[d][k] = pairC(pc, off[k]);
-        }
+        for (int k = 0; k < K; k++) Bv[d][k] = pair(pb, off[k]);
+        row_codes_into<K>(C[d], rc, sp0 + d, planeC(sp0 + d), off, lane, B.nf);
     }
     __syncthreads();
     const int wUp = min(wave + 1, W - 1), wDn = max(wave - 1, 0);
@@ -3506,13 +3536,9 @@ __device__ __forceinline__ void rbgs_restrict_row_body(const Level32 &L, const f
         }
         {
             const float *pb = plane(b, sp + 2);
-            const uint8_t *pc = planeC(sp + 2);
 #pragma unroll
-            for (int k = 0; k < K; k++)
-            {
-                Bv[S2][k] = pair(pb, off[k]);
-                C[S2][k] = pairC(pc, off[k]);
-            }
+            for (int k = 0; k < K; k++) Bv[S2][k] = pair(pb, off[k]);
+            row_codes_into<K>(C[S2], rc, sp + 2, planeC(sp + 2), off, lane, B.nf);
         }
     };
     // the first-colour element at plane sp is (c1 - sp - m - 1) & 1

@@ -64,6 +64,7 @@ struct Tables64
     int unitCoef;   // 3D uniform: every face coefficient is exactly 1.0 (PressureSolver3D.cpp:49-53)
     const uint8_t *c8;   // the fine level's stencil codes (a cell without a Dirichlet neighbour takes a fast path)
     const uint16_t *c16; // and their 2-bit-per-neighbour form (unit-coefficient rows beside a Dirichlet cell)
+    const uint32_t *rowc; // its per-row code descriptors (launch_row_codes), or null
 };
 
 // Device-resident scalars of one solve. Written by the last block of each reduction.

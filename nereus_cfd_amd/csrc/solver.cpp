@@ -897,7 +897,7 @@ int build(nppe_solver *h, const nppe_grid *g)
         HIP_TRY(h, upload(h, h->dTab64, tab.data(), tab.size() * sizeof(double)),
                 "upload FP64 tables");
         h->t64 = Tables64{h->dTab64 + off[0], h->dTab64 + off[1], h->dTab64 + off[2], h->dTab64 + off[3],
-                          h->dTab64 + off[4], h->dTab64 + off[5], scale, unitCoef ? 1 : 0, F.c8, F.c16};
+                          h->dTab64 + off[4], h->dTab64 + off[5], scale, unitCoef ? 1 : 0, F.c8, F.c16, F.v.rowc};
     }
     // every vector in IDP order is sized for the most unknowns the box can hold, so a layout update reuses them
     h->unknownCap = std::max<int64_t>(1, interior_cells(F.host));
