@@ -160,7 +160,13 @@ def gpu_leg(args):
                        "bytes_per_launch": round(dom[1]["bytes"] / dom[1]["launches"]),
                        "timing": "HIP start/stop events recorded by each sampled launch's own dispatch "
                                  "(hipExtLaunchKernel) on the solver's stream",
-                       "rocprof_avg_launch_us": profiled_launch_us() if n == 256 else None}
+                       "rocprof_avg_launch_us": profiled_launch_us() if n == 256 else None,
+                       # bytes_per_launch follows the sweep formula (3s+f)N, one code byte per box cell included;
+                       # every row of this box has a code descriptor, so the sweeps do not read those N bytes
+                       "bytes_moved_per_launch": round(dom[1]["bytes"] / dom[1]["launches"] - (n + 2) ** 3),
+                       "frac_bytes_moved": round((dom[1]["bytes"] / dom[1]["launches"] - (n + 2) ** 3)
+                                                 / (dom[1]["bytes"] / dom[1]["launches"]) * achieved
+                                                 / HBM_PEAK_GBS, 4)}
     s.close()
 
     if not args.no_cg64:
