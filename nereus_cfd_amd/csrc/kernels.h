@@ -19,7 +19,16 @@ namespace nppe
 // (launch_batched), whose block g - start[q] of system q does what that block does in the system's own launch.
 // Kinds of records: a sweep or a restriction gives the same bits for any tiling, so a batch may re-tile it
 // (retile_record); the other kernels' reductions follow their grid, which stays as recorded.
-enum { REC_FIXED = 0, REC_ROW_SWEEP = 1, REC_RESTRICT_ROW = 2, REC_RESTRICT_MARCH = 3, REC_SWEEP_RESTRICT = 4 };
+enum
+{
+    REC_FIXED = 0,
+    REC_ROW_SWEEP = 1,
+    REC_RESTRICT_ROW = 2,
+    REC_RESTRICT_MARCH = 3,
+    REC_SWEEP_RESTRICT = 4,
+    REC_POST2 = 5,   // the post-smoothing pair in one march (its chunk length only)
+    REC_PRE2 = 6     // the pre-smoothing and restriction in one march (its chunk length only, even)
+};
 struct LaunchRecord
 {
     const void *fn = nullptr;   // the batched kernel: kernel(const Args *args, const int *start, int nsys)
@@ -106,14 +115,14 @@ hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float 
                        const Level32 *prolongFrom = nullptr, const float *prolongXc = nullptr);
 hipError_t launch_rz_rows_finalize(hipStream_t s, const Box &B, SolveState *st, double *partials);
 // Both post-smoothing sweeps of a level in one march: x_out = S(S(x_in + P x_c)) (x_c on level Lc; colour c1 first
-// in both), the second sweep three planes behind the first; with rzPartials the second also writes <b, x_out> per
-// row as launch_rbgs does. x_out (another buffer than x_in) is bit for bit that of the two launch_rbgs calls.
+// in both), the second sweep three planes behind the first. x_out (another buffer than x_in) is bit for bit that of
+// the two launch_rbgs calls.
 // rbgs2_level says whether a level has it (3D, uniform, rows of at most 128 cells, the whole-row family;
 // NPPE_FUSE_POST=0 turns it off: the tests compare the two).
 bool rbgs2_level(const Level32 &L, const float *xin, const float *xout, const float *b);
 int64_t rbgs2_blocks(const Box &B, int planesPerChunk);
 hipError_t launch_rbgs2(hipStream_t s, const Level32 &L, const float *xin, float *xout, const float *b, int c1,
-                        int planesPerChunk, SolveState *st, double *rzPartials, const Level32 &Lc, const float *xc);
+                        int planesPerChunk, SolveState *st, const Level32 &Lc, const float *xc);
 // Which sweep kernel launch_rbgs runs for these operands, and the tile heights it accepts for it (every tile shape
 // gives bit-identical results: a colour's updates are independent of each other).
 enum { RBGS_ROW = 0, RBGS_MARCH = 1, RBGS_2D = 2 };

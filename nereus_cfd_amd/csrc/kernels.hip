@@ -1515,10 +1515,11 @@ __device__ __forceinline__ void rbgs_row_body(const Level32 &L, const float *__r
         }
     const int sp0 = s0 - 1;
     RowCodes rc;
-    rc.load(L, mc, sp0 - 1, s1 + 3);
-    // the codes of plane pl into slot c: built from the row's descriptor where it has one, else read
+    if (!NU) rc.load(L, mc, sp0 - 1, s1 + 3);
+    // the codes of plane pl into slot c: built from the row's descriptor where it has one, else read (stretched
+    // levels always read them: their width tables already fill the registers)
     auto loadCodes = [&](int c, int pl) {
-        const uint32_t d = rc.at(pl);
+        const uint32_t d = NU ? 0u : rc.at(pl);
         if (d >> 31)
         {
 #pragma unroll
@@ -1781,12 +1782,12 @@ __device__ __forceinline__ void rbgs_row_body(const Level32 &L, const float *__r
 // (waves 0 and NW - 1 load their outer row, as the single sweep's halo waves do): A1 runs on every wave, A2 on all
 // but the outermost, B1 on all but the two outermost, B2 on the TM output rows. Each stage's values pass to the next
 // through LDS, double-buffered by plane parity, with the one barrier of the step; each stage's update is the single
-// sweep's, from the same values in the same order, so x (and <b, x> per row) is bit for bit that of the two launches.
+// sweep's, from the same values in the same order, so x is bit for bit that of the two launches.
 // b and codes stay in registers from their load (two planes ahead) to their last use by B2 (slots by plane mod 8).
 template <int TM, int K>
 __device__ __forceinline__ void rbgs2_row_body(const Level32 &L, const float *__restrict__ xin, float *__restrict__ xout,
                                                const float *__restrict__ b, int c1, int P, int tilesM,
-                                               const SolveState *st, double *__restrict__ dot, const Prolong &pro,
+                                               const SolveState *st, const Prolong &pro,
                                                int lb)
 {
     if (st->done) return;
@@ -2057,7 +2058,6 @@ __device__ __forceinline__ void rbgs2_row_body(const Level32 &L, const float *__
         {
             float *po = xout + (size_t)q * (size_t)B.sS;
             const float(*rq)[kCols] = b1s[q & 1];
-            double bx = 0.0;
 #pragma unroll
             for (int k = 0; k < K; k++)
             {
@@ -2068,20 +2068,10 @@ __device__ __forceinline__ void rbgs2_row_body(const Level32 &L, const float *__
                                      E ? rq[wave][f + 1] : RB[Rm3][k], E ? RB[Rm3][k] : rq[wave][f - 1]};
                 const float out = update(nb, E ? Bv[Bm3][k].y : Bv[Bm3][k].x, code, q, off[k] + (uint32_t)E);
                 const float va = E ? RB[Rm3][k] : out, vb = E ? out : RB[Rm3][k];
-                if (dot)
-                {
-                    bx += (cc & 64u) ? (double)Bv[Bm3][k].x * (double)va : 0.0;
-                    bx += (cc & (64u << 8)) ? (double)Bv[Bm3][k].y * (double)vb : 0.0;
-                }
                 if (okB[k])
                     *reinterpret_cast<float2 *>(po + off[k]) = make_float2(va, vb);
                 else if (okA[k])
                     po[off[k]] = va;
-            }
-            if (dot)
-            {
-                const double rowDot = wave_sum(bx);
-                if (lane == 0 && rowIn) dot[(size_t)(q - 1) * (size_t)(B.nm - 2) + (size_t)(m - 1)] = rowDot;
             }
         }
     };
@@ -4210,9 +4200,9 @@ template <int TM, int K>
 __global__ __launch_bounds__(64 * (TM + 6)) void k_rbgs2_row(Level32 L, const float *__restrict__ xin,
                                                              float *__restrict__ xout, const float *__restrict__ b,
                                                              int c1, int P, int tilesM, const SolveState *st,
-                                                             double *__restrict__ dot, Prolong pro)
+                                                             Prolong pro)
 {
-    rbgs2_row_body<TM, K>(L, xin, xout, b, c1, P, tilesM, st, dot, pro, logical_block());
+    rbgs2_row_body<TM, K>(L, xin, xout, b, c1, P, tilesM, st, pro, logical_block());
 }
 
 template <int TM, int K>
@@ -4221,7 +4211,7 @@ __global__ __launch_bounds__(64 * (TM + 6)) void kb_rbgs2_row(const RowSweepArgs
 {
     const BatchBlock bb = batch_block(start, nsys);
     const RowSweepArgs a = args[bb.q];
-    rbgs2_row_body<TM, K>(a.L, a.xin, a.xout, a.b, a.c1, a.P, a.tilesM, a.st, a.dot, a.pro, bb.lb);
+    rbgs2_row_body<TM, K>(a.L, a.xin, a.xout, a.b, a.c1, a.P, a.tilesM, a.st, a.pro, bb.lb);
 }
 
 template <int TM, int K>
@@ -4508,9 +4498,8 @@ void row_sweep(hipStream_t s, int grid, int threads, const RowSweepArgs &a)
 template <int TM, int K>
 void row_sweep2(hipStream_t s, int grid, const RowSweepArgs &a)
 {
-    if (record(kb_rbgs2_row<TM, K>, grid, 64 * (TM + 6), a)) return;
-    k_rbgs2_row<TM, K><<<grid, 64 * (TM + 6), 0, s>>>(a.L, a.xin, a.xout, a.b, a.c1, a.P, a.tilesM, a.st, a.dot,
-                                                       a.pro);
+    if (record(kb_rbgs2_row<TM, K>, grid, 64 * (TM + 6), a, REC_POST2, TM, K)) return;
+    k_rbgs2_row<TM, K><<<grid, 64 * (TM + 6), 0, s>>>(a.L, a.xin, a.xout, a.b, a.c1, a.P, a.tilesM, a.st, a.pro);
 }
 
 template <int TM, int K>
@@ -4524,7 +4513,7 @@ void sweep_restrict(hipStream_t s, int grid, const SweepRestrictArgs &a)
 template <int TM>
 void sweep_pre_restrict(hipStream_t s, int grid, const SweepRestrictArgs &a)
 {
-    if (record(kb_rbgs_pre_restrict_row<TM>, grid, 64 * (TM + 8), a)) return;
+    if (record(kb_rbgs_pre_restrict_row<TM>, grid, 64 * (TM + 8), a, REC_PRE2, TM, 1)) return;
     k_rbgs_pre_restrict_row<TM><<<grid, 64 * (TM + 8), 0, s>>>(a.L, a.xout, a.b, a.c1, a.Lc, a.bc, a.P, a.tilesM,
                                                                  a.st);
 }
@@ -4674,6 +4663,24 @@ bool retile_record(LaunchRecord &r, int tm, int p)
         r.threads = 64 * (TM + 4);
         r.fn = fn;
         r.tm = TM;
+        std::memcpy(r.args.data(), &a, sizeof a);
+        return true;
+    }
+    case REC_POST2:
+    {
+        RowSweepArgs a;
+        std::memcpy(&a, r.args.data(), sizeof a);
+        a.P = P;
+        r.grid = a.tilesM * std::max(1, (a.L.B.ns - 2 + P - 1) / P);
+        std::memcpy(r.args.data(), &a, sizeof a);
+        return true;
+    }
+    case REC_PRE2:
+    {
+        SweepRestrictArgs a;
+        std::memcpy(&a, r.args.data(), sizeof a);
+        a.P = Pe;
+        r.grid = a.tilesM * std::max(1, (a.L.B.ns - 2 + Pe - 1) / Pe);
         std::memcpy(r.args.data(), &a, sizeof a);
         return true;
     }
@@ -4900,7 +4907,7 @@ int64_t rbgs2_blocks(const Box &B, int planesPerChunk)
 }
 
 hipError_t launch_rbgs2(hipStream_t s, const Level32 &L, const float *xin, float *xout, const float *b, int c1,
-                        int planesPerChunk, SolveState *st, double *rzPartials, const Level32 &Lc, const float *xc)
+                        int planesPerChunk, SolveState *st, const Level32 &Lc, const float *xc)
 {
     const Box &B = L.B;
     if (!rbgs2_level(L, xin, xout, b) || xin == xout || !xc) return hipErrorInvalidValue;
@@ -4908,7 +4915,7 @@ hipError_t launch_rbgs2(hipStream_t s, const Level32 &L, const float *xin, float
     const int tilesM = (B.nm - 2 + kRbgs2Rows - 1) / kRbgs2Rows;
     const int grid = tilesM * ((B.ns - 2 + P - 1) / P);
     const Prolong pro{xc, Lc.B.sS, (int)Lc.B.sM, Lc.B.ns, Lc.B.nf};
-    const RowSweepArgs ra{L, xin, xout, b, c1, P, tilesM, st, rzPartials, pro};
+    const RowSweepArgs ra{L, xin, xout, b, c1, P, tilesM, st, nullptr, pro};
     row_sweep2<kRbgs2Rows, 1>(s, grid, ra);
     return hipGetLastError();
 }

@@ -201,10 +201,12 @@ hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel, doubl
         const int nu1 = (coarseNu > 0 && l >= 2) ? coarseNu : h->nuPre;
         record_slot(SLOT_DOWN, l);
         DevLevel &C = h->levels[l + 1];
-        if (nu1 == 2 && rbgs_pre_restrict_level(L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l), C.b))
+        if (l >= 1 && nu1 == 2 && rbgs_pre_restrict_level(L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l), C.b))
         {
-            // rows of at most 128 cells: both sweeps from x = 0 and the restriction in one march (b and codes in;
-            // x and the coarse b out)
+            // coarse levels with rows of at most 128 cells: both sweeps from x = 0 and the restriction in one march
+            // (b and codes in; x and the coarse b out). Not on the finest level: where its launch fills the GPU (a
+            // large box, or many systems in one batch) the march's extra halo rows cost more than the launch it
+            // saves
             const double by = (2 * s + f) * N + s * C.cellsBox;
             {
                 Scope sc(h, NPPE_K_SMOOTH_RESTRICT, by);
@@ -293,24 +295,15 @@ hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel, doubl
         const double pb = s * C.cellsBox + 2 * s * N + f * N;
         const int nu2 = (coarseNu > 0 && l >= 2) ? coarseNu : h->nuPost;
         record_slot(SLOT_UP, l);
-        if (nu2 == 2 && rbgs2_level(L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l)))
+        if (l >= 1 && nu2 == 2 && rbgs2_level(L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l)))
         {
-            // rows of at most 128 cells: the correction and both sweeps in one march
-            const bool dot = l == 0 && rzPartials;
+            // coarse levels with rows of at most 128 cells: the correction and both sweeps in one march (not on the
+            // finest level, as above)
             const double by = (3 * s + f) * N + s * C.cellsBox;
-            {
-                Scope sc(h, l == 0 ? NPPE_K_SMOOTH0 : NPPE_K_SMOOTH, by);
-                if ((e = launch_rbgs2(h->stream, L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l), 1, L.post2Planes,
-                                      h->dState, dot ? rzPartials : nullptr, C.v, xbuf(l + 1, cur[l + 1]))) !=
-                    hipSuccess)
-                    return e;
-            }
-            if (dot)
-            {
-                *rzDone = true;
-                Scope sc(h, NPPE_K_CG_DOT, 0.0);
-                if ((e = launch_rz_rows_finalize(h->stream, L.box, h->dState, rzPartials)) != hipSuccess) return e;
-            }
+            Scope sc(h, NPPE_K_SMOOTH, by);
+            if ((e = launch_rbgs2(h->stream, L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l), 1, L.post2Planes,
+                                  h->dState, C.v, xbuf(l + 1, cur[l + 1]))) != hipSuccess)
+                return e;
             cur[l] ^= 1;
             tr.add(by);
             continue;
@@ -1566,8 +1559,9 @@ int tune_group(nppe_solver *h0, PlanGroup &G, void *scratch)
     std::vector<int> tms;
     if (r0.kind == REC_ROW_SWEEP || r0.kind == REC_SWEEP_RESTRICT) retile_rows(r0, tms);
     else tms = {0};
-    const std::vector<int> ps = r0.kind == REC_ROW_SWEEP ? std::vector<int>{1, 2, 4, 8, 12, 16, 24, 32}
-                                                         : std::vector<int>{2, 4, 8, 16, 32};
+    const std::vector<int> ps = r0.kind == REC_ROW_SWEEP || r0.kind == REC_POST2
+                                    ? std::vector<int>{1, 2, 4, 8, 12, 16, 24, 32}
+                                    : std::vector<int>{2, 4, 8, 16, 32};
     const hipStream_t s = h0->stream;
     hipEvent_t a, b;
     HIP_TRY(h0, hipEventCreate(&a), "batch: tune");

@@ -106,16 +106,16 @@ def test_fused_restriction_tile_shapes(tm, p):
 
 
 @pytest.mark.parametrize("n,kind", [
-    (40, "mixed"),                       # level 0 too (rows of 40 cells): the march also writes <b, z> per row
-    ((24, 20, 140), "pressure"),         # level 0 of 140-cell rows keeps two launches; levels 1 and 2 march both
+    (40, "mixed"),                       # the coarse levels of a 40^3 grid: solids, Dirichlet codes
+    ((24, 20, 140), "pressure"),         # 140-cell rows at level 0; levels 1 and 2 march both sweeps
     ((33, 27, 61), "outlet"),            # odd sizes, Dirichlet codes
     ((31, 45, 97), "mixed"),             # odd plane count, solids
     ((100, 90, 24), "neumann"),          # many rows and planes: several row tiles and chunks
 ])
 def test_post_smoothing_pair_equals_two_sweeps(n, kind):
-    """Both post-smoothing sweeps of a level with rows of at most 128 cells run as one march (the second sweep
-    three planes behind the first, three halo rows each side of a tile); every chunk length gives the bits of the
-    two separate launches."""
+    """Both post-smoothing sweeps of a coarse level with rows of at most 128 cells run as one march (the second
+    sweep three planes behind the first, three halo rows each side of a tile); every chunk length gives the bits of
+    the two separate launches."""
     g, _ = make(3, n, kind, False, seed=11)
     b = np.random.default_rng(12).standard_normal(g.unknowns)
     if kind == "neumann":
@@ -128,16 +128,16 @@ def test_post_smoothing_pair_equals_two_sweeps(n, kind):
 
 
 @pytest.mark.parametrize("n,kind", [
-    (40, "mixed"),                       # level 0 too (rows of 40 cells)
-    ((24, 20, 140), "pressure"),         # level 0 of 140-cell rows keeps the two launches; coarser levels march
+    (40, "mixed"),                       # the coarse levels of a 40^3 grid: solids, Dirichlet codes
+    ((24, 20, 140), "pressure"),         # 140-cell rows at level 0; the coarser levels march
     ((33, 27, 61), "outlet"),            # odd sizes: families cut by an odd interior, Dirichlet codes
     ((31, 45, 97), "mixed"),             # odd plane count, solids
     ((100, 90, 24), "neumann"),          # many rows and planes: several row tiles and chunks
 ])
 def test_pre_smoothing_and_restriction_equal_separate_passes(n, kind):
-    """Both pre-smoothing sweeps from x = 0 and the restriction of the residual run as one march on levels with rows
-    of at most 128 cells (b and codes read once, no x read); every chunk length gives the bits of the zero-guess
-    sweep followed by the sweep with the restriction folded in."""
+    """Both pre-smoothing sweeps from x = 0 and the restriction of the residual run as one march on coarse levels
+    with rows of at most 128 cells (b and codes read once, no x read); every chunk length gives the bits of the
+    zero-guess sweep followed by the sweep with the restriction folded in."""
     g, _ = make(3, n, kind, False, seed=16)
     b = np.random.default_rng(17).standard_normal(g.unknowns)
     if kind == "neumann":
