@@ -1,6 +1,6 @@
 """The benchmark configurations' systems, small, against the oracle; and many systems at once: batched solves and
-the solve server return, for every system, exactly what the same solve returns alone (the north star's
-alone-vs-in-the-mix check), with the HBM budget enforced by admission and eviction."""
+the solve server return, for every system, exactly what the same solve returns alone (a system's result does not
+depend on what it shares launches with), with the HBM budget enforced by admission and eviction."""
 import numpy as np
 import pytest
 
