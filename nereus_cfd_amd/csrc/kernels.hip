@@ -836,8 +836,20 @@ __device__ __forceinline__ bool rbgs_cell(const RbgsCtx &C, int pl, int col, int
     return true;
 }
 
+// The coarse correction a sweep folds into x where it reads it (x + P x_c, the injection prolongation): the coarse
+// level's x and its strides (the parent of fine cell (s, m, f) is ((s + 1) / 2, (m + 1) / 2, (f + 1) / 2)).
+struct Prolong
+{
+    const float *xc;
+    int64_t sS;
+    int sM, ns, nf;
+};
+
+// x of plane pl on the tile and its two-cell halo. With pro (2D levels), x + P x_c at the unknowns, as k_prolong
+// forms it (the parent of a cell that is no unknown, or of one whose parent is none, adds 0: such x_c are 0).
 template <int TM>
-__device__ __forceinline__ void rbgs_fetch_x(const RbgsCtx &C, const float *xin, int zero, int pl, float (&v)[Rbgs<TM>::kLoad])
+__device__ __forceinline__ void rbgs_fetch_x(const RbgsCtx &C, const float *xin, int zero, int pl,
+                                             float (&v)[Rbgs<TM>::kLoad], const Prolong *pro = nullptr)
 {
     const Box &B = C.L.B;
     const bool planeOk = !zero && pl >= 0 && pl < B.ns;
@@ -847,7 +859,15 @@ __device__ __forceinline__ void rbgs_fetch_x(const RbgsCtx &C, const float *xin,
         const int i = threadIdx.x + k * kBlockThreads;
         const int lr = i / kLW, lc = i - lr * kLW;
         const int m = C.m0 - 2 + lr, f = C.f0 - 2 + lc;
-        v[k] = (planeOk && lr < TM + 4 && m >= 0 && m < B.nm && f >= 0 && f < B.nf) ? xin[B.at(pl, m, f)] : 0.f;
+        const bool in = planeOk && lr < TM + 4 && m >= 0 && m < B.nm && f >= 0 && f < B.nf;
+        float x = in ? xin[B.at(pl, m, f)] : 0.f;
+        if (pro && in)
+        {
+            const unsigned c = C.L.c8[B.at(pl, m, f)];
+            const float xcv = pro->xc[(int64_t)((m + 1) >> 1) * pro->sM + ((f + 1) >> 1)];
+            x += (c & 64u) ? xcv : 0.f;
+        }
+        v[k] = x;
     }
 }
 
@@ -992,11 +1012,11 @@ __device__ __forceinline__ void rbgs_store(const RbgsCtx &C, float (*X)[TM + 4][
 // first-colour values already final, and plane p-1 leaves for x_out. Global loads for the next plane are issued
 // before the current plane is computed. Colour of cell (s, m, f) is (s + m + f) & 1; c1 is updated first (0: red
 // first, the pre-smoother; 1: its adjoint, the post-smoother). zero: x_in is taken as 0.
-template <int TM>
+template <int TM, bool PRO>
 __global__ __launch_bounds__(kBlockThreads) void k_rbgs(Level32 L, const float *__restrict__ xin,
                                                         float *__restrict__ xout, const float *__restrict__ b, int c1,
                                                         int zero, int planesPerChunk, int tilesF, int tilesM,
-                                                        const SolveState *st)
+                                                        const SolveState *st, Prolong pro)
 {
     if (st->done) return;
     __shared__ float X[4][TM + 4][kLW];
@@ -1008,7 +1028,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_rbgs(Level32 L, const float *
     if (!L.B.threeD)
     {
         float v[Rbgs<TM>::kLoad];
-        rbgs_fetch_x<TM>(C, xin, zero, 0, v);
+        rbgs_fetch_x<TM>(C, xin, zero, 0, v, PRO ? &pro : nullptr);
         rbgs_fetch_cells<TM>(C, b, 0, c1, 0, c2, R);
         rbgs_put_x<TM>(X[0], v);
         __syncthreads();
@@ -1280,13 +1300,6 @@ __device__ __forceinline__ float gs_stretched(const uint16_t *c16, int64_t idx, 
 
 // The coarse correction a whole-row sweep folds in (x_in + P x_c where x is used): the coarse level's iterate and
 // box strides.
-struct Prolong
-{
-    const float *xc;
-    int64_t sS;
-    int sM, ns, nf;
-};
-
 // ---- per-row code descriptors ---------------------------------------------------------------------------------
 //
 // A row whose interior cells 2 .. nf - 3 share one stencil code (most rows of a box: every row of an all-Neumann or
@@ -4924,7 +4937,7 @@ hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float 
                        int zero, int tileRows, int planesPerChunk, SolveState *st, double *rzPartials,
                        const Level32 *prolongFrom, const float *prolongXc)
 {
-    if (prolongFrom && (zero || rbgs_family(L, xin, xout, b) != RBGS_ROW))
+    if (prolongFrom && (zero || (rbgs_family(L, xin, xout, b) != RBGS_ROW && rbgs_family(L, xin, xout, b) != RBGS_2D)))
     {
         // the correction as its own pass, then the plain sweep
         hipError_t e = launch_prolong(s, L, const_cast<float *>(xin), *prolongFrom, prolongXc, st);
@@ -5006,10 +5019,20 @@ hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float 
     const int P = std::max(1, planesPerChunk);
     const int chunks = B.threeD ? std::max(1, (B.ns - 2 + P - 1) / P) : 1;
     const int grid = tilesF * tilesM * chunks;
+    // 2D: the coarse correction folded into the sweep's loads
+    const bool fold = prolongFrom && prolongXc && !B.threeD && !zero;
+    const Prolong pro{fold ? prolongXc : nullptr, fold ? prolongFrom->B.sS : 0, fold ? (int)prolongFrom->B.sM : 0,
+                      fold ? prolongFrom->B.ns : 0, fold ? prolongFrom->B.nf : 0};
     if (TM == 8)
-        k_rbgs<8><<<grid, kBlockThreads, 0, s>>>(L, xin, xout, b, c1, zero, P, tilesF, tilesM, st);
+    {
+        if (fold) k_rbgs<8, true><<<grid, kBlockThreads, 0, s>>>(L, xin, xout, b, c1, zero, P, tilesF, tilesM, st, pro);
+        else k_rbgs<8, false><<<grid, kBlockThreads, 0, s>>>(L, xin, xout, b, c1, zero, P, tilesF, tilesM, st, pro);
+    }
     else
-        k_rbgs<16><<<grid, kBlockThreads, 0, s>>>(L, xin, xout, b, c1, zero, P, tilesF, tilesM, st);
+    {
+        if (fold) k_rbgs<16, true><<<grid, kBlockThreads, 0, s>>>(L, xin, xout, b, c1, zero, P, tilesF, tilesM, st, pro);
+        else k_rbgs<16, false><<<grid, kBlockThreads, 0, s>>>(L, xin, xout, b, c1, zero, P, tilesF, tilesM, st, pro);
+    }
     return hipGetLastError();
 }
 

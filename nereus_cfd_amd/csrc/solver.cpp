@@ -308,10 +308,13 @@ hipError_t vcycle(nppe_solver *h, Traffic &tr, const float *b0, int *zsel, doubl
             tr.add(by);
             continue;
         }
-        // The whole-row sweep folds the prolongation into its first post-smoothing sweep (it reads x + P x_c
-        // where it reads x); other sweeps get the correction as a pass of its own.
+        // The whole-row sweep and the 2D tile sweep fold the prolongation into their first post-smoothing sweep
+        // (it reads x + P x_c where it reads x); other sweeps get the correction as a pass of its own (so do all
+        // with NPPE_FOLD_PROLONG=0: the tests compare the two).
+        const int fam = rbgs_family(L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l));
+        const char *envFold = std::getenv("NPPE_FOLD_PROLONG");
         const bool foldProlong =
-            nu2 >= 1 && rbgs_family(L.v, xbuf(l, cur[l]), xbuf(l, cur[l] ^ 1), bbuf(l)) == RBGS_ROW;
+            nu2 >= 1 && (fam == RBGS_ROW || fam == RBGS_2D) && !(envFold && std::atoi(envFold) == 0);
         if (!foldProlong)
         {
             {

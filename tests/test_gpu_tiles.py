@@ -16,7 +16,8 @@ SHAPES = [("6", "1"), ("6", "5"), ("14", "3"), ("14", "16"), ("4", "2"), ("8", "
 def _solve(g, b, env):
     saved = {k: os.environ.get(k) for k in ("NPPE_RBGS_TM", "NPPE_RBGS_P", "NPPE_AUTOTUNE", "NPPE_RBGS_MARCH",
                                             "NPPE_FUSE_RESTRICT", "NPPE_FUSE_TM", "NPPE_FUSE_P", "NPPE_FUSE_POST",
-                                            "NPPE_POST2_P", "NPPE_FUSE_PRE", "NPPE_PRE2_P", "NPPE_ROW_CODES")}
+                                            "NPPE_POST2_P", "NPPE_FUSE_PRE", "NPPE_PRE2_P", "NPPE_ROW_CODES",
+                                            "NPPE_FOLD_PROLONG")}
     try:
         for k in saved:
             os.environ.pop(k, None)
@@ -192,6 +193,27 @@ def test_row_code_descriptors_equal_read_codes(n, kind, non_uniform):
     if kind == "neumann":
         b -= b.mean()
     ref, rres = _solve(g, b, {"NPPE_ROW_CODES": "0"})
+    x, res = _solve(g, b, {})
+    assert res.iterations == rres.iterations
+    assert np.array_equal(x, ref)
+
+
+@pytest.mark.parametrize("dim,n,kind,non_uniform", [
+    (2, 96, "mixed", False),             # 2D tile sweep: solids, air cells, Dirichlet ring codes
+    (2, (200, 150), "outlet", False),    # 2D, several tiles per row and odd coarse sizes
+    (2, 128, "neumann", False),          # 2D pure Neumann
+    (2, 80, "outlet", True),             # 2D stretched
+    (3, (24, 20, 140), "mixed", False),  # 3D whole-row sweep, two pairs per lane
+    (3, 36, "outlet", True),             # 3D stretched whole-row sweep
+])
+def test_folded_prolongation_equals_separate_pass(dim, n, kind, non_uniform):
+    """The first post-smoothing sweep reads x + P x_c where it reads x (the whole-row and 2D tile sweeps) instead of
+    a prolongation pass writing it first (NPPE_FOLD_PROLONG=0): the same additions, so the same bits."""
+    g, _ = make(dim, n, kind, non_uniform, seed=23)
+    b = np.random.default_rng(24).standard_normal(g.unknowns)
+    if kind == "neumann":
+        b -= b.mean()
+    ref, rres = _solve(g, b, {"NPPE_FOLD_PROLONG": "0"})
     x, res = _solve(g, b, {})
     assert res.iterations == rres.iterations
     assert np.array_equal(x, ref)
