@@ -61,7 +61,7 @@ struct Tables64
     const double *wS, *wM, *wF;
     const double *gS, *gM, *gF;
     double scale;   // MatrixScale3D: 1/(dh*dh) uniform, 1 stretched
-    int unitCoef;   // 3D uniform: every face coefficient is exactly 1.0 (PressureSolver3D.cpp:49-53)
+    int unitCoef;   // every face coefficient is exactly 1.0: 3D uniform (PressureSolver3D.cpp:49-53), 2D square cells
     const uint8_t *c8;   // the fine level's stencil codes (a cell without a Dirichlet neighbour takes a fast path)
     const uint16_t *c16; // and their 2-bit-per-neighbour form (unit-coefficient rows beside a Dirichlet cell)
     const uint32_t *rowc; // its per-row code descriptors (launch_row_codes), or null

@@ -812,6 +812,21 @@ int build(nppe_solver *h, const nppe_grid *g)
     const bool unitCoef = dim == 3 && !h->nonUniform;
     const double scale = h->nonUniform ? 1.0 : 1.0 / (g->dh * g->dh);
     const double sigma = unitCoef ? scale / g->dh : scale;
+    // The FP64 operator's unit-coefficient form: 3D uniform grids, and 2D grids whose widths along both axes are all
+    // one value (square cells): there every face coefficient Width / FaceGap (PressureSolver2D.cpp:125-128) is
+    // exactly 1.0 and every ring factor exactly 2.0, so the form sums the same values in the same order, bit for bit
+    // (NPPE_UNIT2D=0 keeps 2D on the general form: the tests compare the two).
+    bool unit64 = unitCoef;
+    {
+        const char *env = std::getenv("NPPE_UNIT2D");
+        if (dim == 2 && !h->nonUniform && g->width_x && g->width_y && !(env && std::atoi(env) == 0))
+        {
+            const double w0 = g->width_x[0];
+            unit64 = w0 > 0.0;
+            for (int j = 0; j < g->nx && unit64; j++) unit64 = g->width_x[j] == w0;
+            for (int i = 0; i < g->ny && unit64; i++) unit64 = g->width_y[i] == w0;
+        }
+    }
 
     std::vector<HostLevel> hl = coarsen_chain(std::move(fine));
     pt.mark(h, "coarsening");
@@ -893,7 +908,7 @@ int build(nppe_solver *h, const nppe_grid *g)
         HIP_TRY(h, upload(h, h->dTab64, tab.data(), tab.size() * sizeof(double)),
                 "upload FP64 tables");
         h->t64 = Tables64{h->dTab64 + off[0], h->dTab64 + off[1], h->dTab64 + off[2], h->dTab64 + off[3],
-                          h->dTab64 + off[4], h->dTab64 + off[5], scale, unitCoef ? 1 : 0, F.c8, F.c16, F.v.rowc};
+                          h->dTab64 + off[4], h->dTab64 + off[5], scale, unit64 ? 1 : 0, F.c8, F.c16, F.v.rowc};
     }
     // every vector in IDP order is sized for the most unknowns the box can hold, so a layout update reuses them
     h->unknownCap = std::max<int64_t>(1, interior_cells(F.host));

@@ -158,3 +158,29 @@ def test_missed_first_pass_takes_a_second(monkeypatch, n, kind, batched):
         x1 = x1 - x1.mean()
         x2 = x2 - x2.mean()
     assert np.linalg.norm(x2 - x1) / np.linalg.norm(x1) < 1e3 * tol
+
+
+@pytest.mark.parametrize("n,kind", [(64, "neumann"), (48, "outlet"), (56, "mixed"), (60, "freesurface"),
+                                    ((70, 40), "pressure")])
+def test_2d_unit_coefficients_equal_general_operator(monkeypatch, n, kind):
+    """A 2D grid of square cells has every FP64 face coefficient Width / FaceGap exactly 1.0 and every ring factor
+    exactly 2.0, so its FP64 passes take the unit-coefficient form: MGPCG and CG64 solutions bit for bit those of
+    the general operator (NPPE_UNIT2D=0)."""
+    g, og = make(2, n, kind, seed=25)
+    s = O.assemble(og)
+    b = np.random.default_rng(26).standard_normal(s.n)
+    if s.singular:
+        b -= b.mean()
+        b -= b.mean()
+    out = {}
+    for unit in ("0", "1"):
+        monkeypatch.setenv("NPPE_UNIT2D", unit)
+        for method in (N.METHOD_MGPCG, N.METHOD_CG64):
+            with N.Solver(tolerance=1e-9, method=method) as h:
+                h.commit(g)
+                x, res, rc = h.solve(b)
+            assert rc == 0
+            out[(unit, method)] = (x, res.iterations)
+    for method in (N.METHOD_MGPCG, N.METHOD_CG64):
+        assert out[("0", method)][1] == out[("1", method)][1]
+        assert np.array_equal(out[("0", method)][0], out[("1", method)][0])
