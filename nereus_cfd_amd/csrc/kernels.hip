@@ -953,122 +953,77 @@ __device__ __forceinline__ float rbgs_update(const RbgsCtx &C, int pl, int lr, i
 }
 
 template <int TM>
-__device__ __forceinline__ void rbgs_colour1(const RbgsCtx &C, float (*X)[TM + 4][kLW], int pl, int c1,
-                                             const RbgsRegs<TM> &R)
+__device__ __forceinline__ void rbgs_colour1(const RbgsCtx &C, float (*X)[kLW], int c1, const RbgsRegs<TM> &R)
 {
-    float(*Xc)[kLW] = X[pl & 3];
-    float(*Xp)[kLW] = X[(pl + 1) & 3];
-    float(*Xm)[kLW] = X[(pl + 3) & 3];
 #pragma unroll
     for (int k = 0; k < Rbgs<TM>::kC1; k++)
     {
         if (!(R.c1[k] & 64u)) continue;
         int lr, lc;
         int64_t idx;
-        rbgs_cell(C, pl, c1, threadIdx.x + k * kBlockThreads, 1, TM + 2, 1, 33, lr, lc, idx);
-        Xc[lr][lc] = rbgs_update(C, pl, lr, lc, idx, R.c1[k], R.b1[k], Xc, Xp, Xm);
+        rbgs_cell(C, 0, c1, threadIdx.x + k * kBlockThreads, 1, TM + 2, 1, 33, lr, lc, idx);
+        X[lr][lc] = rbgs_update(C, 0, lr, lc, idx, R.c1[k], R.b1[k], X, X, X);
     }
 }
 
 template <int TM>
-__device__ __forceinline__ void rbgs_colour2(const RbgsCtx &C, float (*X)[TM + 4][kLW], int pl, int c2,
-                                             const RbgsRegs<TM> &R)
+__device__ __forceinline__ void rbgs_colour2(const RbgsCtx &C, float (*X)[kLW], int c2, const RbgsRegs<TM> &R)
 {
-    float(*Xc)[kLW] = X[pl & 3];
-    float(*Xp)[kLW] = X[(pl + 1) & 3];
-    float(*Xm)[kLW] = X[(pl + 3) & 3];
 #pragma unroll
     for (int k = 0; k < Rbgs<TM>::kC2; k++)
     {
         if (!(R.c2[k] & 64u)) continue;
         int lr, lc;
         int64_t idx;
-        rbgs_cell(C, pl, c2, threadIdx.x + k * kBlockThreads, 2, TM, 2, 32, lr, lc, idx);
-        Xc[lr][lc] = rbgs_update(C, pl, lr, lc, idx, R.c2[k], R.b2[k], Xc, Xp, Xm);
+        rbgs_cell(C, 0, c2, threadIdx.x + k * kBlockThreads, 2, TM, 2, 32, lr, lc, idx);
+        X[lr][lc] = rbgs_update(C, 0, lr, lc, idx, R.c2[k], R.b2[k], X, X, X);
     }
 }
 
 template <int TM>
-__device__ __forceinline__ void rbgs_store(const RbgsCtx &C, float (*X)[TM + 4][kLW], float *xout, int pl)
+__device__ __forceinline__ void rbgs_store(const RbgsCtx &C, float (*X)[kLW], float *xout)
 {
     const Box &B = C.L.B;
-    float(*Xc)[kLW] = X[pl & 3];
 #pragma unroll
     for (int k = 0; k < Rbgs<TM>::kStore; k++)
     {
         const int i = threadIdx.x + k * kBlockThreads;
         const int r = i / kTF, c = i - r * kTF;
         const int m = C.m0 + r, f = C.f0 + c;
-        if (r < TM && m <= B.nm - 2 && f <= B.nf - 2) xout[B.at(pl, m, f)] = Xc[r + 2][c + 2];
+        if (r < TM && m <= B.nm - 2 && f <= B.nf - 2) xout[B.at(0, m, f)] = X[r + 2][c + 2];
     }
 }
 
-// One full red-black Gauss-Seidel sweep in one pass over memory (x_in -> x_out, never in place).
+// One full red-black Gauss-Seidel sweep of a 2D level in one pass over memory (x_in -> x_out, never in place).
 //
-// A block owns a 64 x TM tile of the mid/fast plane and marches along the slow axis over a chunk of planes. The old
-// x of four consecutive planes sits in LDS with a two-cell halo. At plane p the first colour is updated on the tile
-// plus a one-cell halo (halo values are recomputed from the same old data the neighbouring block uses, so every
-// block sees identical numbers), then the second colour on the tile of plane p-1, whose six neighbours are all
-// first-colour values already final, and plane p-1 leaves for x_out. Global loads for the next plane are issued
-// before the current plane is computed. Colour of cell (s, m, f) is (s + m + f) & 1; c1 is updated first (0: red
-// first, the pre-smoother; 1: its adjoint, the post-smoother). zero: x_in is taken as 0.
+// A block owns a 64 x TM tile of the plane. Its old x, with a two-cell halo, goes to LDS; the first colour is
+// updated on the tile plus a one-cell halo (the halo values are recomputed from the same old data the neighbouring
+// block uses, so every block sees identical numbers), then the second colour on the tile, whose four neighbours are
+// all first-colour values already final, and the tile leaves for x_out. Colour of cell (m, f) is (m + f) & 1; c1 is
+// updated first (0: red first, the pre-smoother; 1: its adjoint, the post-smoother). zero: x_in is taken as 0. PRO:
+// x + P x_c wherever x is read (the coarse correction folded in). 3D levels sweep in the marches below.
 template <int TM, bool PRO>
 __global__ __launch_bounds__(kBlockThreads) void k_rbgs(Level32 L, const float *__restrict__ xin,
                                                         float *__restrict__ xout, const float *__restrict__ b, int c1,
-                                                        int zero, int planesPerChunk, int tilesF, int tilesM,
-                                                        const SolveState *st, Prolong pro)
+                                                        int zero, int tilesF, const SolveState *st, Prolong pro)
 {
     if (st->done) return;
-    __shared__ float X[4][TM + 4][kLW];
+    __shared__ float X[TM + 4][kLW];
     const int lb = logical_block();
-    const int tf = lb % tilesF, t2 = lb / tilesF, tm = t2 % tilesM, chunk = t2 / tilesM;
+    const int tf = lb % tilesF, tm = lb / tilesF;
     const RbgsCtx C{L, 1 + tm * TM, 1 + tf * kTF};
     const int c2 = c1 ^ 1;
     RbgsRegs<TM> R;
-    if (!L.B.threeD)
-    {
-        float v[Rbgs<TM>::kLoad];
-        rbgs_fetch_x<TM>(C, xin, zero, 0, v, PRO ? &pro : nullptr);
-        rbgs_fetch_cells<TM>(C, b, 0, c1, 0, c2, R);
-        rbgs_put_x<TM>(X[0], v);
-        __syncthreads();
-        rbgs_colour1<TM>(C, X, 0, c1, R);
-        __syncthreads();
-        rbgs_colour2<TM>(C, X, 0, c2, R);
-        __syncthreads();
-        rbgs_store<TM>(C, X, xout, 0);
-        return;
-    }
-    const int s0 = 1 + chunk * planesPerChunk;
-    const int s1 = min(s0 + planesPerChunk - 1, L.B.ns - 2);
-    {
-        float v[Rbgs<TM>::kLoad];
-        rbgs_fetch_x<TM>(C, xin, zero, s0 - 2, v);
-        rbgs_put_x<TM>(X[(s0 - 2) & 3], v);
-        rbgs_fetch_x<TM>(C, xin, zero, s0 - 1, v);
-        rbgs_put_x<TM>(X[(s0 - 1) & 3], v);
-        rbgs_fetch_x<TM>(C, xin, zero, s0, R.x);
-        rbgs_fetch_cells<TM>(C, b, s0 - 1, c1, s0 - 2, c2, R);
-    }
-    for (int p = s0 - 1; p <= s1 + 1; p++)
-    {
-        // plane p+1 (fetched last step) into LDS; issue the global loads of the next step
-        rbgs_put_x<TM>(X[(p + 1) & 3], R.x);
-        RbgsRegs<TM> N;
-        rbgs_fetch_x<TM>(C, xin, zero, p + 2, N.x);
-        rbgs_fetch_cells<TM>(C, b, p + 1, c1, p, c2, N);
-        __syncthreads();
-        if (p >= 1 && p <= L.B.ns - 2) rbgs_colour1<TM>(C, X, p, c1, R);
-        __syncthreads();
-        const int q = p - 1;
-        if (q >= s0)
-        {
-            rbgs_colour2<TM>(C, X, q, c2, R);
-            __syncthreads();
-            rbgs_store<TM>(C, X, xout, q);
-        }
-        R = N;
-    }
+    float v[Rbgs<TM>::kLoad];
+    rbgs_fetch_x<TM>(C, xin, zero, 0, v, PRO ? &pro : nullptr);
+    rbgs_fetch_cells<TM>(C, b, 0, c1, 0, c2, R);
+    rbgs_put_x<TM>(X, v);
+    __syncthreads();
+    rbgs_colour1<TM>(C, X, c1, R);
+    __syncthreads();
+    rbgs_colour2<TM>(C, X, c2, R);
+    __syncthreads();
+    rbgs_store<TM>(C, X, xout);
 }
 
 // ---- red-black Gauss-Seidel sweep, 2.5D march (3D boxes) -----------------------------------------------------
@@ -2822,7 +2777,6 @@ __device__ __forceinline__ void cg_dir64_row_body(const Box &B, const Tables64 &
     for (int i = threadIdx.x; i < 2 * (TM + 2) * 2 * kSlots; i += 64 * (TM + 2)) (&ldsP[0][0][0][0])[i] = 0.f;
     auto plane = [&](int pl) { return (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
     auto pair = [](const float *row, uint32_t o) { return *reinterpret_cast<const float2 *>(row + o); };
-    auto pairC = [](const uint8_t *row, uint32_t o) { return (unsigned)*reinterpret_cast<const uint16_t *>(row + o); };
     auto pv = [&](float zv, float pov) { return (float)(first ? (double)zv : (double)zv + beta * (double)pov); };
     // per-plane values in slots by (plane - (s0 - 1)) mod 4
     float2 Zv[4][K], Po[4][K], Pn[4][K];
@@ -2947,7 +2901,6 @@ __device__ __forceinline__ void cg_update64_row_body(const Box &B, const Tables6
     auto plane = [&](int pl) { return (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
     auto pair = [](const float *row, uint32_t o) { return *reinterpret_cast<const float2 *>(row + o); };
     auto pair64 = [](const double *row, uint32_t o) { return *reinterpret_cast<const double2 *>(row + o); };
-    auto pairC = [](const uint8_t *row, uint32_t o) { return (unsigned)*reinterpret_cast<const uint16_t *>(row + o); };
     float2 Pv[4][K];
     unsigned C[4][K];
     double2 Rv[4][K];   // own r (interior waves), loaded two planes ahead
@@ -3318,7 +3271,6 @@ __device__ __forceinline__ void rbgs_restrict_row_body(const Level32 &L, const f
     auto planeC = [&](int pl) { return L.c8 + (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
     auto plane_off = [&](int pl) { return (size_t)min(max(pl, 0), B.ns - 1) * (size_t)B.sS; };
     auto pair = [](const float *row, uint32_t o) { return *reinterpret_cast<const float2 *>(row + o); };
-    auto pairC = [](const uint8_t *row, uint32_t o) { return (unsigned)*reinterpret_cast<const uint16_t *>(row + o); };
     auto gs = [&](float acc, float bv, unsigned code) {
         const float2 t = tab[code & 127u];
         return __uint_as_float(__float_as_uint((acc - bv * L.icu) * t.x) & __float_as_uint(t.y));
@@ -5016,22 +4968,20 @@ hipError_t launch_rbgs(hipStream_t s, const Level32 &L, const float *xin, float 
     const int tilesF = std::max(1, (B.nf - 2 + kTF - 1) / kTF);
     const int TM = tileRows == 8 ? 8 : 16;
     const int tilesM = std::max(1, (B.nm - 2 + TM - 1) / TM);
-    const int P = std::max(1, planesPerChunk);
-    const int chunks = B.threeD ? std::max(1, (B.ns - 2 + P - 1) / P) : 1;
-    const int grid = tilesF * tilesM * chunks;
+    const int grid = tilesF * tilesM;
     // 2D: the coarse correction folded into the sweep's loads
     const bool fold = prolongFrom && prolongXc && !B.threeD && !zero;
     const Prolong pro{fold ? prolongXc : nullptr, fold ? prolongFrom->B.sS : 0, fold ? (int)prolongFrom->B.sM : 0,
                       fold ? prolongFrom->B.ns : 0, fold ? prolongFrom->B.nf : 0};
     if (TM == 8)
     {
-        if (fold) k_rbgs<8, true><<<grid, kBlockThreads, 0, s>>>(L, xin, xout, b, c1, zero, P, tilesF, tilesM, st, pro);
-        else k_rbgs<8, false><<<grid, kBlockThreads, 0, s>>>(L, xin, xout, b, c1, zero, P, tilesF, tilesM, st, pro);
+        if (fold) k_rbgs<8, true><<<grid, kBlockThreads, 0, s>>>(L, xin, xout, b, c1, zero, tilesF, st, pro);
+        else k_rbgs<8, false><<<grid, kBlockThreads, 0, s>>>(L, xin, xout, b, c1, zero, tilesF, st, pro);
     }
     else
     {
-        if (fold) k_rbgs<16, true><<<grid, kBlockThreads, 0, s>>>(L, xin, xout, b, c1, zero, P, tilesF, tilesM, st, pro);
-        else k_rbgs<16, false><<<grid, kBlockThreads, 0, s>>>(L, xin, xout, b, c1, zero, P, tilesF, tilesM, st, pro);
+        if (fold) k_rbgs<16, true><<<grid, kBlockThreads, 0, s>>>(L, xin, xout, b, c1, zero, tilesF, st, pro);
+        else k_rbgs<16, false><<<grid, kBlockThreads, 0, s>>>(L, xin, xout, b, c1, zero, tilesF, st, pro);
     }
     return hipGetLastError();
 }
