@@ -4366,6 +4366,19 @@ __global__ void k_krylov_init(SolveState *st, double tol, double ref, double aim
 
 inline int grid_of(const Box &B) { return std::min(B.blocks(), kMaxGrid); }
 
+// Blocks of a tile-looping kernel resident at once (blocks per CU at its occupancy x CUs, at most kMaxGrid): more
+// blocks than that run a second round of loops as long as the first on a partly idle GPU.
+static int resident_grid(const void *kernel)
+{
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlockThreads, 0) != hipSuccess || per < 1 ||
+        cus < 1)
+        return kMaxGrid;
+    return std::min(kMaxGrid, per * cus);
+}
+
 // Planes per marching chunk: enough chunks for about 4096 blocks, at least 2 planes, even (the restriction pairs
 // planes), at most 64.
 inline int march_planes(const Box &B)
@@ -4739,7 +4752,8 @@ hipError_t launch_update_residual64(hipStream_t s, const Box &B, const Tables64 
     }
     sumX = gate && sumX ? 1 : 0;
     const FinOp op = gate ? (sumX ? FIN_TRUE_RES_SUM : FIN_TRUE_RES) : FIN_SUM1;
-    const int G = grid_of(B);
+    static const int resident = resident_grid(reinterpret_cast<const void *>(k_update_residual64));
+    const int G = std::min(B.blocks(), resident);
     const Red red = red_of(partials, st, op, G);
     k_update_residual64<<<G, kBlockThreads, 0, s>>>(B, T, ty, x, e, xout, b, r32, r64, red, gate, sumX);
     return finish(s, red, G, sumX ? 2 : 1);
@@ -5047,7 +5061,8 @@ hipError_t launch_residual_restrict(hipStream_t s, const Level32 &Lf, const floa
         k_residual_restrict_m<<<G, kBlockThreads, 0, s>>>(Lf, x, b, Lc, bc, st, P);
         return hipGetLastError();
     }
-    k_residual_restrict<<<grid_of(Lc.B), kBlockThreads, 0, s>>>(Lf, x, b, Lc, bc, st);
+    static const int resident = resident_grid(reinterpret_cast<const void *>(k_residual_restrict));
+    k_residual_restrict<<<std::min(Lc.B.blocks(), resident), kBlockThreads, 0, s>>>(Lf, x, b, Lc, bc, st);
     return hipGetLastError();
 }
 
@@ -5211,7 +5226,8 @@ hipError_t launch_cg_dir64(hipStream_t s, const Box &B, const Tables64 &T, const
     // 6-row blocks, two per CU (measured at 256^3: 57 us against 61 us for one 14-row block per CU)
     if (!B.threeD)
     {
-        const int G = grid_of(B);
+        static const int resident = resident_grid(reinterpret_cast<const void *>(k_cg_dir64_g));
+        const int G = std::min(B.blocks(), resident);
         const Red red = red_of(partials, st, FIN_PQ, G);
         k_cg_dir64_g<<<G, kBlockThreads, 0, s>>>(B, T, ty, z, ring, stride, st, red);
         return finish(s, red, G, 1);
@@ -5242,7 +5258,8 @@ hipError_t launch_cg_update64(hipStream_t s, const Box &B, const Tables64 &T, co
 {
     if (!B.threeD)
     {
-        const int G = grid_of(B);
+        static const int resident = resident_grid(reinterpret_cast<const void *>(k_cg_update64_g));
+        const int G = std::min(B.blocks(), resident);
         const Red red = red_of(partials, st, FIN_RR, G);
         k_cg_update64_g<<<G, kBlockThreads, 0, s>>>(B, T, ty, ring, stride, x, r, r32, b, st, red);
         return finish(s, red, G, 1);
