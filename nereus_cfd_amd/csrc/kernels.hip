@@ -4367,7 +4367,7 @@ __global__ void k_krylov_init(SolveState *st, double tol, double ref, double aim
 inline int grid_of(const Box &B) { return std::min(B.blocks(), kMaxGrid); }
 
 // Blocks of a tile-looping kernel resident at once (blocks per CU at its occupancy x CUs, at most kMaxGrid): more
-// blocks than that run a second round of loops as long as the first on a partly idle GPU.
+// blocks than that run in two batches, the second looping as long as the first on a partly idle GPU.
 static int resident_grid(const void *kernel)
 {
     int dev = 0, cus = 0, per = 0;
